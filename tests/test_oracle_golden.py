@@ -1,0 +1,429 @@
+"""Pins the CPU oracle against the reference's known answers, closed forms and brute force.
+
+Every check here is independent of the oracle's own formulas: reference golden values,
+finite differences of energies, brute-force pair sets, a half-shell (Newton's third law)
+traversal, and exact geometric facts.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))
+KAT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "philox_kat.json")))
+
+
+def test_philox_known_answers():
+    for v in KAT["vectors"]:
+        out = po.philox4x32_10([int(x, 16) for x in v["ctr"]], [int(x, 16) for x in v["key"]])
+        assert ["%08x" % x for x in out] == v["out"]
+
+
+def test_u53_range_and_resolution():
+    assert po.u53(0, 0) == 0.0
+    assert po.u53(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 2.0 ** -53
+    assert po.u53(1 << 5, 0) == 2.0 ** -27
+    assert po.u53(0, 1 << 6) == 2.0 ** -53
+
+
+def test_shortest_difference_boundary_rule():
+    box, pbc = [10.0, 10.0, 10.0], [1, 1, 1]
+    # dv = +L/2 stays (strict >), dv = -L/2 flips (<=): boundary_condition_operations.h:70-80
+    d = po.shortest_difference([0, 0, 0], [5.0, -5.0, 4.0], box, pbc)
+    assert list(d) == [5.0, 5.0, 4.0]
+    d = po.shortest_difference([0, 0, 0], [5.0, 0, 0], box, [0, 0, 0])
+    assert list(d) == [5.0, 0.0, 0.0]
+    v = po.fix_position([5.0, -5.0, 27.0], box, pbc)
+    assert list(v) == [-5.0, -5.0, -3.0]
+
+
+def _build(scene, types_extra=()):
+    names = scene["types"]
+    tid = {n: i for i, n in enumerate(names)}
+    s = po.System(len(names), scene["box"], scene["pbc"])
+    for p in scene.get("potentials", []):
+        t1, t2 = tid[p["types"][0]], tid[p["types"][1]]
+        if p["kind"] == "HarmonicRepulsion":
+            s.add_pair_potential(po.POT_HARMONIC_REPULSION, t1, t2, [p["force_constant"], p["interaction_distance"]])
+        elif p["kind"] == "LennardJones":
+            s.add_pair_potential(po.POT_LENNARD_JONES, t1, t2,
+                                 [p["m"], p["n"], p["cutoff"], float(p["shift"]), p["epsilon"], p["sigma"]])
+        elif p["kind"] == "ScreenedElectrostatics":
+            s.add_pair_potential(po.POT_SCREENED_ELECTROSTATICS, t1, t2,
+                                 [p["electrostatic_strength"], p["inverse_screening_depth"], p["repulsion_strength"],
+                                  p["repulsion_distance"], p["exponent"], p["cutoff"]])
+    pos = [p[1] for p in scene["particles"]]
+    types = [tid[p[0]] for p in scene["particles"]]
+    s.add_particles(pos, types, ids=np.arange(len(types)))
+    s.nl_setup(s.max_cutoff())
+    return s
+
+
+def _forces_by_id(s):
+    pos, types, ids, f = s.particles()
+    out = np.zeros((len(ids), 3))
+    out[ids.astype(int)] = f
+    return out
+
+
+def test_harmonic_repulsion_two_particles():
+    g = GOLDEN["harmonic_repulsion_two_particles"]
+    s = _build(g)
+    e1 = s.calculate_forces()
+    e2 = s.calculate_forces()  # calculating twice must not accumulate (TestStateModel.cpp:86)
+    assert e1 == e2
+    assert e1 == pytest.approx(g["energy"], rel=1e-12)
+    np.testing.assert_allclose(_forces_by_id(s), g["forces"], atol=1e-12)
+
+
+def test_harmonic_repulsion_six_particles():
+    g = GOLDEN["harmonic_repulsion_six_particles"]
+    s = _build(g)
+    e = s.calculate_forces()
+    pf = {k: np.array(v) for k, v in g["pair_forces"].items()}
+    expected = np.zeros((6, 3))
+    expected[0] = pf["03"] + pf["05"]
+    expected[1] = pf["13"] + pf["15"]
+    expected[2] = pf["23"] + pf["25"]
+    expected[3] = -pf["03"] - pf["13"] - pf["23"]
+    expected[5] = -pf["05"] - pf["15"] - pf["25"]
+    np.testing.assert_allclose(_forces_by_id(s), expected, atol=1e-12)
+    assert e == pytest.approx(sum(g["pair_energies"].values()), rel=1e-12)
+
+
+def test_lennard_jones_known_answer():
+    g = GOLDEN["lennard_jones"]
+    s = _build(g)
+    e = s.calculate_forces()
+    assert e == pytest.approx(g["energy"], rel=g["energy_rel_tolerance"])
+    np.testing.assert_allclose(_forces_by_id(s), g["forces"], atol=1e-5)
+
+
+def test_screened_electrostatics_known_answer():
+    g = GOLDEN["screened_electrostatics"]
+    s = _build(g)
+    e = s.calculate_forces()
+    assert e == pytest.approx(g["energy"], rel=1e-9)
+    np.testing.assert_allclose(_forces_by_id(s), g["forces"], atol=g["force_tolerance"])
+
+
+def _fd_force(energy, x, h=1e-6):
+    g = np.zeros(3)
+    for d in range(3):
+        xp, xm = np.array(x, float), np.array(x, float)
+        xp[d] += h
+        xm[d] -= h
+        g[d] = (energy(xp) - energy(xm)) / (2 * h)
+    return -g
+
+
+@pytest.mark.parametrize("kind,params,xs", [
+    (po.POT_HARMONIC_REPULSION, [10.0, 1.0], [[0.3, -0.2, 0.5], [0.1, 0.0, 0.0], [0.6, 0.5, 0.4]]),
+    (po.POT_WEAK_INTERACTION, [10.0, 1.0, 0.5, 2.0], [[0.3, -0.2, 0.5], [0.8, 0.6, 0.3], [1.2, 0.9, 0.7],
+                                                     [0.2, 1.5, 0.1]]),
+    (po.POT_LENNARD_JONES, [12, 6, 2.5, 1.0, 1.0, 1.0], [[0.9, 0.2, 0.1], [1.3, -0.4, 0.6], [2.0, 0.1, 0.5]]),
+    (po.POT_SCREENED_ELECTROSTATICS, [-1.0, 1.0, 1.0, 1.0, 6, 8.0], [[1.2, 1.5, -1.7], [0.9, 0.2, 0.1]]),
+])
+def test_pair_force_is_minus_energy_gradient(kind, params, xs):
+    # x_ij = neighbour - self, so moving "self" by +h changes x_ij by -h: F_self = +dE/dx_ij
+    for x in xs:
+        f, _ = po.pair_force_energy(kind, params, x)
+        g = -_fd_force(lambda y: po.pair_force_energy(kind, params, y)[1], x)
+        np.testing.assert_allclose(f, g, rtol=1e-6, atol=1e-8)
+
+
+def test_weak_interaction_continuity_values():
+    # closed-form energies at the piece boundaries: -h at d0, -h/2 at d0 + l/2, 0 at rc
+    k, d0, h, rc = 10.0, 1.0, 0.5, 2.0
+    for r, want in [(d0, -h), (d0 + 0.5 * (rc - d0), -h / 2), (rc - 1e-12, 0.0)]:
+        _, e = po.pair_force_energy(po.POT_WEAK_INTERACTION, [k, d0, h, rc], [r, 0.0, 0.0])
+        assert e == pytest.approx(want, abs=1e-9)
+    _, e = po.pair_force_energy(po.POT_WEAK_INTERACTION, [k, d0, h, rc], [0.0, 0.0, 0.0])
+    assert e == pytest.approx(0.5 * k * d0 * d0 - h)
+
+
+def test_harmonic_bond_known_answer():
+    g = GOLDEN["harmonic_bond"]
+    (fi, fj), e = po.bond_force_energy(g["force_constant"], g["length"], *g["positions"], g["box"], g["pbc"])
+    assert list(fi) == g["forces"][0] and list(fj) == g["forces"][1]  # asserted with == in the reference
+    assert e == g["energy"]
+
+
+@pytest.mark.parametrize("name", ["harmonic_angle", "harmonic_angle_general"])
+def test_harmonic_angle_known_answers(name):
+    g = GOLDEN[name]
+    f, e = po.angle_force_energy(g["force_constant"], g["theta0"], *g["positions"], g["box"], g["pbc"])
+    assert e == pytest.approx(g["energy"], rel=1e-12)
+    np.testing.assert_allclose(np.array(f), g["forces"], atol=g["force_tolerance"])
+
+
+@pytest.mark.parametrize("name", ["cosine_dihedral", "cosine_dihedral_steep"])
+def test_cosine_dihedral_known_answers(name):
+    g = GOLDEN[name]
+    f, e = po.dihedral_force_energy(g["force_constant"], g["multiplicity"], g["phi0"], *g["positions"], g["box"],
+                                    g["pbc"])
+    assert e == pytest.approx(g["energy"], rel=1e-12)
+    np.testing.assert_allclose(np.array(f), g["forces"], atol=g["force_tolerance"])
+
+
+def test_bonded_forces_are_energy_gradients():
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        x = rng.normal(size=(4, 3)) * 0.8 + np.array([[0, 0, 0], [1, 0, 0], [1.5, 0.9, 0], [2.4, 1.0, 0.7]])
+        k, mult, phi0 = 1.3, 3, 0.4
+        f, _ = po.dihedral_force_energy(k, mult, phi0, *x)
+        for a in range(4):
+            def en(y, a=a):
+                xx = x.copy()
+                xx[a] = y
+                return po.dihedral_force_energy(k, mult, phi0, *xx)[1]
+            np.testing.assert_allclose(f[a], _fd_force(en, x[a]), rtol=1e-5, atol=1e-6)
+        f, _ = po.angle_force_energy(2.0, 2.0, *x[:3])
+        for a in range(3):
+            def ea(y, a=a):
+                xx = x[:3].copy()
+                xx[a] = y
+                return po.angle_force_energy(2.0, 2.0, *xx)[1]
+            np.testing.assert_allclose(f[a], _fd_force(ea, x[a]), rtol=1e-5, atol=1e-6)
+
+
+def test_dihedral_zero_force_branch():
+    # |cos phi| < 1e-4 returns zero force but keeps the energy (TorsionPotential.cpp:91-93)
+    x = [[-1, 0, 0], [0, 0, 0], [0, 0, 1], [0, 1, 1]]  # phi = 90 degrees
+    f, e = po.dihedral_force_energy(1.0, 3, 0.0, *x)
+    assert all(np.all(fi == 0) for fi in f)
+    # phi = -pi/2 here, so E = k (1 + cos(-3 pi / 2)) = 1
+    assert e == pytest.approx(1.0, abs=1e-12)
+
+
+def _random_system(n, box, pbc, seed, n_types=2):
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((n, 3)) - 0.5) * np.array(box)
+    types = rng.integers(0, n_types, n)
+    s = po.System(n_types, box, pbc)
+    s.add_particles(pos, types, ids=np.arange(n))
+    return s, pos, types
+
+
+def _pairset(a):
+    return set(map(tuple, a.astype(np.int64).tolist()))
+
+
+@pytest.mark.parametrize("box,pbc,radius,cutoff", [
+    ([10, 10, 10], [1, 1, 1], 1, 1.0), ([10, 10, 10], [1, 1, 1], 2, 1.0), ([10, 10, 10], [1, 1, 1], 3, 1.0),
+    ([7.3, 5.1, 12.2], [1, 0, 1], 1, 1.3), ([3.0, 3.0, 3.0], [1, 1, 1], 1, 1.2), ([2.1, 9.0, 4.0], [0, 0, 0], 1, 1.0),
+    ([1.5, 4.0, 1.5], [0, 0, 0], 1, 1.2), ([4, 4, 4], [1, 1, 1], 1, 1.9),
+])
+def test_cell_list_pairs_equal_brute_force(box, pbc, radius, cutoff):
+    s, _, _ = _random_system(600, box, pbc, seed=11)
+    s.nl_setup(cutoff, radius)
+    nl = s.nl_pairs(cutoff)
+    bf = s.brute_pairs(cutoff)
+    assert len(nl) == len(_pairset(nl)), "a pair was visited twice"
+    assert _pairset(nl) == _pairset(bf)
+
+
+def test_neighbor_list_reference_scenes():
+    g = GOLDEN["neighbor_list_scenes"]
+    for sc in g["scenes"]:
+        s = po.System(1, sc["box"], sc["pbc"])
+        s.add_particles(sc["positions"], [0] * len(sc["positions"]), ids=np.arange(len(sc["positions"])))
+        s.nl_setup(g["cutoff"], sc["radius"])
+        pairs = _pairset(s.nl_pairs(g["cutoff"]))
+        if sc.get("all_pairs"):
+            n = len(sc["positions"])
+            assert pairs == {(i, j) for i in range(n) for j in range(n) if i != j}
+        for p in sc.get("must_contain", []):
+            assert tuple(p) in pairs
+
+
+def test_forces_match_half_shell_traversal():
+    """Full-neighbour cell-list traversal with E/2 equals an i<j brute-force sum with Newton's third law."""
+    box, pbc = [12.0, 12.0, 12.0], [1, 1, 1]
+    s, pos, types = _random_system(700, box, pbc, seed=3)
+    hr = [10.0, 1.0]
+    wi = [10.0, 1.0, 0.5, 2.0]
+    for t1 in range(2):
+        for t2 in range(t1, 2):
+            s.add_pair_potential(po.POT_HARMONIC_REPULSION, t1, t2, hr)
+    s.add_pair_potential(po.POT_WEAK_INTERACTION, 0, 0, wi)
+    s.nl_setup(s.max_cutoff())
+    e = s.calculate_forces()
+    f = _forces_by_id(s)
+    n = len(types)
+    F = np.zeros((n, 3))
+    E = 0.0
+    for i in range(n):
+        for j in range(i + 1, n):
+            x = po.shortest_difference(pos[i], pos[j], box, pbc)
+            d2 = x @ x
+            pots = [(po.POT_HARMONIC_REPULSION, hr, 1.0)]
+            if types[i] == 0 and types[j] == 0:
+                pots.append((po.POT_WEAK_INTERACTION, wi, 2.0))
+            for kind, prm, rc in pots:
+                if d2 < rc * rc:
+                    fi, ei = po.pair_force_energy(kind, prm, x)
+                    F[i] += fi
+                    F[j] -= fi
+                    E += ei
+    np.testing.assert_allclose(f, F, rtol=1e-10, atol=1e-10)
+    assert e == pytest.approx(E, rel=1e-10)
+
+
+def test_euler_bd_closed_form_with_injected_noise():
+    box, pbc = [10.0, 10.0, 10.0], [1, 1, 1]
+    s = po.System(2, box, pbc, kbt=2.0, diffusion=[1.0, 0.5])
+    pos = np.array([[4.9, 0.0, -4.9], [0.0, 1.0, 2.0]])
+    s.add_particles(pos, [0, 1], ids=[0, 1])
+    s.add_pair_potential(po.POT_HARMONIC_REPULSION, 0, 1, [10.0, 1.0])
+    s.nl_setup(1.0)
+    s.calculate_forces()  # particles too far apart: zero force
+    noise = np.array([[1.0, -2.0, 0.5], [0.3, 0.2, -0.1]])
+    dt = 0.01
+    s.euler_bd(dt, noise.ravel())
+    got, _, ids, _ = s.particles()
+    for i, D in enumerate([1.0, 0.5]):
+        want = pos[i] + noise[i] * math.sqrt(2 * D * dt)
+        want = po.fix_position(want, box, pbc)
+        np.testing.assert_array_equal(got[list(ids).index(i)], want)
+
+
+def test_fusion_and_fission_placement():
+    g = GOLDEN["fusion_placement"]
+    s = po.System(3, g["box"], g["pbc"])
+    s.add_reaction(po.RX_FUSION, [0, 1], [2], 1e16, g["educt_distance"] * 10, 0, g["weight1"], g["weight2"])
+    s.add_particles([g["A"], g["B"]], [0, 1], ids=[0, 1])
+    s.nl_setup(s.max_cutoff())
+    s.reactions(po.UA_KEYED, 1.0)
+    pos, types, _, _ = s.particles()
+    assert list(types) == [2]
+    np.testing.assert_allclose(pos[0], g["product"], atol=1e-15)
+
+    g = GOLDEN["fission_distance"]
+    s = po.System(3, g["box"], g["pbc"])
+    s.add_reaction(po.RX_FISSION, [2], [0, 1], 1e16, 0, g["product_distance"], g["weight1"], g["weight2"])
+    s.add_particles([g["C"]], [2], ids=[0])
+    s.reactions(po.UA_KEYED, 1.0)
+    pos, types, ids, _ = s.particles()
+    assert sorted(types) == [0, 1]
+    assert np.linalg.norm(pos[0] - pos[1]) == pytest.approx(g["product_distance"], rel=1e-12)
+    # product ids: the appended product is drawn first, entry1's new id second (ReactionUtils.h:194-201)
+    assert ids[list(types).index(1)] == 1 and ids[list(types).index(0)] == 2
+
+
+def test_fusion_through_periodic_boundary():
+    g = GOLDEN["fusion_through_periodic_boundary"]
+    for mode in (po.UA_KEYED, po.GILLESPIE_LITERAL, po.GILLESPIE_KEYED):
+        s = po.System(1, g["box"], g["pbc"])
+        s.add_reaction(po.RX_FUSION, [0, 0], [0], g["rate"], g["radius"])
+        s.add_particles(g["particles"], [0, 0], ids=[0, 1])
+        s.nl_setup(s.max_cutoff())
+        s.reactions(mode, 1.0)
+        pos, _, _, _ = s.particles()
+        assert len(pos) == 1
+        np.testing.assert_allclose(pos[0], g["product"], rtol=g["product_rel_tolerance"])
+
+
+def test_fission_near_boundary():
+    g = GOLDEN["fission_near_boundary"]
+    for seed in range(5):
+        s = po.System(1, g["box"], g["pbc"])
+        s.set_seed(seed)
+        s.add_reaction(po.RX_FISSION, [0], [0, 0], g["rate"], 0, g["product_distance"])
+        s.add_particles([g["particle"]], [0], ids=[0])
+        s.reactions(po.UA_KEYED, 1.0)
+        pos, _, _, _ = s.particles()
+        assert len(pos) == 2
+        assert (pos[0][0] <= 0) != (pos[1][0] <= 0)
+
+
+def _gillespie_scene():
+    g = GOLDEN["parallel_gillespie_scene"]
+    names = g["types"]
+    tid = {n: i for i, n in enumerate(names)}
+    s = po.System(3, g["box"], g["pbc"], diffusion=[0.25] * 3)
+    for r in g["reactions"]:
+        s.add_reaction(po.RX_FUSION, [tid[r["educts"][0]], tid[r["educts"][1]]], [tid[r["product"]]], r["rate"],
+                       r["radius"])
+    s.add_particles([p[1] for p in g["particles"]], [tid[p[0]] for p in g["particles"]],
+                    ids=np.arange(len(g["particles"])))
+    s.nl_setup(s.max_cutoff())
+    return s, g, tid
+
+
+@pytest.mark.parametrize("mode", [po.UA_KEYED, po.GILLESPIE_LITERAL, po.GILLESPIE_KEYED])
+def test_parallel_gillespie_scene(mode):
+    s, g, tid = _gillespie_scene()
+    s.reactions(mode, g["timestep"])
+    pos, types, _, _ = s.particles()
+    assert len(types) == g["expected_count"]
+    have = [(int(t), tuple(p)) for t, p in zip(types, pos)]
+    for name, p in g["expected_exact"]:
+        assert (tid[name], tuple(float(x) for x in p)) in have
+    for name, p in g["expected_close"]:
+        assert any(t == tid[name] and np.allclose(q, p, rtol=1e-14) for t, q in have)
+
+
+@pytest.mark.parametrize("mode", [po.UA_KEYED, po.GILLESPIE_LITERAL, po.GILLESPIE_KEYED])
+def test_decay_scene_empties_box(mode):
+    g = GOLDEN["decay_scene"]
+    s = po.System(1, g["box"], g["pbc"], diffusion=[g["diffusion"]])
+    s.add_reaction(po.RX_DECAY, [0], [], g["decay_rate"])
+    s.add_reaction(po.RX_FISSION, [0], [0, 0], g["fission_rate"], 0, g["fission_distance"])
+    s.add_particles(np.zeros((g["n_particles"], 3)), [0] * g["n_particles"], ids=np.arange(g["n_particles"]))
+    for t in range(g["steps"]):
+        s.calculate_forces()
+        s.euler_bd(g["timestep"], step=t)
+        s.reactions(mode, g["timestep"], step=t)
+    assert s.n_active == g["expected_remaining"]
+
+
+def test_gillespie_keyed_selection_matches_literal_in_distribution():
+    """Efraimidis-Spirakis keys + greedy over accepted events reproduce performEvents' law.
+
+    Small conflict graph (a chain plus an order-1 event) with unequal rates and acceptance
+    probabilities; the frequency of each outcome pattern is compared by a chi-square test.
+    """
+    p1 = [0, 1, 2, 0, 3]
+    p2 = [1, 2, 3, -1, -1]
+    rate = [1.0, 3.0, 0.5, 2.0, 1.5]
+    acc = [0.6, 0.3, 0.9, 0.5, 0.7]
+    trials = 20000
+    counts = {}
+    for mode in (po.GILLESPIE_LITERAL, po.GILLESPIE_KEYED):
+        c = {}
+        for t in range(trials):
+            out = po.perform_events_abstract(p1, p2, rate, acc, mode, seed=99 + mode, trial=t)
+            key = tuple(out.tolist())
+            c[key] = c.get(key, 0) + 1
+        counts[mode] = c
+    keys = sorted(set(counts[1]) | set(counts[2]))
+    a = np.array([counts[1].get(k, 0) for k in keys], float)
+    b = np.array([counts[2].get(k, 0) for k in keys], float)
+    mask = (a + b) >= 20
+    chi2 = np.sum((a[mask] - b[mask]) ** 2 / (a[mask] + b[mask]))
+    dof = mask.sum() - 1
+    from scipy.stats import chi2 as chi2dist
+    assert chi2dist.sf(chi2, dof) > 1e-3, (chi2, dof, keys, a, b)
+
+
+def test_ua_conserves_complex_counts():
+    # readdy/test/TestReactions.cpp:75-118: #A + #AB and #B + #AB stay constant
+    rng = np.random.default_rng(5)
+    s = po.System(3, [5, 5, 5], [1, 1, 1], diffusion=[1.0, 1.0, 0.0])
+    s.add_reaction(po.RX_FUSION, [0, 1], [2], 0.5, 1.0)
+    s.add_reaction(po.RX_FISSION, [2], [0, 1], 0.5, 0, 1.0)
+    pos = rng.uniform(-2.5, 2.5, (100, 3))
+    s.add_particles(pos, [0, 1] * 50, ids=np.arange(100))
+    s.nl_setup(s.max_cutoff())
+    s.calculate_forces()
+    for t in range(10):
+        for mode in (po.UA_KEYED,):
+            s.loop_step(1.0, mode, t)
+        _, types, _, _ = s.particles()
+        nA, nB, nAB = [(types == k).sum() for k in range(3)]
+        assert nA + nAB == 50 and nB + nAB == 50
