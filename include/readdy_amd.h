@@ -1,0 +1,185 @@
+/*
+ * readdy_amd.h -- C ABI of the MI355X ReaDDy compute engine (libreaddy_amd.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  A failing call
+ * returns a negative status and leaves a message for rdy_last_error() (thread-local).
+ * HIP errors never escape as anything else.
+ *
+ * One engine holds one chemistry (a ReaDDy Context: species, pair potentials, reactions)
+ * and any number of replicas (independent systems with their own box, particles and
+ * Philox seed) that step together in shared launches.  A single ReaDDy simulation is an
+ * engine with one replica.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * readdy/readdy source tree).
+ */
+#ifndef READDY_AMD_H
+#define READDY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RDY_OK 0
+#define RDY_ERR_INVALID (-1)  /* bad argument (std::invalid_argument in the reference)           */
+#define RDY_ERR_HIP (-2)      /* HIP runtime error                                               */
+#define RDY_ERR_CAPACITY (-3) /* a device pool (cells, bins, events, particles) is exhausted     */
+#define RDY_ERR_STATE (-4)    /* call out of order (std::logic_error in the reference)           */
+#define RDY_ERR_NUMERIC (-5)  /* non-finite position produced by the integrator                  */
+
+/* pair potentials: include/readdy/model/potentials/PotentialsOrder2.h:54-323 */
+#define RDY_POT_HARMONIC_REPULSION 0       /* params: force_constant, interaction_distance          */
+#define RDY_POT_WEAK_INTERACTION 1         /* params: force_constant, desired_distance, depth, cutoff */
+#define RDY_POT_LENNARD_JONES 2            /* params: m, n, cutoff, shift, epsilon, sigma           */
+#define RDY_POT_SCREENED_ELECTROSTATICS 3  /* params: C, kappa, D, sigma, exponent, cutoff          */
+
+/* reactions: include/readdy/model/reactions/{Conversion,Fusion,Fission,Enzymatic,Decay}.h */
+#define RDY_RX_CONVERSION 0
+#define RDY_RX_FUSION 1
+#define RDY_RX_FISSION 2
+#define RDY_RX_ENZYMATIC 3 /* educts = {from, catalyst}, products = {to, catalyst} (Enzymatic.h:56-57) */
+#define RDY_RX_DECAY 4
+
+/* reaction handlers: kernels/cpu/src/actions/reactions/{CPUUncontrolledApproximation,CPUGillespie}.cpp */
+#define RDY_HANDLER_UNCONTROLLED_APPROXIMATION 0
+#define RDY_HANDLER_GILLESPIE 1
+#define RDY_HANDLER_NONE 2
+
+typedef struct rdy_engine rdy_engine;
+
+typedef struct {
+    uint32_t kind;
+    uint32_t type1, type2;
+    double params[8];
+} rdy_pair_potential;
+
+typedef struct {
+    uint32_t kind;
+    uint32_t educts[2];
+    uint32_t products[2];
+    double rate;
+    double educt_distance;
+    double product_distance;
+    double weight1, weight2;
+} rdy_reaction;
+
+/* The system description: readdy::model::Context (include/readdy/model/Context.h:69-261). */
+typedef struct {
+    uint32_t n_types;
+    double kbt;
+    const double *diffusion;          /* [n_types]                                            */
+    const uint8_t *topology_flavor;   /* [n_types] or NULL: 1 = topology species (bonded)      */
+    uint32_t n_potentials;
+    const rdy_pair_potential *potentials;
+    uint32_t n_reactions;
+    const rdy_reaction *reactions;
+} rdy_context_desc;
+
+/* Device pools, sized once (288 GB of HBM allows generous defaults). 0 = automatic. */
+typedef struct {
+    double nl_cutoff;         /* neighbour-list cutoff; 0 = Context::calculateMaxCutoff (Context.cpp:126-144) */
+    int32_t cell_radius;      /* cll_radius (KernelConfiguration.h:63): cells of cutoff/radius          */
+    uint32_t bin_capacity;    /* particles per cell slot block; 0 = from the expected density           */
+    uint32_t max_replicas;
+    uint64_t max_cells;       /* total cells over all replicas                                          */
+    uint64_t max_particles;   /* total live particles over all replicas                                 */
+    uint32_t events_per_replica; /* accepted reaction events per replica per step (0 = automatic)       */
+    uint32_t block_size;      /* threads per block of the per-particle kernels (0 = 256)               */
+} rdy_engine_config;
+
+typedef struct {
+    double box[3];
+    int32_t pbc[3];
+    uint64_t seed;            /* Philox key of this replica                                             */
+    uint64_t max_particles;   /* expected peak live particles (sizes its cell grid check and events)    */
+} rdy_replica_desc;
+
+/* ---- lifecycle: replaces KernelProvider::create + CPUKernel construction/initialize
+ *      (readdy/main/plugin/KernelProvider.cpp:183-189, kernels/cpu/src/CPUKernel.cpp:52-83) ---- */
+int rdy_engine_create(const rdy_context_desc *ctx, const rdy_engine_config *cfg, rdy_engine **out);
+int rdy_engine_destroy(rdy_engine *e);
+const char *rdy_last_error(void);
+int rdy_device_info(char *name, size_t name_len, uint64_t *hbm_bytes, int32_t *n_cus);
+
+/* add/remove a replica.  Removing frees its cells and events for the next admission. */
+int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *desc, uint32_t *replica_id);
+int rdy_replica_remove(rdy_engine *e, uint32_t replica);
+/* bytes of HBM a replica of this description would occupy (serving admission) */
+int rdy_replica_footprint(const rdy_engine *e, const rdy_replica_desc *desc, uint64_t *bytes);
+
+/* ---- particles: replaces StateModel::addParticles / getParticles / removeAllParticles
+ *      (include/readdy/model/StateModel.h:66-117, DefaultDataContainer.h:92-118) ----
+ * Particle ids on the device are replica-local uint32 handles.  Uploaded particles get
+ * handles 0..n-1 in upload order (topology species must come first); reaction products
+ * get fresh handles from the replica's counter (Particle::nextId, Particle.h:112-114). */
+int rdy_set_particles(rdy_engine *e, uint32_t replica, uint64_t n, const double *pos, const uint32_t *types);
+uint64_t rdy_particle_count(rdy_engine *e, uint32_t replica);
+/* copies up to cap live particles (order unspecified); returns the count via *n */
+int rdy_get_particles(rdy_engine *e, uint32_t replica, uint64_t cap, double *pos, uint32_t *types,
+                      uint32_t *handles, uint64_t *n);
+/* forces of the last force evaluation, aligned with rdy_get_particles' order of the same state */
+int rdy_get_forces(rdy_engine *e, uint32_t replica, uint64_t cap, double *forces, uint32_t *handles, uint64_t *n);
+
+/* ---- topology bonded terms: replaces GraphTopology::configure's flattened lists and
+ *      CPUCalculate{HarmonicBond,HarmonicAngle,CosineDihedral}Potential
+ *      (readdy/main/model/topologies/GraphTopology.cpp:59-143, CPUTopologyActions.cpp:55-123).
+ * Indices are handles of topology particles of that replica. */
+int rdy_set_bonds(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *idx2, const double *k,
+                  const double *length);
+int rdy_set_angles(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *idx3, const double *k,
+                   const double *theta0);
+int rdy_set_dihedrals(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *idx4, const double *k,
+                      const uint32_t *multiplicity, const double *phi0);
+
+/* ---- actions (include/readdy/model/actions/Actions.h:81-454) ---- */
+/* CellLinkedList::setUpBins (kernels/cpu/src/nl/CellLinkedList.cpp:230-247) */
+int rdy_update_neighbor_list(rdy_engine *e);
+/* CPUCalculateForces::perform (kernels/cpu/src/actions/CPUCalculateForces.cpp:46-212) */
+int rdy_calculate_forces(rdy_engine *e);
+/* total potential energy per replica of the last force evaluation (StateModel::energy) */
+int rdy_get_energy(rdy_engine *e, uint32_t replica, double *energy);
+/* CPUEulerBDIntegrator::perform (kernels/cpu/src/actions/CPUEulerBDIntegrator.cpp:51-103).
+ * noise: NULL for the Philox stream, else a device-independent host array [max_handle][3]
+ * of N(0,1) draws indexed by handle (single-replica engines only; injected-noise parity). */
+int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_handles);
+/* CPUUncontrolledApproximation::perform / CPUGillespie::perform */
+int rdy_react(rdy_engine *e, int handler, double dt);
+/* reaction counts of the last handler call, [n_reactions] (StateModel::reactionCounts) */
+int rdy_get_reaction_counts(rdy_engine *e, uint32_t replica, uint64_t *counts);
+/* reaction counts accumulated since the replica was added, [n_reactions] */
+int rdy_get_reaction_totals(rdy_engine *e, uint32_t replica, uint64_t *counts);
+
+/* ---- the simulation loop: replaces api::SimulationLoop::run's body
+ *      (include/readdy/api/SimulationLoop.h:256-306): per step integrate, neighbour list,
+ *      reactions, neighbour list, forces -- fused into shared launches over all replicas.
+ * Runs n_steps for every active replica; rdy_run_graph captures one step in a hipGraph
+ * and replays it. */
+int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps);
+int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps);
+/* per-replica step budget for serving: a replica stops stepping after this many steps */
+int rdy_replica_set_steps(rdy_engine *e, uint32_t replica, uint64_t steps);
+int rdy_replica_steps_done(rdy_engine *e, uint32_t replica, uint64_t *steps);
+int rdy_synchronize(rdy_engine *e);
+
+/* ---- test exports (parity): the pair set of the cell list, the candidate list ---- */
+/* ordered pairs (handle_i, handle_j) with d2 < cutoff^2 seen by the cell-list traversal */
+int rdy_neighbor_pairs(rdy_engine *e, uint32_t replica, double cutoff, uint64_t cap, uint32_t *pairs,
+                       uint64_t *n);
+/* candidates before the Bernoulli draw: rows {n_educts, handle1, handle2, reaction} */
+int rdy_reaction_candidates(rdy_engine *e, uint32_t replica, uint64_t cap, uint32_t *rows, uint64_t *n);
+
+/* observables: particles per type for every replica, [n_replicas][n_types] */
+int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_replicas);
+
+/* kernel timing of the dominant (fused force/integrate/rebin) kernel, measured with HIP
+ * events on the engine's stream: average microseconds over the launches since the last reset */
+int rdy_profile_reset(rdy_engine *e);
+int rdy_profile_get(rdy_engine *e, double *avg_us_fused, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

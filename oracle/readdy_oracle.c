@@ -112,6 +112,7 @@ void or_fix_position(double *v, const double *box, const int *pbc) {
     for (int d = 0; d < 3; ++d) {
         if (pbc[d]) {
             if (!isfinite(v[d])) continue; /* the reference would spin forever on inf */
+            if (!(fabs(v[d]) < 1024.0 * box[d])) v[d] = fmod(v[d], box[d]);
             while (v[d] >= .5 * box[d]) v[d] -= box[d];
             while (v[d] < -.5 * box[d]) v[d] += box[d];
         }

@@ -1,0 +1,1091 @@
+// Host side of the MI355X ReaDDy engine: device pools, replica admission, the per-step launch
+// sequence (eager and hipGraph), and the extern "C" ABI declared in include/readdy_amd.h.
+#include "rdy_kernels.hip"
+
+#include "../../include/readdy_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace rdy;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(x)                                                                                       \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return set_err(RDY_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostReplica {
+    bool in_use = false;
+    uint64_t cell_base = 0, n_cells = 0;
+    uint32_t topo_base = 0, n_topo = 0;
+    ReplicaDev dev{};
+};
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+}  // namespace
+
+struct rdy_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // chemistry
+    uint32_t n_types = 0, n_rx = 0;
+    double kbt = 1.0;
+    std::vector<double> D;
+    std::vector<uint8_t> topo_flavor;
+    std::vector<rdy_pair_potential> pots_in;
+    std::vector<rdy_reaction> rx_in;
+    // configuration
+    double nl_cutoff = 0.0;
+    int radius = 1;
+    uint32_t cap = 16;
+    uint32_t block = 256;
+    uint32_t max_reps = 1;
+    uint64_t max_cells = 0, max_particles = 0;
+    uint32_t ev_per_rep = RESOLVE_MAX_EVENTS;
+    bool pools_ready = false;
+    std::vector<std::pair<uint64_t, uint64_t>> free_cells;  // (start, length)
+    std::vector<HostReplica> reps_h;
+    uint32_t n_reps_used = 0;  // 1 + highest replica index ever used
+    uint64_t version = 0;      // bumps on any change that invalidates a captured graph
+    // device pools
+    ReplicaDev *reps = nullptr;
+    State *bins[2] = {nullptr, nullptr};
+    uint32_t *counts[2] = {nullptr, nullptr};
+    uint32_t *wl[2] = {nullptr, nullptr};
+    uint32_t *wl_count = nullptr;  // [2]
+    Event *events = nullptr;
+    uint32_t *ev_count = nullptr;
+    unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
+    uint32_t *err = nullptr;
+    double *force_out = nullptr;
+    // tables
+    PotDev *d_pots = nullptr;
+    PairIdx *d_pidx = nullptr;
+    RxDev *d_rx = nullptr;
+    uint32_t *d_o1idx = nullptr, *d_o1list = nullptr, *d_o2list = nullptr;
+    RxPairIdx *d_o2idx = nullptr;
+    double *d_D = nullptr;
+    uint8_t *d_flavor = nullptr;
+    std::vector<RxDev> rx_h;
+    double tables_dt = -1.0;
+    Tables T{};
+    // topology
+    long long *fb = nullptr;
+    uint32_t *loc = nullptr;
+    uint64_t topo_cap = 0, topo_used = 0;
+    std::vector<BondTerm> terms_h[3];
+    BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
+    uint64_t d_terms_cap[3] = {0, 0, 0};
+    // scratch
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double *d_noise = nullptr;
+    size_t noise_cap = 0;
+    int cur = 0;
+    // graph
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint64_t g_version = ~0ull;
+    int g_handler = -1;
+    double g_dt = -1.0;
+    // profiling of the fused kernel
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+    size_t prof_used = 0;
+    double prof_ms = 0.0;
+    uint64_t prof_n = 0;
+
+    Grid grid(int b) const { return Grid{bins[b], counts[b], wl[b], wl_count + b}; }
+    Topo topo() const { return Topo{fb, loc}; }
+};
+
+namespace {
+
+bool is_order2(uint32_t kind) { return kind == RDY_RX_FUSION || kind == RDY_RX_ENZYMATIC; }
+
+double pot_cutoff(const rdy_pair_potential &p) {
+    switch (p.kind) {
+    case RDY_POT_HARMONIC_REPULSION: return p.params[1];
+    case RDY_POT_WEAK_INTERACTION: return p.params[3];
+    case RDY_POT_LENNARD_JONES: return p.params[2];
+    case RDY_POT_SCREENED_ELECTROSTATICS: return p.params[5];
+    default: return 0.0;
+    }
+}
+
+int ensure_scratch(rdy_engine *e, size_t bytes) {
+    if (bytes <= e->scratch_bytes) return RDY_OK;
+    if (e->scratch) HIP_TRY(hipFree(e->scratch));
+    e->scratch = nullptr;
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    HIP_TRY(hipMalloc(&e->scratch, b));
+    e->scratch_bytes = b;
+    return RDY_OK;
+}
+
+int check_errors(rdy_engine *e) {
+    uint32_t h[N_ERR];
+    HIP_TRY(hipMemcpyAsync(h, e->err, sizeof h, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    bool any = false;
+    for (int i = 0; i < N_ERR; ++i) any = any || h[i];
+    if (!any) return RDY_OK;
+    HIP_TRY(hipMemsetAsync(e->err, 0, sizeof(uint32_t) * N_ERR, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (h[ERR_NONFINITE])
+        return set_err(RDY_ERR_NUMERIC, "integrator produced non-finite positions (" + std::to_string(h[ERR_NONFINITE]) +
+                                            " components); reduce the time step");
+    std::string m;
+    if (h[ERR_BIN_OVERFLOW])
+        m += "cell bin overflow (" + std::to_string(h[ERR_BIN_OVERFLOW]) + " particles): raise bin_capacity; ";
+    if (h[ERR_EVENT_OVERFLOW])
+        m += "reaction event segment overflow (" + std::to_string(h[ERR_EVENT_OVERFLOW]) +
+             "): too many accepted reactions per step, reduce the time step; ";
+    if (h[ERR_EVENT_LDS]) m += "more than " + std::to_string(RESOLVE_MAX_EVENTS) + " accepted events in one replica step; ";
+    if (h[ERR_WL_OVERFLOW]) m += "particle pool exhausted: raise max_particles; ";
+    return set_err(RDY_ERR_CAPACITY, m);
+}
+
+// order-dependent tables (p_accept depends on dt)
+int upload_tables(rdy_engine *e, double dt) {
+    if (e->tables_dt == dt && e->d_rx) return RDY_OK;
+    const uint32_t nt = e->n_types;
+    // pair potentials: symmetric type-pair lists in registration order (PotentialRegistry.h:65-70)
+    std::vector<std::vector<uint32_t>> lists((size_t)nt * nt);
+    std::vector<PotDev> pots;
+    for (size_t k = 0; k < e->pots_in.size(); ++k) {
+        const rdy_pair_potential &p = e->pots_in[k];
+        PotDev d{};
+        d.kind = (int32_t)p.kind;
+        const double *q = p.params;
+        switch (p.kind) {
+        case RDY_POT_HARMONIC_REPULSION:
+            d.cut2 = q[1] * q[1];
+            d.p[0] = (float)q[0];
+            d.p[1] = (float)q[1];
+            break;
+        case RDY_POT_WEAK_INTERACTION: {
+            d.cut2 = q[3] * q[3];
+            double len = q[3] - q[1];
+            d.p[0] = (float)q[0];
+            d.p[1] = (float)q[1];
+            d.p[2] = (float)q[2];
+            d.p[3] = (float)q[3];
+            d.p[4] = (float)((1. / (.5 * len)) * (1. / (.5 * len)));
+            d.p[5] = (float)(q[1] + .5 * len);
+            break;
+        }
+        case RDY_POT_LENNARD_JONES: {  // Potentials.cpp:157-171
+            double m = q[0], n = q[1], rc = q[2], eps = q[4], sigma = q[5];
+            double r_min = sigma * std::pow(n / m, 1. / (n - m));
+            double k = -eps / (std::pow(sigma / r_min, m) - std::pow(sigma / r_min, n));
+            d.cut2 = rc * rc;
+            d.ia = (int32_t)m;
+            d.ib = (int32_t)n;
+            d.p[0] = (float)k;
+            d.p[1] = (float)sigma;
+            d.p[2] = q[3] != 0.0 ? (float)(k * (std::pow(sigma / rc, m) - std::pow(sigma / rc, n))) : 0.0f;
+            break;
+        }
+        case RDY_POT_SCREENED_ELECTROSTATICS:
+            d.cut2 = q[5] * q[5];
+            d.ia = (int32_t)q[4];
+            d.p[0] = (float)q[0];
+            d.p[1] = (float)q[1];
+            d.p[2] = (float)q[2];
+            d.p[3] = (float)q[3];
+            break;
+        }
+        uint32_t id = (uint32_t)pots.size();
+        pots.push_back(d);
+        lists[(size_t)p.type1 * nt + p.type2].push_back(id);
+        if (p.type1 != p.type2) lists[(size_t)p.type2 * nt + p.type1].push_back(id);
+    }
+    std::vector<PotDev> flat;
+    std::vector<PairIdx> pidx((size_t)nt * nt);
+    for (size_t k = 0; k < lists.size(); ++k) {
+        pidx[k].start = (uint32_t)flat.size();
+        pidx[k].count = (uint32_t)lists[k].size();
+        pidx[k].maxcut2 = 0.0;
+        for (uint32_t id : lists[k]) {
+            flat.push_back(pots[id]);
+            pidx[k].maxcut2 = std::max(pidx[k].maxcut2, pots[id].cut2);
+        }
+    }
+    // reactions
+    e->rx_h.assign(e->n_rx, RxDev{});
+    std::vector<std::vector<uint32_t>> o1(nt), o2((size_t)nt * nt);
+    for (uint32_t i = 0; i < e->n_rx; ++i) {
+        const rdy_reaction &x = e->rx_in[i];
+        RxDev &d = e->rx_h[i];
+        d.kind = (int32_t)x.kind;
+        d.rate = x.rate;
+        d.p_accept = 1 - std::exp(-x.rate * dt);  // ReactionUtils.h:68-74, approximateRate = false
+        d.educt_dist2 = x.educt_distance * x.educt_distance;
+        d.product_distance = x.product_distance;
+        double w1 = x.weight1, w2 = x.weight2, sum = w1 + w2;
+        if (sum != 1) {  // Fusion.h:61-68, Fission.h:63-70
+            w1 /= sum;
+            w2 /= sum;
+        }
+        d.w1 = w1;
+        d.w2 = w2;
+        d.educts[0] = x.educts[0];
+        d.educts[1] = x.educts[1];
+        d.products[0] = x.products[0];
+        d.products[1] = x.products[1];
+        d.order = is_order2(x.kind) ? 2 : 1;
+        if (d.order == 1) {
+            o1[x.educts[0]].push_back(i);
+        } else {
+            o2[(size_t)x.educts[0] * nt + x.educts[1]].push_back(i);
+            if (x.educts[0] != x.educts[1]) o2[(size_t)x.educts[1] * nt + x.educts[0]].push_back(i);
+        }
+    }
+    std::vector<uint32_t> o1idx(nt + 1), o1list, o2list;
+    std::vector<RxPairIdx> o2idx((size_t)nt * nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        o1idx[t] = (uint32_t)o1list.size();
+        o1list.insert(o1list.end(), o1[t].begin(), o1[t].end());
+    }
+    o1idx[nt] = (uint32_t)o1list.size();
+    bool has_o2 = false;
+    for (size_t k = 0; k < o2.size(); ++k) {
+        o2idx[k].start = (uint32_t)o2list.size();
+        o2idx[k].count = (uint32_t)o2[k].size();
+        has_o2 = has_o2 || !o2[k].empty();
+        o2list.insert(o2list.end(), o2[k].begin(), o2[k].end());
+    }
+    auto up = [&](auto **dst, const auto &v) -> int {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+        if (*dst) HIP_TRY(hipFree(*dst));
+        HIP_TRY(dalloc(dst, v.size()));
+        if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return RDY_OK;
+    };
+    int rc;
+    if ((rc = up(&e->d_pots, flat))) return rc;
+    if ((rc = up(&e->d_pidx, pidx))) return rc;
+    if ((rc = up(&e->d_rx, e->rx_h))) return rc;
+    if ((rc = up(&e->d_o1idx, o1idx))) return rc;
+    if ((rc = up(&e->d_o1list, o1list))) return rc;
+    if ((rc = up(&e->d_o2idx, o2idx))) return rc;
+    if ((rc = up(&e->d_o2list, o2list))) return rc;
+    if ((rc = up(&e->d_D, e->D))) return rc;
+    if (!e->d_flavor) {
+        if ((rc = up(&e->d_flavor, e->topo_flavor))) return rc;
+    }
+    e->T.pots = e->d_pots;
+    e->T.pidx = e->d_pidx;
+    e->T.rx = e->d_rx;
+    e->T.o1idx = e->d_o1idx;
+    e->T.o1list = e->d_o1list;
+    e->T.o2idx = e->d_o2idx;
+    e->T.o2list = e->d_o2list;
+    e->T.diffusion = e->d_D;
+    e->T.n_types = nt;
+    e->T.n_rx = e->n_rx;
+    e->T.has_o2 = has_o2 ? 1u : 0u;
+    e->T.kbt = e->kbt;
+    e->tables_dt = dt;
+    e->version++;
+    return RDY_OK;
+}
+
+int alloc_pools(rdy_engine *e) {
+    if (e->pools_ready) return RDY_OK;
+    if ((unsigned __int128)e->max_cells * e->cap >= ((unsigned __int128)1 << 32))
+        return set_err(RDY_ERR_CAPACITY, "max_cells * bin_capacity must stay below 2^32 slots");
+    const uint64_t slots = e->max_cells * e->cap;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(dalloc(&e->bins[b], slots));
+        HIP_TRY(dalloc(&e->counts[b], e->max_cells));
+        HIP_TRY(hipMemset(e->counts[b], 0, e->max_cells * sizeof(uint32_t)));
+        HIP_TRY(dalloc(&e->wl[b], e->max_particles));
+    }
+    HIP_TRY(dalloc(&e->wl_count, 2));
+    HIP_TRY(hipMemset(e->wl_count, 0, 2 * sizeof(uint32_t)));
+    HIP_TRY(dalloc(&e->reps, e->max_reps));
+    HIP_TRY(hipMemset(e->reps, 0, e->max_reps * sizeof(ReplicaDev)));
+    HIP_TRY(dalloc(&e->events, (size_t)e->max_reps * e->ev_per_rep));
+    HIP_TRY(dalloc(&e->ev_count, e->max_reps));
+    HIP_TRY(hipMemset(e->ev_count, 0, e->max_reps * sizeof(uint32_t)));
+    const size_t nrx = std::max<uint32_t>(e->n_rx, 1);
+    HIP_TRY(dalloc(&e->rx_step, (size_t)e->max_reps * nrx));
+    HIP_TRY(dalloc(&e->rx_total, (size_t)e->max_reps * nrx));
+    HIP_TRY(hipMemset(e->rx_step, 0, (size_t)e->max_reps * nrx * 8));
+    HIP_TRY(hipMemset(e->rx_total, 0, (size_t)e->max_reps * nrx * 8));
+    HIP_TRY(dalloc(&e->e_bonded, e->max_reps));
+    HIP_TRY(hipMemset(e->e_bonded, 0, e->max_reps * 8));
+    HIP_TRY(dalloc(&e->err, N_ERR));
+    HIP_TRY(hipMemset(e->err, 0, N_ERR * sizeof(uint32_t)));
+    e->free_cells.clear();
+    e->free_cells.push_back({0, e->max_cells});
+    e->reps_h.assign(e->max_reps, HostReplica{});
+    e->pools_ready = true;
+    return RDY_OK;
+}
+
+int ensure_force_out(rdy_engine *e) {
+    if (e->force_out) return RDY_OK;
+    HIP_TRY(dalloc(&e->force_out, 3 * e->max_cells * e->cap));
+    HIP_TRY(hipMemset(e->force_out, 0, 3 * e->max_cells * e->cap * sizeof(double)));
+    return RDY_OK;
+}
+
+void cell_dims(const rdy_engine *e, const double *box, int32_t *dims, double *cs) {
+    // CellLinkedList.cpp:64-70
+    double width = e->nl_cutoff / (double)e->radius;
+    for (int d = 0; d < 3; ++d) {
+        double q = std::floor(box[d] / width);
+        dims[d] = (int32_t)std::max(1.0, q);
+        cs[d] = box[d] / (double)dims[d];
+    }
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)std::max<uint64_t>(1, (n + bs - 1) / bs); }
+
+template <class T>
+int put_field(rdy_engine *e, uint32_t r, size_t offset, const T &v) {
+    HIP_TRY(hipMemcpyAsync((char *)(e->reps + r) + offset, &v, sizeof(T), hipMemcpyHostToDevice, e->stream));
+    return RDY_OK;
+}
+
+template <class T>
+int get_field(rdy_engine *e, uint32_t r, size_t offset, T *v) {
+    HIP_TRY(hipMemcpyAsync(v, (char *)(e->reps + r) + offset, sizeof(T), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int check_replica(rdy_engine *e, uint32_t r) {
+    if (!e || !e->pools_ready || r >= e->max_reps || !e->reps_h[r].in_use)
+        return set_err(RDY_ERR_INVALID, "no such replica");
+    return RDY_OK;
+}
+
+// launches the bonded-term kernels on the current buffer
+void launch_bonded(rdy_engine *e) {
+    BondTerm *t;
+    uint32_t n;
+    Grid g = e->grid(e->cur);
+    if ((n = (uint32_t)e->terms_h[0].size()) && (t = e->d_terms[0]))
+        hipLaunchKernelGGL(k_bonds, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, g, e->reps, e->topo(), t, n,
+                           e->e_bonded);
+    if ((n = (uint32_t)e->terms_h[1].size()) && (t = e->d_terms[1]))
+        hipLaunchKernelGGL(k_angles, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, g, e->reps, e->topo(), t, n,
+                           e->e_bonded);
+    if ((n = (uint32_t)e->terms_h[2].size()) && (t = e->d_terms[2]))
+        hipLaunchKernelGGL(k_dihedrals, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, g, e->reps, e->topo(), t, n,
+                           e->e_bonded);
+}
+
+uint32_t wl_blocks(const rdy_engine *e) { return blocks_for(e->max_particles, e->block); }
+
+// the old buffer becomes the next target: zero its cell counters, then its work-list length
+void launch_clear(rdy_engine *e, int b) {
+    Grid old = e->grid(b);
+    hipLaunchKernelGGL(k_clear_old, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, old, e->cap,
+                       (uint32_t)e->max_particles);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, old.wl_count);
+}
+
+// one fused step over all replicas: SimulationLoop.h:280-300 (integrate, NL, reactions, NL; the
+// forces of the resulting state are computed at the start of the next step, fused with its integration)
+void launch_step(rdy_engine *e, int handler, double dt) {
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 2, e->e_bonded);
+    launch_bonded(e);
+    const int c = e->cur, n = 1 - c;
+    if (e->prof && e->prof_used < e->prof_ev.size())
+        (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
+    hipLaunchKernelGGL(k_fused<MODE_STEP>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(n),
+                       e->reps, e->T, e->topo(), e->cap, e->radius, dt, (const double *)nullptr, (double *)nullptr,
+                       (uint32_t)e->max_particles, e->err);
+    if (e->prof && e->prof_used < e->prof_ev.size()) {
+        (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
+        e->prof_used++;
+    }
+    launch_clear(e, c);
+    if (handler != RDY_HANDLER_NONE && e->n_rx > 0) {
+        hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(n), e->reps, e->T,
+                           e->cap, e->radius, handler, 0, e->events, e->ev_count, (uint32_t *)nullptr, 0u, 0u,
+                           (uint32_t)e->max_particles, e->err);
+        hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(n), e->reps, e->T,
+                           e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 1, (uint32_t)e->max_particles,
+                           e->err);
+    }
+    e->cur = n;
+}
+
+void launch_forces(rdy_engine *e, double *force_out) {
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded);
+    launch_bonded(e);
+    const int c = e->cur;
+    hipLaunchKernelGGL(k_fused<MODE_FORCES>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(c),
+                       e->reps, e->T, e->topo(), e->cap, e->radius, 0.0, (const double *)nullptr, force_out,
+                       (uint32_t)e->max_particles, e->err);
+}
+
+int flush_prof(rdy_engine *e) {
+    if (!e->prof) return RDY_OK;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < e->prof_used; ++i) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e->prof_ev[i].first, e->prof_ev[i].second));
+        e->prof_ms += ms;
+        e->prof_n++;
+    }
+    e->prof_used = 0;
+    return RDY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rdy_last_error(void) { return g_err.c_str(); }
+
+int rdy_device_info(char *name, size_t name_len, uint64_t *hbm_bytes, int32_t *n_cus) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, dev));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+    if (n_cus) *n_cus = p.multiProcessorCount;
+    return RDY_OK;
+}
+
+int rdy_engine_create(const rdy_context_desc *ctx, const rdy_engine_config *cfg, rdy_engine **out) {
+    if (!ctx || !out) return set_err(RDY_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (ctx->n_types == 0 || ctx->n_types > MAX_TYPES) return set_err(RDY_ERR_INVALID, "n_types out of range");
+    if (!ctx->diffusion) return set_err(RDY_ERR_INVALID, "diffusion constants missing");
+    for (uint32_t i = 0; i < ctx->n_potentials; ++i) {
+        const rdy_pair_potential &p = ctx->potentials[i];
+        if (p.type1 >= ctx->n_types || p.type2 >= ctx->n_types || p.kind > RDY_POT_SCREENED_ELECTROSTATICS)
+            return set_err(RDY_ERR_INVALID, "invalid pair potential " + std::to_string(i));
+        if (p.kind == RDY_POT_LENNARD_JONES && !(p.params[0] > p.params[1]))
+            return set_err(RDY_ERR_INVALID, "When constructing the LJ potential, the first exponent m was not greater "
+                                            "than the second exponent n");
+        if (!(pot_cutoff(p) > 0)) return set_err(RDY_ERR_INVALID, "pair potential cutoff must be positive");
+    }
+    for (uint32_t i = 0; i < ctx->n_reactions; ++i) {
+        const rdy_reaction &x = ctx->reactions[i];
+        if (x.kind > RDY_RX_DECAY) return set_err(RDY_ERR_INVALID, "invalid reaction kind");
+        int ne = is_order2(x.kind) ? 2 : 1;
+        for (int k = 0; k < ne; ++k)
+            if (x.educts[k] >= ctx->n_types) return set_err(RDY_ERR_INVALID, "reaction educt type out of range");
+        int np = x.kind == RDY_RX_DECAY ? 0 : ((x.kind == RDY_RX_FISSION) ? 2 : 1);
+        for (int k = 0; k < np; ++k)
+            if (x.products[k] >= ctx->n_types) return set_err(RDY_ERR_INVALID, "reaction product type out of range");
+        if (ctx->topology_flavor) {
+            // products of these reactions would leave topologies inconsistent: topology reactions are out of scope
+            bool bad = false;
+            if (x.kind == RDY_RX_ENZYMATIC) bad = ctx->topology_flavor[x.educts[0]] || ctx->topology_flavor[x.products[0]];
+            else {
+                for (int k = 0; k < ne; ++k) bad = bad || ctx->topology_flavor[x.educts[k]];
+                for (int k = 0; k < np; ++k) bad = bad || ctx->topology_flavor[x.products[k]];
+            }
+            if (bad)
+                return set_err(RDY_ERR_INVALID, "reaction " + std::to_string(i) +
+                                                    " consumes or creates topology particles (only an enzymatic "
+                                                    "catalyst may be a topology species)");
+        }
+    }
+    if (ctx->n_reactions > 255) return set_err(RDY_ERR_INVALID, "at most 255 reactions");
+    rdy_engine *e = new rdy_engine();
+    HIP_TRY(hipGetDevice(&e->device));
+    HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->n_types = ctx->n_types;
+    e->kbt = ctx->kbt;
+    e->D.assign(ctx->diffusion, ctx->diffusion + ctx->n_types);
+    e->topo_flavor.assign(ctx->n_types, 0);
+    if (ctx->topology_flavor) e->topo_flavor.assign(ctx->topology_flavor, ctx->topology_flavor + ctx->n_types);
+    e->pots_in.assign(ctx->potentials, ctx->potentials + ctx->n_potentials);
+    e->rx_in.assign(ctx->reactions, ctx->reactions + ctx->n_reactions);
+    e->n_rx = ctx->n_reactions;
+    // Context::calculateMaxCutoff (Context.cpp:126-144)
+    double maxc = 0.0;
+    for (const auto &p : e->pots_in) maxc = std::max(maxc, pot_cutoff(p));
+    for (const auto &x : e->rx_in)
+        if (is_order2(x.kind)) maxc = std::max(maxc, x.educt_distance);
+    rdy_engine_config c{};
+    if (cfg) c = *cfg;
+    e->nl_cutoff = c.nl_cutoff > 0 ? c.nl_cutoff : maxc;
+    if (!(e->nl_cutoff > 0)) e->nl_cutoff = 1.0;  // no interactions: any grid works
+    e->radius = c.cell_radius > 0 ? c.cell_radius : 1;
+    if (e->radius > 3) {
+        delete e;
+        return set_err(RDY_ERR_INVALID, "cell_radius must be 1, 2 or 3");
+    }
+    e->cap = c.bin_capacity ? c.bin_capacity : 16;
+    e->block = c.block_size ? c.block_size : 256;
+    if (e->block % 64 || e->block > 1024) {
+        delete e;
+        return set_err(RDY_ERR_INVALID, "block_size must be a multiple of 64 up to 1024");
+    }
+    e->max_reps = c.max_replicas ? c.max_replicas : 1;
+    if (e->max_reps > MAX_REPLICAS) {
+        delete e;
+        return set_err(RDY_ERR_INVALID, "too many replicas");
+    }
+    e->max_cells = c.max_cells;
+    e->max_particles = c.max_particles;
+    e->ev_per_rep = c.events_per_replica ? std::min<uint32_t>(c.events_per_replica, RESOLVE_MAX_EVENTS)
+                                         : (uint32_t)RESOLVE_MAX_EVENTS;
+    *out = e;
+    return RDY_OK;
+}
+
+int rdy_engine_destroy(rdy_engine *e) {
+    if (!e) return RDY_OK;
+    (void)hipStreamSynchronize(e->stream);
+    for (int b = 0; b < 2; ++b) {
+        if (e->gexec[b]) (void)hipGraphExecDestroy(e->gexec[b]);
+        (void)hipFree(e->bins[b]);
+        (void)hipFree(e->counts[b]);
+        (void)hipFree(e->wl[b]);
+    }
+    void *ptrs[] = {e->wl_count, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+                    e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
+                    e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_terms[2]};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &pe : e->prof_ev) {
+        (void)hipEventDestroy(pe.first);
+        (void)hipEventDestroy(pe.second);
+    }
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return RDY_OK;
+}
+
+int rdy_replica_footprint(const rdy_engine *e, const rdy_replica_desc *d, uint64_t *bytes) {
+    if (!e || !d || !bytes) return set_err(RDY_ERR_INVALID, "null argument");
+    int32_t dims[3];
+    double cs[3];
+    cell_dims(e, d->box, dims, cs);
+    uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
+    *bytes = cells * (2ull * e->cap * sizeof(State) + 2ull * sizeof(uint32_t)) + d->max_particles * 2ull * 4ull +
+             (uint64_t)e->ev_per_rep * sizeof(Event);
+    return RDY_OK;
+}
+
+int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_id) {
+    if (!e || !d || !replica_id) return set_err(RDY_ERR_INVALID, "null argument");
+    for (int k = 0; k < 3; ++k)
+        if (!(d->box[k] > 0)) return set_err(RDY_ERR_INVALID, "box size must be positive");
+    int32_t dims[3];
+    double cs[3];
+    cell_dims(e, d->box, dims, cs);
+    const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
+    if (!e->pools_ready) {
+        if (e->max_cells == 0) e->max_cells = cells;
+        if (e->max_particles == 0) e->max_particles = std::max<uint64_t>(4096, 2 * d->max_particles + 1024);
+        int rc = alloc_pools(e);
+        if (rc) return rc;
+    }
+    uint32_t r = 0;
+    while (r < e->max_reps && e->reps_h[r].in_use) ++r;
+    if (r == e->max_reps) return set_err(RDY_ERR_CAPACITY, "replica table full");
+    // first-fit cell range
+    size_t f = 0;
+    while (f < e->free_cells.size() && e->free_cells[f].second < cells) ++f;
+    if (f == e->free_cells.size()) return set_err(RDY_ERR_CAPACITY, "cell pool exhausted");
+    uint64_t base = e->free_cells[f].first;
+    e->free_cells[f].first += cells;
+    e->free_cells[f].second -= cells;
+    if (e->free_cells[f].second == 0) e->free_cells.erase(e->free_cells.begin() + f);
+    HostReplica &H = e->reps_h[r];
+    H = HostReplica{};
+    H.in_use = true;
+    H.cell_base = base;
+    H.n_cells = cells;
+    ReplicaDev &R = H.dev;
+    for (int k = 0; k < 3; ++k) {
+        R.box[k] = d->box[k];
+        R.cs[k] = cs[k];
+        R.pbc[k] = d->pbc[k] ? 1 : 0;
+        R.dims[k] = dims[k];
+    }
+    R.cell_base = (uint32_t)base;
+    R.n_cells = (uint32_t)cells;
+    R.ev_base = r * e->ev_per_rep;
+    R.ev_cap = e->ev_per_rep;
+    R.handle_next = 0;
+    R.step = 0xFFFFFFFFu;
+    R.active = 1;
+    R.in_use = 1;
+    R.seed = d->seed;
+    R.steps_left = ~0ull;
+    R.steps_done = 0;
+    HIP_TRY(hipMemcpyAsync(e->reps + r, &R, sizeof(ReplicaDev), hipMemcpyHostToDevice, e->stream));
+    const size_t nrx = std::max<uint32_t>(e->n_rx, 1);
+    HIP_TRY(hipMemsetAsync(e->rx_step + (size_t)r * nrx, 0, nrx * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->rx_total + (size_t)r * nrx, 0, nrx * 8, e->stream));
+    e->n_reps_used = std::max(e->n_reps_used, r + 1);
+    e->version++;
+    *replica_id = r;
+    return RDY_OK;
+}
+
+int rdy_replica_remove(rdy_engine *e, uint32_t r) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
+                       (uint32_t)e->max_particles);
+    uint32_t zero = 0;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, in_use), zero))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, active), zero))) return rc;
+    HostReplica &H = e->reps_h[r];
+    e->free_cells.push_back({H.cell_base, H.n_cells});
+    std::sort(e->free_cells.begin(), e->free_cells.end());
+    std::vector<std::pair<uint64_t, uint64_t>> merged;
+    for (auto &fc : e->free_cells) {
+        if (!merged.empty() && merged.back().first + merged.back().second == fc.first) merged.back().second += fc.second;
+        else merged.push_back(fc);
+    }
+    e->free_cells.swap(merged);
+    H.in_use = false;
+    e->version++;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, const uint32_t *types) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (n && (!pos || !types)) return set_err(RDY_ERR_INVALID, "null particle arrays");
+    uint32_t n_topo = 0;
+    bool seen_free = false;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (types[i] >= e->n_types) return set_err(RDY_ERR_INVALID, "particle type out of range");
+        if (e->topo_flavor[types[i]]) {
+            if (seen_free) return set_err(RDY_ERR_INVALID, "topology particles must precede free particles");
+            ++n_topo;
+        } else {
+            seen_free = true;
+        }
+    }
+    // replace: drop the replica's current particles
+    hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
+                       (uint32_t)e->max_particles);
+    HostReplica &H = e->reps_h[r];
+    if (n_topo) {
+        if (e->topo_used + n_topo > e->topo_cap) {
+            uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + n_topo);
+            long long *fb;
+            uint32_t *loc;
+            HIP_TRY(dalloc(&fb, 3 * ncap));
+            HIP_TRY(dalloc(&loc, ncap));
+            HIP_TRY(hipMemsetAsync(fb, 0, 3 * ncap * sizeof(long long), e->stream));
+            if (e->loc) {
+                HIP_TRY(hipMemcpyAsync(loc, e->loc, e->topo_used * 4, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipStreamSynchronize(e->stream));
+                HIP_TRY(hipFree(e->loc));
+                HIP_TRY(hipFree(e->fb));
+            }
+            e->fb = fb;
+            e->loc = loc;
+            e->topo_cap = ncap;
+        }
+        H.topo_base = (uint32_t)e->topo_used;
+        H.n_topo = n_topo;
+        e->topo_used += n_topo;
+    } else {
+        H.topo_base = 0;
+        H.n_topo = 0;
+    }
+    uint32_t hn = (uint32_t)n;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, topo_base), H.topo_base))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, n_topo), H.n_topo))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, handle_next), hn))) return rc;
+    if (n) {
+        size_t bytes = n * (3 * sizeof(double) + sizeof(uint32_t));
+        if ((rc = ensure_scratch(e, bytes))) return rc;
+        double *dpos = (double *)e->scratch;
+        uint32_t *dty = (uint32_t *)(dpos + 3 * n);
+        HIP_TRY(hipMemcpyAsync(dpos, pos, 3 * n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(dty, types, n * sizeof(uint32_t), hipMemcpyHostToDevice, e->stream));
+        if (!e->d_flavor) {
+            HIP_TRY(dalloc(&e->d_flavor, e->n_types));
+            HIP_TRY(hipMemcpyAsync(e->d_flavor, e->topo_flavor.data(), e->n_types, hipMemcpyHostToDevice, e->stream));
+        }
+        hipLaunchKernelGGL(k_upload, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, e->grid(e->cur), e->reps, r,
+                           (uint32_t)n, dpos, dty, (const uint8_t *)e->d_flavor, e->topo(), e->cap,
+                           (uint32_t)e->max_particles, e->err);
+    }
+    e->version++;
+    return check_errors(e);
+}
+
+uint64_t rdy_particle_count(rdy_engine *e, uint32_t r) {
+    uint64_t n = 0;
+    if (rdy_get_particles(e, r, 0, nullptr, nullptr, nullptr, &n)) return 0;
+    return n;
+}
+
+static int gather(rdy_engine *e, uint32_t r, uint64_t cap, double *pos, uint32_t *types, uint32_t *handles,
+                  double *forces, uint64_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    const uint64_t ncap = e->max_particles;
+    size_t bytes = 64 + ncap * (6 * sizeof(double) + 2 * sizeof(uint32_t));
+    if ((rc = ensure_scratch(e, bytes))) return rc;
+    uint32_t *cnt = (uint32_t *)e->scratch;
+    double *dpos = (double *)((char *)e->scratch + 64);
+    double *dfor = dpos + 3 * ncap;
+    uint32_t *dty = (uint32_t *)(dfor + 3 * ncap);
+    uint32_t *dh = dty + ncap;
+    HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_gather, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r, cnt,
+                       (uint32_t)ncap, dpos, dty, dh, (const double *)e->force_out, forces ? dfor : (double *)nullptr,
+                       (uint32_t)e->max_particles);
+    uint32_t hc = 0;
+    HIP_TRY(hipMemcpyAsync(&hc, cnt, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *n = hc;
+    uint64_t m = std::min<uint64_t>(hc, cap);
+    if (m) {
+        if (pos) HIP_TRY(hipMemcpyAsync(pos, dpos, m * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        if (forces) HIP_TRY(hipMemcpyAsync(forces, dfor, m * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        if (types) HIP_TRY(hipMemcpyAsync(types, dty, m * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+        if (handles) HIP_TRY(hipMemcpyAsync(handles, dh, m * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return RDY_OK;
+}
+
+int rdy_get_particles(rdy_engine *e, uint32_t r, uint64_t cap, double *pos, uint32_t *types, uint32_t *handles,
+                      uint64_t *n) {
+    uint64_t dummy;
+    return gather(e, r, cap, pos, types, handles, nullptr, n ? n : &dummy);
+}
+
+int rdy_get_forces(rdy_engine *e, uint32_t r, uint64_t cap, double *forces, uint32_t *handles, uint64_t *n) {
+    if (!e || !e->force_out) return set_err(RDY_ERR_STATE, "no forces have been calculated");
+    uint64_t dummy;
+    return gather(e, r, cap, nullptr, nullptr, handles, forces, n ? n : &dummy);
+}
+
+static int set_terms(rdy_engine *e, uint32_t r, int kind, uint64_t n, const uint32_t *idx, int arity, const double *k,
+                     const double *p0, const uint32_t *mult) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    const HostReplica &H = e->reps_h[r];
+    std::vector<BondTerm> &v = e->terms_h[kind];
+    v.erase(std::remove_if(v.begin(), v.end(), [r](const BondTerm &t) { return t.r == r; }), v.end());
+    for (uint64_t i = 0; i < n; ++i) {
+        BondTerm t{};
+        t.r = r;
+        for (int a = 0; a < arity; ++a) {
+            t.h[a] = idx[arity * i + a];
+            if (t.h[a] >= H.n_topo) return set_err(RDY_ERR_INVALID, "bonded term index is not a topology particle");
+        }
+        t.k = k[i];
+        t.p0 = p0[i];
+        t.mult = mult ? mult[i] : 0;
+        v.push_back(t);
+    }
+    if (v.size() > e->d_terms_cap[kind]) {
+        if (e->d_terms[kind]) HIP_TRY(hipFree(e->d_terms[kind]));
+        e->d_terms_cap[kind] = std::max<uint64_t>(v.size(), 2 * e->d_terms_cap[kind]);
+        HIP_TRY(dalloc(&e->d_terms[kind], e->d_terms_cap[kind]));
+    }
+    if (!v.empty())
+        HIP_TRY(hipMemcpyAsync(e->d_terms[kind], v.data(), v.size() * sizeof(BondTerm), hipMemcpyHostToDevice,
+                               e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->version++;
+    return RDY_OK;
+}
+
+int rdy_set_bonds(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *idx2, const double *k, const double *length) {
+    return set_terms(e, r, 0, n, idx2, 2, k, length, nullptr);
+}
+
+int rdy_set_angles(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *idx3, const double *k,
+                   const double *theta0) {
+    return set_terms(e, r, 1, n, idx3, 3, k, theta0, nullptr);
+}
+
+int rdy_set_dihedrals(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *idx4, const double *k,
+                      const uint32_t *multiplicity, const double *phi0) {
+    return set_terms(e, r, 2, n, idx4, 4, k, phi0, multiplicity);
+}
+
+int rdy_update_neighbor_list(rdy_engine *e) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
+    if (rc) return rc;
+    const int c = e->cur, n = 1 - c;
+    hipLaunchKernelGGL(k_fused<MODE_REBIN>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(n),
+                       e->reps, e->T, e->topo(), e->cap, e->radius, 0.0, (const double *)nullptr, (double *)nullptr,
+                       (uint32_t)e->max_particles, e->err);
+    launch_clear(e, c);
+    e->cur = n;
+    return check_errors(e);
+}
+
+int rdy_calculate_forces(rdy_engine *e) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
+    if (rc) return rc;
+    if ((rc = ensure_force_out(e))) return rc;
+    launch_forces(e, e->force_out);
+    return check_errors(e);
+}
+
+int rdy_get_energy(rdy_engine *e, uint32_t r, double *energy) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    unsigned long long pe = 0, be = 0;
+    if ((rc = get_field(e, r, offsetof(ReplicaDev, energy), &pe))) return rc;
+    HIP_TRY(hipMemcpy(&be, e->e_bonded + r, 8, hipMemcpyDeviceToHost));
+    *energy = 0.5 * (double)(long long)pe * ENERGY_INV + (double)(long long)be * ENERGY_INV;
+    return RDY_OK;
+}
+
+int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_handles) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    const double *dn = nullptr;
+    if (noise) {
+        if (e->n_reps_used != 1) return set_err(RDY_ERR_INVALID, "injected noise needs a single-replica engine");
+        uint32_t hn = 0;
+        if ((rc = get_field(e, 0, offsetof(ReplicaDev, handle_next), &hn))) return rc;
+        if (noise_handles < hn) return set_err(RDY_ERR_INVALID, "noise buffer shorter than the particle handles");
+        if (noise_handles * 3 > e->noise_cap) {
+            if (e->d_noise) HIP_TRY(hipFree(e->d_noise));
+            e->noise_cap = noise_handles * 3;
+            HIP_TRY(dalloc(&e->d_noise, e->noise_cap));
+        }
+        HIP_TRY(hipMemcpyAsync(e->d_noise, noise, noise_handles * 3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+        dn = e->d_noise;
+    }
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 1, e->e_bonded);
+    hipLaunchKernelGGL(k_integrate, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
+                       e->cap, dt, dn, (const double *)e->force_out, (uint32_t)e->max_particles, e->err);
+    return check_errors(e);
+}
+
+int rdy_react(rdy_engine *e, int handler, double dt) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (handler != RDY_HANDLER_UNCONTROLLED_APPROXIMATION && handler != RDY_HANDLER_GILLESPIE)
+        return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    if (e->n_rx == 0) return RDY_OK;
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    const int c = e->cur;
+    hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
+                       e->radius, handler, 0, e->events, e->ev_count, (uint32_t *)nullptr, 0u, 0u,
+                       (uint32_t)e->max_particles, e->err);
+    hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
+                       e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles, e->err);
+    return check_errors(e);
+}
+
+int rdy_get_reaction_counts(rdy_engine *e, uint32_t r, uint64_t *counts) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->n_rx == 0) return RDY_OK;
+    HIP_TRY(hipMemcpyAsync(counts, e->rx_step + (size_t)r * e->n_rx, e->n_rx * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int rdy_get_reaction_totals(rdy_engine *e, uint32_t r, uint64_t *counts) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->n_rx == 0) return RDY_OK;
+    HIP_TRY(hipMemcpyAsync(counts, e->rx_total + (size_t)r * e->n_rx, e->n_rx * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    for (uint64_t s = 0; s < n_steps; ++s) {
+        launch_step(e, handler, dt);
+        if (e->prof && e->prof_used == e->prof_ev.size())
+            if ((rc = flush_prof(e))) return rc;
+        if ((s & 1023) == 1023)
+            if ((rc = check_errors(e))) return rc;
+    }
+    if ((rc = flush_prof(e))) return rc;
+    launch_forces(e, e->force_out);
+    return check_errors(e);
+}
+
+int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    const bool prof = e->prof;
+    e->prof = false;
+    if (e->g_version != e->version || e->g_handler != handler || e->g_dt != dt) {
+        for (int b = 0; b < 2; ++b) {
+            if (e->gexec[b]) HIP_TRY(hipGraphExecDestroy(e->gexec[b]));
+            e->gexec[b] = nullptr;
+        }
+        // capture two steps per parity so that every replay starts and ends on the same buffer
+        const int keep = e->cur;
+        for (int b = 0; b < 2; ++b) {
+            e->cur = b;
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            launch_step(e, handler, dt);
+            launch_step(e, handler, dt);
+            HIP_TRY(hipStreamEndCapture(e->stream, &g));
+            HIP_TRY(hipGraphInstantiate(&e->gexec[b], g, nullptr, nullptr, 0));
+            HIP_TRY(hipGraphDestroy(g));
+        }
+        e->cur = keep;
+        e->g_version = e->version;
+        e->g_handler = handler;
+        e->g_dt = dt;
+    }
+    uint64_t pairs = n_steps / 2;
+    for (uint64_t p = 0; p < pairs; ++p) {
+        HIP_TRY(hipGraphLaunch(e->gexec[e->cur], e->stream));
+        if ((p & 511) == 511)
+            if ((rc = check_errors(e))) return rc;
+    }
+    if (n_steps & 1) launch_step(e, handler, dt);
+    e->prof = prof;
+    launch_forces(e, e->force_out);
+    return check_errors(e);
+}
+
+int rdy_replica_set_steps(rdy_engine *e, uint32_t r, uint64_t steps) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    return put_field(e, r, offsetof(ReplicaDev, steps_left), steps);
+}
+
+int rdy_replica_steps_done(rdy_engine *e, uint32_t r, uint64_t *steps) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    return get_field(e, r, offsetof(ReplicaDev, steps_done), steps);
+}
+
+int rdy_synchronize(rdy_engine *e) {
+    if (!e) return set_err(RDY_ERR_INVALID, "null engine");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_errors(e);
+}
+
+int rdy_neighbor_pairs(rdy_engine *e, uint32_t r, double cutoff, uint64_t cap, uint32_t *pairs, uint64_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    size_t bytes = 64 + 2 * cap * sizeof(uint32_t);
+    if ((rc = ensure_scratch(e, bytes))) return rc;
+    uint32_t *cnt = (uint32_t *)e->scratch;
+    uint32_t *out = cnt + 16;
+    HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_pairs, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->cap,
+                       e->radius, r, cutoff * cutoff, out, (uint32_t)cap, cnt, (uint32_t)e->max_particles);
+    uint32_t hc = 0;
+    HIP_TRY(hipMemcpyAsync(&hc, cnt, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *n = hc;
+    uint64_t m = std::min<uint64_t>(hc, cap);
+    if (m && pairs) {
+        HIP_TRY(hipMemcpy(pairs, out, m * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    return RDY_OK;
+}
+
+int rdy_reaction_candidates(rdy_engine *e, uint32_t r, uint64_t cap, uint32_t *rows, uint64_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if ((rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0))) return rc;
+    size_t bytes = 64 + 4 * cap * sizeof(uint32_t);
+    if ((rc = ensure_scratch(e, bytes))) return rc;
+    uint32_t *cnt = (uint32_t *)e->scratch;
+    uint32_t *out = cnt + 16;
+    HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
+                       e->cap, e->radius, 0, 1, e->events, cnt, out, (uint32_t)cap, r, (uint32_t)e->max_particles,
+                       e->err);
+    uint32_t hc = 0;
+    HIP_TRY(hipMemcpyAsync(&hc, cnt, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    *n = hc;
+    uint64_t m = std::min<uint64_t>(hc, cap);
+    if (m && rows) HIP_TRY(hipMemcpy(rows, out, m * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return check_errors(e);
+}
+
+int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_reps) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    size_t words = (size_t)n_reps * e->n_types;
+    int rc = ensure_scratch(e, words * 8 + 64);
+    if (rc) return rc;
+    unsigned long long *d = (unsigned long long *)e->scratch;
+    HIP_TRY(hipMemsetAsync(d, 0, words * 8, e->stream));
+    hipLaunchKernelGGL(k_count_types, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->n_types, d,
+                       n_reps, (uint32_t)e->max_particles);
+    HIP_TRY(hipMemcpyAsync(counts, d, words * 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int rdy_profile_reset(rdy_engine *e) {
+    if (!e) return set_err(RDY_ERR_INVALID, "null engine");
+    if (e->prof_ev.empty()) {
+        e->prof_ev.resize(256);
+        for (auto &pe : e->prof_ev) {
+            HIP_TRY(hipEventCreate(&pe.first));
+            HIP_TRY(hipEventCreate(&pe.second));
+        }
+    }
+    e->prof = true;
+    e->prof_used = 0;
+    e->prof_ms = 0.0;
+    e->prof_n = 0;
+    return RDY_OK;
+}
+
+int rdy_profile_get(rdy_engine *e, double *avg_us_fused, uint64_t *launches) {
+    if (!e) return set_err(RDY_ERR_INVALID, "null engine");
+    int rc = flush_prof(e);
+    if (rc) return rc;
+    if (avg_us_fused) *avg_us_fused = e->prof_n ? 1000.0 * e->prof_ms / (double)e->prof_n : 0.0;
+    if (launches) *launches = e->prof_n;
+    return RDY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// Kernels of the MI355X ReaDDy engine.  Shared by the engine (rdy_engine.hip); declared here so
+// the host code can launch them by name.
+//
+// Data layout in HBM (one engine):
+//   bins[2][n_cells * cap]  State (32 B): the particle records themselves, grouped by cell.  A
+//                           particle's slot is cell * cap + rank; the record moves to its new
+//                           cell every step (double buffered), so no separate sort pass exists.
+//   counts[2][n_cells]      particles per cell (atomic insertion counters).
+//   wl[2][max_particles]    the work list: slots of live particles of the buffer, appended in
+//                           wave-aggregated blocks (keeps neighbouring particles in neighbouring
+//                           threads across steps).
+//   events[...]             per-replica segments of accepted reaction candidates.
+#pragma once
+
+#include "rdy_device.h"
+
+namespace rdy {
+
+struct Tables {
+    const PotDev *pots;
+    const PairIdx *pidx;     // [n_types * n_types]
+    const RxDev *rx;         // [n_rx]
+    const uint32_t *o1idx;   // [n_types + 1] CSR into o1list
+    const uint32_t *o1list;
+    const RxPairIdx *o2idx;  // [n_types * n_types]
+    const uint32_t *o2list;
+    const double *diffusion; // [n_types]
+    uint32_t n_types;
+    uint32_t n_rx;
+    uint32_t has_o2;         // any order-2 reaction
+    uint32_t pad;
+    double kbt;
+};
+
+struct Grid {  // one bin buffer
+    State *bins;
+    uint32_t *counts;
+    uint32_t *wl;
+    uint32_t *wl_count;
+};
+
+struct Topo {
+    long long *fb;      // [3 * n_topo_total] bonded forces, fixed point 2^-32
+    uint32_t *loc;      // [n_topo_total] current slot of each topology particle
+};
+
+struct Event {
+    unsigned long long key;
+    uint32_t s1, s2;  // slots of entry1 / entry2 (equal for order-1)
+    uint32_t h1, h2;
+    uint32_t rx;
+    uint32_t pad;
+};
+
+// error flags, all counters
+enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
+             N_ERR = 8 };
+
+constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
+constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer
+constexpr int MODE_REBIN = 2;   // re-bin only (neighbour-list update)
+
+constexpr int RESOLVE_THREADS = 256;
+constexpr int RESOLVE_MAX_EVENTS = 1024;  // accepted events per replica per step (LDS resident)
+
+}  // namespace rdy

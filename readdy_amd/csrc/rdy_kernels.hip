@@ -1,0 +1,968 @@
+// Kernels of the MI355X ReaDDy engine (see rdy_kernels.h for the data layout).
+#include "rdy_kernels.h"
+
+namespace rdy {
+
+// -------------------------------------------------------------------------------------------------
+// helpers
+// -------------------------------------------------------------------------------------------------
+
+// XCD-aware block order: the 8 XCDs of an MI355X receive blocks round-robin, so consecutive
+// hardware block ids are remapped to contiguous chunks of the work list per XCD.  Neighbouring
+// particles (and their shared neighbour cells) then stay within one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+    const uint32_t nx = 8;
+    uint32_t per = nblocks / nx;
+    if (per == 0 || b >= per * nx) return b;
+    return (b % nx) * per + b / nx;
+}
+
+__device__ __forceinline__ int axis_cells(int i, int dims, int periodic, int radius, int *out) {
+    if (periodic && dims <= 2 * radius) {
+        for (int j = 0; j < dims; ++j) out[j] = j;
+        return dims;
+    }
+    int n = 0;
+    for (int o = -radius; o <= radius; ++o) {
+        int j = i + o;
+        if (periodic) {
+            j = (j % dims + dims) % dims;
+        } else if (j < 0 || j >= dims) {
+            continue;
+        }
+        out[n++] = j;
+    }
+    return n;
+}
+
+struct RepGeom {
+    double box[3], cs[3];
+    int32_t pbc[3], dims[3];
+    uint32_t cell_base;
+};
+
+__device__ __forceinline__ RepGeom load_geom(const ReplicaDev &R) {
+    RepGeom g;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        g.box[d] = R.box[d];
+        g.cs[d] = R.cs[d];
+        g.pbc[d] = R.pbc[d];
+        g.dims[d] = R.dims[d];
+    }
+    g.cell_base = R.cell_base;
+    return g;
+}
+
+// visits every slot of the stencil around local cell `cl` (own cell included; the caller skips
+// itself): the (2r+1)^3 adjacency of CellLinkedList.cpp:74-137 with the same de-duplication
+template <class F>
+__device__ __forceinline__ void for_each_stencil_slot(const RepGeom &g, const uint32_t *counts, uint32_t cap,
+                                                      int radius, uint32_t cl, F &&f) {
+    const int nz = g.dims[2], ny = g.dims[1];
+    int iz = (int)(cl % (uint32_t)nz);
+    int iy = (int)((cl / (uint32_t)nz) % (uint32_t)ny);
+    int ix = (int)(cl / ((uint32_t)nz * (uint32_t)ny));
+    int ax[7], ay[7], az[7];
+    int nax = axis_cells(ix, g.dims[0], g.pbc[0], radius, ax);
+    int nay = axis_cells(iy, g.dims[1], g.pbc[1], radius, ay);
+    int naz = axis_cells(iz, g.dims[2], g.pbc[2], radius, az);
+    for (int a = 0; a < nax; ++a)
+        for (int b = 0; b < nay; ++b) {
+            uint32_t row = g.cell_base + ((uint32_t)ax[a] * (uint32_t)ny + (uint32_t)ay[b]) * (uint32_t)nz;
+            for (int c = 0; c < naz; ++c) {
+                uint32_t cell = row + (uint32_t)az[c];
+                uint32_t n = counts[cell];
+                n = n < cap ? n : cap;
+                uint32_t base = cell * cap;
+                for (uint32_t k = 0; k < n; ++k) f(base + k);
+            }
+        }
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << __lane_id()) - 1ull;
+}
+
+// wave-aggregated append: returns the position for lanes with want == true
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
+    unsigned long long mask = __ballot(want);
+    uint32_t cnt = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((long long)mask) - 1;
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, cnt);
+        base = __shfl(base, leader);
+    }
+    return base + (uint32_t)__popcll(mask & lanemask_lt());
+}
+
+// per-wave reduction of a fixed-point value into reps[r].<field>; lanes may differ in replica
+__device__ __forceinline__ void wave_add_energy(ReplicaDev *reps, bool valid, uint32_t r, long long e) {
+    unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    int leader = __ffsll((long long)vm) - 1;
+    uint32_t rl = __shfl(r, leader);
+    bool same = !valid || r == rl;
+    if (__all(same)) {
+        long long v = valid ? e : 0;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if ((int)__lane_id() == leader && v != 0) atomicAdd(&reps[rl].energy, (unsigned long long)v);
+    } else if (valid && e != 0) {
+        atomicAdd(&reps[r].energy, (unsigned long long)e);
+    }
+}
+
+// places `st` (position already wrapped) into buffer g; returns the slot or UINT32_MAX
+__device__ __forceinline__ uint32_t bin_insert(const Grid &g, const RepGeom &geo, uint32_t cap, State st,
+                                               uint32_t *err, bool append_wl, uint32_t wl_cap) {
+    const double p[3] = {st.x, st.y, st.z};
+    int32_t c[3];
+    bool outside = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!geo.pbc[d] && !in_box_axis(p[d], geo.box[d])) outside = true;
+        c[d] = cell_coord(p[d], geo.box[d], geo.cs[d], geo.dims[d]);
+    }
+    st.tw = (st.tw & ~TW_OUTSIDE) | (outside ? TW_OUTSIDE : 0u);
+    uint32_t cell = geo.cell_base + ((uint32_t)c[0] * (uint32_t)geo.dims[1] + (uint32_t)c[1]) * (uint32_t)geo.dims[2] +
+                    (uint32_t)c[2];
+    uint32_t k = atomicAdd(&g.counts[cell], 1u);
+    if (k >= cap) {
+        atomicAdd(&err[ERR_BIN_OVERFLOW], 1u);
+        return 0xFFFFFFFFu;
+    }
+    uint32_t slot = cell * cap + k;
+    g.bins[slot] = st;
+    if (append_wl) {
+        uint32_t pos = atomicAdd(g.wl_count, 1u);
+        if (pos < wl_cap) g.wl[pos] = slot;
+        else atomicAdd(&err[ERR_WL_OVERFLOW], 1u);
+    }
+    return slot;
+}
+
+// -------------------------------------------------------------------------------------------------
+// per-step bookkeeping
+// -------------------------------------------------------------------------------------------------
+// mode 0: zero energies; mode 1: advance the Philox step of replicas with budget left
+__global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned long long *e_bonded) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reps) return;
+    ReplicaDev &R = reps[r];
+    R.energy = 0ull;
+    e_bonded[r] = 0ull;
+    if (mode == 1) {
+        if (R.in_use && R.steps_left > 0) {
+            R.active = 1u;
+            R.step += 1u;
+            R.steps_left -= 1u;
+            R.steps_done += 1u;
+        } else {
+            R.active = 0u;
+        }
+    }
+}
+
+__global__ void k_clear_old(Grid old, uint32_t cap, uint32_t max_wl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *old.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i < n) old.counts[old.wl[i] / cap] = 0u;
+}
+
+__global__ void k_reset_counter(uint32_t *c) { *c = 0u; }
+
+// -------------------------------------------------------------------------------------------------
+// the fused per-particle kernel: forces (+ Euler-Maruyama + re-bin)
+// Restates CPUCalculateForces::calculateOrder2 (CPUCalculateForces.cpp:214-268), the integrator
+// (CPUEulerBDIntegrator.cpp:63-73) and CompactCellLinkedList::fillBins (CellLinkedList.cpp:171-228).
+// -------------------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fused(Grid cur, Grid nxt, ReplicaDev *reps, Tables T, Topo topo,
+                                                uint32_t cap, int radius, double dt, const double *noise,
+                                                double *force_out, uint32_t max_wl, uint32_t *err) {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = b * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    bool valid = i < n;
+    uint32_t s = 0;
+    State me = {0.0, 0.0, 0.0, 0u, 0u};
+    if (valid) {
+        s = cur.wl[i];
+        me = cur.bins[s];
+        if (me.tw & TW_DEAD) valid = false;
+    }
+    const uint32_t r = tw_replica(me.tw);
+    long long fx = 0, fy = 0, fz = 0, e = 0;
+    RepGeom geo;
+    uint32_t active = 0, step = 0;
+    uint64_t seed = 0;
+    if (valid) {
+        const ReplicaDev &R = reps[r];
+        geo = load_geom(R);
+        active = R.active;
+        step = R.step;
+        seed = R.seed;
+    }
+    const bool do_forces = valid && (MODE == MODE_FORCES || (MODE == MODE_STEP && active));
+    if (do_forces && !(me.tw & TW_OUTSIDE)) {
+        const uint32_t t = tw_type(me.tw);
+        const PairIdx *prow = T.pidx + (size_t)t * T.n_types;
+        const uint32_t cl = s / cap - geo.cell_base;
+        for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+            if (s2 == s) return;
+            const State o = cur.bins[s2];
+            if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
+            const PairIdx pi = prow[tw_type(o.tw)];
+            if (pi.count == 0) return;
+            const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
+            const double dy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
+            const double dz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
+            const double d2 = dist2(dx, dy, dz);
+            if (!(d2 < pi.maxcut2)) return;
+            float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
+            const float xf = (float)dx, yf = (float)dy, zf = (float)dz, d2f = (float)d2;
+            for (uint32_t q = 0; q < pi.count; ++q) {
+                const PotDev pt = T.pots[pi.start + q];
+                if (d2 < pt.cut2) pe += pair_force_energy(pt, d2f, xf, yf, zf, pfx, pfy, pfz);
+            }
+            fx += to_fixed(pfx, 4294967296.0f);
+            fy += to_fixed(pfy, 4294967296.0f);
+            fz += to_fixed(pfz, 4294967296.0f);
+            e += to_fixed(pe, 16777216.0f);
+        });
+    }
+    if (do_forces && (me.tw & TW_TOPO) && topo.fb) {
+        long long *fb = topo.fb + 3ull * (reps[r].topo_base + me.h);
+        fx += fb[0];
+        fy += fb[1];
+        fz += fb[2];
+        fb[0] = 0;
+        fb[1] = 0;
+        fb[2] = 0;
+    }
+    if (MODE != MODE_REBIN) wave_add_energy(reps, do_forces, r, e);
+    const double Fx = (double)fx * FORCE_INV, Fy = (double)fy * FORCE_INV, Fz = (double)fz * FORCE_INV;
+    if (MODE != MODE_REBIN && force_out && do_forces) {
+        force_out[3ull * s] = Fx;
+        force_out[3ull * s + 1] = Fy;
+        force_out[3ull * s + 2] = Fz;
+    }
+    if (MODE == MODE_FORCES) return;
+
+    // ---- Euler-Maruyama step (CPUEulerBDIntegrator.cpp:63-73) ----
+    State nw = me;
+    if (MODE == MODE_STEP && valid && active) {
+        const double D = T.diffusion[tw_type(me.tw)];
+        double xi[3];
+        if (noise) {
+            xi[0] = noise[3ull * me.h];
+            xi[1] = noise[3ull * me.h + 1];
+            xi[2] = noise[3ull * me.h + 2];
+        } else {
+            uint32_t x4[4];
+            philox(me.h, step, 0u, 0u, seed, x4);
+            normals3(x4, xi);
+        }
+        const double sc = sqrt(2. * D * dt);
+        const double kbt = T.kbt;
+        const double F[3] = {Fx, Fy, Fz};
+        double p[3] = {me.x, me.y, me.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double det = F[d] * D * dt / kbt;
+            double rnd = xi[d] * sc;
+            p[d] += rnd + det;
+            p[d] = wrap(p[d], geo.box[d], geo.pbc[d]);
+            if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
+        }
+        nw.x = p[0];
+        nw.y = p[1];
+        nw.z = p[2];
+    }
+    // ---- re-bin into the other buffer, wave-aggregated work-list append ----
+    uint32_t slot = 0xFFFFFFFFu;
+    if (valid) slot = bin_insert(nxt, geo, cap, nw, err, false, 0);
+    const bool keep = valid && slot != 0xFFFFFFFFu;
+    const uint32_t pos = wave_append(nxt.wl_count, keep);
+    if (keep) {
+        if (pos < max_wl) nxt.wl[pos] = slot;
+        else atomicAdd(&err[ERR_WL_OVERFLOW], 1u);
+        if ((me.tw & TW_TOPO) && topo.loc) topo.loc[reps[r].topo_base + me.h] = slot;
+    }
+}
+
+template __global__ void k_fused<MODE_FORCES>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
+                                              const double *, double *, uint32_t, uint32_t *);
+template __global__ void k_fused<MODE_STEP>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
+                                            const double *, double *, uint32_t, uint32_t *);
+template __global__ void k_fused<MODE_REBIN>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
+                                             const double *, double *, uint32_t, uint32_t *);
+
+// separate integrator action: in place on the current buffer, forces from force_out
+__global__ __launch_bounds__(256) void k_integrate(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, double dt,
+                                                    const double *noise, const double *force_in, uint32_t max_wl,
+                                                    uint32_t *err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const uint32_t s = cur.wl[i];
+    State me = cur.bins[s];
+    if (me.tw & TW_DEAD) return;
+    const ReplicaDev &R = reps[tw_replica(me.tw)];
+    if (!R.active) return;
+    const double D = T.diffusion[tw_type(me.tw)];
+    double xi[3];
+    if (noise) {
+        xi[0] = noise[3ull * me.h];
+        xi[1] = noise[3ull * me.h + 1];
+        xi[2] = noise[3ull * me.h + 2];
+    } else {
+        uint32_t x4[4];
+        philox(me.h, R.step, 0u, 0u, R.seed, x4);
+        normals3(x4, xi);
+    }
+    const double sc = sqrt(2. * D * dt);
+    double p[3] = {me.x, me.y, me.z};
+    for (int d = 0; d < 3; ++d) {
+        double F = force_in ? force_in[3ull * s + d] : 0.0;
+        double det = F * D * dt / T.kbt;
+        double rnd = xi[d] * sc;
+        p[d] += rnd + det;
+        p[d] = wrap(p[d], R.box[d], R.pbc[d]);
+        if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
+    }
+    me.x = p[0];
+    me.y = p[1];
+    me.z = p[2];
+    cur.bins[s] = me;
+}
+
+// -------------------------------------------------------------------------------------------------
+// bonded terms (CPUTopologyActions.cpp:55-123): FP64, fixed-point atomics into the per-particle
+// bonded force; energies into e_bonded[replica] (fixed point 2^-24)
+// -------------------------------------------------------------------------------------------------
+struct BondTerm {
+    uint32_t r, h[4];
+    uint32_t mult;
+    double k, p0;
+};
+
+__device__ __forceinline__ void pos_of(const Grid &g, const Topo &topo, const ReplicaDev &R, uint32_t h, double *p) {
+    const State st = g.bins[topo.loc[R.topo_base + h]];
+    p[0] = st.x;
+    p[1] = st.y;
+    p[2] = st.z;
+}
+
+__device__ __forceinline__ void sdiff(const double *a, const double *b, const ReplicaDev &R, double *o) {
+    for (int d = 0; d < 3; ++d) o[d] = min_image(b[d] - a[d], R.box[d], R.pbc[d]);
+}
+
+__device__ __forceinline__ double dot(const double *a, const double *b) {
+    double s = a[0] * b[0];
+    s = s + a[1] * b[1];
+    s = s + a[2] * b[2];
+    return s;
+}
+
+__device__ __forceinline__ void cross(const double *a, const double *b, double *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ void add_fb(const Topo &topo, const ReplicaDev &R, uint32_t h, const double *f) {
+    long long *fb = topo.fb + 3ull * (R.topo_base + h);
+    for (int d = 0; d < 3; ++d) atomicAdd((unsigned long long *)&fb[d], (unsigned long long)to_fixed_d(f[d], FORCE_SCALE));
+}
+
+__global__ void k_bonds(Grid cur, ReplicaDev *reps, Topo topo, const BondTerm *terms, uint32_t n,
+                        unsigned long long *e_bonded) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BondTerm t = terms[i];
+    const ReplicaDev &R = reps[t.r];
+    if (!R.in_use || t.k == 0) return;
+    double a[3], b[3], x[3];
+    pos_of(cur, topo, R, t.h[0], a);
+    pos_of(cur, topo, R, t.h[1], b);
+    sdiff(a, b, R, x);  // x_ij = x_j - x_i
+    double norm = sqrt(dot(x, x));
+    double c = (2. * t.k * (norm - t.p0) / norm);
+    double fi[3] = {c * x[0], c * x[1], c * x[2]}, fj[3] = {-fi[0], -fi[1], -fi[2]};
+    add_fb(topo, R, t.h[0], fi);
+    add_fb(topo, R, t.h[1], fj);
+    double e = t.k * (norm - t.p0) * (norm - t.p0);
+    atomicAdd(&e_bonded[t.r], (unsigned long long)to_fixed_d(e, ENERGY_SCALE));
+}
+
+__global__ void k_angles(Grid cur, ReplicaDev *reps, Topo topo, const BondTerm *terms, uint32_t n,
+                         unsigned long long *e_bonded) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BondTerm t = terms[i];
+    const ReplicaDev &R = reps[t.r];
+    if (!R.in_use) return;
+    const double SMALL = 1e-10;  // AnglePotential.cpp:49
+    double pi_[3], pj[3], pk[3], xji[3], xjk[3];
+    pos_of(cur, topo, R, t.h[0], pi_);
+    pos_of(cur, topo, R, t.h[1], pj);
+    pos_of(cur, topo, R, t.h[2], pk);
+    sdiff(pj, pi_, R, xji);
+    sdiff(pj, pk, R, xjk);
+    double sp = dot(xji, xjk);
+    double e = t.k * (acos(sp / (sqrt(dot(xji, xji)) * sqrt(dot(xjk, xjk)))) - t.p0) *
+               (acos(sp / (sqrt(dot(xji, xji)) * sqrt(dot(xjk, xjk)))) - t.p0);
+    double nji2 = dot(xji, xji);
+    if (nji2 < SMALL) nji2 = SMALL;
+    double njk2 = dot(xjk, xjk);
+    if (njk2 < SMALL) njk2 = SMALL;
+    double nprod = sqrt(nji2) * sqrt(njk2);
+    if (nprod < SMALL) nprod = SMALL;
+    double inv = 1. / nprod;
+    double ct = fmin(fmax(inv * sp, -1.), 1.);
+    double si = sqrt(1. - ct * ct);
+    if (si < SMALL) si = SMALL;
+    si = 1. / si;
+    double c = 2. * t.k * (acos(ct) - t.p0) * si;
+    double f_i[3], f_j[3], f_k[3];
+    for (int d = 0; d < 3; ++d) {
+        double force_i = c * ct * (1. / nji2) * xji[d] - c * inv * xjk[d];
+        double force_k = -c * inv * xji[d] + c * ct * (1. / njk2) * xjk[d];
+        f_i[d] = -force_i;
+        f_j[d] = force_i + force_k;
+        f_k[d] = -force_k;
+    }
+    add_fb(topo, R, t.h[0], f_i);
+    add_fb(topo, R, t.h[1], f_j);
+    add_fb(topo, R, t.h[2], f_k);
+    atomicAdd(&e_bonded[t.r], (unsigned long long)to_fixed_d(e, ENERGY_SCALE));
+}
+
+// cosine dihedral: energy as TorsionPotential.cpp:53-70; force = analytic gradient of the same
+// angle (closed form of the IUPAC dihedral gradient, phi_ref = -psi - pi), zero when |cos phi| < 1e-4
+__global__ void k_dihedrals(Grid cur, ReplicaDev *reps, Topo topo, const BondTerm *terms, uint32_t n,
+                            unsigned long long *e_bonded) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BondTerm t = terms[i];
+    const ReplicaDev &R = reps[t.r];
+    if (!R.in_use) return;
+    const double SMALL = 1e-4;
+    double pi_[3], pj[3], pk[3], pl[3], xji[3], xkj[3], xkl[3], xjk[3];
+    pos_of(cur, topo, R, t.h[0], pi_);
+    pos_of(cur, topo, R, t.h[1], pj);
+    pos_of(cur, topo, R, t.h[2], pk);
+    pos_of(cur, topo, R, t.h[3], pl);
+    sdiff(pj, pi_, R, xji);
+    sdiff(pk, pj, R, xkj);
+    sdiff(pk, pl, R, xkl);
+    for (int d = 0; d < 3; ++d) xjk[d] = -1. * xkj[d];
+    double m[3], nn[3], mxn[3];
+    cross(xji, xkj, m);
+    cross(xkl, xjk, nn);
+    cross(m, nn, mxn);
+    double jkn = sqrt(dot(xjk, xjk));
+    if (jkn < SMALL) jkn = SMALL;
+    double mnn = sqrt(dot(m, m)) * sqrt(dot(nn, nn));
+    if (mnn < SMALL) mnn = SMALL;
+    double phi_e = -atan2(dot(mxn, xjk) / (mnn * jkn), dot(m, nn) / mnn);
+    double e = t.k * (1 + cos((double)t.mult * phi_e - t.p0));
+    atomicAdd(&e_bonded[t.r], (unsigned long long)to_fixed_d(e, ENERGY_SCALE));
+
+    double x2 = dot(xjk, xjk);
+    if (x2 < SMALL) x2 = SMALL;
+    double m2 = dot(m, m);
+    if (m2 < SMALL) m2 = SMALL;
+    double n2 = dot(nn, nn);
+    if (n2 < SMALL) n2 = SMALL;
+    double mn = sqrt(m2) * sqrt(n2);
+    if (mn < SMALL) mn = SMALL;
+    double cos_phi = dot(nn, m) / mn;
+    if (fabs(cos_phi) < SMALL) return;
+    double sin_phi = dot(mxn, xjk) / (mn * sqrt(x2));
+    double phi = -atan2(sin_phi, cos_phi);
+    double dV = -t.k * (double)t.mult * sin((double)t.mult * phi - t.p0);
+    double b1[3], b2[3], b3[3], A[3], B[3];
+    for (int d = 0; d < 3; ++d) {
+        b1[d] = -xji[d];
+        b2[d] = xjk[d];
+        b3[d] = xkl[d];
+    }
+    cross(b1, b2, A);
+    cross(b2, b3, B);
+    double A2 = dot(A, A), B2 = dot(B, B);
+    if (A2 < SMALL) A2 = SMALL;
+    if (B2 < SMALL) B2 = SMALL;
+    double b2n = sqrt(x2);
+    double p12 = dot(b1, b2) / x2, p32 = dot(b3, b2) / x2;
+    double fi[3], fj[3], fk[3], fl[3];
+    for (int d = 0; d < 3; ++d) {
+        double gi = -b2n * A[d] / A2, gl = b2n * B[d] / B2;
+        fi[d] = dV * gi;
+        fj[d] = dV * (-(1.0 + p12) * gi + p32 * gl);
+        fk[d] = dV * (p12 * gi - (1.0 + p32) * gl);
+        fl[d] = dV * gl;
+    }
+    add_fb(topo, R, t.h[0], fi);
+    add_fb(topo, R, t.h[1], fj);
+    add_fb(topo, R, t.h[2], fk);
+    add_fb(topo, R, t.h[3], fl);
+}
+
+// -------------------------------------------------------------------------------------------------
+// reaction candidate search (CPUUncontrolledApproximation.cpp:70-133, ReactionUtils.h:83-144)
+// Each unordered pair is found once, by the particle with the smaller (type, handle).  Acceptance
+// u < 1 - exp(-rate dt) is drawn per candidate from Philox keyed by the pair's handles, so the
+// candidate set and the draws do not depend on slots, thread order or batching.
+// -------------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long event_key(int handler, const uint32_t x4[4], double rate) {
+    if (handler == 0) return ((unsigned long long)x4[2] << 32) | x4[3];
+    double u = u53(x4[2], x4[3]) + 0x1p-54;
+    double k = -log(u) / rate;  // Efraimidis-Spirakis key, positive: bit order == numeric order
+    return (unsigned long long)__double_as_longlong(k);
+}
+
+__global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, int radius,
+                                                 int handler, int candidates_only, Event *events,
+                                                 uint32_t *ev_count, uint32_t *cand_rows, uint32_t cand_cap,
+                                                 uint32_t only_replica, uint32_t max_wl, uint32_t *err) {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = b * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const uint32_t s = cur.wl[i];
+    const State me = cur.bins[s];
+    if (me.tw & TW_DEAD) return;
+    const uint32_t r = tw_replica(me.tw);
+    if (candidates_only && r != only_replica) return;
+    const ReplicaDev &R = reps[r];
+    if (!candidates_only && !R.active) return;
+    const uint32_t t = tw_type(me.tw);
+    const uint32_t step = R.step;
+    const uint64_t seed = R.seed;
+
+    auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key,
+                    uint32_t ne) {
+        if (candidates_only) {
+            uint32_t p = atomicAdd(&ev_count[0], 1u);
+            if (p < cand_cap) {
+                cand_rows[4 * p] = ne;
+                cand_rows[4 * p + 1] = h1;
+                cand_rows[4 * p + 2] = h2;
+                cand_rows[4 * p + 3] = rx;
+            }
+            return;
+        }
+        uint32_t p = atomicAdd(&ev_count[r], 1u);
+        if (p < R.ev_cap) {
+            Event ev;
+            ev.key = key;
+            ev.s1 = s1;
+            ev.s2 = s2;
+            ev.h1 = h1;
+            ev.h2 = h2;
+            ev.rx = rx;
+            ev.pad = 0;
+            events[R.ev_base + p] = ev;
+        } else {
+            atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
+        }
+    };
+
+    // order 1
+    for (uint32_t k = T.o1idx[t]; k < T.o1idx[t + 1]; ++k) {
+        const uint32_t rx = T.o1list[k];
+        const RxDev X = T.rx[rx];
+        if (!(X.rate > 0)) continue;
+        uint32_t x4[4];
+        philox(me.h, step, 1u | (rx << 8), 0u, seed, x4);
+        if (!candidates_only && !(u53(x4[0], x4[1]) < X.p_accept)) continue;
+        emit(rx, s, s, me.h, me.h, event_key(handler, x4, X.rate), 1u);
+    }
+    // order 2
+    if (!T.has_o2 || (me.tw & TW_OUTSIDE)) return;
+    const RepGeom geo = load_geom(R);
+    const RxPairIdx *orow = T.o2idx + (size_t)t * T.n_types;
+    const uint32_t cl = s / cap - geo.cell_base;
+    for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+        if (s2 == s) return;
+        const State o = cur.bins[s2];
+        if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
+        const uint32_t t2 = tw_type(o.tw);
+        if (t2 < t || (t2 == t && o.h < me.h)) return;
+        const RxPairIdx pi = orow[t2];
+        if (pi.count == 0) return;
+        // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
+        const double dx = min_image(me.x - o.x, geo.box[0], geo.pbc[0]);
+        const double dy = min_image(me.y - o.y, geo.box[1], geo.pbc[1]);
+        const double dz = min_image(me.z - o.z, geo.box[2], geo.pbc[2]);
+        const double d2 = dist2(dx, dy, dz);
+        const uint32_t lo = me.h < o.h ? me.h : o.h, hi = me.h < o.h ? o.h : me.h;
+        for (uint32_t q = 0; q < pi.count; ++q) {
+            const uint32_t rx = T.o2list[pi.start + q];
+            const RxDev X = T.rx[rx];
+            if (!(X.rate > 0 && d2 < X.educt_dist2)) continue;
+            uint32_t x4[4];
+            philox(lo, step, 2u | (rx << 8), hi, seed, x4);
+            if (!candidates_only && !(u53(x4[0], x4[1]) < X.p_accept)) continue;
+            // entry1: the educt of type educts[0]; equal educt types: the lower handle
+            bool me_first;
+            if (X.educts[0] != X.educts[1]) me_first = (t == X.educts[0]);
+            else me_first = me.h < o.h;
+            if (me_first) emit(rx, s, s2, me.h, o.h, event_key(handler, x4, X.rate), 2u);
+            else emit(rx, s2, s, o.h, me.h, event_key(handler, x4, X.rate), 2u);
+        }
+    });
+}
+
+// -------------------------------------------------------------------------------------------------
+// conflict resolution + reaction execution, one workgroup per replica.
+// Both handlers reduce to a greedy maximal independent set over the accepted candidates in
+// ascending key order (UA: uniform keys = the shuffle of CPUUncontrolledApproximation.cpp:198-253;
+// Gillespie: Efraimidis-Spirakis keys = the weighted selection order of algorithm.h:62-136).  The
+// greedy result is computed with parallel rounds (an event wins when it holds the smallest index
+// on all of its particles among undecided events), which equals the sequential scan.
+// Products follow performReaction (ReactionUtils.h:146-227); fresh handles are assigned in key order.
+// -------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t mask) { return (key * 2654435761u) & mask; }
+
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap,
+                                                             Event *events, uint32_t *ev_count,
+                                                             unsigned long long *rx_step, unsigned long long *rx_total,
+                                                             int require_active, uint32_t max_wl, uint32_t *err) {
+    constexpr int E = RESOLVE_MAX_EVENTS;
+    constexpr int HT = 4 * E;  // hash table entries (power of two, > 2 slots per event)
+    __shared__ unsigned long long skey[E];
+    __shared__ uint32_t sidx[E];
+    __shared__ uint32_t hkeys[HT];
+    __shared__ uint32_t hmin[HT];
+    __shared__ uint8_t htaken[HT];
+    __shared__ uint8_t status[E];   // 0 undecided, 1 won, 2 lost
+    __shared__ uint32_t ent1[E], ent2[E];
+    uint32_t *prefix = hmin;  // free once the independent set is decided
+    __shared__ uint32_t s_undecided, s_total;
+    __shared__ uint32_t rxcount[256];
+
+    const uint32_t r = blockIdx.x;
+    ReplicaDev &R = reps[r];
+    const int tid = threadIdx.x;
+    if (!R.in_use || (require_active && !R.active)) {
+        if (tid == 0) ev_count[r] = 0;
+        return;
+    }
+    uint32_t ne = ev_count[r];
+    __syncthreads();
+    if (tid < (int)T.n_rx && tid < 256) {
+        rx_step[(size_t)r * T.n_rx + tid] = 0ull;
+        rxcount[tid] = 0;
+    }
+    if (ne == 0) {
+        if (tid == 0) ev_count[r] = 0;
+        return;
+    }
+    if (ne > R.ev_cap) ne = R.ev_cap;
+    if (ne > (uint32_t)E) {
+        if (tid == 0) atomicAdd(&err[ERR_EVENT_LDS], 1u);
+        ne = E;
+    }
+    const Event *ev = events + R.ev_base;
+    // load and pad to a power of two
+    uint32_t np2 = 1;
+    while (np2 < ne) np2 <<= 1;
+    for (uint32_t k = tid; k < np2; k += blockDim.x) {
+        skey[k] = k < ne ? ev[k].key : ~0ull;
+        sidx[k] = k;
+    }
+    __syncthreads();
+    // bitonic sort of (key, rx, h1, h2) ascending
+    auto less = [&](uint32_t a, uint32_t b) -> bool {
+        if (skey[a] != skey[b]) return skey[a] < skey[b];
+        uint32_t ia = sidx[a], ib = sidx[b];
+        if (ia >= ne || ib >= ne) return ia < ib;
+        const Event &ea = ev[ia], &eb = ev[ib];
+        if (ea.rx != eb.rx) return ea.rx < eb.rx;
+        if (ea.h1 != eb.h1) return ea.h1 < eb.h1;
+        return ea.h2 < eb.h2;
+    };
+    for (uint32_t kk = 2; kk <= np2; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t a = tid; a < np2; a += blockDim.x) {
+                uint32_t bb = a ^ j;
+                if (bb > a) {
+                    bool up = (a & kk) == 0;
+                    bool sw = up ? less(bb, a) : less(a, bb);
+                    if (sw) {
+                        unsigned long long tk = skey[a];
+                        skey[a] = skey[bb];
+                        skey[bb] = tk;
+                        uint32_t ti = sidx[a];
+                        sidx[a] = sidx[bb];
+                        sidx[bb] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // hash table of particle slots
+    for (uint32_t k = tid; k < (uint32_t)HT; k += blockDim.x) {
+        hkeys[k] = 0xFFFFFFFFu;
+        htaken[k] = 0;
+    }
+    __syncthreads();
+    auto insert = [&](uint32_t slot) -> uint32_t {
+        uint32_t h = hslot(slot, HT - 1);
+        while (true) {
+            uint32_t prev = atomicCAS(&hkeys[h], 0xFFFFFFFFu, slot);
+            if (prev == 0xFFFFFFFFu || prev == slot) return h;
+            h = (h + 1) & (HT - 1);
+        }
+    };
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        const Event &e = ev[sidx[k]];
+        ent1[k] = insert(e.s1);
+        ent2[k] = (e.s2 == e.s1) ? ent1[k] : insert(e.s2);
+        status[k] = 0;
+    }
+    __syncthreads();
+    // rounds of the parallel greedy MIS (index in sorted order = priority)
+    for (int round = 0; round < 4096; ++round) {
+        for (uint32_t k = tid; k < (uint32_t)HT; k += blockDim.x) hmin[k] = 0xFFFFFFFFu;
+        if (tid == 0) s_undecided = 0;
+        __syncthreads();
+        for (uint32_t k = tid; k < ne; k += blockDim.x)
+            if (status[k] == 0) {
+                atomicMin(&hmin[ent1[k]], k);
+                atomicMin(&hmin[ent2[k]], k);
+            }
+        __syncthreads();
+        for (uint32_t k = tid; k < ne; k += blockDim.x)
+            if (status[k] == 0 && hmin[ent1[k]] == k && hmin[ent2[k]] == k) {
+                status[k] = 1;
+                htaken[ent1[k]] = 1;
+                htaken[ent2[k]] = 1;
+            }
+        __syncthreads();
+        for (uint32_t k = tid; k < ne; k += blockDim.x)
+            if (status[k] == 0) {
+                if (htaken[ent1[k]] || htaken[ent2[k]]) status[k] = 2;
+                else atomicAdd(&s_undecided, 1u);
+            }
+        __syncthreads();
+        if (s_undecided == 0) break;
+        __syncthreads();
+    }
+    // fresh handles in key order: exclusive scan of products needing a new id
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        uint32_t c = 0;
+        if (status[k] == 1) {
+            const int kind = T.rx[ev[sidx[k]].rx].kind;
+            c = (kind == RX_FISSION) ? 2u : (kind == RX_DECAY ? 0u : 1u);
+        }
+        prefix[k] = c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < ne; ++k) {
+            uint32_t c = prefix[k];
+            prefix[k] = acc;
+            acc += c;
+        }
+        s_total = acc;
+    }
+    __syncthreads();
+    const uint32_t hbase = R.handle_next;
+    const RepGeom geo = load_geom(R);
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        if (status[k] != 1) continue;
+        const Event e = ev[sidx[k]];
+        const RxDev X = T.rx[e.rx];
+        atomicAdd(&rxcount[e.rx & 255u], 1u);
+        State a = cur.bins[e.s1];
+        const uint32_t flags_keep = a.tw & TW_TOPO;
+        const uint32_t hnew = hbase + prefix[k];
+        switch (X.kind) {
+        case RX_DECAY:
+            cur.bins[e.s1].tw = a.tw | TW_DEAD;
+            break;
+        case RX_CONVERSION: {
+            State nw = a;
+            nw.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE);
+            nw.h = hnew;
+            cur.bins[e.s1] = nw;
+            break;
+        }
+        case RX_ENZYMATIC: {
+            // ReactionUtils.h:174-190: the educt that is not the catalyst converts
+            State bst = cur.bins[e.s2];
+            if (tw_type(a.tw) == X.educts[1]) {
+                bst.tw = make_tw(X.products[0], bst.tw & TW_TOPO, r) | (bst.tw & TW_OUTSIDE);
+                bst.h = hnew;
+                cur.bins[e.s2] = bst;
+            } else {
+                a.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE);
+                a.h = hnew;
+                cur.bins[e.s1] = a;
+            }
+            break;
+        }
+        case RX_FISSION: {
+            uint32_t x4[4];
+            double nv[3];
+            philox(a.h, R.step, 3u, 0u, R.seed, x4);
+            normals3(x4, nv);
+            double nrm = sqrt(dist2(nv[0], nv[1], nv[2]));
+            for (int d = 0; d < 3; ++d) nv[d] /= nrm;
+            const double p[3] = {a.x, a.y, a.z};
+            State p2, p1;
+            double q2[3], q1[3];
+            for (int d = 0; d < 3; ++d) {
+                q2[d] = wrap(p[d] - X.w2 * X.product_distance * nv[d], geo.box[d], geo.pbc[d]);
+                q1[d] = wrap(p[d] + X.w1 * X.product_distance * nv[d], geo.box[d], geo.pbc[d]);
+            }
+            p2.x = q2[0]; p2.y = q2[1]; p2.z = q2[2];
+            p2.tw = make_tw(X.products[1], 0u, r);
+            p2.h = hnew;
+            p1.x = q1[0]; p1.y = q1[1]; p1.z = q1[2];
+            p1.tw = make_tw(X.products[0], 0u, r);
+            p1.h = hnew + 1u;
+            cur.bins[e.s1].tw = a.tw | TW_DEAD;
+            bin_insert(cur, geo, cap, p2, err, true, max_wl);
+            bin_insert(cur, geo, cap, p1, err, true, max_wl);
+            break;
+        }
+        case RX_FUSION: {
+            const State bst = cur.bins[e.s2];
+            const double dx = min_image(bst.x - a.x, geo.box[0], geo.pbc[0]);
+            const double dy = min_image(bst.y - a.y, geo.box[1], geo.pbc[1]);
+            const double dz = min_image(bst.z - a.z, geo.box[2], geo.pbc[2]);
+            const double w = (X.educts[0] == tw_type(a.tw)) ? X.w1 : X.w2;
+            State pr;
+            pr.x = wrap(a.x + w * dx, geo.box[0], geo.pbc[0]);
+            pr.y = wrap(a.y + w * dy, geo.box[1], geo.pbc[1]);
+            pr.z = wrap(a.z + w * dz, geo.box[2], geo.pbc[2]);
+            pr.tw = make_tw(X.products[0], 0u, r);
+            pr.h = hnew;
+            cur.bins[e.s1].tw = a.tw | TW_DEAD;
+            cur.bins[e.s2].tw = bst.tw | TW_DEAD;
+            bin_insert(cur, geo, cap, pr, err, true, max_wl);
+            break;
+        }
+        }
+    }
+    __syncthreads();
+    if (tid < (int)T.n_rx && tid < 256) {
+        rx_step[(size_t)r * T.n_rx + tid] = rxcount[tid];
+        rx_total[(size_t)r * T.n_rx + tid] += rxcount[tid];
+    }
+    if (tid == 0) {
+        R.handle_next = hbase + s_total;
+        ev_count[r] = 0;
+    }
+}
+
+// marks every particle of replica r dead in the current buffer
+__global__ void k_kill_replica(Grid cur, uint32_t r, uint32_t max_wl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    uint32_t s = cur.wl[i];
+    State st = cur.bins[s];
+    if (tw_replica(st.tw) == r) cur.bins[s].tw = st.tw | TW_DEAD;
+}
+
+// -------------------------------------------------------------------------------------------------
+// upload, gather, observables, test exports
+// -------------------------------------------------------------------------------------------------
+__global__ void k_upload(Grid g, ReplicaDev *reps, uint32_t r, uint32_t n, const double *pos, const uint32_t *types,
+                         const uint8_t *topo_flavor, Topo topo, uint32_t cap, uint32_t max_wl, uint32_t *err) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ReplicaDev &R = reps[r];
+    const RepGeom geo = load_geom(R);
+    State st;
+    st.x = pos[3ull * i];
+    st.y = pos[3ull * i + 1];
+    st.z = pos[3ull * i + 2];
+    const uint32_t t = types[i];
+    const bool is_topo = topo_flavor && topo_flavor[t];
+    st.tw = make_tw(t, is_topo ? TW_TOPO : 0u, r);
+    st.h = i;
+    uint32_t slot = bin_insert(g, geo, cap, st, err, true, max_wl);
+    if (is_topo && topo.loc && slot != 0xFFFFFFFFu) topo.loc[R.topo_base + i] = slot;
+}
+
+// mode 0: particles (pos, type, handle [, force]); only replica `r`
+__global__ void k_gather(Grid cur, uint32_t r, uint32_t *out_count, uint32_t out_cap, double *pos, uint32_t *types,
+                         uint32_t *handles, const double *force_in, double *forces, uint32_t max_wl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const uint32_t s = cur.wl[i];
+    const State st = cur.bins[s];
+    if ((st.tw & TW_DEAD) || tw_replica(st.tw) != r) return;
+    uint32_t p = atomicAdd(out_count, 1u);
+    if (p >= out_cap) return;
+    if (pos) {
+        pos[3ull * p] = st.x;
+        pos[3ull * p + 1] = st.y;
+        pos[3ull * p + 2] = st.z;
+    }
+    if (types) types[p] = tw_type(st.tw);
+    if (handles) handles[p] = st.h;
+    if (forces) {
+        for (int d = 0; d < 3; ++d) forces[3ull * p + d] = force_in ? force_in[3ull * s + d] : 0.0;
+    }
+}
+
+__global__ void k_count_types(Grid cur, uint32_t n_types, unsigned long long *counts, uint32_t n_reps,
+                              uint32_t max_wl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const State st = cur.bins[cur.wl[i]];
+    if (st.tw & TW_DEAD) return;
+    uint32_t r = tw_replica(st.tw);
+    if (r >= n_reps) return;
+    atomicAdd(&counts[(size_t)r * n_types + tw_type(st.tw)], 1ull);
+}
+
+__global__ void k_pairs(Grid cur, ReplicaDev *reps, uint32_t cap, int radius, uint32_t r, double cut2,
+                        uint32_t *out, uint32_t out_cap, uint32_t *out_count, uint32_t max_wl) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const uint32_t s = cur.wl[i];
+    const State me = cur.bins[s];
+    if ((me.tw & (TW_DEAD | TW_OUTSIDE)) || tw_replica(me.tw) != r) return;
+    const RepGeom geo = load_geom(reps[r]);
+    const uint32_t cl = s / cap - geo.cell_base;
+    for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+        if (s2 == s) return;
+        const State o = cur.bins[s2];
+        if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
+        const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
+        const double dy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
+        const double dz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
+        if (dist2(dx, dy, dz) < cut2) {
+            uint32_t p = atomicAdd(out_count, 1u);
+            if (p < out_cap) {
+                out[2 * p] = me.h;
+                out[2 * p + 1] = o.h;
+            }
+        }
+    });
+}
+
+}  // namespace rdy

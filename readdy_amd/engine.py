@@ -1,0 +1,266 @@
+"""Python handle on the HIP engine (C ABI in include/readdy_amd.h).
+
+An Engine is one chemistry (species, pair potentials, reactions) plus any number of
+replicas that step together in shared launches.  All arrays cross the boundary as plain
+NumPy buffers; device memory stays inside the library.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def bin_capacity_for(particles_per_cell):
+    """Cell-bin capacity for an expected mean occupancy: Poisson mean + 7 sigma + 6, multiple of 4."""
+    lam = max(float(particles_per_cell), 0.0)
+    c = int(math.ceil(lam + 7.0 * math.sqrt(lam) + 6.0))
+    return max(8, (c + 3) // 4 * 4)
+
+
+class Engine:
+    def __init__(self, n_types, diffusion, kbt=1.0, potentials=(), reactions=(), topology_flavor=None,
+                 nl_cutoff=0.0, cell_radius=1, bin_capacity=0, max_replicas=1, max_cells=0, max_particles=0,
+                 events_per_replica=0, block_size=0):
+        self._lib = L.load()
+        self.n_types = int(n_types)
+        self._diff = np.ascontiguousarray(diffusion, dtype=np.float64)
+        assert self._diff.size == self.n_types
+        self._pots = (L.PairPotential * max(len(potentials), 1))()
+        for i, (kind, t1, t2, params) in enumerate(potentials):
+            self._pots[i].kind = kind
+            self._pots[i].type1 = t1
+            self._pots[i].type2 = t2
+            for k, v in enumerate(params):
+                self._pots[i].params[k] = float(v)
+        self._rx = (L.Reaction * max(len(reactions), 1))()
+        self.reaction_kinds = []
+        for i, r in enumerate(reactions):
+            x = self._rx[i]
+            x.kind = r["kind"]
+            ed = list(r.get("educts", [])) + [0, 0]
+            pr = list(r.get("products", [])) + [0, 0]
+            x.educts[0], x.educts[1] = ed[0], ed[1]
+            x.products[0], x.products[1] = pr[0], pr[1]
+            x.rate = r["rate"]
+            x.educt_distance = r.get("educt_distance", 0.0)
+            x.product_distance = r.get("product_distance", 0.0)
+            x.weight1 = r.get("weight1", 0.5)
+            x.weight2 = r.get("weight2", 0.5)
+            self.reaction_kinds.append(r["kind"])
+        self.n_reactions = len(reactions)
+        self._flavor = None
+        if topology_flavor is not None:
+            self._flavor = np.ascontiguousarray(topology_flavor, dtype=np.uint8)
+        ctx = L.ContextDesc()
+        ctx.n_types = self.n_types
+        ctx.kbt = kbt
+        ctx.diffusion = _ptr(self._diff, _dp)
+        ctx.topology_flavor = _ptr(self._flavor, ctypes.POINTER(ctypes.c_uint8)) if self._flavor is not None else None
+        ctx.n_potentials = len(potentials)
+        ctx.potentials = self._pots
+        ctx.n_reactions = len(reactions)
+        ctx.reactions = self._rx
+        cfg = L.EngineConfig()
+        cfg.nl_cutoff = nl_cutoff
+        cfg.cell_radius = cell_radius
+        cfg.bin_capacity = bin_capacity
+        cfg.max_replicas = max_replicas
+        cfg.max_cells = max_cells
+        cfg.max_particles = max_particles
+        cfg.events_per_replica = events_per_replica
+        cfg.block_size = block_size
+        h = ctypes.c_void_p()
+        L.check(self._lib.rdy_engine_create(ctypes.byref(ctx), ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.replicas = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.rdy_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- replicas and particles ----
+    def add_replica(self, box, pbc=(1, 1, 1), seed=0, max_particles=0):
+        d = L.ReplicaDesc()
+        for k in range(3):
+            d.box[k] = float(box[k])
+            d.pbc[k] = int(bool(pbc[k]))
+        d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        d.max_particles = int(max_particles)
+        r = ctypes.c_uint32()
+        L.check(self._lib.rdy_replica_add(self._h, ctypes.byref(d), ctypes.byref(r)))
+        self.replicas.append(r.value)
+        return r.value
+
+    def footprint(self, box, max_particles):
+        d = L.ReplicaDesc()
+        for k in range(3):
+            d.box[k] = float(box[k])
+            d.pbc[k] = 1
+        d.max_particles = int(max_particles)
+        b = ctypes.c_uint64()
+        L.check(self._lib.rdy_replica_footprint(self._h, ctypes.byref(d), ctypes.byref(b)))
+        return b.value
+
+    def remove_replica(self, r):
+        L.check(self._lib.rdy_replica_remove(self._h, r))
+        if r in self.replicas:
+            self.replicas.remove(r)
+
+    def set_particles(self, r, pos, types):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        types = np.ascontiguousarray(types, dtype=np.uint32)
+        assert len(pos) == len(types)
+        L.check(self._lib.rdy_set_particles(self._h, r, len(types), _ptr(pos, _dp), _ptr(types, _u32p)))
+
+    def n_particles(self, r=0):
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_get_particles(self._h, r, 0, None, None, None, ctypes.byref(n)))
+        return n.value
+
+    def get_particles(self, r=0):
+        """(positions, types, handles) of live particles, sorted by handle."""
+        n = self.n_particles(r)
+        pos = np.zeros((n, 3))
+        types = np.zeros(n, dtype=np.uint32)
+        hs = np.zeros(n, dtype=np.uint32)
+        m = ctypes.c_uint64()
+        L.check(self._lib.rdy_get_particles(self._h, r, n, _ptr(pos, _dp), _ptr(types, _u32p), _ptr(hs, _u32p),
+                                            ctypes.byref(m)))
+        o = np.argsort(hs, kind="stable")
+        return pos[o], types[o], hs[o]
+
+    def get_forces(self, r=0):
+        """(forces, handles) of the last force evaluation, sorted by handle."""
+        n = self.n_particles(r)
+        f = np.zeros((n, 3))
+        hs = np.zeros(n, dtype=np.uint32)
+        m = ctypes.c_uint64()
+        L.check(self._lib.rdy_get_forces(self._h, r, n, _ptr(f, _dp), _ptr(hs, _u32p), ctypes.byref(m)))
+        o = np.argsort(hs, kind="stable")
+        return f[o], hs[o]
+
+    def set_bonds(self, r, idx, k, length):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 2)
+        k = np.ascontiguousarray(np.broadcast_to(k, (len(idx),)), dtype=np.float64)
+        length = np.ascontiguousarray(np.broadcast_to(length, (len(idx),)), dtype=np.float64)
+        L.check(self._lib.rdy_set_bonds(self._h, r, len(idx), _ptr(idx, _u32p), _ptr(k, _dp), _ptr(length, _dp)))
+
+    def set_angles(self, r, idx, k, theta0):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)
+        k = np.ascontiguousarray(np.broadcast_to(k, (len(idx),)), dtype=np.float64)
+        t0 = np.ascontiguousarray(np.broadcast_to(theta0, (len(idx),)), dtype=np.float64)
+        L.check(self._lib.rdy_set_angles(self._h, r, len(idx), _ptr(idx, _u32p), _ptr(k, _dp), _ptr(t0, _dp)))
+
+    def set_dihedrals(self, r, idx, k, multiplicity, phi0):
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 4)
+        n = len(idx)
+        k = np.ascontiguousarray(np.broadcast_to(k, (n,)), dtype=np.float64)
+        m = np.ascontiguousarray(np.broadcast_to(multiplicity, (n,)), dtype=np.uint32)
+        p0 = np.ascontiguousarray(np.broadcast_to(phi0, (n,)), dtype=np.float64)
+        L.check(self._lib.rdy_set_dihedrals(self._h, r, n, _ptr(idx, _u32p), _ptr(k, _dp), _ptr(m, _u32p),
+                                            _ptr(p0, _dp)))
+
+    # ---- actions ----
+    def update_neighbor_list(self):
+        L.check(self._lib.rdy_update_neighbor_list(self._h))
+
+    def calculate_forces(self):
+        L.check(self._lib.rdy_calculate_forces(self._h))
+
+    def energy(self, r=0):
+        e = ctypes.c_double()
+        L.check(self._lib.rdy_get_energy(self._h, r, ctypes.byref(e)))
+        return e.value
+
+    def integrate(self, dt, noise=None):
+        if noise is None:
+            L.check(self._lib.rdy_integrate(self._h, dt, None, 0))
+        else:
+            noise = np.ascontiguousarray(noise, dtype=np.float64).reshape(-1, 3)
+            L.check(self._lib.rdy_integrate(self._h, dt, _ptr(noise, _dp), len(noise)))
+
+    def react(self, handler, dt):
+        L.check(self._lib.rdy_react(self._h, handler, dt))
+
+    def reaction_counts(self, r=0):
+        c = np.zeros(max(self.n_reactions, 1), dtype=np.uint64)
+        L.check(self._lib.rdy_get_reaction_counts(self._h, r, _ptr(c, _u64p)))
+        return c[:self.n_reactions]
+
+    def reaction_totals(self, r=0):
+        c = np.zeros(max(self.n_reactions, 1), dtype=np.uint64)
+        L.check(self._lib.rdy_get_reaction_totals(self._h, r, _ptr(c, _u64p)))
+        return c[:self.n_reactions]
+
+    def run(self, handler, dt, n_steps, graph=False):
+        fn = self._lib.rdy_run_graph if graph else self._lib.rdy_run
+        L.check(fn(self._h, handler, dt, int(n_steps)))
+
+    def set_steps(self, r, steps):
+        L.check(self._lib.rdy_replica_set_steps(self._h, r, int(steps)))
+
+    def steps_done(self, r):
+        s = ctypes.c_uint64()
+        L.check(self._lib.rdy_replica_steps_done(self._h, r, ctypes.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        L.check(self._lib.rdy_synchronize(self._h))
+
+    # ---- parity exports ----
+    def neighbor_pairs(self, r, cutoff):
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_neighbor_pairs(self._h, r, cutoff, 0, None, ctypes.byref(n)))
+        cap = n.value
+        out = np.zeros(2 * max(cap, 1), dtype=np.uint32)
+        L.check(self._lib.rdy_neighbor_pairs(self._h, r, cutoff, cap, _ptr(out, _u32p), ctypes.byref(n)))
+        return out[:2 * cap].reshape(-1, 2)
+
+    def reaction_candidates(self, r=0):
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_reaction_candidates(self._h, r, 0, None, ctypes.byref(n)))
+        cap = n.value
+        out = np.zeros(4 * max(cap, 1), dtype=np.uint32)
+        L.check(self._lib.rdy_reaction_candidates(self._h, r, cap, _ptr(out, _u32p), ctypes.byref(n)))
+        return out[:4 * cap].reshape(-1, 4)
+
+    def count_types(self, n_replicas=None):
+        n = n_replicas if n_replicas is not None else (max(self.replicas) + 1 if self.replicas else 1)
+        c = np.zeros(n * self.n_types, dtype=np.uint64)
+        L.check(self._lib.rdy_count_types(self._h, _ptr(c, _u64p), n))
+        return c.reshape(n, self.n_types)
+
+    def profile_reset(self):
+        L.check(self._lib.rdy_profile_reset(self._h))
+
+    def profile_get(self):
+        us = ctypes.c_double()
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_profile_get(self._h, ctypes.byref(us), ctypes.byref(n)))
+        return us.value, n.value
+
+
+def device_info():
+    lib = L.load()
+    name = ctypes.create_string_buffer(256)
+    mem = ctypes.c_uint64()
+    cus = ctypes.c_int32()
+    L.check(lib.rdy_device_info(name, 256, ctypes.byref(mem), ctypes.byref(cus)))
+    return {"name": name.value.decode(), "hbm_bytes": mem.value, "compute_units": cus.value}

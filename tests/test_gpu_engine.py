@@ -1,0 +1,482 @@
+"""GPU parity tests: the HIP engine (through its C ABI) against the CPU oracle.
+
+Bars: neighbour pair sets and reaction candidate sets bit-exact; forces and energies within
+1e-5 relative of the FP64 oracle (FP32 pair arithmetic on FP64 min-image displacements);
+free diffusion with injected noise bit-exact; reaction steps identical event-for-event when both
+sides draw from the same Philox counters.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+L = pytest.importorskip("readdy_amd._lib")
+from readdy_amd import Engine  # noqa: E402
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))
+
+FORCE_TOL = 1e-5   # relative, ||dF|| / ||F_ref|| over all particles
+ENERGY_TOL = 1e-5  # relative
+
+
+def _pairset(a):
+    return set(map(tuple, np.asarray(a, dtype=np.int64).tolist()))
+
+
+def _oracle(n_types, box, pbc, pots, rxs=(), diffusion=None, kbt=1.0):
+    o = po.System(n_types, box, pbc, kbt=kbt, diffusion=diffusion)
+    for kind, a, b, p in pots:
+        o.add_pair_potential(kind, a, b, p)
+    for r in rxs:
+        o.add_reaction(r["kind"], r.get("educts", []), r.get("products", []), r["rate"], r.get("educt_distance", 0.0),
+                       r.get("product_distance", 0.0), r.get("weight1", 0.5), r.get("weight2", 0.5))
+    return o
+
+
+def _engine(n_types, box, pbc, pots, rxs=(), diffusion=None, seed=0, n=1000, kbt=1.0, **kw):
+    diffusion = [1.0] * n_types if diffusion is None else diffusion
+    e = Engine(n_types, diffusion, kbt=kbt, potentials=pots, reactions=list(rxs), **kw)
+    r = e.add_replica(box, pbc, seed=seed, max_particles=4 * n + 64)
+    return e, r
+
+
+def _forces_sorted(o):
+    _, _, ids, f = o.particles()
+    return f[np.argsort(ids)]
+
+
+@pytest.mark.parametrize("box,pbc,radius,cutoff,n", [
+    ([10, 10, 10], [1, 1, 1], 1, 1.0, 2000), ([10, 10, 10], [1, 1, 1], 2, 1.0, 2000),
+    ([7.3, 5.1, 12.2], [1, 0, 1], 1, 1.3, 1500), ([3.0, 3.0, 3.0], [1, 1, 1], 1, 1.2, 300),
+    ([2.1, 9.0, 4.0], [0, 0, 0], 1, 1.0, 600), ([4, 4, 4], [1, 1, 1], 1, 1.9, 400),
+    ([20, 20, 20], [1, 1, 1], 1, 2.0, 4000), ([1.5, 4.0, 1.5], [0, 0, 0], 1, 1.2, 50),
+])
+def test_pair_set_bit_exact(box, pbc, radius, cutoff, n):
+    rng = np.random.default_rng(17)
+    pos = (rng.random((n, 3)) - 0.5) * np.array(box)
+    pos[: n // 10] = np.round(pos[: n // 10] * 4) / 4  # lattice points: exact half-box and cell-edge ties
+    types = np.zeros(n, dtype=np.uint32)
+    pots = [(L.POT_HARMONIC_REPULSION, 0, 0, [1.0, cutoff])]
+    e, r = _engine(1, box, pbc, pots, n=n, cell_radius=radius)
+    e.set_particles(r, pos, types)
+    gpu = e.neighbor_pairs(r, cutoff)
+    o = _oracle(1, box, pbc, [])
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    ref = o.brute_pairs(cutoff)
+    assert len(gpu) == len(_pairset(gpu)), "pair visited twice"
+    assert _pairset(gpu) == _pairset(ref)
+
+
+def test_neighbor_list_reference_scenes():
+    g = GOLDEN["neighbor_list_scenes"]
+    for sc in g["scenes"]:
+        n = len(sc["positions"])
+        pots = [(L.POT_HARMONIC_REPULSION, 0, 0, [1.0, g["cutoff"]])]
+        e, r = _engine(1, sc["box"], sc["pbc"], pots, n=n, cell_radius=min(sc["radius"], 3))
+        e.set_particles(r, sc["positions"], np.zeros(n, dtype=np.uint32))
+        pairs = _pairset(e.neighbor_pairs(r, g["cutoff"]))
+        if sc.get("all_pairs"):
+            assert pairs == {(i, j) for i in range(n) for j in range(n) if i != j}
+        for p in sc.get("must_contain", []):
+            assert tuple(p) in pairs
+
+
+def _system_c0like(n=2000, L_box=20.0, seed=0):
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((n, 3)) - 0.5) * L_box
+    types = np.repeat(np.arange(2, dtype=np.uint32), n // 2)
+    return pos, types
+
+
+def _force_case(pots, n_types, pos, types, box, pbc=(1, 1, 1)):
+    e, r = _engine(n_types, box, pbc, pots, n=len(types))
+    e.set_particles(r, pos, types)
+    e.calculate_forces()
+    f, hs = e.get_forces(r)
+    assert np.array_equal(hs, np.arange(len(types)))
+    o = _oracle(n_types, box, pbc, pots)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    o.nl_setup(o.max_cutoff())
+    e_ref = o.calculate_forces()
+    f_ref = _forces_sorted(o)
+    rel = np.linalg.norm(f - f_ref) / np.linalg.norm(f_ref)
+    pair_scale = np.abs(f_ref).sum()
+    e_rel = abs(e.energy(r) - e_ref) / abs(e_ref)
+    return rel, e_rel, np.linalg.norm(f - f_ref) / pair_scale
+
+
+def test_forces_harmonic_repulsion_c0():
+    pos, types = _system_c0like()
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    rel, e_rel, _ = _force_case(pots, 3, pos, types, [20.0] * 3)
+    assert rel < FORCE_TOL and e_rel < ENERGY_TOL, (rel, e_rel)
+
+
+def test_forces_weak_interaction_c1_chemistry():
+    # C1 chemistry at C1 density (0.125) in a smaller box: HR everywhere + WIPH between A and A
+    n, Lb = 8000, 40.0
+    rng = np.random.default_rng(1)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    rel, e_rel, _ = _force_case(pots, 3, pos, types, [Lb] * 3)
+    assert rel < FORCE_TOL and e_rel < ENERGY_TOL, (rel, e_rel)
+
+
+def test_forces_lennard_jones_and_screened():
+    n, Lb = 3000, 16.0
+    rng = np.random.default_rng(2)
+    # jittered lattice keeps LJ pairs away from the singularity
+    g = np.stack(np.meshgrid(*[np.arange(15)] * 3, indexing="ij"), -1).reshape(-1, 3)[:n].astype(float)
+    pos = (g + 0.5) * (Lb / 15) - Lb / 2 + rng.normal(scale=0.05, size=(n, 3))
+    types = rng.integers(0, 2, n).astype(np.uint32)
+    pots = [(L.POT_LENNARD_JONES, 0, 0, [12, 6, 2.5, 1.0, 1.0, 1.0]),
+            (L.POT_LENNARD_JONES, 0, 1, [12, 6, 2.5, 1.0, 1.0, 1.0]),
+            (L.POT_SCREENED_ELECTROSTATICS, 1, 1, [-1.0, 1.0, 1.0, 1.0, 6, 3.0])]
+    rel, e_rel, _ = _force_case(pots, 2, pos, types, [Lb] * 3)
+    assert rel < FORCE_TOL and e_rel < ENERGY_TOL, (rel, e_rel)
+
+
+@pytest.mark.parametrize("name", ["harmonic_repulsion_two_particles", "harmonic_repulsion_six_particles",
+                                  "lennard_jones", "screened_electrostatics"])
+def test_reference_known_answers_on_gpu(name):
+    g = GOLDEN[name]
+    tid = {t: i for i, t in enumerate(g["types"])}
+    pots = []
+    for p in g["potentials"]:
+        a, b = tid[p["types"][0]], tid[p["types"][1]]
+        if p["kind"] == "HarmonicRepulsion":
+            pots.append((L.POT_HARMONIC_REPULSION, a, b, [p["force_constant"], p["interaction_distance"]]))
+        elif p["kind"] == "LennardJones":
+            pots.append((L.POT_LENNARD_JONES, a, b, [p["m"], p["n"], p["cutoff"], float(p["shift"]), p["epsilon"],
+                                                     p["sigma"]]))
+        else:
+            pots.append((L.POT_SCREENED_ELECTROSTATICS, a, b, [p["electrostatic_strength"],
+                                                               p["inverse_screening_depth"], p["repulsion_strength"],
+                                                               p["repulsion_distance"], p["exponent"], p["cutoff"]]))
+    pos = np.array([q[1] for q in g["particles"]], dtype=float)
+    types = np.array([tid[q[0]] for q in g["particles"]], dtype=np.uint32)
+    e, r = _engine(len(tid), g["box"], g["pbc"], pots, n=len(types))
+    e.set_particles(r, pos, types)
+    e.calculate_forces()
+    f, _ = e.get_forces(r)
+    if "forces" in g:
+        np.testing.assert_allclose(f, g["forces"], rtol=1e-5, atol=1e-6)
+    en = e.energy(r)
+    want = g["energy"] if "energy" in g else sum(g["pair_energies"].values())
+    assert en == pytest.approx(want, rel=1e-5)
+
+
+def _chain(nb, rng):
+    steps = rng.normal(size=(nb, 3))
+    steps /= np.linalg.norm(steps, axis=1)[:, None]
+    return np.cumsum(steps, axis=0) * 1.0
+
+
+def test_bonded_terms_vs_oracle():
+    rng = np.random.default_rng(3)
+    Lb, nch, nb = 30.0, 20, 12
+    pos = []
+    for c in range(nch):
+        x = _chain(nb, rng)
+        x = x - x.mean(0) + (rng.random(3) - 0.5) * (Lb - 12)
+        pos.append(x)
+    pos = np.concatenate(pos)
+    pos = (pos + Lb / 2) % Lb - Lb / 2
+    n = len(pos)
+    types = np.zeros(n, dtype=np.uint32)
+    bonds = [(c * nb + i, c * nb + i + 1) for c in range(nch) for i in range(nb - 1)]
+    angles = [(c * nb + i, c * nb + i + 1, c * nb + i + 2) for c in range(nch) for i in range(nb - 2)]
+    dihs = [(c * nb + i, c * nb + i + 1, c * nb + i + 2, c * nb + i + 3) for c in range(nch) for i in range(nb - 3)]
+    e = Engine(1, [1.0], topology_flavor=[1], nl_cutoff=2.0)
+    r = e.add_replica([Lb] * 3, (1, 1, 1), max_particles=2 * n)
+    e.set_particles(r, pos, types)
+    e.set_bonds(r, bonds, 100.0, 1.0)
+    e.set_angles(r, angles, 10.0, np.pi)
+    e.set_dihedrals(r, dihs, 1.0, 3, 0.0)
+    e.calculate_forces()
+    f, _ = e.get_forces(r)
+    o = po.System(1, [Lb] * 3, [1, 1, 1])
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    for b in bonds:
+        o.add_bond(*b, 100.0, 1.0)
+    for a in angles:
+        o.add_angle(*a, 10.0, np.pi)
+    for d in dihs:
+        o.add_dihedral(*d, 1.0, 3, 0.0)
+    e_ref = o.calculate_forces()
+    f_ref = _forces_sorted(o)
+    assert np.linalg.norm(f - f_ref) / np.linalg.norm(f_ref) < 1e-9
+    assert e.energy(r) == pytest.approx(e_ref, rel=1e-7)
+
+
+def test_free_diffusion_injected_noise_bit_exact():
+    n, Lb = 1000, 10.0
+    rng = np.random.default_rng(4)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.integers(0, 2, n).astype(np.uint32)
+    D = [1.0, 0.37]
+    e = Engine(2, D, kbt=1.3)
+    r = e.add_replica([Lb, Lb, Lb], (1, 0, 1), max_particles=2 * n)
+    e.set_particles(r, pos, types)
+    o = po.System(2, [Lb] * 3, [1, 0, 1], kbt=1.3, diffusion=D)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    for t in range(10):
+        noise = rng.normal(size=(n, 3))
+        e.integrate(0.01, noise)
+        e.update_neighbor_list()
+        o.euler_bd(0.01, noise.ravel())
+    p, _, hs = e.get_particles(r)
+    op, _, ids, _ = o.particles()
+    assert np.array_equal(p, op[np.argsort(ids)])
+
+
+def test_philox_noise_matches_oracle_stream():
+    """Philox-driven BD: identical integer draws; positions agree to transcendental rounding."""
+    n, Lb = 500, 10.0
+    rng = np.random.default_rng(5)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = np.zeros(n, dtype=np.uint32)
+    e = Engine(1, [1.0])
+    r = e.add_replica([Lb] * 3, (1, 1, 1), seed=1234, max_particles=2 * n)
+    e.set_particles(r, pos, types)
+    o = po.System(1, [Lb] * 3, [1, 1, 1])
+    o.set_seed(1234)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    for t in range(5):
+        e.integrate(0.01)
+        o.euler_bd(0.01, step=t)
+    p, _, _ = e.get_particles(r)
+    op, _, ids, _ = o.particles()
+    d = p - op[np.argsort(ids)]
+    d -= Lb * np.round(d / Lb)
+    assert np.abs(d).max() < 1e-12
+
+
+def _c0_chemistry():
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    rxs = [dict(kind=L.RX_FUSION, educts=[0, 1], products=[2], rate=0.1, educt_distance=1.0),
+           dict(kind=L.RX_FISSION, educts=[2], products=[0, 1], rate=0.05, product_distance=1.0)]
+    return pots, rxs
+
+
+def test_reaction_candidates_bit_exact():
+    pots, rxs = _c0_chemistry()
+    rxs = rxs + [dict(kind=L.RX_FUSION, educts=[0, 0], products=[2], rate=0.3, educt_distance=0.7),
+                 dict(kind=L.RX_ENZYMATIC, educts=[1, 2], products=[0, 2], rate=0.2, educt_distance=1.3),
+                 dict(kind=L.RX_DECAY, educts=[1], rate=0.01), dict(kind=L.RX_CONVERSION, educts=[0], products=[1],
+                                                                    rate=0.0)]
+    rng = np.random.default_rng(6)
+    n, Lb = 3000, 20.0
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.integers(0, 3, n).astype(np.uint32)
+    e, r = _engine(3, [Lb] * 3, (1, 1, 1), pots, rxs, n=n)
+    e.set_particles(r, pos, types)
+    gpu = e.reaction_candidates(r)
+    o = _oracle(3, [Lb] * 3, [1, 1, 1], pots, rxs)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    o.nl_setup(o.max_cutoff())
+    ref = o.reaction_candidates()
+    g = {(int(a), int(b), int(c), int(d)) for a, b, c, d in gpu}
+    f = {(int(a), int(b), int(c), int(d)) for a, b, c, d in ref}
+    assert len(g) == len(gpu)
+    assert g == f
+
+
+@pytest.mark.parametrize("handler,mode", [(0, po.UA_KEYED), (1, po.GILLESPIE_KEYED)])
+def test_reaction_steps_match_oracle_event_for_event(handler, mode):
+    """Same positions, same Philox counters: the same events fire and products land identically."""
+    pots, rxs = _c0_chemistry()
+    rxs[0]["rate"] = 20.0
+    rxs[1]["rate"] = 10.0
+    n, Lb, dt = 2000, 20.0, 0.01
+    pos, types = _system_c0like(n, Lb, seed=7)
+    e, r = _engine(3, [Lb] * 3, (1, 1, 1), pots, rxs, diffusion=[1.0, 1.0, 0.5], seed=99, n=n)
+    e.set_particles(r, pos, types)
+    o = _oracle(3, [Lb] * 3, [1, 1, 1], pots, rxs, diffusion=[1.0, 1.0, 0.5])
+    o.set_seed(99)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    o.nl_setup(o.max_cutoff())
+    total = 0
+    for t in range(5):
+        noise = np.random.default_rng(100 + t).normal(size=(4 * n, 3))
+        e.integrate(dt, noise)
+        e.update_neighbor_list()
+        o.euler_bd(dt, noise.ravel(), step=t)
+        o.nl_fill()
+        e.react(handler, dt)
+        o.reactions(mode, dt, step=t)
+        o.nl_fill()
+        gp, gt, gh = e.get_particles(r)
+        op, ot, oid, _ = o.particles()
+        order = np.argsort(oid)
+        assert np.array_equal(gh, oid[order].astype(np.uint32))
+        assert np.array_equal(gt, ot[order].astype(np.uint32))
+        d = gp - op[order]
+        d -= Lb * np.round(d / Lb)
+        assert np.abs(d).max() < 1e-9
+        assert np.array_equal(e.reaction_counts(r), o.reaction_counts())
+        total += int(e.reaction_counts(r).sum())
+        # forces for the next step are taken from the same state on both sides
+        e.calculate_forces()
+        o.calculate_forces()
+    assert total > 50
+
+
+@pytest.mark.parametrize("handler", [0, 1])
+def test_parallel_gillespie_scene_on_gpu(handler):
+    g = GOLDEN["parallel_gillespie_scene"]
+    tid = {t: i for i, t in enumerate(g["types"])}
+    rxs = [dict(kind=L.RX_FUSION, educts=[tid[x["educts"][0]], tid[x["educts"][1]]], products=[tid[x["product"]]],
+                rate=x["rate"], educt_distance=x["radius"]) for x in g["reactions"]]
+    pos = np.array([p[1] for p in g["particles"]], dtype=float)
+    types = np.array([tid[p[0]] for p in g["particles"]], dtype=np.uint32)
+    e, r = _engine(3, g["box"], g["pbc"], [], rxs, diffusion=[0.25] * 3, n=len(types))
+    e.set_particles(r, pos, types)
+    e.react(handler, g["timestep"])
+    p, t, _ = e.get_particles(r)
+    assert len(t) == g["expected_count"]
+    have = [(int(a), tuple(b)) for a, b in zip(t, p)]
+    for name, q in g["expected_exact"]:
+        assert (tid[name], tuple(float(x) for x in q)) in have
+    for name, q in g["expected_close"]:
+        assert any(a == tid[name] and np.allclose(b, q, rtol=1e-14) for a, b in have)
+
+
+@pytest.mark.parametrize("handler", [0, 1])
+def test_decay_scene_on_gpu(handler):
+    g = GOLDEN["decay_scene"]
+    rxs = [dict(kind=L.RX_DECAY, educts=[0], rate=g["decay_rate"]),
+           dict(kind=L.RX_FISSION, educts=[0], products=[0, 0], rate=g["fission_rate"],
+                product_distance=g["fission_distance"])]
+    e, r = _engine(1, g["box"], g["pbc"], [], rxs, diffusion=[g["diffusion"]], n=4 * g["n_particles"])
+    e.set_particles(r, np.zeros((g["n_particles"], 3)), np.zeros(g["n_particles"], dtype=np.uint32))
+    for t in range(g["steps"]):
+        e.calculate_forces()
+        e.integrate(g["timestep"])
+        e.react(handler, g["timestep"])
+    assert e.n_particles(r) == g["expected_remaining"]
+
+
+def test_fusion_through_boundary_and_fission_near_boundary_on_gpu():
+    g = GOLDEN["fusion_through_periodic_boundary"]
+    for handler in (0, 1):
+        rx = [dict(kind=L.RX_FUSION, educts=[0, 0], products=[0], rate=g["rate"], educt_distance=g["radius"])]
+        e, r = _engine(1, g["box"], g["pbc"], [], rx, diffusion=[0.0], n=2)
+        e.set_particles(r, g["particles"], np.zeros(2, dtype=np.uint32))
+        e.react(handler, 1.0)
+        p, _, _ = e.get_particles(r)
+        assert len(p) == 1
+        np.testing.assert_allclose(p[0], g["product"], rtol=g["product_rel_tolerance"])
+    g = GOLDEN["fission_near_boundary"]
+    for seed in range(4):
+        rx = [dict(kind=L.RX_FISSION, educts=[0], products=[0, 0], rate=g["rate"],
+                   product_distance=g["product_distance"])]
+        e, r = _engine(1, g["box"], g["pbc"], [], rx, diffusion=[0.0], seed=seed, n=2)
+        e.set_particles(r, [g["particle"]], np.zeros(1, dtype=np.uint32))
+        e.react(0, 1.0)
+        p, _, _ = e.get_particles(r)
+        assert len(p) == 2 and ((p[0][0] <= 0) != (p[1][0] <= 0))
+
+
+def _c0_engine(graph=False, n=2000, seed=11, max_replicas=1):
+    pots, rxs = _c0_chemistry()
+    pos, types = _system_c0like(n, 20.0, seed=3)
+    e = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs, max_replicas=max_replicas)
+    return e, pos, types
+
+
+def test_fused_loop_equals_separate_actions():
+    """rdy_run (fused kernels) reproduces integrate / NL / react / NL / forces exactly."""
+    e1, pos, types = _c0_engine()
+    r1 = e1.add_replica([20.0] * 3, seed=5, max_particles=8000)
+    e1.set_particles(r1, pos, types)
+    e1.run(0, 0.01, 30)
+    e2, _, _ = _c0_engine()
+    r2 = e2.add_replica([20.0] * 3, seed=5, max_particles=8000)
+    e2.set_particles(r2, pos, types)
+    e2.calculate_forces()
+    for t in range(30):
+        e2.integrate(0.01)
+        e2.update_neighbor_list()
+        e2.react(0, 0.01)
+        e2.update_neighbor_list()
+        e2.calculate_forces()
+    a = e1.get_particles(r1)
+    b = e2.get_particles(r2)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert e1.energy(r1) == e2.energy(r2)
+    assert np.array_equal(e1.reaction_totals(r1), e2.reaction_totals(r2))
+
+
+def test_graph_replay_equals_eager():
+    e1, pos, types = _c0_engine()
+    r1 = e1.add_replica([20.0] * 3, seed=8, max_particles=8000)
+    e1.set_particles(r1, pos, types)
+    e1.run(1, 0.01, 41)
+    e2, _, _ = _c0_engine()
+    r2 = e2.add_replica([20.0] * 3, seed=8, max_particles=8000)
+    e2.set_particles(r2, pos, types)
+    for chunk in (10, 20, 11):  # several replays with host work in between
+        e2.run(1, 0.01, chunk, graph=True)
+        e2.get_particles(r2)
+    a = e1.get_particles(r1)
+    b = e2.get_particles(r2)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_replica_alone_equals_replica_in_batch():
+    """Deterministic fixed-point forces and handle-keyed Philox make a replica batch-invariant."""
+    pots, rxs = _c0_chemistry()
+    e1 = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs)
+    pos, types = _system_c0like(1000, 16.0, seed=21)
+    r = e1.add_replica([16.0] * 3, seed=77, max_particles=4000)
+    e1.set_particles(r, pos, types)
+    e1.run(0, 0.01, 25)
+    alone = e1.get_particles(r)
+    eb = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs, max_replicas=6, max_cells=6 * 16 ** 3,
+                max_particles=40000)
+    ids = []
+    for k in range(6):
+        L_k = [12.0, 16.0, 20.0, 16.0, 9.0, 14.0][k]
+        rk = eb.add_replica([L_k] * 3, seed=77 if k == 3 else 1000 + k)
+        ids.append(rk)
+        if k == 3:
+            eb.set_particles(rk, pos, types)
+        else:
+            p, t = _system_c0like(int(0.25 * L_k ** 3) // 2 * 2, L_k, seed=k)
+            eb.set_particles(rk, p, t)
+    eb.run(0, 0.01, 25)
+    mixed = eb.get_particles(ids[3])
+    for x, y in zip(alone, mixed):
+        assert np.array_equal(x, y)
+
+
+def test_c1_full_size_invariants():
+    """1M particles, C1 chemistry: A+C and B+C conserved, positions wrapped, energy finite."""
+    n, Lb = 1_000_000, 200.0
+    rng = np.random.default_rng(1)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    _, rxs = _c0_chemistry()
+    e = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs)
+    r = e.add_replica([Lb] * 3, seed=1, max_particles=n)
+    e.set_particles(r, pos, types)
+    c0 = e.count_types(1)[0].astype(np.int64)
+    e.run(0, 0.005, 20)
+    c1 = e.count_types(1)[0].astype(np.int64)
+    assert c0[0] + c0[2] == c1[0] + c1[2] and c0[1] + c0[2] == c1[1] + c1[2]
+    assert np.isfinite(e.energy(r))
+    assert e.reaction_totals(r).sum() > 100
+    p, _, _ = e.get_particles(r)
+    assert np.all(p >= -Lb / 2) and np.all(p < Lb / 2)
