@@ -134,6 +134,7 @@ double pot_cutoff(const rdy_pair_potential &p) {
 
 int ensure_scratch(rdy_engine *e, size_t bytes) {
     if (bytes <= e->scratch_bytes) return RDY_OK;
+    HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->scratch) HIP_TRY(hipFree(e->scratch));
     e->scratch = nullptr;
     size_t b = std::max(bytes, (size_t)1 << 20);
@@ -168,6 +169,7 @@ int check_errors(rdy_engine *e) {
 // order-dependent tables (p_accept depends on dt)
 int upload_tables(rdy_engine *e, double dt) {
     if (e->tables_dt == dt && e->d_rx) return RDY_OK;
+    HIP_TRY(hipStreamSynchronize(e->stream));  // old tables may still be read by queued kernels
     const uint32_t nt = e->n_types;
     // pair potentials: symmetric type-pair lists in registration order (PotentialRegistry.h:65-70)
     std::vector<std::vector<uint32_t>> lists((size_t)nt * nt);
@@ -319,25 +321,26 @@ int alloc_pools(rdy_engine *e) {
     for (int b = 0; b < 2; ++b) {
         HIP_TRY(dalloc(&e->bins[b], slots));
         HIP_TRY(dalloc(&e->counts[b], e->max_cells));
-        HIP_TRY(hipMemset(e->counts[b], 0, e->max_cells * sizeof(uint32_t)));
+        HIP_TRY(hipMemsetAsync(e->counts[b], 0, e->max_cells * sizeof(uint32_t), e->stream));
         HIP_TRY(dalloc(&e->wl[b], e->max_particles));
     }
     HIP_TRY(dalloc(&e->wl_count, 2));
-    HIP_TRY(hipMemset(e->wl_count, 0, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(e->wl_count, 0, 2 * sizeof(uint32_t), e->stream));
     HIP_TRY(dalloc(&e->reps, e->max_reps));
-    HIP_TRY(hipMemset(e->reps, 0, e->max_reps * sizeof(ReplicaDev)));
+    HIP_TRY(hipMemsetAsync(e->reps, 0, e->max_reps * sizeof(ReplicaDev), e->stream));
     HIP_TRY(dalloc(&e->events, (size_t)e->max_reps * e->ev_per_rep));
     HIP_TRY(dalloc(&e->ev_count, e->max_reps));
-    HIP_TRY(hipMemset(e->ev_count, 0, e->max_reps * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(e->ev_count, 0, e->max_reps * sizeof(uint32_t), e->stream));
     const size_t nrx = std::max<uint32_t>(e->n_rx, 1);
     HIP_TRY(dalloc(&e->rx_step, (size_t)e->max_reps * nrx));
     HIP_TRY(dalloc(&e->rx_total, (size_t)e->max_reps * nrx));
-    HIP_TRY(hipMemset(e->rx_step, 0, (size_t)e->max_reps * nrx * 8));
-    HIP_TRY(hipMemset(e->rx_total, 0, (size_t)e->max_reps * nrx * 8));
+    HIP_TRY(hipMemsetAsync(e->rx_step, 0, (size_t)e->max_reps * nrx * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->rx_total, 0, (size_t)e->max_reps * nrx * 8, e->stream));
     HIP_TRY(dalloc(&e->e_bonded, e->max_reps));
-    HIP_TRY(hipMemset(e->e_bonded, 0, e->max_reps * 8));
+    HIP_TRY(hipMemsetAsync(e->e_bonded, 0, e->max_reps * 8, e->stream));
     HIP_TRY(dalloc(&e->err, N_ERR));
-    HIP_TRY(hipMemset(e->err, 0, N_ERR * sizeof(uint32_t)));
+    HIP_TRY(hipMemsetAsync(e->err, 0, N_ERR * sizeof(uint32_t), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     e->free_cells.clear();
     e->free_cells.push_back({0, e->max_cells});
     e->reps_h.assign(e->max_reps, HostReplica{});
@@ -348,7 +351,8 @@ int alloc_pools(rdy_engine *e) {
 int ensure_force_out(rdy_engine *e) {
     if (e->force_out) return RDY_OK;
     HIP_TRY(dalloc(&e->force_out, 3 * e->max_cells * e->cap));
-    HIP_TRY(hipMemset(e->force_out, 0, 3 * e->max_cells * e->cap * sizeof(double)));
+    HIP_TRY(hipMemsetAsync(e->force_out, 0, 3 * e->max_cells * e->cap * sizeof(double), e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     return RDY_OK;
 }
 
@@ -813,6 +817,7 @@ static int set_terms(rdy_engine *e, uint32_t r, int kind, uint64_t n, const uint
         v.push_back(t);
     }
     if (v.size() > e->d_terms_cap[kind]) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
         if (e->d_terms[kind]) HIP_TRY(hipFree(e->d_terms[kind]));
         e->d_terms_cap[kind] = std::max<uint64_t>(v.size(), 2 * e->d_terms_cap[kind]);
         HIP_TRY(dalloc(&e->d_terms[kind], e->d_terms_cap[kind]));
@@ -866,7 +871,8 @@ int rdy_get_energy(rdy_engine *e, uint32_t r, double *energy) {
     if (rc) return rc;
     unsigned long long pe = 0, be = 0;
     if ((rc = get_field(e, r, offsetof(ReplicaDev, energy), &pe))) return rc;
-    HIP_TRY(hipMemcpy(&be, e->e_bonded + r, 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(&be, e->e_bonded + r, 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     *energy = 0.5 * (double)(long long)pe * ENERGY_INV + (double)(long long)be * ENERGY_INV;
     return RDY_OK;
 }
@@ -882,6 +888,7 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
         if ((rc = get_field(e, 0, offsetof(ReplicaDev, handle_next), &hn))) return rc;
         if (noise_handles < hn) return set_err(RDY_ERR_INVALID, "noise buffer shorter than the particle handles");
         if (noise_handles * 3 > e->noise_cap) {
+            HIP_TRY(hipStreamSynchronize(e->stream));
             if (e->d_noise) HIP_TRY(hipFree(e->d_noise));
             e->noise_cap = noise_handles * 3;
             HIP_TRY(dalloc(&e->d_noise, e->noise_cap));
@@ -1023,7 +1030,8 @@ int rdy_neighbor_pairs(rdy_engine *e, uint32_t r, double cutoff, uint64_t cap, u
     *n = hc;
     uint64_t m = std::min<uint64_t>(hc, cap);
     if (m && pairs) {
-        HIP_TRY(hipMemcpy(pairs, out, m * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(pairs, out, m * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
     }
     return RDY_OK;
 }
@@ -1045,7 +1053,10 @@ int rdy_reaction_candidates(rdy_engine *e, uint32_t r, uint64_t cap, uint32_t *r
     HIP_TRY(hipStreamSynchronize(e->stream));
     *n = hc;
     uint64_t m = std::min<uint64_t>(hc, cap);
-    if (m && rows) HIP_TRY(hipMemcpy(rows, out, m * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (m && rows) {
+        HIP_TRY(hipMemcpyAsync(rows, out, m * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
     return check_errors(e);
 }
 
