@@ -73,6 +73,10 @@ struct rdy_engine {
     uint32_t *counts[2] = {nullptr, nullptr};
     uint32_t *wl[2] = {nullptr, nullptr};
     uint32_t *wl_count = nullptr;  // [2]
+    uint32_t *ovf_cell[2] = {nullptr, nullptr};
+    uint32_t *ovf_count = nullptr;  // [2]
+    uint32_t ovf_cap = 0;
+    bool auto_cap = false;
     Event *events = nullptr;
     uint32_t *ev_count = nullptr;
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
@@ -114,7 +118,10 @@ struct rdy_engine {
     double prof_ms = 0.0;
     uint64_t prof_n = 0;
 
-    Grid grid(int b) const { return Grid{bins[b], counts[b], wl[b], wl_count + b}; }
+    Grid grid(int b) const {
+        return Grid{bins[b], counts[b], wl[b], wl_count + b, ovf_cell[b], ovf_count + b, (uint32_t)(max_cells * cap),
+                    ovf_cap};
+    }
     Topo topo() const { return Topo{fb, loc}; }
 };
 
@@ -315,17 +322,21 @@ int upload_tables(rdy_engine *e, double dt) {
 
 int alloc_pools(rdy_engine *e) {
     if (e->pools_ready) return RDY_OK;
-    if ((unsigned __int128)e->max_cells * e->cap >= ((unsigned __int128)1 << 32))
+    e->ovf_cap = (uint32_t)std::max<uint64_t>(4096, e->max_particles / 8);
+    if ((unsigned __int128)e->max_cells * e->cap + e->ovf_cap >= ((unsigned __int128)1 << 32))
         return set_err(RDY_ERR_CAPACITY, "max_cells * bin_capacity must stay below 2^32 slots");
-    const uint64_t slots = e->max_cells * e->cap;
+    const uint64_t slots = e->max_cells * e->cap + e->ovf_cap;
     for (int b = 0; b < 2; ++b) {
         HIP_TRY(dalloc(&e->bins[b], slots));
+        HIP_TRY(dalloc(&e->ovf_cell[b], e->ovf_cap));
         HIP_TRY(dalloc(&e->counts[b], e->max_cells));
         HIP_TRY(hipMemsetAsync(e->counts[b], 0, e->max_cells * sizeof(uint32_t), e->stream));
         HIP_TRY(dalloc(&e->wl[b], e->max_particles));
     }
     HIP_TRY(dalloc(&e->wl_count, 2));
     HIP_TRY(hipMemsetAsync(e->wl_count, 0, 2 * sizeof(uint32_t), e->stream));
+    HIP_TRY(dalloc(&e->ovf_count, 2));
+    HIP_TRY(hipMemsetAsync(e->ovf_count, 0, 2 * sizeof(uint32_t), e->stream));
     HIP_TRY(dalloc(&e->reps, e->max_reps));
     HIP_TRY(hipMemsetAsync(e->reps, 0, e->max_reps * sizeof(ReplicaDev), e->stream));
     HIP_TRY(dalloc(&e->events, (size_t)e->max_reps * e->ev_per_rep));
@@ -350,8 +361,9 @@ int alloc_pools(rdy_engine *e) {
 
 int ensure_force_out(rdy_engine *e) {
     if (e->force_out) return RDY_OK;
-    HIP_TRY(dalloc(&e->force_out, 3 * e->max_cells * e->cap));
-    HIP_TRY(hipMemsetAsync(e->force_out, 0, 3 * e->max_cells * e->cap * sizeof(double), e->stream));
+    const uint64_t slots = e->max_cells * e->cap + e->ovf_cap;
+    HIP_TRY(dalloc(&e->force_out, 3 * slots));
+    HIP_TRY(hipMemsetAsync(e->force_out, 0, 3 * slots * sizeof(double), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return RDY_OK;
 }
@@ -410,7 +422,7 @@ void launch_clear(rdy_engine *e, int b) {
     Grid old = e->grid(b);
     hipLaunchKernelGGL(k_clear_old, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, old, e->cap,
                        (uint32_t)e->max_particles);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, old.wl_count);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, old);
 }
 
 // one fused step over all replicas: SimulationLoop.h:280-300 (integrate, NL, reactions, NL; the
@@ -547,6 +559,7 @@ int rdy_engine_create(const rdy_context_desc *ctx, const rdy_engine_config *cfg,
         return set_err(RDY_ERR_INVALID, "cell_radius must be 1, 2 or 3");
     }
     e->cap = c.bin_capacity ? c.bin_capacity : 16;
+    e->auto_cap = c.bin_capacity == 0;
     e->block = c.block_size ? c.block_size : 256;
     if (e->block % 64 || e->block > 1024) {
         delete e;
@@ -573,8 +586,9 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->bins[b]);
         (void)hipFree(e->counts[b]);
         (void)hipFree(e->wl[b]);
+        (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -611,6 +625,12 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     if (!e->pools_ready) {
         if (e->max_cells == 0) e->max_cells = cells;
         if (e->max_particles == 0) e->max_particles = std::max<uint64_t>(4096, 2 * d->max_particles + 1024);
+        if (e->auto_cap) {
+            // mean occupancy of the first replica's grid: Poisson mean + 7 sigma + 6, a multiple of 4
+            double lam = std::max(1.0, (double)d->max_particles / (double)cells);
+            uint32_t c = (uint32_t)std::ceil(lam + 7.0 * std::sqrt(lam) + 6.0);
+            e->cap = std::max<uint32_t>(8, (c + 3) / 4 * 4);
+        }
         int rc = alloc_pools(e);
         if (rc) return rc;
     }

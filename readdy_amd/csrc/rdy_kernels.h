@@ -5,7 +5,9 @@
 //   bins[2][n_cells * cap]  State (32 B): the particle records themselves, grouped by cell.  A
 //                           particle's slot is cell * cap + rank; the record moves to its new
 //                           cell every step (double buffered), so no separate sort pass exists.
-//   counts[2][n_cells]      particles per cell (atomic insertion counters).
+//   counts[2][n_cells]      particles per cell (atomic insertion counters).  A cell holding more
+//                           than cap particles spills into an overflow region after the cell
+//                           blocks (slot >= nbin); its records carry their cell in ovf_cell.
 //   wl[2][max_particles]    the work list: slots of live particles of the buffer, appended in
 //                           wave-aggregated blocks (keeps neighbouring particles in neighbouring
 //                           threads across steps).
@@ -33,10 +35,14 @@ struct Tables {
 };
 
 struct Grid {  // one bin buffer
-    State *bins;
+    State *bins;        // [nbin + ovf_cap]: cell blocks, then the overflow region
     uint32_t *counts;
     uint32_t *wl;
     uint32_t *wl_count;
+    uint32_t *ovf_cell;   // [ovf_cap] cell of each overflow record
+    uint32_t *ovf_count;
+    uint32_t nbin;        // n_cells * cap: first overflow slot
+    uint32_t ovf_cap;
 };
 
 struct Topo {

@@ -56,9 +56,14 @@ __device__ __forceinline__ RepGeom load_geom(const ReplicaDev &R) {
 
 // visits every slot of the stencil around local cell `cl` (own cell included; the caller skips
 // itself): the (2r+1)^3 adjacency of CellLinkedList.cpp:74-137 with the same de-duplication
+__device__ __forceinline__ uint32_t cell_of_slot(const Grid &g, uint32_t s, uint32_t cap) {
+    return s < g.nbin ? s / cap : g.ovf_cell[s - g.nbin];
+}
+
 template <class F>
-__device__ __forceinline__ void for_each_stencil_slot(const RepGeom &g, const uint32_t *counts, uint32_t cap,
-                                                      int radius, uint32_t cl, F &&f) {
+__device__ __forceinline__ void for_each_stencil_slot(const RepGeom &g, const Grid &grid, uint32_t cap, int radius,
+                                                      uint32_t cl, F &&f) {
+    const uint32_t *counts = grid.counts;
     const int nz = g.dims[2], ny = g.dims[1];
     int iz = (int)(cl % (uint32_t)nz);
     int iy = (int)((cl / (uint32_t)nz) % (uint32_t)ny);
@@ -72,10 +77,16 @@ __device__ __forceinline__ void for_each_stencil_slot(const RepGeom &g, const ui
             uint32_t row = g.cell_base + ((uint32_t)ax[a] * (uint32_t)ny + (uint32_t)ay[b]) * (uint32_t)nz;
             for (int c = 0; c < naz; ++c) {
                 uint32_t cell = row + (uint32_t)az[c];
-                uint32_t n = counts[cell];
-                n = n < cap ? n : cap;
-                uint32_t base = cell * cap;
-                for (uint32_t k = 0; k < n; ++k) f(base + k);
+                const uint32_t n = counts[cell];
+                const uint32_t m = n < cap ? n : cap;
+                const uint32_t base = cell * cap;
+                for (uint32_t k = 0; k < m; ++k) f(base + k);
+                if (n > cap) {  // rare: a crowded cell spilled into the overflow region
+                    uint32_t no = *grid.ovf_count;
+                    no = no < grid.ovf_cap ? no : grid.ovf_cap;
+                    for (uint32_t o = 0; o < no; ++o)
+                        if (grid.ovf_cell[o] == cell) f(grid.nbin + o);
+                }
             }
         }
 }
@@ -121,18 +132,25 @@ __device__ __forceinline__ uint32_t bin_insert(const Grid &g, const RepGeom &geo
     bool outside = false;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        if (!geo.pbc[d] && !in_box_axis(p[d], geo.box[d])) outside = true;
+        if (!in_box_axis(p[d], geo.box[d])) outside = true;
         c[d] = cell_coord(p[d], geo.box[d], geo.cs[d], geo.dims[d]);
     }
     st.tw = (st.tw & ~TW_OUTSIDE) | (outside ? TW_OUTSIDE : 0u);
     uint32_t cell = geo.cell_base + ((uint32_t)c[0] * (uint32_t)geo.dims[1] + (uint32_t)c[1]) * (uint32_t)geo.dims[2] +
                     (uint32_t)c[2];
     uint32_t k = atomicAdd(&g.counts[cell], 1u);
-    if (k >= cap) {
-        atomicAdd(&err[ERR_BIN_OVERFLOW], 1u);
-        return 0xFFFFFFFFu;
+    uint32_t slot;
+    if (k < cap) {
+        slot = cell * cap + k;
+    } else {
+        uint32_t o = atomicAdd(g.ovf_count, 1u);
+        if (o >= g.ovf_cap) {
+            atomicAdd(&err[ERR_BIN_OVERFLOW], 1u);
+            return 0xFFFFFFFFu;
+        }
+        g.ovf_cell[o] = cell;
+        slot = g.nbin + o;
     }
-    uint32_t slot = cell * cap + k;
     g.bins[slot] = st;
     if (append_wl) {
         uint32_t pos = atomicAdd(g.wl_count, 1u);
@@ -168,10 +186,13 @@ __global__ void k_clear_old(Grid old, uint32_t cap, uint32_t max_wl) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *old.wl_count;
     n = n < max_wl ? n : max_wl;
-    if (i < n) old.counts[old.wl[i] / cap] = 0u;
+    if (i < n) old.counts[cell_of_slot(old, old.wl[i], cap)] = 0u;
 }
 
-__global__ void k_reset_counter(uint32_t *c) { *c = 0u; }
+__global__ void k_reset_counter(Grid old) {
+    *old.wl_count = 0u;
+    *old.ovf_count = 0u;
+}
 
 // -------------------------------------------------------------------------------------------------
 // the fused per-particle kernel: forces (+ Euler-Maruyama + re-bin)
@@ -210,8 +231,8 @@ __global__ __launch_bounds__(256) void k_fused(Grid cur, Grid nxt, ReplicaDev *r
     if (do_forces && !(me.tw & TW_OUTSIDE)) {
         const uint32_t t = tw_type(me.tw);
         const PairIdx *prow = T.pidx + (size_t)t * T.n_types;
-        const uint32_t cl = s / cap - geo.cell_base;
-        for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+        const uint32_t cl = cell_of_slot(cur, s, cap) - geo.cell_base;
+        for_each_stencil_slot(geo, cur, cap, radius, cl, [&](uint32_t s2) {
             if (s2 == s) return;
             const State o = cur.bins[s2];
             if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
@@ -589,8 +610,8 @@ __global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tabl
     if (!T.has_o2 || (me.tw & TW_OUTSIDE)) return;
     const RepGeom geo = load_geom(R);
     const RxPairIdx *orow = T.o2idx + (size_t)t * T.n_types;
-    const uint32_t cl = s / cap - geo.cell_base;
-    for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+    const uint32_t cl = cell_of_slot(cur, s, cap) - geo.cell_base;
+    for_each_stencil_slot(geo, cur, cap, radius, cl, [&](uint32_t s2) {
         if (s2 == s) return;
         const State o = cur.bins[s2];
         if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
@@ -947,8 +968,8 @@ __global__ void k_pairs(Grid cur, ReplicaDev *reps, uint32_t cap, int radius, ui
     const State me = cur.bins[s];
     if ((me.tw & (TW_DEAD | TW_OUTSIDE)) || tw_replica(me.tw) != r) return;
     const RepGeom geo = load_geom(reps[r]);
-    const uint32_t cl = s / cap - geo.cell_base;
-    for_each_stencil_slot(geo, cur.counts, cap, radius, cl, [&](uint32_t s2) {
+    const uint32_t cl = cell_of_slot(cur, s, cap) - geo.cell_base;
+    for_each_stencil_slot(geo, cur, cap, radius, cl, [&](uint32_t s2) {
         if (s2 == s) return;
         const State o = cur.bins[s2];
         if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;

@@ -320,7 +320,8 @@ def test_reaction_steps_match_oracle_event_for_event(handler, mode):
         assert np.array_equal(gt, ot[order].astype(np.uint32))
         d = gp - op[order]
         d -= Lb * np.round(d / Lb)
-        assert np.abs(d).max() < 1e-9
+        # FP32 pair forces (rel. error ~1e-7) move positions by D dt dF ~ 1e-8 per step
+        assert np.abs(d).max() < 1e-6
         assert np.array_equal(e.reaction_counts(r), o.reaction_counts())
         total += int(e.reaction_counts(r).sum())
         # forces for the next step are taken from the same state on both sides
