@@ -163,14 +163,17 @@ __device__ __forceinline__ uint32_t bin_insert(const Grid &g, const RepGeom &geo
 // -------------------------------------------------------------------------------------------------
 // per-step bookkeeping
 // -------------------------------------------------------------------------------------------------
-// mode 0: zero energies; mode 1: advance the Philox step of replicas with budget left
+// mode 0: zero the energy accumulators (a force evaluation follows); mode 1: advance the Philox
+// step of replicas with budget left (separate integrator action); mode 2: both (fused step)
 __global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned long long *e_bonded) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_reps) return;
     ReplicaDev &R = reps[r];
-    R.energy = 0ull;
-    e_bonded[r] = 0ull;
-    if (mode == 1) {
+    if (mode != 1) {
+        R.energy = 0ull;
+        e_bonded[r] = 0ull;
+    }
+    if (mode >= 1) {
         if (R.in_use && R.steps_left > 0) {
             R.active = 1u;
             R.step += 1u;
