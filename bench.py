@@ -1,0 +1,148 @@
+"""Benchmark: particle-steps/s of the fused MI355X ReaDDy step on the 1M-particle A+B<->C system.
+
+Workload (the headline configuration): periodic 200^3 box, 1,000,000 particles (40% A, 40% B,
+20% C, uniform random, seed 1), harmonic repulsion k=10 d=1.0 between all species pairs,
+weak-interaction piecewise harmonic A-A (k=10, d0=1.0, depth 0.5, cutoff 2.0), fusion A+B->C
+rate 0.1 radius 1.0, fission C->A+B rate 0.05 distance 1.0, D_A = D_B = 1, D_C = 0.5,
+EulerBD dt=0.005, UncontrolledApproximation.  FP32 pair arithmetic on FP64 positions.
+
+Prints one JSON line.  `value` is whole-job particle-steps/s with the state resident in HBM;
+`roofline` prices the fused force/integrate/re-bin kernel (64 algorithmic bytes per particle-step)
+against MI355X's 8 TB/s; `cpu_baseline` is the FP64 CPU oracle timed on a bounded sample of
+the same workload on this host.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M particles; batched ensemble throughput + p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+B_ALG = 64.0  # bytes per particle-step: FP64 position + uint32 species + uint32 handle, read + written
+
+
+def c1_system(n=1_000_000, L=200.0, seed=1):
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((n, 3)) - 0.5) * L
+    types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
+    return pos, types
+
+
+def c1_chemistry(L):
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    rxs = [dict(kind=L.RX_FUSION, educts=[0, 1], products=[2], rate=0.1, educt_distance=1.0),
+           dict(kind=L.RX_FISSION, educts=[2], products=[0, 1], rate=0.05, product_distance=1.0)]
+    return pots, rxs
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", "fused_step_c1_1M_pmc.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n_steps=3):
+    """FP64 oracle on the same 1M-particle system, single thread, a few steps (bounded sample)."""
+    from oracle import pyoracle as po
+    pos, types = c1_system()
+    o = po.System(3, [200.0] * 3, [1, 1, 1], diffusion=[1.0, 1.0, 0.5])
+    o.set_seed(1)
+    for a in range(3):
+        for b in range(a, 3):
+            o.add_pair_potential(po.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0])
+    o.add_pair_potential(po.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0])
+    o.add_reaction(po.RX_FUSION, [0, 1], [2], 0.1, 1.0)
+    o.add_reaction(po.RX_FISSION, [2], [0, 1], 0.05, 0.0, 1.0)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    o.nl_setup(o.max_cutoff())
+    o.calculate_forces()
+    t0 = time.perf_counter()
+    ps = 0
+    for t in range(n_steps):
+        ps += o.n_active
+        o.loop_step(0.005, po.UA_KEYED, t)
+    dt = time.perf_counter() - t0
+    return {"value": ps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n_steps} full steps of the 1M-particle workload (integrate, neighbour list, UA reactions, "
+                      f"neighbour list, forces) by the FP64 C oracle on one host core; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build measures a single MI355X (--gpus {args.gpus} requested). "
+              f"Replica ensembles shard across GPUs by replica; see the sharding notes in the design document.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from readdy_amd import Engine, device_info
+    from readdy_amd import _lib as L
+
+    pos, types = c1_system()
+    pots, rxs = c1_chemistry(L)
+    eng = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs)
+    r = eng.add_replica([200.0] * 3, (1, 1, 1), seed=1, max_particles=len(types))
+    eng.set_particles(r, pos, types)
+    dt = 0.005
+    handler = L.HANDLER_UA
+    graph = not args.eager
+    eng.run(handler, dt, args.warmup, graph=graph)
+    eng.synchronize()
+    n0 = eng.n_particles(r)
+    t0 = time.perf_counter()
+    eng.run(handler, dt, args.steps, graph=graph)  # returns synchronized
+    elapsed = time.perf_counter() - t0
+    n1 = eng.n_particles(r)
+    particle_steps = 0.5 * (n0 + n1) * args.steps
+    value = particle_steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # fused-kernel duration with HIP events on the engine's stream
+    eng.profile_reset()
+    eng.run(handler, dt, args.profile_steps, graph=False)
+    avg_us, launches = eng.profile_get()
+    n_mid = eng.n_particles(r)
+    achieved = (n_mid * B_ALG) / (avg_us * 1e-6) / 1e9  # GB/s
+    traffic = traffic_from_profiles()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64 positions, fp32 pair forces",
+        "data": "synthetic (uniform random positions, seed 1)",
+        "config": {"workload": "A+B<->C, 1,000,000 particles in a periodic 200^3 box, UncontrolledApproximation, "
+                               "dt=0.005, harmonic repulsion + A-A weak interaction", "particles": int(n0),
+                   "box": 200.0, "handler": "UncontrolledApproximation", "graph": graph},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused<MODE_STEP> (forces + Euler-Maruyama + re-bin)",
+                     "avg_kernel_us": avg_us, "launches": launches},
+        "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
+        "device": device_info()["name"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
