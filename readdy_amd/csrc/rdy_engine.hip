@@ -77,6 +77,8 @@ struct rdy_engine {
     uint32_t *ovf_count = nullptr;  // [2]
     uint32_t ovf_cap = 0;
     bool auto_cap = false;
+    uint32_t *tile_sums = nullptr;
+    uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
     uint32_t *ev_count = nullptr;
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
@@ -335,6 +337,7 @@ int alloc_pools(rdy_engine *e) {
     }
     HIP_TRY(dalloc(&e->wl_count, 2));
     HIP_TRY(hipMemsetAsync(e->wl_count, 0, 2 * sizeof(uint32_t), e->stream));
+    HIP_TRY(dalloc(&e->tile_sums, (e->max_cells + SCAN_TILE - 1) / SCAN_TILE + 2));
     HIP_TRY(dalloc(&e->ovf_count, 2));
     HIP_TRY(hipMemsetAsync(e->ovf_count, 0, 2 * sizeof(uint32_t), e->stream));
     HIP_TRY(dalloc(&e->reps, e->max_reps));
@@ -417,6 +420,18 @@ void launch_bonded(rdy_engine *e) {
 
 uint32_t wl_blocks(const rdy_engine *e) { return blocks_for(e->max_particles, e->block); }
 
+// rebuilds buffer b's work list in cell order from its cell counters
+void launch_wl_rebuild(rdy_engine *e, int b) {
+    Grid g = e->grid(b);
+    const uint32_t nc = (uint32_t)std::max<uint64_t>(e->cells_hwm, 1);
+    const uint32_t tiles = (nc + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_wl_tiles, dim3(tiles), dim3(SCAN_THREADS), 0, e->stream, g, e->cap, nc, e->tile_sums);
+    hipLaunchKernelGGL(k_wl_tile_scan, dim3(1), dim3(1024), 0, e->stream, e->tile_sums, tiles, g,
+                       (uint32_t)e->max_particles, e->err);
+    hipLaunchKernelGGL(k_wl_write, dim3(tiles), dim3(SCAN_THREADS), 0, e->stream, g, e->cap, nc,
+                       (const uint32_t *)e->tile_sums, tiles, (uint32_t)e->max_particles);
+}
+
 // the old buffer becomes the next target: zero its cell counters, then its work-list length
 void launch_clear(rdy_engine *e, int b) {
     Grid old = e->grid(b);
@@ -441,6 +456,7 @@ void launch_step(rdy_engine *e, int handler, double dt) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
     }
+    launch_wl_rebuild(e, n);
     launch_clear(e, c);
     if (handler != RDY_HANDLER_NONE && e->n_rx > 0) {
         hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(n), e->reps, e->T,
@@ -588,7 +604,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->wl[b]);
         (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->ovf_count, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -673,6 +689,7 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     HIP_TRY(hipMemsetAsync(e->rx_step + (size_t)r * nrx, 0, nrx * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->rx_total + (size_t)r * nrx, 0, nrx * 8, e->stream));
     e->n_reps_used = std::max(e->n_reps_used, r + 1);
+    e->cells_hwm = std::max<uint64_t>(e->cells_hwm, base + cells);
     e->version++;
     *replica_id = r;
     return RDY_OK;
@@ -764,6 +781,7 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
                            (uint32_t)n, dpos, dty, (const uint8_t *)e->d_flavor, e->topo(), e->cap,
                            (uint32_t)e->max_particles, e->err);
     }
+    launch_wl_rebuild(e, e->cur);
     e->version++;
     return check_errors(e);
 }
@@ -872,6 +890,7 @@ int rdy_update_neighbor_list(rdy_engine *e) {
     hipLaunchKernelGGL(k_fused<MODE_REBIN>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(n),
                        e->reps, e->T, e->topo(), e->cap, e->radius, 0.0, (const double *)nullptr, (double *)nullptr,
                        (uint32_t)e->max_particles, e->err);
+    launch_wl_rebuild(e, n);
     launch_clear(e, c);
     e->cur = n;
     return check_errors(e);

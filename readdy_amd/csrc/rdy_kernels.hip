@@ -198,6 +198,128 @@ __global__ void k_reset_counter(Grid old) {
 }
 
 // -------------------------------------------------------------------------------------------------
+// work-list rebuild in cell order: an exclusive scan of the per-cell counters, then every cell writes
+// the slots of its members.  Neighbouring threads then hold neighbouring particles, so the stencil
+// reads of a wave (and of the blocks an XCD runs) hit the same L2 lines.
+//   pass 1: tile sums (SCAN_TILE cells per block)      pass 2: scan of the tile sums (one block)
+//   pass 3: per-cell offsets + slot writes; the overflow records follow the cell-ordered slots
+// -------------------------------------------------------------------------------------------------
+constexpr uint32_t SCAN_THREADS = 256;
+constexpr uint32_t SCAN_PER_THREAD = 16;
+constexpr uint32_t SCAN_TILE = SCAN_THREADS * SCAN_PER_THREAD;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t warp_sums[SCAN_THREADS / 64];
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) warp_sums[wid] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t w = 0; w < SCAN_THREADS / 64; ++w) {
+        if (w < wid) base += warp_sums[w];
+        tot += warp_sums[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_wl_tiles(Grid g, uint32_t cap, uint32_t n_cells,
+                                                           uint32_t *tile_sums) {
+    const uint32_t c0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+        uint32_t c = c0 + k;
+        if (c < n_cells) {
+            uint32_t n = g.counts[c];
+            sum += n < cap ? n : cap;
+        }
+    }
+    uint32_t total;
+    block_exclusive_scan(sum, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_wl_tile_scan(uint32_t *tile_sums, uint32_t n_tiles, Grid g,
+                                                        uint32_t max_wl, uint32_t *err) {
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n_tiles; base += blockDim.x) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < n_tiles ? tile_sums[i] : 0u;
+        // inclusive scan across the block with wave shuffles + LDS
+        __shared__ uint32_t ws[16];
+        uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+        uint32_t x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(x, off);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) ws[wid] = x;
+        __syncthreads();
+        uint32_t pre = 0, tot = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+            if (w < wid) pre += ws[w];
+            tot += ws[w];
+        }
+        if (i < n_tiles) tile_sums[i] = carry + pre + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t no = *g.ovf_count;
+        no = no < g.ovf_cap ? no : g.ovf_cap;
+        uint32_t n = carry + no;
+        if (n > max_wl) {
+            atomicAdd(&err[ERR_WL_OVERFLOW], 1u);
+            n = max_wl;
+        }
+        *g.wl_count = n;
+        tile_sums[n_tiles] = carry;  // start of the overflow records in the work list
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_wl_write(Grid g, uint32_t cap, uint32_t n_cells,
+                                                           const uint32_t *tile_sums, uint32_t n_tiles,
+                                                           uint32_t max_wl) {
+    const uint32_t c0 = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t cnt[SCAN_PER_THREAD];
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+        uint32_t c = c0 + k;
+        uint32_t n = c < n_cells ? g.counts[c] : 0u;
+        n = n < cap ? n : cap;
+        cnt[k] = n;
+        sum += n;
+    }
+    uint32_t total;
+    uint32_t off = tile_sums[blockIdx.x] + block_exclusive_scan(sum, &total);
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_PER_THREAD; ++k) {
+        const uint32_t c = c0 + k;
+        for (uint32_t q = 0; q < cnt[k]; ++q) {
+            if (off < max_wl) g.wl[off] = c * cap + q;
+            ++off;
+        }
+    }
+    // overflow records (rare) go after the cell-ordered slots
+    if (blockIdx.x == 0) {
+        uint32_t no = *g.ovf_count;
+        no = no < g.ovf_cap ? no : g.ovf_cap;
+        const uint32_t start = tile_sums[n_tiles];
+        for (uint32_t o = threadIdx.x; o < no; o += blockDim.x)
+            if (start + o < max_wl) g.wl[start + o] = g.nbin + o;
+    }
+}
+
+// -------------------------------------------------------------------------------------------------
 // the fused per-particle kernel: forces (+ Euler-Maruyama + re-bin)
 // Restates CPUCalculateForces::calculateOrder2 (CPUCalculateForces.cpp:214-268), the integrator
 // (CPUEulerBDIntegrator.cpp:63-73) and CompactCellLinkedList::fillBins (CellLinkedList.cpp:171-228).
@@ -306,15 +428,10 @@ __global__ __launch_bounds__(256) void k_fused(Grid cur, Grid nxt, ReplicaDev *r
         nw.y = p[1];
         nw.z = p[2];
     }
-    // ---- re-bin into the other buffer, wave-aggregated work-list append ----
-    uint32_t slot = 0xFFFFFFFFu;
-    if (valid) slot = bin_insert(nxt, geo, cap, nw, err, false, 0);
-    const bool keep = valid && slot != 0xFFFFFFFFu;
-    const uint32_t pos = wave_append(nxt.wl_count, keep);
-    if (keep) {
-        if (pos < max_wl) nxt.wl[pos] = slot;
-        else atomicAdd(&err[ERR_WL_OVERFLOW], 1u);
-        if ((me.tw & TW_TOPO) && topo.loc) topo.loc[reps[r].topo_base + me.h] = slot;
+    // ---- re-bin into the other buffer (its work list is rebuilt in cell order afterwards) ----
+    if (valid) {
+        const uint32_t slot = bin_insert(nxt, geo, cap, nw, err, false, 0);
+        if (slot != 0xFFFFFFFFu && (me.tw & TW_TOPO) && topo.loc) topo.loc[reps[r].topo_base + me.h] = slot;
     }
 }
 
@@ -920,7 +1037,7 @@ __global__ void k_upload(Grid g, ReplicaDev *reps, uint32_t r, uint32_t n, const
     const bool is_topo = topo_flavor && topo_flavor[t];
     st.tw = make_tw(t, is_topo ? TW_TOPO : 0u, r);
     st.h = i;
-    uint32_t slot = bin_insert(g, geo, cap, st, err, true, max_wl);
+    uint32_t slot = bin_insert(g, geo, cap, st, err, false, max_wl);
     if (is_topo && topo.loc && slot != 0xFFFFFFFFu) topo.loc[R.topo_base + i] = slot;
 }
 
