@@ -88,6 +88,8 @@ typedef struct {
     uint64_t max_particles;   /* total live particles over all replicas                                 */
     uint32_t events_per_replica; /* accepted reaction events per replica per step (0 = automatic)       */
     uint32_t block_size;      /* threads per block of the per-particle kernels (0 = 256)               */
+    uint32_t brick[3];        /* cells per LDS-staged brick along x, y, z (0 = from the density)        */
+    uint32_t lds_states;      /* particle records a brick may stage in LDS (0 = from the density)        */
 } rdy_engine_config;
 
 typedef struct {

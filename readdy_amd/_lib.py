@@ -46,7 +46,7 @@ class EngineConfig(ctypes.Structure):
     _fields_ = [("nl_cutoff", ctypes.c_double), ("cell_radius", ctypes.c_int32), ("bin_capacity", ctypes.c_uint32),
                 ("max_replicas", ctypes.c_uint32), ("max_cells", ctypes.c_uint64),
                 ("max_particles", ctypes.c_uint64), ("events_per_replica", ctypes.c_uint32),
-                ("block_size", ctypes.c_uint32)]
+                ("block_size", ctypes.c_uint32), ("brick", ctypes.c_uint32 * 3), ("lds_states", ctypes.c_uint32)]
 
 
 class ReplicaDesc(ctypes.Structure):

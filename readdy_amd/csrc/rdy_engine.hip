@@ -1,6 +1,6 @@
 // Host side of the MI355X ReaDDy engine: device pools, replica admission, the per-step launch
 // sequence (eager and hipGraph), and the extern "C" ABI declared in include/readdy_amd.h.
-#include "rdy_kernels.hip"
+#include "rdy_bricks.hip"
 
 #include "../../include/readdy_amd.h"
 
@@ -78,6 +78,15 @@ struct rdy_engine {
     uint32_t ovf_cap = 0;
     bool auto_cap = false;
     uint32_t *tile_sums = nullptr;
+    // LDS-staged bricks
+    uint32_t brick[3] = {0, 0, 0};
+    uint32_t max_states = 0, max_halo = 0;
+    size_t brick_smem = 0;
+    size_t lds_limit = 65536;
+    std::vector<BrickDev> bricks_h;
+    BrickDev *d_bricks = nullptr;
+    size_t d_bricks_cap = 0;
+    bool wl_valid[2] = {true, true};
     uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
     uint32_t *ev_count = nullptr;
@@ -402,6 +411,98 @@ int check_replica(rdy_engine *e, uint32_t r) {
     return RDY_OK;
 }
 
+// splits replica r's grid into bricks (halo mode where the axis is long enough, whole-axis otherwise)
+void make_bricks(rdy_engine *e, uint32_t r, std::vector<BrickDev> &out) {
+    const ReplicaDev &R = e->reps_h[r].dev;
+    std::vector<std::pair<int32_t, int32_t>> seg[3];
+    uint32_t whole = 0, halo[3];
+    for (int a = 0; a < 3; ++a) {
+        const int32_t D = R.dims[a], B = (int32_t)e->brick[a], h = e->radius;
+        const bool w = D <= B || (R.pbc[a] && D < B + 2 * h);
+        if (w) {
+            whole |= 1u << a;
+            seg[a].push_back({0, D});
+            halo[a] = 0;
+        } else {
+            for (int32_t o = 0; o < D; o += B) seg[a].push_back({o, std::min(B, D - o)});
+            halo[a] = (uint32_t)h;
+        }
+    }
+    for (auto &x : seg[0])
+        for (auto &y : seg[1])
+            for (auto &z : seg[2]) {
+                BrickDev b{};
+                b.o[0] = x.first;
+                b.o[1] = y.first;
+                b.o[2] = z.first;
+                b.e[0] = x.second;
+                b.e[1] = y.second;
+                b.e[2] = z.second;
+                for (int a = 0; a < 3; ++a) b.h[a] = (int32_t)halo[a];
+                b.r = r;
+                b.whole = whole;
+                out.push_back(b);
+            }
+}
+
+int upload_bricks(rdy_engine *e) {
+    e->bricks_h.clear();
+    for (uint32_t r = 0; r < e->max_reps; ++r)
+        if (e->reps_h[r].in_use) make_bricks(e, r, e->bricks_h);
+    if (e->bricks_h.size() > e->d_bricks_cap) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (e->d_bricks) HIP_TRY(hipFree(e->d_bricks));
+        e->d_bricks_cap = std::max<size_t>(e->bricks_h.size(), 2 * e->d_bricks_cap);
+        HIP_TRY(dalloc(&e->d_bricks, e->d_bricks_cap));
+    }
+    if (!e->bricks_h.empty())
+        HIP_TRY(hipMemcpyAsync(e->d_bricks, e->bricks_h.data(), e->bricks_h.size() * sizeof(BrickDev),
+                               hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+uint32_t n_bricks(const rdy_engine *e) { return (uint32_t)std::max<size_t>(e->bricks_h.size(), 1); }
+
+// brick shape and LDS budget from the expected particles per cell (overridable in the config)
+int configure_bricks(rdy_engine *e, double lam) {
+    if (e->brick[0] == 0 || e->brick[1] == 0 || e->brick[2] == 0) {
+        if (lam >= 3.0) {
+            e->brick[0] = 4, e->brick[1] = 4, e->brick[2] = 4;
+        } else if (lam >= 0.75) {
+            e->brick[0] = 8, e->brick[1] = 8, e->brick[2] = 4;
+        } else {
+            e->brick[0] = 8, e->brick[1] = 8, e->brick[2] = 8;
+        }
+    }
+    e->max_halo = 1;
+    for (int a = 0; a < 3; ++a) e->max_halo *= e->brick[a] + 2u * (uint32_t)e->radius;
+    if (e->max_halo > 60000) return set_err(RDY_ERR_INVALID, "brick plus halo exceeds 60000 cells");
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess && v > 0)
+        e->lds_limit = (size_t)v;
+    const size_t budget = e->lds_limit - 1024;  // static LDS of the brick kernels
+    uint32_t want = e->max_states;
+    if (want == 0) want = (uint32_t)std::max(1024.0, 2.0 * std::max(lam, 0.25) * e->max_halo + 256.0);
+    want = std::min<uint32_t>(want, 60000);
+    while (want > 256 && brick_smem_bytes(want, e->max_halo) > budget) want -= 128;
+    if (brick_smem_bytes(want, e->max_halo) > budget)
+        return set_err(RDY_ERR_INVALID, "brick does not fit in LDS: use a smaller brick");
+    e->max_states = want;
+    e->brick_smem = brick_smem_bytes(want, e->max_halo);
+    if (e->brick_smem > 65536) {
+        (void)hipFuncSetAttribute((const void *)k_brick<MODE_STEP>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)e->brick_smem);
+        (void)hipFuncSetAttribute((const void *)k_brick<MODE_FORCES>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)e->brick_smem);
+        (void)hipFuncSetAttribute((const void *)k_brick<MODE_REBIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)e->brick_smem);
+        (void)hipFuncSetAttribute((const void *)k_brick_search, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)e->brick_smem);
+    }
+    return RDY_OK;
+}
+
 // launches the bonded-term kernels on the current buffer
 void launch_bonded(rdy_engine *e) {
     BondTerm *t;
@@ -430,6 +531,21 @@ void launch_wl_rebuild(rdy_engine *e, int b) {
                        (uint32_t)e->max_particles, e->err);
     hipLaunchKernelGGL(k_wl_write, dim3(tiles), dim3(SCAN_THREADS), 0, e->stream, g, e->cap, nc,
                        (const uint32_t *)e->tile_sums, tiles, (uint32_t)e->max_particles);
+    e->wl_valid[b] = true;
+}
+
+// the work list is only needed by per-particle helpers (gather, exports, the separate integrator)
+void ensure_wl(rdy_engine *e) {
+    if (!e->wl_valid[e->cur]) launch_wl_rebuild(e, e->cur);
+}
+
+// clears buffer b's cell counters (brick-wise) and its work-list / overflow lengths
+void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
+    if (!e->bricks_h.empty())
+        hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
+                           e->grid(grid_of_bricks_buffer), e->counts[b], e->reps, e->T, (const BrickDev *)e->d_bricks,
+                           e->cap, e->radius, 2, e->events, e->ev_count, e->max_states, e->max_halo, e->err);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
 }
 
 // the old buffer becomes the next target: zero its cell counters, then its work-list length
@@ -449,23 +565,24 @@ void launch_step(rdy_engine *e, int handler, double dt) {
     const int c = e->cur, n = 1 - c;
     if (e->prof && e->prof_used < e->prof_ev.size())
         (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
-    hipLaunchKernelGGL(k_fused<MODE_STEP>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(n),
-                       e->reps, e->T, e->topo(), e->cap, e->radius, dt, (const double *)nullptr, (double *)nullptr,
-                       (uint32_t)e->max_particles, e->err);
+    hipLaunchKernelGGL(k_brick<MODE_STEP>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
+                       e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
+                       e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, e->err);
     if (e->prof && e->prof_used < e->prof_ev.size()) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
     }
-    launch_wl_rebuild(e, n);
-    launch_clear(e, c);
-    if (handler != RDY_HANDLER_NONE && e->n_rx > 0) {
-        hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(n), e->reps, e->T,
-                           e->cap, e->radius, handler, 0, e->events, e->ev_count, (uint32_t *)nullptr, 0u, 0u,
-                           (uint32_t)e->max_particles, e->err);
+    // reaction search on the new buffer, fused with clearing the old buffer's counters
+    const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
+    hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(n),
+                       e->counts[c], e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events,
+                       e->ev_count, e->max_states, e->max_halo, e->err);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(c));
+    if (h != RDY_HANDLER_NONE)
         hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(n), e->reps, e->T,
                            e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 1, (uint32_t)e->max_particles,
                            e->err);
-    }
+    e->wl_valid[n] = false;
     e->cur = n;
 }
 
@@ -474,9 +591,9 @@ void launch_forces(rdy_engine *e, double *force_out) {
     hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded);
     launch_bonded(e);
     const int c = e->cur;
-    hipLaunchKernelGGL(k_fused<MODE_FORCES>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(c),
-                       e->reps, e->T, e->topo(), e->cap, e->radius, 0.0, (const double *)nullptr, force_out,
-                       (uint32_t)e->max_particles, e->err);
+    hipLaunchKernelGGL(k_brick<MODE_FORCES>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
+                       e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
+                       e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo, e->err);
 }
 
 int flush_prof(rdy_engine *e) {
@@ -576,6 +693,8 @@ int rdy_engine_create(const rdy_context_desc *ctx, const rdy_engine_config *cfg,
     }
     e->cap = c.bin_capacity ? c.bin_capacity : 16;
     e->auto_cap = c.bin_capacity == 0;
+    for (int a = 0; a < 3; ++a) e->brick[a] = c.brick[a];
+    e->max_states = c.lds_states;
     e->block = c.block_size ? c.block_size : 256;
     if (e->block % 64 || e->block > 1024) {
         delete e;
@@ -604,7 +723,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->wl[b]);
         (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -641,12 +760,15 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     if (!e->pools_ready) {
         if (e->max_cells == 0) e->max_cells = cells;
         if (e->max_particles == 0) e->max_particles = std::max<uint64_t>(4096, 2 * d->max_particles + 1024);
+        const double lam_raw = (double)d->max_particles / (double)cells;
         if (e->auto_cap) {
             // mean occupancy of the first replica's grid: Poisson mean + 7 sigma + 6, a multiple of 4
-            double lam = std::max(1.0, (double)d->max_particles / (double)cells);
+            double lam = std::max(1.0, lam_raw);
             uint32_t c = (uint32_t)std::ceil(lam + 7.0 * std::sqrt(lam) + 6.0);
             e->cap = std::max<uint32_t>(8, (c + 3) / 4 * 4);
         }
+        int rcc = configure_bricks(e, lam_raw);
+        if (rcc) return rcc;
         int rc = alloc_pools(e);
         if (rc) return rc;
     }
@@ -691,6 +813,8 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     e->n_reps_used = std::max(e->n_reps_used, r + 1);
     e->cells_hwm = std::max<uint64_t>(e->cells_hwm, base + cells);
     e->version++;
+    int rcb = upload_bricks(e);
+    if (rcb) return rcb;
     *replica_id = r;
     return RDY_OK;
 }
@@ -698,12 +822,16 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
 int rdy_replica_remove(rdy_engine *e, uint32_t r) {
     int rc = check_replica(e, r);
     if (rc) return rc;
+    ensure_wl(e);
     hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
                        (uint32_t)e->max_particles);
     uint32_t zero = 0;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, in_use), zero))) return rc;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, active), zero))) return rc;
     HostReplica &H = e->reps_h[r];
+    for (int b = 0; b < 2; ++b)
+        hipLaunchKernelGGL(k_release_cells, dim3(blocks_for(H.n_cells, 256)), dim3(256), 0, e->stream, e->grid(b),
+                           (uint32_t)H.cell_base, (uint32_t)H.n_cells);
     e->free_cells.push_back({H.cell_base, H.n_cells});
     std::sort(e->free_cells.begin(), e->free_cells.end());
     std::vector<std::pair<uint64_t, uint64_t>> merged;
@@ -714,8 +842,9 @@ int rdy_replica_remove(rdy_engine *e, uint32_t r) {
     e->free_cells.swap(merged);
     H.in_use = false;
     e->version++;
+    e->wl_valid[0] = e->wl_valid[1] = false;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    return RDY_OK;
+    return upload_bricks(e);
 }
 
 int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, const uint32_t *types) {
@@ -734,6 +863,7 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
         }
     }
     // replace: drop the replica's current particles
+    ensure_wl(e);
     hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
                        (uint32_t)e->max_particles);
     HostReplica &H = e->reps_h[r];
@@ -781,7 +911,7 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
                            (uint32_t)n, dpos, dty, (const uint8_t *)e->d_flavor, e->topo(), e->cap,
                            (uint32_t)e->max_particles, e->err);
     }
-    launch_wl_rebuild(e, e->cur);
+    e->wl_valid[e->cur] = false;
     e->version++;
     return check_errors(e);
 }
@@ -805,6 +935,7 @@ static int gather(rdy_engine *e, uint32_t r, uint64_t cap, double *pos, uint32_t
     uint32_t *dty = (uint32_t *)(dfor + 3 * ncap);
     uint32_t *dh = dty + ncap;
     HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    ensure_wl(e);
     hipLaunchKernelGGL(k_gather, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r, cnt,
                        (uint32_t)ncap, dpos, dty, dh, (const double *)e->force_out, forces ? dfor : (double *)nullptr,
                        (uint32_t)e->max_particles);
@@ -887,11 +1018,11 @@ int rdy_update_neighbor_list(rdy_engine *e) {
     int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
     if (rc) return rc;
     const int c = e->cur, n = 1 - c;
-    hipLaunchKernelGGL(k_fused<MODE_REBIN>, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->grid(n),
-                       e->reps, e->T, e->topo(), e->cap, e->radius, 0.0, (const double *)nullptr, (double *)nullptr,
-                       (uint32_t)e->max_particles, e->err);
-    launch_wl_rebuild(e, n);
-    launch_clear(e, c);
+    hipLaunchKernelGGL(k_brick<MODE_REBIN>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
+                       e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
+                       e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, e->err);
+    launch_clear_bricks(e, c, n);
+    e->wl_valid[n] = false;
     e->cur = n;
     return check_errors(e);
 }
@@ -936,6 +1067,7 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
         dn = e->d_noise;
     }
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    ensure_wl(e);
     hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 1, e->e_bonded);
     hipLaunchKernelGGL(k_integrate, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
                        e->cap, dt, dn, (const double *)e->force_out, (uint32_t)e->max_particles, e->err);
@@ -951,9 +1083,9 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
     if (e->n_rx == 0) return RDY_OK;
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
     const int c = e->cur;
-    hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
-                       e->radius, handler, 0, e->events, e->ev_count, (uint32_t *)nullptr, 0u, 0u,
-                       (uint32_t)e->max_particles, e->err);
+    hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(c),
+                       (uint32_t *)nullptr, e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler,
+                       e->events, e->ev_count, e->max_states, e->max_halo, e->err);
     hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
                        e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles, e->err);
     return check_errors(e);
@@ -1061,6 +1193,7 @@ int rdy_neighbor_pairs(rdy_engine *e, uint32_t r, double cutoff, uint64_t cap, u
     uint32_t *cnt = (uint32_t *)e->scratch;
     uint32_t *out = cnt + 16;
     HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    ensure_wl(e);
     hipLaunchKernelGGL(k_pairs, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->cap,
                        e->radius, r, cutoff * cutoff, out, (uint32_t)cap, cnt, (uint32_t)e->max_particles);
     uint32_t hc = 0;
@@ -1084,6 +1217,7 @@ int rdy_reaction_candidates(rdy_engine *e, uint32_t r, uint64_t cap, uint32_t *r
     uint32_t *cnt = (uint32_t *)e->scratch;
     uint32_t *out = cnt + 16;
     HIP_TRY(hipMemsetAsync(cnt, 0, 4, e->stream));
+    ensure_wl(e);
     hipLaunchKernelGGL(k_search, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
                        e->cap, e->radius, 0, 1, e->events, cnt, out, (uint32_t)cap, r, (uint32_t)e->max_particles,
                        e->err);
@@ -1106,6 +1240,7 @@ int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_reps) {
     if (rc) return rc;
     unsigned long long *d = (unsigned long long *)e->scratch;
     HIP_TRY(hipMemsetAsync(d, 0, words * 8, e->stream));
+    ensure_wl(e);
     hipLaunchKernelGGL(k_count_types, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->n_types, d,
                        n_reps, (uint32_t)e->max_particles);
     HIP_TRY(hipMemcpyAsync(counts, d, words * 8, hipMemcpyDeviceToHost, e->stream));

@@ -668,22 +668,12 @@ __device__ __forceinline__ unsigned long long event_key(int handler, const uint3
     return (unsigned long long)__double_as_longlong(k);
 }
 
-__global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, int radius,
-                                                 int handler, int candidates_only, Event *events,
-                                                 uint32_t *ev_count, uint32_t *cand_rows, uint32_t cand_cap,
-                                                 uint32_t only_replica, uint32_t max_wl, uint32_t *err) {
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = b * blockDim.x + threadIdx.x;
-    uint32_t n = *cur.wl_count;
-    n = n < max_wl ? n : max_wl;
-    if (i >= n) return;
-    const uint32_t s = cur.wl[i];
-    const State me = cur.bins[s];
-    if (me.tw & TW_DEAD) return;
+// candidates of the particle in slot s (searching from the smaller (type, handle) of every pair)
+__device__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap, int radius,
+                           int handler, int candidates_only, Event *events, uint32_t *ev_count, uint32_t *cand_rows,
+                           uint32_t cand_cap, uint32_t s, const State &me, uint32_t *err) {
     const uint32_t r = tw_replica(me.tw);
-    if (candidates_only && r != only_replica) return;
     const ReplicaDev &R = reps[r];
-    if (!candidates_only && !R.active) return;
     const uint32_t t = tw_type(me.tw);
     const uint32_t step = R.step;
     const uint64_t seed = R.seed;
@@ -760,6 +750,24 @@ __global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tabl
             else emit(rx, s2, s, o.h, me.h, event_key(handler, x4, X.rate), 2u);
         }
     });
+}
+
+__global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, int radius,
+                                                 int handler, int candidates_only, Event *events,
+                                                 uint32_t *ev_count, uint32_t *cand_rows, uint32_t cand_cap,
+                                                 uint32_t only_replica, uint32_t max_wl, uint32_t *err) {
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = b * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i >= n) return;
+    const uint32_t s = cur.wl[i];
+    const State me = cur.bins[s];
+    if (me.tw & TW_DEAD) return;
+    const uint32_t r = tw_replica(me.tw);
+    if (candidates_only && r != only_replica) return;
+    if (!candidates_only && !reps[r].active) return;
+    search_one(cur, reps, T, cap, radius, handler, candidates_only, events, ev_count, cand_rows, cand_cap, s, me, err);
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -1018,6 +1026,18 @@ __global__ void k_kill_replica(Grid cur, uint32_t r, uint32_t max_wl) {
     uint32_t s = cur.wl[i];
     State st = cur.bins[s];
     if (tw_replica(st.tw) == r) cur.bins[s].tw = st.tw | TW_DEAD;
+}
+
+// frees the cells of a removed replica in one buffer: counters to zero, overflow records detached
+__global__ void k_release_cells(Grid g, uint32_t base, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g.counts[base + i] = 0u;
+    uint32_t no = *g.ovf_count;
+    no = no < g.ovf_cap ? no : g.ovf_cap;
+    for (uint32_t o = i; o < no; o += gridDim.x * blockDim.x) {
+        uint32_t c = g.ovf_cell[o];
+        if (c >= base && c < base + n) g.ovf_cell[o] = 0xFFFFFFFFu;
+    }
 }
 
 // -------------------------------------------------------------------------------------------------

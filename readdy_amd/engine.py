@@ -30,7 +30,7 @@ def bin_capacity_for(particles_per_cell):
 class Engine:
     def __init__(self, n_types, diffusion, kbt=1.0, potentials=(), reactions=(), topology_flavor=None,
                  nl_cutoff=0.0, cell_radius=1, bin_capacity=0, max_replicas=1, max_cells=0, max_particles=0,
-                 events_per_replica=0, block_size=0):
+                 events_per_replica=0, block_size=0, brick=(0, 0, 0), lds_states=0):
         self._lib = L.load()
         self.n_types = int(n_types)
         self._diff = np.ascontiguousarray(diffusion, dtype=np.float64)
@@ -79,6 +79,9 @@ class Engine:
         cfg.max_particles = max_particles
         cfg.events_per_replica = events_per_replica
         cfg.block_size = block_size
+        for k in range(3):
+            cfg.brick[k] = int(brick[k])
+        cfg.lds_states = lds_states
         h = ctypes.c_void_p()
         L.check(self._lib.rdy_engine_create(ctypes.byref(ctx), ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
