@@ -68,8 +68,9 @@ struct PotDev {
 };
 
 struct PairIdx {   // potentials of an ordered type pair: pots[start .. start + count)
-    double maxcut2;
-    uint32_t start, count;
+    double maxcut2;    // largest cutoff^2 of the list (exact FP64 gate)
+    float maxcut2_f;   // the same in FP32 (prefilter threshold before the rounding margin)
+    uint16_t start, count;
 };
 
 struct RxDev {
@@ -79,7 +80,8 @@ struct RxDev {
 };
 
 struct RxPairIdx {  // order-2 reactions of an ordered type pair: rxlist[start .. start + count)
-    uint32_t start, count;
+    float maxr2_f;     // largest educt distance^2 of the list (FP32 prefilter threshold before margin)
+    uint16_t start, count;
 };
 
 // ---- Philox4x32-10, from Salmon, Moraes, Dror & Shaw, SC'11 ------------------------------------

@@ -243,14 +243,16 @@ int upload_tables(rdy_engine *e, double dt) {
     std::vector<PotDev> flat;
     std::vector<PairIdx> pidx((size_t)nt * nt);
     for (size_t k = 0; k < lists.size(); ++k) {
-        pidx[k].start = (uint32_t)flat.size();
-        pidx[k].count = (uint32_t)lists[k].size();
+        pidx[k].start = (uint16_t)flat.size();
+        pidx[k].count = (uint16_t)lists[k].size();
         pidx[k].maxcut2 = 0.0;
         for (uint32_t id : lists[k]) {
             flat.push_back(pots[id]);
             pidx[k].maxcut2 = std::max(pidx[k].maxcut2, pots[id].cut2);
         }
+        pidx[k].maxcut2_f = (float)pidx[k].maxcut2;
     }
+    if (flat.size() > 65535) return set_err(RDY_ERR_INVALID, "too many pair potentials");
     // reactions
     e->rx_h.assign(e->n_rx, RxDev{});
     std::vector<std::vector<uint32_t>> o1(nt), o2((size_t)nt * nt);
@@ -290,8 +292,11 @@ int upload_tables(rdy_engine *e, double dt) {
     o1idx[nt] = (uint32_t)o1list.size();
     bool has_o2 = false;
     for (size_t k = 0; k < o2.size(); ++k) {
-        o2idx[k].start = (uint32_t)o2list.size();
-        o2idx[k].count = (uint32_t)o2[k].size();
+        o2idx[k].start = (uint16_t)o2list.size();
+        o2idx[k].count = (uint16_t)o2[k].size();
+        double m2 = 0.0;
+        for (uint32_t rx : o2[k]) m2 = std::max(m2, e->rx_h[rx].educt_dist2);
+        o2idx[k].maxr2_f = (float)m2;
         has_o2 = has_o2 || !o2[k].empty();
         o2list.insert(o2list.end(), o2[k].begin(), o2[k].end());
     }
@@ -325,6 +330,7 @@ int upload_tables(rdy_engine *e, double dt) {
     e->T.n_types = nt;
     e->T.n_rx = e->n_rx;
     e->T.has_o2 = has_o2 ? 1u : 0u;
+    e->T.n_pots = (uint32_t)flat.size();
     e->T.kbt = e->kbt;
     e->tables_dt = dt;
     e->version++;
@@ -1162,6 +1168,7 @@ int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
             if ((rc = check_errors(e))) return rc;
     }
     if (n_steps & 1) launch_step(e, handler, dt);
+    e->wl_valid[0] = e->wl_valid[1] = false;  // replays bypass launch_step's bookkeeping
     e->prof = prof;
     launch_forces(e, e->force_out);
     return check_errors(e);

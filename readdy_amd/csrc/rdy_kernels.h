@@ -30,7 +30,7 @@ struct Tables {
     uint32_t n_types;
     uint32_t n_rx;
     uint32_t has_o2;         // any order-2 reaction
-    uint32_t pad;
+    uint32_t n_pots;         // entries of pots
     double kbt;
 };
 

@@ -418,20 +418,21 @@ def test_fused_loop_equals_separate_actions():
 
 
 def test_graph_replay_equals_eager():
+    """Graph replays, with host reads in between, track an eager engine step for step."""
     e1, pos, types = _c0_engine()
     r1 = e1.add_replica([20.0] * 3, seed=8, max_particles=8000)
     e1.set_particles(r1, pos, types)
-    e1.run(1, 0.01, 41)
     e2, _, _ = _c0_engine()
     r2 = e2.add_replica([20.0] * 3, seed=8, max_particles=8000)
     e2.set_particles(r2, pos, types)
-    for chunk in (10, 20, 11):  # several replays with host work in between
+    for chunk in (10, 20, 11, 40):
+        e1.run(1, 0.01, chunk)
         e2.run(1, 0.01, chunk, graph=True)
-        e2.get_particles(r2)
-    a = e1.get_particles(r1)
-    b = e2.get_particles(r2)
-    for x, y in zip(a, b):
-        assert np.array_equal(x, y)
+        a = e1.get_particles(r1)
+        b = e2.get_particles(r2)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        assert np.array_equal(e1.count_types(1), e2.count_types(1))
 
 
 def test_replica_alone_equals_replica_in_batch():
