@@ -35,10 +35,9 @@ constexpr int BRICK_THREADS = 256;
 constexpr uint32_t LDS_TABLE = 256;  // type pairs whose tables are staged in LDS (n_types <= 16)
 constexpr uint32_t LDS_POTS = 64;
 
-struct BrickSmem {  // carved out of dynamic LDS
+struct BrickSmem {  // carved out of dynamic LDS (24 bytes per staged record)
     float4 *P;         // [max_states] local-frame FP32 position, w = tw bits
-    State *S;          // [max_states]
-    uint32_t *slot;    // [max_states]
+    uint32_t *slot;    // [max_states] bin slot (the FP64 record is read from the bins when needed)
     uint16_t *hc_of;   // [max_states]
     uint16_t *inner;   // [max_states]
     uint32_t *hcnt;    // [max_halo]
@@ -51,8 +50,6 @@ __device__ __forceinline__ BrickSmem carve(char *base, uint32_t max_states, uint
     char *p = base;
     m.P = (float4 *)p;
     p += sizeof(float4) * max_states;
-    m.S = (State *)p;
-    p += sizeof(State) * max_states;
     m.slot = (uint32_t *)p;
     p += 4 * max_states;
     m.hcnt = (uint32_t *)p;
@@ -68,7 +65,7 @@ __device__ __forceinline__ BrickSmem carve(char *base, uint32_t max_states, uint
 }
 
 size_t brick_smem_bytes(uint32_t max_states, uint32_t max_halo) {
-    return (sizeof(float4) + sizeof(State) + 4 + 2 + 2) * (size_t)max_states + 12 * (size_t)max_halo + 4 + 64;
+    return (sizeof(float4) + 4 + 2 + 2) * (size_t)max_states + 12 * (size_t)max_halo + 4 + 64;
 }
 
 struct Frame {       // local frame of a brick
@@ -152,7 +149,6 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
     // 3. copy the records, one per thread, cell found by binary search over hstart
     auto put = [&](uint32_t j, uint32_t hc, uint32_t s) {
         const State st = g.bins[s];
-        m.S[j] = st;
         m.slot[j] = s;
         const uint32_t hz = hc % H[2], hy = (hc / H[2]) % H[1], hx = hc / (H[2] * H[1]);
         const uint32_t hh[3] = {hx, hy, hz};
@@ -179,7 +175,8 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
         }
         const uint32_t hc = lo, k = j - m.hstart[hc];
         m.hc_of[j] = (uint16_t)hc;
-        if (k < cap) put(j, hc, m.hcell[hc] * cap + k);
+        const uint32_t cell = m.hcell[hc];
+        if (k < cap && cell != 0xFFFFFFFFu) put(j, hc, cell * cap + k);
     }
     // overflow records of crowded cells (rare): scan the overflow list
     for (uint32_t hc = tid; hc < nh; hc += BRICK_THREADS) {
@@ -195,7 +192,6 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
                 ++found;
             }
         for (; found < n - cap; ++found) {  // counters ahead of a truncated overflow list: dead records
-            m.S[base + found].tw = TW_DEAD;
             m.P[base + found] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
             m.slot[base + found] = 0xFFFFFFFFu;
         }
@@ -346,8 +342,9 @@ __device__ __forceinline__ bool load_pair_tables(PairLds &t, const Tables &T, fl
 }
 
 // accumulates the pair forces/energy on staged record j from staged neighbours
-__device__ __forceinline__ void staged_forces(const BrickSmem &m, const BrickDev &B, const RepGeom &geo,
-                                              const Frame &F, const uint32_t *H, int radius, const Tables &T,
+__device__ __forceinline__ void staged_forces(const BrickSmem &m, const State *bins, const BrickDev &B,
+                                              const RepGeom &geo, const Frame &F, const uint32_t *H, int radius,
+                                              const Tables &T,
                                               const PairLds &pt, bool lds_tables, uint32_t j, const State &me,
                                               long long &fx, long long &fy, long long &fz, long long &e) {
     const uint32_t t = tw_type(me.tw);
@@ -381,7 +378,7 @@ __device__ __forceinline__ void staged_forces(const BrickSmem &m, const BrickDev
             }
             if (d2p < thr) {
                 if (lds_tables) pi = pt.pidx[k];
-                const State o = m.S[q];
+                const State o = bins[m.slot[q]];
                 const double ddx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
                 const double ddy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
                 const double ddz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
@@ -563,10 +560,11 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
     if (total != 0xFFFFFFFFu) {
         for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) {
             const uint32_t j = m.inner[ii];
-            const State me = m.S[j];
-            if (me.tw & TW_DEAD) continue;
+            if (__float_as_uint(m.P[j].w) & TW_DEAD) continue;
+            const State me = cur.bins[m.slot[j]];
             long long fx = 0, fy = 0, fz = 0, e = 0;
-            if (!(me.tw & TW_OUTSIDE)) staged_forces(m, B, geo, F, H, radius, T, ptab, lds_tables, j, me, fx, fy, fz, e);
+            if (!(me.tw & TW_OUTSIDE))
+                staged_forces(m, cur.bins, B, geo, F, H, radius, T, ptab, lds_tables, j, me, fx, fy, fz, e);
             e_blk += e;
             finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, m.slot[j], me, true, fx, fy, fz,
                                   err);
@@ -636,10 +634,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     const uint64_t seed = R.seed;
     for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) {
         const uint32_t j = m.inner[ii];
-        const State me = m.S[j];
-        if (me.tw & TW_DEAD) continue;
-        const uint32_t t = tw_type(me.tw);
+        if (__float_as_uint(m.P[j].w) & TW_DEAD) continue;
         const uint32_t s = m.slot[j];
+        const State me = cur.bins[s];
+        const uint32_t t = tw_type(me.tw);
         auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key) {
             uint32_t p = atomicAdd(&ev_count[r], 1u);
             if (p < R.ev_cap) {
@@ -675,7 +673,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
             it.q++;
             const uint32_t tw2 = __float_as_uint(pq.w);
             const uint32_t t2 = tw_type(tw2);
-            if (q != j && !(tw2 & (TW_DEAD | TW_OUTSIDE)) && (t2 > t || (t2 == t && m.S[q].h > me.h))) {
+            if (q != j && !(tw2 & (TW_DEAD | TW_OUTSIDE)) && (t2 > t || (t2 == t && cur.bins[m.slot[q]].h > me.h))) {
                 const uint32_t kk = t * nt + t2;
                 float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
                 if (F.whole) {
@@ -696,7 +694,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
                                    : -1.0f;
                 }
                 if (d2p < thr) {
-                    const State o = m.S[q];
+                    const State o = cur.bins[m.slot[q]];
                     // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
                     const double ddx = min_image(me.x - o.x, geo.box[0], geo.pbc[0]);
                     const double ddy = min_image(me.y - o.y, geo.box[1], geo.pbc[1]);

@@ -162,6 +162,8 @@ int ensure_scratch(rdy_engine *e, size_t bytes) {
 }
 
 int check_errors(rdy_engine *e) {
+    hipError_t le = hipGetLastError();  // launch-time errors (e.g. a launch configuration the device rejects)
+    if (le != hipSuccess) return set_err(RDY_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(le));
     uint32_t h[N_ERR];
     HIP_TRY(hipMemcpyAsync(h, e->err, sizeof h, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -487,9 +489,20 @@ int configure_bricks(rdy_engine *e, double lam) {
     int v = 0;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device) == hipSuccess && v > 0)
         e->lds_limit = (size_t)v;
-    const size_t budget = e->lds_limit - 1024;  // static LDS of the brick kernels
+    // static LDS of the brick kernels comes out of the same per-workgroup budget
+    size_t static_lds = 0;
+    const void *kernels[] = {(const void *)k_brick<MODE_STEP>, (const void *)k_brick<MODE_FORCES>,
+                             (const void *)k_brick<MODE_REBIN>, (const void *)k_brick_search};
+    for (const void *k : kernels) {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, k) == hipSuccess) static_lds = std::max(static_lds, (size_t)fa.sharedSizeBytes);
+    }
+    if (static_lds == 0) static_lds = 16384;
+    if (e->lds_limit < static_lds + 8192) return set_err(RDY_ERR_INVALID, "not enough LDS per workgroup");
+    const size_t budget = e->lds_limit - static_lds - 256;
+    // staged records: the expected halo occupancy with headroom; rarer fuller halos take the slow path
     uint32_t want = e->max_states;
-    if (want == 0) want = (uint32_t)std::max(1024.0, 2.0 * std::max(lam, 0.25) * e->max_halo + 256.0);
+    if (want == 0) want = (uint32_t)std::max(768.0, 1.5 * std::max(lam, 0.25) * e->max_halo + 128.0);
     want = std::min<uint32_t>(want, 60000);
     while (want > 256 && brick_smem_bytes(want, e->max_halo) > budget) want -= 128;
     if (brick_smem_bytes(want, e->max_halo) > budget)
