@@ -2,14 +2,19 @@
 //
 // A brick is a box of cells of one replica.  Its workgroup copies the brick plus a halo of
 // `radius` cells (periodic images wrapped, non-periodic ones clipped) from the cell bins into LDS
-// once, packs every cell's members contiguously (z-fastest, so the three z-neighbours of a cell
-// form one contiguous run), and evaluates the (2r+1)^3 stencil of every particle of the brick
-// against LDS.  Each staged record carries its position twice: FP64 (the exact reference gate
-// d2 < rc^2 and the min-image displacement that feeds the pair math) and FP32 in the brick's
-// local frame with periodic images already shifted, used as a conservative prefilter: a candidate
-// is dropped in FP32 only when it is farther than the cutoff by more than the FP32 rounding bound,
-// so the exact pair set is unchanged.  A brick whose halo holds more particles than the LDS budget
-// takes a slow path that reads neighbours straight from the bins.
+// once, packs every cell's members contiguously (z-fastest, so the z-neighbours of a cell form one
+// contiguous run), and walks the (2r+1)^3 stencil of every particle of the brick against LDS.
+// Each staged record carries an FP32 position in the brick's local frame with periodic images
+// already shifted, used as a conservative prefilter: a candidate is dropped in FP32 only when it
+// is farther than the cutoff by more than the FP32 rounding bound, so the exact pair set is
+// unchanged.  The exact work (FP64 min-image displacement, the reference gate d2 < rc^2, the pair
+// math, the reaction draws) runs on the survivors only, and not per lane: each wave streams its
+// survivors through a ring in LDS and evaluates them 64 at a time with every lane busy, because a
+// per-lane evaluation would make the whole wave execute the exact branch whenever any one lane
+// had a survivor (almost always).  Forces are gathered per particle as 64-bit fixed-point sums
+// (LDS atomics; integer addition is order-free, so the result stays deterministic).
+// A brick whose halo holds more particles than the LDS budget takes a slow path that reads
+// neighbours straight from the bins.
 //
 // Per axis a brick is either in halo mode (extent e, halo r on both sides; the host guarantees
 // e + 2r <= dims on periodic axes, so halo cells are distinct and the local displacement is the
@@ -23,17 +28,49 @@
 namespace rdy {
 
 struct BrickDev {
-    int32_t o[3];  // origin cell
-    int32_t e[3];  // extent in cells
-    int32_t h[3];  // halo width (0 in whole-axis mode)
-    uint32_t r;    // replica
+    int32_t o[3];    // origin cell
+    int32_t e[3];    // extent in cells
+    int32_t h[3];    // halo width (0 in whole-axis mode)
+    uint32_t r;      // replica
     uint32_t whole;  // bit a: axis a spans the grid
-    uint32_t pad;
+    // ceil(2^32 / d) for d = H1 * H2, H2 and e1 (H = e + 2h; 0 when d == 1): exact n / d as a high
+    // multiply for n, d < 2^16
+    uint32_t m_hh, m_h2, m_e1;
+    uint32_t pad[2];
 };
 
 constexpr int BRICK_THREADS = 256;
-constexpr uint32_t LDS_TABLE = 256;  // type pairs whose tables are staged in LDS (n_types <= 16)
-constexpr uint32_t LDS_POTS = 64;
+constexpr int BRICK_WAVES = BRICK_THREADS / 64;
+constexpr uint32_t LDS_TABLE = 64;  // type pairs whose tables are staged in LDS (n_types <= 8)
+constexpr uint32_t LDS_POTS = 32;
+constexpr uint32_t RING = 128;      // per-wave ring of prefilter survivors (power of two, >= 127)
+
+__host__ __device__ inline uint32_t div_magic(uint32_t d) {
+    return d > 1 ? (uint32_t)((0x100000000ull + d - 1) / d) : 0u;
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m) { return m ? __umulhi(n, m) : n; }
+
+struct Halo {  // halo box of a brick, z fastest: hc = (hx * H1 + hy) * H2 + hz
+    uint32_t H0, H1, H2, HH, nh, m_hh, m_h2;
+    __device__ __forceinline__ void decode(uint32_t hc, uint32_t &hx, uint32_t &hy, uint32_t &hz) const {
+        hx = fdiv(hc, m_hh);
+        const uint32_t rem = hc - hx * HH;
+        hy = fdiv(rem, m_h2);
+        hz = rem - hy * H2;
+    }
+};
+
+__device__ __forceinline__ Halo make_halo(const BrickDev &B) {
+    Halo s;
+    s.H0 = (uint32_t)(B.e[0] + 2 * B.h[0]);
+    s.H1 = (uint32_t)(B.e[1] + 2 * B.h[1]);
+    s.H2 = (uint32_t)(B.e[2] + 2 * B.h[2]);
+    s.HH = s.H1 * s.H2;
+    s.nh = s.H0 * s.HH;
+    s.m_hh = B.m_hh;
+    s.m_h2 = B.m_h2;
+    return s;
+}
 
 struct BrickSmem {  // carved out of dynamic LDS (24 bytes per staged record)
     float4 *P;         // [max_states] local-frame FP32 position, w = tw bits
@@ -91,27 +128,30 @@ __device__ __forceinline__ Frame make_frame(const BrickDev &B, const RepGeom &ge
     return f;
 }
 
+// prefilter threshold: cutoff^2 widened by the FP32 rounding bound (negative: nothing to find)
+__device__ __forceinline__ float prefilter_thr(float cut2_f, bool any, float margin) {
+    return any ? cut2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(cut2_f) * margin + margin * margin : -1.0f;
+}
+
 // stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
-// they do not fit (the caller then takes the slow path).  Also returns the number of inner records.
+// they do not fit (the caller then takes the slow path).  Also returns the number of inner records
+// (the brick's own cells), listed in m.inner.
 __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom &geo, const Frame &F, uint32_t cap,
-                                BrickSmem &m, uint32_t max_states, uint32_t *n_inner_out, uint32_t *H) {
-    __shared__ uint32_t s_total, s_inner, s_wave[BRICK_THREADS / 64];
+                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t *n_inner_out) {
+    __shared__ uint32_t s_total;
     const uint32_t tid = threadIdx.x;
-    H[0] = B.e[0] + 2 * B.h[0];
-    H[1] = B.e[1] + 2 * B.h[1];
-    H[2] = B.e[2] + 2 * B.h[2];
-    const uint32_t nh = H[0] * H[1] * H[2];
+    const uint32_t nh = Hs.nh;
     // 1. cell counters of the halo box
     for (uint32_t hc = tid; hc < nh; hc += BRICK_THREADS) {
-        const uint32_t hz = hc % H[2], hy = (hc / H[2]) % H[1], hx = hc / (H[2] * H[1]);
-        const uint32_t hh[3] = {hx, hy, hz};
+        uint32_t hh[3];
+        Hs.decode(hc, hh[0], hh[1], hh[2]);
         int32_t c[3];
         bool inside = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             int32_t x = B.o[a] - B.h[a] + (int32_t)hh[a];
             if (x < 0 || x >= geo.dims[a]) {
-                if (geo.pbc[a]) x = (x % geo.dims[a] + geo.dims[a]) % geo.dims[a];
+                if (geo.pbc[a]) x = x < 0 ? x + geo.dims[a] : x - geo.dims[a];  // |overhang| <= r < dims
                 else inside = false;
             }
             c[a] = x;
@@ -144,176 +184,257 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
         s_total = total;
     }
     __syncthreads();
-    if (s_total > max_states) return 0xFFFFFFFFu;
     total = s_total;
-    // 3. copy the records, one per thread, cell found by binary search over hstart
-    auto put = [&](uint32_t j, uint32_t hc, uint32_t s) {
-        const State st = g.bins[s];
-        m.slot[j] = s;
-        const uint32_t hz = hc % H[2], hy = (hc / H[2]) % H[1], hx = hc / (H[2] * H[1]);
-        const uint32_t hh[3] = {hx, hy, hz};
-        const double p[3] = {st.x, st.y, st.z};
-        float l[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            double v = p[a] - F.org[a];
-            if (!(F.whole & (1u << a))) {  // unwrap halo images into the local frame
-                const int32_t u = B.o[a] - B.h[a] + (int32_t)hh[a];
-                if (u < 0) v -= geo.box[a];
-                else if (u >= geo.dims[a]) v += geo.box[a];
-            }
-            l[a] = (float)v;
-        }
-        m.P[j] = make_float4(l[0], l[1], l[2], __uint_as_float(st.tw));
-    };
-    for (uint32_t j = tid; j < total; j += BRICK_THREADS) {
-        uint32_t lo = 0, hi = nh;  // largest hc with hstart[hc] <= j
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (m.hstart[mid] <= j) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t hc = lo, k = j - m.hstart[hc];
-        m.hc_of[j] = (uint16_t)hc;
-        const uint32_t cell = m.hcell[hc];
-        if (k < cap && cell != 0xFFFFFFFFu) put(j, hc, cell * cap + k);
-    }
-    // overflow records of crowded cells (rare): scan the overflow list
+    if (total > max_states) return 0xFFFFFFFFu;
+    // 3. copy the records cell by cell
     for (uint32_t hc = tid; hc < nh; hc += BRICK_THREADS) {
         const uint32_t n = m.hcnt[hc];
-        if (n <= cap) continue;
+        if (n == 0) continue;
         const uint32_t cell = m.hcell[hc];
-        uint32_t no = *g.ovf_count;
-        no = no < g.ovf_cap ? no : g.ovf_cap;
-        uint32_t found = 0, base = m.hstart[hc] + cap;
-        for (uint32_t o = 0; o < no && found < n - cap; ++o)
-            if (g.ovf_cell[o] == cell) {
-                put(base + found, hc, g.nbin + o);
-                ++found;
+        const uint32_t j0 = m.hstart[hc];
+        uint32_t hh[3];
+        Hs.decode(hc, hh[0], hh[1], hh[2]);
+        double shift[3];  // unwraps halo images into the local frame
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            shift[a] = 0.0;
+            if (!(F.whole & (1u << a))) {
+                const int32_t u = B.o[a] - B.h[a] + (int32_t)hh[a];
+                if (u < 0) shift[a] = -geo.box[a];
+                else if (u >= geo.dims[a]) shift[a] = geo.box[a];
             }
-        for (; found < n - cap; ++found) {  // counters ahead of a truncated overflow list: dead records
-            m.P[base + found] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
-            m.slot[base + found] = 0xFFFFFFFFu;
+        }
+        auto put = [&](uint32_t j, uint32_t s) {
+            const State st = g.bins[s];
+            m.slot[j] = s;
+            m.hc_of[j] = (uint16_t)hc;
+            const float lx = (float)((st.x - F.org[0]) + shift[0]);
+            const float ly = (float)((st.y - F.org[1]) + shift[1]);
+            const float lz = (float)((st.z - F.org[2]) + shift[2]);
+            m.P[j] = make_float4(lx, ly, lz, __uint_as_float(st.tw));
+        };
+        const uint32_t nb = n < cap ? n : cap;
+        for (uint32_t k = 0; k < nb; ++k) put(j0 + k, cell * cap + k);
+        if (n > cap) {  // overflow records of a crowded cell (rare): scan the overflow list
+            uint32_t no = *g.ovf_count;
+            no = no < g.ovf_cap ? no : g.ovf_cap;
+            uint32_t found = 0;
+            const uint32_t base = j0 + cap;
+            for (uint32_t o = 0; o < no && found < n - cap; ++o)
+                if (g.ovf_cell[o] == cell) {
+                    put(base + found, g.nbin + o);
+                    ++found;
+                }
+            for (; found < n - cap; ++found) {  // counters ahead of a truncated overflow list: dead records
+                m.P[base + found] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
+                m.slot[base + found] = 0xFFFFFFFFu;
+                m.hc_of[base + found] = (uint16_t)hc;
+            }
         }
     }
-    if (tid == 0) s_inner = 0;
+    // 4. the brick's own records: one contiguous range per inner (x, y) row
+    const uint32_t e1 = (uint32_t)B.e[1], e2 = (uint32_t)B.e[2];
+    const uint32_t nrow = (uint32_t)B.e[0] * e1;
+    uint32_t n_inner = 0;
+    for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
+        const uint32_t r = r0 + tid;
+        uint32_t a = 0, c = 0;
+        if (r < nrow) {
+            const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
+            const uint32_t row = (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
+            a = m.hstart[row];
+            c = m.hstart[row + e2] - a;
+        }
+        uint32_t tot;
+        const uint32_t o = n_inner + block_exclusive_scan(c, &tot);
+        for (uint32_t k = 0; k < c; ++k) m.inner[o + k] = (uint16_t)(a + k);
+        n_inner += tot;
+    }
     __syncthreads();
-    // 4. compact the indices of the brick's own records
-    for (uint32_t base = 0; base < total; base += BRICK_THREADS) {
-        const uint32_t j = base + tid;
-        bool in = false;
-        if (j < total) {
-            const uint32_t hc = m.hc_of[j];
-            const uint32_t hz = hc % H[2], hy = (hc / H[2]) % H[1], hx = hc / (H[2] * H[1]);
-            in = hx >= (uint32_t)B.h[0] && hx < (uint32_t)(B.h[0] + B.e[0]) && hy >= (uint32_t)B.h[1] &&
-                 hy < (uint32_t)(B.h[1] + B.e[1]) && hz >= (uint32_t)B.h[2] && hz < (uint32_t)(B.h[2] + B.e[2]);
-        }
-        const unsigned long long mask = __ballot(in);
-        const uint32_t w = tid / 64;
-        if (__lane_id() == 0) s_wave[w] = (uint32_t)__popcll(mask);
-        __syncthreads();
-        uint32_t wbase = s_inner;
-        for (uint32_t q = 0; q < w; ++q) wbase += s_wave[q];
-        if (in) m.inner[wbase + (uint32_t)__popcll(mask & lanemask_lt())] = (uint16_t)j;
-        __syncthreads();
-        if (tid == 0)
-            for (uint32_t q = 0; q < BRICK_THREADS / 64; ++q) s_inner += s_wave[q];
-        __syncthreads();
-    }
-    *n_inner_out = s_inner;
+    *n_inner_out = n_inner;
     return total;
 }
 
-// stencil around halo cell hc as contiguous runs of staged records: (x, y) rows times z-ranges.
-// Halo-mode axes use offsets -r..r; whole-axis mode wraps inside the grid (all cells if dims <= 2r).
-struct StencilIter {
-    int nx, ny, nzr;
-    int bx, by, bz;   // base coordinates
-    int mx, my, mz;   // axis modes: 0 halo, 1 whole (all cells), 2 whole (wrapped offsets)
-    int Dx, Dy, Dz;
-    uint32_t H1, H2;
-    int ia, ib, iz;
-    uint32_t q, qe;
+// ---- stencil walkers: visit the staged records of the (2r+1)^3 stencil of halo cell hc as
+// contiguous runs [q, qe) -------------------------------------------------------------------------
+struct WalkCtx {
+    Halo Hs;
+    uint32_t whole;
+    int32_t dims[3];
+    int radius;
+    uint32_t n;         // 2r + 1
+    uint32_t back;      // hc offset of the stencil's first cell (all-halo mode)
+    uint32_t row_step;  // hc step from the last z-run of an x-slab to the first of the next
 };
 
-__device__ __forceinline__ void axis_mode(bool whole, int D, int radius, int hcoord, int &n, int &base, int &mode) {
-    if (!whole) {
-        n = 2 * radius + 1;
-        base = hcoord - radius;
-        mode = 0;
-    } else if (D <= 2 * radius) {
-        n = D;
-        base = 0;
-        mode = 1;
-    } else {
-        n = 2 * radius + 1;
-        base = hcoord - radius;
-        mode = 2;
+__device__ __forceinline__ WalkCtx make_walk(const BrickDev &B, const RepGeom &geo, const Halo &Hs, int radius) {
+    WalkCtx c;
+    c.Hs = Hs;
+    c.whole = B.whole;
+    c.dims[0] = geo.dims[0];
+    c.dims[1] = geo.dims[1];
+    c.dims[2] = geo.dims[2];
+    c.radius = radius;
+    c.n = 2u * (uint32_t)radius + 1u;
+    c.back = (uint32_t)radius * (Hs.HH + Hs.H2 + 1u);
+    c.row_step = Hs.HH - (c.n - 1u) * Hs.H2;
+    return c;
+}
+
+template <bool ALL_HALO>
+struct Walk;
+
+// every axis in halo mode: (2r+1)^2 z-runs of 2r+1 cells each, all inside the halo box
+template <>
+struct Walk<true> {
+    uint32_t q, qe, cur, ia, ib;
+    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const uint32_t *hs) {
+        cur = hc - c.back;
+        ia = ib = 0;
+        q = hs[cur];
+        qe = hs[cur + c.n];
+        return q < qe || advance(c, hs);
     }
-}
-
-__device__ __forceinline__ int axis_elem(int k, int base, int mode, int D) {
-    return mode == 2 ? ((base + k) % D + D) % D : base + k;
-}
-
-__device__ __forceinline__ void stencil_range(StencilIter &it, const uint32_t *hstart) {
-    const int x = axis_elem(it.ia, it.bx, it.mx, it.Dx);
-    const int y = axis_elem(it.ib, it.by, it.my, it.Dy);
-    const uint32_t row = ((uint32_t)x * it.H1 + (uint32_t)y) * it.H2;
-    int z0, z1;
-    if (it.mz == 2) {
-        z0 = axis_elem(it.iz, it.bz, 2, it.Dz);
-        z1 = z0 + 1;
-    } else if (it.mz == 1) {
-        z0 = 0;
-        z1 = it.Dz;
-    } else {
-        z0 = it.bz;
-        z1 = z0 + it.Dz;  // halo mode: Dz holds the run length 2r+1
-    }
-    it.q = hstart[row + (uint32_t)z0];
-    it.qe = hstart[row + (uint32_t)z1];
-}
-
-__device__ __forceinline__ StencilIter stencil_begin(const BrickDev &B, const RepGeom &geo, const uint32_t *H,
-                                                     int radius, uint32_t hc, const uint32_t *hstart) {
-    StencilIter it;
-    const uint32_t hz = hc % H[2], hy = (hc / H[2]) % H[1], hx = hc / (H[2] * H[1]);
-    axis_mode(B.whole & 1u, geo.dims[0], radius, (int)hx, it.nx, it.bx, it.mx);
-    axis_mode(B.whole & 2u, geo.dims[1], radius, (int)hy, it.ny, it.by, it.my);
-    int nz;
-    axis_mode(B.whole & 4u, geo.dims[2], radius, (int)hz, nz, it.bz, it.mz);
-    it.Dx = geo.dims[0];
-    it.Dy = geo.dims[1];
-    if (it.mz == 0) {  // one contiguous z-run of 2r+1 cells
-        it.nzr = 1;
-        it.Dz = 2 * radius + 1;
-    } else if (it.mz == 1) {
-        it.nzr = 1;
-        it.Dz = geo.dims[2];
-    } else {
-        it.nzr = nz;
-        it.Dz = geo.dims[2];
-    }
-    it.H1 = H[1];
-    it.H2 = H[2];
-    it.ia = it.ib = it.iz = 0;
-    stencil_range(it, hstart);
-    return it;
-}
-
-// advances to the next non-empty run; false when the stencil is exhausted
-__device__ __forceinline__ bool stencil_next(StencilIter &it, const uint32_t *hstart) {
-    while (true) {
-        if (++it.iz >= it.nzr) {
-            it.iz = 0;
-            if (++it.ib >= it.ny) {
-                it.ib = 0;
-                if (++it.ia >= it.nx) return false;
+    __device__ __forceinline__ bool advance(const WalkCtx &c, const uint32_t *hs) {
+        do {
+            if (++ib == c.n) {
+                ib = 0;
+                if (++ia == c.n) return false;
+                cur += c.row_step;
+            } else {
+                cur += c.Hs.H2;
             }
+            q = hs[cur];
+            qe = hs[cur + c.n];
+        } while (q >= qe);
+        return true;
+    }
+};
+
+// general case: per axis halo mode (offsets -r..r), whole-axis mode with all cells (dims <= 2r) or
+// whole-axis mode with wrapped offsets; z as one run unless wrapped
+template <>
+struct Walk<false> {
+    int nx, ny, nzr;
+    int bx, by, bz;  // base coordinates
+    int mx, my, mz;  // axis modes: 0 halo, 1 whole (all cells), 2 whole (wrapped offsets)
+    int Dz;
+    int ia, ib, iz;
+    uint32_t q, qe;
+
+    __device__ __forceinline__ static void axis_mode(bool whole, int D, int radius, int hcoord, int &n, int &base,
+                                                     int &mode) {
+        if (!whole) {
+            n = 2 * radius + 1;
+            base = hcoord - radius;
+            mode = 0;
+        } else if (D <= 2 * radius) {
+            n = D;
+            base = 0;
+            mode = 1;
+        } else {
+            n = 2 * radius + 1;
+            base = hcoord - radius;
+            mode = 2;
         }
-        stencil_range(it, hstart);
-        if (it.q < it.qe) return true;
+    }
+    __device__ __forceinline__ static int elem(int k, int base, int mode, int D) {
+        if (mode != 2) return base + k;
+        int v = base + k;  // |overhang| <= r < D
+        return v < 0 ? v + D : (v >= D ? v - D : v);
+    }
+    __device__ __forceinline__ void range(const WalkCtx &c, const uint32_t *hs) {
+        const int x = elem(ia, bx, mx, c.dims[0]);
+        const int y = elem(ib, by, my, c.dims[1]);
+        const uint32_t row = ((uint32_t)x * c.Hs.H1 + (uint32_t)y) * c.Hs.H2;
+        int z0, z1;
+        if (mz == 2) {
+            z0 = elem(iz, bz, 2, c.dims[2]);
+            z1 = z0 + 1;
+        } else {
+            z0 = bz;
+            z1 = bz + Dz;
+        }
+        q = hs[row + (uint32_t)z0];
+        qe = hs[row + (uint32_t)z1];
+    }
+    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const uint32_t *hs) {
+        uint32_t hx, hy, hz;
+        c.Hs.decode(hc, hx, hy, hz);
+        axis_mode(c.whole & 1u, c.dims[0], c.radius, (int)hx, nx, bx, mx);
+        axis_mode(c.whole & 2u, c.dims[1], c.radius, (int)hy, ny, by, my);
+        int nz;
+        axis_mode(c.whole & 4u, c.dims[2], c.radius, (int)hz, nz, bz, mz);
+        if (mz == 0) {  // one contiguous z-run of 2r+1 cells
+            nzr = 1;
+            Dz = 2 * c.radius + 1;
+        } else if (mz == 1) {  // the whole axis
+            nzr = 1;
+            Dz = c.dims[2];
+        } else {
+            nzr = nz;
+            Dz = 1;
+        }
+        ia = ib = iz = 0;
+        range(c, hs);
+        return q < qe || advance(c, hs);
+    }
+    __device__ __forceinline__ bool advance(const WalkCtx &c, const uint32_t *hs) {
+        do {
+            if (++iz >= nzr) {
+                iz = 0;
+                if (++ib >= ny) {
+                    ib = 0;
+                    if (++ia >= nx) return false;
+                }
+            }
+            range(c, hs);
+        } while (q >= qe);
+        return true;
+    }
+};
+
+// makes LDS writes of this wave visible to its other lanes (and orders the compiler's accesses)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Streams candidate pairs through the wave's ring: every active lane walks the stencil of its own
+// staged record (halo cell hc) and offers each staged record q to keep(q); survivors are appended
+// as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the wave.
+// Must be called by all 64 lanes of a wave (wave-uniform control flow around it).
+template <bool ALL_HALO, class Keep, class Eval>
+__device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
+                                              uint32_t hc, Keep &&keep, Eval &&eval) {
+    const uint32_t lane = __lane_id();
+    Walk<ALL_HALO> w;
+    bool more = false;
+    if (active) more = w.begin(c, hc, hs);
+    uint32_t head = 0, tail = 0;  // wave-uniform
+    while (__ballot(more)) {
+        bool pass = false;
+        uint32_t q = 0;
+        if (more) {
+            q = w.q++;
+            pass = keep(q);
+            if (w.q >= w.qe) more = w.advance(c, hs);
+        }
+        const unsigned long long mask = __ballot(pass);
+        if (pass) ring[(tail + (uint32_t)__popcll(mask & lanemask_lt())) & (RING - 1)] = (lane << 16) | q;
+        tail += (uint32_t)__popcll(mask);
+        if (tail - head >= 64) {
+            wave_lds_sync();
+            eval(ring[(head + lane) & (RING - 1)]);
+            head += 64;
+            wave_lds_sync();
+        }
+    }
+    if (tail != head) {
+        wave_lds_sync();
+        if (lane < tail - head) eval(ring[(head + lane) & (RING - 1)]);
+        wave_lds_sync();
     }
 }
 
@@ -331,9 +452,7 @@ __device__ __forceinline__ bool load_pair_tables(PairLds &t, const Tables &T, fl
         for (uint32_t k = threadIdx.x; k < np; k += BRICK_THREADS) {
             const PairIdx pi = T.pidx[k];
             t.pidx[k] = pi;
-            // prefilter threshold: cutoff^2 widened by the FP32 rounding bound (negative: no potentials)
-            t.thr[k] = pi.count ? pi.maxcut2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(pi.maxcut2_f) * margin + margin * margin
-                                : -1.0f;
+            t.thr[k] = prefilter_thr(pi.maxcut2_f, pi.count != 0, margin);
         }
         for (uint32_t k = threadIdx.x; k < T.n_pots && k < LDS_POTS; k += BRICK_THREADS) t.pots[k] = T.pots[k];
     }
@@ -341,68 +460,14 @@ __device__ __forceinline__ bool load_pair_tables(PairLds &t, const Tables &T, fl
     return fits;
 }
 
-// accumulates the pair forces/energy on staged record j from staged neighbours
-__device__ __forceinline__ void staged_forces(const BrickSmem &m, const State *bins, const BrickDev &B,
-                                              const RepGeom &geo, const Frame &F, const uint32_t *H, int radius,
-                                              const Tables &T,
-                                              const PairLds &pt, bool lds_tables, uint32_t j, const State &me,
-                                              long long &fx, long long &fy, long long &fz, long long &e) {
-    const uint32_t t = tw_type(me.tw);
-    const uint32_t nt = T.n_types;
-    const float4 pj = m.P[j];
-    StencilIter it = stencil_begin(B, geo, H, radius, m.hc_of[j], m.hstart);
-    if (!(it.q < it.qe) && !stencil_next(it, m.hstart)) return;
-    while (true) {
-        const uint32_t q = it.q;
-        const float4 pq = m.P[q];
-        it.q++;
-        const uint32_t tw2 = __float_as_uint(pq.w);
-        if (q != j && !(tw2 & (TW_DEAD | TW_OUTSIDE))) {
-            const uint32_t k = t * nt + tw_type(tw2);
-            float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
-            if (F.whole) {
-                if (F.whole & 1u) dx = dx > .5f * F.L[0] ? dx - F.L[0] : (dx <= -.5f * F.L[0] ? dx + F.L[0] : dx);
-                if (F.whole & 2u) dy = dy > .5f * F.L[1] ? dy - F.L[1] : (dy <= -.5f * F.L[1] ? dy + F.L[1] : dy);
-                if (F.whole & 4u) dz = dz > .5f * F.L[2] ? dz - F.L[2] : (dz <= -.5f * F.L[2] ? dz + F.L[2] : dz);
-            }
-            const float d2p = dx * dx + dy * dy + dz * dz;
-            float thr;
-            PairIdx pi;
-            if (lds_tables) {
-                thr = pt.thr[k];
-            } else {
-                pi = T.pidx[k];
-                thr = pi.count ? pi.maxcut2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(pi.maxcut2_f) * F.margin +
-                                     F.margin * F.margin
-                               : -1.0f;
-            }
-            if (d2p < thr) {
-                if (lds_tables) pi = pt.pidx[k];
-                const State o = bins[m.slot[q]];
-                const double ddx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
-                const double ddy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
-                const double ddz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
-                const double d2 = dist2(ddx, ddy, ddz);
-                if (d2 < pi.maxcut2) {
-                    float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
-                    const float xf = (float)ddx, yf = (float)ddy, zf = (float)ddz, d2f = (float)d2;
-                    for (uint32_t p = 0; p < pi.count; ++p) {
-                        const uint32_t pidx = pi.start + p;
-                        const PotDev pot = (lds_tables && pidx < LDS_POTS) ? pt.pots[pidx] : T.pots[pidx];
-                        if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
-                    }
-                    fx += to_fixed(pfx, 4294967296.0f);
-                    fy += to_fixed(pfy, 4294967296.0f);
-                    fz += to_fixed(pfz, 4294967296.0f);
-                    e += to_fixed(pe, 16777216.0f);
-                }
-            }
-        }
-        if (!(it.q < it.qe) && !stencil_next(it, m.hstart)) break;
-    }
+// FP32 min image of a local-frame displacement on whole-axis axes (halo axes need none)
+__device__ __forceinline__ void frame_min_image(const Frame &F, float &dx, float &dy, float &dz) {
+    if (F.whole & 1u) dx = dx > .5f * F.L[0] ? dx - F.L[0] : (dx <= -.5f * F.L[0] ? dx + F.L[0] : dx);
+    if (F.whole & 2u) dy = dy > .5f * F.L[1] ? dy - F.L[1] : (dy <= -.5f * F.L[1] ? dy + F.L[1] : dy);
+    if (F.whole & 4u) dz = dz > .5f * F.L[2] ? dz - F.L[2] : (dz <= -.5f * F.L[2] ? dz + F.L[2] : dz);
 }
 
-// the same sum reading neighbours from the bins (slow path for bricks that do not fit in LDS)
+// the pair sum reading neighbours from the bins (slow path for bricks that do not fit in LDS)
 __device__ __forceinline__ void global_forces(const Grid &cur, const RepGeom &geo, uint32_t cap, int radius,
                                               const Tables &T, uint32_t s, const State &me, long long &fx,
                                               long long &fy, long long &fz, long long &e) {
@@ -529,6 +594,13 @@ __device__ __forceinline__ void for_each_own_record(const Grid &cur, const Brick
     }
 }
 
+struct ForceWave {  // per-wave LDS of k_brick
+    uint32_t ring[RING];
+    unsigned long long acc[3][64];  // fixed-point force of the wave's 64 particles
+    double x[64], y[64], z[64];     // their FP64 positions
+    uint32_t tw[64];
+};
+
 // forces (+ Euler-Maruyama + re-bin) of every particle of one brick; MODE as in k_fused
 template <int MODE>
 __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, ReplicaDev *reps, Tables T, Topo topo,
@@ -537,6 +609,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                                                          uint32_t max_halo, uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ PairLds ptab;
+    __shared__ ForceWave wave_lds[BRICK_WAVES];
     const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
     const ReplicaDev &R = reps[B.r];
     if (!R.in_use) return;
@@ -553,21 +626,88 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
     }
     const Frame F = make_frame(B, geo);
     const bool lds_tables = load_pair_tables(ptab, T, F.margin);
-    BrickSmem m = carve(smem, max_states, max_halo);
-    uint32_t H[3], n_inner = 0;
-    const uint32_t total = stage_brick(cur, B, geo, F, cap, m, max_states, &n_inner, H);
+    const BrickSmem m = carve(smem, max_states, max_halo);
+    const Halo Hs = make_halo(B);
+    uint32_t n_inner = 0;
+    const uint32_t total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
     long long e_blk = 0;
     if (total != 0xFFFFFFFFu) {
-        for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) {
-            const uint32_t j = m.inner[ii];
-            if (__float_as_uint(m.P[j].w) & TW_DEAD) continue;
-            const State me = cur.bins[m.slot[j]];
-            long long fx = 0, fy = 0, fz = 0, e = 0;
-            if (!(me.tw & TW_OUTSIDE))
-                staged_forces(m, cur.bins, B, geo, F, H, radius, T, ptab, lds_tables, j, me, fx, fy, fz, e);
-            e_blk += e;
-            finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, m.slot[j], me, true, fx, fy, fz,
-                                  err);
+        const uint32_t nt = T.n_types;
+        const WalkCtx wc = make_walk(B, geo, Hs, radius);
+        ForceWave &W = wave_lds[threadIdx.x / 64];
+        const uint32_t lane = __lane_id();
+        // the exact pair force of a survivor (entry = owner lane << 16 | staged neighbour)
+        auto eval = [&](uint32_t ent) {
+            const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
+            const State o = cur.bins[m.slot[q]];
+            const uint32_t k = tw_type(W.tw[ol]) * nt + tw_type(o.tw);
+            const PairIdx pi = lds_tables ? ptab.pidx[k] : T.pidx[k];
+            // CPUCalculateForces.cpp:226-259: x_ij = shortestDifference(x_i, x_j), gate d2 < rc^2
+            const double ddx = min_image(o.x - W.x[ol], geo.box[0], geo.pbc[0]);
+            const double ddy = min_image(o.y - W.y[ol], geo.box[1], geo.pbc[1]);
+            const double ddz = min_image(o.z - W.z[ol], geo.box[2], geo.pbc[2]);
+            const double d2 = dist2(ddx, ddy, ddz);
+            if (!(d2 < pi.maxcut2)) return;
+            float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
+            const float xf = (float)ddx, yf = (float)ddy, zf = (float)ddz, d2f = (float)d2;
+            for (uint32_t p = 0; p < pi.count; ++p) {
+                const uint32_t pidx = pi.start + p;
+                const PotDev pot = (lds_tables && pidx < LDS_POTS) ? ptab.pots[pidx] : T.pots[pidx];
+                if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
+            }
+            atomicAdd(&W.acc[0][ol], (unsigned long long)to_fixed(pfx, 4294967296.0f));
+            atomicAdd(&W.acc[1][ol], (unsigned long long)to_fixed(pfy, 4294967296.0f));
+            atomicAdd(&W.acc[2][ol], (unsigned long long)to_fixed(pfz, 4294967296.0f));
+            e_blk += to_fixed(pe, 16777216.0f);
+        };
+        for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
+            const uint32_t ii = base + lane;
+            bool valid = ii < n_inner;
+            uint32_t j = 0;
+            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) {
+                j = m.inner[ii];
+                pj = m.P[j];
+                valid = !(__float_as_uint(pj.w) & TW_DEAD);
+            }
+            State me{};
+            uint32_t sj = 0;
+            if (valid) {
+                sj = m.slot[j];
+                me = cur.bins[sj];
+            }
+            W.x[lane] = me.x;
+            W.y[lane] = me.y;
+            W.z[lane] = me.z;
+            W.tw[lane] = me.tw;
+            W.acc[0][lane] = 0ull;
+            W.acc[1][lane] = 0ull;
+            W.acc[2][lane] = 0ull;
+            wave_lds_sync();
+            const bool active = valid && !(me.tw & TW_OUTSIDE);
+            const uint32_t row = tw_type(me.tw) * nt;
+            auto keep = [&](uint32_t q) -> bool {
+                if (q == j) return false;
+                const float4 pq = m.P[q];
+                const uint32_t tw2 = __float_as_uint(pq.w);
+                if (tw2 & (TW_DEAD | TW_OUTSIDE)) return false;
+                float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
+                if (F.whole) frame_min_image(F, dx, dy, dz);
+                const float d2p = dx * dx + dy * dy + dz * dz;
+                const uint32_t k = row + tw_type(tw2);
+                const float thr =
+                    lds_tables ? ptab.thr[k] : prefilter_thr(T.pidx[k].maxcut2_f, T.pidx[k].count != 0, F.margin);
+                return d2p < thr;
+            };
+            const uint32_t hc = m.hc_of[j];
+            if (wc.whole == 0) pair_pipeline<true>(wc, m.hstart, W.ring, active, hc, keep, eval);
+            else pair_pipeline<false>(wc, m.hstart, W.ring, active, hc, keep, eval);
+            wave_lds_sync();
+            if (valid)
+                finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, sj, me, true,
+                                      (long long)W.acc[0][lane], (long long)W.acc[1][lane],
+                                      (long long)W.acc[2][lane], err);
+            wave_lds_sync();
         }
     } else {
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
@@ -581,6 +721,12 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
     block_add_energy(reps, B.r, e_blk);
 }
 
+struct SearchWave {  // per-wave LDS of k_brick_search
+    uint32_t ring[RING];
+    double x[64], y[64], z[64];
+    uint32_t tw[64], h[64], s[64];
+};
+
 // reaction candidate search of one brick on the current buffer (see search_one for the rules),
 // fused with clearing the previous buffer's counters of the brick's cells
 __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32_t *old_counts, ReplicaDev *reps,
@@ -591,6 +737,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     extern __shared__ __align__(16) char smem[];
     __shared__ float s_thr[LDS_TABLE];
     __shared__ RxPairIdx s_o2[LDS_TABLE];
+    __shared__ SearchWave wave_lds[BRICK_WAVES];
     const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
     const ReplicaDev &R = reps[B.r];
     const RepGeom geo = load_geom(R);
@@ -613,16 +760,15 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
         for (uint32_t k = threadIdx.x; k < nt * nt; k += BRICK_THREADS) {
             const RxPairIdx pi = T.o2idx[k];
             s_o2[k] = pi;
-            s_thr[k] = pi.count ? pi.maxr2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(pi.maxr2_f) * F.margin +
-                                      F.margin * F.margin
-                                : -1.0f;
+            s_thr[k] = prefilter_thr(pi.maxr2_f, pi.count != 0, F.margin);
         }
     }
     __syncthreads();
-    BrickSmem m = carve(smem, max_states, max_halo);
-    uint32_t H[3], n_inner = 0;
-    const uint32_t total = (T.has_o2 || T.o1idx[nt] > 0) ? stage_brick(cur, B, geo, F, cap, m, max_states, &n_inner, H)
-                                                          : 0u;
+    const BrickSmem m = carve(smem, max_states, max_halo);
+    const Halo Hs = make_halo(B);
+    uint32_t n_inner = 0;
+    const uint32_t total =
+        (T.has_o2 || T.o1idx[nt] > 0) ? stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner) : 0u;
     if (total == 0xFFFFFFFFu) {  // slow path: search from the brick's own cells straight from the bins
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
             if (!(me.tw & TW_DEAD))
@@ -632,90 +778,110 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     }
     const uint32_t r = B.r, step = R.step;
     const uint64_t seed = R.seed;
-    for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) {
-        const uint32_t j = m.inner[ii];
-        if (__float_as_uint(m.P[j].w) & TW_DEAD) continue;
-        const uint32_t s = m.slot[j];
-        const State me = cur.bins[s];
-        const uint32_t t = tw_type(me.tw);
-        auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key) {
-            uint32_t p = atomicAdd(&ev_count[r], 1u);
-            if (p < R.ev_cap) {
-                Event ev;
-                ev.key = key;
-                ev.s1 = s1;
-                ev.s2 = s2;
-                ev.h1 = h1;
-                ev.h2 = h2;
-                ev.rx = rx;
-                ev.pad = 0;
-                events[R.ev_base + p] = ev;
-            } else {
-                atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
-            }
-        };
-        for (uint32_t k = T.o1idx[t]; k < T.o1idx[t + 1]; ++k) {
-            const uint32_t rx = T.o1list[k];
+    auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key) {
+        uint32_t p = atomicAdd(&ev_count[r], 1u);
+        if (p < R.ev_cap) {
+            Event ev;
+            ev.key = key;
+            ev.s1 = s1;
+            ev.s2 = s2;
+            ev.h1 = h1;
+            ev.h2 = h2;
+            ev.rx = rx;
+            ev.pad = 0;
+            events[R.ev_base + p] = ev;
+        } else {
+            atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
+        }
+    };
+    const WalkCtx wc = make_walk(B, geo, Hs, radius);
+    SearchWave &W = wave_lds[threadIdx.x / 64];
+    const uint32_t lane = __lane_id();
+    // the exact test and draws of a surviving pair (entry = owner lane << 16 | staged neighbour)
+    auto eval = [&](uint32_t ent) {
+        const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
+        const uint32_t s2 = m.slot[q];
+        const State o = cur.bins[s2];
+        const uint32_t mh = W.h[ol], t = tw_type(W.tw[ol]);
+        const uint32_t kk = t * nt + tw_type(o.tw);
+        const RxPairIdx pi = lds_tables ? s_o2[kk] : T.o2idx[kk];
+        // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
+        const double ddx = min_image(W.x[ol] - o.x, geo.box[0], geo.pbc[0]);
+        const double ddy = min_image(W.y[ol] - o.y, geo.box[1], geo.pbc[1]);
+        const double ddz = min_image(W.z[ol] - o.z, geo.box[2], geo.pbc[2]);
+        const double d2 = dist2(ddx, ddy, ddz);
+        const uint32_t lo = mh < o.h ? mh : o.h, hi = mh < o.h ? o.h : mh;
+        for (uint32_t p = 0; p < pi.count; ++p) {
+            const uint32_t rx = T.o2list[pi.start + p];
             const RxDev X = T.rx[rx];
-            if (!(X.rate > 0)) continue;
+            if (!(X.rate > 0 && d2 < X.educt_dist2)) continue;
             uint32_t x4[4];
-            philox(me.h, step, 1u | (rx << 8), 0u, seed, x4);
+            philox(lo, step, 2u | (rx << 8), hi, seed, x4);
             if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
-            emit(rx, s, s, me.h, me.h, event_key(handler, x4, X.rate));
+            // entry1: the educt of type educts[0]; equal educt types: the lower handle
+            const bool me_first = (X.educts[0] != X.educts[1]) ? (t == X.educts[0]) : (mh < o.h);
+            if (me_first) emit(rx, W.s[ol], s2, mh, o.h, event_key(handler, x4, X.rate));
+            else emit(rx, s2, W.s[ol], o.h, mh, event_key(handler, x4, X.rate));
         }
-        if (!T.has_o2 || (me.tw & TW_OUTSIDE)) continue;
-        const float4 pj = m.P[j];
-        StencilIter it = stencil_begin(B, geo, H, radius, m.hc_of[j], m.hstart);
-        if (!(it.q < it.qe) && !stencil_next(it, m.hstart)) continue;
-        while (true) {
-            const uint32_t q = it.q;
-            const float4 pq = m.P[q];
-            it.q++;
-            const uint32_t tw2 = __float_as_uint(pq.w);
-            const uint32_t t2 = tw_type(tw2);
-            if (q != j && !(tw2 & (TW_DEAD | TW_OUTSIDE)) && (t2 > t || (t2 == t && cur.bins[m.slot[q]].h > me.h))) {
-                const uint32_t kk = t * nt + t2;
-                float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
-                if (F.whole) {
-                    if (F.whole & 1u) dx = dx > .5f * F.L[0] ? dx - F.L[0] : (dx <= -.5f * F.L[0] ? dx + F.L[0] : dx);
-                    if (F.whole & 2u) dy = dy > .5f * F.L[1] ? dy - F.L[1] : (dy <= -.5f * F.L[1] ? dy + F.L[1] : dy);
-                    if (F.whole & 4u) dz = dz > .5f * F.L[2] ? dz - F.L[2] : (dz <= -.5f * F.L[2] ? dz + F.L[2] : dz);
-                }
-                const float d2p = dx * dx + dy * dy + dz * dz;
-                RxPairIdx pi;
-                float thr;
-                if (lds_tables) {
-                    thr = s_thr[kk];
-                    pi = s_o2[kk];
-                } else {
-                    pi = T.o2idx[kk];
-                    thr = pi.count ? pi.maxr2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(pi.maxr2_f) * F.margin +
-                                         F.margin * F.margin
-                                   : -1.0f;
-                }
-                if (d2p < thr) {
-                    const State o = cur.bins[m.slot[q]];
-                    // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
-                    const double ddx = min_image(me.x - o.x, geo.box[0], geo.pbc[0]);
-                    const double ddy = min_image(me.y - o.y, geo.box[1], geo.pbc[1]);
-                    const double ddz = min_image(me.z - o.z, geo.box[2], geo.pbc[2]);
-                    const double d2 = dist2(ddx, ddy, ddz);
-                    const uint32_t lo = me.h < o.h ? me.h : o.h, hi = me.h < o.h ? o.h : me.h;
-                    for (uint32_t p = 0; p < pi.count; ++p) {
-                        const uint32_t rx = T.o2list[pi.start + p];
-                        const RxDev X = T.rx[rx];
-                        if (!(X.rate > 0 && d2 < X.educt_dist2)) continue;
-                        uint32_t x4[4];
-                        philox(lo, step, 2u | (rx << 8), hi, seed, x4);
-                        if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
-                        bool me_first = (X.educts[0] != X.educts[1]) ? (t == X.educts[0]) : (me.h < o.h);
-                        if (me_first) emit(rx, s, m.slot[q], me.h, o.h, event_key(handler, x4, X.rate));
-                        else emit(rx, m.slot[q], s, o.h, me.h, event_key(handler, x4, X.rate));
-                    }
-                }
+    };
+    for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
+        const uint32_t ii = base + lane;
+        bool valid = ii < n_inner;
+        uint32_t j = 0;
+        float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            j = m.inner[ii];
+            pj = m.P[j];
+            valid = !(__float_as_uint(pj.w) & TW_DEAD);
+        }
+        State me{};
+        uint32_t sj = 0;
+        if (valid) {
+            sj = m.slot[j];
+            me = cur.bins[sj];
+        }
+        const uint32_t t = tw_type(me.tw);
+        if (valid) {  // order 1 (CPUUncontrolledApproximation.cpp:81-95)
+            for (uint32_t k = T.o1idx[t]; k < T.o1idx[t + 1]; ++k) {
+                const uint32_t rx = T.o1list[k];
+                const RxDev X = T.rx[rx];
+                if (!(X.rate > 0)) continue;
+                uint32_t x4[4];
+                philox(me.h, step, 1u | (rx << 8), 0u, seed, x4);
+                if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
+                emit(rx, sj, sj, me.h, me.h, event_key(handler, x4, X.rate));
             }
-            if (!(it.q < it.qe) && !stencil_next(it, m.hstart)) break;
         }
+        if (!T.has_o2) continue;  // block-uniform
+        W.x[lane] = me.x;
+        W.y[lane] = me.y;
+        W.z[lane] = me.z;
+        W.tw[lane] = me.tw;
+        W.h[lane] = me.h;
+        W.s[lane] = sj;
+        wave_lds_sync();
+        const bool active = valid && !(me.tw & TW_OUTSIDE);
+        const uint32_t row = t * nt;
+        // each unordered pair once: the lower type finds it; equal types: the lower bin slot
+        auto keep = [&](uint32_t q) -> bool {
+            if (q == j) return false;
+            const float4 pq = m.P[q];
+            const uint32_t tw2 = __float_as_uint(pq.w);
+            if (tw2 & (TW_DEAD | TW_OUTSIDE)) return false;
+            const uint32_t t2 = tw_type(tw2);
+            if (t2 < t || (t2 == t && m.slot[q] < sj)) return false;
+            float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
+            if (F.whole) frame_min_image(F, dx, dy, dz);
+            const float d2p = dx * dx + dy * dy + dz * dz;
+            const uint32_t kk = row + t2;
+            const float thr =
+                lds_tables ? s_thr[kk] : prefilter_thr(T.o2idx[kk].maxr2_f, T.o2idx[kk].count != 0, F.margin);
+            return d2p < thr;
+        };
+        const uint32_t hc = m.hc_of[j];
+        if (wc.whole == 0) pair_pipeline<true>(wc, m.hstart, W.ring, active, hc, keep, eval);
+        else pair_pipeline<false>(wc, m.hstart, W.ring, active, hc, keep, eval);
+        wave_lds_sync();
     }
 }
 

@@ -449,6 +449,10 @@ void make_bricks(rdy_engine *e, uint32_t r, std::vector<BrickDev> &out) {
                 for (int a = 0; a < 3; ++a) b.h[a] = (int32_t)halo[a];
                 b.r = r;
                 b.whole = whole;
+                const uint32_t H1 = (uint32_t)(b.e[1] + 2 * b.h[1]), H2 = (uint32_t)(b.e[2] + 2 * b.h[2]);
+                b.m_hh = div_magic(H1 * H2);
+                b.m_h2 = div_magic(H2);
+                b.m_e1 = div_magic((uint32_t)b.e[1]);
                 out.push_back(b);
             }
 }

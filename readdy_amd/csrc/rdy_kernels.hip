@@ -657,7 +657,7 @@ __global__ void k_dihedrals(Grid cur, ReplicaDev *reps, Topo topo, const BondTer
 
 // -------------------------------------------------------------------------------------------------
 // reaction candidate search (CPUUncontrolledApproximation.cpp:70-133, ReactionUtils.h:83-144)
-// Each unordered pair is found once, by the particle with the smaller (type, handle).  Acceptance
+// Each unordered pair is found once, by the particle with the smaller (type, bin slot).  Acceptance
 // u < 1 - exp(-rate dt) is drawn per candidate from Philox keyed by the pair's handles, so the
 // candidate set and the draws do not depend on slots, thread order or batching.
 // -------------------------------------------------------------------------------------------------
@@ -668,7 +668,7 @@ __device__ __forceinline__ unsigned long long event_key(int handler, const uint3
     return (unsigned long long)__double_as_longlong(k);
 }
 
-// candidates of the particle in slot s (searching from the smaller (type, handle) of every pair)
+// candidates of the particle in slot s (searching from the smaller (type, bin slot) of every pair)
 __device__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap, int radius,
                            int handler, int candidates_only, Event *events, uint32_t *ev_count, uint32_t *cand_rows,
                            uint32_t cand_cap, uint32_t s, const State &me, uint32_t *err) {
@@ -726,7 +726,7 @@ __device__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, u
         const State o = cur.bins[s2];
         if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
         const uint32_t t2 = tw_type(o.tw);
-        if (t2 < t || (t2 == t && o.h < me.h)) return;
+        if (t2 < t || (t2 == t && s2 < s)) return;
         const RxPairIdx pi = orow[t2];
         if (pi.count == 0) return;
         // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
