@@ -25,6 +25,8 @@
 
 #include "rdy_kernels.hip"
 
+#include <type_traits>
+
 namespace rdy {
 
 struct BrickDev {
@@ -41,8 +43,6 @@ struct BrickDev {
 
 constexpr int BRICK_THREADS = 256;
 constexpr int BRICK_WAVES = BRICK_THREADS / 64;
-constexpr uint32_t LDS_TABLE = 64;  // type pairs whose tables are staged in LDS (n_types <= 8)
-constexpr uint32_t LDS_POTS = 32;
 constexpr uint32_t RING = 128;      // per-wave ring of prefilter survivors (power of two, >= 127)
 
 __host__ __device__ inline uint32_t div_magic(uint32_t d) {
@@ -136,8 +136,11 @@ __device__ __forceinline__ float prefilter_thr(float cut2_f, bool any, float mar
 // stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
 // they do not fit (the caller then takes the slow path).  Also returns the number of inner records
 // (the brick's own cells), listed in m.inner.
+// The inner list holds the records whose type bit is set in `first` ahead of the others (stable
+// otherwise), so that waves see particles of like stencil reach.
 __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom &geo, const Frame &F, uint32_t cap,
-                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t *n_inner_out) {
+                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t first,
+                                uint32_t *n_inner_out) {
     __shared__ uint32_t s_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t nh = Hs.nh;
@@ -232,23 +235,40 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
             }
         }
     }
-    // 4. the brick's own records: one contiguous range per inner (x, y) row
+    __syncthreads();
+    // 4. the brick's own records: one contiguous range per inner (x, y) row, partitioned by class.
+    // Pass 0 counts the first class, pass 1 writes (a packed scan carries both class counts).
     const uint32_t e1 = (uint32_t)B.e[1], e2 = (uint32_t)B.e[2];
     const uint32_t nrow = (uint32_t)B.e[0] * e1;
-    uint32_t n_inner = 0;
-    for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
-        const uint32_t r = r0 + tid;
-        uint32_t a = 0, c = 0;
-        if (r < nrow) {
-            const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
-            const uint32_t row = (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
-            a = m.hstart[row];
-            c = m.hstart[row + e2] - a;
+    uint32_t n_first = 0, n_inner = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t base1 = 0, base2 = 0;
+        for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
+            const uint32_t r = r0 + tid;
+            uint32_t a = 0, c = 0, c1 = 0;
+            if (r < nrow) {
+                const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
+                const uint32_t row =
+                    (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
+                a = m.hstart[row];
+                c = m.hstart[row + e2] - a;
+                if (first)
+                    for (uint32_t k = 0; k < c; ++k) c1 += (first >> tw_type(__float_as_uint(m.P[a + k].w))) & 1u;
+            }
+            uint32_t tot;
+            const uint32_t o = block_exclusive_scan(c1 | ((c - c1) << 16), &tot);
+            if (pass == 1) {
+                uint32_t o1 = base1 + (o & 0xFFFFu), o2 = n_first + base2 + (o >> 16);
+                for (uint32_t k = 0; k < c; ++k) {
+                    const bool f = first && ((first >> tw_type(__float_as_uint(m.P[a + k].w))) & 1u);
+                    m.inner[f ? o1++ : o2++] = (uint16_t)(a + k);
+                }
+            }
+            base1 += tot & 0xFFFFu;
+            base2 += tot >> 16;
         }
-        uint32_t tot;
-        const uint32_t o = n_inner + block_exclusive_scan(c, &tot);
-        for (uint32_t k = 0; k < c; ++k) m.inner[o + k] = (uint16_t)(a + k);
-        n_inner += tot;
+        n_first = base1;
+        n_inner = base1 + base2;
     }
     __syncthreads();
     *n_inner_out = n_inner;
@@ -263,8 +283,8 @@ struct WalkCtx {
     int32_t dims[3];
     int radius;
     uint32_t n;         // 2r + 1
-    uint32_t back;      // hc offset of the stencil's first cell (all-halo mode)
-    uint32_t row_step;  // hc step from the last z-run of an x-slab to the first of the next
+    int32_t h[3];       // halo widths
+    float cs[3], inv_cs[3];
 };
 
 __device__ __forceinline__ WalkCtx make_walk(const BrickDev &B, const RepGeom &geo, const Halo &Hs, int radius) {
@@ -276,36 +296,59 @@ __device__ __forceinline__ WalkCtx make_walk(const BrickDev &B, const RepGeom &g
     c.dims[2] = geo.dims[2];
     c.radius = radius;
     c.n = 2u * (uint32_t)radius + 1u;
-    c.back = (uint32_t)radius * (Hs.HH + Hs.H2 + 1u);
-    c.row_step = Hs.HH - (c.n - 1u) * Hs.H2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c.h[a] = B.h[a];
+        c.cs[a] = (float)geo.cs[a];
+        c.inv_cs[a] = (float)(1.0 / geo.cs[a]);
+    }
     return c;
 }
 
 template <bool ALL_HALO>
 struct Walk;
 
-// every axis in halo mode: (2r+1)^2 z-runs of 2r+1 cells each, all inside the halo box
+// every axis in halo mode: the cells within `reach` of the particle at local position p in halo
+// cell hc, per axis offsets lo..hi inside -r..r (cells that overlap [p - reach, p + reach]; reach
+// carries the FP32 rounding slack, so the range is a superset of the exact one), walked as
+// (hi0 - lo0 + 1) * (hi1 - lo1 + 1) z-runs of hi2 - lo2 + 1 cells
 template <>
 struct Walk<true> {
-    uint32_t q, qe, cur, ia, ib;
-    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const uint32_t *hs) {
-        cur = hc - c.back;
+    uint32_t q, qe, cur, ia, ib, nx, ny, nz, row_step;
+    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const float4 &p, float reach,
+                                          const uint32_t *hs) {
+        uint32_t hh[3];
+        c.Hs.decode(hc, hh[0], hh[1], hh[2]);
+        const float pp[3] = {p.x, p.y, p.z};
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float u = pp[a] - (float)((int)hh[a] - c.h[a]) * c.cs[a];  // position inside its cell
+            // clamped in FP32 first (a huge reach or a NaN position selects the full stencil)
+            lo[a] = min(0, (int)fmaxf(floorf((u - reach) * c.inv_cs[a]), -(float)c.radius));
+            hi[a] = max(0, (int)fminf(floorf((u + reach) * c.inv_cs[a]), (float)c.radius));
+        }
+        nx = (uint32_t)(hi[0] - lo[0] + 1);
+        ny = (uint32_t)(hi[1] - lo[1] + 1);
+        nz = (uint32_t)(hi[2] - lo[2] + 1);
+        row_step = c.Hs.HH - (ny - 1u) * c.Hs.H2;
+        cur = (uint32_t)((int)hc + lo[0] * (int)c.Hs.HH + lo[1] * (int)c.Hs.H2 + lo[2]);
         ia = ib = 0;
         q = hs[cur];
-        qe = hs[cur + c.n];
+        qe = hs[cur + nz];
         return q < qe || advance(c, hs);
     }
     __device__ __forceinline__ bool advance(const WalkCtx &c, const uint32_t *hs) {
         do {
-            if (++ib == c.n) {
+            if (++ib == ny) {
                 ib = 0;
-                if (++ia == c.n) return false;
-                cur += c.row_step;
+                if (++ia == nx) return false;
+                cur += row_step;
             } else {
                 cur += c.Hs.H2;
             }
             q = hs[cur];
-            qe = hs[cur + c.n];
+            qe = hs[cur + nz];
         } while (q >= qe);
         return true;
     }
@@ -358,7 +401,8 @@ struct Walk<false> {
         q = hs[row + (uint32_t)z0];
         qe = hs[row + (uint32_t)z1];
     }
-    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const uint32_t *hs) {
+    __device__ __forceinline__ bool begin(const WalkCtx &c, uint32_t hc, const float4 &, float,
+                                          const uint32_t *hs) {
         uint32_t hx, hy, hz;
         c.Hs.decode(hc, hx, hy, hz);
         axis_mode(c.whole & 1u, c.dims[0], c.radius, (int)hx, nx, bx, mx);
@@ -402,16 +446,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Streams candidate pairs through the wave's ring: every active lane walks the stencil of its own
-// staged record (halo cell hc) and offers each staged record q to keep(q); survivors are appended
+// staged record (halo cell hc, local position p, reach) and offers each staged record q to keep(q);
+// survivors are appended
 // as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the wave.
 // Must be called by all 64 lanes of a wave (wave-uniform control flow around it).
 template <bool ALL_HALO, class Keep, class Eval>
 __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
-                                              uint32_t hc, Keep &&keep, Eval &&eval) {
+                                              uint32_t hc, const float4 &p, float reach, Keep &&keep, Eval &&eval) {
     const uint32_t lane = __lane_id();
     Walk<ALL_HALO> w;
     bool more = false;
-    if (active) more = w.begin(c, hc, hs);
+    if (active) more = w.begin(c, hc, p, reach, hs);
     uint32_t head = 0, tail = 0;  // wave-uniform
     while (__ballot(more)) {
         bool pass = false;
@@ -438,26 +483,106 @@ __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *
     }
 }
 
-// pair tables staged in LDS for small type counts (block-uniform choice)
-struct PairLds {
-    PairIdx pidx[LDS_TABLE];
-    float thr[LDS_TABLE];
-    PotDev pots[LDS_POTS];
+// ---- per-block type tables, carved from dynamic LDS ahead of the staged records -----------------
+// Staged bricks need n_types <= MAX_STAGED_TYPES (larger type counts take the slow path).
+constexpr uint32_t MAX_STAGED_TYPES = 16;
+
+__host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// bytes of the table region for nt types and np potentials (the larger of the force and search sets)
+__host__ __device__ inline size_t table_bytes(uint32_t nt, uint32_t np) {
+    const size_t n2 = (size_t)nt * nt;
+    const size_t f = align16(sizeof(PairIdx) * n2) + align16(sizeof(PotDev) * np) + align16(4 * n2) + align16(4 * nt);
+    const size_t r = align16(sizeof(RxPairIdx) * n2) + align16(4 * n2) + align16(4 * nt);
+    return f > r ? f : r;
+}
+
+struct ForceTables {
+    PairIdx *pidx;  // [nt * nt]
+    PotDev *pots;   // [n_pots]
+    float *thr;     // [nt * nt] prefilter thresholds
+    float *reach;   // [nt] largest cutoff of the type's row plus rounding slack (< 0: no potentials)
 };
 
-__device__ __forceinline__ bool load_pair_tables(PairLds &t, const Tables &T, float margin) {
-    const uint32_t np = T.n_types * T.n_types;
-    const bool fits = np <= LDS_TABLE;
-    if (fits) {
-        for (uint32_t k = threadIdx.x; k < np; k += BRICK_THREADS) {
-            const PairIdx pi = T.pidx[k];
-            t.pidx[k] = pi;
-            t.thr[k] = prefilter_thr(pi.maxcut2_f, pi.count != 0, margin);
+struct SearchTables {
+    RxPairIdx *o2;  // [nt * nt]
+    float *thr;     // [nt * nt]
+    float *reach;   // [nt] (< 0: no pair reactions)
+};
+
+// reach of a type from the FP32 squared cutoffs of its row, widened like the prefilter
+__device__ __forceinline__ float reach_of(float max_cut2_f, bool any, float margin) {
+    return any ? sqrtf(max_cut2_f) * (1.0f + 1.0e-6f) + 2.0f * margin : -1.0f;
+}
+
+__device__ __forceinline__ ForceTables load_force_tables(char *base, const Tables &T, float margin) {
+    const uint32_t nt = T.n_types, n2 = nt * nt;
+    ForceTables t;
+    char *p = base;
+    t.pidx = (PairIdx *)p;
+    p += align16(sizeof(PairIdx) * n2);
+    t.pots = (PotDev *)p;
+    p += align16(sizeof(PotDev) * T.n_pots);
+    t.thr = (float *)p;
+    p += align16(4 * n2);
+    t.reach = (float *)p;
+    for (uint32_t k = threadIdx.x; k < n2; k += BRICK_THREADS) {
+        const PairIdx pi = T.pidx[k];
+        t.pidx[k] = pi;
+        t.thr[k] = prefilter_thr(pi.maxcut2_f, pi.count != 0, margin);
+    }
+    for (uint32_t k = threadIdx.x; k < T.n_pots; k += BRICK_THREADS) t.pots[k] = T.pots[k];
+    if (threadIdx.x < nt) {
+        float m2 = 0.f;
+        bool any = false;
+        for (uint32_t u = 0; u < nt; ++u) {
+            const PairIdx pi = T.pidx[threadIdx.x * nt + u];
+            if (pi.count) {
+                any = true;
+                m2 = fmaxf(m2, pi.maxcut2_f);
+            }
         }
-        for (uint32_t k = threadIdx.x; k < T.n_pots && k < LDS_POTS; k += BRICK_THREADS) t.pots[k] = T.pots[k];
+        t.reach[threadIdx.x] = reach_of(m2, any, margin);
     }
     __syncthreads();
-    return fits;
+    return t;
+}
+
+__device__ __forceinline__ SearchTables load_search_tables(char *base, const Tables &T, float margin) {
+    const uint32_t nt = T.n_types, n2 = nt * nt;
+    SearchTables t;
+    char *p = base;
+    t.o2 = (RxPairIdx *)p;
+    p += align16(sizeof(RxPairIdx) * n2);
+    t.thr = (float *)p;
+    p += align16(4 * n2);
+    t.reach = (float *)p;
+    for (uint32_t k = threadIdx.x; k < n2; k += BRICK_THREADS) {
+        const RxPairIdx pi = T.o2idx[k];
+        t.o2[k] = pi;
+        t.thr[k] = prefilter_thr(pi.maxr2_f, pi.count != 0, margin);
+    }
+    if (threadIdx.x < nt) {
+        float m2 = 0.f;
+        bool any = false;
+        for (uint32_t u = 0; u < nt; ++u) {
+            const RxPairIdx pi = T.o2idx[threadIdx.x * nt + u];
+            if (pi.count) {
+                any = true;
+                m2 = fmaxf(m2, pi.maxr2_f);
+            }
+        }
+        t.reach[threadIdx.x] = reach_of(m2, any, margin);
+    }
+    __syncthreads();
+    return t;
+}
+
+// types whose bit is set: reach above `above` (waves are filled with these first)
+__device__ __forceinline__ uint32_t types_above(const float *reach, uint32_t nt, float above) {
+    uint32_t w = 0;
+    for (uint32_t t = 0; t < nt; ++t) w |= (reach[t] > above ? 1u : 0u) << t;
+    return w;
 }
 
 // FP32 min image of a local-frame displacement on whole-axis axes (halo axes need none)
@@ -608,7 +733,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                                                          const double *noise, double *force_out, uint32_t max_states,
                                                          uint32_t max_halo, uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ PairLds ptab;
     __shared__ ForceWave wave_lds[BRICK_WAVES];
     const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
     const ReplicaDev &R = reps[B.r];
@@ -624,15 +748,20 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         });
         return;
     }
-    const Frame F = make_frame(B, geo);
-    const bool lds_tables = load_pair_tables(ptab, T, F.margin);
-    const BrickSmem m = carve(smem, max_states, max_halo);
-    const Halo Hs = make_halo(B);
-    uint32_t n_inner = 0;
-    const uint32_t total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
     long long e_blk = 0;
+    uint32_t total = 0xFFFFFFFFu, n_inner = 0;
+    const uint32_t nt = T.n_types;
+    BrickSmem m;
+    ForceTables ft;
+    const Frame F = make_frame(B, geo);
+    const Halo Hs = make_halo(B);
+    if (nt <= MAX_STAGED_TYPES) {
+        ft = load_force_tables(smem, T, F.margin);
+        m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
+        const float half_cell = 0.5f * (float)fmin(geo.cs[0], fmin(geo.cs[1], geo.cs[2]));
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, types_above(ft.reach, nt, half_cell), &n_inner);
+    }
     if (total != 0xFFFFFFFFu) {
-        const uint32_t nt = T.n_types;
         const WalkCtx wc = make_walk(B, geo, Hs, radius);
         ForceWave &W = wave_lds[threadIdx.x / 64];
         const uint32_t lane = __lane_id();
@@ -640,8 +769,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         auto eval = [&](uint32_t ent) {
             const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
             const State o = cur.bins[m.slot[q]];
-            const uint32_t k = tw_type(W.tw[ol]) * nt + tw_type(o.tw);
-            const PairIdx pi = lds_tables ? ptab.pidx[k] : T.pidx[k];
+            const PairIdx pi = ft.pidx[tw_type(W.tw[ol]) * nt + tw_type(o.tw)];
             // CPUCalculateForces.cpp:226-259: x_ij = shortestDifference(x_i, x_j), gate d2 < rc^2
             const double ddx = min_image(o.x - W.x[ol], geo.box[0], geo.pbc[0]);
             const double ddy = min_image(o.y - W.y[ol], geo.box[1], geo.pbc[1]);
@@ -651,8 +779,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
             const float xf = (float)ddx, yf = (float)ddy, zf = (float)ddz, d2f = (float)d2;
             for (uint32_t p = 0; p < pi.count; ++p) {
-                const uint32_t pidx = pi.start + p;
-                const PotDev pot = (lds_tables && pidx < LDS_POTS) ? ptab.pots[pidx] : T.pots[pidx];
+                const PotDev pot = ft.pots[pi.start + p];
                 if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
             }
             atomicAdd(&W.acc[0][ol], (unsigned long long)to_fixed(pfx, 4294967296.0f));
@@ -660,55 +787,57 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             atomicAdd(&W.acc[2][ol], (unsigned long long)to_fixed(pfz, 4294967296.0f));
             e_blk += to_fixed(pe, 16777216.0f);
         };
-        for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
-            const uint32_t ii = base + lane;
-            bool valid = ii < n_inner;
-            uint32_t j = 0;
-            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid) {
-                j = m.inner[ii];
-                pj = m.P[j];
-                valid = !(__float_as_uint(pj.w) & TW_DEAD);
+        auto rounds = [&](auto all_halo) {
+            constexpr bool AH = decltype(all_halo)::value;
+            for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
+                const uint32_t ii = base + lane;
+                bool valid = ii < n_inner;
+                uint32_t j = 0;
+                float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (valid) {
+                    j = m.inner[ii];
+                    pj = m.P[j];
+                    valid = !(__float_as_uint(pj.w) & TW_DEAD);
+                }
+                State me{};
+                uint32_t sj = 0;
+                if (valid) {
+                    sj = m.slot[j];
+                    me = cur.bins[sj];
+                }
+                W.x[lane] = me.x;
+                W.y[lane] = me.y;
+                W.z[lane] = me.z;
+                W.tw[lane] = me.tw;
+                W.acc[0][lane] = 0ull;
+                W.acc[1][lane] = 0ull;
+                W.acc[2][lane] = 0ull;
+                wave_lds_sync();
+                const uint32_t t = tw_type(me.tw);
+                const float reach = ft.reach[t];
+                const bool active = valid && !(me.tw & TW_OUTSIDE) && reach >= 0.f;
+                const float *thr_row = ft.thr + t * nt;
+                // prefilter, branch-free: one LDS read of the candidate, one of the threshold
+                auto keep = [&](uint32_t q) -> bool {
+                    const float4 pq = m.P[q];
+                    const uint32_t tw2 = __float_as_uint(pq.w);
+                    float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
+                    if constexpr (!AH) frame_min_image(F, dx, dy, dz);
+                    const float d2p = dx * dx + dy * dy + dz * dz;
+                    const float thr = thr_row[tw_type(tw2)];
+                    return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
+                };
+                pair_pipeline<AH>(wc, m.hstart, W.ring, active, m.hc_of[j], pj, reach, keep, eval);
+                wave_lds_sync();
+                if (valid)
+                    finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, sj, me, true,
+                                          (long long)W.acc[0][lane], (long long)W.acc[1][lane],
+                                          (long long)W.acc[2][lane], err);
+                wave_lds_sync();
             }
-            State me{};
-            uint32_t sj = 0;
-            if (valid) {
-                sj = m.slot[j];
-                me = cur.bins[sj];
-            }
-            W.x[lane] = me.x;
-            W.y[lane] = me.y;
-            W.z[lane] = me.z;
-            W.tw[lane] = me.tw;
-            W.acc[0][lane] = 0ull;
-            W.acc[1][lane] = 0ull;
-            W.acc[2][lane] = 0ull;
-            wave_lds_sync();
-            const bool active = valid && !(me.tw & TW_OUTSIDE);
-            const uint32_t row = tw_type(me.tw) * nt;
-            auto keep = [&](uint32_t q) -> bool {
-                if (q == j) return false;
-                const float4 pq = m.P[q];
-                const uint32_t tw2 = __float_as_uint(pq.w);
-                if (tw2 & (TW_DEAD | TW_OUTSIDE)) return false;
-                float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
-                if (F.whole) frame_min_image(F, dx, dy, dz);
-                const float d2p = dx * dx + dy * dy + dz * dz;
-                const uint32_t k = row + tw_type(tw2);
-                const float thr =
-                    lds_tables ? ptab.thr[k] : prefilter_thr(T.pidx[k].maxcut2_f, T.pidx[k].count != 0, F.margin);
-                return d2p < thr;
-            };
-            const uint32_t hc = m.hc_of[j];
-            if (wc.whole == 0) pair_pipeline<true>(wc, m.hstart, W.ring, active, hc, keep, eval);
-            else pair_pipeline<false>(wc, m.hstart, W.ring, active, hc, keep, eval);
-            wave_lds_sync();
-            if (valid)
-                finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, sj, me, true,
-                                      (long long)W.acc[0][lane], (long long)W.acc[1][lane],
-                                      (long long)W.acc[2][lane], err);
-            wave_lds_sync();
-        }
+        };
+        if (wc.whole == 0) rounds(std::true_type{});
+        else rounds(std::false_type{});
     } else {
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
             if (me.tw & TW_DEAD) return;
@@ -735,8 +864,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
                                                                 uint32_t *ev_count, uint32_t max_states,
                                                                 uint32_t max_halo, uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
-    __shared__ float s_thr[LDS_TABLE];
-    __shared__ RxPairIdx s_o2[LDS_TABLE];
     __shared__ SearchWave wave_lds[BRICK_WAVES];
     const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
     const ReplicaDev &R = reps[B.r];
@@ -754,21 +881,18 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     }
     if (handler > 1 || !R.in_use || !R.active) return;
     const uint32_t nt = T.n_types;
-    const bool lds_tables = nt * nt <= LDS_TABLE;
+    if (!T.has_o2 && T.o1idx[nt] == 0) return;  // no reactions
     const Frame F = make_frame(B, geo);
-    if (lds_tables) {
-        for (uint32_t k = threadIdx.x; k < nt * nt; k += BRICK_THREADS) {
-            const RxPairIdx pi = T.o2idx[k];
-            s_o2[k] = pi;
-            s_thr[k] = prefilter_thr(pi.maxr2_f, pi.count != 0, F.margin);
-        }
-    }
-    __syncthreads();
-    const BrickSmem m = carve(smem, max_states, max_halo);
     const Halo Hs = make_halo(B);
-    uint32_t n_inner = 0;
-    const uint32_t total =
-        (T.has_o2 || T.o1idx[nt] > 0) ? stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner) : 0u;
+    uint32_t total = 0xFFFFFFFFu, n_inner = 0;
+    BrickSmem m;
+    SearchTables st;
+    if (nt <= MAX_STAGED_TYPES) {
+        st = load_search_tables(smem, T, F.margin);
+        m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
+        // particles with pair reactions first
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, types_above(st.reach, nt, -0.5f), &n_inner);
+    }
     if (total == 0xFFFFFFFFu) {  // slow path: search from the brick's own cells straight from the bins
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
             if (!(me.tw & TW_DEAD))
@@ -803,8 +927,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
         const uint32_t s2 = m.slot[q];
         const State o = cur.bins[s2];
         const uint32_t mh = W.h[ol], t = tw_type(W.tw[ol]);
-        const uint32_t kk = t * nt + tw_type(o.tw);
-        const RxPairIdx pi = lds_tables ? s_o2[kk] : T.o2idx[kk];
+        const RxPairIdx pi = st.o2[t * nt + tw_type(o.tw)];
         // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
         const double ddx = min_image(W.x[ol] - o.x, geo.box[0], geo.pbc[0]);
         const double ddy = min_image(W.y[ol] - o.y, geo.box[1], geo.pbc[1]);
@@ -824,65 +947,66 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
             else emit(rx, s2, W.s[ol], o.h, mh, event_key(handler, x4, X.rate));
         }
     };
-    for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
-        const uint32_t ii = base + lane;
-        bool valid = ii < n_inner;
-        uint32_t j = 0;
-        float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (valid) {
-            j = m.inner[ii];
-            pj = m.P[j];
-            valid = !(__float_as_uint(pj.w) & TW_DEAD);
-        }
-        State me{};
-        uint32_t sj = 0;
-        if (valid) {
-            sj = m.slot[j];
-            me = cur.bins[sj];
-        }
-        const uint32_t t = tw_type(me.tw);
-        if (valid) {  // order 1 (CPUUncontrolledApproximation.cpp:81-95)
-            for (uint32_t k = T.o1idx[t]; k < T.o1idx[t + 1]; ++k) {
-                const uint32_t rx = T.o1list[k];
-                const RxDev X = T.rx[rx];
-                if (!(X.rate > 0)) continue;
-                uint32_t x4[4];
-                philox(me.h, step, 1u | (rx << 8), 0u, seed, x4);
-                if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
-                emit(rx, sj, sj, me.h, me.h, event_key(handler, x4, X.rate));
+    auto rounds = [&](auto all_halo) {
+        constexpr bool AH = decltype(all_halo)::value;
+        for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
+            const uint32_t ii = base + lane;
+            bool valid = ii < n_inner;
+            uint32_t j = 0;
+            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (valid) {
+                j = m.inner[ii];
+                pj = m.P[j];
+                valid = !(__float_as_uint(pj.w) & TW_DEAD);
             }
+            State me{};
+            uint32_t sj = 0;
+            if (valid) {
+                sj = m.slot[j];
+                me = cur.bins[sj];
+            }
+            const uint32_t t = tw_type(me.tw);
+            if (valid) {  // order 1 (CPUUncontrolledApproximation.cpp:81-95)
+                for (uint32_t k = T.o1idx[t]; k < T.o1idx[t + 1]; ++k) {
+                    const uint32_t rx = T.o1list[k];
+                    const RxDev X = T.rx[rx];
+                    if (!(X.rate > 0)) continue;
+                    uint32_t x4[4];
+                    philox(me.h, step, 1u | (rx << 8), 0u, seed, x4);
+                    if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
+                    emit(rx, sj, sj, me.h, me.h, event_key(handler, x4, X.rate));
+                }
+            }
+            if (!T.has_o2) continue;  // block-uniform
+            W.x[lane] = me.x;
+            W.y[lane] = me.y;
+            W.z[lane] = me.z;
+            W.tw[lane] = me.tw;
+            W.h[lane] = me.h;
+            W.s[lane] = sj;
+            wave_lds_sync();
+            const float reach = st.reach[t];
+            const bool active = valid && !(me.tw & TW_OUTSIDE) && reach >= 0.f;
+            const float *thr_row = st.thr + t * nt;
+            // each unordered pair once: the lower type finds it; equal types: the lower bin slot
+            // (which also excludes the particle itself); branch-free
+            auto keep = [&](uint32_t q) -> bool {
+                const float4 pq = m.P[q];
+                const uint32_t tw2 = __float_as_uint(pq.w);
+                const uint32_t t2 = tw_type(tw2);
+                float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
+                if constexpr (!AH) frame_min_image(F, dx, dy, dz);
+                const float d2p = dx * dx + dy * dy + dz * dz;
+                const float thr = thr_row[t2];
+                const uint32_t sq = m.slot[q];
+                return ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & ((t2 > t) | ((t2 == t) & (sq > sj))) & (d2p < thr);
+            };
+            pair_pipeline<AH>(wc, m.hstart, W.ring, active, m.hc_of[j], pj, reach, keep, eval);
+            wave_lds_sync();
         }
-        if (!T.has_o2) continue;  // block-uniform
-        W.x[lane] = me.x;
-        W.y[lane] = me.y;
-        W.z[lane] = me.z;
-        W.tw[lane] = me.tw;
-        W.h[lane] = me.h;
-        W.s[lane] = sj;
-        wave_lds_sync();
-        const bool active = valid && !(me.tw & TW_OUTSIDE);
-        const uint32_t row = t * nt;
-        // each unordered pair once: the lower type finds it; equal types: the lower bin slot
-        auto keep = [&](uint32_t q) -> bool {
-            if (q == j) return false;
-            const float4 pq = m.P[q];
-            const uint32_t tw2 = __float_as_uint(pq.w);
-            if (tw2 & (TW_DEAD | TW_OUTSIDE)) return false;
-            const uint32_t t2 = tw_type(tw2);
-            if (t2 < t || (t2 == t && m.slot[q] < sj)) return false;
-            float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
-            if (F.whole) frame_min_image(F, dx, dy, dz);
-            const float d2p = dx * dx + dy * dy + dz * dz;
-            const uint32_t kk = row + t2;
-            const float thr =
-                lds_tables ? s_thr[kk] : prefilter_thr(T.o2idx[kk].maxr2_f, T.o2idx[kk].count != 0, F.margin);
-            return d2p < thr;
-        };
-        const uint32_t hc = m.hc_of[j];
-        if (wc.whole == 0) pair_pipeline<true>(wc, m.hstart, W.ring, active, hc, keep, eval);
-        else pair_pipeline<false>(wc, m.hstart, W.ring, active, hc, keep, eval);
-        wave_lds_sync();
-    }
+    };
+    if (wc.whole == 0) rounds(std::true_type{});
+    else rounds(std::false_type{});
 }
 
 }  // namespace rdy

@@ -502,17 +502,23 @@ int configure_bricks(rdy_engine *e, double lam) {
         if (hipFuncGetAttributes(&fa, k) == hipSuccess) static_lds = std::max(static_lds, (size_t)fa.sharedSizeBytes);
     }
     if (static_lds == 0) static_lds = 16384;
-    if (e->lds_limit < static_lds + 8192) return set_err(RDY_ERR_INVALID, "not enough LDS per workgroup");
-    const size_t budget = e->lds_limit - static_lds - 256;
-    // staged records: the expected halo occupancy with headroom; rarer fuller halos take the slow path
+    // per-block type tables precede the staged records in dynamic LDS
+    const size_t tables = table_bytes(std::min(e->T.n_types, MAX_STAGED_TYPES), e->T.n_pots);
+    if (e->lds_limit < static_lds + tables + 8192) return set_err(RDY_ERR_INVALID, "not enough LDS per workgroup");
+    const size_t budget = e->lds_limit - static_lds - tables - 256;
+    // staged records: the expected halo occupancy plus ten standard deviations of a uniform (Poisson)
+    // fill; fuller halos (clustered systems) take the slow path
     uint32_t want = e->max_states;
-    if (want == 0) want = (uint32_t)std::max(768.0, 1.5 * std::max(lam, 0.25) * e->max_halo + 128.0);
+    if (want == 0) {
+        const double mean = std::max(lam, 0.25) * e->max_halo;
+        want = (uint32_t)std::max(512.0, mean + 10.0 * std::sqrt(mean) + 64.0);
+    }
     want = std::min<uint32_t>(want, 60000);
     while (want > 256 && brick_smem_bytes(want, e->max_halo) > budget) want -= 128;
     if (brick_smem_bytes(want, e->max_halo) > budget)
         return set_err(RDY_ERR_INVALID, "brick does not fit in LDS: use a smaller brick");
     e->max_states = want;
-    e->brick_smem = brick_smem_bytes(want, e->max_halo);
+    e->brick_smem = tables + brick_smem_bytes(want, e->max_halo);
     if (e->brick_smem > 65536) {
         (void)hipFuncSetAttribute((const void *)k_brick<MODE_STEP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)e->brick_smem);
