@@ -101,7 +101,7 @@ __device__ __forceinline__ BrickSmem carve(char *base, uint32_t max_states, uint
     return m;
 }
 
-size_t brick_smem_bytes(uint32_t max_states, uint32_t max_halo) {
+__host__ __device__ inline size_t brick_smem_bytes(uint32_t max_states, uint32_t max_halo) {
     return (sizeof(float4) + 4 + 2 + 2) * (size_t)max_states + 12 * (size_t)max_halo + 4 + 64;
 }
 
@@ -633,6 +633,7 @@ __device__ __forceinline__ void block_add_energy(ReplicaDev *reps, uint32_t r, l
         for (int w = 0; w < BRICK_THREADS / 64; ++w) t += s_e[w];
         if (t != 0) atomicAdd(&reps[r].energy, (unsigned long long)t);
     }
+    __syncthreads();  // s_e may be written again by a following call
 }
 
 // one particle: bonded force, optional force output, integration and re-binning
@@ -722,19 +723,40 @@ __device__ __forceinline__ void for_each_own_record(const Grid &cur, const Brick
 struct ForceWave {  // per-wave LDS of k_brick
     uint32_t ring[RING];
     unsigned long long acc[3][64];  // fixed-point force of the wave's 64 particles
-    double x[64], y[64], z[64];     // their FP64 positions
-    uint32_t tw[64];
+    uint32_t s[64];                 // their bin slots
 };
+
+// Bricks whose halo does not fit in LDS are listed for a second kernel that reads neighbours
+// straight from the bins.  list: [0] count, [1] blocks of the second kernel done, [2..] bricks.
+__device__ __forceinline__ void list_slow_brick(uint32_t *list, uint32_t b) {
+    if (threadIdx.x == 0) list[2 + atomicAdd(&list[0], 1u)] = b;
+}
+
+// the last block of the second kernel clears the list for the next launch (every block has read
+// the count before it counts itself done)
+__device__ __forceinline__ void release_slow_list(uint32_t *list) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&list[1], 1u) == gridDim.x - 1) {
+            list[0] = 0u;
+            list[1] = 0u;
+            __threadfence();
+        }
+    }
+}
 
 // forces (+ Euler-Maruyama + re-bin) of every particle of one brick; MODE as in k_fused
 template <int MODE>
 __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, ReplicaDev *reps, Tables T, Topo topo,
                                                          const BrickDev *bricks, uint32_t cap, int radius, double dt,
                                                          const double *noise, double *force_out, uint32_t max_states,
-                                                         uint32_t max_halo, uint32_t *err) {
+                                                         uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
+                                                         uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ ForceWave wave_lds[BRICK_WAVES];
-    const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
+    const uint32_t bidx = xcd_block(blockIdx.x, gridDim.x);
+    const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
     if (!R.in_use) return;
     const RepGeom geo = load_geom(R);
@@ -756,6 +778,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
     const Frame F = make_frame(B, geo);
     const Halo Hs = make_halo(B);
     if (nt <= MAX_STAGED_TYPES) {
+        if (table_bytes(nt, T.n_pots) + brick_smem_bytes(max_states, max_halo) > smem_bytes) {  // block-uniform
+            if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
+            return;
+        }
         ft = load_force_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
         const float half_cell = 0.5f * (float)fmin(geo.cs[0], fmin(geo.cs[1], geo.cs[2]));
@@ -768,12 +794,13 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         // the exact pair force of a survivor (entry = owner lane << 16 | staged neighbour)
         auto eval = [&](uint32_t ent) {
             const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
+            const State me = cur.bins[W.s[ol]];
             const State o = cur.bins[m.slot[q]];
-            const PairIdx pi = ft.pidx[tw_type(W.tw[ol]) * nt + tw_type(o.tw)];
+            const PairIdx pi = ft.pidx[tw_type(me.tw) * nt + tw_type(o.tw)];
             // CPUCalculateForces.cpp:226-259: x_ij = shortestDifference(x_i, x_j), gate d2 < rc^2
-            const double ddx = min_image(o.x - W.x[ol], geo.box[0], geo.pbc[0]);
-            const double ddy = min_image(o.y - W.y[ol], geo.box[1], geo.pbc[1]);
-            const double ddz = min_image(o.z - W.z[ol], geo.box[2], geo.pbc[2]);
+            const double ddx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
+            const double ddy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
+            const double ddz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
             const double d2 = dist2(ddx, ddy, ddz);
             if (!(d2 < pi.maxcut2)) return;
             float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
@@ -805,10 +832,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                     sj = m.slot[j];
                     me = cur.bins[sj];
                 }
-                W.x[lane] = me.x;
-                W.y[lane] = me.y;
-                W.z[lane] = me.z;
-                W.tw[lane] = me.tw;
+                W.s[lane] = sj;
                 W.acc[0][lane] = 0ull;
                 W.acc[1][lane] = 0ull;
                 W.acc[2][lane] = 0ull;
@@ -838,7 +862,24 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         };
         if (wc.whole == 0) rounds(std::true_type{});
         else rounds(std::false_type{});
+        block_add_energy(reps, B.r, e_blk);
     } else {
+        list_slow_brick(slow, bidx);
+    }
+}
+
+// the listed bricks of k_brick, neighbours read straight from the bins
+template <int MODE>
+__global__ __launch_bounds__(BRICK_THREADS) void k_brick_slow(Grid cur, Grid nxt, ReplicaDev *reps, Tables T,
+                                                              Topo topo, const BrickDev *bricks, uint32_t cap,
+                                                              int radius, double dt, const double *noise,
+                                                              double *force_out, uint32_t *slow, uint32_t *err) {
+    const uint32_t n = atomicAdd(&slow[0], 0u);
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // block-uniform
+        const BrickDev B = bricks[slow[2 + i]];
+        const ReplicaDev &R = reps[B.r];
+        const RepGeom geo = load_geom(R);
+        long long e_blk = 0;
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
             if (me.tw & TW_DEAD) return;
             long long fx = 0, fy = 0, fz = 0, e = 0;
@@ -846,14 +887,14 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             e_blk += e;
             finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, s, me, true, fx, fy, fz, err);
         });
+        block_add_energy(reps, B.r, e_blk);
     }
-    block_add_energy(reps, B.r, e_blk);
+    release_slow_list(slow);
 }
 
 struct SearchWave {  // per-wave LDS of k_brick_search
     uint32_t ring[RING];
-    double x[64], y[64], z[64];
-    uint32_t tw[64], h[64], s[64];
+    uint32_t s[64];  // bin slots of the wave's 64 particles
 };
 
 // reaction candidate search of one brick on the current buffer (see search_one for the rules),
@@ -862,10 +903,12 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
                                                                 Tables T, const BrickDev *bricks, uint32_t cap,
                                                                 int radius, int handler, Event *events,
                                                                 uint32_t *ev_count, uint32_t max_states,
-                                                                uint32_t max_halo, uint32_t *err) {
+                                                                uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
+                                                                uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ SearchWave wave_lds[BRICK_WAVES];
-    const BrickDev B = bricks[xcd_block(blockIdx.x, gridDim.x)];
+    const uint32_t bidx = xcd_block(blockIdx.x, gridDim.x);
+    const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
     const RepGeom geo = load_geom(R);
     // clear the old buffer's counters of this brick's cells (the old buffer is not read again)
@@ -888,16 +931,17 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     BrickSmem m;
     SearchTables st;
     if (nt <= MAX_STAGED_TYPES) {
+        if (table_bytes(nt, T.n_pots) + brick_smem_bytes(max_states, max_halo) > smem_bytes) {  // block-uniform
+            if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
+            return;
+        }
         st = load_search_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
         // particles with pair reactions first
         total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, types_above(st.reach, nt, -0.5f), &n_inner);
     }
-    if (total == 0xFFFFFFFFu) {  // slow path: search from the brick's own cells straight from the bins
-        for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
-            if (!(me.tw & TW_DEAD))
-                search_one(cur, reps, T, cap, radius, handler, 0, events, ev_count, nullptr, 0, s, me, err);
-        });
+    if (total == 0xFFFFFFFFu) {
+        list_slow_brick(slow, bidx);
         return;
     }
     const uint32_t r = B.r, step = R.step;
@@ -924,14 +968,15 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     // the exact test and draws of a surviving pair (entry = owner lane << 16 | staged neighbour)
     auto eval = [&](uint32_t ent) {
         const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
-        const uint32_t s2 = m.slot[q];
+        const uint32_t s1 = W.s[ol], s2 = m.slot[q];
+        const State me = cur.bins[s1];
         const State o = cur.bins[s2];
-        const uint32_t mh = W.h[ol], t = tw_type(W.tw[ol]);
+        const uint32_t mh = me.h, t = tw_type(me.tw);
         const RxPairIdx pi = st.o2[t * nt + tw_type(o.tw)];
         // distSquared(neighbor.pos, entry.pos) (CPUUncontrolledApproximation.cpp:108)
-        const double ddx = min_image(W.x[ol] - o.x, geo.box[0], geo.pbc[0]);
-        const double ddy = min_image(W.y[ol] - o.y, geo.box[1], geo.pbc[1]);
-        const double ddz = min_image(W.z[ol] - o.z, geo.box[2], geo.pbc[2]);
+        const double ddx = min_image(me.x - o.x, geo.box[0], geo.pbc[0]);
+        const double ddy = min_image(me.y - o.y, geo.box[1], geo.pbc[1]);
+        const double ddz = min_image(me.z - o.z, geo.box[2], geo.pbc[2]);
         const double d2 = dist2(ddx, ddy, ddz);
         const uint32_t lo = mh < o.h ? mh : o.h, hi = mh < o.h ? o.h : mh;
         for (uint32_t p = 0; p < pi.count; ++p) {
@@ -943,8 +988,8 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
             if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
             // entry1: the educt of type educts[0]; equal educt types: the lower handle
             const bool me_first = (X.educts[0] != X.educts[1]) ? (t == X.educts[0]) : (mh < o.h);
-            if (me_first) emit(rx, W.s[ol], s2, mh, o.h, event_key(handler, x4, X.rate));
-            else emit(rx, s2, W.s[ol], o.h, mh, event_key(handler, x4, X.rate));
+            if (me_first) emit(rx, s1, s2, mh, o.h, event_key(handler, x4, X.rate));
+            else emit(rx, s2, s1, o.h, mh, event_key(handler, x4, X.rate));
         }
     };
     auto rounds = [&](auto all_halo) {
@@ -978,11 +1023,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
                 }
             }
             if (!T.has_o2) continue;  // block-uniform
-            W.x[lane] = me.x;
-            W.y[lane] = me.y;
-            W.z[lane] = me.z;
-            W.tw[lane] = me.tw;
-            W.h[lane] = me.h;
             W.s[lane] = sj;
             wave_lds_sync();
             const float reach = st.reach[t];
@@ -1007,6 +1047,23 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
     };
     if (wc.whole == 0) rounds(std::true_type{});
     else rounds(std::false_type{});
+}
+
+// the listed bricks of k_brick_search, searched straight from the bins
+__global__ __launch_bounds__(BRICK_THREADS) void k_brick_search_slow(Grid cur, ReplicaDev *reps, Tables T,
+                                                                     const BrickDev *bricks, uint32_t cap, int radius,
+                                                                     int handler, Event *events, uint32_t *ev_count,
+                                                                     uint32_t *slow, uint32_t *err) {
+    const uint32_t n = atomicAdd(&slow[0], 0u);
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const BrickDev B = bricks[slow[2 + i]];
+        const RepGeom geo = load_geom(reps[B.r]);
+        for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
+            if (!(me.tw & TW_DEAD))
+                search_one(cur, reps, T, cap, radius, handler, 0, events, ev_count, nullptr, 0, s, me, err);
+        });
+    }
+    release_slow_list(slow);
 }
 
 }  // namespace rdy

@@ -113,16 +113,17 @@ __device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
 
 // three N(0,1) draws by Box-Muller from one Philox block (v_k = (x_k + 1/2) / 2^32)
 __device__ __forceinline__ void normals3(const uint32_t x[4], double n[3]) {
-    const double two_pi = 6.283185307179586;
     double v0 = ((double)x[0] + 0.5) * (1.0 / 4294967296.0);
     double v1 = ((double)x[1] + 0.5) * (1.0 / 4294967296.0);
     double v2 = ((double)x[2] + 0.5) * (1.0 / 4294967296.0);
     double v3 = ((double)x[3] + 0.5) * (1.0 / 4294967296.0);
     double r0 = sqrt(-2.0 * log(v0));
     double r1 = sqrt(-2.0 * log(v2));
-    double s1, c1, s3, c3;
-    sincos(two_pi * v1, &s1, &c1);
-    sincos(two_pi * v3, &s3, &c3);
+    // sin/cos(2 pi v) as sinpi/cospi(2 v): the same values up to the last bit, without the
+    // large-argument reduction path (v < 1)
+    double s1, c1;
+    sincospi(2.0 * v1, &s1, &c1);
+    const double c3 = cospi(2.0 * v3);
     n[0] = r0 * c1;
     n[1] = r0 * s1;
     n[2] = r1 * c3;
@@ -140,9 +141,22 @@ __device__ __forceinline__ double min_image(double dv, double L, int32_t periodi
 
 // while-loop wrap (boundary_condition_operations.h:53-61); non-finite values are left as they are,
 // and a coordinate more than 1024 box lengths out is first reduced with fmod so the loops stay short
+// fmod(v, L) for finite v and L > 0, by binary long division: every subtraction a - m has
+// m <= a < 2m, so it is exact (Sterbenz) and the result equals the exactly rounded fmod
+__device__ __forceinline__ double fmod_exact(double v, double L) {
+    double a = fabs(v);
+    if (a >= L) {
+        int e = ilogb(a) - ilogb(L);
+        double m = ldexp(L, e);
+        for (; e >= 0; --e, m *= 0.5)
+            if (a >= m) a -= m;
+    }
+    return copysign(a, v);
+}
+
 __device__ __forceinline__ double wrap(double v, double L, int32_t periodic) {
     if (periodic && isfinite(v)) {
-        if (!(fabs(v) < 1024.0 * L)) v = fmod(v, L);
+        if (!(fabs(v) < 1024.0 * L)) v = fmod_exact(v, L);
         while (v >= .5 * L) v -= L;
         while (v < -.5 * L) v += L;
     }

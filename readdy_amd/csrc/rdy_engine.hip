@@ -82,10 +82,12 @@ struct rdy_engine {
     uint32_t brick[3] = {0, 0, 0};
     uint32_t max_states = 0, max_halo = 0;
     size_t brick_smem = 0;
+    size_t tables_lds = 0;  // bytes of brick_smem reserved for the per-block type tables
     size_t lds_limit = 65536;
     std::vector<BrickDev> bricks_h;
     BrickDev *d_bricks = nullptr;
     size_t d_bricks_cap = 0;
+    uint32_t *d_slow = nullptr;  // two lists of bricks left to the bin-reading kernels (forces, search)
     bool wl_valid[2] = {true, true};
     uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
@@ -181,6 +183,7 @@ int check_errors(rdy_engine *e) {
     if (h[ERR_EVENT_OVERFLOW])
         m += "reaction event segment overflow (" + std::to_string(h[ERR_EVENT_OVERFLOW]) +
              "): too many accepted reactions per step, reduce the time step; ";
+    if (h[ERR_LDS_LAYOUT]) return set_err(RDY_ERR_STATE, "brick kernels: LDS layout exceeds the reserved size");
     if (h[ERR_EVENT_LDS]) m += "more than " + std::to_string(RESOLVE_MAX_EVENTS) + " accepted events in one replica step; ";
     if (h[ERR_WL_OVERFLOW]) m += "particle pool exhausted: raise max_particles; ";
     return set_err(RDY_ERR_CAPACITY, m);
@@ -334,6 +337,10 @@ int upload_tables(rdy_engine *e, double dt) {
     e->T.has_o2 = has_o2 ? 1u : 0u;
     e->T.n_pots = (uint32_t)flat.size();
     e->T.kbt = e->kbt;
+    // the brick kernels place the type tables ahead of the staged records in the dynamic LDS that
+    // configure_bricks reserved (from the registered potentials, before these tables existed)
+    if (e->brick_smem && table_bytes(std::min(nt, MAX_STAGED_TYPES), e->T.n_pots) > e->tables_lds)
+        return set_err(RDY_ERR_STATE, "type tables exceed the LDS reserved for them");
     e->tables_dt = dt;
     e->version++;
     return RDY_OK;
@@ -464,8 +471,11 @@ int upload_bricks(rdy_engine *e) {
     if (e->bricks_h.size() > e->d_bricks_cap) {
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (e->d_bricks) HIP_TRY(hipFree(e->d_bricks));
+        if (e->d_slow) HIP_TRY(hipFree(e->d_slow));
         e->d_bricks_cap = std::max<size_t>(e->bricks_h.size(), 2 * e->d_bricks_cap);
         HIP_TRY(dalloc(&e->d_bricks, e->d_bricks_cap));
+        HIP_TRY(dalloc(&e->d_slow, 2 * (e->d_bricks_cap + 2)));
+        HIP_TRY(hipMemsetAsync(e->d_slow, 0, 2 * (e->d_bricks_cap + 2) * sizeof(uint32_t), e->stream));
     }
     if (!e->bricks_h.empty())
         HIP_TRY(hipMemcpyAsync(e->d_bricks, e->bricks_h.data(), e->bricks_h.size() * sizeof(BrickDev),
@@ -475,6 +485,12 @@ int upload_bricks(rdy_engine *e) {
 }
 
 uint32_t n_bricks(const rdy_engine *e) { return (uint32_t)std::max<size_t>(e->bricks_h.size(), 1); }
+
+// list of bricks for the bin-reading forces (0) or search (1) kernel
+uint32_t *slow_list(const rdy_engine *e, int which) { return e->d_slow + (size_t)which * (e->d_bricks_cap + 2); }
+
+// blocks of the bin-reading kernels (they loop over the listed bricks; none listed: they just exit)
+uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 1024u); }
 
 // brick shape and LDS budget from the expected particles per cell (overridable in the config)
 int configure_bricks(rdy_engine *e, double lam) {
@@ -502,8 +518,11 @@ int configure_bricks(rdy_engine *e, double lam) {
         if (hipFuncGetAttributes(&fa, k) == hipSuccess) static_lds = std::max(static_lds, (size_t)fa.sharedSizeBytes);
     }
     if (static_lds == 0) static_lds = 16384;
-    // per-block type tables precede the staged records in dynamic LDS
-    const size_t tables = table_bytes(std::min(e->T.n_types, MAX_STAGED_TYPES), e->T.n_pots);
+    // per-block type tables precede the staged records in dynamic LDS; they are uploaded later, so
+    // size them from the registered potentials (each lands in at most two ordered type-pair lists)
+    const size_t tables =
+        table_bytes(std::min(e->n_types, MAX_STAGED_TYPES), (uint32_t)(2 * e->pots_in.size()));
+    e->tables_lds = tables;
     if (e->lds_limit < static_lds + tables + 8192) return set_err(RDY_ERR_INVALID, "not enough LDS per workgroup");
     const size_t budget = e->lds_limit - static_lds - tables - 256;
     // staged records: the expected halo occupancy plus ten standard deviations of a uniform (Poisson)
@@ -573,7 +592,8 @@ void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
     if (!e->bricks_h.empty())
         hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
                            e->grid(grid_of_bricks_buffer), e->counts[b], e->reps, e->T, (const BrickDev *)e->d_bricks,
-                           e->cap, e->radius, 2, e->events, e->ev_count, e->max_states, e->max_halo, e->err);
+                           e->cap, e->radius, 2, e->events, e->ev_count, e->max_states, e->max_halo, 0u, slow_list(e, 1),
+                           e->err);
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
 }
 
@@ -596,16 +616,24 @@ void launch_step(rdy_engine *e, int handler, double dt) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
     hipLaunchKernelGGL(k_brick<MODE_STEP>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
                        e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, e->err);
+                       e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
+                       slow_list(e, 0), e->err);
     if (e->prof && e->prof_used < e->prof_ev.size()) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
     }
+    hipLaunchKernelGGL(k_brick_slow<MODE_STEP>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
+                       e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap, e->radius, dt,
+                       (const double *)nullptr, (double *)nullptr, slow_list(e, 0), e->err);
     // reaction search on the new buffer, fused with clearing the old buffer's counters
     const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
     hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(n),
                        e->counts[c], e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events,
-                       e->ev_count, e->max_states, e->max_halo, e->err);
+                       e->ev_count, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    if (h != RDY_HANDLER_NONE)
+        hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(n),
+                           e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events, e->ev_count,
+                           slow_list(e, 1), e->err);
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(c));
     if (h != RDY_HANDLER_NONE)
         hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(n), e->reps, e->T,
@@ -622,7 +650,11 @@ void launch_forces(rdy_engine *e, double *force_out) {
     const int c = e->cur;
     hipLaunchKernelGGL(k_brick<MODE_FORCES>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
                        e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo, e->err);
+                       e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
+                       slow_list(e, 0), e->err);
+    hipLaunchKernelGGL(k_brick_slow<MODE_FORCES>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream,
+                       e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
+                       e->radius, 0.0, (const double *)nullptr, force_out, slow_list(e, 0), e->err);
 }
 
 int flush_prof(rdy_engine *e) {
@@ -752,7 +784,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->wl[b]);
         (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -1049,7 +1081,8 @@ int rdy_update_neighbor_list(rdy_engine *e) {
     const int c = e->cur, n = 1 - c;
     hipLaunchKernelGGL(k_brick<MODE_REBIN>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
                        e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, e->err);
+                       e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
+                       slow_list(e, 0), e->err);
     launch_clear_bricks(e, c, n);
     e->wl_valid[n] = false;
     e->cur = n;
@@ -1114,7 +1147,10 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
     const int c = e->cur;
     hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(c),
                        (uint32_t *)nullptr, e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler,
-                       e->events, e->ev_count, e->max_states, e->max_halo, e->err);
+                       e->events, e->ev_count, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
+                       e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler, e->events,
+                       e->ev_count, slow_list(e, 1), e->err);
     hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
                        e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles, e->err);
     return check_errors(e);

@@ -60,7 +60,7 @@ struct Event {
 
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
-             ERR_LDS_SEARCH = 5, N_ERR = 8 };
+             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, N_ERR = 8 };
 
 constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
 constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer

@@ -93,8 +93,8 @@ def _system_c0like(n=2000, L_box=20.0, seed=0):
     return pos, types
 
 
-def _force_case(pots, n_types, pos, types, box, pbc=(1, 1, 1)):
-    e, r = _engine(n_types, box, pbc, pots, n=len(types))
+def _force_case(pots, n_types, pos, types, box, pbc=(1, 1, 1), **kw):
+    e, r = _engine(n_types, box, pbc, pots, n=len(types), **kw)
     e.set_particles(r, pos, types)
     e.calculate_forces()
     f, hs = e.get_forces(r)
@@ -117,7 +117,10 @@ def test_forces_harmonic_repulsion_c0():
     assert rel < FORCE_TOL and e_rel < ENERGY_TOL, (rel, e_rel)
 
 
-def test_forces_weak_interaction_c1_chemistry():
+# lds_states: 0 sizes the LDS staging automatically (every brick staged); 600 leaves about half of
+# the bricks (~600 halo records each) to the bin-reading kernel; 128 leaves all of them to it
+@pytest.mark.parametrize("lds", [0, 600, 128])
+def test_forces_weak_interaction_c1_chemistry(lds):
     # C1 chemistry at C1 density (0.125) in a smaller box: HR everywhere + WIPH between A and A
     n, Lb = 8000, 40.0
     rng = np.random.default_rng(1)
@@ -125,7 +128,7 @@ def test_forces_weak_interaction_c1_chemistry():
     types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
     pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
     pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
-    rel, e_rel, _ = _force_case(pots, 3, pos, types, [Lb] * 3)
+    rel, e_rel, _ = _force_case(pots, 3, pos, types, [Lb] * 3, lds_states=lds)
     assert rel < FORCE_TOL and e_rel < ENERGY_TOL, (rel, e_rel)
 
 
@@ -289,15 +292,17 @@ def test_reaction_candidates_bit_exact():
     assert g == f
 
 
-@pytest.mark.parametrize("handler,mode", [(0, po.UA_KEYED), (1, po.GILLESPIE_KEYED)])
-def test_reaction_steps_match_oracle_event_for_event(handler, mode):
+# lds 256: about half of the bricks (~250 halo records) searched by the bin-reading kernel; 128: all
+@pytest.mark.parametrize("handler,mode,lds", [(0, po.UA_KEYED, 0), (1, po.GILLESPIE_KEYED, 0), (0, po.UA_KEYED, 256),
+                                              (1, po.GILLESPIE_KEYED, 128)])
+def test_reaction_steps_match_oracle_event_for_event(handler, mode, lds):
     """Same positions, same Philox counters: the same events fire and products land identically."""
     pots, rxs = _c0_chemistry()
     rxs[0]["rate"] = 20.0
     rxs[1]["rate"] = 10.0
     n, Lb, dt = 2000, 20.0, 0.01
     pos, types = _system_c0like(n, Lb, seed=7)
-    e, r = _engine(3, [Lb] * 3, (1, 1, 1), pots, rxs, diffusion=[1.0, 1.0, 0.5], seed=99, n=n)
+    e, r = _engine(3, [Lb] * 3, (1, 1, 1), pots, rxs, diffusion=[1.0, 1.0, 0.5], seed=99, n=n, lds_states=lds)
     e.set_particles(r, pos, types)
     o = _oracle(3, [Lb] * 3, [1, 1, 1], pots, rxs, diffusion=[1.0, 1.0, 0.5])
     o.set_seed(99)
@@ -386,11 +391,31 @@ def test_fusion_through_boundary_and_fission_near_boundary_on_gpu():
         assert len(p) == 2 and ((p[0][0] <= 0) != (p[1][0] <= 0))
 
 
-def _c0_engine(graph=False, n=2000, seed=11, max_replicas=1):
+def _c0_engine(graph=False, n=2000, seed=11, max_replicas=1, **kw):
     pots, rxs = _c0_chemistry()
     pos, types = _system_c0like(n, 20.0, seed=3)
-    e = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs, max_replicas=max_replicas)
+    e = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs, max_replicas=max_replicas, **kw)
     return e, pos, types
+
+
+@pytest.mark.parametrize("lds,graph", [(128, False), (256, True)])
+def test_bin_reading_bricks_equal_staged_bricks(lds, graph):
+    """Bricks left to the bin-reading kernels give the staged kernels' trajectory bit for bit
+    (fixed-point force sums, the same pair arithmetic, the same candidate rule), also under graph
+    replay, where the brick lists are cleared by the kernels themselves."""
+    e1, pos, types = _c0_engine()
+    r1 = e1.add_replica([20.0] * 3, seed=6, max_particles=8000)
+    e1.set_particles(r1, pos, types)
+    e2, _, _ = _c0_engine(lds_states=lds)
+    r2 = e2.add_replica([20.0] * 3, seed=6, max_particles=8000)
+    e2.set_particles(r2, pos, types)
+    for chunk in (7, 13):
+        e1.run(0, 0.01, chunk)
+        e2.run(0, 0.01, chunk, graph=graph)
+        for x, y in zip(e1.get_particles(r1), e2.get_particles(r2)):
+            assert np.array_equal(x, y)
+        assert e1.energy(r1) == e2.energy(r2)
+        assert np.array_equal(e1.reaction_totals(r1), e2.reaction_totals(r2))
 
 
 def test_fused_loop_equals_separate_actions():
