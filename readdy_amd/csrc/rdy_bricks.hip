@@ -43,7 +43,8 @@ struct BrickDev {
 
 constexpr int BRICK_THREADS = 256;
 constexpr int BRICK_WAVES = BRICK_THREADS / 64;
-constexpr uint32_t RING = 128;      // per-wave ring of prefilter survivors (power of two, >= 127)
+constexpr uint32_t BATCH = 4;       // candidates a lane takes from its current run per iteration
+constexpr uint32_t RING = 512;      // per-wave ring of prefilter survivors (power of two, >= 63 + 64 * BATCH)
 
 __host__ __device__ inline uint32_t div_magic(uint32_t d) {
     return d > 1 ? (uint32_t)((0x100000000ull + d - 1) / d) : 0u;
@@ -72,15 +73,22 @@ __device__ __forceinline__ Halo make_halo(const BrickDev &B) {
     return s;
 }
 
-struct BrickSmem {  // carved out of dynamic LDS (24 bytes per staged record)
+struct BrickSmem {  // carved out of dynamic LDS
     float4 *P;         // [max_states] local-frame FP32 position, w = tw bits
     uint32_t *slot;    // [max_states] bin slot (the FP64 record is read from the bins when needed)
+    uint32_t *hstart;  // [max_halo + 1]
     uint16_t *hc_of;   // [max_states]
     uint16_t *inner;   // [max_states]
+    // transient: the halo cells' counters and ids while staging, then the waves' survivor rings
     uint32_t *hcnt;    // [max_halo]
-    uint32_t *hstart;  // [max_halo + 1]
     uint32_t *hcell;   // [max_halo]
+    uint32_t *rings;   // [BRICK_WAVES][RING]
 };
+
+__host__ __device__ inline size_t transient_bytes(uint32_t max_halo) {
+    const size_t staging = 8 * (size_t)max_halo, rings = 4 * (size_t)BRICK_WAVES * RING;
+    return staging > rings ? staging : rings;
+}
 
 __device__ __forceinline__ BrickSmem carve(char *base, uint32_t max_states, uint32_t max_halo) {
     BrickSmem m;
@@ -89,20 +97,22 @@ __device__ __forceinline__ BrickSmem carve(char *base, uint32_t max_states, uint
     p += sizeof(float4) * max_states;
     m.slot = (uint32_t *)p;
     p += 4 * max_states;
-    m.hcnt = (uint32_t *)p;
-    p += 4 * max_halo;
     m.hstart = (uint32_t *)p;
     p += 4 * (max_halo + 1);
-    m.hcell = (uint32_t *)p;
-    p += 4 * max_halo;
     m.hc_of = (uint16_t *)p;
     p += 2 * max_states;
     m.inner = (uint16_t *)p;
+    p += 2 * max_states;
+    p = base + ((p - base + 15) & ~(ptrdiff_t)15);
+    m.hcnt = (uint32_t *)p;
+    m.hcell = m.hcnt + max_halo;
+    m.rings = (uint32_t *)p;
     return m;
 }
 
 __host__ __device__ inline size_t brick_smem_bytes(uint32_t max_states, uint32_t max_halo) {
-    return (sizeof(float4) + 4 + 2 + 2) * (size_t)max_states + 12 * (size_t)max_halo + 4 + 64;
+    const size_t fixed = (sizeof(float4) + 4 + 2 + 2) * (size_t)max_states + 4 * ((size_t)max_halo + 1);
+    return ((fixed + 15) & ~(size_t)15) + transient_bytes(max_halo);
 }
 
 struct Frame {       // local frame of a brick
@@ -136,11 +146,8 @@ __device__ __forceinline__ float prefilter_thr(float cut2_f, bool any, float mar
 // stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
 // they do not fit (the caller then takes the slow path).  Also returns the number of inner records
 // (the brick's own cells), listed in m.inner.
-// The inner list holds the records whose type bit is set in `first` ahead of the others (stable
-// otherwise), so that waves see particles of like stencil reach.
 __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom &geo, const Frame &F, uint32_t cap,
-                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t first,
-                                uint32_t *n_inner_out) {
+                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t *n_inner_out) {
     __shared__ uint32_t s_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t nh = Hs.nh;
@@ -235,40 +242,23 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
             }
         }
     }
-    __syncthreads();
-    // 4. the brick's own records: one contiguous range per inner (x, y) row, partitioned by class.
-    // Pass 0 counts the first class, pass 1 writes (a packed scan carries both class counts).
+    // 4. the brick's own records: one contiguous range per inner (x, y) row
     const uint32_t e1 = (uint32_t)B.e[1], e2 = (uint32_t)B.e[2];
     const uint32_t nrow = (uint32_t)B.e[0] * e1;
-    uint32_t n_first = 0, n_inner = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        uint32_t base1 = 0, base2 = 0;
-        for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
-            const uint32_t r = r0 + tid;
-            uint32_t a = 0, c = 0, c1 = 0;
-            if (r < nrow) {
-                const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
-                const uint32_t row =
-                    (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
-                a = m.hstart[row];
-                c = m.hstart[row + e2] - a;
-                if (first)
-                    for (uint32_t k = 0; k < c; ++k) c1 += (first >> tw_type(__float_as_uint(m.P[a + k].w))) & 1u;
-            }
-            uint32_t tot;
-            const uint32_t o = block_exclusive_scan(c1 | ((c - c1) << 16), &tot);
-            if (pass == 1) {
-                uint32_t o1 = base1 + (o & 0xFFFFu), o2 = n_first + base2 + (o >> 16);
-                for (uint32_t k = 0; k < c; ++k) {
-                    const bool f = first && ((first >> tw_type(__float_as_uint(m.P[a + k].w))) & 1u);
-                    m.inner[f ? o1++ : o2++] = (uint16_t)(a + k);
-                }
-            }
-            base1 += tot & 0xFFFFu;
-            base2 += tot >> 16;
+    uint32_t n_inner = 0;
+    for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
+        const uint32_t r = r0 + tid;
+        uint32_t a = 0, c = 0;
+        if (r < nrow) {
+            const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
+            const uint32_t row = (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
+            a = m.hstart[row];
+            c = m.hstart[row + e2] - a;
         }
-        n_first = base1;
-        n_inner = base1 + base2;
+        uint32_t tot;
+        const uint32_t o = n_inner + block_exclusive_scan(c, &tot);
+        for (uint32_t k = 0; k < c; ++k) m.inner[o + k] = (uint16_t)(a + k);
+        n_inner += tot;
     }
     __syncthreads();
     *n_inner_out = n_inner;
@@ -446,30 +436,42 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Streams candidate pairs through the wave's ring: every active lane walks the stencil of its own
-// staged record (halo cell hc, local position p, reach) and offers each staged record q to keep(q);
-// survivors are appended
-// as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the wave.
-// Must be called by all 64 lanes of a wave (wave-uniform control flow around it).
+// staged record (halo cell hc, local position p, reach), taking up to BATCH candidates of its
+// current run per iteration (independent LDS reads), and offers each to keep(q); survivors are
+// appended as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the
+// wave.  Must be called by all 64 lanes of a wave (wave-uniform control flow around it).
 template <bool ALL_HALO, class Keep, class Eval>
 __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
                                               uint32_t hc, const float4 &p, float reach, Keep &&keep, Eval &&eval) {
     const uint32_t lane = __lane_id();
+    const unsigned long long lt = lanemask_lt();
     Walk<ALL_HALO> w;
+    w.q = w.qe = 0;
     bool more = false;
     if (active) more = w.begin(c, hc, p, reach, hs);
     uint32_t head = 0, tail = 0;  // wave-uniform
     while (__ballot(more)) {
-        bool pass = false;
-        uint32_t q = 0;
+        uint32_t q0 = 0, nq = 0;
         if (more) {
-            q = w.q++;
-            pass = keep(q);
+            q0 = w.q;
+            nq = min(w.qe - w.q, BATCH);
+            w.q += nq;
             if (w.q >= w.qe) more = w.advance(c, hs);
         }
-        const unsigned long long mask = __ballot(pass);
-        if (pass) ring[(tail + (uint32_t)__popcll(mask & lanemask_lt())) & (RING - 1)] = (lane << 16) | q;
-        tail += (uint32_t)__popcll(mask);
-        if (tail - head >= 64) {
+        // candidates q0 .. q0 + nq - 1; reads past the batch repeat its last record (masked out)
+        const uint32_t last = nq ? q0 + nq - 1u : q0;
+        bool k[BATCH];
+#pragma unroll
+        for (uint32_t i = 0; i < BATCH; ++i) k[i] = keep(min(q0 + i, last)) & (i < nq);
+        uint32_t pos = tail;
+#pragma unroll
+        for (uint32_t i = 0; i < BATCH; ++i) {
+            const unsigned long long mask = __ballot(k[i]);
+            if (k[i]) ring[(pos + (uint32_t)__popcll(mask & lt)) & (RING - 1)] = (lane << 16) | (q0 + i);
+            pos += (uint32_t)__popcll(mask);
+        }
+        tail = pos;
+        while (tail - head >= 64) {
             wave_lds_sync();
             eval(ring[(head + lane) & (RING - 1)]);
             head += 64;
@@ -578,12 +580,6 @@ __device__ __forceinline__ SearchTables load_search_tables(char *base, const Tab
     return t;
 }
 
-// types whose bit is set: reach above `above` (waves are filled with these first)
-__device__ __forceinline__ uint32_t types_above(const float *reach, uint32_t nt, float above) {
-    uint32_t w = 0;
-    for (uint32_t t = 0; t < nt; ++t) w |= (reach[t] > above ? 1u : 0u) << t;
-    return w;
-}
 
 // FP32 min image of a local-frame displacement on whole-axis axes (halo axes need none)
 __device__ __forceinline__ void frame_min_image(const Frame &F, float &dx, float &dy, float &dz) {
@@ -721,7 +717,6 @@ __device__ __forceinline__ void for_each_own_record(const Grid &cur, const Brick
 }
 
 struct ForceWave {  // per-wave LDS of k_brick
-    uint32_t ring[RING];
     unsigned long long acc[3][64];  // fixed-point force of the wave's 64 particles
     uint32_t s[64];                 // their bin slots
 };
@@ -732,8 +727,14 @@ __device__ __forceinline__ void list_slow_brick(uint32_t *list, uint32_t b) {
     if (threadIdx.x == 0) list[2 + atomicAdd(&list[0], 1u)] = b;
 }
 
+// count of listed bricks; with none listed the second kernel's blocks exit at once (and nothing
+// needs clearing)
+__device__ __forceinline__ uint32_t slow_count(const uint32_t *list) {
+    return __atomic_load_n(list, __ATOMIC_RELAXED);
+}
+
 // the last block of the second kernel clears the list for the next launch (every block has read
-// the count before it counts itself done)
+// the nonzero count before it counts itself done)
 __device__ __forceinline__ void release_slow_list(uint32_t *list) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -746,16 +747,18 @@ __device__ __forceinline__ void release_slow_list(uint32_t *list) {
     }
 }
 
-// forces (+ Euler-Maruyama + re-bin) of every particle of one brick; MODE as in k_fused
-template <int MODE>
+// forces (+ Euler-Maruyama + re-bin) of every particle of one brick; MODE as in k_fused.
+// AH: the launch covers bricks brick0.. that are in halo mode on every axis (else: none is).
+template <int MODE, bool AH>
 __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, ReplicaDev *reps, Tables T, Topo topo,
-                                                         const BrickDev *bricks, uint32_t cap, int radius, double dt,
+                                                         const BrickDev *bricks, uint32_t brick0, uint32_t cap,
+                                                         int radius, double dt,
                                                          const double *noise, double *force_out, uint32_t max_states,
                                                          uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
                                                          uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ ForceWave wave_lds[BRICK_WAVES];
-    const uint32_t bidx = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t bidx = brick0 + xcd_block(blockIdx.x, gridDim.x);
     const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
     if (!R.in_use) return;
@@ -784,11 +787,14 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         }
         ft = load_force_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
-        const float half_cell = 0.5f * (float)fmin(geo.cs[0], fmin(geo.cs[1], geo.cs[2]));
-        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, types_above(ft.reach, nt, half_cell), &n_inner);
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
     }
     if (total != 0xFFFFFFFFu) {
         const WalkCtx wc = make_walk(B, geo, Hs, radius);
+        if (AH != (wc.whole == 0)) {  // launch and brick disagree (host invariant): refuse the brick
+            if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
+            return;
+        }
         ForceWave &W = wave_lds[threadIdx.x / 64];
         const uint32_t lane = __lane_id();
         // the exact pair force of a survivor (entry = owner lane << 16 | staged neighbour)
@@ -814,8 +820,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             atomicAdd(&W.acc[2][ol], (unsigned long long)to_fixed(pfz, 4294967296.0f));
             e_blk += to_fixed(pe, 16777216.0f);
         };
-        auto rounds = [&](auto all_halo) {
-            constexpr bool AH = decltype(all_halo)::value;
+        auto rounds = [&]() {
             for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
                 const uint32_t ii = base + lane;
                 bool valid = ii < n_inner;
@@ -851,7 +856,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                     const float thr = thr_row[tw_type(tw2)];
                     return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
                 };
-                pair_pipeline<AH>(wc, m.hstart, W.ring, active, m.hc_of[j], pj, reach, keep, eval);
+                pair_pipeline<AH>(wc, m.hstart, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
                 wave_lds_sync();
                 if (valid)
                     finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, sj, me, true,
@@ -860,8 +865,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                 wave_lds_sync();
             }
         };
-        if (wc.whole == 0) rounds(std::true_type{});
-        else rounds(std::false_type{});
+        rounds();
         block_add_energy(reps, B.r, e_blk);
     } else {
         list_slow_brick(slow, bidx);
@@ -874,7 +878,8 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_slow(Grid cur, Grid nxt
                                                               Topo topo, const BrickDev *bricks, uint32_t cap,
                                                               int radius, double dt, const double *noise,
                                                               double *force_out, uint32_t *slow, uint32_t *err) {
-    const uint32_t n = atomicAdd(&slow[0], 0u);
+    const uint32_t n = slow_count(slow);
+    if (n == 0) return;  // grid-uniform
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // block-uniform
         const BrickDev B = bricks[slow[2 + i]];
         const ReplicaDev &R = reps[B.r];
@@ -893,30 +898,36 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_slow(Grid cur, Grid nxt
 }
 
 struct SearchWave {  // per-wave LDS of k_brick_search
-    uint32_t ring[RING];
     uint32_t s[64];  // bin slots of the wave's 64 particles
 };
 
 // reaction candidate search of one brick on the current buffer (see search_one for the rules),
-// fused with clearing the previous buffer's counters of the brick's cells
-__global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32_t *old_counts, ReplicaDev *reps,
-                                                                Tables T, const BrickDev *bricks, uint32_t cap,
+// fused with clearing the previous buffer (clear_old): the counters of the brick's cells, and (block
+// 0) its work-list and overflow lengths
+template <bool AH>
+__global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid old, int clear_old, ReplicaDev *reps,
+                                                                Tables T, const BrickDev *bricks, uint32_t brick0,
+                                                                uint32_t cap,
                                                                 int radius, int handler, Event *events,
                                                                 uint32_t *ev_count, uint32_t max_states,
                                                                 uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
                                                                 uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ SearchWave wave_lds[BRICK_WAVES];
-    const uint32_t bidx = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t bidx = brick0 + xcd_block(blockIdx.x, gridDim.x);
     const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
     const RepGeom geo = load_geom(R);
-    // clear the old buffer's counters of this brick's cells (the old buffer is not read again)
-    if (old_counts) {
+    // clear the old buffer (it is not read again before it becomes the next target)
+    if (clear_old) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *old.wl_count = 0u;
+            *old.ovf_count = 0u;
+        }
         const uint32_t ncell = (uint32_t)(B.e[0] * B.e[1] * B.e[2]);
         for (uint32_t ci = threadIdx.x; ci < ncell; ci += BRICK_THREADS) {
             const uint32_t lz = ci % B.e[2], ly = (ci / B.e[2]) % B.e[1], lx = ci / (B.e[2] * B.e[1]);
-            old_counts[geo.cell_base +
+            old.counts[geo.cell_base +
                        ((uint32_t)(B.o[0] + lx) * (uint32_t)geo.dims[1] + (uint32_t)(B.o[1] + ly)) *
                            (uint32_t)geo.dims[2] +
                        (uint32_t)(B.o[2] + lz)] = 0u;
@@ -937,8 +948,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
         }
         st = load_search_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
-        // particles with pair reactions first
-        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, types_above(st.reach, nt, -0.5f), &n_inner);
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
     }
     if (total == 0xFFFFFFFFu) {
         list_slow_brick(slow, bidx);
@@ -963,6 +973,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
         }
     };
     const WalkCtx wc = make_walk(B, geo, Hs, radius);
+    if (AH != (wc.whole == 0)) {  // launch and brick disagree (host invariant): refuse the brick
+        if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
+        return;
+    }
     SearchWave &W = wave_lds[threadIdx.x / 64];
     const uint32_t lane = __lane_id();
     // the exact test and draws of a surviving pair (entry = owner lane << 16 | staged neighbour)
@@ -992,8 +1006,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
             else emit(rx, s2, s1, o.h, mh, event_key(handler, x4, X.rate));
         }
     };
-    auto rounds = [&](auto all_halo) {
-        constexpr bool AH = decltype(all_halo)::value;
+    auto rounds = [&]() {
         for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
             const uint32_t ii = base + lane;
             bool valid = ii < n_inner;
@@ -1041,12 +1054,11 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, uint32
                 const uint32_t sq = m.slot[q];
                 return ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & ((t2 > t) | ((t2 == t) & (sq > sj))) & (d2p < thr);
             };
-            pair_pipeline<AH>(wc, m.hstart, W.ring, active, m.hc_of[j], pj, reach, keep, eval);
+            pair_pipeline<AH>(wc, m.hstart, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
             wave_lds_sync();
         }
     };
-    if (wc.whole == 0) rounds(std::true_type{});
-    else rounds(std::false_type{});
+    rounds();
 }
 
 // the listed bricks of k_brick_search, searched straight from the bins
@@ -1054,7 +1066,8 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search_slow(Grid cur, R
                                                                      const BrickDev *bricks, uint32_t cap, int radius,
                                                                      int handler, Event *events, uint32_t *ev_count,
                                                                      uint32_t *slow, uint32_t *err) {
-    const uint32_t n = atomicAdd(&slow[0], 0u);
+    const uint32_t n = slow_count(slow);
+    if (n == 0) return;  // grid-uniform
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const BrickDev B = bricks[slow[2 + i]];
         const RepGeom geo = load_geom(reps[B.r]);

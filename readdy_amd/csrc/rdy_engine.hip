@@ -9,6 +9,7 @@
 #include <cstddef>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace rdy;
@@ -88,6 +89,7 @@ struct rdy_engine {
     BrickDev *d_bricks = nullptr;
     size_t d_bricks_cap = 0;
     uint32_t *d_slow = nullptr;  // two lists of bricks left to the bin-reading kernels (forces, search)
+    uint32_t n_bricks_ah = 0;    // leading bricks of bricks_h in halo mode on every axis
     bool wl_valid[2] = {true, true};
     uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
@@ -468,6 +470,10 @@ int upload_bricks(rdy_engine *e) {
     e->bricks_h.clear();
     for (uint32_t r = 0; r < e->max_reps; ++r)
         if (e->reps_h[r].in_use) make_bricks(e, r, e->bricks_h);
+    // bricks in halo mode on every axis first: they run in their own (specialised) launches
+    auto ah_end = std::stable_partition(e->bricks_h.begin(), e->bricks_h.end(),
+                                        [](const BrickDev &b) { return b.whole == 0; });
+    e->n_bricks_ah = (uint32_t)(ah_end - e->bricks_h.begin());
     if (e->bricks_h.size() > e->d_bricks_cap) {
         HIP_TRY(hipStreamSynchronize(e->stream));
         if (e->d_bricks) HIP_TRY(hipFree(e->d_bricks));
@@ -486,11 +492,19 @@ int upload_bricks(rdy_engine *e) {
 
 uint32_t n_bricks(const rdy_engine *e) { return (uint32_t)std::max<size_t>(e->bricks_h.size(), 1); }
 
+// launch(all_halo, blocks, first_brick) for the all-halo bricks, then for the others
+template <class F>
+void for_brick_groups(const rdy_engine *e, F &&launch) {
+    const uint32_t n = (uint32_t)e->bricks_h.size(), na = e->n_bricks_ah;
+    if (na) launch(std::true_type{}, na, 0u);
+    if (n > na) launch(std::false_type{}, n - na, na);
+}
+
 // list of bricks for the bin-reading forces (0) or search (1) kernel
 uint32_t *slow_list(const rdy_engine *e, int which) { return e->d_slow + (size_t)which * (e->d_bricks_cap + 2); }
 
 // blocks of the bin-reading kernels (they loop over the listed bricks; none listed: they just exit)
-uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 1024u); }
+uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 256u); }
 
 // brick shape and LDS budget from the expected particles per cell (overridable in the config)
 int configure_bricks(rdy_engine *e, double lam) {
@@ -511,8 +525,10 @@ int configure_bricks(rdy_engine *e, double lam) {
         e->lds_limit = (size_t)v;
     // static LDS of the brick kernels comes out of the same per-workgroup budget
     size_t static_lds = 0;
-    const void *kernels[] = {(const void *)k_brick<MODE_STEP>, (const void *)k_brick<MODE_FORCES>,
-                             (const void *)k_brick<MODE_REBIN>, (const void *)k_brick_search};
+    const void *kernels[] = {(const void *)k_brick<MODE_STEP, true>,   (const void *)k_brick<MODE_STEP, false>,
+                             (const void *)k_brick<MODE_FORCES, true>, (const void *)k_brick<MODE_FORCES, false>,
+                             (const void *)k_brick<MODE_REBIN, false>, (const void *)k_brick_search<true>,
+                             (const void *)k_brick_search<false>};
     for (const void *k : kernels) {
         hipFuncAttributes fa;
         if (hipFuncGetAttributes(&fa, k) == hipSuccess) static_lds = std::max(static_lds, (size_t)fa.sharedSizeBytes);
@@ -538,16 +554,9 @@ int configure_bricks(rdy_engine *e, double lam) {
         return set_err(RDY_ERR_INVALID, "brick does not fit in LDS: use a smaller brick");
     e->max_states = want;
     e->brick_smem = tables + brick_smem_bytes(want, e->max_halo);
-    if (e->brick_smem > 65536) {
-        (void)hipFuncSetAttribute((const void *)k_brick<MODE_STEP>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)e->brick_smem);
-        (void)hipFuncSetAttribute((const void *)k_brick<MODE_FORCES>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)e->brick_smem);
-        (void)hipFuncSetAttribute((const void *)k_brick<MODE_REBIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)e->brick_smem);
-        (void)hipFuncSetAttribute((const void *)k_brick_search, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)e->brick_smem);
-    }
+    if (e->brick_smem > 65536)
+        for (const void *k : kernels)
+            (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->brick_smem);
     return RDY_OK;
 }
 
@@ -590,11 +599,12 @@ void ensure_wl(rdy_engine *e) {
 // clears buffer b's cell counters (brick-wise) and its work-list / overflow lengths
 void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
     if (!e->bricks_h.empty())
-        hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
-                           e->grid(grid_of_bricks_buffer), e->counts[b], e->reps, e->T, (const BrickDev *)e->d_bricks,
-                           e->cap, e->radius, 2, e->events, e->ev_count, e->max_states, e->max_halo, 0u, slow_list(e, 1),
-                           e->err);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
+        hipLaunchKernelGGL(k_brick_search<false>, dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
+                           e->grid(grid_of_bricks_buffer), e->grid(b), 1, e->reps, e->T, (const BrickDev *)e->d_bricks,
+                           0u, e->cap, e->radius, 2, e->events, e->ev_count, e->max_states, e->max_halo, 0u,
+                           slow_list(e, 1), e->err);
+    else
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
 }
 
 // the old buffer becomes the next target: zero its cell counters, then its work-list length
@@ -614,10 +624,12 @@ void launch_step(rdy_engine *e, int handler, double dt) {
     const int c = e->cur, n = 1 - c;
     if (e->prof && e->prof_used < e->prof_ev.size())
         (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
-    hipLaunchKernelGGL(k_brick<MODE_STEP>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
-                       e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
-                       slow_list(e, 0), e->err);
+    for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
+        hipLaunchKernelGGL((k_brick<MODE_STEP, decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
+                           e->stream, e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks,
+                           b0, e->cap, e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states,
+                           e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
+    });
     if (e->prof && e->prof_used < e->prof_ev.size()) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
@@ -627,14 +639,16 @@ void launch_step(rdy_engine *e, int handler, double dt) {
                        (const double *)nullptr, (double *)nullptr, slow_list(e, 0), e->err);
     // reaction search on the new buffer, fused with clearing the old buffer's counters
     const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
-    hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(n),
-                       e->counts[c], e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events,
-                       e->ev_count, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
+        hipLaunchKernelGGL((k_brick_search<decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
+                           e->stream, e->grid(n), e->grid(c), 1, e->reps, e->T, (const BrickDev *)e->d_bricks, b0,
+                           e->cap, e->radius, h, e->events, e->ev_count, e->max_states, e->max_halo,
+                           (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    });
     if (h != RDY_HANDLER_NONE)
         hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(n),
                            e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events, e->ev_count,
                            slow_list(e, 1), e->err);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(c));
     if (h != RDY_HANDLER_NONE)
         hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(n), e->reps, e->T,
                            e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 1, (uint32_t)e->max_particles,
@@ -648,10 +662,12 @@ void launch_forces(rdy_engine *e, double *force_out) {
     hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded);
     launch_bonded(e);
     const int c = e->cur;
-    hipLaunchKernelGGL(k_brick<MODE_FORCES>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
-                       e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
-                       slow_list(e, 0), e->err);
+    for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
+        hipLaunchKernelGGL((k_brick<MODE_FORCES, decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
+                           e->stream, e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks,
+                           b0, e->cap, e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo,
+                           (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
+    });
     hipLaunchKernelGGL(k_brick_slow<MODE_FORCES>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream,
                        e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
                        e->radius, 0.0, (const double *)nullptr, force_out, slow_list(e, 0), e->err);
@@ -1079,9 +1095,9 @@ int rdy_update_neighbor_list(rdy_engine *e) {
     int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
     if (rc) return rc;
     const int c = e->cur, n = 1 - c;
-    hipLaunchKernelGGL(k_brick<MODE_REBIN>, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream,
-                       e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, (uint32_t)e->brick_smem,
+    hipLaunchKernelGGL((k_brick<MODE_REBIN, false>), dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
+                       e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, 0u, e->cap,
+                       e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, 0u,
                        slow_list(e, 0), e->err);
     launch_clear_bricks(e, c, n);
     e->wl_valid[n] = false;
@@ -1145,9 +1161,12 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
     if (e->n_rx == 0) return RDY_OK;
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
     const int c = e->cur;
-    hipLaunchKernelGGL(k_brick_search, dim3(n_bricks(e)), dim3(BRICK_THREADS), e->brick_smem, e->stream, e->grid(c),
-                       (uint32_t *)nullptr, e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler,
-                       e->events, e->ev_count, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
+        hipLaunchKernelGGL((k_brick_search<decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
+                           e->stream, e->grid(c), e->grid(c), 0, e->reps, e->T, (const BrickDev *)e->d_bricks, b0,
+                           e->cap, e->radius, handler, e->events, e->ev_count, e->max_states, e->max_halo,
+                           (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
+    });
     hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
                        e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler, e->events,
                        e->ev_count, slow_list(e, 1), e->err);
