@@ -87,6 +87,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (a known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build measures a single MI355X (--gpus {args.gpus} requested). "
@@ -96,6 +99,15 @@ def main():
 
     from readdy_amd import Engine, device_info
     from readdy_amd import _lib as L
+
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     pos, types = c1_system()
     pots, rxs = c1_chemistry(L)
@@ -134,7 +146,7 @@ def main():
                    "box": 200.0, "handler": "UncontrolledApproximation", "graph": graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused<MODE_STEP> (forces + Euler-Maruyama + re-bin)",
+                     "kernel": "k_brick<MODE_STEP, true> (forces + Euler-Maruyama + re-bin of one brick)",
                      "avg_kernel_us": avg_us, "launches": launches},
         "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
         "device": device_info()["name"],
