@@ -1,0 +1,505 @@
+// The "MI355X" kernel plugin (libreaddy_amd_plugin.so): a readdy_amd::model::Kernel whose state
+// model and actions drive the HIP engine through its C ABI (include/readdy_amd.h).
+//
+// Plugin entry points (the reference's route for kernel libraries, KernelProvider.cpp:135-153,
+// CPUKernel.cpp:89-95): extern "C" const char* name() and extern "C" Kernel* createKernel().
+//
+// Particle ids are the engine's replica-local handles: particles uploaded in one batch get handles
+// 0..n-1 (topology particles first, in topology order), reaction products fresh handles
+// (Particle::nextId, Particle.h:112-114).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/readdy_amd.h"
+#include "readdy_host.h"
+
+namespace readdy_amd {
+namespace kernel {
+
+using namespace model;
+
+static void ok(int rc, const char *what) {
+    if (rc != RDY_OK) {
+        const std::string msg = std::string(what) + ": " + rdy_last_error();
+        if (rc == RDY_ERR_INVALID) throw std::invalid_argument(msg);
+        if (rc == RDY_ERR_STATE) throw std::logic_error(msg);
+        throw DeviceError(msg);
+    }
+}
+
+class MI355XKernel;
+
+class MI355XStateModel : public StateModel {
+public:
+    explicit MI355XStateModel(MI355XKernel *k) : _k(k) {}
+    void addParticle(const Particle &p) override { addParticles({p}); }
+    void addParticles(const std::vector<Particle> &ps) override;
+    std::vector<Particle> getParticles() const override;
+    std::vector<Vec3> getParticlePositions() const override;
+    std::vector<Vec3> getParticleForces() const override;
+    void initializeNeighborList(scalar) override;
+    void updateNeighborList() override;
+    void clearNeighborList() override {}
+    scalar energy() const override;
+    scalar time() const override { return _time; }
+    void setTime(scalar t) override { _time = t; }
+    GraphTopology *addTopology(const std::string &type, const std::vector<Particle> &particles) override;
+    std::vector<GraphTopology *> getTopologies() override {
+        std::vector<GraphTopology *> v;
+        for (auto &t : _topologies) v.push_back(t.get());
+        return v;
+    }
+    void configureTopologies() override;
+    std::vector<uint64_t> reactionCounts() const override;
+    void clear() override;
+
+    // uploads the pending particles (topology particles first) if the device copy is stale
+    void flush();
+
+private:
+    MI355XKernel *_k;
+    scalar _time = 0;
+    std::vector<Particle> _topoParticles, _freeParticles;  // pending host copies (before the first flush)
+    std::vector<std::unique_ptr<GraphTopology>> _topologies;
+    bool _dirty = false, _uploaded = false, _topoDirty = false;
+};
+
+class MI355XKernel : public Kernel {
+public:
+    MI355XKernel() : Kernel("MI355X"), _state(this), _actions(this) {
+        // the reference passes no configuration to third-party kernels (KernelConfiguration.cpp:91-98),
+        // so the Philox key comes from the environment
+        if (const char *s = std::getenv("READDY_AMD_SEED")) _seed = std::strtoull(s, nullptr, 0);
+    }
+    ~MI355XKernel() override {
+        if (_engine) rdy_engine_destroy(_engine);
+    }
+    StateModel &stateModel() override { return _state; }
+    const StateModel &stateModel() const override { return _state; }
+    ActionFactory &actions() override { return _actions; }
+    bool supportsGillespie() const override { return true; }
+    void initialize() override { engine(); }
+    bool runFused(const std::string &scheduler, scalar dt, std::size_t steps) override {
+        engine();
+        _state.flush();
+        const int h = scheduler == "Gillespie" ? RDY_HANDLER_GILLESPIE
+                      : scheduler == "UncontrolledApproximation" ? RDY_HANDLER_UNCONTROLLED_APPROXIMATION
+                                                                 : RDY_HANDLER_NONE;
+        ok(rdy_run_graph(_engine, h, dt, steps), "run");
+        return true;
+    }
+
+    // the engine and its single replica, created from the context on first use
+    rdy_engine *engine() {
+        if (_engine) return _engine;
+        const Context &c = context();
+        const auto &types = c.particleTypes().types();
+        if (types.empty()) throw std::logic_error("the context has no particle types");
+        std::vector<double> D;
+        std::vector<uint8_t> flavor;
+        for (const auto &t : types) {
+            D.push_back(t.diffusionConstant);
+            flavor.push_back(t.topologyFlavor ? 1 : 0);
+        }
+        std::vector<rdy_pair_potential> pots;
+        for (const auto &p : c.potentials().potentialsOrder2()) {
+            rdy_pair_potential q{};
+            q.kind = p.kind;
+            q.type1 = p.type1;
+            q.type2 = p.type2;
+            for (int i = 0; i < 8; ++i) q.params[i] = p.params[i];
+            pots.push_back(q);
+        }
+        std::vector<rdy_reaction> rxs;
+        for (const auto &r : c.reactions().reactions()) {
+            rdy_reaction q{};
+            switch (r.type) {
+            case ReactionType::Conversion: q.kind = RDY_RX_CONVERSION; break;
+            case ReactionType::Fusion: q.kind = RDY_RX_FUSION; break;
+            case ReactionType::Fission: q.kind = RDY_RX_FISSION; break;
+            case ReactionType::Enzymatic: q.kind = RDY_RX_ENZYMATIC; break;
+            case ReactionType::Decay: q.kind = RDY_RX_DECAY; break;
+            }
+            for (std::size_t i = 0; i < r.educts.size() && i < 2; ++i) q.educts[i] = r.educts[i];
+            for (std::size_t i = 0; i < r.products.size() && i < 2; ++i) q.products[i] = r.products[i];
+            q.rate = r.rate;
+            q.educt_distance = r.eductDistance;
+            q.product_distance = r.productDistance;
+            q.weight1 = r.weight1;
+            q.weight2 = r.weight2;
+            rxs.push_back(q);
+        }
+        rdy_context_desc cd{};
+        cd.n_types = (uint32_t)types.size();
+        cd.kbt = c.kBT();
+        cd.diffusion = D.data();
+        cd.topology_flavor = flavor.data();
+        cd.n_potentials = (uint32_t)pots.size();
+        cd.potentials = pots.data();
+        cd.n_reactions = (uint32_t)rxs.size();
+        cd.reactions = rxs.data();
+        rdy_engine_config cfg{};
+        cfg.nl_cutoff = _nlCutoff;
+        cfg.max_replicas = 1;
+        ok(rdy_engine_create(&cd, &cfg, &_engine), "engine");
+        return _engine;
+    }
+
+    // adds the single replica for n particles (box and periodicity from the context)
+    void replica(uint64_t n) {
+        if (_hasReplica) return;
+        rdy_replica_desc d{};
+        for (int a = 0; a < 3; ++a) {
+            d.box[a] = context().boxSize()[a];
+            d.pbc[a] = context().periodicBoundaryConditions()[a] ? 1 : 0;
+        }
+        d.seed = _seed;
+        d.max_particles = std::max<uint64_t>(2 * n, 1024);
+        ok(rdy_replica_add(engine(), &d, &_replica), "replica");
+        _hasReplica = true;
+    }
+
+    uint32_t replicaId() const { return _replica; }
+    bool hasReplica() const { return _hasReplica; }
+    void setSeed(uint64_t s) { _seed = s; }
+    void setNeighborListCutoff(scalar c) {
+        if (_engine) throw std::logic_error("the neighbour-list cutoff is fixed once the engine exists");
+        _nlCutoff = c;
+    }
+
+private:
+    MI355XStateModel _state;
+
+public:
+    class Actions : public ActionFactory {
+    public:
+        explicit Actions(MI355XKernel *k) : _k(k) {}
+        std::unique_ptr<Action> addParticles(const std::vector<Particle> &ps) override;
+        std::unique_ptr<TimeStepDependentAction> eulerBDIntegrator(scalar dt) override;
+        std::unique_ptr<Action> calculateForces() override;
+        std::unique_ptr<Action> createNeighborList(scalar d) override;
+        std::unique_ptr<Action> updateNeighborList() override;
+        std::unique_ptr<Action> clearNeighborList() override;
+        std::unique_ptr<Action> evaluateCompartments() override;
+        std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) override;
+        std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) override;
+        std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) override;
+        std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) override;
+        std::unique_ptr<Action> initializeKernel() override;
+
+    private:
+        MI355XKernel *_k;
+    };
+
+private:
+    Actions _actions;
+    rdy_engine *_engine = nullptr;
+    uint32_t _replica = 0;
+    bool _hasReplica = false;
+    uint64_t _seed = 0;
+    scalar _nlCutoff = 0;  // 0: Context::calculateMaxCutoff
+
+    friend class MI355XStateModel;
+};
+
+// ---- state model -----------------------------------------------------------------------------------
+void MI355XStateModel::addParticles(const std::vector<Particle> &ps) {
+    const auto &reg = _k->context().particleTypes();
+    for (const auto &p : ps) {
+        if (reg.infoOf(p.type()).topologyFlavor)
+            throw std::invalid_argument("topology particles are added through addTopology");
+        _freeParticles.push_back(p);
+    }
+    _dirty = true;
+}
+
+GraphTopology *MI355XStateModel::addTopology(const std::string &type, const std::vector<Particle> &particles) {
+    const auto &reg = _k->context().particleTypes();
+    std::vector<std::size_t> idx;
+    for (const auto &p : particles) {
+        if (!reg.infoOf(p.type()).topologyFlavor)
+            throw std::invalid_argument("a topology holds particles of topology species only");
+        idx.push_back(_topoParticles.size());
+        _topoParticles.push_back(p);
+    }
+    _topologies.push_back(std::make_unique<GraphTopology>(type, idx));
+    _dirty = _topoDirty = true;
+    return _topologies.back().get();
+}
+
+void MI355XStateModel::flush() {
+    if (!_dirty) return;
+    // particles already on the device stay; their current state is read back and re-uploaded
+    // together with the new ones (the upload assigns handles in order)
+    std::vector<Particle> all;
+    if (_uploaded) {
+        // re-uploading renumbers handles in upload order, which would break the bonded lists
+        if (!_topoParticles.empty())
+            throw std::logic_error("with topologies present, particles and topologies must be added before the "
+                                   "first action runs");
+        all = getParticles();
+    }
+    std::vector<double> pos;
+    std::vector<uint32_t> types;
+    auto put = [&](const Particle &p) {
+        pos.push_back(p.pos().x);
+        pos.push_back(p.pos().y);
+        pos.push_back(p.pos().z);
+        types.push_back(p.type());
+    };
+    if (!_uploaded)
+        for (const auto &p : _topoParticles) put(p);
+    for (const auto &p : all) put(p);
+    for (const auto &p : _freeParticles) put(p);
+    _k->replica(types.size());
+    ok(rdy_set_particles(_k->engine(), _k->replicaId(), types.size(), pos.data(), types.data()), "set particles");
+    _freeParticles.clear();
+    _dirty = false;
+    _uploaded = true;
+    if (_topoDirty) configureTopologies();
+}
+
+// GraphTopology::configure (GraphTopology.cpp:59-143) for every topology, flattened per replica
+void MI355XStateModel::configureTopologies() {
+    const auto &cfg = _k->context().topologyRegistry();
+    std::vector<uint32_t> b2, a3, d4, mult;
+    std::vector<double> bk, bl, ak, at, dk, dp;
+    for (const auto &top : _topologies) {
+        const auto &pi = top->particleIndices();
+        auto type = [&](std::size_t v) { return _topoParticles[pi[v]].type(); };
+        top->findNTuples(
+            [&](std::size_t v1, std::size_t v2) {
+                auto it = cfg.bonds.find({type(v1), type(v2)});
+                if (it == cfg.bonds.end())
+                    throw std::invalid_argument("topology edge " + std::to_string(pi[v1]) + " -- " +
+                                                std::to_string(pi[v2]) + " has no bond configured");
+                for (const auto &c : it->second) {
+                    b2.push_back((uint32_t)pi[v1]);
+                    b2.push_back((uint32_t)pi[v2]);
+                    bk.push_back(c.forceConstant);
+                    bl.push_back(c.length);
+                }
+            },
+            [&](std::size_t v1, std::size_t v2, std::size_t v3) {
+                auto it = cfg.angles.find({type(v1), type(v2), type(v3)});
+                if (it == cfg.angles.end()) return;
+                for (const auto &c : it->second) {
+                    a3.insert(a3.end(), {(uint32_t)pi[v1], (uint32_t)pi[v2], (uint32_t)pi[v3]});
+                    ak.push_back(c.forceConstant);
+                    at.push_back(c.equilibriumAngle);
+                }
+            },
+            [&](std::size_t v1, std::size_t v2, std::size_t v3, std::size_t v4) {
+                auto it = cfg.torsions.find({type(v1), type(v2), type(v3), type(v4)});
+                if (it == cfg.torsions.end()) return;
+                for (const auto &c : it->second) {
+                    d4.insert(d4.end(), {(uint32_t)pi[v1], (uint32_t)pi[v2], (uint32_t)pi[v3], (uint32_t)pi[v4]});
+                    dk.push_back(c.forceConstant);
+                    mult.push_back(c.multiplicity);
+                    dp.push_back(c.phi_0);
+                }
+            });
+    }
+    if (!_uploaded) {
+        flush();  // uploads and comes back here
+        return;
+    }
+    rdy_engine *e = _k->engine();
+    const uint32_t r = _k->replicaId();
+    ok(rdy_set_bonds(e, r, bk.size(), b2.data(), bk.data(), bl.data()), "bonds");
+    ok(rdy_set_angles(e, r, ak.size(), a3.data(), ak.data(), at.data()), "angles");
+    ok(rdy_set_dihedrals(e, r, dk.size(), d4.data(), dk.data(), mult.data(), dp.data()), "dihedrals");
+    _topoDirty = false;
+}
+
+std::vector<Particle> MI355XStateModel::getParticles() const {
+    std::vector<Particle> out;
+    if (!_k->hasReplica()) {
+        out = _topoParticles;
+        out.insert(out.end(), _freeParticles.begin(), _freeParticles.end());
+        return out;
+    }
+    rdy_engine *e = const_cast<MI355XKernel *>(_k)->engine();
+    const uint64_t n = rdy_particle_count(e, _k->replicaId());
+    std::vector<double> pos(3 * n + 3);
+    std::vector<uint32_t> types(n + 1), handles(n + 1);
+    uint64_t got = 0;
+    ok(rdy_get_particles(e, _k->replicaId(), n, pos.data(), types.data(), handles.data(), &got), "get particles");
+    out.reserve(got);
+    for (uint64_t i = 0; i < got; ++i)
+        out.emplace_back(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], (ParticleTypeId)types[i], handles[i]);
+    // the device export lists particles in no particular order; ordering by id makes repeated
+    // calls (and getParticleForces) agree
+    std::sort(out.begin(), out.end(), [](const Particle &a, const Particle &b) { return a.id() < b.id(); });
+    return out;
+}
+
+std::vector<Vec3> MI355XStateModel::getParticlePositions() const {
+    std::vector<Vec3> v;
+    for (const auto &p : getParticles()) v.push_back(p.pos());
+    return v;
+}
+
+// forces of the last force evaluation, ordered by particle id like getParticles()
+std::vector<Vec3> MI355XStateModel::getParticleForces() const {
+    std::vector<Vec3> v;
+    if (!_k->hasReplica()) return v;
+    rdy_engine *e = const_cast<MI355XKernel *>(_k)->engine();
+    const uint64_t n = rdy_particle_count(e, _k->replicaId());
+    std::vector<double> f(3 * n + 3);
+    std::vector<uint32_t> handles(n + 1);
+    uint64_t got = 0;
+    ok(rdy_get_forces(e, _k->replicaId(), n, f.data(), handles.data(), &got), "get forces");
+    std::vector<uint64_t> order(got);
+    for (uint64_t i = 0; i < got; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) { return handles[a] < handles[b]; });
+    v.reserve(got);
+    for (uint64_t i : order) v.emplace_back(f[3 * i], f[3 * i + 1], f[3 * i + 2]);
+    return v;
+}
+
+void MI355XStateModel::initializeNeighborList(scalar) {
+    flush();
+    ok(rdy_update_neighbor_list(_k->engine()), "neighbour list");
+}
+
+void MI355XStateModel::updateNeighborList() {
+    flush();
+    ok(rdy_update_neighbor_list(_k->engine()), "neighbour list");
+}
+
+scalar MI355XStateModel::energy() const {
+    if (!_k->hasReplica()) return 0;
+    double en = 0;
+    ok(rdy_get_energy(const_cast<MI355XKernel *>(_k)->engine(), _k->replicaId(), &en), "energy");
+    return en;
+}
+
+std::vector<uint64_t> MI355XStateModel::reactionCounts() const {
+    const auto n = _k->context().reactions().reactions().size();
+    std::vector<uint64_t> c(n, 0);
+    if (_k->hasReplica() && n)
+        ok(rdy_get_reaction_counts(const_cast<MI355XKernel *>(_k)->engine(), _k->replicaId(), c.data()),
+           "reaction counts");
+    return c;
+}
+
+void MI355XStateModel::clear() {
+    if (_k->hasReplica()) ok(rdy_set_particles(_k->engine(), _k->replicaId(), 0, nullptr, nullptr), "clear");
+    _topoParticles.clear();
+    _freeParticles.clear();
+    _topologies.clear();
+    _dirty = _topoDirty = false;
+    _time = 0;
+}
+
+// ---- actions ---------------------------------------------------------------------------------------
+namespace {
+
+template <class F>
+class FnAction : public Action {
+public:
+    explicit FnAction(F f) : _f(std::move(f)) {}
+    void perform() override { _f(); }
+
+private:
+    F _f;
+};
+
+template <class F>
+class FnStepAction : public TimeStepDependentAction {
+public:
+    FnStepAction(scalar dt, F f) : TimeStepDependentAction(dt), _f(std::move(f)) {}
+    void perform() override { _f(_dt); }
+
+private:
+    F _f;
+};
+
+template <class F>
+std::unique_ptr<Action> action(F f) {
+    return std::make_unique<FnAction<F>>(std::move(f));
+}
+
+template <class F>
+std::unique_ptr<TimeStepDependentAction> step_action(scalar dt, F f) {
+    return std::make_unique<FnStepAction<F>>(dt, std::move(f));
+}
+
+}  // namespace
+
+std::unique_ptr<Action> MI355XKernel::Actions::addParticles(const std::vector<Particle> &ps) {
+    return action([k = _k, ps] { k->stateModel().addParticles(ps); });
+}
+
+// CPUEulerBDIntegrator::perform (CPUEulerBDIntegrator.cpp:51-103), Philox normal stream
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::eulerBDIntegrator(scalar dt) {
+    return step_action(dt, [k = _k](scalar h) {
+        k->_state.flush();
+        ok(rdy_integrate(k->engine(), h, nullptr, 0), "integrate");
+    });
+}
+
+// CPUCalculateForces::perform (CPUCalculateForces.cpp:46-212)
+std::unique_ptr<Action> MI355XKernel::Actions::calculateForces() {
+    return action([k = _k] {
+        k->_state.flush();
+        ok(rdy_calculate_forces(k->engine()), "forces");
+    });
+}
+
+std::unique_ptr<Action> MI355XKernel::Actions::createNeighborList(scalar d) {
+    return action([k = _k, d] { k->stateModel().initializeNeighborList(d); });
+}
+
+std::unique_ptr<Action> MI355XKernel::Actions::updateNeighborList() {
+    return action([k = _k] { k->stateModel().updateNeighborList(); });
+}
+
+std::unique_ptr<Action> MI355XKernel::Actions::clearNeighborList() {
+    return action([] {});
+}
+
+std::unique_ptr<Action> MI355XKernel::Actions::evaluateCompartments() {
+    return action([] {});  // no compartments are registered in this context
+}
+
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::uncontrolledApproximation(scalar dt) {
+    return step_action(dt, [k = _k](scalar h) {
+        k->_state.flush();
+        ok(rdy_react(k->engine(), RDY_HANDLER_UNCONTROLLED_APPROXIMATION, h), "uncontrolled approximation");
+    });
+}
+
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::gillespie(scalar dt) {
+    return step_action(dt, [k = _k](scalar h) {
+        k->_state.flush();
+        ok(rdy_react(k->engine(), RDY_HANDLER_GILLESPIE, h), "gillespie");
+    });
+}
+
+// the CPU kernel does not provide DetailedBalance either (SCPU-only in the reference)
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::detailedBalance(scalar) {
+    throw std::invalid_argument("DetailedBalance reaction handler is not available on the MI355X kernel");
+}
+
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::evaluateTopologyReactions(scalar dt) {
+    return step_action(dt, [](scalar) {});  // no topology reactions are registered in this context
+}
+
+std::unique_ptr<Action> MI355XKernel::Actions::initializeKernel() {
+    return action([k = _k] { k->initialize(); });
+}
+
+}  // namespace kernel
+}  // namespace readdy_amd
+
+// ---- plugin entry points -------------------------------------------------------------------------------
+extern "C" const char *name() { return "MI355X"; }
+
+extern "C" readdy_amd::model::Kernel *createKernel() { return new readdy_amd::kernel::MI355XKernel(); }

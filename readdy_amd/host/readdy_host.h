@@ -1,0 +1,373 @@
+// Host-side C++ core of the MI355X ReaDDy kernel: the object contract a ReaDDy simulation talks to
+// (Context, Kernel, StateModel, ActionFactory / Action, KernelProvider, SimulationLoop), shaped like
+// the reference's readdy::model / readdy::plugin / readdy::api classes so that a caller written
+// against those reads the same here.  Every device operation goes through the C ABI of
+// libreaddy_amd.so (include/readdy_amd.h); this layer only owns the host bookkeeping (registries,
+// particle-id mapping, topology graphs) and turns status codes into exceptions.
+//
+// Reference interfaces mirrored (paths relative to the readdy/readdy tree):
+//   Context                 include/readdy/model/Context.h:69-261
+//   ParticleTypeRegistry    include/readdy/model/_internal/ParticleTypeRegistry.h:60-199
+//   PotentialRegistry       include/readdy/model/potentials/PotentialRegistry.h:55-380
+//   ReactionRegistry        include/readdy/model/reactions/ReactionRegistry.h:59-210
+//   TopologyRegistry (potential configuration only)
+//                           include/readdy/model/topologies/TopologyRegistry.h, api/PotentialConfiguration.h:136-161
+//   Kernel                  include/readdy/model/Kernel.h:71-235
+//   StateModel              include/readdy/model/StateModel.h:66-117
+//   Action / ActionFactory  include/readdy/model/actions/Action.h:58-79, ActionFactory.h:93-128
+//   KernelProvider          include/readdy/plugin/KernelProvider.h, readdy/main/plugin/KernelProvider.cpp:68-209
+//   SimulationLoop          include/readdy/api/SimulationLoop.h:71-306
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+namespace readdy_amd {
+
+using scalar = double;
+
+struct Vec3 {
+    scalar x = 0, y = 0, z = 0;
+    Vec3() = default;
+    Vec3(scalar a, scalar b, scalar c) : x(a), y(b), z(c) {}
+    scalar operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+using ParticleTypeId = uint16_t;  // common.h:68
+using ParticleId = uint64_t;      // common.h:69
+
+namespace model {
+
+// ---- registries ---------------------------------------------------------------------------------
+struct ParticleTypeInfo {
+    std::string name;
+    ParticleTypeId typeId;
+    scalar diffusionConstant;
+    bool topologyFlavor;  // particleflavor::TOPOLOGY (ParticleTypeRegistry.h:52-58)
+};
+
+class ParticleTypeRegistry {
+public:
+    void add(const std::string &name, scalar diffusionConstant, bool topologyFlavor = false);
+    void addTopologyType(const std::string &name, scalar diffusionConstant) { add(name, diffusionConstant, true); }
+    ParticleTypeId idOf(const std::string &name) const;
+    const std::string &nameOf(ParticleTypeId id) const;
+    const ParticleTypeInfo &infoOf(ParticleTypeId id) const;
+    scalar diffusionConstantOf(const std::string &name) const { return infoOf(idOf(name)).diffusionConstant; }
+    std::size_t nTypes() const { return _types.size(); }
+    const std::vector<ParticleTypeInfo> &types() const { return _types; }
+
+private:
+    std::vector<ParticleTypeInfo> _types;
+    std::unordered_map<std::string, ParticleTypeId> _byName;
+};
+
+struct PairPotentialDesc {
+    uint32_t kind;  // RDY_POT_*
+    ParticleTypeId type1, type2;
+    std::array<scalar, 8> params{};
+    scalar cutoff() const;
+};
+
+class PotentialRegistry {
+public:
+    explicit PotentialRegistry(const ParticleTypeRegistry *types) : _types(types) {}
+    void rebind(const ParticleTypeRegistry *types) { _types = types; }  // after a Context copy
+    // PotentialRegistry.h:97-160 (order 2)
+    void addHarmonicRepulsion(const std::string &t1, const std::string &t2, scalar forceConstant,
+                              scalar interactionDistance);
+    void addWeakInteractionPiecewiseHarmonic(const std::string &t1, const std::string &t2, scalar forceConstant,
+                                             scalar desiredDistance, scalar depth, scalar cutoff);
+    void addLennardJones(const std::string &t1, const std::string &t2, unsigned m, unsigned n, scalar cutoff,
+                         bool shift, scalar epsilon, scalar sigma);
+    void addScreenedElectrostatics(const std::string &t1, const std::string &t2, scalar electrostaticStrength,
+                                   scalar inverseScreeningDepth, scalar repulsionStrength, scalar repulsionDistance,
+                                   unsigned exponent, scalar cutoff);
+    const std::vector<PairPotentialDesc> &potentialsOrder2() const { return _order2; }
+
+private:
+    const ParticleTypeRegistry *_types;
+    std::vector<PairPotentialDesc> _order2;
+};
+
+enum class ReactionType { Conversion, Fusion, Fission, Enzymatic, Decay };  // Reaction.h:60-61
+
+struct ReactionDesc {
+    std::string name;
+    ReactionType type;
+    std::vector<ParticleTypeId> educts, products;
+    scalar rate = 0, eductDistance = 0, productDistance = 0, weight1 = .5, weight2 = .5;
+};
+
+class ReactionRegistry {
+public:
+    explicit ReactionRegistry(const ParticleTypeRegistry *types) : _types(types) {}
+    void rebind(const ParticleTypeRegistry *types) { _types = types; }  // after a Context copy
+    // ReactionRegistry.h:106-158
+    void addConversion(const std::string &name, const std::string &from, const std::string &to, scalar rate);
+    void addEnzymatic(const std::string &name, const std::string &catalyst, const std::string &from,
+                      const std::string &to, scalar rate, scalar eductDistance);
+    void addFission(const std::string &name, const std::string &from, const std::string &to1,
+                    const std::string &to2, scalar rate, scalar productDistance, scalar weight1 = .5,
+                    scalar weight2 = .5);
+    void addFusion(const std::string &name, const std::string &from1, const std::string &from2,
+                   const std::string &to, scalar rate, scalar eductDistance, scalar weight1 = .5,
+                   scalar weight2 = .5);
+    void addDecay(const std::string &name, const std::string &type, scalar rate);
+    const std::vector<ReactionDesc> &reactions() const { return _reactions; }
+    std::size_t idOf(const std::string &name) const;
+
+private:
+    const ParticleTypeRegistry *_types;
+    std::vector<ReactionDesc> _reactions;
+};
+
+struct BondConfiguration { scalar forceConstant, length; };                         // PotentialConfiguration.h
+struct AngleConfiguration { scalar forceConstant, equilibriumAngle; };
+struct TorsionConfiguration { scalar forceConstant; unsigned multiplicity; scalar phi_0; };
+
+class TopologyRegistry {
+public:
+    explicit TopologyRegistry(const ParticleTypeRegistry *types) : _types(types) {}
+    void rebind(const ParticleTypeRegistry *types) { _types = types; }  // after a Context copy
+    void configureBondPotential(const std::string &t1, const std::string &t2, BondConfiguration cfg);
+    void configureAnglePotential(const std::string &t1, const std::string &t2, const std::string &t3,
+                                 AngleConfiguration cfg);
+    void configureTorsionPotential(const std::string &t1, const std::string &t2, const std::string &t3,
+                                   const std::string &t4, TorsionConfiguration cfg);
+    // the potential configuration, keyed by ordered type tuples (both directions are registered)
+    std::map<std::tuple<ParticleTypeId, ParticleTypeId>, std::vector<BondConfiguration>> bonds;
+    std::map<std::tuple<ParticleTypeId, ParticleTypeId, ParticleTypeId>, std::vector<AngleConfiguration>> angles;
+    std::map<std::tuple<ParticleTypeId, ParticleTypeId, ParticleTypeId, ParticleTypeId>,
+             std::vector<TorsionConfiguration>>
+        torsions;
+
+private:
+    const ParticleTypeRegistry *_types;
+};
+
+class Context {
+public:
+    Context();
+    Context(const Context &o);
+    Context &operator=(const Context &o);
+    ParticleTypeRegistry &particleTypes() { return _types; }
+    const ParticleTypeRegistry &particleTypes() const { return _types; }
+    PotentialRegistry &potentials() { return _potentials; }
+    const PotentialRegistry &potentials() const { return _potentials; }
+    ReactionRegistry &reactions() { return _reactions; }
+    const ReactionRegistry &reactions() const { return _reactions; }
+    TopologyRegistry &topologyRegistry() { return _topologies; }
+    const TopologyRegistry &topologyRegistry() const { return _topologies; }
+    std::array<scalar, 3> &boxSize() { return _box; }
+    const std::array<scalar, 3> &boxSize() const { return _box; }
+    std::array<bool, 3> &periodicBoundaryConditions() { return _pbc; }
+    const std::array<bool, 3> &periodicBoundaryConditions() const { return _pbc; }
+    scalar &kBT() { return _kbt; }
+    scalar kBT() const { return _kbt; }
+    // Context::calculateMaxCutoff (Context.cpp:126-144): pair potentials and order-2 educt distances
+    scalar calculateMaxCutoff() const;
+
+private:
+    void rebind();
+    ParticleTypeRegistry _types;
+    PotentialRegistry _potentials{&_types};
+    ReactionRegistry _reactions{&_types};
+    TopologyRegistry _topologies{&_types};
+    std::array<scalar, 3> _box{{1, 1, 1}};
+    std::array<bool, 3> _pbc{{true, true, true}};
+    scalar _kbt = 1;
+};
+
+class Particle {  // Particle.h:60-120
+public:
+    Particle(scalar x, scalar y, scalar z, ParticleTypeId type, ParticleId id = 0) : _pos(x, y, z), _type(type), _id(id) {}
+    Particle(Vec3 pos, ParticleTypeId type, ParticleId id = 0) : _pos(pos), _type(type), _id(id) {}
+    const Vec3 &pos() const { return _pos; }
+    ParticleTypeId type() const { return _type; }
+    ParticleId id() const { return _id; }
+
+private:
+    Vec3 _pos;
+    ParticleTypeId _type;
+    ParticleId _id;
+};
+
+// A linear or branched topology: its particles and graph edges (GraphTopology.h; the graph
+// library's findNTuples enumerates its bonds, angles and dihedrals, Graph_detail.h:39-78)
+class GraphTopology {
+public:
+    GraphTopology(std::string type, std::vector<std::size_t> particleIndices)
+        : _type(std::move(type)), _particles(std::move(particleIndices)), _neighbors(_particles.size()) {}
+    void addEdge(std::size_t v1, std::size_t v2);  // vertex indices into particleIndices()
+    const std::vector<std::size_t> &particleIndices() const { return _particles; }
+    const std::string &type() const { return _type; }
+    // visits every edge, 3-path and 4-path once, in the reference's enumeration order
+    void findNTuples(const std::function<void(std::size_t, std::size_t)> &pair,
+                     const std::function<void(std::size_t, std::size_t, std::size_t)> &triple,
+                     const std::function<void(std::size_t, std::size_t, std::size_t, std::size_t)> &quad) const;
+
+private:
+    std::string _type;
+    std::vector<std::size_t> _particles;
+    std::vector<std::vector<std::size_t>> _neighbors;
+};
+
+// ---- actions -----------------------------------------------------------------------------------
+class Action {  // Action.h:58-79
+public:
+    virtual ~Action() = default;
+    virtual void perform() = 0;
+};
+
+class TimeStepDependentAction : public Action {
+public:
+    explicit TimeStepDependentAction(scalar dt) : _dt(dt) {}
+    scalar timeStep() const { return _dt; }
+    void setTimeStep(scalar dt) { _dt = dt; }
+
+protected:
+    scalar _dt;
+};
+
+class ActionFactory {  // ActionFactory.h:93-128
+public:
+    virtual ~ActionFactory() = default;
+    virtual std::unique_ptr<Action> addParticles(const std::vector<Particle> &particles) = 0;
+    virtual std::unique_ptr<TimeStepDependentAction> eulerBDIntegrator(scalar dt) = 0;
+    virtual std::unique_ptr<Action> calculateForces() = 0;
+    virtual std::unique_ptr<Action> createNeighborList(scalar interactionDistance) = 0;
+    virtual std::unique_ptr<Action> updateNeighborList() = 0;
+    virtual std::unique_ptr<Action> clearNeighborList() = 0;
+    virtual std::unique_ptr<Action> evaluateCompartments() = 0;
+    virtual std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) = 0;
+    virtual std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) = 0;
+    virtual std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) = 0;
+    virtual std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) = 0;
+    virtual std::unique_ptr<Action> initializeKernel() = 0;
+};
+
+class StateModel {  // StateModel.h:66-117
+public:
+    virtual ~StateModel() = default;
+    virtual void addParticle(const Particle &p) = 0;
+    virtual void addParticles(const std::vector<Particle> &p) = 0;
+    virtual std::vector<Particle> getParticles() const = 0;     // ordered by particle id
+    virtual std::vector<Vec3> getParticlePositions() const = 0;  // aligned with getParticles()
+    virtual std::vector<Vec3> getParticleForces() const = 0;     // aligned with getParticles()
+    virtual void initializeNeighborList(scalar interactionDistance) = 0;
+    virtual void updateNeighborList() = 0;
+    virtual void clearNeighborList() = 0;
+    virtual scalar energy() const = 0;
+    virtual scalar time() const = 0;
+    virtual void setTime(scalar t) = 0;
+    // adds a topology: its particles are placed first on the device (handles 0..), the graph edges
+    // and the topology registry's configuration give its bonds, angles and dihedrals
+    virtual GraphTopology *addTopology(const std::string &type, const std::vector<Particle> &particles) = 0;
+    virtual std::vector<GraphTopology *> getTopologies() = 0;
+    virtual void configureTopologies() = 0;  // GraphTopology::configure for all topologies
+    virtual std::vector<uint64_t> reactionCounts() const = 0;  // of the last reaction handler call
+    virtual void clear() = 0;
+};
+
+class Kernel {  // Kernel.h:71-235
+public:
+    explicit Kernel(std::string name) : _name(std::move(name)) {}
+    virtual ~Kernel() = default;
+    Kernel(const Kernel &) = delete;
+    Kernel &operator=(const Kernel &) = delete;
+    const std::string &name() const { return _name; }
+    Context &context() { return _context; }
+    const Context &context() const { return _context; }
+    virtual StateModel &stateModel() = 0;
+    virtual const StateModel &stateModel() const = 0;
+    virtual ActionFactory &actions() = 0;
+    virtual bool supportsGillespie() const { return false; }
+    virtual void initialize() {}
+    // optional: `steps` steps of the SimulationLoop action sequence (integrate, neighbour list,
+    // reactions, neighbour list, forces) run on the device without returning to the host in
+    // between; false when the kernel has no such path
+    virtual bool runFused(const std::string &scheduler, scalar dt, std::size_t steps) {
+        (void)scheduler, (void)dt, (void)steps;
+        return false;
+    }
+
+protected:
+    std::string _name;
+    Context _context;
+};
+
+}  // namespace model
+
+namespace plugin {
+
+// KernelProvider.cpp:68-209: built-in factories (route A) and dlopen'ed plugin libraries exporting
+// extern "C" const char* name() and extern "C" readdy_amd::model::Kernel* createKernel() (route B)
+class KernelProvider {
+public:
+    using factory = std::function<model::Kernel *()>;
+    struct KernelDeleter {
+        std::shared_ptr<void> library;  // keeps the plugin mapped while a kernel lives
+        void operator()(model::Kernel *k) const;
+    };
+    using kernel_ptr = std::unique_ptr<model::Kernel, KernelDeleter>;
+
+    static KernelProvider &getInstance();
+    void add(const std::string &name, factory f);
+    void loadKernelsFromDirectory(const std::string &directory);
+    void loadKernelLibrary(const std::string &path);
+    kernel_ptr create(const std::string &name) const;
+    std::vector<std::string> availableKernels() const;
+    static bool isSharedLibrary(const std::string &path);
+
+private:
+    struct Entry {
+        factory make;
+        std::shared_ptr<void> library;
+    };
+    std::map<std::string, Entry> _factories;
+};
+
+}  // namespace plugin
+
+namespace api {
+
+// SimulationLoop.h:256-306: integrate, neighbour list, reactions, neighbour list, forces per step
+class SimulationLoop {
+public:
+    SimulationLoop(model::Kernel *kernel, scalar timeStep);
+    void useReactionScheduler(const std::string &name);  // "UncontrolledApproximation" | "Gillespie"
+    void evaluateForces(bool v) { _forces = v; }
+    void evaluateReactions(bool v) { _reactions = v; }
+    // called after every step with the step index (observables); none registered: the loop hands the
+    // whole run to the device-fused path (rdy_run), which performs the same actions in the same order
+    void addCallback(std::function<void(std::size_t)> cb) { _callbacks.push_back(std::move(cb)); }
+    void run(std::size_t steps);
+    scalar timeStep() const { return _dt; }
+
+private:
+    model::Kernel *_kernel;
+    scalar _dt;
+    std::string _scheduler = "Gillespie";  // the reference default when supported (SimulationLoop.h:104-109)
+    bool _forces = true, _reactions = true;
+    std::vector<std::function<void(std::size_t)>> _callbacks;
+};
+
+}  // namespace api
+
+// thrown for every failing device call; carries the engine's message
+class DeviceError : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+}  // namespace readdy_amd

@@ -1,0 +1,324 @@
+// C++ boundary test: loads the MI355X kernel plugin the way a ReaDDy host loads kernel libraries
+// (KernelProvider::loadKernelsFromDirectory, then create("MI355X")), configures systems through
+// the Context registries and drives them through StateModel / ActionFactory / SimulationLoop.
+// Results are checked against the FP64 C oracle (oracle/readdy_oracle.c) on identical inputs.
+//
+// usage: test_plugin <plugin directory> [--provider-only]     exit status 0 = all checks passed
+// (--provider-only: plugin loading, kernel creation and error behaviour; no device work)
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../readdy_amd/host/readdy_host.h"
+
+extern "C" {
+#include "../../oracle/readdy_oracle.h"
+}
+
+using namespace readdy_amd;
+
+static int failures = 0;
+#define CHECK(cond, ...)                                   \
+    do {                                                   \
+        if (!(cond)) {                                     \
+            std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+            std::printf(__VA_ARGS__);                      \
+            std::printf("\n");                             \
+            ++failures;                                    \
+        }                                                  \
+    } while (0)
+
+struct System {
+    std::vector<double> pos;
+    std::vector<int> types;
+};
+
+static System uniform(int nA, int nB, double L, unsigned seed) {
+    std::mt19937_64 g(seed);
+    std::uniform_real_distribution<double> u(-0.5 * L, 0.5 * L);
+    System s;
+    for (int i = 0; i < nA + nB; ++i) {
+        for (int a = 0; a < 3; ++a) s.pos.push_back(u(g));
+        s.types.push_back(i < nA ? 0 : 1);
+    }
+    return s;
+}
+
+// C0 chemistry (harmonic repulsion on all pairs, A+B<->C) in the kernel's context
+static void c0_context(model::Context &c, double L) {
+    c.boxSize() = {L, L, L};
+    c.periodicBoundaryConditions() = {true, true, true};
+    c.particleTypes().add("A", 1.0);
+    c.particleTypes().add("B", 1.0);
+    c.particleTypes().add("C", 0.5);
+    const char *t[3] = {"A", "B", "C"};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) c.potentials().addHarmonicRepulsion(t[a], t[b], 10.0, 1.0);
+    c.reactions().addFusion("fus", "A", "B", "C", 20.0, 1.0);
+    c.reactions().addFission("fis", "C", "A", "B", 10.0, 1.0);
+}
+
+static or_system *c0_oracle(const System &s, double L) {
+    const double box[3] = {L, L, L};
+    const int pbc[3] = {1, 1, 1};
+    const double D[3] = {1.0, 1.0, 0.5};
+    or_system *o = or_create(3, box, pbc, 1.0, D);
+    const double hr[2] = {10.0, 1.0};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) or_add_pair_potential(o, OR_POT_HARMONIC_REPULSION, a, b, hr);
+    or_add_reaction(o, OR_RX_FUSION, 0, 1, 2, -1, 20.0, 1.0, 0.0, 0.5, 0.5);
+    or_add_reaction(o, OR_RX_FISSION, 2, -1, 0, 1, 10.0, 0.0, 1.0, 0.5, 0.5);
+    std::vector<uint64_t> ids(s.types.size());
+    for (size_t i = 0; i < ids.size(); ++i) ids[i] = i;
+    or_add_particles(o, (int)s.types.size(), s.pos.data(), s.types.data(), ids.data());
+    or_nl_setup(o, or_max_cutoff(o), 1);
+    return o;
+}
+
+struct OracleState {
+    std::map<uint64_t, std::array<double, 3>> pos, force;
+    std::map<uint64_t, int> type;
+};
+
+static OracleState oracle_state(const or_system *o) {
+    const size_t n = or_n_active(o);
+    std::vector<double> p(3 * n), f(3 * n);
+    std::vector<int> t(n);
+    std::vector<uint64_t> id(n);
+    or_get_particles(o, n, p.data(), t.data(), id.data(), f.data());
+    OracleState s;
+    for (size_t i = 0; i < n; ++i) {
+        s.pos[id[i]] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+        s.force[id[i]] = {f[3 * i], f[3 * i + 1], f[3 * i + 2]};
+        s.type[id[i]] = t[i];
+    }
+    return s;
+}
+
+static double wrapd(double d, double L) { return d - L * std::round(d / L); }
+
+static void test_provider_and_errors(const std::string &dir) {
+    auto &prov = plugin::KernelProvider::getInstance();
+    prov.loadKernelsFromDirectory(dir);
+    bool found = false;
+    for (const auto &n : prov.availableKernels()) found = found || n == "MI355X";
+    CHECK(found, "MI355X not among the available kernels");
+    bool threw = false;
+    try {
+        prov.create("NoSuchKernel");
+    } catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw, "unknown kernel name must throw std::invalid_argument");
+    auto k = prov.create("MI355X");
+    CHECK(k->name() == "MI355X", "kernel name");
+    CHECK(k->supportsGillespie(), "MI355X supports Gillespie");
+    threw = false;
+    try {
+        k->actions().detailedBalance(0.1);
+    } catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw, "DetailedBalance must throw");
+    std::printf("provider: ok\n");
+}
+
+// forces and energy of the C0 system against the oracle; then five loop steps event for event
+static void test_c0_forces_and_loop(const std::string &) {
+    const double L = 20.0;
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    c0_context(k->context(), L);
+    const System s = uniform(1000, 1000, L, 7);
+    std::vector<model::Particle> ps;
+    for (size_t i = 0; i < s.types.size(); ++i)
+        ps.emplace_back(s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2], (ParticleTypeId)s.types[i]);
+    k->actions().addParticles(ps)->perform();
+    k->actions().calculateForces()->perform();
+    or_system *o = c0_oracle(s, L);
+    double e_ref = 0;
+    or_calculate_forces(o, &e_ref, nullptr);
+    const auto ref = oracle_state(o);
+    const auto parts = k->stateModel().getParticles();
+    const auto forces = k->stateModel().getParticleForces();
+    CHECK(parts.size() == s.types.size(), "particle count %zu", parts.size());
+    double num = 0, den = 0;
+    for (size_t i = 0; i < parts.size() && i < forces.size(); ++i) {
+        const auto &fr = ref.force.at(parts[i].id());
+        for (int a = 0; a < 3; ++a) {
+            num += (forces[i][a] - fr[a]) * (forces[i][a] - fr[a]);
+            den += fr[a] * fr[a];
+        }
+    }
+    const double rel = std::sqrt(num / den);
+    const double e_rel = std::fabs(k->stateModel().energy() - e_ref) / std::fabs(e_ref);
+    if (!(rel < 1e-5)) {  // the particles that disagree most
+        std::vector<std::pair<double, size_t>> d;
+        for (size_t i = 0; i < parts.size() && i < forces.size(); ++i) {
+            const auto &fr = ref.force.at(parts[i].id());
+            d.emplace_back(std::fabs(forces[i].x - fr[0]) + std::fabs(forces[i].y - fr[1]) + std::fabs(forces[i].z - fr[2]), i);
+        }
+        std::sort(d.rbegin(), d.rend());
+        for (size_t k2 = 0; k2 < 4 && k2 < d.size(); ++k2) {
+            const size_t i = d[k2].second;
+            const auto &fr = ref.force.at(parts[i].id());
+            std::printf("  particle %llu at (%.4f %.4f %.4f): kernel (%.6g %.6g %.6g) oracle (%.6g %.6g %.6g)\n",
+                        (unsigned long long)parts[i].id(), parts[i].pos().x, parts[i].pos().y, parts[i].pos().z,
+                        forces[i].x, forces[i].y, forces[i].z, fr[0], fr[1], fr[2]);
+        }
+    }
+    CHECK(rel < 1e-5, "force relative error %.3g", rel);
+    CHECK(e_rel < 1e-5, "energy relative error %.3g", e_rel);
+    std::printf("c0 forces: rel %.3g, energy rel %.3g\n", rel, e_rel);
+
+    // SimulationLoop with a per-step callback (the action-by-action path) against or_loop_step
+    // (same Philox counters, UncontrolledApproximation with keyed order)
+    const std::uint64_t seed = 0;  // the kernel's Philox key (READDY_AMD_SEED unset)
+    or_set_seed(o, seed);
+    api::SimulationLoop loop(k.get(), 0.01);
+    loop.useReactionScheduler("UncontrolledApproximation");
+    int step_checked = 0;
+    double worst = 0;
+    uint64_t events = 0;
+    loop.addCallback([&](std::size_t t) {
+        if (t == 0) return;
+        double e = 0;
+        or_loop_step(o, 0.01, OR_UA_KEYED, (uint32_t)(t - 1), nullptr, 0, &e);
+        const auto r = oracle_state(o);
+        const auto p = k->stateModel().getParticles();
+        CHECK(p.size() == r.pos.size(), "step %zu: %zu vs %zu particles", t, p.size(), r.pos.size());
+        for (const auto &q : p) {
+            auto it = r.pos.find(q.id());
+            if (it == r.pos.end()) {
+                CHECK(false, "step %zu: handle %llu missing in the oracle", t, (unsigned long long)q.id());
+                continue;
+            }
+            CHECK(r.type.at(q.id()) == q.type(), "step %zu: type of %llu", t, (unsigned long long)q.id());
+            for (int a = 0; a < 3; ++a) worst = std::max(worst, std::fabs(wrapd(q.pos()[a] - it->second[a], L)));
+        }
+        const auto counts = k->stateModel().reactionCounts();
+        int nrc = 0;
+        const uint64_t *oc = or_reaction_counts(o, &nrc);
+        for (int i = 0; i < nrc && i < (int)counts.size(); ++i) {
+            CHECK(counts[i] == oc[i], "step %zu: reaction %d count %llu vs %llu", t, i,
+                  (unsigned long long)counts[i], (unsigned long long)oc[i]);
+            events += counts[i];
+        }
+        ++step_checked;
+    });
+    loop.run(5);
+    CHECK(step_checked == 5, "callback steps %d", step_checked);
+    CHECK(worst < 1e-6, "largest position difference %.3g", worst);
+    CHECK(events > 20, "too few reaction events (%llu) to exercise the handler", (unsigned long long)events);
+    std::printf("c0 loop: 5 steps, %llu events, worst position difference %.3g\n", (unsigned long long)events, worst);
+    or_destroy(o);
+
+    // device-fused loop: conservation of A + C and B + C
+    api::SimulationLoop fused(k.get(), 0.01);
+    fused.useReactionScheduler("Gillespie");
+    fused.run(50);
+    int nA = 0, nB = 0, nC = 0;
+    for (const auto &q : k->stateModel().getParticles()) (q.type() == 0 ? nA : q.type() == 1 ? nB : nC)++;
+    CHECK(nA + nC == 1000 && nB + nC == 1000, "conservation: A %d B %d C %d", nA, nB, nC);
+    CHECK(std::fabs(k->stateModel().time() - (5 + 50) * 0.01) < 1e-12, "simulation time %g", k->stateModel().time());
+    std::printf("c0 fused Gillespie: A %d B %d C %d\n", nA, nB, nC);
+}
+
+// a bead chain topology: GraphTopology edges -> bonds / angles / dihedrals via the registry
+static void test_topology() {
+    const double L = 30.0;
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {L, L, L};
+    c.particleTypes().addTopologyType("bead", 1.0);
+    c.particleTypes().add("X", 1.0);
+    c.topologyRegistry().configureBondPotential("bead", "bead", {100.0, 1.0});
+    c.topologyRegistry().configureAnglePotential("bead", "bead", "bead", {10.0, M_PI});
+    c.topologyRegistry().configureTorsionPotential("bead", "bead", "bead", "bead", {1.0, 3, 0.0});
+    c.potentials().addHarmonicRepulsion("X", "X", 10.0, 1.0);
+    c.potentials().addHarmonicRepulsion("X", "bead", 10.0, 1.0);
+    const int nb = 12;
+    std::mt19937_64 g(3);
+    std::normal_distribution<double> nd(0, 1);
+    std::vector<model::Particle> chain;
+    std::vector<double> pos;
+    std::vector<int> types;
+    double x[3] = {0, 0, 0};
+    for (int i = 0; i < nb; ++i) {
+        double d[3] = {nd(g), nd(g), nd(g)};
+        const double n = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int a = 0; a < 3; ++a) x[a] += i ? d[a] / n : 0.0;
+        chain.emplace_back(x[0], x[1], x[2], c.particleTypes().idOf("bead"));
+        pos.insert(pos.end(), x, x + 3);
+        types.push_back(0);
+    }
+    auto *top = k->stateModel().addTopology("chain", chain);
+    for (int i = 0; i + 1 < nb; ++i) top->addEdge(i, i + 1);
+    std::uniform_real_distribution<double> u(-0.5 * L, 0.5 * L);
+    std::vector<model::Particle> free;
+    for (int i = 0; i < 300; ++i) {
+        double p[3] = {u(g), u(g), u(g)};
+        free.emplace_back(p[0], p[1], p[2], c.particleTypes().idOf("X"));
+        pos.insert(pos.end(), p, p + 3);
+        types.push_back(1);
+    }
+    k->stateModel().addParticles(free);
+    k->stateModel().configureTopologies();
+    k->actions().calculateForces()->perform();
+
+    const double box[3] = {L, L, L};
+    const int pbc[3] = {1, 1, 1};
+    const double D[2] = {1.0, 1.0};
+    or_system *o = or_create(2, box, pbc, 1.0, D);
+    const double hr[2] = {10.0, 1.0};
+    or_add_pair_potential(o, OR_POT_HARMONIC_REPULSION, 1, 1, hr);
+    or_add_pair_potential(o, OR_POT_HARMONIC_REPULSION, 1, 0, hr);
+    std::vector<uint64_t> ids(types.size());
+    for (size_t i = 0; i < ids.size(); ++i) ids[i] = i;
+    or_add_particles(o, (int)types.size(), pos.data(), types.data(), ids.data());
+    for (int i = 0; i + 1 < nb; ++i) or_add_bond(o, i, i + 1, 100.0, 1.0);
+    for (int i = 0; i + 2 < nb; ++i) or_add_angle(o, i, i + 1, i + 2, 10.0, M_PI);
+    for (int i = 0; i + 3 < nb; ++i) or_add_dihedral(o, i, i + 1, i + 2, i + 3, 1.0, 3, 0.0);
+    or_nl_setup(o, or_max_cutoff(o), 1);
+    double e_ref = 0;
+    or_calculate_forces(o, &e_ref, nullptr);
+    const auto ref = oracle_state(o);
+    const auto parts = k->stateModel().getParticles();
+    const auto forces = k->stateModel().getParticleForces();
+    double num = 0, den = 0;
+    for (size_t i = 0; i < parts.size(); ++i) {
+        const auto &fr = ref.force.at(parts[i].id());
+        for (int a = 0; a < 3; ++a) {
+            num += (forces[i][a] - fr[a]) * (forces[i][a] - fr[a]);
+            den += fr[a] * fr[a];
+        }
+    }
+    const double rel = std::sqrt(num / den), e_rel = std::fabs(k->stateModel().energy() - e_ref) / std::fabs(e_ref);
+    CHECK(rel < 1e-5 && e_rel < 1e-5, "topology forces rel %.3g energy rel %.3g", rel, e_rel);
+    std::printf("topology: %d beads + 300 crowders, force rel %.3g, energy rel %.3g\n", nb, rel, e_rel);
+    or_destroy(o);
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        std::printf("usage: %s <plugin directory>\n", argv[0]);
+        return 2;
+    }
+    const bool provider_only = argc > 2 && std::string(argv[2]) == "--provider-only";
+    try {
+        test_provider_and_errors(argv[1]);
+        if (!provider_only) {
+            test_c0_forces_and_loop(argv[1]);
+            test_topology();
+        }
+    } catch (const std::exception &e) {
+        std::printf("FAIL exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
+    return failures ? 1 : 0;
+}
