@@ -1,4 +1,5 @@
-"""Benchmark: particle-steps/s of the fused MI355X ReaDDy step on the 1M-particle A+B<->C system.
+"""Benchmark: particle-steps/s of the fused MI355X ReaDDy step on the 1M-particle A+B<->C system,
+and the batched 4096-replica ensemble against the plain one-launch-per-replica path.
 
 Workload (the headline configuration): periodic 200^3 box, 1,000,000 particles (40% A, 40% B,
 20% C, uniform random, seed 1), harmonic repulsion k=10 d=1.0 between all species pairs,
@@ -78,6 +79,30 @@ def cpu_baseline(n_steps=3):
                       f"neighbour list, forces) by the FP64 C oracle on one host core; {dt:.1f} s"}
 
 
+def ensemble_bench(n_replicas, steps):
+    """Batched replica ensemble (4096 replicas of the A+B<->C system, 500..50000 particles each at
+    density 0.25) against the plain one-engine-per-replica path; cell size from the autotuner."""
+    from readdy_amd import ensemble as E
+    n, box = E.sample(n_replicas)
+    sub = max(1, min(512, n_replicas))
+    best, tried = E.autotune(n[:sub], box[:sub], steps=20)
+    b = E.Batched(n, box, nl_cutoff=best["nl_cutoff"])
+    rate, lat = b.timed(steps, chunk=10, warmup=10)
+    particles = int(b.particles())
+    b.close()
+    ids = list(range(0, n_replicas, max(1, n_replicas // 32)))
+    p = E.Plain(n, box, nl_cutoff=best["nl_cutoff"], replica_ids=ids)
+    plain = p.timed(3)
+    p.close()
+    return {"replicas": int(n_replicas), "particles": particles,
+            "batched_particle_steps_per_s": rate, "plain_particle_steps_per_s": plain, "speedup": rate / plain,
+            "step_ms_p50": lat["p50_ms"], "step_ms_p99": lat["p99_ms"],
+            "batched_roofline_frac": rate * B_ALG / (HBM_PEAK_GBS * 1e9),
+            "steps": steps, "autotune": {"chosen": best, "tried": tried},
+            "plain_sample": f"{len(ids)} replicas evenly spaced through the sample, one engine each, 3 steps",
+            "latency": "per-step wall time of the batched loop over chunks of 10 graph-replayed steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +112,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-replicas", type=int, default=4096)
+    ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (a known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -119,6 +147,7 @@ def main():
     graph = not args.eager
     eng.run(handler, dt, args.warmup, graph=graph)
     eng.synchronize()
+    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
     n0 = eng.n_particles(r)
     t0 = time.perf_counter()
     eng.run(handler, dt, args.steps, graph=graph)  # returns synchronized
@@ -151,6 +180,12 @@ def main():
         "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
         "device": device_info()["name"],
     }
+    eng.close()
+    print(f"bench: 1M-particle system {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(args.ensemble_replicas, args.ensemble_steps)
+        print(f"bench: ensemble {out['ensemble']['batched_particle_steps_per_s']:.4g} particle-steps/s", file=sys.stderr,
+              flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

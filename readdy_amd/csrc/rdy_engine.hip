@@ -90,6 +90,7 @@ struct rdy_engine {
     size_t d_bricks_cap = 0;
     uint32_t *d_slow = nullptr;  // two lists of bricks left to the bin-reading kernels (forces, search)
     uint32_t n_bricks_ah = 0;    // leading bricks of bricks_h in halo mode on every axis
+    bool bricks_dirty = false;   // replicas changed since the last brick upload
     bool wl_valid[2] = {true, true};
     uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
@@ -492,6 +493,14 @@ int upload_bricks(rdy_engine *e) {
 
 uint32_t n_bricks(const rdy_engine *e) { return (uint32_t)std::max<size_t>(e->bricks_h.size(), 1); }
 
+// brick tables of the current replicas on the device (before any brick launch or graph capture)
+int ensure_bricks(rdy_engine *e) {
+    if (!e->bricks_dirty) return RDY_OK;
+    int rc = upload_bricks(e);
+    if (rc == RDY_OK) e->bricks_dirty = false;
+    return rc;
+}
+
 // launch(all_halo, blocks, first_brick) for the all-halo bricks, then for the others
 template <class F>
 void for_brick_groups(const rdy_engine *e, F &&launch) {
@@ -890,8 +899,7 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     e->n_reps_used = std::max(e->n_reps_used, r + 1);
     e->cells_hwm = std::max<uint64_t>(e->cells_hwm, base + cells);
     e->version++;
-    int rcb = upload_bricks(e);
-    if (rcb) return rcb;
+    e->bricks_dirty = true;  // uploaded before the next brick launch (adding many replicas stays linear)
     *replica_id = r;
     return RDY_OK;
 }
@@ -921,7 +929,8 @@ int rdy_replica_remove(rdy_engine *e, uint32_t r) {
     e->version++;
     e->wl_valid[0] = e->wl_valid[1] = false;
     HIP_TRY(hipStreamSynchronize(e->stream));
-    return upload_bricks(e);
+    e->bricks_dirty = true;
+    return RDY_OK;
 }
 
 int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, const uint32_t *types) {
@@ -1092,6 +1101,7 @@ int rdy_set_dihedrals(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *idx
 
 int rdy_update_neighbor_list(rdy_engine *e) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (int rb = ensure_bricks(e)) return rb;
     int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
     if (rc) return rc;
     const int c = e->cur, n = 1 - c;
@@ -1107,6 +1117,7 @@ int rdy_update_neighbor_list(rdy_engine *e) {
 
 int rdy_calculate_forces(rdy_engine *e) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (int rb = ensure_bricks(e)) return rb;
     int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
     if (rc) return rc;
     if ((rc = ensure_force_out(e))) return rc;
@@ -1154,6 +1165,7 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
 
 int rdy_react(rdy_engine *e, int handler, double dt) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (int rb = ensure_bricks(e)) return rb;
     if (handler != RDY_HANDLER_UNCONTROLLED_APPROXIMATION && handler != RDY_HANDLER_GILLESPIE)
         return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
@@ -1195,6 +1207,7 @@ int rdy_get_reaction_totals(rdy_engine *e, uint32_t r, uint64_t *counts) {
 
 int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (int rb = ensure_bricks(e)) return rb;
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
     if (rc) return rc;
@@ -1212,6 +1225,7 @@ int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
 
 int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (int rb = ensure_bricks(e)) return rb;
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
     if (rc) return rc;

@@ -507,3 +507,23 @@ def test_c1_full_size_invariants():
     assert e.reaction_totals(r).sum() > 100
     p, _, _ = e.get_particles(r)
     assert np.all(p >= -Lb / 2) and np.all(p < Lb / 2)
+
+
+def test_ensemble_batched_equals_plain():
+    """Replicas of uneven size packed into one engine step exactly like one engine per replica
+    (same Philox keys; fixed-point forces make the result independent of the packing), with the
+    ensemble's own cell size and bin capacity."""
+    from readdy_amd import ensemble as E
+    n, box = E.sample(12, seed=11, n_min=200, n_max=3000)
+    b = E.Batched(n, box, nl_cutoff=2.0)
+    p = E.Plain(n, box, nl_cutoff=1.0)
+    b.run(15, graph=True)
+    for e, _ in p.engines:
+        e.run(L.HANDLER_UA, E.DT, 15)
+    for rid, (e, r) in zip(b.replicas, p.engines):
+        xb, tb, hb = b.engine.get_particles(rid)
+        xp, tp, hp = e.get_particles(r)
+        assert np.array_equal(hb, hp) and np.array_equal(tb, tp)
+        assert np.array_equal(xb, xp)
+    b.close()
+    p.close()
