@@ -616,14 +616,6 @@ void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
 }
 
-// the old buffer becomes the next target: zero its cell counters, then its work-list length
-void launch_clear(rdy_engine *e, int b) {
-    Grid old = e->grid(b);
-    hipLaunchKernelGGL(k_clear_old, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, old, e->cap,
-                       (uint32_t)e->max_particles);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, old);
-}
-
 // one fused step over all replicas: SimulationLoop.h:280-300 (integrate, NL, reactions, NL; the
 // forces of the resulting state are computed at the start of the next step, fused with its integration)
 void launch_step(rdy_engine *e, int handler, double dt) {
@@ -706,8 +698,9 @@ int rdy_device_info(char *name, size_t name_len, uint64_t *hbm_bytes, int32_t *n
     HIP_TRY(hipGetDevice(&dev));
     hipDeviceProp_t p;
     HIP_TRY(hipGetDeviceProperties(&p, dev));
-    if (name && name_len) {
-        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    if (name && name_len) {  // the marketing name can be empty on some runtimes: report the CU count too
+        std::snprintf(name, name_len, "%s (%s, %d CUs, %.0f GiB)", p.name[0] ? p.name : "AMD GPU", p.gcnArchName,
+                      p.multiProcessorCount, (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
     }
     if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
     if (n_cus) *n_cus = p.multiProcessorCount;

@@ -185,13 +185,6 @@ __global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned 
     }
 }
 
-__global__ void k_clear_old(Grid old, uint32_t cap, uint32_t max_wl) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = *old.wl_count;
-    n = n < max_wl ? n : max_wl;
-    if (i < n) old.counts[cell_of_slot(old, old.wl[i], cap)] = 0u;
-}
-
 __global__ void k_reset_counter(Grid old) {
     *old.wl_count = 0u;
     *old.ovf_count = 0u;
@@ -318,129 +311,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_wl_write(Grid g, uint32_t cap,
             if (start + o < max_wl) g.wl[start + o] = g.nbin + o;
     }
 }
-
-// -------------------------------------------------------------------------------------------------
-// the fused per-particle kernel: forces (+ Euler-Maruyama + re-bin)
-// Restates CPUCalculateForces::calculateOrder2 (CPUCalculateForces.cpp:214-268), the integrator
-// (CPUEulerBDIntegrator.cpp:63-73) and CompactCellLinkedList::fillBins (CellLinkedList.cpp:171-228).
-// -------------------------------------------------------------------------------------------------
-template <int MODE>
-__global__ __launch_bounds__(256) void k_fused(Grid cur, Grid nxt, ReplicaDev *reps, Tables T, Topo topo,
-                                                uint32_t cap, int radius, double dt, const double *noise,
-                                                double *force_out, uint32_t max_wl, uint32_t *err) {
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = b * blockDim.x + threadIdx.x;
-    uint32_t n = *cur.wl_count;
-    n = n < max_wl ? n : max_wl;
-    bool valid = i < n;
-    uint32_t s = 0;
-    State me = {0.0, 0.0, 0.0, 0u, 0u};
-    if (valid) {
-        s = cur.wl[i];
-        me = cur.bins[s];
-        if (me.tw & TW_DEAD) valid = false;
-    }
-    const uint32_t r = tw_replica(me.tw);
-    long long fx = 0, fy = 0, fz = 0, e = 0;
-    RepGeom geo;
-    uint32_t active = 0, step = 0;
-    uint64_t seed = 0;
-    if (valid) {
-        const ReplicaDev &R = reps[r];
-        geo = load_geom(R);
-        active = R.active;
-        step = R.step;
-        seed = R.seed;
-    }
-    const bool do_forces = valid && (MODE == MODE_FORCES || (MODE == MODE_STEP && active));
-    if (do_forces && !(me.tw & TW_OUTSIDE)) {
-        const uint32_t t = tw_type(me.tw);
-        const PairIdx *prow = T.pidx + (size_t)t * T.n_types;
-        const uint32_t cl = cell_of_slot(cur, s, cap) - geo.cell_base;
-        for_each_stencil_slot(geo, cur, cap, radius, cl, [&](uint32_t s2) {
-            if (s2 == s) return;
-            const State o = cur.bins[s2];
-            if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
-            const PairIdx pi = prow[tw_type(o.tw)];
-            if (pi.count == 0) return;
-            const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
-            const double dy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
-            const double dz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
-            const double d2 = dist2(dx, dy, dz);
-            if (!(d2 < pi.maxcut2)) return;
-            float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
-            const float xf = (float)dx, yf = (float)dy, zf = (float)dz, d2f = (float)d2;
-            for (uint32_t q = 0; q < pi.count; ++q) {
-                const PotDev pt = T.pots[pi.start + q];
-                if (d2 < pt.cut2) pe += pair_force_energy(pt, d2f, xf, yf, zf, pfx, pfy, pfz);
-            }
-            fx += to_fixed(pfx, 4294967296.0f);
-            fy += to_fixed(pfy, 4294967296.0f);
-            fz += to_fixed(pfz, 4294967296.0f);
-            e += to_fixed(pe, 16777216.0f);
-        });
-    }
-    if (do_forces && (me.tw & TW_TOPO) && topo.fb) {
-        long long *fb = topo.fb + 3ull * (reps[r].topo_base + me.h);
-        fx += fb[0];
-        fy += fb[1];
-        fz += fb[2];
-        fb[0] = 0;
-        fb[1] = 0;
-        fb[2] = 0;
-    }
-    if (MODE != MODE_REBIN) wave_add_energy(reps, do_forces, r, e);
-    const double Fx = (double)fx * FORCE_INV, Fy = (double)fy * FORCE_INV, Fz = (double)fz * FORCE_INV;
-    if (MODE != MODE_REBIN && force_out && do_forces) {
-        force_out[3ull * s] = Fx;
-        force_out[3ull * s + 1] = Fy;
-        force_out[3ull * s + 2] = Fz;
-    }
-    if (MODE == MODE_FORCES) return;
-
-    // ---- Euler-Maruyama step (CPUEulerBDIntegrator.cpp:63-73) ----
-    State nw = me;
-    if (MODE == MODE_STEP && valid && active) {
-        const double D = T.diffusion[tw_type(me.tw)];
-        double xi[3];
-        if (noise) {
-            xi[0] = noise[3ull * me.h];
-            xi[1] = noise[3ull * me.h + 1];
-            xi[2] = noise[3ull * me.h + 2];
-        } else {
-            uint32_t x4[4];
-            philox(me.h, step, 0u, 0u, seed, x4);
-            normals3(x4, xi);
-        }
-        const double sc = sqrt(2. * D * dt);
-        const double kbt = T.kbt;
-        const double F[3] = {Fx, Fy, Fz};
-        double p[3] = {me.x, me.y, me.z};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double det = F[d] * D * dt / kbt;
-            double rnd = xi[d] * sc;
-            p[d] += rnd + det;
-            p[d] = wrap(p[d], geo.box[d], geo.pbc[d]);
-            if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
-        }
-        nw.x = p[0];
-        nw.y = p[1];
-        nw.z = p[2];
-    }
-    // ---- re-bin into the other buffer (its work list is rebuilt in cell order afterwards) ----
-    if (valid) {
-        const uint32_t slot = bin_insert(nxt, geo, cap, nw, err, false, 0);
-        if (slot != 0xFFFFFFFFu && (me.tw & TW_TOPO) && topo.loc) topo.loc[reps[r].topo_base + me.h] = slot;
-    }
-}
-
-template __global__ void k_fused<MODE_FORCES>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
-                                              const double *, double *, uint32_t, uint32_t *);
-template __global__ void k_fused<MODE_STEP>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
-                                            const double *, double *, uint32_t, uint32_t *);
-template __global__ void k_fused<MODE_REBIN>(Grid, Grid, ReplicaDev *, Tables, Topo, uint32_t, int, double,
-                                             const double *, double *, uint32_t, uint32_t *);
 
 // separate integrator action: in place on the current buffer, forces from force_out
 __global__ __launch_bounds__(256) void k_integrate(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, double dt,
