@@ -103,6 +103,23 @@ def ensemble_bench(n_replicas, steps):
             "latency": "per-step wall time of the batched loop over chunks of 10 graph-replayed steps"}
 
 
+def serving_bench(n_requests, step_scale):
+    """Continuous batching of a finite Poisson trace (200 req/s) of the 1M-particle system's chemistry
+    at 1000..200000 particles, steps 1000..100000 scaled by step_scale, handlers 50/50, 32 GiB HBM budget,
+    observables written to HDF5 off the critical path."""
+    import tempfile
+    from readdy_amd import serving as S
+    trace = S.poisson_trace(n_requests, step_scale=step_scale)
+    with tempfile.TemporaryDirectory() as d:
+        srv = S.Server(hbm_budget_bytes=32 << 30, max_slots=512, interval_steps=100, out_dir=d)
+        _, summary = srv.run(trace)
+        srv.close()
+    summary.update({"trace": f"first {n_requests} arrivals of Poisson(200/s); steps log-uniform [1000, 100000] "
+                             f"x {step_scale}", "hbm_budget_gib": 32,
+                    "roofline_frac": summary["particle_steps_per_s"] * B_ALG / (HBM_PEAK_GBS * 1e9)})
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +130,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-step-scale", type=float, default=0.1)
     ap.add_argument("--ensemble-replicas", type=int, default=4096)
     ap.add_argument("--ensemble-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -185,6 +205,10 @@ def main():
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(args.ensemble_replicas, args.ensemble_steps)
         print(f"bench: ensemble {out['ensemble']['batched_particle_steps_per_s']:.4g} particle-steps/s", file=sys.stderr,
+              flush=True)
+    if not args.no_serving:
+        out["serving"] = serving_bench(args.serving_requests, args.serving_step_scale)
+        print(f"bench: serving {out['serving']['particle_steps_per_s']:.4g} particle-steps/s", file=sys.stderr,
               flush=True)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)

@@ -164,6 +164,11 @@ int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps);
 /* per-replica step budget for serving: a replica stops stepping after this many steps */
 int rdy_replica_set_steps(rdy_engine *e, uint32_t replica, uint64_t steps);
 int rdy_replica_steps_done(rdy_engine *e, uint32_t replica, uint64_t *steps);
+/* per-replica reaction handler (RDY_HANDLER_*; -1 = the handler the step is launched with), so that
+ * UncontrolledApproximation and Gillespie systems share launches; a launch with RDY_HANDLER_NONE
+ * performs no reactions for any replica.  No reference counterpart (one handler per simulation,
+ * SimulationLoop.h:104-109). */
+int rdy_replica_set_handler(rdy_engine *e, uint32_t replica, int handler);
 int rdy_synchronize(rdy_engine *e);
 
 /* ---- test exports (parity): the pair set of the cell list, the candidate list ---- */

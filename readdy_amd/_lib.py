@@ -91,6 +91,7 @@ SIGNATURES = {
     "rdy_run_graph": (_int, [_vp, _int, _d, _u64]),
     "rdy_replica_set_steps": (_int, [_vp, _u32, _u64]),
     "rdy_replica_steps_done": (_int, [_vp, _u32, _u64p]),
+    "rdy_replica_set_handler": (_int, [_vp, _u32, _int]),
     "rdy_synchronize": (_int, [_vp]),
     "rdy_neighbor_pairs": (_int, [_vp, _u32, _d, _u64, _u32p, _u64p]),
     "rdy_reaction_candidates": (_int, [_vp, _u32, _u64, _u32p, _u64p]),

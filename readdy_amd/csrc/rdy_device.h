@@ -53,6 +53,8 @@ struct ReplicaDev {
     uint64_t steps_left;   // remaining step budget (serving)
     uint64_t steps_done;
     uint32_t topo_base, n_topo;
+    int32_t handler;       // reaction handler of this replica (-1: the one the step is launched with)
+    uint32_t pad;
     unsigned long long energy;  // two's-complement int64, fixed point 2^-24
 };
 

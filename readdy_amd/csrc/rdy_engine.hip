@@ -885,6 +885,7 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     R.seed = d->seed;
     R.steps_left = ~0ull;
     R.steps_done = 0;
+    R.handler = -1;
     HIP_TRY(hipMemcpyAsync(e->reps + r, &R, sizeof(ReplicaDev), hipMemcpyHostToDevice, e->stream));
     const size_t nrx = std::max<uint32_t>(e->n_rx, 1);
     HIP_TRY(hipMemsetAsync(e->rx_step + (size_t)r * nrx, 0, nrx * 8, e->stream));
@@ -1269,6 +1270,14 @@ int rdy_replica_steps_done(rdy_engine *e, uint32_t r, uint64_t *steps) {
     int rc = check_replica(e, r);
     if (rc) return rc;
     return get_field(e, r, offsetof(ReplicaDev, steps_done), steps);
+}
+
+int rdy_replica_set_handler(rdy_engine *e, uint32_t r, int handler) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (handler < -1 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    const int32_t h = handler;
+    return put_field(e, r, offsetof(ReplicaDev, handler), h);
 }
 
 int rdy_synchronize(rdy_engine *e) {

@@ -547,6 +547,7 @@ __device__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, u
     const uint32_t t = tw_type(me.tw);
     const uint32_t step = R.step;
     const uint64_t seed = R.seed;
+    if (handler <= 1 && R.handler >= 0) handler = R.handler;  // the replica's own handler
 
     auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key,
                     uint32_t ne) {

@@ -219,6 +219,10 @@ class Engine:
     def set_steps(self, r, steps):
         L.check(self._lib.rdy_replica_set_steps(self._h, r, int(steps)))
 
+    def set_handler(self, r, handler):
+        """Reaction handler of replica r (L.HANDLER_*; -1: the handler passed to run/react)."""
+        L.check(self._lib.rdy_replica_set_handler(self._h, r, int(handler)))
+
     def steps_done(self, r):
         s = ctypes.c_uint64()
         L.check(self._lib.rdy_replica_steps_done(self._h, r, ctypes.byref(s)))
