@@ -84,8 +84,9 @@ typedef struct {
     int32_t cell_radius;      /* cll_radius (KernelConfiguration.h:63): cells of cutoff/radius          */
     uint32_t bin_capacity;    /* particles per cell slot block; 0 = from the expected density           */
     uint32_t max_replicas;
-    uint64_t max_cells;       /* total cells over all replicas                                          */
-    uint64_t max_particles;   /* total live particles over all replicas                                 */
+    uint64_t max_cells;       /* total cells over all replicas (0 = max_replicas x the first replica's)  */
+    uint64_t max_particles;   /* total live particles over all replicas (0 = max_replicas x the first
+                                 replica's 2 max_particles + 1024, at least 4096)                       */
     uint32_t events_per_replica; /* accepted reaction events per replica per step (0 = automatic)       */
     uint32_t block_size;      /* threads per block of the per-particle kernels (0 = 256)               */
     uint32_t brick[3];        /* cells per LDS-staged brick along x, y, z (0 = from the density)        */

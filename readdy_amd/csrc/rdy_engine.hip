@@ -837,8 +837,10 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     cell_dims(e, d->box, dims, cs);
     const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
     if (!e->pools_ready) {
-        if (e->max_cells == 0) e->max_cells = cells;
-        if (e->max_particles == 0) e->max_particles = std::max<uint64_t>(4096, 2 * d->max_particles + 1024);
+        // automatic pools: max_replicas copies of the first replica's grid and particle budget
+        if (e->max_cells == 0) e->max_cells = cells * e->max_reps;
+        if (e->max_particles == 0)
+            e->max_particles = std::max<uint64_t>(4096, (2 * d->max_particles + 1024) * e->max_reps);
         const double lam_raw = (double)d->max_particles / (double)cells;
         if (e->auto_cap) {
             // mean occupancy of the first replica's grid: Poisson mean + 7 sigma + 6, a multiple of 4
