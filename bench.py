@@ -8,9 +8,11 @@ rate 0.1 radius 1.0, fission C->A+B rate 0.05 distance 1.0, D_A = D_B = 1, D_C =
 EulerBD dt=0.005, UncontrolledApproximation.  FP32 pair arithmetic on FP64 positions.
 
 Prints one JSON line.  `value` is whole-job particle-steps/s with the state resident in HBM;
-`roofline` prices the fused force/integrate/re-bin kernel (64 algorithmic bytes per particle-step)
-against MI355X's 8 TB/s; `cpu_baseline` is the FP64 CPU oracle timed on a bounded sample of
-the same workload on this host.
+`roofline` prices the step's dominant kernel, the stencil walk k_brick_fs (pair forces + reaction
+candidates), at the 32 bytes of particle state per particle it must read, against MI355X's 8 TB/s;
+`step_roofline_frac` prices the whole step at 64 algorithmic bytes per particle-step (the state read
+and written once).  `cpu_baseline` is the FP64 CPU oracle timed on a bounded sample of the same
+workload on this host.
 """
 import argparse
 import json
@@ -27,6 +29,9 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M particles; batched ensemble throughput + p99 vs plain"
 HBM_PEAK_GBS = 8000.0
 B_ALG = 64.0  # bytes per particle-step: FP64 position + uint32 species + uint32 handle, read + written
+B_WALK = 32.0  # bytes per particle the stencil-walk kernel must read: the same record, read once
+PMC_PROFILE = "brick_fs_c1_1M_pmc.md"
+WALK_KERNEL = "k_brick_fs<true> (pair forces + reaction candidates of one brick of cells)"
 
 
 def c1_system(n=1_000_000, L=200.0, seed=1):
@@ -45,8 +50,8 @@ def c1_chemistry(L):
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3 --pmc summary, if any."""
-    p = os.path.join(ROOT, "profiles", "fused_step_c1_1M_pmc.md")
+    """HBM bytes per launch of the walk kernel from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", PMC_PROFILE)
     if not os.path.exists(p):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
@@ -177,12 +182,12 @@ def main():
     value = particle_steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
 
-    # fused-kernel duration with HIP events on the engine's stream
+    # walk-kernel duration with HIP events on the engine's stream
     eng.profile_reset()
     eng.run(handler, dt, args.profile_steps, graph=False)
     avg_us, launches = eng.profile_get()
     n_mid = eng.n_particles(r)
-    achieved = (n_mid * B_ALG) / (avg_us * 1e-6) / 1e9  # GB/s
+    achieved = (n_mid * B_WALK) / (avg_us * 1e-6) / 1e9  # GB/s
     traffic = traffic_from_profiles()
 
     out = {
@@ -195,7 +200,7 @@ def main():
                    "box": 200.0, "handler": "UncontrolledApproximation", "graph": graph},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_brick<MODE_STEP, true> (forces + Euler-Maruyama + re-bin of one brick)",
+                     "kernel": WALK_KERNEL, "algorithmic_bytes_per_particle": B_WALK,
                      "avg_kernel_us": avg_us, "launches": launches},
         "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
         "device": device_info()["name"],

@@ -491,12 +491,17 @@ constexpr uint32_t MAX_STAGED_TYPES = 16;
 
 __host__ __device__ inline size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
-// bytes of the table region for nt types and np potentials (the larger of the force and search sets)
+// bytes of the force tables for nt types and np potentials
+__host__ __device__ inline size_t force_table_bytes(uint32_t nt, uint32_t np) {
+    const size_t n2 = (size_t)nt * nt;
+    return align16(sizeof(PairIdx) * n2) + align16(sizeof(PotDev) * np) + align16(4 * n2) + align16(4 * nt);
+}
+
+// bytes of the table region for nt types and np potentials: the force tables followed by the
+// search tables (the fused forces + search kernel holds both; the others use one of them)
 __host__ __device__ inline size_t table_bytes(uint32_t nt, uint32_t np) {
     const size_t n2 = (size_t)nt * nt;
-    const size_t f = align16(sizeof(PairIdx) * n2) + align16(sizeof(PotDev) * np) + align16(4 * n2) + align16(4 * nt);
-    const size_t r = align16(sizeof(RxPairIdx) * n2) + align16(4 * n2) + align16(4 * nt);
-    return f > r ? f : r;
+    return force_table_bytes(nt, np) + align16(sizeof(RxPairIdx) * n2) + align16(4 * n2) + align16(4 * nt);
 }
 
 struct ForceTables {
@@ -604,17 +609,12 @@ __device__ __forceinline__ void global_forces(const Grid &cur, const RepGeom &ge
         const double dy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
         const double dz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
         const double d2 = dist2(dx, dy, dz);
-        if (!(d2 < pi.maxcut2)) return;
-        float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
-        const float xf = (float)dx, yf = (float)dy, zf = (float)dz, d2f = (float)d2;
-        for (uint32_t p = 0; p < pi.count; ++p) {
-            const PotDev pt = T.pots[pi.start + p];
-            if (d2 < pt.cut2) pe += pair_force_energy(pt, d2f, xf, yf, zf, pfx, pfy, pfz);
-        }
-        fx += to_fixed(pfx, 4294967296.0f);
-        fy += to_fixed(pfy, 4294967296.0f);
-        fz += to_fixed(pfz, 4294967296.0f);
-        e += to_fixed(pe, 16777216.0f);
+        long long px, py, pz, pe;
+        if (!pair_fixed(T.pots, pi, dx, dy, dz, d2, px, py, pz, pe)) return;
+        fx += px;
+        fy += py;
+        fz += pz;
+        e += pe;
     });
 }
 
@@ -656,6 +656,7 @@ __device__ __forceinline__ void finish_particle(const Grid &nxt, const ReplicaDe
     }
     if (MODE == MODE_FORCES) return;
     State nw = me;
+    nw.tw &= ~TW_NEW;
     if (MODE == MODE_STEP && R.active) {
         // CPUEulerBDIntegrator.cpp:63-73
         const double D = T.diffusion[tw_type(me.tw)];
@@ -808,17 +809,12 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             const double ddy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
             const double ddz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
             const double d2 = dist2(ddx, ddy, ddz);
-            if (!(d2 < pi.maxcut2)) return;
-            float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
-            const float xf = (float)ddx, yf = (float)ddy, zf = (float)ddz, d2f = (float)d2;
-            for (uint32_t p = 0; p < pi.count; ++p) {
-                const PotDev pot = ft.pots[pi.start + p];
-                if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
-            }
-            atomicAdd(&W.acc[0][ol], (unsigned long long)to_fixed(pfx, 4294967296.0f));
-            atomicAdd(&W.acc[1][ol], (unsigned long long)to_fixed(pfy, 4294967296.0f));
-            atomicAdd(&W.acc[2][ol], (unsigned long long)to_fixed(pfz, 4294967296.0f));
-            e_blk += to_fixed(pe, 16777216.0f);
+            long long px, py, pz, pe;
+            if (!pair_fixed(ft.pots, pi, ddx, ddy, ddz, d2, px, py, pz, pe)) return;
+            atomicAdd(&W.acc[0][ol], (unsigned long long)px);
+            atomicAdd(&W.acc[1][ol], (unsigned long long)py);
+            atomicAdd(&W.acc[2][ol], (unsigned long long)pz);
+            e_blk += pe;
         };
         auto rounds = [&]() {
             for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform

@@ -13,7 +13,7 @@
 namespace rdy {
 
 // ---- particle record: 32 bytes, the unit of the cell bins -------------------------------------
-// tw packs the species (bits 0-9), flags (bits 10-12) and the replica index (bits 13-31).
+// tw packs the species (bits 0-9), flags (bits 10-13) and the replica index (bits 14-31).
 struct alignas(16) State {
     double x, y, z;
     uint32_t tw;
@@ -25,9 +25,10 @@ constexpr uint32_t TW_TYPE_MASK = 0x3FFu;
 constexpr uint32_t TW_OUTSIDE = 1u << 10;  // outside the box on a non-periodic axis: never binned as a neighbour
 constexpr uint32_t TW_DEAD = 1u << 11;     // removed by a reaction in this step
 constexpr uint32_t TW_TOPO = 1u << 12;     // member of a topology (has bonded terms)
-constexpr uint32_t TW_REPLICA_SHIFT = 13;
+constexpr uint32_t TW_NEW = 1u << 13;      // inserted or retyped by a reaction of this step (fused step only)
+constexpr uint32_t TW_REPLICA_SHIFT = 14;
 constexpr uint32_t MAX_TYPES = 1024;
-constexpr uint32_t MAX_REPLICAS = 1u << 19;
+constexpr uint32_t MAX_REPLICAS = 1u << 18;
 
 __host__ __device__ inline uint32_t tw_type(uint32_t tw) { return tw & TW_TYPE_MASK; }
 __host__ __device__ inline uint32_t tw_replica(uint32_t tw) { return tw >> TW_REPLICA_SHIFT; }
@@ -54,7 +55,7 @@ struct ReplicaDev {
     uint64_t steps_done;
     uint32_t topo_base, n_topo;
     int32_t handler;       // reaction handler of this replica (-1: the one the step is launched with)
-    uint32_t pad;
+    uint32_t integ;        // fused step: the walk of this step also integrates the next one
     unsigned long long energy;  // two's-complement int64, fixed point 2^-24
 };
 
@@ -270,6 +271,31 @@ __device__ __forceinline__ long long to_fixed(float v, float scale) {
     const float lim = 9.0e18f;
     s = fminf(fmaxf(s, -lim), lim);
     return (long long)s;
+}
+
+// The force on a particle from one neighbour, summed over the potentials of their ordered type
+// pair, in 2^-32 fixed point (fx, fy, fz), and the pair energy in 2^-24 fixed point (e).  dx, dy, dz
+// is the FP64 minimum-image displacement neighbour - particle and d2 its FP64 square; the pair
+// counts only when d2 < the pair's largest cutoff^2 (CPUCalculateForces.cpp:243), else every output
+// is zero and the result is false.  Every pair-force sum of the engine goes through this function,
+// so equal inputs give equal integers wherever a sum is formed or corrected.  ENERGY = false
+// leaves e zero (the force terms are the same).
+template <bool ENERGY = true>
+__device__ __forceinline__ bool pair_fixed(const PotDev *pots, const PairIdx &pi, double dx, double dy, double dz,
+                                           double d2, long long &fx, long long &fy, long long &fz, long long &e) {
+    fx = fy = fz = e = 0;
+    if (!(d2 < pi.maxcut2)) return false;
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
+    const float xf = (float)dx, yf = (float)dy, zf = (float)dz, d2f = (float)d2;
+    for (uint32_t p = 0; p < pi.count; ++p) {
+        const PotDev pot = pots[pi.start + p];
+        if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
+    }
+    fx = to_fixed(pfx, 4294967296.0f);
+    fy = to_fixed(pfy, 4294967296.0f);
+    fz = to_fixed(pfz, 4294967296.0f);
+    if (ENERGY) e = to_fixed(pe, 16777216.0f);
+    return true;
 }
 
 __device__ __forceinline__ long long to_fixed_d(double v, double scale) {

@@ -1,6 +1,6 @@
 // Host side of the MI355X ReaDDy engine: device pools, replica admission, the per-step launch
 // sequence (eager and hipGraph), and the extern "C" ABI declared in include/readdy_amd.h.
-#include "rdy_bricks.hip"
+#include "rdy_step.hip"
 
 #include "../../include/readdy_amd.h"
 
@@ -98,6 +98,20 @@ struct rdy_engine {
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
     uint32_t *err = nullptr;
     double *force_out = nullptr;
+    // fused step (rdy_step.hip): the walk's per-brick force rows (2^-32 fixed point), the records
+    // changed by reactions, the records moved again and the copies withdrawn; force_out and the
+    // energies are formed on demand after a run (export_valid)
+    long long *gstage = nullptr;
+    uint64_t gstage_rows = 0;
+    Edit *edits = nullptr;
+    uint32_t *edit_count = nullptr;  // [3]: changed records, records to move again, copies to withdraw
+    uint32_t edit_cap = 0;
+    uint32_t *aff_bits = nullptr;
+    uint32_t *aff_list = nullptr;
+    uint32_t aff_cap = 0;
+    uint2 *wd_list = nullptr;
+    uint32_t wd_cap = 0;
+    bool export_valid = true;
     // tables
     PotDev *d_pots = nullptr;
     PairIdx *d_pidx = nullptr;
@@ -189,6 +203,9 @@ int check_errors(rdy_engine *e) {
     if (h[ERR_LDS_LAYOUT]) return set_err(RDY_ERR_STATE, "brick kernels: LDS layout exceeds the reserved size");
     if (h[ERR_EVENT_LDS]) m += "more than " + std::to_string(RESOLVE_MAX_EVENTS) + " accepted events in one replica step; ";
     if (h[ERR_WL_OVERFLOW]) m += "particle pool exhausted: raise max_particles; ";
+    if (h[ERR_EDIT_OVERFLOW])
+        m += "changed-record list of the fused step overflowed (" + std::to_string(h[ERR_EDIT_OVERFLOW]) +
+             "): too many reactions per step, reduce the time step; ";
     return set_err(RDY_ERR_CAPACITY, m);
 }
 
@@ -389,6 +406,35 @@ int alloc_pools(rdy_engine *e) {
     return RDY_OK;
 }
 
+uint32_t n_bricks(const rdy_engine *e);
+
+int ensure_step_pools(rdy_engine *e) {
+    // force rows of the walk: max_states per brick (a brick's own records are among its staged ones)
+    const uint64_t rows = (uint64_t)n_bricks(e) * e->max_states;
+    if (rows > e->gstage_rows) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (e->gstage) HIP_TRY(hipFree(e->gstage));
+        e->gstage = nullptr;
+        HIP_TRY(dalloc(&e->gstage, 3 * rows));
+        e->gstage_rows = rows;
+    }
+    if (e->edits) return RDY_OK;
+    const uint64_t slots = e->max_cells * e->cap + e->ovf_cap;
+    HIP_TRY(dalloc(&e->aff_bits, slots / 32 + 1));
+    HIP_TRY(hipMemsetAsync(e->aff_bits, 0, (slots / 32 + 1) * sizeof(uint32_t), e->stream));
+    // at most three changed records per accepted event (a fusion kills two and inserts one); each
+    // changes the forces of its neighbours within the cutoff
+    e->edit_cap = (uint32_t)std::min<uint64_t>(3ull * e->max_reps * e->ev_per_rep, 1ull << 22);
+    e->aff_cap = (uint32_t)std::min<uint64_t>(32ull * e->edit_cap, std::max<uint64_t>(slots, 1ull << 16));
+    e->wd_cap = e->aff_cap;
+    HIP_TRY(dalloc(&e->edits, e->edit_cap));
+    HIP_TRY(dalloc(&e->aff_list, e->aff_cap));
+    HIP_TRY(dalloc(&e->wd_list, e->wd_cap));
+    HIP_TRY(dalloc(&e->edit_count, 3));
+    HIP_TRY(hipMemsetAsync(e->edit_count, 0, 3 * sizeof(uint32_t), e->stream));
+    return RDY_OK;
+}
+
 int ensure_force_out(rdy_engine *e) {
     if (e->force_out) return RDY_OK;
     const uint64_t slots = e->max_cells * e->cap + e->ovf_cap;
@@ -534,7 +580,7 @@ int configure_bricks(rdy_engine *e, double lam) {
         e->lds_limit = (size_t)v;
     // static LDS of the brick kernels comes out of the same per-workgroup budget
     size_t static_lds = 0;
-    const void *kernels[] = {(const void *)k_brick<MODE_STEP, true>,   (const void *)k_brick<MODE_STEP, false>,
+    const void *kernels[] = {(const void *)k_brick_step<true>,         (const void *)k_brick_step<false>,
                              (const void *)k_brick<MODE_FORCES, true>, (const void *)k_brick<MODE_FORCES, false>,
                              (const void *)k_brick<MODE_REBIN, false>, (const void *)k_brick_search<true>,
                              (const void *)k_brick_search<false>};
@@ -616,51 +662,82 @@ void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, e->stream, e->grid(b));
 }
 
-// one fused step over all replicas: SimulationLoop.h:280-300 (integrate, NL, reactions, NL; the
-// forces of the resulting state are computed at the start of the next step, fused with its integration)
-void launch_step(rdy_engine *e, int handler, double dt) {
-    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 2, e->e_bonded);
-    launch_bonded(e);
+// workgroups of the fix-up kernels (their waves / threads stride over the change lists)
+constexpr uint32_t FIX_BLOCKS = 1024;
+
+// one walk of k_brick_step from cur into the other buffer (prologue: the first walk of a run)
+void launch_walk(rdy_engine *e, int prologue, int handler, double dt) {
     const int c = e->cur, n = 1 - c;
-    if (e->prof && e->prof_used < e->prof_ev.size())
+    if (e->prof && !prologue && e->prof_used < e->prof_ev.size())
         (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
     for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
-        hipLaunchKernelGGL((k_brick<MODE_STEP, decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
-                           e->stream, e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks,
-                           b0, e->cap, e->radius, dt, (const double *)nullptr, (double *)nullptr, e->max_states,
-                           e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
+        hipLaunchKernelGGL((k_brick_step<decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
+                           e->stream, e->grid(c), e->grid(n), prologue, e->reps, e->T, e->topo(),
+                           (const BrickDev *)e->d_bricks, b0, e->cap, e->radius, handler, dt, e->events, e->ev_count,
+                           e->gstage, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
     });
-    if (e->prof && e->prof_used < e->prof_ev.size()) {
+    if (e->prof && !prologue && e->prof_used < e->prof_ev.size()) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
     }
-    hipLaunchKernelGGL(k_brick_slow<MODE_STEP>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
-                       e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap, e->radius, dt,
-                       (const double *)nullptr, (double *)nullptr, slow_list(e, 0), e->err);
-    // reaction search on the new buffer, fused with clearing the old buffer's counters
+    hipLaunchKernelGGL(k_brick_step_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
+                       e->grid(n), prologue, e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
+                       e->radius, handler, dt, e->events, e->ev_count, slow_list(e, 0), e->err);
+}
+
+// the first walk of a run: the forces of the current state and step 1's move (bonded terms first)
+void launch_prologue(rdy_engine *e, double dt) {
+    launch_bonded(e);
+    launch_walk(e, 1, (int)RDY_HANDLER_NONE, dt);
+    e->wl_valid[1 - e->cur] = false;
+    e->cur = 1 - e->cur;
+}
+
+// the end of a run: the buffer the last walk read is cleared, and so are the bonded forces
+void launch_epilogue(rdy_engine *e) {
+    const uint64_t nc = std::max<uint64_t>(e->cells_hwm, 1);
+    const uint64_t nfb = e->fb ? 3 * e->topo_used : 0;
+    hipLaunchKernelGGL(k_clear_grid, dim3(blocks_for(std::max(nc, nfb), 256)), dim3(256), 0, e->stream,
+                       e->grid(1 - e->cur), (uint32_t)nc, e->fb, (uint32_t)nfb);
+}
+
+// one fused step over all replicas (SimulationLoop.h:280-300, rdy_step.hip): the walk over x'(t)
+// yields step t's candidates and, optimistically, step t+1's move; the reactions, the exact
+// corrections and the redone moves follow.  last: the launch's final step copies instead of moving.
+void launch_step(rdy_engine *e, int handler, double dt, int last) {
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    const int c = e->cur, n = 1 - c;
+    const uint64_t nc = std::max<uint64_t>(e->cells_hwm, 1);
+    const uint64_t nfb = e->fb ? 3 * e->topo_used : 0;
+    const uint64_t nthreads = std::max<uint64_t>(std::max<uint64_t>(nr, nc), nfb);
+    hipLaunchKernelGGL(k_step_begin, dim3(blocks_for(nthreads, 256)), dim3(256), 0, e->stream, e->reps, nr, last,
+                       e->grid(n), (uint32_t)nc, e->fb, (uint32_t)nfb, e->edit_count, e->e_bonded);
+    launch_bonded(e);
     const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
-    for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
-        hipLaunchKernelGGL((k_brick_search<decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
-                           e->stream, e->grid(n), e->grid(c), 1, e->reps, e->T, (const BrickDev *)e->d_bricks, b0,
-                           e->cap, e->radius, h, e->events, e->ev_count, e->max_states, e->max_halo,
-                           (uint32_t)e->brick_smem, slow_list(e, 1), e->err);
-    });
-    if (h != RDY_HANDLER_NONE)
-        hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(n),
-                           e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, h, e->events, e->ev_count,
-                           slow_list(e, 1), e->err);
-    if (h != RDY_HANDLER_NONE)
-        hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(n), e->reps, e->T,
+    launch_walk(e, 0, h, dt);
+    if (h != RDY_HANDLER_NONE) {
+        hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T,
                            e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 1, (uint32_t)e->max_particles,
-                           e->err);
+                           e->edits, e->edit_count, e->edit_cap, e->err);
+        hipLaunchKernelGGL(k_fix_sites, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c),
+                           (const ReplicaDev *)e->reps, e->T, e->cap, e->radius, (const Edit *)e->edits,
+                           (const uint32_t *)e->edit_count, e->edit_cap, e->aff_bits, e->aff_list, e->edit_count + 1,
+                           e->aff_cap, e->wd_list, e->edit_count + 2, e->wd_cap, e->err);
+        hipLaunchKernelGGL(k_fix_withdraw, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c), e->grid(n),
+                           (const ReplicaDev *)e->reps, e->cap, (const uint2 *)e->wd_list,
+                           (const uint32_t *)(e->edit_count + 2), e->wd_cap);
+        hipLaunchKernelGGL(k_fix_advance, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c), e->grid(n),
+                           (const ReplicaDev *)e->reps, e->T, e->topo(), e->cap, e->radius, dt, e->aff_bits,
+                           (const uint32_t *)e->aff_list, (const uint32_t *)(e->edit_count + 1), e->aff_cap, e->err);
+    }
     e->wl_valid[n] = false;
     e->cur = n;
 }
 
 void launch_forces(rdy_engine *e, double *force_out) {
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded,
+                       (uint32_t *)nullptr);
     launch_bonded(e);
     const int c = e->cur;
     for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
@@ -672,6 +749,16 @@ void launch_forces(rdy_engine *e, double *force_out) {
     hipLaunchKernelGGL(k_brick_slow<MODE_FORCES>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream,
                        e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
                        e->radius, 0.0, (const double *)nullptr, force_out, slow_list(e, 0), e->err);
+}
+
+// force_out and the energies after a fused run are formed when first asked for
+int ensure_export(rdy_engine *e) {
+    if (e->export_valid) return RDY_OK;
+    int rc = ensure_force_out(e);
+    if (rc) return rc;
+    launch_forces(e, e->force_out);
+    e->export_valid = true;
+    return RDY_OK;
 }
 
 int flush_prof(rdy_engine *e) {
@@ -803,7 +890,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->ovf_cell[b]);
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
-                    e->force_out, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
+                    e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
@@ -1043,7 +1130,10 @@ int rdy_get_particles(rdy_engine *e, uint32_t r, uint64_t cap, double *pos, uint
 }
 
 int rdy_get_forces(rdy_engine *e, uint32_t r, uint64_t cap, double *forces, uint32_t *handles, uint64_t *n) {
-    if (!e || !e->force_out) return set_err(RDY_ERR_STATE, "no forces have been calculated");
+    if (!e) return set_err(RDY_ERR_STATE, "no forces have been calculated");
+    int rc = ensure_export(e);
+    if (rc) return rc;
+    if (!e->force_out) return set_err(RDY_ERR_STATE, "no forces have been calculated");
     uint64_t dummy;
     return gather(e, r, cap, nullptr, nullptr, handles, forces, n ? n : &dummy);
 }
@@ -1118,12 +1208,15 @@ int rdy_calculate_forces(rdy_engine *e) {
     if (rc) return rc;
     if ((rc = ensure_force_out(e))) return rc;
     launch_forces(e, e->force_out);
+    e->export_valid = true;
     return check_errors(e);
 }
+
 
 int rdy_get_energy(rdy_engine *e, uint32_t r, double *energy) {
     int rc = check_replica(e, r);
     if (rc) return rc;
+    if ((rc = ensure_export(e))) return rc;
     unsigned long long pe = 0, be = 0;
     if ((rc = get_field(e, r, offsetof(ReplicaDev, energy), &pe))) return rc;
     HIP_TRY(hipMemcpyAsync(&be, e->e_bonded + r, 8, hipMemcpyDeviceToHost, e->stream));
@@ -1152,8 +1245,10 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
         dn = e->d_noise;
     }
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    if ((rc = ensure_export(e))) return rc;
     ensure_wl(e);
-    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 1, e->e_bonded);
+    hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 1, e->e_bonded,
+                       (uint32_t *)nullptr);
     hipLaunchKernelGGL(k_integrate, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
                        e->cap, dt, dn, (const double *)e->force_out, (uint32_t)e->max_particles, e->err);
     return check_errors(e);
@@ -1179,7 +1274,8 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
                        e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler, e->events,
                        e->ev_count, slow_list(e, 1), e->err);
     hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
-                       e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles, e->err);
+                       e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles,
+                       (Edit *)nullptr, (uint32_t *)nullptr, 0u, e->err);
     return check_errors(e);
 }
 
@@ -1207,15 +1303,19 @@ int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
     if (rc) return rc;
+    if (n_steps == 0) return RDY_OK;
+    if ((rc = ensure_step_pools(e))) return rc;
+    e->export_valid = false;
+    launch_prologue(e, dt);
     for (uint64_t s = 0; s < n_steps; ++s) {
-        launch_step(e, handler, dt);
+        launch_step(e, handler, dt, s + 1 == n_steps);
         if (e->prof && e->prof_used == e->prof_ev.size())
             if ((rc = flush_prof(e))) return rc;
         if ((s & 1023) == 1023)
             if ((rc = check_errors(e))) return rc;
     }
+    launch_epilogue(e);
     if ((rc = flush_prof(e))) return rc;
-    launch_forces(e, e->force_out);
     return check_errors(e);
 }
 
@@ -1225,6 +1325,10 @@ int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
     if (rc) return rc;
+    if (n_steps == 0) return RDY_OK;
+    if ((rc = ensure_step_pools(e))) return rc;
+    e->export_valid = false;
+    launch_prologue(e, dt);
     const bool prof = e->prof;
     e->prof = false;
     if (e->g_version != e->version || e->g_handler != handler || e->g_dt != dt) {
@@ -1238,8 +1342,8 @@ int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
             e->cur = b;
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            launch_step(e, handler, dt);
-            launch_step(e, handler, dt);
+            launch_step(e, handler, dt, 0);
+            launch_step(e, handler, dt, 0);
             HIP_TRY(hipStreamEndCapture(e->stream, &g));
             HIP_TRY(hipGraphInstantiate(&e->gexec[b], g, nullptr, nullptr, 0));
             HIP_TRY(hipGraphDestroy(g));
@@ -1249,16 +1353,18 @@ int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
         e->g_handler = handler;
         e->g_dt = dt;
     }
-    uint64_t pairs = n_steps / 2;
+    // every step but the last from the graphs (the last one copies instead of moving)
+    const uint64_t generic = n_steps - 1, pairs = generic / 2;
     for (uint64_t p = 0; p < pairs; ++p) {
         HIP_TRY(hipGraphLaunch(e->gexec[e->cur], e->stream));
         if ((p & 511) == 511)
             if ((rc = check_errors(e))) return rc;
     }
-    if (n_steps & 1) launch_step(e, handler, dt);
+    if (generic & 1) launch_step(e, handler, dt, 0);
+    launch_step(e, handler, dt, 1);
+    launch_epilogue(e);
     e->wl_valid[0] = e->wl_valid[1] = false;  // replays bypass launch_step's bookkeeping
     e->prof = prof;
-    launch_forces(e, e->force_out);
     return check_errors(e);
 }
 

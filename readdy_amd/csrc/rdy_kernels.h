@@ -58,9 +58,16 @@ struct Event {
     uint32_t pad;
 };
 
+// a record changed by a reaction of the fused step: slot in the current buffer, kind (bits 0-1:
+// killed 1, inserted 2, retyped 3) | type before the change << 2, replica, handle before the change
+// (retyped records)
+struct Edit {
+    uint32_t slot, info, r, pad;
+};
+
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
-             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, N_ERR = 8 };
+             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, N_ERR = 8 };
 
 constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
 constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer

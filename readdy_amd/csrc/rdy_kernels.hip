@@ -165,8 +165,10 @@ __device__ __forceinline__ uint32_t bin_insert(const Grid &g, const RepGeom &geo
 // -------------------------------------------------------------------------------------------------
 // mode 0: zero the energy accumulators (a force evaluation follows); mode 1: advance the Philox
 // step of replicas with budget left (separate integrator action); mode 2: both (fused step)
-__global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned long long *e_bonded) {
+__global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned long long *e_bonded,
+                          uint32_t *edit_count) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0 && edit_count) *edit_count = 0u;
     if (r >= n_reps) return;
     ReplicaDev &R = reps[r];
     if (mode != 1) {
@@ -652,10 +654,36 @@ __global__ __launch_bounds__(256) void k_search(Grid cur, ReplicaDev *reps, Tabl
 // -------------------------------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t hslot(uint32_t key, uint32_t mask) { return (key * 2654435761u) & mask; }
 
+// With edits (the fused step), resolve lists every record it kills, inserts or retypes, and flags
+// inserted and retyped records TW_NEW; k_fix_sites / k_fix_advance (rdy_step.hip) then bring the
+// other buffer to the post-reaction state.
+enum : uint32_t { EDIT_NONE = 0, EDIT_KILLED = 1, EDIT_INSERTED = 2, EDIT_RETYPED = 3 };
+
+__device__ __forceinline__ void fixed_add3(long long *g, long long x, long long y, long long z) {
+    atomicAdd((unsigned long long *)g, (unsigned long long)x);
+    atomicAdd((unsigned long long *)g + 1, (unsigned long long)y);
+    atomicAdd((unsigned long long *)g + 2, (unsigned long long)z);
+}
+
+// the k-th cell of the de-duplicated stencil axis of cell i (the set axis_cells lists)
+__device__ __forceinline__ int axis_count(int i, int D, int p, int r) {
+    if (p) return D <= 2 * r ? D : 2 * r + 1;
+    return min(D - 1, i + r) - max(0, i - r) + 1;
+}
+__device__ __forceinline__ int axis_elem(int k, int i, int D, int p, int r) {
+    if (p) {
+        if (D <= 2 * r) return k;
+        const int j = i - r + k;  // |overhang| <= r < D
+        return j < 0 ? j + D : (j >= D ? j - D : j);
+    }
+    return max(0, i - r) + k;
+}
+
 __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap,
                                                              Event *events, uint32_t *ev_count,
                                                              unsigned long long *rx_step, unsigned long long *rx_total,
-                                                             int require_active, uint32_t max_wl, uint32_t *err) {
+                                                             int require_active, uint32_t max_wl, Edit *edits,
+                                                             uint32_t *edit_count, uint32_t edit_cap, uint32_t *err) {
     constexpr int E = RESOLVE_MAX_EVENTS;
     constexpr int HT = 4 * E;  // hash table entries (power of two, > 2 slots per event)
     __shared__ unsigned long long skey[E];
@@ -800,6 +828,15 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
     __syncthreads();
     const uint32_t hbase = R.handle_next;
     const RepGeom geo = load_geom(R);
+    // the fused step lists the changed records (edits) and flags inserted / retyped ones TW_NEW
+    const bool fix = edits != nullptr;
+    const uint32_t tw_new = fix ? TW_NEW : 0u;
+    auto edit = [&](uint32_t slot, uint32_t kind, uint32_t old_type, uint32_t old_handle) {
+        if (!fix || slot == 0xFFFFFFFFu) return;
+        const uint32_t p = atomicAdd(edit_count, 1u);
+        if (p < edit_cap) edits[p] = Edit{slot, kind | (old_type << 2), r, old_handle};
+        else atomicAdd(&err[ERR_EDIT_OVERFLOW], 1u);
+    };
     for (uint32_t k = tid; k < ne; k += blockDim.x) {
         if (status[k] != 1) continue;
         const Event e = ev[sidx[k]];
@@ -811,23 +848,27 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
         switch (X.kind) {
         case RX_DECAY:
             cur.bins[e.s1].tw = a.tw | TW_DEAD;
+            edit(e.s1, EDIT_KILLED, 0, 0);
             break;
         case RX_CONVERSION: {
             State nw = a;
-            nw.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE);
+            nw.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE) | tw_new;
             nw.h = hnew;
             cur.bins[e.s1] = nw;
+            edit(e.s1, EDIT_RETYPED, tw_type(a.tw), a.h);
             break;
         }
         case RX_ENZYMATIC: {
             // ReactionUtils.h:174-190: the educt that is not the catalyst converts
             State bst = cur.bins[e.s2];
             if (tw_type(a.tw) == X.educts[1]) {
-                bst.tw = make_tw(X.products[0], bst.tw & TW_TOPO, r) | (bst.tw & TW_OUTSIDE);
+                edit(e.s2, EDIT_RETYPED, tw_type(bst.tw), bst.h);
+                bst.tw = make_tw(X.products[0], bst.tw & TW_TOPO, r) | (bst.tw & TW_OUTSIDE) | tw_new;
                 bst.h = hnew;
                 cur.bins[e.s2] = bst;
             } else {
-                a.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE);
+                edit(e.s1, EDIT_RETYPED, tw_type(a.tw), a.h);
+                a.tw = make_tw(X.products[0], flags_keep, r) | (a.tw & TW_OUTSIDE) | tw_new;
                 a.h = hnew;
                 cur.bins[e.s1] = a;
             }
@@ -848,14 +889,15 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
                 q1[d] = wrap(p[d] + X.w1 * X.product_distance * nv[d], geo.box[d], geo.pbc[d]);
             }
             p2.x = q2[0]; p2.y = q2[1]; p2.z = q2[2];
-            p2.tw = make_tw(X.products[1], 0u, r);
+            p2.tw = make_tw(X.products[1], tw_new, r);
             p2.h = hnew;
             p1.x = q1[0]; p1.y = q1[1]; p1.z = q1[2];
-            p1.tw = make_tw(X.products[0], 0u, r);
+            p1.tw = make_tw(X.products[0], tw_new, r);
             p1.h = hnew + 1u;
             cur.bins[e.s1].tw = a.tw | TW_DEAD;
-            bin_insert(cur, geo, cap, p2, err, true, max_wl);
-            bin_insert(cur, geo, cap, p1, err, true, max_wl);
+            edit(e.s1, EDIT_KILLED, 0, 0);
+            edit(bin_insert(cur, geo, cap, p2, err, true, max_wl), EDIT_INSERTED, 0, 0);
+            edit(bin_insert(cur, geo, cap, p1, err, true, max_wl), EDIT_INSERTED, 0, 0);
             break;
         }
         case RX_FUSION: {
@@ -868,11 +910,13 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
             pr.x = wrap(a.x + w * dx, geo.box[0], geo.pbc[0]);
             pr.y = wrap(a.y + w * dy, geo.box[1], geo.pbc[1]);
             pr.z = wrap(a.z + w * dz, geo.box[2], geo.pbc[2]);
-            pr.tw = make_tw(X.products[0], 0u, r);
+            pr.tw = make_tw(X.products[0], tw_new, r);
             pr.h = hnew;
             cur.bins[e.s1].tw = a.tw | TW_DEAD;
             cur.bins[e.s2].tw = bst.tw | TW_DEAD;
-            bin_insert(cur, geo, cap, pr, err, true, max_wl);
+            edit(e.s1, EDIT_KILLED, 0, 0);
+            edit(e.s2, EDIT_KILLED, 0, 0);
+            edit(bin_insert(cur, geo, cap, pr, err, true, max_wl), EDIT_INSERTED, 0, 0);
             break;
         }
         }

@@ -442,6 +442,54 @@ def test_fused_loop_equals_separate_actions():
     assert np.array_equal(e1.reaction_totals(r1), e2.reaction_totals(r2))
 
 
+@pytest.mark.parametrize("handler", [0, 1])
+def test_fused_loop_equals_separate_actions_every_reaction_kind(handler):
+    """A high-rate chemistry with every reaction kind (decay, conversion, enzymatic, fusion, fission)
+    kills, inserts and retypes tens of particles per step, often next to each other and across
+    species with different cutoffs.  The fused step stores the forces of the walk before the
+    reactions and corrects them around every edit; the result must equal integrate / NL / react /
+    NL / forces bit for bit, under UA and under Gillespie."""
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 0.8 + 0.1 * (a + b)]) for a in range(4) for b in range(a, 4)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 3, [10.0, 1.0, 0.5, 1.6]))
+    rxs = [dict(kind=L.RX_FUSION, educts=[0, 1], products=[2], rate=5.0, educt_distance=1.0),
+           dict(kind=L.RX_FISSION, educts=[2], products=[0, 1], rate=3.0, product_distance=0.9),
+           dict(kind=L.RX_ENZYMATIC, educts=[1, 2], products=[3, 2], rate=4.0, educt_distance=1.2),
+           dict(kind=L.RX_CONVERSION, educts=[3], products=[0], rate=2.0),
+           dict(kind=L.RX_DECAY, educts=[3], rate=0.5)]
+    diff = [1.0, 0.8, 0.5, 1.2]
+    rng = np.random.default_rng(17)
+    n, Lb, dt, steps = 2400, 16.0, 0.02, 25
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.choice(np.arange(4, dtype=np.uint32), n)
+
+    def make():
+        e = Engine(4, diff, potentials=pots, reactions=rxs)
+        r = e.add_replica([Lb] * 3, seed=31, max_particles=2 * n)
+        e.set_particles(r, pos, types)
+        return e, r
+
+    e1, r1 = make()
+    e1.run(handler, dt, steps)
+    e2, r2 = make()
+    e2.calculate_forces()
+    for _ in range(steps):
+        e2.integrate(dt)
+        e2.update_neighbor_list()
+        e2.react(handler, dt)
+        e2.update_neighbor_list()
+        e2.calculate_forces()
+    for x, y in zip(e1.get_particles(r1), e2.get_particles(r2)):
+        assert np.array_equal(x, y)
+    assert e1.energy(r1) == e2.energy(r2)
+    f1, h1 = e1.get_forces(r1)
+    f2, h2 = e2.get_forces(r2)
+    o1, o2 = np.argsort(h1), np.argsort(h2)
+    assert np.array_equal(h1[o1], h2[o2]) and np.array_equal(f1[o1], f2[o2])
+    totals = e1.reaction_totals(r1)
+    assert np.array_equal(totals, e2.reaction_totals(r2))
+    assert np.all(totals > 0) and totals.sum() > 40 * steps  # every kind fired, tens per step
+
+
 def test_graph_replay_equals_eager():
     """Graph replays, with host reads in between, track an eager engine step for step."""
     e1, pos, types = _c0_engine()
