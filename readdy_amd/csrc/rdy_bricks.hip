@@ -145,50 +145,76 @@ __device__ __forceinline__ float prefilter_thr(float cut2_f, bool any, float mar
 
 // stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
 // they do not fit (the caller then takes the slow path).  Also returns the number of inner records
-// (the brick's own cells), listed in m.inner.
+// (the brick's own cells), listed in m.inner.  Two global round trips: the counters of each thread's
+// contiguous chunk of halo cells, loaded together, then (after one scan) the records, spread evenly
+// over the threads and loaded together.
+constexpr uint32_t STAGE_UNROLL = 4;  // loads of one thread in flight together
+
 __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom &geo, const Frame &F, uint32_t cap,
                                 const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t *n_inner_out) {
     __shared__ uint32_t s_total;
     const uint32_t tid = threadIdx.x;
     const uint32_t nh = Hs.nh;
-    // 1. cell counters of the halo box
-    for (uint32_t hc = tid; hc < nh; hc += BRICK_THREADS) {
+    // the global cell of halo cell hc (periodic images wrapped), UINT32_MAX outside a non-periodic edge
+    auto halo_cell = [&](uint32_t hc) -> uint32_t {
         uint32_t hh[3];
         Hs.decode(hc, hh[0], hh[1], hh[2]);
         int32_t c[3];
-        bool inside = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             int32_t x = B.o[a] - B.h[a] + (int32_t)hh[a];
             if (x < 0 || x >= geo.dims[a]) {
-                if (geo.pbc[a]) x = x < 0 ? x + geo.dims[a] : x - geo.dims[a];  // |overhang| <= r < dims
-                else inside = false;
+                if (!geo.pbc[a]) return 0xFFFFFFFFu;
+                x = x < 0 ? x + geo.dims[a] : x - geo.dims[a];  // |overhang| <= r < dims
             }
             c[a] = x;
         }
-        uint32_t n = 0, cell = 0xFFFFFFFFu;
-        if (inside) {
-            cell = geo.cell_base + ((uint32_t)c[0] * (uint32_t)geo.dims[1] + (uint32_t)c[1]) * (uint32_t)geo.dims[2] +
-                   (uint32_t)c[2];
-            n = g.counts[cell];
-        }
-        m.hcnt[hc] = n;
-        m.hcell[hc] = cell;
-    }
-    __syncthreads();
-    // 2. exclusive prefix over the halo cells (each thread a contiguous chunk)
+        return geo.cell_base + ((uint32_t)c[0] * (uint32_t)geo.dims[1] + (uint32_t)c[1]) * (uint32_t)geo.dims[2] +
+               (uint32_t)c[2];
+    };
+    // 1. counters of this thread's halo cells c0 .. c0 + per - 1
     const uint32_t per = (nh + BRICK_THREADS - 1) / BRICK_THREADS;
     const uint32_t c0 = tid * per;
     uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (c0 + k < nh) sum += m.hcnt[c0 + k];
+    if (per <= STAGE_UNROLL) {
+        uint32_t cl[STAGE_UNROLL], n[STAGE_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
+            const uint32_t hc = c0 + u;
+            cl[u] = 0xFFFFFFFFu;
+            n[u] = 0;
+            if (u < per && hc < nh) {
+                cl[u] = halo_cell(hc);
+                if (cl[u] != 0xFFFFFFFFu) n[u] = g.counts[cl[u]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
+            const uint32_t hc = c0 + u;
+            if (u < per && hc < nh) {
+                m.hcnt[hc] = n[u];
+                m.hcell[hc] = cl[u];
+                sum += n[u];
+            }
+        }
+    } else {
+        for (uint32_t hc = c0; hc < c0 + per && hc < nh; ++hc) {
+            const uint32_t cl = halo_cell(hc);
+            const uint32_t n = cl != 0xFFFFFFFFu ? g.counts[cl] : 0u;
+            m.hcnt[hc] = n;
+            m.hcell[hc] = cl;
+            sum += n;
+        }
+    }
+    // 2. exclusive prefix over the halo cells; every record slot learns its halo cell
     uint32_t total;
     uint32_t off = block_exclusive_scan(sum, &total);
-    for (uint32_t k = 0; k < per; ++k)
-        if (c0 + k < nh) {
-            m.hstart[c0 + k] = off;
-            off += m.hcnt[c0 + k];
-        }
+    for (uint32_t hc = c0; hc < c0 + per && hc < nh; ++hc) {
+        m.hstart[hc] = off;
+        const uint32_t n = m.hcnt[hc];
+        for (uint32_t k = 0; k < n && off + k < max_states; ++k) m.hc_of[off + k] = (uint16_t)hc;
+        off += n;
+    }
     if (tid == 0) {
         m.hstart[nh] = total;
         s_total = total;
@@ -196,12 +222,25 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
     __syncthreads();
     total = s_total;
     if (total > max_states) return 0xFFFFFFFFu;
-    // 3. copy the records cell by cell
-    for (uint32_t hc = tid; hc < nh; hc += BRICK_THREADS) {
-        const uint32_t n = m.hcnt[hc];
-        if (n == 0) continue;
-        const uint32_t cell = m.hcell[hc];
-        const uint32_t j0 = m.hstart[hc];
+    // 3. the records: thread tid copies records tid, tid + 256, ...
+    auto fetch_slot = [&](uint32_t j, uint32_t &hc) -> uint32_t {
+        hc = m.hc_of[j];
+        const uint32_t k = j - m.hstart[hc], cell = m.hcell[hc];
+        if (k < cap) return cell * cap + k;
+        // the (k - cap)-th overflow record of a crowded cell (rare); beyond a truncated overflow
+        // list the counter runs ahead of the records: a dead placeholder
+        uint32_t no = *g.ovf_count;
+        no = no < g.ovf_cap ? no : g.ovf_cap;
+        for (uint32_t o = 0, seen = 0; o < no; ++o)
+            if (g.ovf_cell[o] == cell && seen++ == k - cap) return g.nbin + o;
+        return 0xFFFFFFFFu;
+    };
+    auto put = [&](uint32_t j, uint32_t hc, uint32_t s, const State &st) {
+        if (s == 0xFFFFFFFFu) {
+            m.P[j] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
+            m.slot[j] = 0xFFFFFFFFu;
+            return;
+        }
         uint32_t hh[3];
         Hs.decode(hc, hh[0], hh[1], hh[2]);
         double shift[3];  // unwraps halo images into the local frame
@@ -214,32 +253,29 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
                 else if (u >= geo.dims[a]) shift[a] = geo.box[a];
             }
         }
-        auto put = [&](uint32_t j, uint32_t s) {
-            const State st = g.bins[s];
-            m.slot[j] = s;
-            m.hc_of[j] = (uint16_t)hc;
-            const float lx = (float)((st.x - F.org[0]) + shift[0]);
-            const float ly = (float)((st.y - F.org[1]) + shift[1]);
-            const float lz = (float)((st.z - F.org[2]) + shift[2]);
-            m.P[j] = make_float4(lx, ly, lz, __uint_as_float(st.tw));
-        };
-        const uint32_t nb = n < cap ? n : cap;
-        for (uint32_t k = 0; k < nb; ++k) put(j0 + k, cell * cap + k);
-        if (n > cap) {  // overflow records of a crowded cell (rare): scan the overflow list
-            uint32_t no = *g.ovf_count;
-            no = no < g.ovf_cap ? no : g.ovf_cap;
-            uint32_t found = 0;
-            const uint32_t base = j0 + cap;
-            for (uint32_t o = 0; o < no && found < n - cap; ++o)
-                if (g.ovf_cell[o] == cell) {
-                    put(base + found, g.nbin + o);
-                    ++found;
-                }
-            for (; found < n - cap; ++found) {  // counters ahead of a truncated overflow list: dead records
-                m.P[base + found] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
-                m.slot[base + found] = 0xFFFFFFFFu;
-                m.hc_of[base + found] = (uint16_t)hc;
+        m.slot[j] = s;
+        const float lx = (float)((st.x - F.org[0]) + shift[0]);
+        const float ly = (float)((st.y - F.org[1]) + shift[1]);
+        const float lz = (float)((st.z - F.org[2]) + shift[2]);
+        m.P[j] = make_float4(lx, ly, lz, __uint_as_float(st.tw));
+    };
+    for (uint32_t j0 = 0; j0 < total; j0 += STAGE_UNROLL * BRICK_THREADS) {
+        uint32_t sl[STAGE_UNROLL], hcs[STAGE_UNROLL];
+        State st[STAGE_UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
+            const uint32_t j = j0 + u * BRICK_THREADS + tid;
+            sl[u] = 0xFFFFFFFFu;
+            hcs[u] = 0;
+            if (j < total) {
+                sl[u] = fetch_slot(j, hcs[u]);
+                if (sl[u] != 0xFFFFFFFFu) st[u] = g.bins[sl[u]];
             }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
+            const uint32_t j = j0 + u * BRICK_THREADS + tid;
+            if (j < total) put(j, hcs[u], sl[u], st[u]);
         }
     }
     // 4. the brick's own records: one contiguous range per inner (x, y) row

@@ -562,16 +562,49 @@ uint32_t *slow_list(const rdy_engine *e, int which) { return e->d_slow + (size_t
 uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 256u); }
 
 // brick shape and LDS budget from the expected particles per cell (overridable in the config)
-int configure_bricks(rdy_engine *e, double lam) {
-    if (e->brick[0] == 0 || e->brick[1] == 0 || e->brick[2] == 0) {
-        if (lam >= 3.0) {
-            e->brick[0] = 4, e->brick[1] = 4, e->brick[2] = 4;
-        } else if (lam >= 0.75) {
-            e->brick[0] = 8, e->brick[1] = 8, e->brick[2] = 4;
-        } else {
-            e->brick[0] = 8, e->brick[1] = 8, e->brick[2] = 8;
-        }
-    }
+// Brick shape for mean cell occupancy lam on the grid dims: the brick's own particles fill one round
+// of the block's lanes (mean + 3 sigma <= BRICK_THREADS; two rounds above 3 particles per cell,
+// where one-round bricks would be tiny), since a brick whose particles spill into a further round
+// keeps its whole block waiting on one wave.  Among those shapes the one with the fewest staged cells
+// over the whole grid (halos included) plus a fixed cost per brick (its round trips, counted as
+// 64 / lam cells) wins; ties go to the larger brick.
+void choose_brick(double lam, const int32_t *dims, int radius, uint32_t out[3]) {
+    lam = std::max(lam, 1.0 / 64.0);
+    const double x = (-3.0 + std::sqrt(9.0 + 4.0 * BRICK_THREADS)) / 2.0;  // sqrt of the one-round mean
+    const double vmax = (lam >= 3.0 ? 2.0 : 1.0) * x * x / lam;
+    double best = 0.0;
+    bool found = false;
+    for (uint32_t ex = 1; ex <= 16; ++ex)
+        for (uint32_t ey = 1; ey <= 16; ++ey)
+            for (uint32_t ez = 1; ez <= 16; ++ez) {
+                const double V = (double)ex * ey * ez;
+                const double halo = (double)(ex + 2 * radius) * (ey + 2 * radius) * (ez + 2 * radius);
+                if (V > vmax || halo * std::max(lam, 0.25) > 1500.0) continue;
+                double staged = 1.0, n_bricks = 1.0;
+                const uint32_t ee[3] = {ex, ey, ez};
+                for (int a = 0; a < 3; ++a) {
+                    const double D = (double)dims[a];
+                    if ((double)ee[a] + 2.0 * radius > D) {
+                        staged *= D;  // whole-axis brick: no halo
+                    } else {
+                        const double n = std::ceil(D / ee[a]);
+                        staged *= D + 2.0 * radius * n;
+                        n_bricks *= n;
+                    }
+                }
+                const double cost = staged + n_bricks * 64.0 / lam;
+                const uint32_t vol = ex * ey * ez, bvol = out[0] * out[1] * out[2];
+                if (!found || cost < best * (1.0 - 1e-9) || (cost <= best * (1.0 + 1e-9) && vol > bvol)) {
+                    best = cost;
+                    out[0] = ex, out[1] = ey, out[2] = ez;
+                    found = true;
+                }
+            }
+    if (!found) out[0] = out[1] = out[2] = 2;
+}
+
+int configure_bricks(rdy_engine *e, double lam, const int32_t *dims) {
+    if (e->brick[0] == 0 || e->brick[1] == 0 || e->brick[2] == 0) choose_brick(lam, dims, e->radius, e->brick);
     e->max_halo = 1;
     for (int a = 0; a < 3; ++a) e->max_halo *= e->brick[a] + 2u * (uint32_t)e->radius;
     if (e->max_halo > 60000) return set_err(RDY_ERR_INVALID, "brick plus halo exceeds 60000 cells");
@@ -935,7 +968,7 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
             uint32_t c = (uint32_t)std::ceil(lam + 7.0 * std::sqrt(lam) + 6.0);
             e->cap = std::max<uint32_t>(8, (c + 3) / 4 * 4);
         }
-        int rcc = configure_bricks(e, lam_raw);
+        int rcc = configure_bricks(e, lam_raw, dims);
         if (rcc) return rcc;
         int rc = alloc_pools(e);
         if (rc) return rc;

@@ -113,6 +113,50 @@ __global__ void k_clear_grid(Grid g, uint32_t n_cells, long long *fb, uint32_t n
     }
 }
 
+// Orders the brick's own records so that those of the longest-reach types come first (a stable
+// partition of m.inner): a wave then holds particles of like stencil reach, and the short-reach waves
+// finish early instead of idling behind their long-reach lanes.  tmp: n_inner halves of free LDS.
+__device__ void order_inner_by_reach(const BrickSmem &m, uint32_t n_inner, const float *reach_tab, uint32_t nt,
+                                     uint16_t *tmp) {
+    __shared__ float s_rmax;
+    if (threadIdx.x == 0) {
+        float r = -1.f;
+        for (uint32_t t = 0; t < nt; ++t) r = fmaxf(r, reach_tab[t]);
+        s_rmax = r;
+    }
+    __syncthreads();
+    const float rmax = s_rmax;
+    auto longest = [&](uint32_t j) -> bool {
+        const uint32_t tw = __float_as_uint(m.P[j].w);
+        return !(tw & TW_DEAD) && reach_tab[tw_type(tw)] >= rmax;
+    };
+    uint32_t n0 = 0;
+    for (uint32_t c0 = 0; c0 < n_inner; c0 += BRICK_THREADS) {  // count the long-reach records
+        const uint32_t ii = c0 + threadIdx.x;
+        uint32_t tot;
+        block_exclusive_scan(ii < n_inner && longest(m.inner[ii]) ? 1u : 0u, &tot);
+        n0 += tot;
+    }
+    uint32_t o0 = 0, o1 = n0;
+    for (uint32_t c0 = 0; c0 < n_inner; c0 += BRICK_THREADS) {
+        const uint32_t ii = c0 + threadIdx.x;
+        uint32_t j = 0;
+        bool f = false;
+        if (ii < n_inner) {
+            j = m.inner[ii];
+            f = longest(j);
+        }
+        uint32_t t0;
+        const uint32_t r0 = block_exclusive_scan(f ? 1u : 0u, &t0);
+        if (ii < n_inner) tmp[f ? o0 + r0 : o1 + (ii - c0 - r0)] = (uint16_t)j;
+        o0 += t0;
+        o1 += min(n_inner - c0, (uint32_t)BRICK_THREADS) - t0;
+    }
+    __syncthreads();
+    for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) m.inner[ii] = tmp[ii];
+    __syncthreads();
+}
+
 // The walk of one brick over cur = x'(t): pair forces, stored per slot when a fix-up may need them
 // (gforce[3 s ..], 2^-32 fixed point); step t's reaction candidates (order 1 and 2, the draws of
 // k_brick_search); and each particle moved to step t+1 (or copied) into nxt, fwd[s] = its new slot.
@@ -176,6 +220,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
         if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
         return;
     }
+    const float *thr_tab = o2_on ? st.thr : ft.thr;
+    const float *reach_tab = o2_on ? st.reach : ft.reach;
+    if (n_inner > BRICK_THREADS && 2 * n_inner <= transient_bytes(max_halo))  // block-uniform
+        order_inner_by_reach(m, n_inner, reach_tab, nt, (uint16_t *)m.rings);
     const uint32_t r = B.r, step = R.step;
     const uint64_t seed = R.seed;
     auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key) {
@@ -194,8 +242,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
             atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
         }
     };
-    const float *thr_tab = o2_on ? st.thr : ft.thr;
-    const float *reach_tab = o2_on ? st.reach : ft.reach;
     long long *gs = gstage + 3ull * max_states * bidx;  // n_inner <= max_states rows of this brick
     ForceWave &W = wave_lds[threadIdx.x / 64];
     const uint32_t lane = __lane_id();
