@@ -728,7 +728,30 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
         sidx[k] = k;
     }
     __syncthreads();
-    // bitonic sort of (key, rx, h1, h2) ascending
+    // sort of (key, rx, h1, h2) ascending.  Keys (Philox words) are distinct but for a vanishing
+    // chance: the rank of each key among all keys (every thread reads the same key at once) places
+    // it directly; equal keys, detected by a doubly taken rank, fall back to a bitonic network.
+    for (uint32_t k = tid; k < ne; k += blockDim.x) hkeys[k] = 0u;
+    __syncthreads();
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        const unsigned long long key = skey[k];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < ne; ++j) rank += skey[j] < key ? 1u : 0u;
+        hmin[k] = rank;
+        atomicAdd(&hkeys[rank], 1u);
+    }
+    if (tid == 0) s_undecided = 0;
+    __syncthreads();
+    for (uint32_t k = tid; k < ne; k += blockDim.x)
+        if (hkeys[k] != 1u) s_undecided = 1;  // an equal key
+    __syncthreads();
+    const bool ranked = s_undecided == 0;  // block-uniform
+    if (ranked) {
+        for (uint32_t k = tid; k < ne; k += blockDim.x) hkeys[hmin[k]] = k;
+        __syncthreads();
+        for (uint32_t k = tid; k < ne; k += blockDim.x) sidx[k] = hkeys[k];
+        __syncthreads();
+    }
     auto less = [&](uint32_t a, uint32_t b) -> bool {
         if (skey[a] != skey[b]) return skey[a] < skey[b];
         uint32_t ia = sidx[a], ib = sidx[b];
@@ -738,7 +761,7 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
         if (ea.h1 != eb.h1) return ea.h1 < eb.h1;
         return ea.h2 < eb.h2;
     };
-    for (uint32_t kk = 2; kk <= np2; kk <<= 1) {
+    for (uint32_t kk = 2; !ranked && kk <= np2; kk <<= 1) {
         for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
             for (uint32_t a = tid; a < np2; a += blockDim.x) {
                 uint32_t bb = a ^ j;
@@ -816,14 +839,18 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
         prefix[k] = c;
     }
     __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t k = 0; k < ne; ++k) {
-            uint32_t c = prefix[k];
+    {  // exclusive scan: each thread a contiguous chunk of events
+        const uint32_t per = (ne + blockDim.x - 1) / blockDim.x, k0 = tid * per;
+        uint32_t sum = 0;
+        for (uint32_t k = k0; k < k0 + per && k < ne; ++k) sum += prefix[k];
+        uint32_t tot;
+        uint32_t acc = block_exclusive_scan(sum, &tot);
+        for (uint32_t k = k0; k < k0 + per && k < ne; ++k) {
+            const uint32_t c = prefix[k];
             prefix[k] = acc;
             acc += c;
         }
-        s_total = acc;
+        if (tid == 0) s_total = tot;
     }
     __syncthreads();
     const uint32_t hbase = R.handle_next;
