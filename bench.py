@@ -8,10 +8,10 @@ rate 0.1 radius 1.0, fission C->A+B rate 0.05 distance 1.0, D_A = D_B = 1, D_C =
 EulerBD dt=0.005, UncontrolledApproximation.  FP32 pair arithmetic on FP64 positions.
 
 Prints one JSON line.  `value` is whole-job particle-steps/s with the state resident in HBM;
-`roofline` prices the step's dominant kernel, the stencil walk k_brick_fs (pair forces + reaction
-candidates), at the 32 bytes of particle state per particle it must read, against MI355X's 8 TB/s;
-`step_roofline_frac` prices the whole step at 64 algorithmic bytes per particle-step (the state read
-and written once).  `cpu_baseline` is the FP64 CPU oracle timed on a bounded sample of the same
+`roofline` prices the step's dominant kernel, the stencil walk k_brick_step (pair forces, reaction
+candidates and the next Euler-Maruyama move of one brick of cells), at 64 algorithmic bytes per
+particle (its FP64 position, species and handle read once and written once), against MI355X's
+8 TB/s; `step_roofline_frac` prices the whole step the same way.  `cpu_baseline` is the FP64 CPU oracle timed on a bounded sample of the same
 workload on this host.
 """
 import argparse
@@ -29,9 +29,9 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M particles; batched ensemble throughput + p99 vs plain"
 HBM_PEAK_GBS = 8000.0
 B_ALG = 64.0  # bytes per particle-step: FP64 position + uint32 species + uint32 handle, read + written
-B_WALK = 32.0  # bytes per particle the stencil-walk kernel must read: the same record, read once
-PMC_PROFILE = "brick_fs_c1_1M_pmc.md"
-WALK_KERNEL = "k_brick_fs<true> (pair forces + reaction candidates of one brick of cells)"
+B_WALK = 64.0  # bytes per particle of the stencil-walk kernel: the same record, read once and written once
+PMC_PROFILE = "brick_step_c1_1M_pmc.md"
+WALK_KERNEL = "k_brick_step<true> (pair forces, reaction candidates and the next move of one brick of cells)"
 
 
 def c1_system(n=1_000_000, L=200.0, seed=1):
