@@ -113,6 +113,90 @@ __global__ void k_clear_grid(Grid g, uint32_t n_cells, long long *fb, uint32_t n
     }
 }
 
+// The type tables of the walk, read into registers before the brick and its replica are known (one
+// entry per thread: n_types <= MAX_STAGED_TYPES gives n_types^2 <= BRICK_THREADS), so that their loads
+// overlap the brick's, then placed in LDS with the thresholds of the brick's frame: the force tables
+// (ft) and the order-2 reaction index (st.o2); st.thr / st.reach hold the larger of the force and
+// reaction thresholds / reaches.
+struct TablePrefetch {  // raw words (registers, not a stack object)
+    uint4 pi;
+    uint2 ri;
+    uint4 pot0, pot1, pot2;
+};
+static_assert(sizeof(PairIdx) == 16 && sizeof(RxPairIdx) == 8 && sizeof(PotDev) == 48, "table entry sizes");
+
+__device__ __forceinline__ TablePrefetch prefetch_tables(const Tables &T) {
+    TablePrefetch f{};
+    const uint32_t k = threadIdx.x, n2 = T.n_types * T.n_types;
+    if (k < n2) {
+        f.pi = reinterpret_cast<const uint4 *>(T.pidx)[k];
+        f.ri = reinterpret_cast<const uint2 *>(T.o2idx)[k];
+    }
+    if (k < T.n_pots) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(T.pots) + 3ull * k;
+        f.pot0 = q[0];
+        f.pot1 = q[1];
+        f.pot2 = q[2];
+    }
+    return f;
+}
+
+__device__ __forceinline__ void place_tables(char *base, const Tables &T, const TablePrefetch &f, float margin,
+                                             ForceTables &ft, SearchTables &st) {
+    const uint32_t nt = T.n_types, n2 = nt * nt, k = threadIdx.x;
+    char *p = base;
+    ft.pidx = (PairIdx *)p;
+    p += align16(sizeof(PairIdx) * n2);
+    ft.pots = (PotDev *)p;
+    p += align16(sizeof(PotDev) * T.n_pots);
+    ft.thr = (float *)p;
+    p += align16(4 * n2);
+    ft.reach = (float *)p;
+    p = base + force_table_bytes(nt, T.n_pots);
+    st.o2 = (RxPairIdx *)p;
+    p += align16(sizeof(RxPairIdx) * n2);
+    st.thr = (float *)p;
+    p += align16(4 * n2);
+    st.reach = (float *)p;
+    if (k < n2) {
+        reinterpret_cast<uint4 *>(ft.pidx)[k] = f.pi;
+        reinterpret_cast<uint2 *>(st.o2)[k] = f.ri;
+        const PairIdx pi = ft.pidx[k];
+        const RxPairIdx ri = st.o2[k];
+        const float tf = prefilter_thr(pi.maxcut2_f, pi.count != 0, margin);
+        ft.thr[k] = tf;
+        st.thr[k] = fmaxf(tf, prefilter_thr(ri.maxr2_f, ri.count != 0, margin));
+    }
+    if (k < T.n_pots) {
+        uint4 *q = reinterpret_cast<uint4 *>(ft.pots) + 3ull * k;
+        q[0] = f.pot0;
+        q[1] = f.pot1;
+        q[2] = f.pot2;
+    }
+    for (uint32_t q = k + BRICK_THREADS; q < T.n_pots; q += BRICK_THREADS) ft.pots[q] = T.pots[q];
+    __syncthreads();
+    if (k < nt) {
+        float mf = 0.f, mr = 0.f;
+        bool af = false, ar = false;
+        for (uint32_t u = 0; u < nt; ++u) {
+            const PairIdx pi = ft.pidx[k * nt + u];
+            const RxPairIdx ri = st.o2[k * nt + u];
+            if (pi.count) {
+                af = true;
+                mf = fmaxf(mf, pi.maxcut2_f);
+            }
+            if (ri.count) {
+                ar = true;
+                mr = fmaxf(mr, ri.maxr2_f);
+            }
+        }
+        const float rf = reach_of(mf, af, margin);
+        ft.reach[k] = rf;
+        st.reach[k] = fmaxf(rf, reach_of(mr, ar, margin));
+    }
+    __syncthreads();
+}
+
 // Orders the brick's own records so that those of the longest-reach types come first (a stable
 // partition of m.inner): a wave then holds particles of like stencil reach, and the short-reach waves
 // finish early instead of idling behind their long-reach lanes.  tmp: n_inner halves of free LDS.
@@ -172,6 +256,9 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
                                                               uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ ForceWave wave_lds[BRICK_WAVES];
+    const bool staged_types = T.n_types <= MAX_STAGED_TYPES;
+    TablePrefetch tpf;
+    if (staged_types) tpf = prefetch_tables(T);
     const uint32_t bidx = brick0 + xcd_block(blockIdx.x, gridDim.x);
     const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
@@ -197,17 +284,13 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
     BrickSmem m;
     ForceTables ft;
     SearchTables st;
-    if (nt <= MAX_STAGED_TYPES) {
+    if (staged_types) {
         if (table_bytes(nt, T.n_pots) + brick_smem_bytes(max_states, max_halo) > smem_bytes) {  // block-uniform
             if (threadIdx.x == 0) atomicAdd(&err[ERR_LDS_LAYOUT], 1u);
             return;
         }
-        ft = load_force_tables(smem, T, F.margin);
-        st = load_search_tables(smem + force_table_bytes(nt, T.n_pots), T, F.margin);
         // with reactions on, the walk reaches the larger of the force and reaction cutoffs
-        for (uint32_t k = threadIdx.x; k < nt * nt; k += BRICK_THREADS) st.thr[k] = fmaxf(st.thr[k], ft.thr[k]);
-        if (threadIdx.x < nt) st.reach[threadIdx.x] = fmaxf(st.reach[threadIdx.x], ft.reach[threadIdx.x]);
-        __syncthreads();
+        place_tables(smem, T, tpf, F.margin, ft, st);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
         total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
     }
