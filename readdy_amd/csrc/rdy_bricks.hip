@@ -45,6 +45,7 @@ constexpr int BRICK_THREADS = 256;
 constexpr int BRICK_WAVES = BRICK_THREADS / 64;
 constexpr uint32_t BATCH = 4;       // candidates a lane takes from its current run per iteration
 constexpr uint32_t RING = 512;      // per-wave ring of prefilter survivors (power of two, >= 63 + 64 * BATCH)
+constexpr uint32_t NO_ENTRY = 0xFFFFFFFFu;  // an idle lane's entry in the last batch of the ring
 
 __host__ __device__ inline uint32_t div_magic(uint32_t d) {
     return d > 1 ? (uint32_t)((0x100000000ull + d - 1) / d) : 0u;
@@ -475,7 +476,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // staged record (halo cell hc, local position p, reach), taking up to BATCH candidates of its
 // current run per iteration (independent LDS reads), and offers each to keep(q); survivors are
 // appended as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the
-// wave.  Must be called by all 64 lanes of a wave (wave-uniform control flow around it).
+// wave.  The last, partial batch too runs eval on every lane, idle lanes with NO_ENTRY (so that eval
+// may read other lanes' registers).  Must be called by all 64 lanes of a wave (wave-uniform control
+// flow around it).
 template <bool ALL_HALO, class Keep, class Eval>
 __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
                                               uint32_t hc, const float4 &p, float reach, Keep &&keep, Eval &&eval) {
@@ -516,7 +519,7 @@ __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *
     }
     if (tail != head) {
         wave_lds_sync();
-        if (lane < tail - head) eval(ring[(head + lane) & (RING - 1)]);
+        eval(lane < tail - head ? ring[(head + lane) & (RING - 1)] : NO_ENTRY);
         wave_lds_sync();
     }
 }
@@ -836,6 +839,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         const uint32_t lane = __lane_id();
         // the exact pair force of a survivor (entry = owner lane << 16 | staged neighbour)
         auto eval = [&](uint32_t ent) {
+            if (ent == NO_ENTRY) return;
             const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
             const State me = cur.bins[W.s[ol]];
             const State o = cur.bins[m.slot[q]];
@@ -1015,6 +1019,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
     const uint32_t lane = __lane_id();
     // the exact test and draws of a surviving pair (entry = owner lane << 16 | staged neighbour)
     auto eval = [&](uint32_t ent) {
+        if (ent == NO_ENTRY) return;
         const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
         const uint32_t s1 = W.s[ol], s2 = m.slot[q];
         const State me = cur.bins[s1];

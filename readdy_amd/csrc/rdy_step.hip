@@ -330,10 +330,19 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
     const uint32_t lane = __lane_id();
     // a surviving pair (entry = owner lane << 16 | staged neighbour): the exact FP64 gates, the
     // pair force on the owner, and (each unordered pair once) the order-2 draws
+    State me{};  // this lane's own record in the current round (the walk's owner records)
     auto eval = [&](uint32_t ent) {
-        const uint32_t ol = ent >> 16, q = ent & 0xFFFFu;
+        // the owner's record from the owner lane's registers (every lane takes part in the shuffles)
+        const uint32_t ol = ent == NO_ENTRY ? 0u : ent >> 16, q = ent & 0xFFFFu;
+        State me_o;
+        me_o.x = __shfl(me.x, (int)ol);
+        me_o.y = __shfl(me.y, (int)ol);
+        me_o.z = __shfl(me.z, (int)ol);
+        me_o.tw = __shfl(me.tw, (int)ol);
+        me_o.h = __shfl(me.h, (int)ol);
+        if (ent == NO_ENTRY) return;
+        const State &me = me_o;
         const uint32_t s1 = W.s[ol], s2 = m.slot[q];
-        const State me = cur.bins[s1];
         const State o = cur.bins[s2];
         const uint32_t t1 = tw_type(me.tw), t2 = tw_type(o.tw);
         // CPUCalculateForces.cpp:226-259: x_ij = shortestDifference(x_i, x_j), gate d2 < rc^2.
@@ -378,7 +387,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
             pj = m.P[j];
             valid = !(__float_as_uint(pj.w) & TW_DEAD);
         }
-        State me{};
+        me = State{};
         uint32_t sj = 0;
         if (valid) {
             sj = m.slot[j];
