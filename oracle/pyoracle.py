@@ -58,6 +58,7 @@ def lib():
                                       ctypes.c_double, ctypes.c_double]
         L.or_set_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_id_counter.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_add_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _u64p]
         L.or_add_bond.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
                                   ctypes.c_double]
@@ -247,6 +248,10 @@ class System:
 
     def set_id_counter(self, next_id):
         lib().or_set_id_counter(self._h, next_id)
+
+    def set_threads(self, n):
+        """OpenMP threads for forces, integration and the keyed candidate search (default 1)"""
+        lib().or_set_threads(self._h, int(n))
 
     def add_particles(self, pos, types, ids=None):
         pos, pp = _d(np.asarray(pos, dtype=np.float64).reshape(-1, 3))

@@ -465,6 +465,7 @@ struct or_system {
     size_t *list;      /* [n + 1] */
     size_t list_len;
     uint64_t counts[OR_MAX_RX];
+    int threads; /* > 1: forces, integration and the candidate search on that many OpenMP threads */
 };
 
 or_system *or_create(int n_types, const double *box, const int *pbc, double kbt, const double *diffusion) {
@@ -481,6 +482,7 @@ or_system *or_create(int n_types, const double *box, const int *pbc, double kbt,
     s->kbt = kbt;
     for (int t = 0; t < n_types; ++t) s->D[t] = diffusion[t];
     s->seed = 0;
+    s->threads = 1;
     return s;
 }
 
@@ -535,6 +537,16 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
 
 int or_set_seed(or_system *s, uint64_t seed) {
     s->seed = seed;
+    return 0;
+}
+
+/* Thread count of the timing baseline (the reference's CPU kernel runs a static-partition thread
+ * pool, kernels/cpu/include/readdy/kernel/cpu/util/config.h).  Forces and positions are the same
+ * for any count; the energy is summed in another order (FP64 rounding may differ in the last bits)
+ * and the keyed handlers' candidates are listed in another order (both sort them by their keys
+ * before use).  The literal Gillespie path depends on list order and gathers on one thread. */
+int or_set_threads(or_system *s, int n) {
+    s->threads = n < 1 ? 1 : n;
     return 0;
 }
 
@@ -867,7 +879,24 @@ int or_calculate_forces(or_system *s, double *energy, double *virial9) {
     }
     if (s->n_pots > 0) {
         if (!s->nl_set) return fail("forces need a neighbour list");
-        for_each_pair(s, force_visit, &u);
+        if (s->threads > 1) {  /* each particle sums its own pairs: static partition over cells */
+#pragma omp parallel num_threads(s->threads)
+            {
+                force_ud local;
+                memset(&local, 0, sizeof local);
+#pragma omp for schedule(static)
+                for (size_t c = 0; c < s->n_cells; ++c)
+                    for (size_t p1 = s->head[c]; p1 != 0; p1 = s->list[p1])
+                        for_each_neighbor(s, p1 - 1, c, force_visit, &local);
+#pragma omp critical
+                {
+                    u.energy += local.energy;
+                    for (int a = 0; a < 9; ++a) u.virial[a] += local.virial[a];
+                }
+            }
+        } else {
+            for_each_pair(s, force_visit, &u);
+        }
     }
     if (energy) *energy = e_top + u.energy;
     if (virial9) memcpy(virial9, u.virial, sizeof u.virial);
@@ -880,13 +909,17 @@ int or_calculate_forces(or_system *s, double *energy, double *virial9) {
  * --------------------------------------------------------------------------------------------- */
 int or_euler_bd(or_system *s, double dt, const double *noise_by_id, size_t noise_len, uint32_t step) {
     double kbt = s->kbt;
+    if (noise_by_id)
+        for (size_t i = 0; i < s->n; ++i)
+            if (!s->e[i].deactivated && 3 * s->e[i].id + 2 >= noise_len)
+                return fail("noise buffer too short for particle id");
+#pragma omp parallel for schedule(static) num_threads(s->threads) if (s->threads > 1)
     for (size_t i = 0; i < s->n; ++i) {
         entry_t *en = &s->e[i];
         if (en->deactivated) continue;
         double D = s->D[en->type];
         double xi[3];
         if (noise_by_id) {
-            if (3 * en->id + 2 >= noise_len) return fail("noise buffer too short for particle id");
             xi[0] = noise_by_id[3 * en->id];
             xi[1] = noise_by_id[3 * en->id + 1];
             xi[2] = noise_by_id[3 * en->id + 2];
@@ -1004,7 +1037,26 @@ static void gather(or_system *s, ev_ud *u) {
             push_event(u, &e);
         }
     }
-    if (s->nl_set) for_each_pair(s, rx_visit, u);
+    if (!s->nl_set) return;
+    if (s->threads > 1 && u->draw) {  /* per-thread candidate lists, appended in thread order */
+#pragma omp parallel num_threads(s->threads)
+        {
+            ev_ud local = *u;
+            local.ev = NULL;
+            local.n = local.cap = 0;
+#pragma omp for schedule(static)
+            for (size_t c = 0; c < s->n_cells; ++c)
+                for (size_t p1 = s->head[c]; p1 != 0; p1 = s->list[p1]) for_each_neighbor(s, p1 - 1, c, rx_visit, &local);
+#pragma omp for ordered schedule(static, 1)
+            for (int t = 0; t < s->threads; ++t) {
+#pragma omp ordered
+                for (size_t k = 0; k < local.n; ++k) push_event(u, &local.ev[k]);
+            }
+            free(local.ev);
+        }
+    } else {
+        for_each_pair(s, rx_visit, u);
+    }
 }
 
 size_t or_reaction_candidates(or_system *s, size_t cap, uint64_t *out) {

@@ -72,6 +72,7 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
                     double educt_distance, double product_distance, double weight1, double weight2);
 int or_set_seed(or_system *s, uint64_t seed);
 int or_set_id_counter(or_system *s, uint64_t next_id);
+int or_set_threads(or_system *s, int n);
 int or_add_particles(or_system *s, int n, const double *pos, const int *types, const uint64_t *ids);
 int or_add_bond(or_system *s, size_t i1, size_t i2, double k, double l0);
 int or_add_angle(or_system *s, size_t i1, size_t i2, size_t i3, double k, double theta0);

@@ -427,3 +427,33 @@ def test_ua_conserves_complex_counts():
         _, types, _, _ = s.particles()
         nA, nB, nAB = [(types == k).sum() for k in range(3)]
         assert nA + nAB == 50 and nB + nAB == 50
+
+
+@pytest.mark.parametrize("mode", [po.UA_KEYED, po.GILLESPIE_KEYED])
+def test_threaded_timing_baseline_equals_one_thread(mode):
+    # the multithreaded oracle (timing baseline only) follows the one-thread run exactly: each particle
+    # sums its own pairs in the same order, and the keyed handlers sort candidates by their keys
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(-6, 6, (2000, 3))
+    types = rng.integers(0, 3, 2000).astype(np.int32)
+    runs = []
+    for threads in (1, 4):
+        s = po.System(3, [12, 12, 12], [1, 1, 1], diffusion=[1.0, 1.0, 0.5])
+        s.set_seed(3)
+        for a in range(3):
+            for b in range(a, 3):
+                s.add_pair_potential(po.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0])
+        s.add_pair_potential(po.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0])
+        s.add_reaction(po.RX_FUSION, [0, 1], [2], 5.0, 1.0)
+        s.add_reaction(po.RX_FISSION, [2], [0, 1], 2.0, 0.0, 1.0)
+        s.add_particles(pos, types, ids=np.arange(2000))
+        s.set_threads(threads)
+        s.nl_setup(s.max_cutoff())
+        s.calculate_forces()
+        energies = [s.loop_step(0.01, mode, t) for t in range(5)]
+        runs.append((s.particles(), energies, s.reaction_counts()))
+    (p1, e1, c1), (p4, e4, c4) = runs
+    for a, b in zip(p1, p4):
+        assert np.array_equal(a, b)
+    assert np.array_equal(c1, c4) and c1.sum() > 0
+    np.testing.assert_allclose(e1, e4, rtol=1e-12)
