@@ -38,8 +38,13 @@ struct BrickDev {
     // ceil(2^32 / d) for d = H1 * H2, H2 and e1 (H = e + 2h; 0 when d == 1): exact n / d as a high
     // multiply for n, d < 2^16
     uint32_t m_hh, m_h2, m_e1;
-    uint32_t pad[2];
+    // the replica's grid, so that the halo's cells are known before the replica's record arrives
+    uint32_t cell_base;
+    int32_t dims[3];
+    uint32_t pbc;    // bit a: the replica's axis a is periodic
+    uint32_t pad;
 };
+static_assert(sizeof(BrickDev) == 80, "brick record size");
 
 constexpr int BRICK_THREADS = 256;
 constexpr int BRICK_WAVES = BRICK_THREADS / 64;
@@ -144,84 +149,147 @@ __device__ __forceinline__ float prefilter_thr(float cut2_f, bool any, float mar
     return any ? cut2_f * (1.0f + 3.0e-6f) + 2.0f * sqrtf(cut2_f) * margin + margin * margin : -1.0f;
 }
 
-// stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
-// they do not fit (the caller then takes the slow path).  Also returns the number of inner records
-// (the brick's own cells), listed in m.inner.  Two global round trips: the counters of each thread's
-// contiguous chunk of halo cells, loaded together, then (after one scan) the records, spread evenly
-// over the threads and loaded together.
+// the global cell of halo cell hc of brick B (periodic images wrapped), UINT32_MAX outside a
+// non-periodic edge; from the brick record alone
+__device__ __forceinline__ uint32_t halo_global_cell(const BrickDev &B, const Halo &Hs, uint32_t hc) {
+    uint32_t hh[3];
+    Hs.decode(hc, hh[0], hh[1], hh[2]);
+    int32_t c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int32_t x = B.o[a] - B.h[a] + (int32_t)hh[a];
+        if (x < 0 || x >= B.dims[a]) {
+            if (!(B.pbc & (1u << a))) return 0xFFFFFFFFu;
+            x = x < 0 ? x + B.dims[a] : x - B.dims[a];  // |overhang| <= r < dims
+        }
+        c[a] = x;
+    }
+    return B.cell_base + ((uint32_t)c[0] * (uint32_t)B.dims[1] + (uint32_t)c[1]) * (uint32_t)B.dims[2] + (uint32_t)c[2];
+}
+
+// Staging a brick: the counters of each thread's contiguous chunk of halo cells (one global round
+// trip, issued from the brick record alone, so that they travel with the replica's record), one
+// block scan that places every halo cell's records and lists the brick's own (inner) records, then
+// the records themselves, spread evenly over the threads (the second round trip).
 constexpr uint32_t STAGE_UNROLL = 4;  // loads of one thread in flight together
 
+struct HaloCounts {  // this thread's chunk of halo cells (registers): global cell and counter
+    uint32_t cl[STAGE_UNROLL], n[STAGE_UNROLL];
+};
+
+// chunk of thread tid: halo cells tid * per .. tid * per + per - 1 (per <= STAGE_UNROLL; larger halos
+// load their counters inside stage_brick)
+__device__ __forceinline__ HaloCounts load_halo_counts(const Grid &g, const BrickDev &B, const Halo &Hs) {
+    HaloCounts h;
+    const uint32_t nh = Hs.nh, per = (nh + BRICK_THREADS - 1) / BRICK_THREADS, c0 = threadIdx.x * per;
+#pragma unroll
+    for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
+        h.cl[u] = 0xFFFFFFFFu;
+        h.n[u] = 0;
+        if (per <= STAGE_UNROLL && u < per && c0 + u < nh) {
+            h.cl[u] = halo_global_cell(B, Hs, c0 + u);
+            if (h.cl[u] != 0xFFFFFFFFu) h.n[u] = g.counts[h.cl[u]];
+        }
+    }
+    return h;
+}
+
+// inclusive-exclusive pair scan: the exclusive prefixes of a and b over the block and their totals
+// (two values, one pair of barriers)
+__device__ __forceinline__ uint2 block_exclusive_scan2(uint32_t a, uint32_t b, uint2 *total) {
+    __shared__ uint2 wave_sums[BRICK_WAVES];
+    const uint32_t lane = __lane_id(), wid = threadIdx.x / 64;
+    uint32_t x = a, y = b;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t xa = __shfl_up(x, off), ya = __shfl_up(y, off);
+        if (lane >= (uint32_t)off) {
+            x += xa;
+            y += ya;
+        }
+    }
+    if (lane == 63) wave_sums[wid] = make_uint2(x, y);
+    __syncthreads();
+    uint2 base = make_uint2(0u, 0u), tot = make_uint2(0u, 0u);
+#pragma unroll
+    for (uint32_t w = 0; w < BRICK_WAVES; ++w) {
+        const uint2 ws = wave_sums[w];
+        if (w < wid) {
+            base.x += ws.x;
+            base.y += ws.y;
+        }
+        tot.x += ws.x;
+        tot.y += ws.y;
+    }
+    __syncthreads();
+    *total = tot;
+    return make_uint2(base.x + x - a, base.y + y - b);
+}
+
+// stages buffer g around brick B into LDS; returns the number of staged records, or UINT32_MAX if
+// they do not fit (the caller then takes the slow path).  Also returns the number of inner records
+// (the brick's own cells), listed in m.inner in halo order.  hc: load_halo_counts of this brick.
 __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom &geo, const Frame &F, uint32_t cap,
-                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, uint32_t *n_inner_out) {
-    __shared__ uint32_t s_total;
+                                const BrickSmem &m, uint32_t max_states, const Halo &Hs, const HaloCounts &hcn,
+                                uint32_t *n_inner_out) {
+    __shared__ uint32_t s_total, s_inner;
     const uint32_t tid = threadIdx.x;
     const uint32_t nh = Hs.nh;
-    // the global cell of halo cell hc (periodic images wrapped), UINT32_MAX outside a non-periodic edge
-    auto halo_cell = [&](uint32_t hc) -> uint32_t {
-        uint32_t hh[3];
-        Hs.decode(hc, hh[0], hh[1], hh[2]);
-        int32_t c[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            int32_t x = B.o[a] - B.h[a] + (int32_t)hh[a];
-            if (x < 0 || x >= geo.dims[a]) {
-                if (!geo.pbc[a]) return 0xFFFFFFFFu;
-                x = x < 0 ? x + geo.dims[a] : x - geo.dims[a];  // |overhang| <= r < dims
-            }
-            c[a] = x;
-        }
-        return geo.cell_base + ((uint32_t)c[0] * (uint32_t)geo.dims[1] + (uint32_t)c[1]) * (uint32_t)geo.dims[2] +
-               (uint32_t)c[2];
-    };
-    // 1. counters of this thread's halo cells c0 .. c0 + per - 1
     const uint32_t per = (nh + BRICK_THREADS - 1) / BRICK_THREADS;
     const uint32_t c0 = tid * per;
-    uint32_t sum = 0;
+    // a halo cell is inner when every coordinate lies in the brick's own range
+    auto inner_cell = [&](uint32_t hc) -> bool {
+        uint32_t hh[3];
+        Hs.decode(hc, hh[0], hh[1], hh[2]);
+        bool in = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) in = in && hh[a] >= (uint32_t)B.h[a] && hh[a] < (uint32_t)(B.h[a] + B.e[a]);
+        return in;
+    };
+    // 1. counters of this thread's halo cells c0 .. c0 + per - 1 (already loaded for small halos)
+    uint32_t sum = 0, sum_in = 0;
     if (per <= STAGE_UNROLL) {
-        uint32_t cl[STAGE_UNROLL], n[STAGE_UNROLL];
-#pragma unroll
-        for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
-            const uint32_t hc = c0 + u;
-            cl[u] = 0xFFFFFFFFu;
-            n[u] = 0;
-            if (u < per && hc < nh) {
-                cl[u] = halo_cell(hc);
-                if (cl[u] != 0xFFFFFFFFu) n[u] = g.counts[cl[u]];
-            }
-        }
 #pragma unroll
         for (uint32_t u = 0; u < STAGE_UNROLL; ++u) {
             const uint32_t hc = c0 + u;
             if (u < per && hc < nh) {
-                m.hcnt[hc] = n[u];
-                m.hcell[hc] = cl[u];
-                sum += n[u];
+                m.hcnt[hc] = hcn.n[u];
+                m.hcell[hc] = hcn.cl[u];
+                sum += hcn.n[u];
+                if (inner_cell(hc)) sum_in += hcn.n[u];
             }
         }
     } else {
         for (uint32_t hc = c0; hc < c0 + per && hc < nh; ++hc) {
-            const uint32_t cl = halo_cell(hc);
+            const uint32_t cl = halo_global_cell(B, Hs, hc);
             const uint32_t n = cl != 0xFFFFFFFFu ? g.counts[cl] : 0u;
             m.hcnt[hc] = n;
             m.hcell[hc] = cl;
             sum += n;
+            if (inner_cell(hc)) sum_in += n;
         }
     }
-    // 2. exclusive prefix over the halo cells; every record slot learns its halo cell
-    uint32_t total;
-    uint32_t off = block_exclusive_scan(sum, &total);
+    // 2. exclusive prefixes over the halo cells: every record slot learns its halo cell, and the
+    // inner records are listed (in halo order)
+    uint2 tot;
+    const uint2 off2 = block_exclusive_scan2(sum, sum_in, &tot);
+    uint32_t off = off2.x, off_in = off2.y;
     for (uint32_t hc = c0; hc < c0 + per && hc < nh; ++hc) {
         m.hstart[hc] = off;
         const uint32_t n = m.hcnt[hc];
         for (uint32_t k = 0; k < n && off + k < max_states; ++k) m.hc_of[off + k] = (uint16_t)hc;
+        if (inner_cell(hc)) {
+            for (uint32_t k = 0; k < n && off_in + k < max_states; ++k) m.inner[off_in + k] = (uint16_t)(off + k);
+            off_in += n;
+        }
         off += n;
     }
     if (tid == 0) {
-        m.hstart[nh] = total;
-        s_total = total;
+        m.hstart[nh] = tot.x;
+        s_total = tot.x;
+        s_inner = tot.y;
     }
     __syncthreads();
-    total = s_total;
+    const uint32_t total = s_total;
     if (total > max_states) return 0xFFFFFFFFu;
     // 3. the records: thread tid copies records tid, tid + 256, ...
     auto fetch_slot = [&](uint32_t j, uint32_t &hc) -> uint32_t {
@@ -279,26 +347,8 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
             if (j < total) put(j, hcs[u], sl[u], st[u]);
         }
     }
-    // 4. the brick's own records: one contiguous range per inner (x, y) row
-    const uint32_t e1 = (uint32_t)B.e[1], e2 = (uint32_t)B.e[2];
-    const uint32_t nrow = (uint32_t)B.e[0] * e1;
-    uint32_t n_inner = 0;
-    for (uint32_t r0 = 0; r0 < nrow; r0 += BRICK_THREADS) {
-        const uint32_t r = r0 + tid;
-        uint32_t a = 0, c = 0;
-        if (r < nrow) {
-            const uint32_t rx = fdiv(r, B.m_e1), ry = r - rx * e1;
-            const uint32_t row = (((uint32_t)B.h[0] + rx) * Hs.H1 + ((uint32_t)B.h[1] + ry)) * Hs.H2 + (uint32_t)B.h[2];
-            a = m.hstart[row];
-            c = m.hstart[row + e2] - a;
-        }
-        uint32_t tot;
-        const uint32_t o = n_inner + block_exclusive_scan(c, &tot);
-        for (uint32_t k = 0; k < c; ++k) m.inner[o + k] = (uint16_t)(a + k);
-        n_inner += tot;
-    }
     __syncthreads();
-    *n_inner_out = n_inner;
+    *n_inner_out = s_inner;
     return total;
 }
 
@@ -827,7 +877,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         }
         ft = load_force_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
-        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, load_halo_counts(cur, B, Hs), &n_inner);
     }
     if (total != 0xFFFFFFFFu) {
         const WalkCtx wc = make_walk(B, geo, Hs, radius);
@@ -986,7 +1036,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
         }
         st = load_search_tables(smem, T, F.margin);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
-        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, load_halo_counts(cur, B, Hs), &n_inner);
     }
     if (total == 0xFFFFFFFFu) {
         list_slow_brick(slow, bidx);

@@ -118,6 +118,7 @@ struct rdy_engine {
     RxDev *d_rx = nullptr;
     uint32_t *d_o1idx = nullptr, *d_o1list = nullptr, *d_o2list = nullptr;
     RxPairIdx *d_o2idx = nullptr;
+    float2 *d_rowmax = nullptr;
     double *d_D = nullptr;
     uint8_t *d_flavor = nullptr;
     std::vector<RxDev> rx_h;
@@ -325,6 +326,15 @@ int upload_tables(rdy_engine *e, double dt) {
         has_o2 = has_o2 || !o2[k].empty();
         o2list.insert(o2list.end(), o2[k].begin(), o2[k].end());
     }
+    std::vector<float2> rowmax(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        float mf = -1.f, mr = -1.f;
+        for (uint32_t u = 0; u < nt; ++u) {
+            if (pidx[(size_t)t * nt + u].count) mf = std::max(mf, pidx[(size_t)t * nt + u].maxcut2_f);
+            if (o2idx[(size_t)t * nt + u].count) mr = std::max(mr, o2idx[(size_t)t * nt + u].maxr2_f);
+        }
+        rowmax[t] = make_float2(mf, mr);
+    }
     auto up = [&](auto **dst, const auto &v) -> int {
         using T = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
         if (*dst) HIP_TRY(hipFree(*dst));
@@ -340,6 +350,7 @@ int upload_tables(rdy_engine *e, double dt) {
     if ((rc = up(&e->d_o1list, o1list))) return rc;
     if ((rc = up(&e->d_o2idx, o2idx))) return rc;
     if ((rc = up(&e->d_o2list, o2list))) return rc;
+    if ((rc = up(&e->d_rowmax, rowmax))) return rc;
     if ((rc = up(&e->d_D, e->D))) return rc;
     if (!e->d_flavor) {
         if ((rc = up(&e->d_flavor, e->topo_flavor))) return rc;
@@ -352,6 +363,7 @@ int upload_tables(rdy_engine *e, double dt) {
     e->T.o2idx = e->d_o2idx;
     e->T.o2list = e->d_o2list;
     e->T.diffusion = e->d_D;
+    e->T.rowmax = e->d_rowmax;
     e->T.n_types = nt;
     e->T.n_rx = e->n_rx;
     e->T.has_o2 = has_o2 ? 1u : 0u;
@@ -505,6 +517,11 @@ void make_bricks(rdy_engine *e, uint32_t r, std::vector<BrickDev> &out) {
                 for (int a = 0; a < 3; ++a) b.h[a] = (int32_t)halo[a];
                 b.r = r;
                 b.whole = whole;
+                for (int a = 0; a < 3; ++a) {
+                    b.dims[a] = R.dims[a];
+                    if (R.pbc[a]) b.pbc |= 1u << a;
+                }
+                b.cell_base = R.cell_base;
                 const uint32_t H1 = (uint32_t)(b.e[1] + 2 * b.h[1]), H2 = (uint32_t)(b.e[2] + 2 * b.h[2]);
                 b.m_hh = div_magic(H1 * H2);
                 b.m_h2 = div_magic(H2);
@@ -924,7 +941,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);

@@ -27,6 +27,8 @@ struct Tables {
     const RxPairIdx *o2idx;  // [n_types * n_types]
     const uint32_t *o2list;
     const double *diffusion; // [n_types]
+    const float2 *rowmax;    // [n_types] largest FP32 squared cutoff of the type's potentials (x) and of
+                             // its order-2 reactions (y); -1 where it has none
     uint32_t n_types;
     uint32_t n_rx;
     uint32_t has_o2;         // any order-2 reaction

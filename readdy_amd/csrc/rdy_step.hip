@@ -117,11 +117,14 @@ __global__ void k_clear_grid(Grid g, uint32_t n_cells, long long *fb, uint32_t n
 // entry per thread: n_types <= MAX_STAGED_TYPES gives n_types^2 <= BRICK_THREADS), so that their loads
 // overlap the brick's, then placed in LDS with the thresholds of the brick's frame: the force tables
 // (ft) and the order-2 reaction index (st.o2); st.thr / st.reach hold the larger of the force and
-// reaction thresholds / reaches.
+// reaction thresholds / reaches.  Every entry is formed from the placing thread's own words (the
+// reaches from the per-type row maxima), so the placement needs no barrier of its own: the staging
+// that follows has barriers before anything reads the tables.
 struct TablePrefetch {  // raw words (registers, not a stack object)
     uint4 pi;
     uint2 ri;
     uint4 pot0, pot1, pot2;
+    float2 rm;
 };
 static_assert(sizeof(PairIdx) == 16 && sizeof(RxPairIdx) == 8 && sizeof(PotDev) == 48, "table entry sizes");
 
@@ -138,6 +141,7 @@ __device__ __forceinline__ TablePrefetch prefetch_tables(const Tables &T) {
         f.pot1 = q[1];
         f.pot2 = q[2];
     }
+    if (k < T.n_types) f.rm = T.rowmax[k];
     return f;
 }
 
@@ -174,27 +178,11 @@ __device__ __forceinline__ void place_tables(char *base, const Tables &T, const 
         q[2] = f.pot2;
     }
     for (uint32_t q = k + BRICK_THREADS; q < T.n_pots; q += BRICK_THREADS) ft.pots[q] = T.pots[q];
-    __syncthreads();
     if (k < nt) {
-        float mf = 0.f, mr = 0.f;
-        bool af = false, ar = false;
-        for (uint32_t u = 0; u < nt; ++u) {
-            const PairIdx pi = ft.pidx[k * nt + u];
-            const RxPairIdx ri = st.o2[k * nt + u];
-            if (pi.count) {
-                af = true;
-                mf = fmaxf(mf, pi.maxcut2_f);
-            }
-            if (ri.count) {
-                ar = true;
-                mr = fmaxf(mr, ri.maxr2_f);
-            }
-        }
-        const float rf = reach_of(mf, af, margin);
+        const float rf = reach_of(f.rm.x, f.rm.x >= 0.f, margin);
         ft.reach[k] = rf;
-        st.reach[k] = fmaxf(rf, reach_of(mr, ar, margin));
+        st.reach[k] = fmaxf(rf, reach_of(f.rm.y, f.rm.y >= 0.f, margin));
     }
-    __syncthreads();
 }
 
 // Orders the brick's own records so that those of the longest-reach types come first (a stable
@@ -261,6 +249,9 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
     if (staged_types) tpf = prefetch_tables(T);
     const uint32_t bidx = brick0 + xcd_block(blockIdx.x, gridDim.x);
     const BrickDev B = bricks[bidx];
+    // the halo counters load from the brick record alone, alongside the replica's record
+    const Halo Hs = make_halo(B);
+    const HaloCounts hcn = load_halo_counts(cur, B, Hs);
     const ReplicaDev &R = reps[B.r];
     if (!R.in_use) return;
     const RepGeom geo = load_geom(R);
@@ -279,7 +270,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
     const bool o2_on = rx_on && T.has_o2;
     const uint32_t nt = T.n_types;
     const Frame F = make_frame(B, geo);
-    const Halo Hs = make_halo(B);
     uint32_t total = 0xFFFFFFFFu, n_inner = 0;
     BrickSmem m;
     ForceTables ft;
@@ -292,7 +282,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
         // with reactions on, the walk reaches the larger of the force and reaction cutoffs
         place_tables(smem, T, tpf, F.margin, ft, st);
         m = carve(smem + table_bytes(nt, T.n_pots), max_states, max_halo);
-        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, &n_inner);
+        total = stage_brick(cur, B, geo, F, cap, m, max_states, Hs, hcn, &n_inner);
     }
     if (total == 0xFFFFFFFFu) {
         list_slow_brick(slow, bidx);
