@@ -716,7 +716,9 @@ void launch_clear_bricks(rdy_engine *e, int b, int grid_of_bricks_buffer) {
 constexpr uint32_t FIX_BLOCKS = 1024;
 
 // one walk of k_brick_step from cur into the other buffer (prologue: the first walk of a run)
-void launch_walk(rdy_engine *e, int prologue, int handler, double dt) {
+// the walk of a step (or the prologue); slow_later: the bricks listed for the bin-reading path are
+// walked by k_resolve_step instead of a follow-up launch
+void launch_walk(rdy_engine *e, int prologue, int handler, double dt, bool slow_later = false) {
     const int c = e->cur, n = 1 - c;
     if (e->prof && !prologue && e->prof_used < e->prof_ev.size())
         (void)hipEventRecord(e->prof_ev[e->prof_used].first, e->stream);
@@ -730,6 +732,7 @@ void launch_walk(rdy_engine *e, int prologue, int handler, double dt) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
         e->prof_used++;
     }
+    if (slow_later) return;
     hipLaunchKernelGGL(k_brick_step_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
                        e->grid(n), prologue, e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
                        e->radius, handler, dt, e->events, e->ev_count, slow_list(e, 0), e->err);
@@ -764,11 +767,12 @@ void launch_step(rdy_engine *e, int handler, double dt, int last) {
                        e->grid(n), (uint32_t)nc, e->fb, (uint32_t)nfb, e->edit_count, e->e_bonded);
     launch_bonded(e);
     const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
-    launch_walk(e, 0, h, dt);
+    launch_walk(e, 0, h, dt, h != RDY_HANDLER_NONE);
     if (h != RDY_HANDLER_NONE) {
-        hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T,
-                           e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 1, (uint32_t)e->max_particles,
-                           e->edits, e->edit_count, e->edit_cap, e->err);
+        hipLaunchKernelGGL(k_resolve_step, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->grid(n),
+                           e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap, e->radius, h, dt,
+                           e->events, e->ev_count, slow_list(e, 0), e->rx_step, e->rx_total,
+                           (uint32_t)e->max_particles, e->edits, e->edit_count, e->edit_cap, e->err);
         hipLaunchKernelGGL(k_fix_sites, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c),
                            (const ReplicaDev *)e->reps, e->T, e->cap, e->radius, (const Edit *)e->edits,
                            (const uint32_t *)e->edit_count, e->edit_cap, e->aff_bits, e->aff_list, e->edit_count + 1,

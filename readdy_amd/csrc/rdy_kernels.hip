@@ -541,7 +541,7 @@ __device__ __forceinline__ unsigned long long event_key(int handler, const uint3
 }
 
 // candidates of the particle in slot s (searching from the smaller (type, bin slot) of every pair)
-__device__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap, int radius,
+__device__ __forceinline__ void search_one(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap, int radius,
                            int handler, int candidates_only, Event *events, uint32_t *ev_count, uint32_t *cand_rows,
                            uint32_t cand_cap, uint32_t s, const State &me, uint32_t *err) {
     const uint32_t r = tw_replica(me.tw);
@@ -679,11 +679,12 @@ __device__ __forceinline__ int axis_elem(int k, int i, int D, int p, int r) {
     return max(0, i - r) + k;
 }
 
-__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap,
-                                                             Event *events, uint32_t *ev_count,
-                                                             unsigned long long *rx_step, unsigned long long *rx_total,
-                                                             int require_active, uint32_t max_wl, Edit *edits,
-                                                             uint32_t *edit_count, uint32_t edit_cap, uint32_t *err) {
+// the reactions of replica r = blockIdx.x (one workgroup of RESOLVE_THREADS)
+__device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap,
+                                                Event *events, uint32_t *ev_count, unsigned long long *rx_step,
+                                                unsigned long long *rx_total, int require_active, uint32_t max_wl,
+                                                Edit *edits, uint32_t *edit_count, uint32_t edit_cap,
+                                                uint32_t *err) {
     constexpr int E = RESOLVE_MAX_EVENTS;
     constexpr int HT = 4 * E;  // hash table entries (power of two, > 2 slots per event)
     __shared__ unsigned long long skey[E];
@@ -957,6 +958,15 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
         R.handle_next = hbase + s_total;
         ev_count[r] = 0;
     }
+}
+
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap,
+                                                             Event *events, uint32_t *ev_count,
+                                                             unsigned long long *rx_step, unsigned long long *rx_total,
+                                                             int require_active, uint32_t max_wl, Edit *edits,
+                                                             uint32_t *edit_count, uint32_t edit_cap, uint32_t *err) {
+    resolve_replica(cur, reps, T, cap, events, ev_count, rx_step, rx_total, require_active, max_wl, edits, edit_count,
+                    edit_cap, err);
 }
 
 // marks every particle of replica r dead in the current buffer

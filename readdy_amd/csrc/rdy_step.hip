@@ -460,6 +460,42 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step_slow(Grid cur, Gri
     release_slow_list(slow);
 }
 
+// A fused step's reactions (resolve_replica, one workgroup per replica), preceded by the replica's
+// bricks that k_brick_step listed for the bin-reading path (their candidates join the replica's list
+// first).  The walk then needs no follow-up launch, which costs as much as a short kernel's work.
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve_step(Grid cur, Grid nxt, ReplicaDev *reps, Tables T,
+                                                                  Topo topo, const BrickDev *bricks, uint32_t cap,
+                                                                  int radius, int handler, double dt,
+                                                                  Event *events, uint32_t *ev_count, uint32_t *slow,
+                                                                  unsigned long long *rx_step,
+                                                                  unsigned long long *rx_total, uint32_t max_wl,
+                                                                  Edit *edits, uint32_t *edit_count,
+                                                                  uint32_t edit_cap, uint32_t *err) {
+    const uint32_t n = slow_count(slow);  // the same for every block: the walk has ended
+    for (uint32_t i = 0; i < n; ++i) {    // block-uniform
+        const BrickDev B = bricks[slow[2 + i]];
+        if (B.r != blockIdx.x) continue;
+        const ReplicaDev &R = reps[B.r];
+        const RepGeom geo = load_geom(R);
+        const bool integ = R.integ != 0u;
+        const uint32_t step_next = R.step + 1u;
+        for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
+            if (me.tw & TW_DEAD) return;
+            long long fx = 0, fy = 0, fz = 0, e = 0;
+            if (!(me.tw & TW_OUTSIDE)) global_forces(cur, geo, cap, radius, T, s, me, fx, fy, fz, e);
+            if (R.active) search_one(cur, reps, T, cap, radius, handler, 0, events, ev_count, nullptr, 0, s, me, err);
+            advance_particle(nxt, R, geo, T, topo, cap, dt, step_next, integ, me, fx, fy, fz, err);
+        });
+    }
+    if (n) {
+        __syncthreads();
+        __threadfence();  // the listed bricks' candidates before the replica's list is read
+    }
+    resolve_replica(cur, reps, T, cap, events, ev_count, rx_step, rx_total, 1, max_wl, edits, edit_count, edit_cap,
+                    err);
+    if (n) release_slow_list(slow);
+}
+
 // the cells of the (2r+1)^3 stencil of local cell cl, de-duplicated as axis_cells does, spread over
 // the lanes of a wave: fn(cell) for each cell of this lane
 template <class Fn>
