@@ -181,6 +181,15 @@ int rdy_reaction_candidates(rdy_engine *e, uint32_t replica, uint64_t cap, uint3
 
 /* observables: particles per type for every replica, [n_replicas][n_types] */
 int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_replicas);
+/* radial distribution counts of one replica, replacing the pair loop of RadialDistribution::evaluate
+ * (readdy/main/model/observables/RadialDistribution.cpp:75-103): every ordered pair of live particles
+ * (from, to), from's type in from_types and to's in to_types, adds one to counts[b] where borders[b] <=
+ * |shortestDifference| < borders[b + 1] (upper_bound); *n_from = live particles of the from types.
+ * 2 <= n_borders <= 4097, borders ascending.  The distribution itself, counts / (4/3 pi (r_b+1^3 -
+ * r_b^3) n_from particleToDensity) (:105-123), is formed by the caller. */
+int rdy_rdf_counts(rdy_engine *e, uint32_t replica, const double *borders, uint32_t n_borders,
+                   const uint32_t *from_types, uint32_t n_from_types, const uint32_t *to_types,
+                   uint32_t n_to_types, uint64_t *counts, uint64_t *n_from);
 
 /* kernel timing of the dominant (fused force/integrate/rebin) kernel, measured with HIP
  * events on the engine's stream: average microseconds over the launches since the last reset */

@@ -59,6 +59,8 @@ def lib():
         L.or_set_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_id_counter.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_rdf_counts.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_int, _u64p,
+                                    _u64p]
         L.or_add_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _u64p]
         L.or_add_bond.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_double,
                                   ctypes.c_double]
@@ -248,6 +250,17 @@ class System:
 
     def set_id_counter(self, next_id):
         lib().or_set_id_counter(self._h, next_id)
+
+    def rdf_counts(self, borders, from_types, to_types):
+        """RadialDistribution counts (brute force over ordered pairs) and the number of 'from' particles"""
+        b, bp = _d(np.asarray(borders, dtype=np.float64))
+        f = np.ascontiguousarray(from_types, dtype=np.int32)
+        t = np.ascontiguousarray(to_types, dtype=np.int32)
+        counts = np.zeros(max(len(b) - 1, 0), dtype=np.uint64)
+        nf = np.zeros(1, dtype=np.uint64)
+        _check(lib().or_rdf_counts(self._h, bp, len(b), f.ctypes.data_as(_ip), len(f), t.ctypes.data_as(_ip), len(t),
+                                   counts.ctypes.data_as(_u64p), nf.ctypes.data_as(_u64p)))
+        return counts, int(nf[0])
 
     def set_threads(self, n):
         """OpenMP threads for forces, integration and the keyed candidate search (default 1)"""

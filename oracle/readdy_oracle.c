@@ -550,6 +550,40 @@ int or_set_threads(or_system *s, int n) {
     return 0;
 }
 
+int or_rdf_counts(const or_system *s, const double *borders, int n_borders, const int *from_types, int n_from,
+                  const int *to_types, int n_to, uint64_t *counts, uint64_t *n_from_out) {
+    if (n_borders < 2) return fail("bin borders need at least two entries");
+    for (int b = 0; b + 1 < n_borders; ++b) counts[b] = 0;
+    uint64_t nf = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        const entry_t *p = &s->e[i];
+        if (p->deactivated) continue;
+        int is_from = 0;
+        for (int k = 0; k < n_from; ++k) is_from |= p->type == from_types[k];
+        if (!is_from) continue;
+        ++nf;
+        for (size_t j = 0; j < s->n; ++j) {
+            const entry_t *q = &s->e[j];
+            if (q->deactivated || q->id == p->id) continue;
+            int is_to = 0;
+            for (int k = 0; k < n_to; ++k) is_to |= q->type == to_types[k];
+            if (!is_to) continue;
+            const double dist = sqrt(or_dist2(p->pos, q->pos, s->box, s->pbc));
+            /* std::upper_bound: the first border greater than dist */
+            int lo = 0, hi = n_borders;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (dist < borders[mid]) hi = mid;
+                else lo = mid + 1;
+            }
+            /* lo == 0 (below the first border) would index counts[-1] in the reference; not counted here */
+            if (lo > 0 && lo < n_borders) counts[lo - 1]++;
+        }
+    }
+    *n_from_out = nf;
+    return 0;
+}
+
 int or_set_id_counter(or_system *s, uint64_t next_id) {
     s->next_id = next_id;
     return 0;

@@ -73,6 +73,12 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
 int or_set_seed(or_system *s, uint64_t seed);
 int or_set_id_counter(or_system *s, uint64_t next_id);
 int or_set_threads(or_system *s, int n);
+/* RadialDistribution::evaluate counts (RadialDistribution.cpp:75-103): every ordered pair (from, to) of
+ * live particles with distinct ids, type of from in from_types and of to in to_types, adds one to the bin
+ * upper_bound(borders, sqrt(distSquared)) - 1 when the distance lies below the last border; *n_from is
+ * the number of live particles of the from types.  Brute force over all pairs. */
+int or_rdf_counts(const or_system *s, const double *borders, int n_borders, const int *from_types, int n_from,
+                  const int *to_types, int n_to, uint64_t *counts, uint64_t *n_from_out);
 int or_add_particles(or_system *s, int n, const double *pos, const int *types, const uint64_t *ids);
 int or_add_bond(or_system *s, size_t i1, size_t i2, double k, double l0);
 int or_add_angle(or_system *s, size_t i1, size_t i2, size_t i3, double k, double theta0);

@@ -1,6 +1,6 @@
 // Host side of the MI355X ReaDDy engine: device pools, replica admission, the per-step launch
 // sequence (eager and hipGraph), and the extern "C" ABI declared in include/readdy_amd.h.
-#include "rdy_step.hip"
+#include "rdy_observe.hip"
 
 #include "../../include/readdy_amd.h"
 
@@ -1509,6 +1509,74 @@ int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_reps) {
     HIP_TRY(hipMemcpyAsync(counts, d, words * 8, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return RDY_OK;
+}
+
+int rdy_rdf_counts(rdy_engine *e, uint32_t r, const double *borders, uint32_t n_borders, const uint32_t *from_types,
+                   uint32_t n_from_types, const uint32_t *to_types, uint32_t n_to_types, uint64_t *counts,
+                   uint64_t *n_from) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (!borders || !counts || !n_from || n_borders < 2 || n_borders > RDF_MAX_BORDERS)
+        return set_err(RDY_ERR_INVALID, "radial distribution: 2 to 4097 bin borders and output arrays required");
+    for (uint32_t b = 0; b + 1 < n_borders; ++b)
+        if (!(borders[b] <= borders[b + 1])) return set_err(RDY_ERR_INVALID, "bin borders must ascend");
+    const uint32_t words = (e->n_types + 31) / 32;
+    std::vector<uint32_t> masks(2 * (size_t)words, 0u);
+    auto fill = [&](const uint32_t *t, uint32_t n, uint32_t *m) -> int {
+        for (uint32_t k = 0; k < n; ++k) {
+            if (t[k] >= e->n_types) return set_err(RDY_ERR_INVALID, "radial distribution: no such species");
+            m[t[k] >> 5] |= 1u << (t[k] & 31u);
+        }
+        return RDY_OK;
+    };
+    if ((rc = fill(from_types, n_from_types, masks.data()))) return rc;
+    if ((rc = fill(to_types, n_to_types, masks.data() + words))) return rc;
+    // stencil radius per axis: every cell within the last border of the particle's cell
+    const ReplicaDev &R = e->reps_h[r].dev;
+    const double rmax = borders[n_borders - 1];
+    int rad[3];
+    for (int a = 0; a < 3; ++a) {
+        const double c = std::ceil(rmax / R.cs[a]);
+        rad[a] = (int)std::min<double>(std::max(c, 1.0), (double)R.dims[a]);
+    }
+    const uint32_t outside_cap = (uint32_t)std::min<uint64_t>(e->max_particles, 1u << 20);
+    const size_t nbins = n_borders - 1;
+    size_t off_b = 0, off_m = off_b + 8 * (size_t)n_borders, off_c = (off_m + 8 * words + 15) & ~(size_t)15;
+    size_t off_n = off_c + 8 * nbins, off_o = off_n + 16, bytes = off_o + 4 * (size_t)outside_cap;
+    if ((rc = ensure_scratch(e, bytes))) return rc;
+    char *base = (char *)e->scratch;
+    double *d_b = (double *)(base + off_b);
+    uint32_t *d_m = (uint32_t *)(base + off_m);
+    unsigned long long *d_c = (unsigned long long *)(base + off_c);
+    unsigned long long *d_n = (unsigned long long *)(base + off_n);
+    uint32_t *d_no = (uint32_t *)(d_n + 1);
+    uint32_t *d_out = (uint32_t *)(base + off_o);
+    HIP_TRY(hipMemcpyAsync(d_b, borders, 8 * (size_t)n_borders, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d_m, masks.data(), 8 * (size_t)words, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(d_c, 0, 8 * nbins + 16, e->stream));
+    ensure_wl(e);
+    const Grid g = e->grid(e->cur);
+    hipLaunchKernelGGL(k_rdf, dim3(blocks_for(e->max_particles, 256)), dim3(256), 0, e->stream, g,
+                       (const ReplicaDev *)e->reps, r, e->cap, (const double *)d_b, n_borders, (const uint32_t *)d_m,
+                       (const uint32_t *)(d_m + words), rad[0], rad[1], rad[2], d_c, d_n, d_out, outside_cap, d_no,
+                       (uint32_t)e->max_particles);
+    uint32_t no = 0;
+    HIP_TRY(hipMemcpyAsync(&no, d_no, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (no > outside_cap) return set_err(RDY_ERR_CAPACITY, "radial distribution: too many particles outside the box");
+    if (no) {
+        const uint64_t pairs = (uint64_t)no * e->max_particles;
+        if ((pairs + 255) / 256 > 0x7FFFFFFFull)
+            return set_err(RDY_ERR_CAPACITY, "radial distribution: too many particles outside the box");
+        hipLaunchKernelGGL(k_rdf_outside, dim3((uint32_t)((pairs + 255) / 256)), dim3(256), 0, e->stream, g,
+                           (const ReplicaDev *)e->reps, r, (const double *)d_b, n_borders, (const uint32_t *)d_m,
+                           (const uint32_t *)(d_m + words), d_c, (const uint32_t *)d_out, (const uint32_t *)d_no,
+                           (uint32_t)e->max_particles);
+    }
+    HIP_TRY(hipMemcpyAsync(counts, d_c, 8 * nbins, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(n_from, d_n, 8, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_errors(e);
 }
 
 int rdy_profile_reset(rdy_engine *e) {

@@ -254,6 +254,27 @@ class Engine:
         L.check(self._lib.rdy_count_types(self._h, _ptr(c, _u64p), n))
         return c.reshape(n, self.n_types)
 
+    def rdf_counts(self, replica, borders, from_types, to_types):
+        """Radial-distribution pair counts per bin and the number of 'from' particles (the pair loop of
+        RadialDistribution::evaluate, RadialDistribution.cpp:75-103, over the cell list on the device)"""
+        b = np.ascontiguousarray(borders, dtype=np.float64)
+        f = np.ascontiguousarray(from_types, dtype=np.uint32)
+        t = np.ascontiguousarray(to_types, dtype=np.uint32)
+        c = np.zeros(max(len(b) - 1, 0), dtype=np.uint64)
+        nf = np.zeros(1, dtype=np.uint64)
+        L.check(self._lib.rdy_rdf_counts(self._h, replica, _ptr(b, _dp), len(b), _ptr(f, _u32p), len(f),
+                                         _ptr(t, _u32p), len(t), _ptr(c, _u64p), _ptr(nf, _u64p)))
+        return c, int(nf[0])
+
+    def radial_distribution(self, replica, borders, from_types, to_types, particle_to_density):
+        """(bin centers, distribution) as RadialDistribution::evaluate forms them
+        (RadialDistribution.cpp:105-123, 136-144): counts / (4/3 pi (r_hi^3 - r_lo^3) n_from density)"""
+        b = np.asarray(borders, dtype=np.float64)
+        counts, nf = self.rdf_counts(replica, b, from_types, to_types)
+        centers = (b[:-1] + b[1:]) / 2
+        shell = 4. / 3. * np.pi * (np.power(b[1:], 3.) - np.power(b[:-1], 3.))
+        return centers, counts.astype(np.float64) / (shell * nf * particle_to_density)
+
     def profile_reset(self):
         L.check(self._lib.rdy_profile_reset(self._h))
 

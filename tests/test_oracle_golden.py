@@ -457,3 +457,49 @@ def test_threaded_timing_baseline_equals_one_thread(mode):
         assert np.array_equal(a, b)
     assert np.array_equal(c1, c4) and c1.sum() > 0
     np.testing.assert_allclose(e1, e4, rtol=1e-12)
+
+
+def _rdf_numpy(pos, types, ids, box, pbc, borders, frm, to):
+    """RadialDistribution counts by numpy: minimum-image distances of all ordered pairs, np.searchsorted
+    (side='right' is std::upper_bound)"""
+    d = pos[None, :, :] - pos[:, None, :]
+    for a in range(3):
+        if pbc[a]:
+            d[..., a] = np.where(d[..., a] > box[a] / 2, d[..., a] - box[a], d[..., a])
+            d[..., a] = np.where(d[..., a] <= -box[a] / 2, d[..., a] + box[a], d[..., a])
+    dist = np.sqrt((d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2)
+    use = np.isin(types, frm)[:, None] & np.isin(types, to)[None, :] & (ids[:, None] != ids[None, :])
+    idx = np.searchsorted(borders, dist[use], side="right")
+    idx = idx[(idx > 0) & (idx < len(borders))]
+    return np.bincount(idx - 1, minlength=len(borders) - 1).astype(np.uint64), int(np.isin(types, frm).sum())
+
+
+def test_rdf_two_particles_known_answer():
+    # the reference's observable test scene (test_observables_io.py:201-238): A at (-2.5, 0, 0), B at the
+    # origin, borders arange(0, 5, 0.01), counted from A to B: one pair at distance 2.5
+    s = po.System(2, [10.0] * 3, [1, 1, 1])
+    s.add_particles(np.array([[-2.5, 0, 0], [0, 0, 0]]), [0, 1], ids=np.arange(2))
+    borders = np.arange(0, 5, 0.01)
+    counts, nf = s.rdf_counts(borders, [0], [1])
+    assert nf == 1 and counts.sum() == 1
+    k = int(np.nonzero(counts)[0][0])
+    assert borders[k] <= 2.5 < borders[k + 1]
+    counts, nf = s.rdf_counts(borders, [1], [1])  # no B-B pair
+    assert nf == 1 and counts.sum() == 0
+
+
+@pytest.mark.parametrize("pbc", [(1, 1, 1), (1, 0, 1)])
+def test_rdf_counts_equal_numpy_brute_force(pbc):
+    rng = np.random.default_rng(12)
+    box = np.array([8.0, 9.0, 10.0])
+    pos = (rng.random((300, 3)) - 0.5) * box
+    types = rng.integers(0, 3, 300).astype(np.int32)
+    ids = np.arange(300)
+    s = po.System(3, box, list(pbc))
+    s.add_particles(pos, types, ids=ids)
+    borders = np.concatenate([[0.3], np.linspace(0.5, 4.0, 36)])  # first border above 0: closer pairs uncounted
+    for frm, to in [([0, 1, 2], [0, 1, 2]), ([0], [1, 2]), ([2], [2])]:
+        counts, nf = s.rdf_counts(borders, frm, to)
+        want, wnf = _rdf_numpy(pos, types, ids, box, pbc, borders, frm, to)
+        assert nf == wnf
+        assert np.array_equal(counts, want)
