@@ -112,6 +112,29 @@ def ensemble_bench(n_replicas, steps):
             "latency": "per-step wall time of the batched loop over chunks of 10 graph-replayed steps"}
 
 
+def rdf_bench(eng, r, pos, types, sample=10000):
+    """Radial distribution of the 1M-particle state (all species, 100 bins up to 5.0) on the device, and the
+    reference's all-pairs host loop (RadialDistribution.cpp:75-103) as the oracle runs it on a sample of
+    `sample` particles, scaled by (N / sample)^2 (the loop's cost depends on the pair count only)."""
+    from oracle import pyoracle as po
+    borders = np.linspace(0.0, 5.0, 101)
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        counts, nf = eng.rdf_counts(r, borders, [0, 1, 2], [0, 1, 2])
+        times.append(time.perf_counter() - t0)
+    dev = float(np.median(times))
+    o = po.System(3, [200.0] * 3, [1, 1, 1])
+    o.add_particles(pos[:sample], types[:sample].astype(np.int32), ids=np.arange(sample))
+    t0 = time.perf_counter()
+    o.rdf_counts(borders, [0, 1, 2], [0, 1, 2])
+    host = (time.perf_counter() - t0) * (len(types) / sample) ** 2
+    return {"particles": int(nf), "bins": len(borders) - 1, "r_max": 5.0, "device_s": dev,
+            "pairs_counted": int(counts.sum()), "host_all_pairs_s_extrapolated": host,
+            "speedup": host / dev,
+            "host_sample": f"oracle all-pairs loop on {sample} particles, one core, scaled by (N/{sample})^2"}
+
+
 def serving_bench(n_requests, step_scale):
     """Continuous batching of a finite Poisson trace (200 req/s) of the 1M-particle system's chemistry
     at 1000..200000 particles, steps 1000..100000 scaled by step_scale, handlers 50/50, 32 GiB HBM budget,
@@ -140,6 +163,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-rdf", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=200)
     ap.add_argument("--serving-step-scale", type=float, default=0.1)
     ap.add_argument("--ensemble-replicas", type=int, default=4096)
@@ -209,6 +233,9 @@ def main():
         "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
         "device": device_info()["name"],
     }
+    if not args.no_rdf:
+        out["rdf"] = rdf_bench(eng, r, pos, types)
+        print(f"bench: radial distribution {out['rdf']['device_s'] * 1e3:.3g} ms", file=sys.stderr, flush=True)
     eng.close()
     print(f"bench: 1M-particle system {value:.4g} particle-steps/s", file=sys.stderr, flush=True)
     if not args.no_ensemble:

@@ -190,6 +190,12 @@ int rdy_count_types(rdy_engine *e, uint64_t *counts, uint32_t n_replicas);
 int rdy_rdf_counts(rdy_engine *e, uint32_t replica, const double *borders, uint32_t n_borders,
                    const uint32_t *from_types, uint32_t n_from_types, const uint32_t *to_types,
                    uint32_t n_to_types, uint64_t *counts, uint64_t *n_from);
+/* particle histogram along one axis of one replica, replacing CPUHistogramAlongAxis::evaluate
+ * (kernels/cpu/src/observables/CPUObservables.cpp:85-136): a live particle of the listed types adds
+ * one to counts[u - 1], u = upper_bound(borders, position[axis]), for 1 <= u < n_borders - 1 (the CPU
+ * kernel's bounds: its last bin stays empty).  2 <= n_borders <= 4097. */
+int rdy_histogram_along_axis(rdy_engine *e, uint32_t replica, const double *borders, uint32_t n_borders,
+                             const uint32_t *types, uint32_t n_types, uint32_t axis, uint64_t *counts);
 
 /* kernel timing of the dominant (fused force/integrate/rebin) kernel, measured with HIP
  * events on the engine's stream: average microseconds over the launches since the last reset */

@@ -59,6 +59,8 @@ def lib():
         L.or_set_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_id_counter.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_histogram_along_axis.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                              _u64p]
         L.or_rdf_counts.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_int, _u64p,
                                     _u64p]
         L.or_add_particles.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _u64p]
@@ -261,6 +263,15 @@ class System:
         _check(lib().or_rdf_counts(self._h, bp, len(b), f.ctypes.data_as(_ip), len(f), t.ctypes.data_as(_ip), len(t),
                                    counts.ctypes.data_as(_u64p), nf.ctypes.data_as(_u64p)))
         return counts, int(nf[0])
+
+    def histogram_along_axis(self, borders, types, axis):
+        """CPUHistogramAlongAxis counts (the CPU kernel's bins: the last one stays empty)"""
+        b, bp = _d(np.asarray(borders, dtype=np.float64))
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        counts = np.zeros(max(len(b) - 1, 0), dtype=np.uint64)
+        _check(lib().or_histogram_along_axis(self._h, bp, len(b), t.ctypes.data_as(_ip), len(t), int(axis),
+                                             counts.ctypes.data_as(_u64p)))
+        return counts
 
     def set_threads(self, n):
         """OpenMP threads for forces, integration and the keyed candidate search (default 1)"""

@@ -584,6 +584,30 @@ int or_rdf_counts(const or_system *s, const double *borders, int n_borders, cons
     return 0;
 }
 
+int or_histogram_along_axis(const or_system *s, const double *borders, int n_borders, const int *types, int n_types,
+                            int axis, uint64_t *counts) {
+    if (n_borders < 2) return fail("bin borders need at least two entries");
+    if (axis < 0 || axis > 2) return fail("axis must be 0, 1 or 2");
+    const int n_bins = n_borders - 1;
+    for (int b = 0; b < n_bins; ++b) counts[b] = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        const entry_t *p = &s->e[i];
+        if (p->deactivated) continue;
+        int in = 0;
+        for (int k = 0; k < n_types; ++k) in |= p->type == types[k];
+        if (!in) continue;
+        const double x = p->pos[axis];
+        int lo = 0, hi = n_borders;  /* std::upper_bound */
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (x < borders[mid]) hi = mid;
+            else lo = mid + 1;
+        }
+        if (lo < n_borders && lo >= 1 && lo < n_bins) counts[lo - 1]++;  /* CPUObservables.cpp:107-114 */
+    }
+    return 0;
+}
+
 int or_set_id_counter(or_system *s, uint64_t next_id) {
     s->next_id = next_id;
     return 0;

@@ -73,6 +73,12 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
 int or_set_seed(or_system *s, uint64_t seed);
 int or_set_id_counter(or_system *s, uint64_t next_id);
 int or_set_threads(or_system *s, int n);
+/* CPUHistogramAlongAxis::evaluate (kernels/cpu/src/observables/CPUObservables.cpp:85-136): each live
+ * particle of the listed types adds one to bin upper_bound(borders, pos[axis]) - 1 when that index is
+ * at least 0 and below n_borders - 2 (the CPU kernel compares against the bin count, so the last bin
+ * stays empty). */
+int or_histogram_along_axis(const or_system *s, const double *borders, int n_borders, const int *types, int n_types,
+                            int axis, uint64_t *counts);
 /* RadialDistribution::evaluate counts (RadialDistribution.cpp:75-103): every ordered pair (from, to) of
  * live particles with distinct ids, type of from in from_types and of to in to_types, adds one to the bin
  * upper_bound(borders, sqrt(distSquared)) - 1 when the distance lies below the last border; *n_from is

@@ -1579,6 +1579,36 @@ int rdy_rdf_counts(rdy_engine *e, uint32_t r, const double *borders, uint32_t n_
     return check_errors(e);
 }
 
+int rdy_histogram_along_axis(rdy_engine *e, uint32_t r, const double *borders, uint32_t n_borders,
+                             const uint32_t *types, uint32_t n_types, uint32_t axis, uint64_t *counts) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (!borders || !counts || n_borders < 2 || n_borders > RDF_MAX_BORDERS || axis > 2)
+        return set_err(RDY_ERR_INVALID, "histogram: 2 to 4097 bin borders, an axis 0..2 and an output array required");
+    for (uint32_t b = 0; b + 1 < n_borders; ++b)
+        if (!(borders[b] <= borders[b + 1])) return set_err(RDY_ERR_INVALID, "bin borders must ascend");
+    const uint32_t words = (e->n_types + 31) / 32;
+    std::vector<uint32_t> mask(words, 0u);
+    for (uint32_t k = 0; k < n_types; ++k) {
+        if (types[k] >= e->n_types) return set_err(RDY_ERR_INVALID, "histogram: no such species");
+        mask[types[k] >> 5] |= 1u << (types[k] & 31u);
+    }
+    const size_t nbins = n_borders - 1;
+    const size_t off_m = 8 * (size_t)n_borders, off_c = (off_m + 4 * words + 15) & ~(size_t)15;
+    if ((rc = ensure_scratch(e, off_c + 8 * nbins))) return rc;
+    char *base = (char *)e->scratch;
+    HIP_TRY(hipMemcpyAsync(base, borders, 8 * (size_t)n_borders, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(base + off_m, mask.data(), 4 * (size_t)words, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(base + off_c, 0, 8 * nbins, e->stream));
+    ensure_wl(e);
+    hipLaunchKernelGGL(k_hist_axis, dim3(blocks_for(e->max_particles, 256)), dim3(256), 0, e->stream,
+                       e->grid(e->cur), r, (const double *)base, n_borders, (const uint32_t *)(base + off_m),
+                       (int)axis, (unsigned long long *)(base + off_c), (uint32_t)e->max_particles);
+    HIP_TRY(hipMemcpyAsync(counts, base + off_c, 8 * nbins, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_errors(e);
+}
+
 int rdy_profile_reset(rdy_engine *e) {
     if (!e) return set_err(RDY_ERR_INVALID, "null engine");
     if (e->prof_ev.empty()) {

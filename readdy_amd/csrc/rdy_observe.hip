@@ -126,4 +126,32 @@ __global__ __launch_bounds__(256) void k_rdf_outside(Grid cur, const ReplicaDev 
     if (!(b.tw & TW_OUTSIDE) && in_mask(from_mask, tw_type(b.tw)) && in_mask(to_mask, tw_type(a.tw))) count(b, a);
 }
 
+// HistogramAlongAxis as the CPU kernel evaluates it (kernels/cpu/src/observables/CPUObservables.cpp:
+// 85-136): each live record of replica r whose type is in mask adds one to bin u - 1, u =
+// upper_bound(borders, position[axis]), when 1 <= u < nb - 1 (the CPU kernel compares u with the bin
+// count, so the last bin stays empty).  One thread per work-list record, per-block LDS histogram.
+__global__ __launch_bounds__(256) void k_hist_axis(Grid cur, uint32_t r, const double *borders, uint32_t nb,
+                                                   const uint32_t *mask, int axis, unsigned long long *counts,
+                                                   uint32_t max_wl) {
+    __shared__ double sb[RDF_MAX_BORDERS];
+    __shared__ uint32_t hist[RDF_MAX_BORDERS - 1];
+    for (uint32_t k = threadIdx.x; k < nb; k += blockDim.x) sb[k] = borders[k];
+    for (uint32_t k = threadIdx.x; k + 1 < nb; k += blockDim.x) hist[k] = 0u;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *cur.wl_count;
+    n = n < max_wl ? n : max_wl;
+    if (i < n) {
+        const State me = cur.bins[cur.wl[i]];
+        if (!(me.tw & TW_DEAD) && tw_replica(me.tw) == r && in_mask(mask, tw_type(me.tw))) {
+            const double x = axis == 0 ? me.x : (axis == 1 ? me.y : me.z);
+            const uint32_t u = upper_bound_border(sb, nb, x);
+            if (u >= 1u && u + 1u < nb) atomicAdd(&hist[u - 1], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k + 1 < nb; k += blockDim.x)
+        if (hist[k]) atomicAdd(&counts[k], (unsigned long long)hist[k]);
+}
+
 }  // namespace rdy

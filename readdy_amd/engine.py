@@ -266,6 +266,15 @@ class Engine:
                                          _ptr(t, _u32p), len(t), _ptr(c, _u64p), _ptr(nf, _u64p)))
         return c, int(nf[0])
 
+    def histogram_along_axis(self, replica, borders, types, axis):
+        """HistogramAlongAxis counts as the reference's CPU kernel forms them (CPUObservables.cpp:85-136)"""
+        b = np.ascontiguousarray(borders, dtype=np.float64)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        c = np.zeros(max(len(b) - 1, 0), dtype=np.uint64)
+        L.check(self._lib.rdy_histogram_along_axis(self._h, replica, _ptr(b, _dp), len(b), _ptr(t, _u32p), len(t),
+                                                   int(axis), _ptr(c, _u64p)))
+        return c
+
     def radial_distribution(self, replica, borders, from_types, to_types, particle_to_density):
         """(bin centers, distribution) as RadialDistribution::evaluate forms them
         (RadialDistribution.cpp:105-123, 136-144): counts / (4/3 pi (r_hi^3 - r_lo^3) n_from density)"""

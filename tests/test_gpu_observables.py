@@ -115,3 +115,21 @@ def test_rdf_at_one_million_particles_counts_the_cell_list_pairs():
     n_pairs = len(e.neighbor_pairs(r, 2.0))
     # sqrt(d2) < 2 <=> d2 < 4 but for d2 within an ulp below 4 (where sqrt rounds to 2): none at these sizes
     assert int(counts[0]) == n_pairs
+
+
+@pytest.mark.parametrize("pbc", [(1, 1, 1), (1, 0, 1)])
+def test_histogram_along_axis_bit_exact(pbc):
+    box = [14.0, 11.0, 9.0]
+    outside = 1 if pbc == (1, 0, 1) else None
+    pos, types = _scene(3000, box, seed=21, outside_axis=outside)
+    e = Engine(3, [1.0] * 3, potentials=_pots(3))
+    r = e.add_replica(box, pbc, seed=1, max_particles=12000)
+    e.set_particles(r, pos, types)
+    o = po.System(3, box, list(pbc))
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    borders = np.linspace(-7.5, 7.5, 151)
+    for axis in range(3):
+        for sel in ([0, 1, 2], [2], [0, 1]):
+            got = e.histogram_along_axis(r, borders, sel, axis)
+            want = o.histogram_along_axis(borders, sel, axis)
+            assert np.array_equal(got, want) and want.sum() > 0

@@ -503,3 +503,24 @@ def test_rdf_counts_equal_numpy_brute_force(pbc):
         want, wnf = _rdf_numpy(pos, types, ids, box, pbc, borders, frm, to)
         assert nf == wnf
         assert np.array_equal(counts, want)
+
+
+def test_histogram_along_axis_equals_numpy():
+    # CPUObservables.cpp:85-136: upper_bound over the borders; bins 1 .. n_bins - 1 of the upper bound
+    # (the CPU kernel tests the index against the bin count, so the last bin never fills)
+    rng = np.random.default_rng(3)
+    box = [10.0, 12.0, 8.0]
+    pos = (rng.random((2000, 3)) - 0.5) * np.array(box)
+    types = rng.integers(0, 3, 2000).astype(np.int32)
+    s = po.System(3, box, [1, 1, 1])
+    s.add_particles(pos, types, ids=np.arange(2000))
+    borders = np.linspace(-5.0, 5.0, 41)
+    for axis in range(3):
+        for sel in ([0, 1, 2], [1], [0, 2]):
+            got = s.histogram_along_axis(borders, sel, axis)
+            x = pos[np.isin(types, sel), axis]
+            u = np.searchsorted(borders, x, side="right")
+            u = u[(u >= 1) & (u < len(borders) - 1)]
+            want = np.bincount(u - 1, minlength=len(borders) - 1).astype(np.uint64)
+            assert np.array_equal(got, want)
+            assert got[-1] == 0
