@@ -90,12 +90,13 @@ def cpu_baseline(n_steps=5):
 
 def ensemble_bench(n_replicas, steps):
     """Batched replica ensemble (4096 replicas of the A+B<->C system, 500..50000 particles each at
-    density 0.25) against the plain one-engine-per-replica path; cell size from the autotuner."""
+    density 0.25) against the plain one-engine-per-replica path; cell size and brick shape from the
+    autotuner."""
     from readdy_amd import ensemble as E
     n, box = E.sample(n_replicas)
     sub = max(1, min(512, n_replicas))
     best, tried = E.autotune(n[:sub], box[:sub], steps=20)
-    b = E.Batched(n, box, nl_cutoff=best["nl_cutoff"])
+    b = E.Batched(n, box, nl_cutoff=best["nl_cutoff"], brick=tuple(best["brick"]))
     rate, lat = b.timed(steps, chunk=10, warmup=10)
     particles = int(b.particles())
     b.close()

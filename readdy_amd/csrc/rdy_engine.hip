@@ -500,7 +500,13 @@ void make_bricks(rdy_engine *e, uint32_t r, std::vector<BrickDev> &out) {
             seg[a].push_back({0, D});
             halo[a] = 0;
         } else {
-            for (int32_t o = 0; o < D; o += B) seg[a].push_back({o, std::min(B, D - o)});
+            // ceil(D / B) segments of nearly equal length (at most B), so that no brick is a thin sliver
+            const int32_t n = (D + B - 1) / B;
+            for (int32_t k = 0, o = 0; k < n; ++k) {
+                const int32_t len = D / n + (k < D % n ? 1 : 0);
+                seg[a].push_back({o, len});
+                o += len;
+            }
             halo[a] = (uint32_t)h;
         }
     }

@@ -8,8 +8,8 @@ key.  `batched` packs every replica into one engine, so each step is one set of 
 of them (captured in a hipGraph); `plain` runs the same kernels once per replica per step (one
 engine per replica), the path batching is measured against.
 
-The autotuner picks the neighbour-list cell size and the per-cell bin capacity for an ensemble by
-timing short runs of each candidate.
+The autotuner picks the neighbour-list cell size and the brick shape for an ensemble by timing short
+runs of each candidate (the per-cell bin capacity follows from the expected occupancy).
 """
 import math
 import time
@@ -150,16 +150,28 @@ class Plain:
             e.close()
 
 
-def autotune(n, box, candidates=((1.0, 0), (1.5, 0), (2.0, 0)), steps=20):
-    """Times short batched runs of each (cell size, bin capacity) candidate; returns the fastest
-    with all timings."""
+def half_round_brick(cell):
+    """A cube of cells whose expected particles fill about half of a brick's 256 lanes (two waves): the
+    engine's own choice fills one round of lanes, and with many small replicas the smaller brick can
+    win (fewer waves per block, so a block's slowest wave holds its LDS for less time)."""
+    lam = DENSITY * cell ** 3
+    e = int(round((128.0 / lam) ** (1.0 / 3.0)))
+    e = max(2, min(16, e))
+    return (e, e, e)
+
+
+def autotune(n, box, cells=(1.0, 1.5, 2.0), steps=20):
+    """Times short batched runs of each (cell size, brick shape) candidate, the brick being the engine's
+    one-round choice or half_round_brick; returns the fastest with all timings."""
     results = []
-    for cutoff, cap in candidates:
-        b = Batched(n, box, nl_cutoff=cutoff, bin_capacity=cap)
-        try:
-            rate, _ = b.timed(steps, chunk=steps, warmup=5)
-            results.append({"nl_cutoff": cutoff, "bin_capacity": b.cap, "particle_steps_per_s": rate})
-        finally:
-            b.close()
+    for cutoff in cells:
+        for brick in ((0, 0, 0), half_round_brick(cutoff)):
+            b = Batched(n, box, nl_cutoff=cutoff, brick=brick)
+            try:
+                rate, _ = b.timed(steps, chunk=steps, warmup=5)
+                results.append({"nl_cutoff": cutoff, "brick": list(brick), "bin_capacity": b.cap,
+                                "particle_steps_per_s": rate})
+            finally:
+                b.close()
     best = max(results, key=lambda x: x["particle_steps_per_s"])
     return best, results
