@@ -83,9 +83,17 @@ def cpu_baseline(n_steps=5):
         ps += o.n_active
         o.loop_step(0.005, po.UA_KEYED, t)
     dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
+    except OSError:
+        pass
+    phases = {k: v / n_steps * 1e3 for k, v in o.phase_times().items()}
     return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n_steps} full steps of the 1M-particle workload (integrate, neighbour list, UA reactions, "
-                      f"neighbour list, forces) by the FP64 C oracle on {threads} OpenMP threads; {dt:.1f} s"}
+                      f"neighbour list, forces) by the FP64 C oracle on {threads} OpenMP threads; {dt:.1f} s",
+            "phase_ms_per_step": phases, "cpu_model": model, "nproc": os.cpu_count()}
 
 
 def ensemble_bench(n_replicas, steps):

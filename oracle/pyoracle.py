@@ -59,6 +59,8 @@ def lib():
         L.or_set_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_id_counter.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_phase_times.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.or_phase_times.restype = None
         L.or_histogram_along_axis.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
                                               _u64p]
         L.or_rdf_counts.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_int, _u64p,
@@ -272,6 +274,12 @@ class System:
         _check(lib().or_histogram_along_axis(self._h, bp, len(b), t.ctypes.data_as(_ip), len(t), int(axis),
                                              counts.ctypes.data_as(_u64p)))
         return counts
+
+    def phase_times(self, reset=False):
+        """seconds spent in loop_step's phases: integrator, neighborList, reactions, forces"""
+        out = np.zeros(4)
+        lib().or_phase_times(self._h, out.ctypes.data_as(_dp), int(bool(reset)))
+        return dict(zip(("integrator", "neighborList", "reactions", "forces"), out.tolist()))
 
     def set_threads(self, n):
         """OpenMP threads for forces, integration and the keyed candidate search (default 1)"""

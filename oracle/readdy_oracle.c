@@ -39,6 +39,7 @@
 
 #include <math.h>
 #include <stdio.h>
+#include <time.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -465,6 +466,7 @@ struct or_system {
     size_t *list;      /* [n + 1] */
     size_t list_len;
     uint64_t counts[OR_MAX_RX];
+    double phase_s[4]; /* or_loop_step wall time: integrator, neighborList (both fills), reactions, forces */
     int threads; /* > 1: forces, integration and the candidate search on that many OpenMP threads */
 };
 
@@ -1449,11 +1451,37 @@ const uint64_t *or_reaction_counts(const or_system *s, int *n) {
 }
 
 /* SimulationLoop::run body: include/readdy/api/SimulationLoop.h:256-306 */
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 int or_loop_step(or_system *s, double dt, int mode, uint32_t step, const double *noise_by_id, size_t noise_len,
                  double *energy) {
+    /* phases timed like the reference's readdy::util::Timer scopes (integrator, neighborList, reactions,
+     * forces; SimulationLoop.h:280-300) */
+    double t0 = now_s();
     if (or_euler_bd(s, dt, noise_by_id, noise_len, step)) return -1;
+    double t1 = now_s();
     if (or_nl_fill(s)) return -1;
+    double t2 = now_s();
     if (s->n_rx > 0 && or_reactions(s, mode, dt, step, NULL)) return -1;
+    double t3 = now_s();
     if (or_nl_fill(s)) return -1;
-    return or_calculate_forces(s, energy, NULL);
+    double t4 = now_s();
+    int rc = or_calculate_forces(s, energy, NULL);
+    double t5 = now_s();
+    s->phase_s[0] += t1 - t0;
+    s->phase_s[1] += (t2 - t1) + (t4 - t3);
+    s->phase_s[2] += t3 - t2;
+    s->phase_s[3] += t5 - t4;
+    return rc;
+}
+
+void or_phase_times(or_system *s, double *out4, int reset) {
+    for (int k = 0; k < 4; ++k) {
+        out4[k] = s->phase_s[k];
+        if (reset) s->phase_s[k] = 0.0;
+    }
 }

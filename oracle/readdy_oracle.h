@@ -73,6 +73,8 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
 int or_set_seed(or_system *s, uint64_t seed);
 int or_set_id_counter(or_system *s, uint64_t next_id);
 int or_set_threads(or_system *s, int n);
+/* accumulated wall seconds of or_loop_step's phases: integrator, neighbour list, reactions, forces */
+void or_phase_times(or_system *s, double *out4, int reset);
 /* CPUHistogramAlongAxis::evaluate (kernels/cpu/src/observables/CPUObservables.cpp:85-136): each live
  * particle of the listed types adds one to bin upper_bound(borders, pos[axis]) - 1 when that index is
  * at least 0 and below n_borders - 2 (the CPU kernel compares against the bin count, so the last bin
