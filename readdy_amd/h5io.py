@@ -1,0 +1,211 @@
+"""ctypes binding of libreaddy_h5.so: observable and trajectory datasets in ReaDDy's HDF5 layout.
+
+Writers (one file per call sequence: create, datasets, attributes, close):
+  n_particles          /readdy/observables/<name>/{data [T][n_types], time [T]}
+                       (NParticles, readdy/main/model/observables/Observables.cpp)
+  reaction_counts      /readdy/observables/<name>/{counts/<reaction id> [T], time [T]}
+                       (ReactionCounts, readdy/main/model/observables/Observables.cpp)
+  radial_distribution  /readdy/observables/<name>/{bin_centers [B], distribution [T][B], time [T]}
+                       (RadialDistribution.cpp:158-170)
+  flat_trajectory      /readdy/trajectory/<name>/{records [R], limits [T][2], time [T]}
+                       (FlatTrajectory, Trajectory.cpp:112-158; records are {id u64, typeId u16,
+                       flavor u8, pos f64[3]} as in Types.cpp:79-88, limits a frame's [begin, end))
+  config               /readdy/config/{particle_types, registered_reactions, general}
+                       (IOUtils.cpp:58-176: what readdy.Trajectory reads for type names, reactions,
+                       kBT, box and periodicity)
+Readers (tests and tools): read_u64, read_f64, read_records, read_string.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u16p = ctypes.POINTER(ctypes.c_uint16)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def lib():
+    """The loaded library (raises OSError when it is not built)."""
+    global _lib
+    if _lib is None:
+        h = ctypes.CDLL(os.path.join(_HERE, "libreaddy_h5.so"))
+        h.rdy_h5_create.restype = ctypes.c_int64
+        h.rdy_h5_create.argtypes = [ctypes.c_char_p]
+        for fn in ("rdy_h5_n_particles", "rdy_h5_reaction_counts"):
+            getattr(h, fn).argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
+                                       _u64p]
+        h.rdy_h5_radial_distribution.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                 _u64p, _f64p, _f64p]
+        h.rdy_h5_flat_trajectory.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p, _u64p,
+                                             _u16p, _u8p, _f64p]
+        cpp = ctypes.POINTER(ctypes.c_char_p)
+        h.rdy_h5_particle_types.argtypes = [ctypes.c_int64, ctypes.c_uint32, cpp, _f64p, cpp]
+        h.rdy_h5_registered_reactions.argtypes = [ctypes.c_int64, ctypes.c_uint32, cpp, _u64p, _u64p, _u64p, _f64p,
+                                                  _f64p, _f64p, _u16p, _u16p]
+        h.rdy_h5_general.argtypes = [ctypes.c_int64, ctypes.c_char_p]
+        h.rdy_h5_read_string.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p]
+        h.rdy_h5_attribute.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p]
+        h.rdy_h5_close.argtypes = [ctypes.c_int64]
+        h.rdy_h5_read_u64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p]
+        h.rdy_h5_read_f64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _f64p, _u64p]
+        h.rdy_h5_read_records.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u16p, _u8p,
+                                          _f64p, _u64p]
+        _lib = h
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class File:
+    """One HDF5 file being written; every write raises on failure."""
+
+    def __init__(self, path):
+        self.path = path
+        self.f = lib().rdy_h5_create(path.encode())
+        if self.f < 0:
+            raise OSError(f"cannot create {path}")
+
+    def _check(self, rc, what):
+        if rc:
+            raise OSError(f"HDF5 write of {what} failed in {self.path}")
+
+    def n_particles(self, name, time, counts):
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        c = np.ascontiguousarray(counts, dtype=np.uint64).reshape(len(t), -1)
+        self._check(lib().rdy_h5_n_particles(self.f, name.encode(), len(t), c.shape[1], _p(t, _u64p),
+                                             _p(c, _u64p)), name)
+
+    def reaction_counts(self, name, time, counts):
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        c = np.ascontiguousarray(counts, dtype=np.uint64).reshape(len(t), -1)
+        self._check(lib().rdy_h5_reaction_counts(self.f, name.encode(), len(t), c.shape[1], _p(t, _u64p),
+                                                 _p(c, _u64p)), name)
+
+    def radial_distribution(self, name, time, bin_centers, distribution):
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        c = np.ascontiguousarray(bin_centers, dtype=np.float64)
+        d = np.ascontiguousarray(distribution, dtype=np.float64).reshape(len(t), len(c))
+        self._check(lib().rdy_h5_radial_distribution(self.f, name.encode(), len(t), len(c), _p(t, _u64p),
+                                                     _p(c, _f64p), _p(d, _f64p)), name)
+
+    def flat_trajectory(self, name, time, frames):
+        """frames: one (ids, type ids, flavors, positions [n][3]) tuple per entry of time."""
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        if len(frames) != len(t):
+            raise ValueError("one frame per time point")
+        sizes = np.array([len(f[0]) for f in frames], dtype=np.uint64)
+        cat = lambda k, dt: np.ascontiguousarray(np.concatenate([np.asarray(f[k], dtype=dt).ravel() for f in frames])
+                                                 if frames else np.zeros(0, dt))
+        ids, types, flav, pos = cat(0, np.uint64), cat(1, np.uint16), cat(2, np.uint8), cat(3, np.float64)
+        if not (len(ids) == len(types) == len(flav) == len(pos) // 3 == int(sizes.sum())):
+            raise ValueError("frame arrays of unequal length")
+        self._check(lib().rdy_h5_flat_trajectory(self.f, name.encode(), len(t), _p(t, _u64p), _p(sizes, _u64p),
+                                                 _p(ids, _u64p), _p(types, _u16p), _p(flav, _u8p),
+                                                 _p(pos, _f64p)), name)
+
+    def config(self, types, box, pbc=(True, True, True), kbt=1.0, reactions=()):
+        """types: (name, diffusion constant[, flavor]) per type id; reactions: dicts with name, id, educts,
+        products (type ids), rate, educt_distance, product_distance."""
+        cstr = lambda xs: (ctypes.c_char_p * max(1, len(xs)))(*[x.encode() for x in xs])
+        names = [t[0] for t in types]
+        flav = [t[2] if len(t) > 2 else "NORMAL" for t in types]
+        diff = np.ascontiguousarray([t[1] for t in types], dtype=np.float64)
+        self._check(lib().rdy_h5_particle_types(self.f, len(types), cstr(names), _p(diff, _f64p), cstr(flav)),
+                    "particle_types")
+        if reactions:
+            col = lambda k, dt: np.ascontiguousarray([r[k] for r in reactions], dtype=dt)
+            pad = lambda xs: (list(xs) + [0, 0])[:2]
+            ed = np.ascontiguousarray([pad(r["educts"]) for r in reactions], dtype=np.uint16)
+            pr = np.ascontiguousarray([pad(r["products"]) for r in reactions], dtype=np.uint16)
+            ne = np.ascontiguousarray([len(r["educts"]) for r in reactions], dtype=np.uint64)
+            npr = np.ascontiguousarray([len(r["products"]) for r in reactions], dtype=np.uint64)
+            ids, rate = col("id", np.uint64), col("rate", np.float64)
+            edist = np.ascontiguousarray([r.get("educt_distance", 0.0) for r in reactions], dtype=np.float64)
+            pdist = np.ascontiguousarray([r.get("product_distance", 0.0) for r in reactions], dtype=np.float64)
+            self._check(lib().rdy_h5_registered_reactions(
+                self.f, len(reactions), cstr([r["name"] for r in reactions]), _p(ids, _u64p), _p(ne, _u64p),
+                _p(npr, _u64p), _p(rate, _f64p), _p(edist, _f64p), _p(pdist, _f64p), _p(ed, _u16p),
+                _p(pr, _u16p)), "registered_reactions")
+        box = [float(b) for b in box]
+        general = {"kbt": float(kbt), "box_volume": float(np.prod(box)), "box_size": box,
+                   "pbc": [bool(p) for p in pbc]}
+        self._check(lib().rdy_h5_general(self.f, json.dumps(general).encode()), "general")
+
+    def attribute(self, key, value):
+        self._check(lib().rdy_h5_attribute(self.f, key.encode(), str(value).encode()), key)
+
+    def close(self):
+        if self.f >= 0:
+            rc = lib().rdy_h5_close(self.f)
+            self.f = -1
+            self._check(rc, "close")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _size(path, dataset):
+    """Element count of a dataset (a read into a zero-capacity buffer reports it)."""
+    n = ctypes.c_uint64(1 << 63)
+    one = np.zeros(1, np.uint64)
+    lib().rdy_h5_read_u64(path.encode(), dataset.encode(), 0, _p(one, _u64p), ctypes.byref(n))
+    if n.value == 1 << 63:
+        raise OSError(f"no dataset {dataset} in {path}")
+    return n.value
+
+
+def read_u64(path, dataset):
+    out = np.zeros(max(1, _size(path, dataset)), dtype=np.uint64)
+    n = ctypes.c_uint64()
+    if lib().rdy_h5_read_u64(path.encode(), dataset.encode(), len(out), _p(out, _u64p), ctypes.byref(n)):
+        raise OSError(f"cannot read {dataset} of {path}")
+    return out[:n.value]
+
+
+def read_f64(path, dataset):
+    out = np.zeros(max(1, _size(path, dataset)), dtype=np.float64)
+    n = ctypes.c_uint64()
+    if lib().rdy_h5_read_f64(path.encode(), dataset.encode(), len(out), _p(out, _f64p), ctypes.byref(n)):
+        raise OSError(f"cannot read {dataset} of {path}")
+    return out[:n.value]
+
+
+def read_records(path, dataset):
+    """(ids, type ids, flavors, positions [R][3]) of a trajectory's records dataset."""
+    cap = max(1, _size(path, dataset))
+    ids = np.zeros(cap, np.uint64)
+    types = np.zeros(cap, np.uint16)
+    flav = np.zeros(cap, np.uint8)
+    pos = np.zeros((cap, 3), np.float64)
+    n = ctypes.c_uint64()
+    if lib().rdy_h5_read_records(path.encode(), dataset.encode(), cap, _p(ids, _u64p), _p(types, _u16p),
+                                 _p(flav, _u8p), _p(pos, _f64p), ctypes.byref(n)):
+        raise OSError(f"cannot read {dataset} of {path}")
+    k = n.value
+    return ids[:k], types[:k], flav[:k], pos[:k]
+
+
+def read_string(path, dataset, cap=1 << 16):
+    buf = ctypes.create_string_buffer(cap)
+    if lib().rdy_h5_read_string(path.encode(), dataset.encode(), cap, buf):
+        raise OSError(f"cannot read {dataset} of {path}")
+    return buf.value.decode()
+
+
+def read_trajectory(path, name):
+    """The frames of /readdy/trajectory/<name>: (time, [(ids, types, flavors, positions) per frame])."""
+    g = f"/readdy/trajectory/{name}"
+    t = read_u64(path, g + "/time")
+    lim = read_u64(path, g + "/limits").reshape(len(t), 2)
+    ids, types, flav, pos = read_records(path, g + "/records")
+    return t, [(ids[a:b], types[a:b], flav[a:b], pos[a:b]) for a, b in lim]

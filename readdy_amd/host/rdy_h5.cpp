@@ -5,6 +5,17 @@
 //   /readdy/observables/<name>/time      [T]          uint64 step index
 //   /readdy/observables/<name>/data      [T][n_types] uint64 particles per type        (NParticles.cpp:83)
 //   /readdy/observables/<name>/counts/<reaction id>   [T] uint64 events per interval  (ReactionCounts)
+//   /readdy/observables/<name>/bin_centers [B], distribution [T][B] double, time [T]
+//                                                   (RadialDistribution.cpp:158-170)
+//   /readdy/trajectory/<name>/records [R] {id u64, typeId u16, flavor u8, pos f64[3]},
+//                             limits [T][2] uint64 (a frame's [begin, end) in records), time [T]
+//                                                   (FlatTrajectory, Trajectory.cpp:112-158; Types.cpp:79-88)
+//   /readdy/config/particle_types        [n] {name str, flavor str, type_id u64, diffusion_constant f64}
+//   /readdy/config/registered_reactions  [n] {name str, id u64, n_educts u64, n_products u64, rate,
+//                                         educt_distance, product_distance f64, educt_types u16[2],
+//                                         product_types u16[2]}
+//   /readdy/config/general               JSON string {kbt, box_volume, box_size, pbc}
+//                                                   (IOUtils.cpp:58-176, as readdy.Trajectory reads them)
 // plus string attributes on /readdy/config describing the request.
 //
 // C ABI (all return 0 on success, -1 on an HDF5 error):
@@ -13,10 +24,27 @@
 //                          const uint64_t *time, const uint64_t *counts);
 //   int rdy_h5_reaction_counts(int64_t f, const char *name, uint64_t T, uint32_t n_reactions,
 //                              const uint64_t *time, const uint64_t *counts);   counts [T][n_reactions]
+//   int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t n_bins,
+//                                  const uint64_t *time, const double *centers, const double *dist);
+//   int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time,
+//                              const uint64_t *frame_sizes, const uint64_t *ids, const uint16_t *types,
+//                              const uint8_t *flavors, const double *pos);          pos [R][3]
+//   int rdy_h5_particle_types(int64_t f, uint32_t n, const char *const *names, const double *diffusion,
+//                             const char *const *flavors);             type ids 0 .. n-1
+//   int rdy_h5_registered_reactions(int64_t f, uint32_t n, const char *const *names, const uint64_t *ids,
+//                                   const uint64_t *n_educts, const uint64_t *n_products, const double *rate,
+//                                   const double *educt_distance, const double *product_distance,
+//                                   const uint16_t *educt_types, const uint16_t *product_types);   [n][2]
+//   int rdy_h5_general(int64_t f, const char *json);
 //   int rdy_h5_attribute(int64_t f, const char *key, const char *value);
 //   int rdy_h5_close(int64_t f);
 //   int rdy_h5_read_u64(const char *path, const char *dataset, uint64_t cap, uint64_t *out, uint64_t *n);
 //       reads a uint64 dataset by absolute path (checks and tools); *n = its element count
+//   int rdy_h5_read_f64(const char *path, const char *dataset, uint64_t cap, double *out, uint64_t *n);
+//   int rdy_h5_read_records(const char *path, const char *dataset, uint64_t cap, uint64_t *ids,
+//                           uint16_t *types, uint8_t *flavors, double *pos, uint64_t *n);
+//   int rdy_h5_read_string(const char *path, const char *dataset, uint64_t cap, char *out);
+//       a scalar string dataset into out (NUL-terminated; -1 when it needs more than cap bytes)
 #include <hdf5.h>
 
 #include <cstdint>
@@ -40,6 +68,87 @@ hid_t observable_group(hid_t f, const char *name) {
     hid_t g2 = ensure_group(g1, name);
     H5Gclose(g1);
     return g2;
+}
+
+// one FlatTrajectory record in memory (the reference's TrajectoryEntry fields; TrajectoryEntry.h)
+struct Record {
+    uint64_t id;
+    uint16_t typeId;
+    uint8_t flavor;
+    double pos[3];
+};
+
+// memory and file compound types of Record (the file type packed, little endian)
+void record_types(hid_t *mem, hid_t *file) {
+    const hsize_t three[1] = {3};
+    hid_t am = H5Tarray_create2(H5T_NATIVE_DOUBLE, 1, three);
+    *mem = H5Tcreate(H5T_COMPOUND, sizeof(Record));
+    H5Tinsert(*mem, "id", HOFFSET(Record, id), H5T_NATIVE_UINT64);
+    H5Tinsert(*mem, "typeId", HOFFSET(Record, typeId), H5T_NATIVE_UINT16);
+    H5Tinsert(*mem, "flavor", HOFFSET(Record, flavor), H5T_NATIVE_UINT8);
+    H5Tinsert(*mem, "pos", HOFFSET(Record, pos), am);
+    H5Tclose(am);
+    hid_t af = H5Tarray_create2(H5T_IEEE_F64LE, 1, three);
+    *file = H5Tcreate(H5T_COMPOUND, 8 + 2 + 1 + 24);
+    H5Tinsert(*file, "id", 0, H5T_STD_U64LE);
+    H5Tinsert(*file, "typeId", 8, H5T_STD_U16LE);
+    H5Tinsert(*file, "flavor", 10, H5T_STD_U8LE);
+    H5Tinsert(*file, "pos", 11, af);
+    H5Tclose(af);
+}
+
+// /readdy/config, created as needed
+hid_t config_group(hid_t f) {
+    hid_t g0 = ensure_group(f, "readdy");
+    if (g0 < 0) return -1;
+    hid_t g = ensure_group(g0, "config");
+    H5Gclose(g0);
+    return g;
+}
+
+// a variable-length C string type
+hid_t vlen_string() {
+    hid_t t = H5Tcopy(H5T_C_S1);
+    H5Tset_size(t, H5T_VARIABLE);
+    return t;
+}
+
+// writes n records of a compound type (used as memory and file type) as a 1-d dataset
+int write_compound(hid_t g, const char *name, hid_t type, hsize_t n, const void *data) {
+    hid_t space = H5Screate_simple(1, &n, nullptr);
+    if (space < 0) return -1;
+    hid_t ds = H5Dcreate2(g, name, type, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    int rc = ds < 0 ? -1 : (H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0 ? -1 : 0);
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(space);
+    return rc;
+}
+
+struct TypeInfo {  // ParticleTypeInfo (IOUtils.h:71-76)
+    const char *name;
+    const char *flavor;
+    uint64_t type_id;
+    double diffusion_constant;
+};
+
+struct ReactionInfo {  // ReactionInfo (IOUtils.h:59-70)
+    const char *name;
+    uint64_t id, n_educts, n_products;
+    double rate, educt_distance, product_distance;
+    uint16_t educt_types[2], product_types[2];
+};
+
+int write_f64(hid_t g, const char *name, int rank, const hsize_t *dims, const double *data) {
+    hid_t space = H5Screate_simple(rank, dims, nullptr);
+    if (space < 0) return -1;
+    hid_t ds = H5Dcreate2(g, name, H5T_IEEE_F64LE, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    int rc = -1;
+    if (ds >= 0) {
+        rc = H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0 ? -1 : 0;
+        H5Dclose(ds);
+    }
+    H5Sclose(space);
+    return rc;
 }
 
 int write_u64(hid_t g, const char *name, int rank, const hsize_t *dims, const uint64_t *data) {
@@ -93,11 +202,128 @@ int rdy_h5_reaction_counts(int64_t f, const char *name, uint64_t T, uint32_t n_r
     return rc;
 }
 
-int rdy_h5_attribute(int64_t f, const char *key, const char *value) {
+int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t n_bins, const uint64_t *time,
+                               const double *centers, const double *dist) {
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    hsize_t dt[1] = {T}, dc[1] = {n_bins}, dd[2] = {T, n_bins};
+    int rc = write_f64(g, "bin_centers", 1, dc, centers);
+    if (rc == 0) rc = write_f64(g, "distribution", 2, dd, dist);
+    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time, const uint64_t *frame_sizes,
+                           const uint64_t *ids, const uint16_t *types, const uint8_t *flavors, const double *pos) {
     hid_t g0 = ensure_group((hid_t)f, "readdy");
     if (g0 < 0) return -1;
-    hid_t g = ensure_group(g0, "config");
+    hid_t g1 = ensure_group(g0, "trajectory");
     H5Gclose(g0);
+    if (g1 < 0) return -1;
+    hid_t g = ensure_group(g1, name);
+    H5Gclose(g1);
+    if (g < 0) return -1;
+    uint64_t total = 0;
+    std::vector<uint64_t> limits(2 * T);
+    for (uint64_t t = 0; t < T; ++t) {
+        limits[2 * t] = total;
+        total += frame_sizes[t];
+        limits[2 * t + 1] = total;
+    }
+    std::vector<Record> rec(total);
+    for (uint64_t k = 0; k < total; ++k) {
+        rec[k].id = ids[k];
+        rec[k].typeId = types[k];
+        rec[k].flavor = flavors[k];
+        for (int a = 0; a < 3; ++a) rec[k].pos[a] = pos[3 * k + a];
+    }
+    hid_t mem, file;
+    record_types(&mem, &file);
+    hsize_t dr[1] = {total};
+    hid_t space = H5Screate_simple(1, dr, nullptr);
+    hid_t ds = H5Dcreate2(g, "records", file, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    int rc = ds < 0 ? -1 : (H5Dwrite(ds, mem, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0 ? -1 : 0);
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(space);
+    H5Tclose(mem);
+    H5Tclose(file);
+    hsize_t dl[2] = {T, 2}, dt[1] = {T};
+    if (rc == 0) rc = write_u64(g, "limits", 2, dl, limits.data());
+    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_particle_types(int64_t f, uint32_t n, const char *const *names, const double *diffusion,
+                          const char *const *flavors) {
+    hid_t g = config_group((hid_t)f);
+    if (g < 0) return -1;
+    std::vector<TypeInfo> info(n);
+    for (uint32_t k = 0; k < n; ++k) info[k] = {names[k], flavors[k], k, diffusion[k]};
+    hid_t s = vlen_string();
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(TypeInfo));
+    H5Tinsert(t, "name", HOFFSET(TypeInfo, name), s);
+    H5Tinsert(t, "flavor", HOFFSET(TypeInfo, flavor), s);
+    H5Tinsert(t, "type_id", HOFFSET(TypeInfo, type_id), H5T_NATIVE_UINT64);
+    H5Tinsert(t, "diffusion_constant", HOFFSET(TypeInfo, diffusion_constant), H5T_NATIVE_DOUBLE);
+    const int rc = write_compound(g, "particle_types", t, n, info.data());
+    H5Tclose(t);
+    H5Tclose(s);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_registered_reactions(int64_t f, uint32_t n, const char *const *names, const uint64_t *ids,
+                                const uint64_t *n_educts, const uint64_t *n_products, const double *rate,
+                                const double *educt_distance, const double *product_distance,
+                                const uint16_t *educt_types, const uint16_t *product_types) {
+    hid_t g = config_group((hid_t)f);
+    if (g < 0) return -1;
+    std::vector<ReactionInfo> info(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        info[k] = {names[k], ids[k], n_educts[k], n_products[k], rate[k], educt_distance[k], product_distance[k],
+                   {educt_types[2 * k], educt_types[2 * k + 1]}, {product_types[2 * k], product_types[2 * k + 1]}};
+    }
+    const hsize_t two[1] = {2};
+    hid_t s = vlen_string();
+    hid_t pair = H5Tarray_create2(H5T_NATIVE_UINT16, 1, two);
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(ReactionInfo));
+    H5Tinsert(t, "name", HOFFSET(ReactionInfo, name), s);
+    H5Tinsert(t, "id", HOFFSET(ReactionInfo, id), H5T_NATIVE_UINT64);
+    H5Tinsert(t, "n_educts", HOFFSET(ReactionInfo, n_educts), H5T_NATIVE_UINT64);
+    H5Tinsert(t, "n_products", HOFFSET(ReactionInfo, n_products), H5T_NATIVE_UINT64);
+    H5Tinsert(t, "rate", HOFFSET(ReactionInfo, rate), H5T_NATIVE_DOUBLE);
+    H5Tinsert(t, "educt_distance", HOFFSET(ReactionInfo, educt_distance), H5T_NATIVE_DOUBLE);
+    H5Tinsert(t, "product_distance", HOFFSET(ReactionInfo, product_distance), H5T_NATIVE_DOUBLE);
+    H5Tinsert(t, "educt_types", HOFFSET(ReactionInfo, educt_types), pair);
+    H5Tinsert(t, "product_types", HOFFSET(ReactionInfo, product_types), pair);
+    const int rc = write_compound(g, "registered_reactions", t, n, info.data());
+    H5Tclose(t);
+    H5Tclose(pair);
+    H5Tclose(s);
+    H5Gclose(g);
+    return rc;
+}
+
+// a scalar NUL-terminated fixed-length string, the form H5LTmake_dataset_string writes
+int rdy_h5_general(int64_t f, const char *json) {
+    hid_t g = config_group((hid_t)f);
+    if (g < 0) return -1;
+    hid_t type = H5Tcopy(H5T_C_S1);
+    H5Tset_size(type, std::string(json).size() + 1);
+    hid_t space = H5Screate(H5S_SCALAR);
+    hid_t ds = H5Dcreate2(g, "general", type, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    const int rc = ds < 0 ? -1 : (H5Dwrite(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, json) < 0 ? -1 : 0);
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(space);
+    H5Tclose(type);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_attribute(int64_t f, const char *key, const char *value) {
+    hid_t g = config_group((hid_t)f);
     if (g < 0) return -1;
     hid_t type = H5Tcopy(H5T_C_S1);
     H5Tset_size(type, std::string(value).size() + 1);
@@ -125,6 +351,78 @@ int rdy_h5_read_u64(const char *path, const char *dataset, uint64_t cap, uint64_
         if (count >= 0 && (uint64_t)count <= cap)
             rc = H5Dread(ds, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0 ? -1 : 0;
         H5Sclose(space);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int rdy_h5_read_f64(const char *path, const char *dataset, uint64_t cap, double *out, uint64_t *n) {
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = -1;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t space = H5Dget_space(ds);
+        const hssize_t count = H5Sget_simple_extent_npoints(space);
+        *n = count < 0 ? 0 : (uint64_t)count;
+        if (count >= 0 && (uint64_t)count <= cap)
+            rc = H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0 ? -1 : 0;
+        H5Sclose(space);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int rdy_h5_read_records(const char *path, const char *dataset, uint64_t cap, uint64_t *ids, uint16_t *types,
+                        uint8_t *flavors, double *pos, uint64_t *n) {
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = -1;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t space = H5Dget_space(ds);
+        const hssize_t count = H5Sget_simple_extent_npoints(space);
+        *n = count < 0 ? 0 : (uint64_t)count;
+        if (count >= 0 && (uint64_t)count <= cap) {
+            std::vector<Record> rec((size_t)count);
+            hid_t mem, file;
+            record_types(&mem, &file);
+            rc = H5Dread(ds, mem, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0 ? -1 : 0;
+            H5Tclose(mem);
+            H5Tclose(file);
+            for (size_t k = 0; rc == 0 && k < rec.size(); ++k) {
+                ids[k] = rec[k].id;
+                types[k] = rec[k].typeId;
+                flavors[k] = rec[k].flavor;
+                for (int a = 0; a < 3; ++a) pos[3 * k + a] = rec[k].pos[a];
+            }
+        }
+        H5Sclose(space);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int rdy_h5_read_string(const char *path, const char *dataset, uint64_t cap, char *out) {
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = -1;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t ft = H5Dget_type(ds);
+        if (H5Tget_class(ft) == H5T_STRING && !H5Tis_variable_str(ft)) {
+            const size_t size = H5Tget_size(ft);
+            if (size + 1 <= cap) {
+                hid_t mt = H5Tcopy(H5T_C_S1);
+                H5Tset_size(mt, size + 1);
+                rc = H5Dread(ds, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0 ? -1 : 0;
+                H5Tclose(mt);
+            }
+        }
+        H5Tclose(ft);
         H5Dclose(ds);
     }
     H5Fclose(f);

@@ -7,7 +7,8 @@ every admitted request (one hipGraph-replayed batch per scheduling interval), ea
 own step budget and handler.  Between intervals the server samples observables for all replicas at
 once (particles per type, reaction totals), retires finished requests (final state read back, slot
 freed) and admits new ones.  Observables go to HDF5 files in ReaDDy's layout on a writer thread,
-off the critical path.
+off the critical path; a server built with record_trajectory also reads every replica's particles
+back at each interval and writes them as a FlatTrajectory (one frame per interval).
 
 Every request's trajectory depends only on its own seed and steps (replica-local handles and
 Philox counters, fixed-point force sums), so a result is the same alone and in any mix.
@@ -17,7 +18,6 @@ repulsion between all species, A-A weak interaction, A+B<->C) at density 0.125, 
 log-uniform in [1000, 200000], step counts log-uniform in [1000, 100000] (scaled by `step_scale`
 for finite traces), handlers 50/50, Poisson arrivals.
 """
-import ctypes
 import math
 import os
 import queue
@@ -28,13 +28,13 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
+from . import h5io
 from .engine import Engine
 
 DENSITY = 0.125
 DT = 0.005
 DIFFUSION = [1.0, 1.0, 0.5]
 HANDLERS = {"UncontrolledApproximation": L.HANDLER_UA, "Gillespie": L.HANDLER_GILLESPIE}
-_HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def chemistry():
@@ -76,6 +76,7 @@ class Result:
     observed_steps: list = field(default_factory=list)
     observed_counts: list = field(default_factory=list)
     observed_reactions: list = field(default_factory=list)
+    observed_frames: list = field(default_factory=list)  # (handles, types, positions) per interval
 
     @property
     def latency(self):
@@ -95,20 +96,13 @@ def poisson_trace(n_requests, rate_per_s=200.0, n_range=(1000, 200000), steps_ra
 
 
 class H5Writer:
-    """Writes each finished request's observables to <dir>/request_<rid>.h5 on a background thread."""
+    """Writes each finished request's observables (and its trajectory, when recorded) to
+    <dir>/request_<rid>.h5 on a background thread."""
 
     def __init__(self, directory):
         self.dir = directory
         os.makedirs(directory, exist_ok=True)
-        lib = ctypes.CDLL(os.path.join(_HERE, "libreaddy_h5.so"))
-        lib.rdy_h5_create.restype = ctypes.c_int64
-        lib.rdy_h5_create.argtypes = [ctypes.c_char_p]
-        u64p = ctypes.POINTER(ctypes.c_uint64)
-        for fn in ("rdy_h5_n_particles", "rdy_h5_reaction_counts"):
-            getattr(lib, fn).argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, u64p, u64p]
-        lib.rdy_h5_attribute.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p]
-        lib.rdy_h5_close.argtypes = [ctypes.c_int64]
-        self.lib = lib
+        h5io.lib()  # fails here, not on the writer thread, when the library is missing
         self.q = queue.Queue()
         self.errors = []
         self.written = []
@@ -123,30 +117,26 @@ class H5Writer:
         self.t.join()
 
     def _run(self):
-        u64p = ctypes.POINTER(ctypes.c_uint64)
         while True:
             res = self.q.get()
             if res is None:
                 return
             try:
                 path = os.path.join(self.dir, f"request_{res.request.rid}.h5")
-                f = self.lib.rdy_h5_create(path.encode())
-                if f < 0:
-                    raise RuntimeError(f"cannot create {path}")
-                t = np.ascontiguousarray(res.observed_steps, dtype=np.uint64)
-                c = np.ascontiguousarray(res.observed_counts, dtype=np.uint64).reshape(len(t), -1)
-                r = np.ascontiguousarray(res.observed_reactions, dtype=np.uint64).reshape(len(t), -1)
-                rc = self.lib.rdy_h5_n_particles(f, b"n_particles", len(t), c.shape[1], t.ctypes.data_as(u64p),
-                                                 c.ctypes.data_as(u64p))
-                rc |= self.lib.rdy_h5_reaction_counts(f, b"reaction_counts", len(t), r.shape[1],
-                                                      t.ctypes.data_as(u64p), r.ctypes.data_as(u64p))
-                for k, v in (("handler", res.request.handler), ("particles", str(res.request.n)),
-                             ("steps", str(res.request.steps)), ("seed", str(res.request.seed)),
-                             ("timestep", repr(DT))):
-                    rc |= self.lib.rdy_h5_attribute(f, k.encode(), v.encode())
-                rc |= self.lib.rdy_h5_close(f)
-                if rc:
-                    raise RuntimeError(f"HDF5 write failed for {path}")
+                with h5io.File(path) as f:
+                    f.config([(name, d) for name, d in zip("ABC", DIFFUSION)], [res.request.box] * 3,
+                             reactions=[dict(rx, id=k, name=("fusion", "fission")[k])
+                                        for k, rx in enumerate(chemistry()[1])])
+                    t = res.observed_steps
+                    f.n_particles("n_particles", t, res.observed_counts)
+                    f.reaction_counts("reaction_counts", t, res.observed_reactions)
+                    if res.observed_frames:
+                        # every particle of these requests is a normal one (flavor 0)
+                        f.flat_trajectory("trajectory", t, [(h, ty, np.zeros(len(h), np.uint8), x)
+                                                            for x, ty, h in res.observed_frames])
+                    for k, v in (("handler", res.request.handler), ("particles", res.request.n),
+                                 ("steps", res.request.steps), ("seed", res.request.seed), ("timestep", repr(DT))):
+                        f.attribute(k, v)
                 self.written.append(path)
             except Exception as exc:  # reported by the server after the run
                 self.errors.append(exc)
@@ -161,7 +151,7 @@ class Server:
     """
 
     def __init__(self, hbm_budget_bytes, max_slots=512, interval_steps=100, nl_cutoff=2.0, out_dir=None,
-                 max_particles_per_request=200000):
+                 max_particles_per_request=200000, record_trajectory=False):
         pots, rxs = chemistry()
         self.interval = int(interval_steps)
         self.budget = int(hbm_budget_bytes)
@@ -174,6 +164,7 @@ class Server:
                              max_replicas=max_slots, max_cells=max_cells, max_particles=max_particles)
         self.max_slots = max_slots
         self.writer = H5Writer(out_dir) if out_dir else None
+        self.record_trajectory = bool(record_trajectory)
         self.n_reactions = len(rxs)
 
     def footprint(self, req):
@@ -244,6 +235,8 @@ class Server:
                 res.observed_steps.append(done)
                 res.observed_counts.append(counts[r].copy())
                 res.observed_reactions.append(eng.reaction_totals(r).copy())
+                if self.record_trajectory:
+                    res.observed_frames.append(eng.get_particles(r))
                 if done >= req.steps:
                     if eng.steps_done(r) != req.steps:
                         raise RuntimeError(f"request {req.rid}: {eng.steps_done(r)} steps done, {req.steps} asked")

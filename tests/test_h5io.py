@@ -1,0 +1,112 @@
+"""HDF5 output in ReaDDy's layout (CPU): FlatTrajectory records/limits/time (Trajectory.cpp:112-158,
+record fields as in Types.cpp:79-88) and RadialDistribution bin_centers/distribution/time
+(RadialDistribution.cpp:158-170) written through libreaddy_h5.so and read back; the dataset paths,
+the compound's field names and types checked with h5dump where the image has it."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+h5io = pytest.importorskip("readdy_amd.h5io")
+try:
+    h5io.lib()
+except OSError:
+    pytest.skip("libreaddy_h5.so not built", allow_module_level=True)
+
+
+def _frames(rng, sizes):
+    out = []
+    for n in sizes:
+        ids = rng.permutation(10 * n + 1)[:n].astype(np.uint64)
+        out.append((ids, rng.integers(0, 7, n).astype(np.uint16), rng.integers(0, 2, n).astype(np.uint8),
+                    rng.normal(size=(n, 3))))
+    return out
+
+
+def test_flat_trajectory_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    frames = _frames(rng, [5, 0, 17, 1])  # an empty frame keeps its [k, k) limits
+    time = [0, 10, 20, 35]
+    path = str(tmp_path / "traj.h5")
+    with h5io.File(path) as f:
+        f.flat_trajectory("traj", time, frames)
+        f.attribute("note", "round trip")
+    t, got = h5io.read_trajectory(path, "traj")
+    assert np.array_equal(t, time)
+    lim = h5io.read_u64(path, "/readdy/trajectory/traj/limits").reshape(-1, 2)
+    ends = np.cumsum([len(fr[0]) for fr in frames])
+    assert np.array_equal(lim[:, 1], ends) and np.array_equal(lim[1:, 0], ends[:-1]) and lim[0, 0] == 0
+    for (i1, t1, f1, p1), (i2, t2, f2, p2) in zip(frames, got):
+        assert np.array_equal(i1, i2) and np.array_equal(t1, t2) and np.array_equal(f1, f2)
+        assert np.array_equal(p1, p2)  # FP64 stored as FP64
+
+
+def test_radial_distribution_round_trip(tmp_path):
+    path = str(tmp_path / "rdf.h5")
+    borders = np.linspace(0.0, 3.0, 31)
+    centers = 0.5 * (borders[1:] + borders[:-1])
+    dist = np.random.default_rng(2).random((3, 30))
+    with h5io.File(path) as f:
+        f.radial_distribution("rdf", [0, 100, 200], centers, dist)
+        f.n_particles("n_particles", [0, 100, 200], [[1, 2], [3, 4], [5, 6]])
+    g = "/readdy/observables/rdf"
+    assert np.array_equal(h5io.read_f64(path, g + "/bin_centers"), centers)
+    assert np.array_equal(h5io.read_f64(path, g + "/distribution").reshape(3, 30), dist)
+    assert np.array_equal(h5io.read_u64(path, g + "/time"), [0, 100, 200])
+    assert np.array_equal(h5io.read_u64(path, "/readdy/observables/n_particles/data"), [1, 2, 3, 4, 5, 6])
+
+
+def test_missing_dataset_and_bad_frames_raise(tmp_path):
+    path = str(tmp_path / "x.h5")
+    with h5io.File(path) as f:
+        with pytest.raises(ValueError):
+            f.flat_trajectory("t", [0, 1], _frames(np.random.default_rng(3), [2]))
+        f.n_particles("n", [0], [[1]])
+    with pytest.raises(OSError):
+        h5io.read_u64(path, "/readdy/observables/none/time")
+
+
+@pytest.mark.skipif(shutil.which("h5dump") is None and not os.path.exists("/opt/conda/bin/h5dump"),
+                    reason="h5dump not in the image")
+def test_layout_seen_by_h5dump(tmp_path):
+    path = str(tmp_path / "traj.h5")
+    with h5io.File(path) as f:
+        f.flat_trajectory("traj", [0], _frames(np.random.default_rng(4), [3]))
+    exe = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    out = subprocess.run([exe, "-H", path], capture_output=True, text=True, check=True).stdout
+    for needle in ('GROUP "readdy"', 'GROUP "trajectory"', 'GROUP "traj"', 'DATASET "records"',
+                   'H5T_STD_U64LE "id"', 'H5T_STD_U16LE "typeId"', 'H5T_STD_U8LE "flavor"',
+                   'H5T_ARRAY { [3] H5T_IEEE_F64LE } "pos"', 'DATASET "limits"', 'DATASET "time"'):
+        assert needle in out, needle
+
+
+def _h5dump():
+    return shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+
+
+def test_config_group(tmp_path):
+    """/readdy/config as IOUtils.cpp:58-176 writes it: the general JSON (kbt, box_volume, box_size, pbc),
+    particle types with ids in order, registered reactions with their educt/product types."""
+    import json
+    path = str(tmp_path / "cfg.h5")
+    rxs = [dict(name="fus", id=0, educts=[0, 1], products=[2], rate=0.1, educt_distance=1.0),
+           dict(name="fis", id=1, educts=[2], products=[0, 1], rate=0.05, product_distance=1.5)]
+    with h5io.File(path) as f:
+        f.config([("A", 1.0), ("B", 1.0), ("C", 0.5, "TOPOLOGY")], [10.0, 12.0, 14.0], (True, False, True), 2.5, rxs)
+    g = json.loads(h5io.read_string(path, "/readdy/config/general"))
+    assert g == {"kbt": 2.5, "box_volume": 1680.0, "box_size": [10.0, 12.0, 14.0], "pbc": [True, False, True]}
+    exe = _h5dump()
+    if exe is None:
+        return
+    out = subprocess.run([exe, "-d", "/readdy/config/particle_types", path], capture_output=True, text=True,
+                         check=True).stdout
+    flat = " ".join(out.split())
+    for rec in ('"A", "NORMAL", 0, 1', '"B", "NORMAL", 1, 1', '"C", "TOPOLOGY", 2, 0.5'):
+        assert rec in flat, (rec, flat)
+    out = subprocess.run([exe, "-d", "/readdy/config/registered_reactions", path], capture_output=True, text=True,
+                         check=True).stdout
+    flat = " ".join(out.split())
+    assert '"fus", 0, 2, 1, 0.1, 1, 0, [ 0, 1 ], [ 2, 0 ]' in flat, flat
+    assert '"fis", 1, 1, 2, 0.05, 0, 1.5, [ 2, 0 ], [ 0, 1 ]' in flat, flat

@@ -1,14 +1,14 @@
 """Serving loop: trace generation and request state on the CPU; on the GPU, continuous batching
 gives every request the result it has alone, and observables land in HDF5 in ReaDDy's layout."""
-import ctypes
+import json
 import os
 
 import numpy as np
 import pytest
 
+from readdy_amd import h5io
 from readdy_amd import serving as S
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_trace_is_poisson_and_log_uniform():
@@ -32,13 +32,7 @@ def test_request_state_is_reproducible_and_in_the_box():
 
 
 def _h5_read(path, dataset):
-    lib = ctypes.CDLL(os.path.join(ROOT, "readdy_amd", "libreaddy_h5.so"))
-    out = np.zeros(1 << 16, dtype=np.uint64)
-    n = ctypes.c_uint64()
-    rc = lib.rdy_h5_read_u64(path.encode(), dataset.encode(), ctypes.c_uint64(len(out)),
-                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(n))
-    assert rc == 0, (path, dataset)
-    return out[:n.value]
+    return h5io.read_u64(path, dataset)
 
 
 @pytest.mark.gpu
@@ -63,3 +57,56 @@ def test_served_results_equal_results_alone(tmp_path):
         assert len(c0) == len(t) and c0[-1] == totals[0]
     # a budget that holds at most a few requests at a time made them queue and overlap
     assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
+
+
+@pytest.mark.gpu
+def test_served_trajectory_frames(tmp_path):
+    """With record_trajectory, request_<rid>.h5 holds one FlatTrajectory frame per observed interval:
+    the frame times are the observed step counts, each frame's size is the particle count observed
+    then, and the last frame is the request's final state (handles as ids, flavor 0)."""
+    trace = S.poisson_trace(3, rate_per_s=400.0, n_range=(500, 2000), steps_range=(30, 120), seed=12)
+    srv = S.Server(hbm_budget_bytes=128 << 20, max_slots=2, interval_steps=20, out_dir=str(tmp_path),
+                   record_trajectory=True)
+    results, _ = srv.run(trace, keep_final=True)
+    srv.close()
+    for req in trace:
+        res = results[req.rid]
+        path = os.path.join(str(tmp_path), f"request_{req.rid}.h5")
+        print("trajectory of request", req.rid, req.n, req.steps, flush=True)
+        t, frames = h5io.read_trajectory(path, "trajectory")
+        assert np.array_equal(t, res.observed_steps) and t[-1] == req.steps
+        for (ids, ty, fl, x), c in zip(frames, res.observed_counts):
+            assert len(ids) == int(np.sum(c)) and np.array_equal(np.bincount(ty, minlength=3), c)
+            assert not fl.any()
+        xf, tf, hf = res.final
+        ids, ty, _, x = frames[-1]
+        general = json.loads(h5io.read_string(path, "/readdy/config/general"))
+        assert general["box_size"] == [req.box] * 3 and general["pbc"] == [True] * 3
+        assert np.array_equal(ids, hf) and np.array_equal(ty, tf) and np.array_equal(x, xf)
+
+
+def test_writer_thread_files(tmp_path):
+    """The writer thread's file for a finished request (no GPU: a Result filled by hand)."""
+    req = S.Request(3, 40, 60, "Gillespie", 77, 0.0)
+    res = S.Result(req)
+    rng = np.random.default_rng(0)
+    for step, n in ((20, 40), (40, 39), (60, 41)):
+        ty = rng.integers(0, 3, n).astype(np.uint32)
+        res.observed_steps.append(step)
+        res.observed_counts.append(np.bincount(ty, minlength=3))
+        res.observed_reactions.append(np.array([step // 20, step // 30], dtype=np.uint64))
+        res.observed_frames.append((rng.normal(size=(n, 3)), ty, np.arange(n, dtype=np.uint64) + step))
+    w = S.H5Writer(str(tmp_path))
+    w.submit(res)
+    w.close()
+    assert not w.errors, w.errors
+    path = w.written[0]
+    assert path.endswith("request_3.h5")
+    t, frames = h5io.read_trajectory(path, "trajectory")
+    assert list(t) == [20, 40, 60]
+    for (x, ty, h), (ids, ty2, fl, x2) in zip(res.observed_frames, frames):
+        assert np.array_equal(ids, h) and np.array_equal(ty2, ty) and np.array_equal(x2, x) and not fl.any()
+    assert np.array_equal(h5io.read_u64(path, "/readdy/observables/n_particles/data").reshape(3, 3),
+                          np.array(res.observed_counts))
+    assert list(h5io.read_u64(path, "/readdy/observables/reaction_counts/counts/1")) == [0, 1, 2]
+    assert json.loads(h5io.read_string(path, "/readdy/config/general"))["box_size"] == [req.box] * 3
