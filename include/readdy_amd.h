@@ -154,6 +154,26 @@ int rdy_react(rdy_engine *e, int handler, double dt);
 int rdy_get_reaction_counts(rdy_engine *e, uint32_t replica, uint64_t *counts);
 /* reaction counts accumulated since the replica was added, [n_reactions] */
 int rdy_get_reaction_totals(rdy_engine *e, uint32_t replica, uint64_t *counts);
+/* one performed reaction: readdy::model::reactions::ReactionRecord (ReactionRecord.h:54-63), filled as
+ * the CPU kernel's performReaction fills it (ReactionUtils.h:155-225): educts' handles and types before
+ * the reaction (both entries equal for order 1), products' handles after it (0 where none), position =
+ * (pos1 + pos2) / 2 wrapped into the box; reaction = index in rdy_context_desc.reactions, type = RDY_RX_* */
+typedef struct {
+    uint32_t reaction;
+    int32_t type;
+    uint32_t educts[2];
+    uint32_t products[2];
+    uint32_t types_from[2];
+    double position[3];
+} rdy_reaction_record;
+/* Context::recordReactionsWithPositions (Context.h:69-261): on = 1 makes every reaction step keep its
+ * records (one pool of 64 B x 1024 per replica slot, allocated on the first switch-on); off by default */
+int rdy_record_reactions(rdy_engine *e, int on);
+/* the records of the replica's last reaction step (StateModel::reactionRecords, as the Reactions
+ * observable reads them, Reactions.cpp:60-90), in event-key order; *n = their number (records beyond
+ * cap are not written) */
+int rdy_get_reaction_records(rdy_engine *e, uint32_t replica, uint64_t cap, rdy_reaction_record *out,
+                             uint64_t *n);
 
 /* ---- the simulation loop: replaces api::SimulationLoop::run's body
  *      (include/readdy/api/SimulationLoop.h:256-306): per step integrate, neighbour list,

@@ -92,6 +92,10 @@ def lib():
         L.or_reaction_candidates.restype = ctypes.c_size_t
         L.or_reaction_candidates.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p]
         L.or_reactions.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_uint32, _u64p]
+        L.or_set_record_reactions.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_record_reactions.restype = None
+        L.or_reaction_records.restype = ctypes.c_void_p
+        L.or_reaction_records.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_reaction_counts.restype = _u64p
         L.or_reaction_counts.argtypes = [ctypes.c_void_p, _ip]
         L.or_loop_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _dp,
@@ -367,6 +371,22 @@ class System:
         n = ctypes.c_uint64()
         _check(lib().or_reactions(self._h, mode, dt, step, ctypes.byref(n)))
         return n.value
+
+    # or_reaction_record (readdy_oracle.h)
+    RECORD_DTYPE = np.dtype([("reaction", np.uint32), ("type", np.int32), ("educts", np.uint64, 2),
+                             ("products", np.uint64, 2), ("types_from", np.uint32, 2), ("position", np.float64, 3)])
+
+    def set_record_reactions(self, on=True):
+        lib().or_set_record_reactions(self._h, 1 if on else 0)
+
+    def reaction_records(self):
+        """Records of the last reactions() call, in the order the reactions were performed."""
+        n = ctypes.c_size_t()
+        p = lib().or_reaction_records(self._h, ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros(0, dtype=self.RECORD_DTYPE)
+        raw = ctypes.string_at(p, n.value * self.RECORD_DTYPE.itemsize)
+        return np.frombuffer(raw, dtype=self.RECORD_DTYPE).copy()
 
     def reaction_counts(self):
         n = ctypes.c_int()

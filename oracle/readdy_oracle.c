@@ -466,6 +466,9 @@ struct or_system {
     size_t *list;      /* [n + 1] */
     size_t list_len;
     uint64_t counts[OR_MAX_RX];
+    int record;                 /* keep reaction records (or_set_record_reactions) */
+    or_reaction_record *recs;
+    size_t n_recs, cap_recs;
     double phase_s[4]; /* or_loop_step wall time: integrator, neighborList (both fills), reactions, forces */
     int threads; /* > 1: forces, integration and the candidate search on that many OpenMP threads */
 };
@@ -498,6 +501,7 @@ void or_destroy(or_system *s) {
     free(s->adj);
     free(s->head);
     free(s->list);
+    free(s->recs);
     free(s);
 }
 
@@ -1162,6 +1166,25 @@ static void perform_reaction(or_system *s, const event_t *ev, update_t *up, uint
     const rx_t *r = &s->rx[ev->rx];
     entry_t *e1 = &s->e[ev->idx1];
     entry_t *e2 = &s->e[ev->idx2];
+    or_reaction_record *rec = NULL;
+    if (s->record) {
+        /* ReactionUtils.h:155-162: type, where = plain mean of the two positions, educts, types_from;
+         * the caller then applies fixPosition (CPUUncontrolledApproximation.cpp:214-217) */
+        if (s->n_recs == s->cap_recs) {
+            s->cap_recs = s->cap_recs ? 2 * s->cap_recs : 64;
+            s->recs = (or_reaction_record *)realloc(s->recs, s->cap_recs * sizeof(or_reaction_record));
+        }
+        rec = &s->recs[s->n_recs++];
+        memset(rec, 0, sizeof *rec);
+        rec->reaction = (uint32_t)ev->rx;
+        rec->type = r->kind;
+        for (int d = 0; d < 3; ++d) rec->where[d] = (e1->pos[d] + e2->pos[d]) / 2.;
+        or_fix_position(rec->where, s->box, s->pbc);
+        rec->educts[0] = e1->id;
+        rec->educts[1] = e2->id;
+        rec->types_from[0] = (uint32_t)e1->type;
+        rec->types_from[1] = (uint32_t)e2->type;
+    }
     switch (r->kind) {
     case OR_RX_DECAY:
         upd_removed(up, ev->idx1);
@@ -1169,6 +1192,7 @@ static void perform_reaction(or_system *s, const event_t *ev, update_t *up, uint
     case OR_RX_CONVERSION:
         e1->type = r->products[0];
         e1->id = next_id(s);
+        if (rec) rec->products[0] = e1->id;
         break;
     case OR_RX_ENZYMATIC:
         if (e1->type == r->educts[1]) {
@@ -1177,6 +1201,10 @@ static void perform_reaction(or_system *s, const event_t *ev, update_t *up, uint
         } else {
             e1->type = r->products[0];
             e1->id = next_id(s);
+        }
+        if (rec) {
+            rec->products[0] = e1->id;
+            rec->products[1] = e2->id;
         }
         break;
     case OR_RX_FISSION: {
@@ -1197,6 +1225,10 @@ static void perform_reaction(or_system *s, const event_t *ev, update_t *up, uint
         e1->id = next_id(s);
         for (int d = 0; d < 3; ++d) e1->pos[d] += r->w1 * r->product_distance * n3[d];
         or_fix_position(e1->pos, s->box, s->pbc);
+        if (rec) {
+            rec->products[0] = e1->id;
+            rec->products[1] = ne.id;
+        }
         break;
     }
     case OR_RX_FUSION: {
@@ -1212,6 +1244,7 @@ static void perform_reaction(or_system *s, const event_t *ev, update_t *up, uint
         upd_new(up, &ne);
         upd_removed(up, ev->idx1);
         upd_removed(up, ev->idx2);
+        if (rec) rec->products[0] = ne.id;
         break;
     }
     }
@@ -1360,8 +1393,9 @@ int or_perform_events_abstract(int n, const int64_t *p1, const int64_t *p2, cons
 }
 
 int or_reactions(or_system *s, int mode, double dt, uint32_t step, uint64_t *n_performed) {
-    /* counts are reset at the start of each handler call: CPUUncontrolledApproximation.cpp:143-148 */
+    /* counts and records are reset at the start of each handler call: CPUUncontrolledApproximation.cpp:143-148 */
     memset(s->counts, 0, sizeof s->counts);
+    s->n_recs = 0;
     ev_ud u;
     memset(&u, 0, sizeof u);
     u.dt = dt;
@@ -1443,6 +1477,16 @@ int or_reactions(or_system *s, int mode, double dt, uint32_t step, uint64_t *n_p
     free(u.ev);
     if (n_performed) *n_performed = done;
     return 0;
+}
+
+void or_set_record_reactions(or_system *s, int on) {
+    s->record = on != 0;
+    s->n_recs = 0;
+}
+
+const or_reaction_record *or_reaction_records(const or_system *s, size_t *n) {
+    if (n) *n = s->n_recs;
+    return s->recs;
 }
 
 const uint64_t *or_reaction_counts(const or_system *s, int *n) {

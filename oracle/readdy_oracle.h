@@ -113,6 +113,19 @@ int or_euler_bd(or_system *s, double dt, const double *noise_by_id, size_t noise
 size_t or_reaction_candidates(or_system *s, size_t cap, uint64_t *out);
 int or_reactions(or_system *s, int mode, double dt, uint32_t step, uint64_t *n_performed);
 const uint64_t *or_reaction_counts(const or_system *s, int *n);
+/* ReactionRecord (include/readdy/model/reactions/ReactionRecord.h:54-63) as performReaction fills it
+ * (kernels/cpu ReactionUtils.h:155-225); kept for the last or_reactions call when switched on
+ * (Context::recordReactionsWithPositions), cleared at its start (CPUUncontrolledApproximation.cpp:143-145) */
+typedef struct {
+    uint32_t reaction;
+    int32_t type;
+    uint64_t educts[2];
+    uint64_t products[2];
+    uint32_t types_from[2];
+    double where[3];
+} or_reaction_record;
+void or_set_record_reactions(or_system *s, int on);
+const or_reaction_record *or_reaction_records(const or_system *s, size_t *n);
 /* one SimulationLoop step (include/readdy/api/SimulationLoop.h:256-306): integrate, NL, reactions, NL, forces */
 int or_loop_step(or_system *s, double dt, int mode, uint32_t step, const double *noise_by_id, size_t noise_len,
                  double *energy);

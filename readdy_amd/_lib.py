@@ -87,6 +87,8 @@ SIGNATURES = {
     "rdy_react": (_int, [_vp, _int, _d]),
     "rdy_get_reaction_counts": (_int, [_vp, _u32, _u64p]),
     "rdy_get_reaction_totals": (_int, [_vp, _u32, _u64p]),
+    "rdy_record_reactions": (_int, [_vp, _int]),
+    "rdy_get_reaction_records": (_int, [_vp, _u32, _u64, _vp, _u64p]),
     "rdy_run": (_int, [_vp, _int, _d, _u64]),
     "rdy_run_graph": (_int, [_vp, _int, _d, _u64]),
     "rdy_replica_set_steps": (_int, [_vp, _u32, _u64]),

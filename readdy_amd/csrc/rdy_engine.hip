@@ -95,6 +95,9 @@ struct rdy_engine {
     uint64_t cells_hwm = 0;  // end of the highest cell range handed out (the scan covers [0, hwm))
     Event *events = nullptr;
     uint32_t *ev_count = nullptr;
+    RxRecord *recs = nullptr;  // reaction records of each replica's last reaction step (when recording)
+    uint32_t *rec_count = nullptr;
+    bool recording = false;
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
     uint32_t *err = nullptr;
     double *force_out = nullptr;
@@ -581,6 +584,9 @@ void for_brick_groups(const rdy_engine *e, F &&launch) {
 // list of bricks for the bin-reading forces (0) or search (1) kernel
 uint32_t *slow_list(const rdy_engine *e, int which) { return e->d_slow + (size_t)which * (e->d_bricks_cap + 2); }
 
+// the record pool handed to the resolve kernels (null unless reactions are being recorded)
+RxRecord *rec_ptr(const rdy_engine *e) { return e->recording ? e->recs : nullptr; }
+
 // blocks of the bin-reading kernels (they loop over the listed bricks; none listed: they just exit)
 uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 256u); }
 
@@ -778,7 +784,8 @@ void launch_step(rdy_engine *e, int handler, double dt, int last) {
         hipLaunchKernelGGL(k_resolve_step, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->grid(n),
                            e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap, e->radius, h, dt,
                            e->events, e->ev_count, slow_list(e, 0), e->rx_step, e->rx_total,
-                           (uint32_t)e->max_particles, e->edits, e->edit_count, e->edit_cap, e->err);
+                           (uint32_t)e->max_particles, e->edits, e->edit_count, e->edit_cap, rec_ptr(e),
+                           e->recording ? e->rec_count : nullptr, e->err);
         hipLaunchKernelGGL(k_fix_sites, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c),
                            (const ReplicaDev *)e->reps, e->T, e->cap, e->radius, (const Edit *)e->edits,
                            (const uint32_t *)e->edit_count, e->edit_cap, e->aff_bits, e->aff_list, e->edit_count + 1,
@@ -949,7 +956,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->wl[b]);
         (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
                     e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_rowmax, e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -1335,7 +1342,8 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
                        e->ev_count, slow_list(e, 1), e->err);
     hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
                        e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles,
-                       (Edit *)nullptr, (uint32_t *)nullptr, 0u, e->err);
+                       (Edit *)nullptr, (uint32_t *)nullptr, 0u, rec_ptr(e), e->recording ? e->rec_count : nullptr,
+                       e->err);
     return check_errors(e);
 }
 
@@ -1345,6 +1353,54 @@ int rdy_get_reaction_counts(rdy_engine *e, uint32_t r, uint64_t *counts) {
     if (e->n_rx == 0) return RDY_OK;
     HIP_TRY(hipMemcpyAsync(counts, e->rx_step + (size_t)r * e->n_rx, e->n_rx * 8, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+int rdy_record_reactions(rdy_engine *e, int on) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (on && !e->recs) {
+        HIP_TRY(dalloc(&e->recs, (size_t)e->max_reps * RESOLVE_MAX_EVENTS));
+        HIP_TRY(dalloc(&e->rec_count, e->max_reps));
+        HIP_TRY(hipMemsetAsync(e->rec_count, 0, e->max_reps * sizeof(uint32_t), e->stream));
+    }
+    e->recording = on != 0;
+    e->version++;  // captured steps carry the record pointers
+    return RDY_OK;
+}
+
+int rdy_get_reaction_records(rdy_engine *e, uint32_t r, uint64_t cap, rdy_reaction_record *out, uint64_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (!n) return set_err(RDY_ERR_INVALID, "n is NULL");
+    *n = 0;
+    if (!e->recs || !e->recording) return set_err(RDY_ERR_STATE, "reaction records are not being kept");
+    uint32_t ne = 0;
+    HIP_TRY(hipMemcpyAsync(&ne, e->rec_count + r, sizeof ne, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    ne = std::min<uint32_t>(ne, RESOLVE_MAX_EVENTS);
+    std::vector<RxRecord> h(ne);
+    if (ne) {
+        HIP_TRY(hipMemcpyAsync(h.data(), e->recs + (size_t)r * RESOLVE_MAX_EVENTS, ne * sizeof(RxRecord),
+                               hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    uint64_t k = 0;
+    for (const RxRecord &x : h) {
+        if (!x.valid) continue;
+        if (k < cap && out) {
+            rdy_reaction_record &o = out[k];
+            o.reaction = x.rx;
+            o.type = (int32_t)x.kind;
+            for (int j = 0; j < 2; ++j) {
+                o.educts[j] = x.educts[j];
+                o.products[j] = x.products[j];
+                o.types_from[j] = x.types_from[j];
+            }
+            for (int d = 0; d < 3; ++d) o.position[d] = x.where[d];
+        }
+        ++k;
+    }
+    *n = k;
     return RDY_OK;
 }
 

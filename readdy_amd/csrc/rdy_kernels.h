@@ -67,6 +67,18 @@ struct Edit {
     uint32_t slot, info, r, pad;
 };
 
+// one performed reaction as ReactionUtils.h:155-225 fills a ReactionRecord: the educts' handles and
+// types before the reaction, the products' handles, where = (pos1 + pos2) / 2 wrapped into the box
+// (the plain mean, no minimum image).  Slot k of a replica's region holds sorted event k; valid is 0
+// for events that did not fire.
+struct RxRecord {
+    uint32_t rx, kind;
+    uint32_t educts[2], products[2], types_from[2];
+    double where[3];
+    uint32_t valid, pad;
+};
+static_assert(sizeof(RxRecord) == 64, "RxRecord layout");
+
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
              ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, N_ERR = 8 };

@@ -684,7 +684,7 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
                                                 Event *events, uint32_t *ev_count, unsigned long long *rx_step,
                                                 unsigned long long *rx_total, int require_active, uint32_t max_wl,
                                                 Edit *edits, uint32_t *edit_count, uint32_t edit_cap,
-                                                uint32_t *err) {
+                                                RxRecord *recs, uint32_t *rec_count, uint32_t *err) {
     constexpr int E = RESOLVE_MAX_EVENTS;
     constexpr int HT = 4 * E;  // hash table entries (power of two, > 2 slots per event)
     __shared__ unsigned long long skey[E];
@@ -712,7 +712,10 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
         rxcount[tid] = 0;
     }
     if (ne == 0) {
-        if (tid == 0) ev_count[r] = 0;
+        if (tid == 0) {
+            ev_count[r] = 0;
+            if (recs) rec_count[r] = 0;
+        }
         return;
     }
     if (ne > R.ev_cap) ne = R.ev_cap;
@@ -865,14 +868,46 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
         if (p < edit_cap) edits[p] = Edit{slot, kind | (old_type << 2), r, old_handle};
         else atomicAdd(&err[ERR_EDIT_OVERFLOW], 1u);
     };
+    RxRecord *rrec = recs ? recs + (size_t)r * E : nullptr;
     for (uint32_t k = tid; k < ne; k += blockDim.x) {
-        if (status[k] != 1) continue;
+        if (status[k] != 1) {
+            if (rrec) rrec[k].valid = 0u;
+            continue;
+        }
         const Event e = ev[sidx[k]];
         const RxDev X = T.rx[e.rx];
         atomicAdd(&rxcount[e.rx & 255u], 1u);
         State a = cur.bins[e.s1];
         const uint32_t flags_keep = a.tw & TW_TOPO;
         const uint32_t hnew = hbase + prefix[k];
+        if (rrec) {
+            const State b = cur.bins[e.s2];
+            RxRecord rc;
+            rc.rx = e.rx;
+            rc.kind = (uint32_t)X.kind;
+            rc.educts[0] = a.h;
+            rc.educts[1] = b.h;
+            rc.types_from[0] = tw_type(a.tw);
+            rc.types_from[1] = tw_type(b.tw);
+            rc.where[0] = wrap((a.x + b.x) / 2., geo.box[0], geo.pbc[0]);
+            rc.where[1] = wrap((a.y + b.y) / 2., geo.box[1], geo.pbc[1]);
+            rc.where[2] = wrap((a.z + b.z) / 2., geo.box[2], geo.pbc[2]);
+            // products as the record holds them after the reaction (ReactionUtils.h:170-224)
+            rc.products[0] = rc.products[1] = 0u;
+            if (X.kind == RX_CONVERSION || X.kind == RX_FUSION) rc.products[0] = hnew;
+            if (X.kind == RX_FISSION) {
+                rc.products[0] = hnew + 1u;  // the educt's record, renamed after the new particle's id
+                rc.products[1] = hnew;
+            }
+            if (X.kind == RX_ENZYMATIC) {
+                const bool a_is_catalyst = tw_type(a.tw) == X.educts[1];
+                rc.products[0] = a_is_catalyst ? a.h : hnew;
+                rc.products[1] = a_is_catalyst ? hnew : b.h;
+            }
+            rc.valid = 1u;
+            rc.pad = 0u;
+            rrec[k] = rc;
+        }
         switch (X.kind) {
         case RX_DECAY:
             cur.bins[e.s1].tw = a.tw | TW_DEAD;
@@ -957,6 +992,7 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
     if (tid == 0) {
         R.handle_next = hbase + s_total;
         ev_count[r] = 0;
+        if (recs) rec_count[r] = ne;
     }
 }
 
@@ -964,9 +1000,10 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
                                                              Event *events, uint32_t *ev_count,
                                                              unsigned long long *rx_step, unsigned long long *rx_total,
                                                              int require_active, uint32_t max_wl, Edit *edits,
-                                                             uint32_t *edit_count, uint32_t edit_cap, uint32_t *err) {
+                                                             uint32_t *edit_count, uint32_t edit_cap, RxRecord *recs,
+                                                             uint32_t *rec_count, uint32_t *err) {
     resolve_replica(cur, reps, T, cap, events, ev_count, rx_step, rx_total, require_active, max_wl, edits, edit_count,
-                    edit_cap, err);
+                    edit_cap, recs, rec_count, err);
 }
 
 // marks every particle of replica r dead in the current buffer

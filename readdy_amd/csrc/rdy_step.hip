@@ -470,7 +470,8 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve_step(Grid cur, Grid
                                                                   unsigned long long *rx_step,
                                                                   unsigned long long *rx_total, uint32_t max_wl,
                                                                   Edit *edits, uint32_t *edit_count,
-                                                                  uint32_t edit_cap, uint32_t *err) {
+                                                                  uint32_t edit_cap, RxRecord *recs,
+                                                                  uint32_t *rec_count, uint32_t *err) {
     const uint32_t n = slow_count(slow);  // the same for every block: the walk has ended
     for (uint32_t i = 0; i < n; ++i) {    // block-uniform
         const BrickDev B = bricks[slow[2 + i]];
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve_step(Grid cur, Grid
         __threadfence();  // the listed bricks' candidates before the replica's list is read
     }
     resolve_replica(cur, reps, T, cap, events, ev_count, rx_step, rx_total, 1, max_wl, edits, edit_count, edit_cap,
-                    err);
+                    recs, rec_count, err);
     if (n) release_slow_list(slow);
 }
 

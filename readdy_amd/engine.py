@@ -212,6 +212,23 @@ class Engine:
         L.check(self._lib.rdy_get_reaction_totals(self._h, r, _ptr(c, _u64p)))
         return c[:self.n_reactions]
 
+    # rdy_reaction_record (include/readdy_amd.h), a ReactionRecord (ReactionRecord.h:54-63)
+    RECORD_DTYPE = np.dtype([("reaction", np.uint32), ("type", np.int32), ("educts", np.uint32, 2),
+                             ("products", np.uint32, 2), ("types_from", np.uint32, 2), ("position", np.float64, 3)])
+
+    def record_reactions(self, on=True):
+        """Keep every reaction step's records (Context::recordReactionsWithPositions)."""
+        L.check(self._lib.rdy_record_reactions(self._h, 1 if on else 0))
+
+    def reaction_records(self, r=0):
+        """The records of the replica's last reaction step, in event-key order (RECORD_DTYPE)."""
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_get_reaction_records(self._h, r, 0, None, ctypes.byref(n)))
+        out = np.zeros(max(1, n.value), dtype=self.RECORD_DTYPE)
+        L.check(self._lib.rdy_get_reaction_records(self._h, r, len(out), out.ctypes.data_as(ctypes.c_void_p),
+                                                   ctypes.byref(n)))
+        return out[:n.value]
+
     def run(self, handler, dt, n_steps, graph=False):
         fn = self._lib.rdy_run_graph if graph else self._lib.rdy_run
         L.check(fn(self._h, handler, dt, int(n_steps)))

@@ -7,13 +7,15 @@ Writers (one file per call sequence: create, datasets, attributes, close):
                        (ReactionCounts, readdy/main/model/observables/Observables.cpp)
   radial_distribution  /readdy/observables/<name>/{bin_centers [B], distribution [T][B], time [T]}
                        (RadialDistribution.cpp:158-170)
+  reactions            /readdy/observables/<name>/{records [T] (variable-length lists of reaction
+                       records), time [T]} (Reactions.cpp:72-86; record fields as in Types.cpp:66-77)
   flat_trajectory      /readdy/trajectory/<name>/{records [R], limits [T][2], time [T]}
                        (FlatTrajectory, Trajectory.cpp:112-158; records are {id u64, typeId u16,
                        flavor u8, pos f64[3]} as in Types.cpp:79-88, limits a frame's [begin, end))
   config               /readdy/config/{particle_types, registered_reactions, general}
                        (IOUtils.cpp:58-176: what readdy.Trajectory reads for type names, reactions,
                        kBT, box and periodicity)
-Readers (tests and tools): read_u64, read_f64, read_records, read_string.
+Readers (tests and tools): read_u64, read_f64, read_records, read_string, read_reactions.
 """
 import ctypes
 import json
@@ -26,7 +28,12 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u16p = ctypes.POINTER(ctypes.c_uint16)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 _lib = None
+
+# a reaction record as read back (the fields of the reference's ReactionRecord type)
+REACTION_DTYPE = np.dtype([("reaction", np.uint64), ("type", np.int32), ("educts", np.uint64, 2),
+                           ("products", np.uint64, 2), ("types_from", np.uint16, 2), ("position", np.float64, 3)])
 
 
 def lib():
@@ -49,6 +56,10 @@ def lib():
                                                   _f64p, _f64p, _u16p, _u16p]
         h.rdy_h5_general.argtypes = [ctypes.c_int64, ctypes.c_char_p]
         h.rdy_h5_read_string.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p]
+        h.rdy_h5_reactions.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p, _u64p, _i32p,
+                                       _u64p, _u64p, _f64p, _u16p]
+        h.rdy_h5_read_reactions.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
+                                            _u64p, _i32p, _u64p, _u64p, _f64p, _u16p, _u64p, _u64p]
         h.rdy_h5_attribute.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p]
         h.rdy_h5_close.argtypes = [ctypes.c_int64]
         h.rdy_h5_read_u64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p]
@@ -109,6 +120,23 @@ class File:
         self._check(lib().rdy_h5_flat_trajectory(self.f, name.encode(), len(t), _p(t, _u64p), _p(sizes, _u64p),
                                                  _p(ids, _u64p), _p(types, _u16p), _p(flav, _u8p),
                                                  _p(pos, _f64p)), name)
+
+    def reactions(self, name, time, frames):
+        """frames: one array of reaction records per entry of time, each with the fields reaction, type,
+        educts [2], products [2], types_from [2], position [3] (Engine.reaction_records' layout)."""
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        if len(frames) != len(t):
+            raise ValueError("one frame per time point")
+        sizes = np.array([len(f) for f in frames], dtype=np.uint64)
+        cat = lambda k, dt, w: (np.ascontiguousarray(np.concatenate([np.asarray(f[k], dtype=dt).reshape(-1, w)
+                                                                     for f in frames]), dtype=dt)
+                                if int(sizes.sum()) else np.zeros((0, w), dt))
+        rid, typ = cat("reaction", np.uint64, 1), cat("type", np.int32, 1)
+        ed, pr = cat("educts", np.uint64, 2), cat("products", np.uint64, 2)
+        tf, pos = cat("types_from", np.uint16, 2), cat("position", np.float64, 3)
+        self._check(lib().rdy_h5_reactions(self.f, name.encode(), len(t), _p(t, _u64p), _p(sizes, _u64p),
+                                           _p(rid, _u64p), _p(typ, _i32p), _p(ed, _u64p), _p(pr, _u64p),
+                                           _p(pos, _f64p), _p(tf, _u16p)), name)
 
     def config(self, types, box, pbc=(True, True, True), kbt=1.0, reactions=()):
         """types: (name, diffusion constant[, flavor]) per type id; reactions: dicts with name, id, educts,
@@ -200,6 +228,31 @@ def read_string(path, dataset, cap=1 << 16):
     if lib().rdy_h5_read_string(path.encode(), dataset.encode(), cap, buf):
         raise OSError(f"cannot read {dataset} of {path}")
     return buf.value.decode()
+
+
+def read_reactions(path, name):
+    """(time, [records per frame]) of /readdy/observables/<name> (REACTION_DTYPE)."""
+    g = f"/readdy/observables/{name}"
+    t = read_u64(path, g + "/time")
+    nf, n = ctypes.c_uint64(), ctypes.c_uint64()
+    sizes = np.zeros(max(1, len(t)), np.uint64)
+    one = np.zeros(1, np.uint64)
+    # first pass: the record count (a zero-capacity read reports it)
+    lib().rdy_h5_read_reactions(path.encode(), (g + "/records").encode(), len(sizes), _p(sizes, _u64p), 0,
+                                _p(one, _u64p), None, None, None, None, None, ctypes.byref(nf), ctypes.byref(n))
+    cap = max(1, n.value)
+    rid, typ = np.zeros(cap, np.uint64), np.zeros(cap, np.int32)
+    ed, pr = np.zeros((cap, 2), np.uint64), np.zeros((cap, 2), np.uint64)
+    tf, pos = np.zeros((cap, 2), np.uint16), np.zeros((cap, 3), np.float64)
+    if lib().rdy_h5_read_reactions(path.encode(), (g + "/records").encode(), len(sizes), _p(sizes, _u64p), cap,
+                                   _p(rid, _u64p), _p(typ, _i32p), _p(ed, _u64p), _p(pr, _u64p), _p(pos, _f64p),
+                                   _p(tf, _u16p), ctypes.byref(nf), ctypes.byref(n)):
+        raise OSError(f"cannot read {g}/records of {path}")
+    out = np.zeros(n.value, REACTION_DTYPE)
+    out["reaction"], out["type"], out["educts"] = rid[:n.value], typ[:n.value], ed[:n.value]
+    out["products"], out["types_from"], out["position"] = pr[:n.value], tf[:n.value], pos[:n.value]
+    ends = np.cumsum(sizes[:nf.value].astype(np.int64))
+    return t, [out[b - int(s):b] for b, s in zip(ends, sizes[:nf.value])]
 
 
 def read_trajectory(path, name):

@@ -10,6 +10,9 @@
 //   /readdy/trajectory/<name>/records [R] {id u64, typeId u16, flavor u8, pos f64[3]},
 //                             limits [T][2] uint64 (a frame's [begin, end) in records), time [T]
 //                                                   (FlatTrajectory, Trajectory.cpp:112-158; Types.cpp:79-88)
+//   /readdy/observables/<name>/records [T] vlen of {reaction_id u64, reaction_type i32, educts u64[2],
+//                             products u64[2], position f64[3], types_from u16[2]}, time [T]
+//                                                   (Reactions.cpp:72-86; Types.cpp:66-77)
 //   /readdy/config/particle_types        [n] {name str, flavor str, type_id u64, diffusion_constant f64}
 //   /readdy/config/registered_reactions  [n] {name str, id u64, n_educts u64, n_products u64, rate,
 //                                         educt_distance, product_distance f64, educt_types u16[2],
@@ -29,6 +32,10 @@
 //   int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time,
 //                              const uint64_t *frame_sizes, const uint64_t *ids, const uint16_t *types,
 //                              const uint8_t *flavors, const double *pos);          pos [R][3]
+//   int rdy_h5_reactions(int64_t f, const char *name, uint64_t T, const uint64_t *time,
+//                        const uint64_t *frame_sizes, const uint64_t *reaction_id, const int32_t *type,
+//                        const uint64_t *educts, const uint64_t *products, const double *position,
+//                        const uint16_t *types_from);          educts, products, types_from [R][2]; position [R][3]
 //   int rdy_h5_particle_types(int64_t f, uint32_t n, const char *const *names, const double *diffusion,
 //                             const char *const *flavors);             type ids 0 .. n-1
 //   int rdy_h5_registered_reactions(int64_t f, uint32_t n, const char *const *names, const uint64_t *ids,
@@ -43,6 +50,10 @@
 //   int rdy_h5_read_f64(const char *path, const char *dataset, uint64_t cap, double *out, uint64_t *n);
 //   int rdy_h5_read_records(const char *path, const char *dataset, uint64_t cap, uint64_t *ids,
 //                           uint16_t *types, uint8_t *flavors, double *pos, uint64_t *n);
+//   int rdy_h5_read_reactions(const char *path, const char *dataset, uint64_t cap_frames, uint64_t *frame_sizes,
+//                             uint64_t cap, uint64_t *reaction_id, int32_t *type, uint64_t *educts,
+//                             uint64_t *products, double *position, uint16_t *types_from, uint64_t *n_frames,
+//                             uint64_t *n);   *n_frames, *n = frames and records in the file
 //   int rdy_h5_read_string(const char *path, const char *dataset, uint64_t cap, char *out);
 //       a scalar string dataset into out (NUL-terminated; -1 when it needs more than cap bytes)
 #include <hdf5.h>
@@ -122,6 +133,34 @@ int write_compound(hid_t g, const char *name, hid_t type, hsize_t n, const void 
     if (ds >= 0) H5Dclose(ds);
     H5Sclose(space);
     return rc;
+}
+
+// one reaction record in memory (the fields the reference's ReactionRecord type writes, Types.cpp:66-77)
+struct RecordH5 {
+    uint64_t reaction_id;
+    int32_t reaction_type;
+    uint64_t educts[2];
+    uint64_t products[2];
+    double position[3];
+    uint16_t types_from[2];
+};
+
+hid_t reaction_record_type() {
+    const hsize_t two[1] = {2}, three[1] = {3};
+    hid_t u64x2 = H5Tarray_create2(H5T_NATIVE_UINT64, 1, two);
+    hid_t f64x3 = H5Tarray_create2(H5T_NATIVE_DOUBLE, 1, three);
+    hid_t u16x2 = H5Tarray_create2(H5T_NATIVE_UINT16, 1, two);
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(RecordH5));
+    H5Tinsert(t, "reaction_id", HOFFSET(RecordH5, reaction_id), H5T_NATIVE_UINT64);
+    H5Tinsert(t, "reaction_type", HOFFSET(RecordH5, reaction_type), H5T_NATIVE_INT32);
+    H5Tinsert(t, "educts", HOFFSET(RecordH5, educts), u64x2);
+    H5Tinsert(t, "products", HOFFSET(RecordH5, products), u64x2);
+    H5Tinsert(t, "position", HOFFSET(RecordH5, position), f64x3);
+    H5Tinsert(t, "types_from", HOFFSET(RecordH5, types_from), u16x2);
+    H5Tclose(u64x2);
+    H5Tclose(f64x3);
+    H5Tclose(u16x2);
+    return t;
 }
 
 struct TypeInfo {  // ParticleTypeInfo (IOUtils.h:71-76)
@@ -250,6 +289,47 @@ int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64
     H5Tclose(file);
     hsize_t dl[2] = {T, 2}, dt[1] = {T};
     if (rc == 0) rc = write_u64(g, "limits", 2, dl, limits.data());
+    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_reactions(int64_t f, const char *name, uint64_t T, const uint64_t *time, const uint64_t *frame_sizes,
+                     const uint64_t *reaction_id, const int32_t *type, const uint64_t *educts,
+                     const uint64_t *products, const double *position, const uint16_t *types_from) {
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    uint64_t total = 0;
+    for (uint64_t t = 0; t < T; ++t) total += frame_sizes[t];
+    std::vector<RecordH5> rec(total);
+    for (uint64_t k = 0; k < total; ++k) {
+        RecordH5 &x = rec[k];
+        x.reaction_id = reaction_id[k];
+        x.reaction_type = type[k];
+        for (int j = 0; j < 2; ++j) {
+            x.educts[j] = educts[2 * k + j];
+            x.products[j] = products[2 * k + j];
+            x.types_from[j] = types_from[2 * k + j];
+        }
+        for (int d = 0; d < 3; ++d) x.position[d] = position[3 * k + d];
+    }
+    std::vector<hvl_t> frames(T);
+    uint64_t at = 0;
+    for (uint64_t t = 0; t < T; ++t) {
+        frames[t].len = frame_sizes[t];
+        frames[t].p = frame_sizes[t] ? (void *)(rec.data() + at) : nullptr;
+        at += frame_sizes[t];
+    }
+    hid_t ct = reaction_record_type();
+    hid_t vt = H5Tvlen_create(ct);
+    hsize_t dt[1] = {T};
+    hid_t space = H5Screate_simple(1, dt, nullptr);
+    hid_t ds = H5Dcreate2(g, "records", vt, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    int rc = ds < 0 ? -1 : (H5Dwrite(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) < 0 ? -1 : 0);
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(space);
+    H5Tclose(vt);
+    H5Tclose(ct);
     if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
@@ -399,6 +479,53 @@ int rdy_h5_read_records(const char *path, const char *dataset, uint64_t cap, uin
                 for (int a = 0; a < 3; ++a) pos[3 * k + a] = rec[k].pos[a];
             }
         }
+        H5Sclose(space);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int rdy_h5_read_reactions(const char *path, const char *dataset, uint64_t cap_frames, uint64_t *frame_sizes,
+                          uint64_t cap, uint64_t *reaction_id, int32_t *type, uint64_t *educts, uint64_t *products,
+                          double *position, uint16_t *types_from, uint64_t *n_frames, uint64_t *n) {
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = -1;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t space = H5Dget_space(ds);
+        const hssize_t T = H5Sget_simple_extent_npoints(space);
+        hid_t ct = reaction_record_type();
+        hid_t vt = H5Tvlen_create(ct);
+        std::vector<hvl_t> frames(T > 0 ? (size_t)T : 0);
+        if (T >= 0 && H5Dread(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) >= 0) {
+            uint64_t total = 0;
+            for (const hvl_t &v : frames) total += v.len;
+            *n_frames = (uint64_t)T;
+            *n = total;
+            if ((uint64_t)T <= cap_frames && total <= cap) {
+                uint64_t k = 0;
+                for (size_t t = 0; t < frames.size(); ++t) {
+                    frame_sizes[t] = frames[t].len;
+                    const RecordH5 *x = (const RecordH5 *)frames[t].p;
+                    for (size_t i = 0; i < frames[t].len; ++i, ++k) {
+                        reaction_id[k] = x[i].reaction_id;
+                        type[k] = x[i].reaction_type;
+                        for (int j = 0; j < 2; ++j) {
+                            educts[2 * k + j] = x[i].educts[j];
+                            products[2 * k + j] = x[i].products[j];
+                            types_from[2 * k + j] = x[i].types_from[j];
+                        }
+                        for (int d = 0; d < 3; ++d) position[3 * k + d] = x[i].position[d];
+                    }
+                }
+                rc = 0;
+            }
+            H5Dvlen_reclaim(vt, space, H5P_DEFAULT, frames.data());
+        }
+        H5Tclose(vt);
+        H5Tclose(ct);
         H5Sclose(space);
         H5Dclose(ds);
     }

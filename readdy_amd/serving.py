@@ -8,7 +8,9 @@ own step budget and handler.  Between intervals the server samples observables f
 once (particles per type, reaction totals), retires finished requests (final state read back, slot
 freed) and admits new ones.  Observables go to HDF5 files in ReaDDy's layout on a writer thread,
 off the critical path; a server built with record_trajectory also reads every replica's particles
-back at each interval and writes them as a FlatTrajectory (one frame per interval).
+back at each interval and writes them as a FlatTrajectory (one frame per interval), and one built
+with record_reactions writes the reaction records of each interval's last step (the Reactions
+observable with the interval as its stride).
 
 Every request's trajectory depends only on its own seed and steps (replica-local handles and
 Philox counters, fixed-point force sums), so a result is the same alone and in any mix.
@@ -77,6 +79,7 @@ class Result:
     observed_counts: list = field(default_factory=list)
     observed_reactions: list = field(default_factory=list)
     observed_frames: list = field(default_factory=list)  # (handles, types, positions) per interval
+    observed_records: list = field(default_factory=list)  # reaction records of each interval's last step
 
     @property
     def latency(self):
@@ -130,6 +133,8 @@ class H5Writer:
                     t = res.observed_steps
                     f.n_particles("n_particles", t, res.observed_counts)
                     f.reaction_counts("reaction_counts", t, res.observed_reactions)
+                    if res.observed_records:
+                        f.reactions("reactions", t, res.observed_records)
                     if res.observed_frames:
                         # every particle of these requests is a normal one (flavor 0)
                         f.flat_trajectory("trajectory", t, [(h, ty, np.zeros(len(h), np.uint8), x)
@@ -151,7 +156,7 @@ class Server:
     """
 
     def __init__(self, hbm_budget_bytes, max_slots=512, interval_steps=100, nl_cutoff=2.0, out_dir=None,
-                 max_particles_per_request=200000, record_trajectory=False):
+                 max_particles_per_request=200000, record_trajectory=False, record_reactions=False):
         pots, rxs = chemistry()
         self.interval = int(interval_steps)
         self.budget = int(hbm_budget_bytes)
@@ -165,6 +170,7 @@ class Server:
         self.max_slots = max_slots
         self.writer = H5Writer(out_dir) if out_dir else None
         self.record_trajectory = bool(record_trajectory)
+        self.record_reactions = bool(record_reactions)
         self.n_reactions = len(rxs)
 
     def footprint(self, req):
@@ -200,6 +206,8 @@ class Server:
                 eng.set_particles(rid, pos, types)
                 eng.set_handler(rid, HANDLERS[req.handler])
                 eng.set_steps(rid, req.steps)
+                if self.record_reactions:
+                    eng.record_reactions(True)
                 used += fp
                 res = Result(req, admitted=now)
                 results[req.rid] = res
@@ -237,6 +245,8 @@ class Server:
                 res.observed_reactions.append(eng.reaction_totals(r).copy())
                 if self.record_trajectory:
                     res.observed_frames.append(eng.get_particles(r))
+                if self.record_reactions:
+                    res.observed_records.append(eng.reaction_records(r))
                 if done >= req.steps:
                     if eng.steps_done(r) != req.steps:
                         raise RuntimeError(f"request {req.rid}: {eng.steps_done(r)} steps done, {req.steps} asked")

@@ -110,3 +110,31 @@ def test_config_group(tmp_path):
     flat = " ".join(out.split())
     assert '"fus", 0, 2, 1, 0.1, 1, 0, [ 0, 1 ], [ 2, 0 ]' in flat, flat
     assert '"fis", 1, 1, 2, 0.05, 0, 1.5, [ 2, 0 ], [ 0, 1 ]' in flat, flat
+
+
+def test_reactions_observable_round_trip(tmp_path):
+    """Reactions.cpp:72-86: one variable-length list of records per time point (empty lists too)."""
+    rng = np.random.default_rng(6)
+    frames = []
+    for n in (0, 4, 1, 0, 7):
+        f = np.zeros(n, h5io.REACTION_DTYPE)
+        f["reaction"] = rng.integers(0, 5, n)
+        f["type"] = rng.integers(0, 5, n)
+        f["educts"] = rng.integers(0, 1 << 40, (n, 2))
+        f["products"] = rng.integers(0, 1 << 40, (n, 2))
+        f["types_from"] = rng.integers(0, 9, (n, 2))
+        f["position"] = rng.normal(size=(n, 3))
+        frames.append(f)
+    path = str(tmp_path / "rx.h5")
+    with h5io.File(path) as fh:
+        fh.reactions("reactions", [0, 1, 2, 3, 4], frames)
+    t, got = h5io.read_reactions(path, "reactions")
+    assert list(t) == [0, 1, 2, 3, 4] and [len(g) for g in got] == [len(f) for f in frames]
+    for f, g in zip(frames, got):
+        assert f.tobytes() == g.tobytes()
+    exe = _h5dump()
+    if exe:
+        out = subprocess.run([exe, "-H", path], capture_output=True, text=True, check=True).stdout
+        for needle in ('DATASET "records"', 'H5T_VLEN', 'H5T_STD_U64LE "reaction_id"', 'H5T_STD_I32LE "reaction_type"',
+                       '"educts"', '"products"', '"position"', '"types_from"'):
+            assert needle in out, needle

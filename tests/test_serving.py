@@ -66,7 +66,7 @@ def test_served_trajectory_frames(tmp_path):
     then, and the last frame is the request's final state (handles as ids, flavor 0)."""
     trace = S.poisson_trace(3, rate_per_s=400.0, n_range=(500, 2000), steps_range=(30, 120), seed=12)
     srv = S.Server(hbm_budget_bytes=128 << 20, max_slots=2, interval_steps=20, out_dir=str(tmp_path),
-                   record_trajectory=True)
+                   record_trajectory=True, record_reactions=True)
     results, _ = srv.run(trace, keep_final=True)
     srv.close()
     for req in trace:
@@ -80,6 +80,20 @@ def test_served_trajectory_frames(tmp_path):
             assert not fl.any()
         xf, tf, hf = res.final
         ids, ty, _, x = frames[-1]
+        # the records of each interval's last step against the frame read right after that step:
+        # fusion A+B->C and fission C->A+B products are live with their types, the educts are gone
+        tr, recs = h5io.read_reactions(path, "reactions")
+        assert np.array_equal(tr, t) and sum(len(x) for x in recs) > 0
+        for (fids, fty, _, _), rec in zip(frames, recs):
+            live = dict(zip(fids.tolist(), fty.tolist()))
+            for x in rec:
+                ed, pr = [int(v) for v in x["educts"]], [int(v) for v in x["products"]]
+                assert all(v not in live for v in ed)
+                if x["type"] == 1:
+                    assert x["reaction"] == 0 and live[pr[0]] == 2 and sorted(x["types_from"]) == [0, 1]
+                else:
+                    assert x["type"] == 2 and x["reaction"] == 1 and list(x["types_from"]) == [2, 2]
+                    assert live[pr[0]] == 0 and live[pr[1]] == 1
         general = json.loads(h5io.read_string(path, "/readdy/config/general"))
         assert general["box_size"] == [req.box] * 3 and general["pbc"] == [True] * 3
         assert np.array_equal(ids, hf) and np.array_equal(ty, tf) and np.array_equal(x, xf)
