@@ -63,12 +63,14 @@ def test_served_results_equal_results_alone(tmp_path):
 def test_served_trajectory_frames(tmp_path):
     """With record_trajectory, request_<rid>.h5 holds one FlatTrajectory frame per observed interval:
     the frame times are the observed step counts, each frame's size is the particle count observed
-    then, and the last frame is the request's final state (handles as ids, flavor 0)."""
-    trace = S.poisson_trace(3, rate_per_s=400.0, n_range=(500, 2000), steps_range=(30, 120), seed=12)
-    srv = S.Server(hbm_budget_bytes=128 << 20, max_slots=2, interval_steps=20, out_dir=str(tmp_path),
+    then, and the last frame is the request's final state (handles as ids, flavor 0).  With
+    record_reactions, the Reactions observable holds the records of each interval's last step."""
+    trace = S.poisson_trace(3, rate_per_s=400.0, n_range=(3000, 6000), steps_range=(30, 120), seed=12)
+    srv = S.Server(hbm_budget_bytes=256 << 20, max_slots=2, interval_steps=5, out_dir=str(tmp_path),
                    record_trajectory=True, record_reactions=True)
     results, _ = srv.run(trace, keep_final=True)
     srv.close()
+    n_records = 0
     for req in trace:
         res = results[req.rid]
         path = os.path.join(str(tmp_path), f"request_{req.rid}.h5")
@@ -83,20 +85,22 @@ def test_served_trajectory_frames(tmp_path):
         # the records of each interval's last step against the frame read right after that step:
         # fusion A+B->C and fission C->A+B products are live with their types, the educts are gone
         tr, recs = h5io.read_reactions(path, "reactions")
-        assert np.array_equal(tr, t) and sum(len(x) for x in recs) > 0
+        assert np.array_equal(tr, t)
+        n_records += sum(len(f) for f in recs)
         for (fids, fty, _, _), rec in zip(frames, recs):
             live = dict(zip(fids.tolist(), fty.tolist()))
-            for x in rec:
-                ed, pr = [int(v) for v in x["educts"]], [int(v) for v in x["products"]]
+            for one in rec:
+                ed, pr = [int(v) for v in one["educts"]], [int(v) for v in one["products"]]
                 assert all(v not in live for v in ed)
-                if x["type"] == 1:
-                    assert x["reaction"] == 0 and live[pr[0]] == 2 and sorted(x["types_from"]) == [0, 1]
+                if one["type"] == 1:
+                    assert one["reaction"] == 0 and live[pr[0]] == 2 and sorted(one["types_from"]) == [0, 1]
                 else:
-                    assert x["type"] == 2 and x["reaction"] == 1 and list(x["types_from"]) == [2, 2]
+                    assert one["type"] == 2 and one["reaction"] == 1 and list(one["types_from"]) == [2, 2]
                     assert live[pr[0]] == 0 and live[pr[1]] == 1
         general = json.loads(h5io.read_string(path, "/readdy/config/general"))
         assert general["box_size"] == [req.box] * 3 and general["pbc"] == [True] * 3
         assert np.array_equal(ids, hf) and np.array_equal(ty, tf) and np.array_equal(x, xf)
+    assert n_records > 5  # about 0.5 events per step at these sizes, sampled every fifth step
 
 
 def test_writer_thread_files(tmp_path):
