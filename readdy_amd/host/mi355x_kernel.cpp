@@ -56,6 +56,7 @@ public:
     }
     void configureTopologies() override;
     std::vector<uint64_t> reactionCounts() const override;
+    std::vector<ReactionRecord> reactionRecords() const override;
     void clear() override;
 
     // uploads the pending particles (topology particles first) if the device copy is stale
@@ -90,8 +91,18 @@ public:
         const int h = scheduler == "Gillespie" ? RDY_HANDLER_GILLESPIE
                       : scheduler == "UncontrolledApproximation" ? RDY_HANDLER_UNCONTROLLED_APPROXIMATION
                                                                  : RDY_HANDLER_NONE;
+        syncRecording();
         ok(rdy_run_graph(_engine, h, dt, steps), "run");
         return true;
+    }
+
+    // switches the engine's reaction records on once the context asks for them (after the first
+    // upload: the engine keeps them per replica)
+    void syncRecording() {
+        if (_hasReplica && context().recordReactionsWithPositions() != _recording) {
+            ok(rdy_record_reactions(_engine, context().recordReactionsWithPositions() ? 1 : 0), "record reactions");
+            _recording = context().recordReactionsWithPositions();
+        }
     }
 
     // the engine and its single replica, created from the context on first use
@@ -201,6 +212,7 @@ private:
     rdy_engine *_engine = nullptr;
     uint32_t _replica = 0;
     bool _hasReplica = false;
+    bool _recording = false;
     uint64_t _seed = 0;
     scalar _nlCutoff = 0;  // 0: Context::calculateMaxCutoff
 
@@ -389,6 +401,29 @@ std::vector<uint64_t> MI355XStateModel::reactionCounts() const {
     return c;
 }
 
+std::vector<ReactionRecord> MI355XStateModel::reactionRecords() const {
+    std::vector<ReactionRecord> out;
+    if (!_k->hasReplica() || !_k->_recording) return out;
+    rdy_engine *e = const_cast<MI355XKernel *>(_k)->engine();
+    uint64_t n = 0;
+    ok(rdy_get_reaction_records(e, _k->replicaId(), 0, nullptr, &n), "reaction records");
+    std::vector<rdy_reaction_record> r(n);
+    ok(rdy_get_reaction_records(e, _k->replicaId(), n, r.data(), &n), "reaction records");
+    for (const auto &x : r) {
+        ReactionRecord o;
+        o.type = x.type;
+        o.id = x.reaction;
+        for (int j = 0; j < 2; ++j) {
+            o.educts[j] = x.educts[j];
+            o.products[j] = x.products[j];
+            o.types_from[j] = (ParticleTypeId)x.types_from[j];
+        }
+        o.where = Vec3(x.position[0], x.position[1], x.position[2]);
+        out.push_back(o);
+    }
+    return out;
+}
+
 void MI355XStateModel::clear() {
     if (_k->hasReplica()) ok(rdy_set_particles(_k->engine(), _k->replicaId(), 0, nullptr, nullptr), "clear");
     _topoParticles.clear();
@@ -472,6 +507,7 @@ std::unique_ptr<Action> MI355XKernel::Actions::evaluateCompartments() {
 std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::uncontrolledApproximation(scalar dt) {
     return step_action(dt, [k = _k](scalar h) {
         k->_state.flush();
+        k->syncRecording();
         ok(rdy_react(k->engine(), RDY_HANDLER_UNCONTROLLED_APPROXIMATION, h), "uncontrolled approximation");
     });
 }
@@ -479,6 +515,7 @@ std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::uncontrolledAppr
 std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::gillespie(scalar dt) {
     return step_action(dt, [k = _k](scalar h) {
         k->_state.flush();
+        k->syncRecording();
         ok(rdy_react(k->engine(), RDY_HANDLER_GILLESPIE, h), "gillespie");
     });
 }

@@ -164,7 +164,7 @@ Context::Context() = default;
 
 Context::Context(const Context &o)
     : _types(o._types), _potentials(o._potentials), _reactions(o._reactions), _topologies(o._topologies),
-      _box(o._box), _pbc(o._pbc), _kbt(o._kbt) {
+      _box(o._box), _pbc(o._pbc), _kbt(o._kbt), _recordReactions(o._recordReactions) {
     rebind();
 }
 
@@ -177,6 +177,7 @@ Context &Context::operator=(const Context &o) {
     _box = o._box;
     _pbc = o._pbc;
     _kbt = o._kbt;
+    _recordReactions = o._recordReactions;
     rebind();
     return *this;
 }

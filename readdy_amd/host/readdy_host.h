@@ -108,6 +108,17 @@ struct ReactionDesc {
     scalar rate = 0, eductDistance = 0, productDistance = 0, weight1 = .5, weight2 = .5;
 };
 
+// one performed reaction (ReactionRecord.h:54-63): type as ReactionType, educts' ids and types before
+// the reaction, products' ids after it, where = (pos1 + pos2) / 2 in the box, id = reaction index
+struct ReactionRecord {
+    int type = 0;
+    std::array<ParticleId, 2> educts{{0, 0}};
+    std::array<ParticleId, 2> products{{0, 0}};
+    std::array<ParticleTypeId, 2> types_from{{0, 0}};
+    Vec3 where;
+    std::size_t id = 0;
+};
+
 class ReactionRegistry {
 public:
     explicit ReactionRegistry(const ParticleTypeRegistry *types) : _types(types) {}
@@ -174,6 +185,9 @@ public:
     const std::array<bool, 3> &periodicBoundaryConditions() const { return _pbc; }
     scalar &kBT() { return _kbt; }
     scalar kBT() const { return _kbt; }
+    // Context::recordReactionsWithPositions (Context.h): keep each reaction step's records
+    bool &recordReactionsWithPositions() { return _recordReactions; }
+    bool recordReactionsWithPositions() const { return _recordReactions; }
     // Context::calculateMaxCutoff (Context.cpp:126-144): pair potentials and order-2 educt distances
     scalar calculateMaxCutoff() const;
 
@@ -186,6 +200,7 @@ private:
     std::array<scalar, 3> _box{{1, 1, 1}};
     std::array<bool, 3> _pbc{{true, true, true}};
     scalar _kbt = 1;
+    bool _recordReactions = false;
 };
 
 class Particle {  // Particle.h:60-120
@@ -276,6 +291,8 @@ public:
     virtual std::vector<GraphTopology *> getTopologies() = 0;
     virtual void configureTopologies() = 0;  // GraphTopology::configure for all topologies
     virtual std::vector<uint64_t> reactionCounts() const = 0;  // of the last reaction handler call
+    // of the last reaction handler call, when the context records them (empty otherwise)
+    virtual std::vector<ReactionRecord> reactionRecords() const = 0;
     virtual void clear() = 0;
 };
 

@@ -179,6 +179,9 @@ static void test_c0_forces_and_loop(const std::string &) {
     // (same Philox counters, UncontrolledApproximation with keyed order)
     const std::uint64_t seed = 0;  // the kernel's Philox key (READDY_AMD_SEED unset)
     or_set_seed(o, seed);
+    k->context().recordReactionsWithPositions() = true;  // records compared step by step below
+    or_set_record_reactions(o, 1);
+    uint64_t records_checked = 0;
     api::SimulationLoop loop(k.get(), 0.01);
     loop.useReactionScheduler("UncontrolledApproximation");
     int step_checked = 0;
@@ -208,12 +211,39 @@ static void test_c0_forces_and_loop(const std::string &) {
                   (unsigned long long)counts[i], (unsigned long long)oc[i]);
             events += counts[i];
         }
+        // the step's reaction records: the same events with the same ids, types and positions
+        auto recs = k->stateModel().reactionRecords();
+        size_t no = 0;
+        const or_reaction_record *orr = or_reaction_records(o, &no);
+        std::vector<or_reaction_record> want(orr, orr + no);
+        auto key = [](uint64_t a, uint64_t b) { return std::make_pair(a, b); };
+        std::sort(recs.begin(), recs.end(), [&](const model::ReactionRecord &a, const model::ReactionRecord &b) {
+            return key(a.educts[0], a.educts[1]) < key(b.educts[0], b.educts[1]);
+        });
+        std::sort(want.begin(), want.end(), [&](const or_reaction_record &a, const or_reaction_record &b) {
+            return key(a.educts[0], a.educts[1]) < key(b.educts[0], b.educts[1]);
+        });
+        CHECK(recs.size() == want.size(), "step %zu: %zu records vs %zu", t, recs.size(), want.size());
+        for (size_t i = 0; i < recs.size() && i < want.size(); ++i) {
+            const auto &g = recs[i];
+            const auto &w = want[i];
+            bool same = g.type == w.type && g.id == w.reaction;
+            for (int j = 0; j < 2; ++j)
+                same = same && g.educts[j] == w.educts[j] && g.products[j] == w.products[j] &&
+                       g.types_from[j] == w.types_from[j];
+            for (int a = 0; a < 3; ++a) same = same && std::fabs(wrapd(g.where[a] - w.where[a], L)) < 1e-6;
+            CHECK(same, "step %zu: record %zu differs (educts %llu %llu)", t, i, (unsigned long long)w.educts[0],
+                  (unsigned long long)w.educts[1]);
+            ++records_checked;
+        }
         ++step_checked;
     });
     loop.run(5);
     CHECK(step_checked == 5, "callback steps %d", step_checked);
     CHECK(worst < 1e-6, "largest position difference %.3g", worst);
     CHECK(events > 20, "too few reaction events (%llu) to exercise the handler", (unsigned long long)events);
+    CHECK(records_checked == events, "records %llu vs events %llu", (unsigned long long)records_checked,
+          (unsigned long long)events);
     std::printf("c0 loop: 5 steps, %llu events, worst position difference %.3g\n", (unsigned long long)events, worst);
     or_destroy(o);
 
