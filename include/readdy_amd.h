@@ -175,6 +175,22 @@ int rdy_record_reactions(rdy_engine *e, int on);
 int rdy_get_reaction_records(rdy_engine *e, uint32_t replica, uint64_t cap, rdy_reaction_record *out,
                              uint64_t *n);
 
+/* BreakBonds (include/readdy/model/actions/Actions.h:218-341) with its BreakConfig (Actions.h:190-211):
+ * a bond edge (the consecutive bond terms of one replica on one handle pair, i.e. all bond
+ * configurations of a topology edge) whose end types form a listed unordered pair and whose summed
+ * harmonic energy exceeds the pair's threshold breaks with probability 1 - exp(-rate dt) (Philox
+ * uniform per edge and call).  Writes the broken edges as (replica, h1, h2) rows sorted ascending, at
+ * most cap of them; *n_broken = their number.  The bonded terms are left as they are: the caller
+ * removes the edges from its graphs and sets the replica's bonds, angles and dihedrals again
+ * (GraphTopology::configure), as the reference's structural reaction does.  rate <= 0 is an error
+ * (BreakConfig::addBreakablePair). */
+typedef struct {
+    uint32_t type1, type2;
+    double threshold_energy, rate;
+} rdy_break_pair;
+int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_pair *pairs, uint64_t cap,
+                    uint32_t *broken, uint64_t *n_broken);
+
 /* ---- the simulation loop: replaces api::SimulationLoop::run's body
  *      (include/readdy/api/SimulationLoop.h:256-306): per step integrate, neighbour list,
  *      reactions, neighbour list, forces -- fused into shared launches over all replicas.

@@ -88,6 +88,7 @@ SIGNATURES = {
     "rdy_get_reaction_counts": (_int, [_vp, _u32, _u64p]),
     "rdy_get_reaction_totals": (_int, [_vp, _u32, _u64p]),
     "rdy_record_reactions": (_int, [_vp, _int]),
+    "rdy_break_bonds": (_int, [_vp, ctypes.c_double, _u32, _vp, _u64, _u32p, _u64p]),
     "rdy_get_reaction_records": (_int, [_vp, _u32, _u64, _vp, _u64p]),
     "rdy_run": (_int, [_vp, _int, _d, _u64]),
     "rdy_run_graph": (_int, [_vp, _int, _d, _u64]),

@@ -5,6 +5,7 @@
 #include "../../include/readdy_amd.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -98,6 +99,7 @@ struct rdy_engine {
     RxRecord *recs = nullptr;  // reaction records of each replica's last reaction step (when recording)
     uint32_t *rec_count = nullptr;
     bool recording = false;
+    uint32_t break_calls = 0;  // Philox counter of rdy_break_bonds
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
     uint32_t *err = nullptr;
     double *force_out = nullptr;
@@ -1555,6 +1557,58 @@ int rdy_reaction_candidates(rdy_engine *e, uint32_t r, uint64_t cap, uint32_t *r
         HIP_TRY(hipMemcpyAsync(rows, out, m * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     }
+    return check_errors(e);
+}
+
+int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_pair *pairs, uint64_t cap,
+                    uint32_t *broken, uint64_t *n_broken) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (!n_broken || (n_pairs && !pairs) || (cap && !broken)) return set_err(RDY_ERR_INVALID, "null argument");
+    if (!(dt > 0.0)) return set_err(RDY_ERR_INVALID, "break bonds: time step must be positive");
+    *n_broken = 0;
+    const uint32_t nt = e->n_types;
+    // an unordered type pair -> (threshold, rate); the first entry of a pair wins, as map::emplace
+    std::vector<double> tab(2ull * nt * nt, 0.0);
+    for (uint32_t k = 0; k < n_pairs; ++k) {
+        const rdy_break_pair &q = pairs[k];
+        if (q.type1 >= nt || q.type2 >= nt) return set_err(RDY_ERR_INVALID, "break bonds: no such particle type");
+        if (!(q.rate > 0.0)) return set_err(RDY_ERR_INVALID, "Rate of breaking a bond must be positive");
+        for (const auto ab : {std::make_pair(q.type1, q.type2), std::make_pair(q.type2, q.type1)}) {
+            double *c = &tab[2ull * (ab.first * nt + ab.second)];
+            if (c[1] > 0.0) continue;
+            c[0] = q.threshold_energy;
+            c[1] = q.rate;
+        }
+    }
+    const uint32_t n = (uint32_t)e->terms_h[0].size();
+    const uint32_t call = e->break_calls++;
+    if (n == 0 || !e->d_terms[0]) return RDY_OK;
+    const size_t off_c = (8 * tab.size() + 15) & ~(size_t)15, off_o = off_c + 16;
+    const uint32_t ocap = (uint32_t)std::min<uint64_t>(n, (uint64_t)UINT32_MAX / 3);
+    int rc = ensure_scratch(e, off_o + 12ull * ocap);
+    if (rc) return rc;
+    char *base = (char *)e->scratch;
+    HIP_TRY(hipMemcpyAsync(base, tab.data(), 8 * tab.size(), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(base + off_c, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_break_bonds, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, e->grid(e->cur),
+                       (const ReplicaDev *)e->reps, e->topo(), (const BondTerm *)e->d_terms[0], n,
+                       (const double2 *)base, nt, dt, call, (uint32_t *)(base + off_o), ocap,
+                       (uint32_t *)(base + off_c));
+    uint32_t got = 0;
+    HIP_TRY(hipMemcpyAsync(&got, base + off_c, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    got = std::min(got, ocap);  // at most one entry per edge, and edges <= terms
+    std::vector<uint32_t> h(3ull * got);
+    if (got) {
+        HIP_TRY(hipMemcpyAsync(h.data(), base + off_o, 12ull * got, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    std::vector<std::array<uint32_t, 3>> v(got);
+    for (uint32_t k = 0; k < got; ++k) v[k] = {h[3 * k], h[3 * k + 1], h[3 * k + 2]};
+    std::sort(v.begin(), v.end());
+    for (uint64_t k = 0; k < v.size() && k < cap; ++k)
+        for (int j = 0; j < 3; ++j) broken[3 * k + j] = v[k][j];
+    *n_broken = v.size();
     return check_errors(e);
 }
 

@@ -413,6 +413,48 @@ __global__ void k_bonds(Grid cur, ReplicaDev *reps, Topo topo, const BondTerm *t
     atomicAdd(&e_bonded[t.r], (unsigned long long)to_fixed_d(e, ENERGY_SCALE));
 }
 
+// BreakBonds (Actions.h:235-290): one thread per bond term.  The first term of each run of terms on
+// the same edge (same replica and handle pair, consecutive as the configuration lists an edge's bond
+// configurations) sums the run's harmonic energies (evaluateEdgeEnergy, Actions.h:291-341); when the
+// end types form a breakable pair (brk[t1 * nt + t2] = {threshold, rate}, rate > 0) and the energy
+// exceeds the threshold, the edge breaks with probability 1 - exp(-rate dt): a Philox uniform keyed by
+// the edge's handles and the call counter.  Broken edges are listed as (replica, h1, h2).
+__global__ void k_break_bonds(Grid cur, const ReplicaDev *reps, Topo topo, const BondTerm *terms, uint32_t n,
+                              const double2 *brk, uint32_t nt, double dt, uint32_t call, uint32_t *out,
+                              uint32_t cap, uint32_t *count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BondTerm t = terms[i];
+    auto same_edge = [&](const BondTerm &u) { return u.r == t.r && u.h[0] == t.h[0] && u.h[1] == t.h[1]; };
+    if (i > 0 && same_edge(terms[i - 1])) return;
+    const ReplicaDev &R = reps[t.r];
+    if (!R.in_use) return;
+    const State s0 = cur.bins[topo.loc[R.topo_base + t.h[0]]];
+    const State s1 = cur.bins[topo.loc[R.topo_base + t.h[1]]];
+    const double2 cfg = brk[tw_type(s0.tw) * nt + tw_type(s1.tw)];
+    if (!(cfg.y > 0.0)) return;
+    const double a[3] = {s0.x, s0.y, s0.z}, b[3] = {s1.x, s1.y, s1.z};
+    double x[3];
+    sdiff(a, b, R, x);
+    const double norm = sqrt(dot(x, x));
+    double energy = 0.0;
+    for (uint32_t j = i; j < n; ++j) {
+        const BondTerm u = terms[j];
+        if (!same_edge(u)) break;
+        energy += u.k * (norm - u.p0) * (norm - u.p0);
+    }
+    if (!(energy > cfg.x)) return;
+    uint32_t x4[4];
+    philox(t.h[0], call, 5u, t.h[1], R.seed, x4);
+    if (!(u53(x4[0], x4[1]) < 1.0 - exp(-cfg.y * dt))) return;
+    const uint32_t p = atomicAdd(count, 1u);
+    if (p < cap) {
+        out[3ull * p] = t.r;
+        out[3ull * p + 1] = t.h[0];
+        out[3ull * p + 2] = t.h[1];
+    }
+}
+
 __global__ void k_angles(Grid cur, ReplicaDev *reps, Topo topo, const BondTerm *terms, uint32_t n,
                          unsigned long long *e_bonded) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

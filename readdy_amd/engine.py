@@ -164,6 +164,28 @@ class Engine:
         k = np.ascontiguousarray(np.broadcast_to(k, (len(idx),)), dtype=np.float64)
         length = np.ascontiguousarray(np.broadcast_to(length, (len(idx),)), dtype=np.float64)
         L.check(self._lib.rdy_set_bonds(self._h, r, len(idx), _ptr(idx, _u32p), _ptr(k, _dp), _ptr(length, _dp)))
+        self._bond_terms = getattr(self, "_bond_terms", {})
+        self._bond_terms[r] = len(idx)
+
+    # rdy_break_pair (include/readdy_amd.h)
+    BREAK_PAIR_DTYPE = np.dtype([("type1", np.uint32), ("type2", np.uint32), ("threshold_energy", np.float64),
+                                 ("rate", np.float64)], align=True)
+
+    def break_bonds(self, dt, pairs):
+        """BreakBonds' device pass: pairs = [(type1, type2, threshold energy, rate)]; returns the broken
+        edges as rows (replica, h1, h2), sorted.  The bonded terms stay as set: the caller sets them again
+        without the broken edges (and the angles and dihedrals across them)."""
+        p = np.zeros(len(pairs), dtype=self.BREAK_PAIR_DTYPE)
+        for i, (a, b, thr, rate) in enumerate(pairs):
+            p[i] = (a, b, thr, rate)
+        cap = max(1, sum(getattr(self, "_bond_terms", {}).values()))  # an edge breaks once; edges <= terms
+        out = np.zeros((cap, 3), dtype=np.uint32)
+        n = ctypes.c_uint64()
+        L.check(self._lib.rdy_break_bonds(self._h, float(dt), len(p), p.ctypes.data_as(ctypes.c_void_p), cap,
+                                          _ptr(out, _u32p), ctypes.byref(n)))
+        if n.value > cap:
+            raise RuntimeError("more broken bonds than the output holds")
+        return out[:n.value]
 
     def set_angles(self, r, idx, k, theta0):
         idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1, 3)

@@ -9,6 +9,7 @@
 // (Particle::nextId, Particle.h:112-114).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -61,6 +62,9 @@ public:
 
     // uploads the pending particles (topology particles first) if the device copy is stale
     void flush();
+    // BreakBonds: the device picks the edges that break; their topologies lose the edges, split into
+    // their connected components and the bonded terms are configured again
+    void breakBonds(scalar dt, const BreakConfig &config);
 
 private:
     MI355XKernel *_k;
@@ -200,6 +204,7 @@ public:
         std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) override;
         std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) override;
         std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) override;
+        std::unique_ptr<TimeStepDependentAction> breakBonds(scalar dt, BreakConfig config) override;
         std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) override;
         std::unique_ptr<Action> initializeKernel() override;
 
@@ -424,6 +429,55 @@ std::vector<ReactionRecord> MI355XStateModel::reactionRecords() const {
     return out;
 }
 
+void MI355XStateModel::breakBonds(scalar dt, const BreakConfig &config) {
+    flush();
+    if (!_k->hasReplica() || _topologies.empty()) return;
+    if (_topoDirty) configureTopologies();
+    std::vector<rdy_break_pair> pairs;
+    for (const auto &p : config.pairs()) pairs.push_back({p.type1, p.type2, p.thresholdEnergy, p.rate});
+    // every edge breaks at most once: room for all of them (a second call would draw anew)
+    std::size_t n_edges = 0;
+    for (const auto &t : _topologies)
+        t->findNTuples([&](std::size_t, std::size_t) { ++n_edges; }, [](std::size_t, std::size_t, std::size_t) {},
+                       [](std::size_t, std::size_t, std::size_t, std::size_t) {});
+    std::vector<uint32_t> broken(3 * std::max<std::size_t>(n_edges, 1));
+    uint64_t nb = 0;
+    ok(rdy_break_bonds(_k->engine(), dt, (uint32_t)pairs.size(), pairs.data(), n_edges, broken.data(), &nb),
+       "break bonds");
+    if (nb == 0) return;
+    if (nb > n_edges) throw std::logic_error("break bonds: more broken edges than edges");
+    // topology particle index (the device handle) -> (topology, vertex)
+    std::vector<std::pair<std::size_t, std::size_t>> where(_topoParticles.size(), {SIZE_MAX, 0});
+    for (std::size_t ti = 0; ti < _topologies.size(); ++ti) {
+        const auto &pi = _topologies[ti]->particleIndices();
+        for (std::size_t v = 0; v < pi.size(); ++v) where.at(pi[v]) = {ti, v};
+    }
+    std::vector<char> touched(_topologies.size(), 0);
+    for (uint64_t k = 0; k < nb; ++k) {
+        const auto a = where.at(broken[3 * k + 1]), b = where.at(broken[3 * k + 2]);
+        if (a.first == SIZE_MAX || a.first != b.first || !_topologies[a.first]->removeEdge(a.second, b.second))
+            throw std::logic_error("break bonds: a broken bond is not an edge of one topology");
+        touched[a.first] = 1;
+    }
+    // a topology that lost edges is replaced by its connected components (in place, then appended), as the
+    // structural reaction's recipe does (Actions.h:262-287); every particle here is of topology flavour, so
+    // single particles stay one-particle topologies
+    std::vector<std::unique_ptr<GraphTopology>> next, appended;
+    for (std::size_t ti = 0; ti < _topologies.size(); ++ti) {
+        if (!touched[ti]) {
+            next.push_back(std::move(_topologies[ti]));
+            continue;
+        }
+        auto parts = _topologies[ti]->connectedComponents();
+        for (std::size_t c = 0; c < parts.size(); ++c)
+            (c == 0 ? next : appended).push_back(std::make_unique<GraphTopology>(std::move(parts[c])));
+    }
+    for (auto &t : appended) next.push_back(std::move(t));
+    _topologies = std::move(next);
+    _topoDirty = true;
+    configureTopologies();
+}
+
 void MI355XStateModel::clear() {
     if (_k->hasReplica()) ok(rdy_set_particles(_k->engine(), _k->replicaId(), 0, nullptr, nullptr), "clear");
     _topoParticles.clear();
@@ -521,6 +575,10 @@ std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::gillespie(scalar
 }
 
 // the CPU kernel does not provide DetailedBalance either (SCPU-only in the reference)
+std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::breakBonds(scalar dt, BreakConfig config) {
+    return step_action(dt, [k = _k, config](scalar h) { k->_state.breakBonds(h, config); });
+}
+
 std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::detailedBalance(scalar) {
     throw std::invalid_argument("DetailedBalance reaction handler is not available on the MI355X kernel");
 }

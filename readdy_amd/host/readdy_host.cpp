@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 #include "../../include/readdy_amd.h"
@@ -204,6 +205,58 @@ void GraphTopology::addEdge(std::size_t v1, std::size_t v2) {
     if (std::find(_neighbors[v1].begin(), _neighbors[v1].end(), v2) != _neighbors[v1].end()) return;
     _neighbors[v1].push_back(v2);
     _neighbors[v2].push_back(v1);
+}
+
+bool GraphTopology::hasEdge(std::size_t v1, std::size_t v2) const {
+    if (v1 >= _particles.size() || v2 >= _particles.size()) return false;
+    return std::find(_neighbors[v1].begin(), _neighbors[v1].end(), v2) != _neighbors[v1].end();
+}
+
+bool GraphTopology::removeEdge(std::size_t v1, std::size_t v2) {
+    if (!hasEdge(v1, v2)) return false;
+    _neighbors[v1].erase(std::find(_neighbors[v1].begin(), _neighbors[v1].end(), v2));
+    _neighbors[v2].erase(std::find(_neighbors[v2].begin(), _neighbors[v2].end(), v1));
+    return true;
+}
+
+std::vector<GraphTopology> GraphTopology::connectedComponents() const {
+    const std::size_t n = _particles.size();
+    std::vector<std::size_t> comp(n, SIZE_MAX);
+    std::size_t nc = 0;
+    for (std::size_t s = 0; s < n; ++s) {  // breadth-first labelling, components ordered by lowest vertex
+        if (comp[s] != SIZE_MAX) continue;
+        std::vector<std::size_t> queue{s};
+        comp[s] = nc;
+        for (std::size_t q = 0; q < queue.size(); ++q)
+            for (std::size_t m : _neighbors[queue[q]])
+                if (comp[m] == SIZE_MAX) {
+                    comp[m] = nc;
+                    queue.push_back(m);
+                }
+        ++nc;
+    }
+    std::vector<std::vector<std::size_t>> members(nc);
+    std::vector<std::size_t> local(n);
+    for (std::size_t v = 0; v < n; ++v) {
+        local[v] = members[comp[v]].size();
+        members[comp[v]].push_back(v);
+    }
+    std::vector<GraphTopology> out;
+    for (std::size_t c = 0; c < nc; ++c) {
+        std::vector<std::size_t> idx;
+        for (std::size_t v : members[c]) idx.push_back(_particles[v]);
+        GraphTopology t(_type, idx);
+        for (std::size_t v : members[c])
+            for (std::size_t m : _neighbors[v]) t._neighbors[local[v]].push_back(local[m]);
+        out.push_back(std::move(t));
+    }
+    return out;
+}
+
+void BreakConfig::addBreakablePair(ParticleTypeId type1, ParticleTypeId type2, scalar thresholdEnergy, scalar rate) {
+    if (!(rate > 0.))  // Actions.h:193-195
+        throw std::logic_error("Rate of breaking a bond must be positive, was " + std::to_string(rate));
+    _pairs.push_back({type1, type2, thresholdEnergy, rate});
 }
 
 // edges once (first seen from the lower-visited vertex), 3-paths (n2, v, n) with n2 < n around each

@@ -224,6 +224,12 @@ public:
     GraphTopology(std::string type, std::vector<std::size_t> particleIndices)
         : _type(std::move(type)), _particles(std::move(particleIndices)), _neighbors(_particles.size()) {}
     void addEdge(std::size_t v1, std::size_t v2);  // vertex indices into particleIndices()
+    bool removeEdge(std::size_t v1, std::size_t v2);  // false when there is no such edge
+    bool hasEdge(std::size_t v1, std::size_t v2) const;
+    std::size_t nParticles() const { return _particles.size(); }
+    // the connected components as topologies of the same type, vertices in ascending order and each
+    // vertex's neighbours in their present order (one component: a copy of this topology)
+    std::vector<GraphTopology> connectedComponents() const;
     const std::vector<std::size_t> &particleIndices() const { return _particles; }
     const std::string &type() const { return _type; }
     // visits every edge, 3-path and 4-path once, in the reference's enumeration order
@@ -254,6 +260,21 @@ protected:
     scalar _dt;
 };
 
+// BreakConfig (Actions.h:190-211): per unordered pair of particle types, the energy above which a bond
+// may break and the rate at which it then breaks
+class BreakConfig {
+public:
+    struct Pair {
+        ParticleTypeId type1, type2;
+        scalar thresholdEnergy, rate;
+    };
+    void addBreakablePair(ParticleTypeId type1, ParticleTypeId type2, scalar thresholdEnergy, scalar rate);
+    const std::vector<Pair> &pairs() const { return _pairs; }
+
+private:
+    std::vector<Pair> _pairs;
+};
+
 class ActionFactory {  // ActionFactory.h:93-128
 public:
     virtual ~ActionFactory() = default;
@@ -267,6 +288,8 @@ public:
     virtual std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) = 0;
     virtual std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) = 0;
     virtual std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) = 0;
+    // ActionFactory::breakBonds (Actions.h:218-341)
+    virtual std::unique_ptr<TimeStepDependentAction> breakBonds(scalar dt, BreakConfig config) = 0;
     virtual std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) = 0;
     virtual std::unique_ptr<Action> initializeKernel() = 0;
 };

@@ -333,6 +333,123 @@ static void test_topology() {
     or_destroy(o);
 }
 
+// BreakBonds on the reference's scenes (readdy/test/TestBreakingBonds.cpp:55-313): bonds of force
+// constant 1 and length 1 stretched to 2 carry energy 1; thresholds 0.9 and 1.1 at rate 1e10 and
+// dt 1 break them surely or never
+static auto bb_kernel() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {10., 10., 10.};
+    for (const char *t : {"A", "B", "C"}) c.particleTypes().addTopologyType(t, 1.0);
+    return k;
+}
+
+static std::map<uint64_t, std::pair<int, Vec3>> bb_particles(model::Kernel &k) {
+    std::map<uint64_t, std::pair<int, Vec3>> m;
+    for (const auto &p : k.stateModel().getParticles()) m[p.id()] = {p.type(), p.pos()};
+    return m;
+}
+
+static void test_break_bonds() {
+    using model::BreakConfig;
+    {  // a dimer A-A: breaks under threshold 0.9, not under 1.1; the bond force goes with the bond
+        for (const double thr : {0.9, 1.1}) {
+            auto k = bb_kernel();
+            auto &c = k->context();
+            c.topologyRegistry().configureBondPotential("A", "A", {1., 1.});
+            const auto A = c.particleTypes().idOf("A");
+            auto *t = k->stateModel().addTopology("T", {{0., 0., 0., A}, {0., 0., 2., A}});
+            t->addEdge(0, 1);
+            k->actions().calculateForces()->perform();
+            const auto f0 = k->stateModel().getParticleForces();
+            CHECK(f0.size() == 2 && std::fabs(std::fabs(f0[0].z) - 2.0) < 1e-12, "dimer bond force %g", f0[0].z);
+            BreakConfig cfg;
+            cfg.addBreakablePair(A, A, thr, 1e10);
+            k->actions().breakBonds(1., cfg)->perform();
+            const auto tops = k->stateModel().getTopologies();
+            if (thr < 1.) {
+                CHECK(tops.size() == 2 && tops[0]->nParticles() == 1 && tops[1]->nParticles() == 1,
+                      "dimer: %zu topologies after breaking", tops.size());
+            } else {
+                CHECK(tops.size() == 1 && tops[0]->nParticles() == 2 && tops[0]->type() == "T",
+                      "dimer: %zu topologies kept", tops.size());
+            }
+            k->actions().calculateForces()->perform();
+            const auto f1 = k->stateModel().getParticleForces();
+            const double want = thr < 1. ? 0.0 : 2.0;
+            CHECK(f1.size() == 2 && std::fabs(std::fabs(f1[0].z) - want) < 1e-12 &&
+                      std::fabs(std::fabs(f1[1].z) - want) < 1e-12,
+                  "dimer force after break bonds %g (want %g)", f1.size() ? f1[0].z : -1., want);
+            const auto ps = bb_particles(*k);
+            CHECK(ps.size() == 2 && ps.at(0).second.z == 0. && ps.at(1).second.z == 2., "dimer particles moved");
+        }
+        bool threw = false;
+        try {
+            BreakConfig cfg;
+            cfg.addBreakablePair(0, 0, 1., 0.);
+        } catch (const std::logic_error &) {
+            threw = true;
+        }
+        CHECK(threw, "a non-positive break rate must throw std::logic_error");
+    }
+    // a linear trimer A(z=-1) - A(z=1) - B(z=3)
+    for (int which = 0; which < 3; ++which) {  // A-B breakable, A-A breakable, both with threshold 1.1
+        auto k = bb_kernel();
+        auto &c = k->context();
+        c.topologyRegistry().configureBondPotential("A", "A", {1., 1.});
+        c.topologyRegistry().configureBondPotential("B", "A", {1., 1.});
+        const auto A = c.particleTypes().idOf("A"), B = c.particleTypes().idOf("B");
+        auto *t = k->stateModel().addTopology("T", {{0., 0., -1., A}, {0., 0., 1., A}, {0., 0., 3., B}});
+        t->addEdge(0, 1);
+        t->addEdge(1, 2);
+        BreakConfig cfg;
+        if (which == 0) cfg.addBreakablePair(A, B, 0.9, 1e10);
+        if (which == 1) cfg.addBreakablePair(A, A, 0.9, 1e10);
+        if (which == 2) {
+            cfg.addBreakablePair(A, A, 1.1, 1e10);
+            cfg.addBreakablePair(A, B, 1.1, 1e10);
+        }
+        k->actions().breakBonds(1., cfg)->perform();
+        const auto tops = k->stateModel().getTopologies();
+        const auto ps = bb_particles(*k);
+        if (which == 2) {
+            CHECK(tops.size() == 1 && tops[0]->nParticles() == 3, "trimer kept: %zu topologies", tops.size());
+        } else {
+            CHECK(tops.size() == 2, "trimer: %zu topologies", tops.size());
+            const auto *pair = tops.size() == 2 ? (tops[0]->nParticles() == 2 ? tops[0] : tops[1]) : nullptr;
+            CHECK(pair && pair->nParticles() == 2, "trimer: no two-particle topology");
+            if (pair) {
+                std::vector<int> ty;
+                for (auto i : pair->particleIndices()) ty.push_back(ps.at(i).first);
+                std::sort(ty.begin(), ty.end());
+                const std::vector<int> want = which == 0 ? std::vector<int>{A, A} : std::vector<int>{A, B};
+                CHECK(ty == want, "trimer case %d: wrong particles in the two-particle topology", which);
+            }
+        }
+        CHECK(ps.size() == 3 && ps.at(0).second.z == -1. && ps.at(1).second.z == 1. && ps.at(2).second.z == 3.,
+              "trimer particles moved");
+    }
+    {  // three dimers A-A, B-B, C-C broken one after the other: six one-particle topologies
+        auto k = bb_kernel();
+        auto &c = k->context();
+        for (const char *n : {"A", "B", "C"}) c.topologyRegistry().configureBondPotential(n, n, {1., 1.});
+        for (const char *n : {"A", "B", "C"}) {
+            const auto T = c.particleTypes().idOf(n);
+            k->stateModel().addTopology("T", {{0., 0., 0., T}, {0., 0., 2., T}})->addEdge(0, 1);
+        }
+        BreakConfig cfg;
+        for (const char *n : {"A", "B", "C"}) {
+            const auto T = c.particleTypes().idOf(n);
+            cfg.addBreakablePair(T, T, 0.9, 1e10);
+            k->actions().breakBonds(1., cfg)->perform();
+        }
+        const auto tops = k->stateModel().getTopologies();
+        CHECK(tops.size() == 6, "three dimers: %zu topologies", tops.size());
+        for (const auto *tp : tops) CHECK(tp->nParticles() == 1, "three dimers: a topology of %zu", tp->nParticles());
+    }
+    std::printf("break bonds: reference scenes passed\n");
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::printf("usage: %s <plugin directory>\n", argv[0]);
@@ -344,6 +461,7 @@ int main(int argc, char **argv) {
         if (!provider_only) {
             test_c0_forces_and_loop(argv[1]);
             test_topology();
+            test_break_bonds();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

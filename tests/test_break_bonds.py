@@ -1,0 +1,111 @@
+"""BreakBonds' device pass (Actions.h:218-341): an edge whose end types form a breakable pair and whose
+summed harmonic bond energy exceeds the pair's threshold breaks with probability 1 - exp(-rate dt).
+
+The edge energies are the oracle's (or_bond_force_energy, pinned by the reference's bond known answers),
+so at rate 1e10 (probability 1 in FP64) the broken set must equal the set the oracle's energies select,
+bit for bit; at a finite rate the broken fraction must match 1 - exp(-rate dt) within 4 sigma, with new
+draws on every call and the same draws for the same seed and call.  The reference's own scenes
+(TestBreakingBonds.cpp) run through the C++ plugin in tests/cpp/test_plugin.cpp.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+L = pytest.importorskip("readdy_amd._lib")
+from readdy_amd import Engine  # noqa: E402
+
+BOX = 40.0
+
+
+def _dimers(n, rng, types=(0, 1, 2)):
+    """n dimers with random orientation and length in [0.5, 2.5]; type per end from `types`."""
+    c = (rng.random((n, 3)) - 0.5) * (BOX - 4)
+    d = rng.normal(size=(n, 3))
+    d *= (rng.uniform(0.5, 2.5, n) / np.linalg.norm(d, axis=1))[:, None]
+    pos = np.empty((2 * n, 3))
+    pos[0::2], pos[1::2] = c, c + d
+    pos = (pos + BOX / 2) % BOX - BOX / 2
+    t = rng.choice(np.array(types, dtype=np.uint32), 2 * n)
+    return pos, t, np.arange(2 * n, dtype=np.uint32).reshape(-1, 2)
+
+
+def _engine(n_types, pos, types, bonds, k, l0, seed=7, replicas=1):
+    e = Engine(n_types, [1.0] * n_types, topology_flavor=[1] * n_types, nl_cutoff=2.0, max_replicas=replicas)
+    rs = []
+    for q in range(replicas):
+        rs.append(e.add_replica([BOX] * 3, (1, 1, 1), seed=seed + q, max_particles=2 * len(pos)))
+        e.set_particles(rs[-1], pos, types)
+        e.set_bonds(rs[-1], bonds, k, l0)
+    return e, rs
+
+
+def _oracle_energy(pos, i, j, k, l0):
+    return po.bond_force_energy(k, l0, pos[i], pos[j], [BOX] * 3, [1, 1, 1])[1]
+
+
+def test_sure_breaks_equal_the_oracle_energies():
+    rng = np.random.default_rng(1)
+    pos, types, bonds = _dimers(3000, rng)
+    k = rng.uniform(0.5, 2.0, len(bonds))
+    l0 = rng.uniform(0.8, 1.5, len(bonds))
+    e, (r,) = _engine(3, pos, types, bonds, k, l0)
+    pairs = [(0, 0, 0.3, 1e10), (0, 1, 0.6, 1e10), (2, 1, 1.0, 1e10)]  # (1, 1), (0, 2), (2, 2) never break
+    got = {tuple(int(v) for v in row) for row in e.break_bonds(1.0, pairs)}
+    thr = {(0, 0): 0.3, (0, 1): 0.6, (1, 0): 0.6, (1, 2): 1.0, (2, 1): 1.0}
+    want = set()
+    for b, (i, j) in enumerate(bonds):
+        key = (int(types[i]), int(types[j]))
+        if key in thr and _oracle_energy(pos, i, j, k[b], l0[b]) > thr[key]:
+            want.add((r, int(i), int(j)))
+    print("sure breaks", len(got), "of", len(bonds), flush=True)
+    assert got == want and 200 < len(want) < len(bonds) - 200
+
+
+def test_configurations_of_one_edge_add_up():
+    """Two bond configurations on one edge: each alone below the threshold, their sum above it."""
+    rng = np.random.default_rng(2)
+    n = 400
+    pos, types, bonds = _dimers(n, rng, types=(0,))
+    length = np.linalg.norm(((pos[1::2] - pos[0::2]) + BOX / 2) % BOX - BOX / 2, axis=1)
+    twice = np.repeat(bonds, 2, axis=0)            # edge b listed twice, consecutively
+    l0 = np.repeat(length - np.sqrt(0.6), 2)        # energy 0.6 per configuration (k = 1)
+    e, (r,) = _engine(1, pos, types, twice, 1.0, l0)
+    got = e.break_bonds(1.0, [(0, 0, 1.0, 1e10)])
+    assert len(got) == n  # 1.2 > 1.0 for every edge, each edge listed once
+    e1, (r1,) = _engine(1, pos, types, bonds, 1.0, l0[::2])
+    assert len(e1.break_bonds(1.0, [(0, 0, 1.0, 1e10)])) == 0  # 0.6 < 1.0
+
+
+def test_finite_rate_breaks_at_the_rate_with_fresh_draws():
+    rng = np.random.default_rng(3)
+    pos, types, bonds = _dimers(6000, rng, types=(0,))
+    e, (r,) = _engine(1, pos, types, bonds, 10.0, 0.2)  # every edge far above the threshold
+    rate, dt = 0.7, 0.5
+    p = 1.0 - np.exp(-rate * dt)
+    sets = []
+    for _ in range(3):
+        b = e.break_bonds(dt, [(0, 0, 0.0, rate)])
+        frac = len(b) / len(bonds)
+        assert abs(frac - p) < 4 * np.sqrt(p * (1 - p) / len(bonds)), frac
+        sets.append({tuple(x) for x in b.tolist()})
+    assert sets[0] != sets[1] and sets[1] != sets[2]  # each call draws anew
+    e2, _ = _engine(1, pos, types, bonds, 10.0, 0.2)  # same seed, same call number: same draws
+    assert {tuple(x) for x in e2.break_bonds(dt, [(0, 0, 0.0, rate)]).tolist()} == sets[0]
+
+
+def test_replicas_and_argument_errors():
+    rng = np.random.default_rng(4)
+    pos, types, bonds = _dimers(500, rng, types=(0,))
+    e, rs = _engine(1, pos, types, bonds, 1.0, 0.5, replicas=3)
+    got = e.break_bonds(1.0, [(0, 0, 0.0, 1e10)])
+    # the same dimers in three replicas: every stretched edge of each, sorted by replica
+    stretched = sum(1 for i, j in bonds if _oracle_energy(pos, i, j, 1.0, 0.5) > 0.0)
+    assert len(got) == 3 * stretched and list(np.unique(got[:, 0])) == rs
+    assert np.array_equal(got, got[np.lexsort((got[:, 2], got[:, 1], got[:, 0]))])
+    with pytest.raises(L.EngineError):
+        e.break_bonds(1.0, [(0, 0, 1.0, 0.0)])  # BreakConfig: the rate must be positive
+    with pytest.raises(L.EngineError):
+        e.break_bonds(1.0, [(0, 5, 1.0, 1.0)])  # no such type
