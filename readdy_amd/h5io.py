@@ -7,6 +7,7 @@ Writers (one file per call sequence: create, datasets, attributes, close):
                        (ReactionCounts, readdy/main/model/observables/Observables.cpp)
   radial_distribution  /readdy/observables/<name>/{bin_centers [B], distribution [T][B], time [T]}
                        (RadialDistribution.cpp:158-170)
+  histogram_along_axis /readdy/observables/<name>/{data [T][B] f64, time [T]} (HistogramAlongAxis.cpp:83-96)
   reactions            /readdy/observables/<name>/{records [T] (variable-length lists of reaction
                        records), time [T]} (Reactions.cpp:72-86; record fields as in Types.cpp:66-77)
   flat_trajectory      /readdy/trajectory/<name>/{records [R], limits [T][2], time [T]}
@@ -60,6 +61,8 @@ def lib():
                                        _u64p, _u64p, _f64p, _u16p]
         h.rdy_h5_read_reactions.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, ctypes.c_uint64,
                                             _u64p, _i32p, _u64p, _u64p, _f64p, _u16p, _u64p, _u64p]
+        h.rdy_h5_series_f64.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
+                                        _f64p]
         h.rdy_h5_attribute.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p]
         h.rdy_h5_close.argtypes = [ctypes.c_int64]
         h.rdy_h5_read_u64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p]
@@ -106,6 +109,12 @@ class File:
         self._check(lib().rdy_h5_radial_distribution(self.f, name.encode(), len(t), len(c), _p(t, _u64p),
                                                      _p(c, _f64p), _p(d, _f64p)), name)
 
+    def histogram_along_axis(self, name, time, data):
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        d = np.ascontiguousarray(data, dtype=np.float64).reshape(len(t), -1)
+        self._check(lib().rdy_h5_series_f64(self.f, name.encode(), len(t), d.shape[1], _p(t, _u64p), _p(d, _f64p)),
+                    name)
+
     def flat_trajectory(self, name, time, frames):
         """frames: one (ids, type ids, flavors, positions [n][3]) tuple per entry of time."""
         t = np.ascontiguousarray(time, dtype=np.uint64)
@@ -150,10 +159,10 @@ class File:
         if reactions:
             col = lambda k, dt: np.ascontiguousarray([r[k] for r in reactions], dtype=dt)
             pad = lambda xs: (list(xs) + [0, 0])[:2]
-            ed = np.ascontiguousarray([pad(r["educts"]) for r in reactions], dtype=np.uint16)
-            pr = np.ascontiguousarray([pad(r["products"]) for r in reactions], dtype=np.uint16)
-            ne = np.ascontiguousarray([len(r["educts"]) for r in reactions], dtype=np.uint64)
-            npr = np.ascontiguousarray([len(r["products"]) for r in reactions], dtype=np.uint64)
+            ed = np.ascontiguousarray([pad(r.get("educts", [])) for r in reactions], dtype=np.uint16)
+            pr = np.ascontiguousarray([pad(r.get("products", [])) for r in reactions], dtype=np.uint16)
+            ne = np.ascontiguousarray([len(r.get("educts", [])) for r in reactions], dtype=np.uint64)
+            npr = np.ascontiguousarray([len(r.get("products", [])) for r in reactions], dtype=np.uint64)
             ids, rate = col("id", np.uint64), col("rate", np.float64)
             edist = np.ascontiguousarray([r.get("educt_distance", 0.0) for r in reactions], dtype=np.float64)
             pdist = np.ascontiguousarray([r.get("product_distance", 0.0) for r in reactions], dtype=np.float64)
@@ -257,7 +266,7 @@ def read_reactions(path, name):
 
 def read_trajectory(path, name):
     """The frames of /readdy/trajectory/<name>: (time, [(ids, types, flavors, positions) per frame])."""
-    g = f"/readdy/trajectory/{name}"
+    g = f"/readdy/trajectory/{name}" if name else "/readdy/trajectory"
     t = read_u64(path, g + "/time")
     lim = read_u64(path, g + "/limits").reshape(len(t), 2)
     ids, types, flav, pos = read_records(path, g + "/records")

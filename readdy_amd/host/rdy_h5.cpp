@@ -7,6 +7,7 @@
 //   /readdy/observables/<name>/counts/<reaction id>   [T] uint64 events per interval  (ReactionCounts)
 //   /readdy/observables/<name>/bin_centers [B], distribution [T][B] double, time [T]
 //                                                   (RadialDistribution.cpp:158-170)
+//   /readdy/observables/<name>/data [T][B] double, time [T]     (HistogramAlongAxis.cpp:83-96)
 //   /readdy/trajectory/<name>/records [R] {id u64, typeId u16, flavor u8, pos f64[3]},
 //                             limits [T][2] uint64 (a frame's [begin, end) in records), time [T]
 //                                                   (FlatTrajectory, Trajectory.cpp:112-158; Types.cpp:79-88)
@@ -29,6 +30,8 @@
 //                              const uint64_t *time, const uint64_t *counts);   counts [T][n_reactions]
 //   int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t n_bins,
 //                                  const uint64_t *time, const double *centers, const double *dist);
+//   int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, const uint64_t *time,
+//                         const double *data);                    data [T][width]
 //   int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time,
 //                              const uint64_t *frame_sizes, const uint64_t *ids, const uint16_t *types,
 //                              const uint8_t *flavors, const double *pos);          pos [R][3]
@@ -253,6 +256,17 @@ int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t
     return rc;
 }
 
+int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, const uint64_t *time,
+                      const double *data) {
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    hsize_t dt[1] = {T}, dd[2] = {T, width};
+    int rc = write_f64(g, "data", 2, dd, data);
+    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
 int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time, const uint64_t *frame_sizes,
                            const uint64_t *ids, const uint16_t *types, const uint8_t *flavors, const double *pos) {
     hid_t g0 = ensure_group((hid_t)f, "readdy");
@@ -260,8 +274,12 @@ int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64
     hid_t g1 = ensure_group(g0, "trajectory");
     H5Gclose(g0);
     if (g1 < 0) return -1;
-    hid_t g = ensure_group(g1, name);
-    H5Gclose(g1);
+    // an empty name puts the datasets into /readdy/trajectory itself (Trajectory.cpp:115-116)
+    hid_t g = g1;
+    if (name[0]) {
+        g = ensure_group(g1, name);
+        H5Gclose(g1);
+    }
     if (g < 0) return -1;
     uint64_t total = 0;
     std::vector<uint64_t> limits(2 * T);

@@ -1,0 +1,141 @@
+"""Observables of one replica recorded at a stride and written to HDF5 in ReaDDy's layout.
+
+The reference's simulation loop evaluates every registered observable before the first step and after
+each step t whose number is a multiple of the observable's stride (SimulationLoop.h:268-300,
+Observable.h:144-147: stride 0 means every step).  Recorder does the same around the engine's fused
+steps: it runs the replica in chunks that end where the next observable is due, evaluates the due ones
+on the device state, and keeps the series on the host; write() puts them into an HDF5 file with the
+configuration group that readdy.Trajectory reads.
+
+  n_particles           NParticles: particles per type                       /readdy/observables/<name>
+  reaction_counts       ReactionCounts: events per reaction of the last step  (its handler call)
+  reactions             Reactions: the last step's reaction records
+  radial_distribution   RadialDistribution (RadialDistribution.cpp:75-124), on the device
+  histogram_along_axis  HistogramAlongAxis (CPUObservables.cpp:85-136), on the device
+  trajectory            FlatTrajectory: every particle's id, type and position /readdy/trajectory/<name>
+"""
+import math
+
+import numpy as np
+
+from . import h5io
+
+
+class Recorder:
+    def __init__(self, engine, replica=0):
+        self.e = engine
+        self.r = replica
+        self.t = 0                 # steps run through this recorder
+        self._obs = []             # dicts: kind, name, stride, args, time, values
+        self._evaluated_at = None  # the step whose observables were evaluated last
+
+    def _add(self, kind, name, stride, **args):
+        if any(o["name"] == name and o["group"] == _group(kind) for o in self._obs):
+            raise ValueError(f"an observable named {name!r} is already recorded")
+        if kind == "reactions":
+            self.e.record_reactions(True)
+        self._obs.append(dict(kind=kind, name=name, group=_group(kind), stride=int(stride), args=args, time=[],
+                              values=[]))
+
+    def n_particles(self, stride, name="n_particles"):
+        self._add("n_particles", name, stride)
+
+    def reaction_counts(self, stride, name="reaction_counts"):
+        self._add("reaction_counts", name, stride)
+
+    def reactions(self, stride, name="reactions"):
+        self._add("reactions", name, stride)
+
+    def radial_distribution(self, stride, borders, from_types, to_types, density, name="radial_distribution"):
+        self._add("rdf", name, stride, borders=np.asarray(borders, dtype=np.float64), frm=list(from_types),
+                  to=list(to_types), density=float(density))
+
+    def histogram_along_axis(self, stride, borders, types, axis, name="histogram_along_axis"):
+        self._add("histogram", name, stride, borders=np.asarray(borders, dtype=np.float64), types=list(types),
+                  axis=int(axis))
+
+    def trajectory(self, stride, name=""):
+        self._add("trajectory", name, stride)
+
+    # ------------------------------------------------------------------------------------ running
+    def _due(self, o, t):
+        return o["stride"] == 0 or t % o["stride"] == 0
+
+    def _evaluate(self, t):
+        if self._evaluated_at == t:
+            return
+        self._evaluated_at = t
+        e, r = self.e, self.r
+        for o in self._obs:
+            if not self._due(o, t):
+                continue
+            k, a = o["kind"], o["args"]
+            if k == "n_particles":
+                v = e.count_types(r + 1)[r].copy()
+            elif k == "reaction_counts":
+                v = e.reaction_counts(r).copy()
+            elif k == "reactions":
+                v = e.reaction_records(r)
+            elif k == "rdf":
+                v = e.radial_distribution(r, a["borders"], a["frm"], a["to"], a["density"])[1]
+            elif k == "histogram":
+                v = e.histogram_along_axis(r, a["borders"], a["types"], a["axis"]).astype(np.float64)
+            else:  # trajectory: (handles, types, positions)
+                pos, types, handles = e.get_particles(r)
+                v = (handles, types, pos)
+            o["time"].append(t)
+            o["values"].append(v)
+
+    def _next_due(self, t):
+        nxt = [t + 1 if o["stride"] == 0 else (t // o["stride"] + 1) * o["stride"] for o in self._obs]
+        return min(nxt) if nxt else math.inf
+
+    def run(self, handler, dt, n_steps, graph=True):
+        """n_steps fused steps of the replica's engine, observables evaluated where due (and at the
+        recorder's first step 0)."""
+        self._evaluate(self.t)
+        end = self.t + int(n_steps)
+        while self.t < end:
+            k = int(min(self._next_due(self.t), end) - self.t)
+            self.e.run(handler, dt, k, graph=graph)
+            self.t += k
+            self._evaluate(self.t)
+
+    def series(self, name, kind=None):
+        """(time, values) of the observable `name`."""
+        for o in self._obs:
+            if o["name"] == name and (kind is None or o["kind"] == kind):
+                return np.asarray(o["time"], dtype=np.uint64), o["values"]
+        raise KeyError(name)
+
+    # ------------------------------------------------------------------------------------ output
+    def write(self, path, type_names, box, pbc=(True, True, True), kbt=1.0, diffusion=None, reactions=(),
+              topology_types=()):
+        """One HDF5 file: the recorded series plus /readdy/config (types, reactions, general).
+        reactions: the engine's reaction dicts (kind, educts, products, rate, ...) with a "name"."""
+        diffusion = [0.0] * len(type_names) if diffusion is None else list(diffusion)
+        with h5io.File(path) as f:
+            f.config([(n, d, "TOPOLOGY" if n in topology_types else "NORMAL") for n, d in zip(type_names, diffusion)],
+                     box, pbc, kbt, [dict(rx, id=i, name=rx.get("name", f"reaction_{i}")) for i, rx in
+                                     enumerate(reactions)])
+            for o in self._obs:
+                k, t, v = o["kind"], o["time"], o["values"]
+                if k == "n_particles":
+                    f.n_particles(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
+                elif k == "reaction_counts":
+                    f.reaction_counts(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
+                elif k == "reactions":
+                    f.reactions(o["name"], t, v)
+                elif k == "rdf":
+                    b = o["args"]["borders"]
+                    f.radial_distribution(o["name"], t, (b[:-1] + b[1:]) / 2, np.asarray(v).reshape(len(t), -1))
+                elif k == "histogram":
+                    f.histogram_along_axis(o["name"], t, np.asarray(v).reshape(len(t), -1))
+                else:  # topology particles would carry flavor 1 (TOPOLOGY)
+                    flav = np.array([1 if n in topology_types else 0 for n in type_names], dtype=np.uint8)
+                    f.flat_trajectory(o["name"], t, [(h, ty, flav[np.asarray(ty, dtype=np.int64)], x)
+                                                     for h, ty, x in v])
+
+
+def _group(kind):
+    return "trajectory" if kind == "trajectory" else "observables"
