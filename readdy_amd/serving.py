@@ -132,7 +132,9 @@ class H5Writer:
                                         for k, rx in enumerate(chemistry()[1])])
                     t = res.observed_steps
                     f.n_particles("n_particles", t, res.observed_counts)
-                    f.reaction_counts("reaction_counts", t, res.observed_reactions)
+                    # totals since admission, in ReactionCounts' layout (the reference's observable holds
+                    # the last step's counts, hence another name)
+                    f.reaction_counts("reaction_totals", t, res.observed_reactions)
                     if res.observed_records:
                         f.reactions("reactions", t, res.observed_records)
                     if res.observed_frames:

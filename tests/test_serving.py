@@ -53,7 +53,7 @@ def test_served_results_equal_results_alone(tmp_path):
         t = _h5_read(path, "/readdy/observables/n_particles/time")
         d = _h5_read(path, "/readdy/observables/n_particles/data").reshape(len(t), 3)
         assert t[-1] == req.steps and np.array_equal(d[-1], np.bincount(tm, minlength=3))
-        c0 = _h5_read(path, "/readdy/observables/reaction_counts/counts/0")
+        c0 = _h5_read(path, "/readdy/observables/reaction_totals/counts/0")
         assert len(c0) == len(t) and c0[-1] == totals[0]
     # a budget that holds at most a few requests at a time made them queue and overlap
     assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
@@ -126,5 +126,5 @@ def test_writer_thread_files(tmp_path):
         assert np.array_equal(ids, h) and np.array_equal(ty2, ty) and np.array_equal(x2, x) and not fl.any()
     assert np.array_equal(h5io.read_u64(path, "/readdy/observables/n_particles/data").reshape(3, 3),
                           np.array(res.observed_counts))
-    assert list(h5io.read_u64(path, "/readdy/observables/reaction_counts/counts/1")) == [0, 1, 2]
+    assert list(h5io.read_u64(path, "/readdy/observables/reaction_totals/counts/1")) == [0, 1, 2]
     assert json.loads(h5io.read_string(path, "/readdy/config/general"))["box_size"] == [req.box] * 3
