@@ -1048,6 +1048,8 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     const size_t nrx = std::max<uint32_t>(e->n_rx, 1);
     HIP_TRY(hipMemsetAsync(e->rx_step + (size_t)r * nrx, 0, nrx * 8, e->stream));
     HIP_TRY(hipMemsetAsync(e->rx_total + (size_t)r * nrx, 0, nrx * 8, e->stream));
+    if (e->rec_count)  // a reused slot holds no records of its previous replica
+        HIP_TRY(hipMemsetAsync(e->rec_count + r, 0, sizeof(uint32_t), e->stream));
     e->n_reps_used = std::max(e->n_reps_used, r + 1);
     e->cells_hwm = std::max<uint64_t>(e->cells_hwm, base + cells);
     e->version++;

@@ -234,3 +234,18 @@ def test_gpu_fused_step_records_equal_separate_actions(handler, graph):
         g, w = e1.reaction_records(a), e2.reaction_records(b)
         assert len(g) > 10 and g.tobytes() == w.tobytes()  # same fields, same key order
         assert np.array_equal(np.bincount(g["reaction"], minlength=5).astype(np.uint64), e2.reaction_counts(b))
+
+
+@pytest.mark.gpu
+def test_gpu_reused_slot_starts_without_records():
+    """A replica added into the slot of a removed one reports no records before its first reaction step."""
+    Engine = _gpu()
+    e = Engine(3, [0.0] * 3, reactions=_rxs_gpu(SCENE_RXS), max_replicas=1)
+    r = e.add_replica([10.0] * 3, seed=4, max_particles=64)
+    e.set_particles(r, SCENE_POS, SCENE_TYPES)
+    e.record_reactions(True)
+    e.run(0, 0.01, 1)
+    assert len(e.reaction_records(r)) == 2
+    e.remove_replica(r)
+    r2 = e.add_replica([10.0] * 3, seed=5, max_particles=64)
+    assert r2 == r and len(e.reaction_records(r2)) == 0
