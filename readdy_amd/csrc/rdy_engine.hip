@@ -34,6 +34,7 @@ struct HostReplica {
     bool in_use = false;
     uint64_t cell_base = 0, n_cells = 0;
     uint32_t topo_base = 0, n_topo = 0;
+    bool has_particles = false;  // particles were uploaded since the replica was added
     ReplicaDev dev{};
 };
 
@@ -1102,11 +1103,15 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
             seen_free = true;
         }
     }
-    // replace: drop the replica's current particles
-    ensure_wl(e);
-    hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
-                       (uint32_t)e->max_particles);
+    // replace: drop the replica's current particles (a replica that never held any has none to drop,
+    // which keeps filling many fresh replicas linear in their particles)
     HostReplica &H = e->reps_h[r];
+    if (H.has_particles) {
+        ensure_wl(e);
+        hipLaunchKernelGGL(k_kill_replica, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), r,
+                           (uint32_t)e->max_particles);
+    }
+    H.has_particles = true;
     if (n_topo) {
         if (e->topo_used + n_topo > e->topo_cap) {
             uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + n_topo);

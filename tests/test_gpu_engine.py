@@ -575,3 +575,23 @@ def test_ensemble_batched_equals_plain():
         assert np.array_equal(xb, xp)
     b.close()
     p.close()
+
+
+def test_set_particles_replaces_and_fresh_replicas_keep_their_neighbours():
+    """A second upload into a replica drops its previous particles; uploading into a fresh replica (no
+    drop pass) leaves the other replicas' particles alone."""
+    rng = np.random.default_rng(12)
+    e = Engine(2, [1.0, 1.0], potentials=[(L.POT_HARMONIC_REPULSION, 0, 1, [10.0, 1.0])], max_replicas=3)
+    r0 = e.add_replica([10.0] * 3, seed=1, max_particles=1000)
+    e.set_particles(r0, (rng.random((300, 3)) - 0.5) * 10, rng.integers(0, 2, 300).astype(np.uint32))
+    r1 = e.add_replica([12.0] * 3, seed=2, max_particles=1000)
+    p1 = (rng.random((200, 3)) - 0.5) * 12
+    e.set_particles(r1, p1, np.zeros(200, np.uint32))
+    assert e.n_particles(r0) == 300 and e.n_particles(r1) == 200
+    p0 = (rng.random((120, 3)) - 0.5) * 10
+    e.set_particles(r0, p0, np.ones(120, np.uint32))  # replaces r0's 300
+    x0, t0, _ = e.get_particles(r0)
+    assert len(t0) == 120 and np.all(t0 == 1) and np.array_equal(x0, p0)
+    x1, _, _ = e.get_particles(r1)
+    assert np.array_equal(x1, p1)
+    assert np.array_equal(e.count_types(2), [[0, 120], [200, 0]])
