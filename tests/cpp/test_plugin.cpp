@@ -450,7 +450,66 @@ static void test_break_bonds() {
     std::printf("break bonds: reference scenes passed\n");
 }
 
+// host core only (no kernel): edge removal and the component split BreakBonds relies on, on a graph
+// with a cycle and a branch, against the same components built edge by edge
+using Tuples = std::vector<std::vector<std::size_t>>;
+static Tuples enumerate(const model::GraphTopology &t) {
+    Tuples out;
+    t.findNTuples([&](std::size_t a, std::size_t b) { out.push_back({a, b}); },
+                  [&](std::size_t a, std::size_t b, std::size_t c) { out.push_back({a, b, c}); },
+                  [&](std::size_t a, std::size_t b, std::size_t c, std::size_t d) { out.push_back({a, b, c, d}); });
+    return out;
+}
+
+static void test_graph_ops() {
+    // particles 10..17; triangle 0-1-2, bridge 2-3, branch 3-4-5 and 3-6-7
+    model::GraphTopology g("T", {10, 11, 12, 13, 14, 15, 16, 17});
+    for (auto [a, b] : std::vector<std::pair<int, int>>{{0, 1}, {1, 2}, {2, 0}, {2, 3}, {3, 4}, {4, 5}, {3, 6}, {6, 7}})
+        g.addEdge(a, b);
+    CHECK(!g.removeEdge(0, 5), "removing a missing edge must report false");
+    CHECK(g.connectedComponents().size() == 1, "one component before any removal");
+    CHECK(g.removeEdge(0, 1) && !g.hasEdge(1, 0), "edge 0-1 removed in both directions");
+    CHECK(g.connectedComponents().size() == 1, "a cycle edge does not split");
+    CHECK(g.removeEdge(3, 2), "bridge removed (given in the other order)");
+    const auto parts = g.connectedComponents();
+    CHECK(parts.size() == 2, "the bridge splits into %zu components", parts.size());
+    if (parts.size() == 2) {
+        CHECK((parts[0].particleIndices() == std::vector<std::size_t>{10, 11, 12}), "first component's particles");
+        CHECK((parts[1].particleIndices() == std::vector<std::size_t>{13, 14, 15, 16, 17}), "second component's particles");
+        model::GraphTopology a("T", {10, 11, 12}), b("T", {13, 14, 15, 16, 17});
+        a.addEdge(1, 2);
+        a.addEdge(2, 0);
+        b.addEdge(0, 1);
+        b.addEdge(1, 2);
+        b.addEdge(0, 3);
+        b.addEdge(3, 4);
+        CHECK(enumerate(parts[0]) == enumerate(a), "first component's bonds / angles / dihedrals");
+        CHECK(enumerate(parts[1]) == enumerate(b), "second component's bonds / angles / dihedrals");
+        CHECK(enumerate(parts[1]).size() == 4 + 3 + 2, "branch: 4 edges, 3 angles, 2 dihedrals");
+        CHECK(parts[0].type() == "T" && parts[1].type() == "T", "components keep the topology type");
+    }
+    for (const double rate : {0.0, -1.0}) {
+        bool threw = false;
+        try {
+            model::BreakConfig c;
+            c.addBreakablePair(0, 1, 1.0, rate);
+        } catch (const std::logic_error &) {
+            threw = true;
+        }
+        CHECK(threw, "break rate %g must throw", rate);
+    }
+    model::BreakConfig c;
+    c.addBreakablePair(0, 1, 0.5, 2.0);
+    CHECK(c.pairs().size() == 1 && c.pairs()[0].rate == 2.0, "breakable pair kept");
+    std::printf("graph operations: passed\n");
+}
+
 int main(int argc, char **argv) {
+    if (argc > 1 && std::string(argv[1]) == "--host-only") {
+        test_graph_ops();
+        std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
+        return failures ? 1 : 0;
+    }
     if (argc < 2) {
         std::printf("usage: %s <plugin directory>\n", argv[0]);
         return 2;

@@ -1,6 +1,7 @@
 """The C++ boundary: the MI355X kernel plugin loaded through the host core's KernelProvider.
 
-CPU tests check the plugin's C-linkage entry points and the provider's loading and error paths;
+CPU tests check the plugin's C-linkage entry points, the provider's loading and error paths, and the
+host core's graph operations that BreakBonds uses (edge removal, component split);
 the GPU test runs tests/cpp/test_plugin, which drives the C0 system and a bead-chain topology
 through Context / StateModel / ActionFactory / SimulationLoop and checks them against the FP64
 oracle (forces, energies, five UncontrolledApproximation steps event for event, conservation
@@ -51,5 +52,13 @@ def test_cpp_plugin_against_oracle():
     _need_build()
     out = subprocess.run([TEST_BIN, PLUGIN_DIR], capture_output=True, text=True, timeout=300)
     print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "ALL PASSED" in out.stdout
+
+
+def test_host_core_graph_operations():
+    """Edge removal and the connected-component split of the host core, on the CPU."""
+    _need_build()
+    out = subprocess.run([TEST_BIN, "--host-only"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "ALL PASSED" in out.stdout
