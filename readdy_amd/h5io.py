@@ -8,6 +8,10 @@ Writers (one file per call sequence: create, datasets, attributes, close):
   radial_distribution  /readdy/observables/<name>/{bin_centers [B], distribution [T][B], time [T]}
                        (RadialDistribution.cpp:158-170)
   histogram_along_axis /readdy/observables/<name>/{data [T][B] f64, time [T]} (HistogramAlongAxis.cpp:83-96)
+  positions, forces     /readdy/observables/<name>/{data [T] (variable-length lists of {x, y, z}), time}
+                       (Positions.cpp:71-86, Forces.cpp:80-91)
+  particles            /readdy/observables/<name>/{types, ids, positions [T] (variable-length), time}
+                       (Particles.cpp:62-92)
   reactions            /readdy/observables/<name>/{records [T] (variable-length lists of reaction
                        records), time [T]} (Reactions.cpp:72-86; record fields as in Types.cpp:66-77)
   flat_trajectory      /readdy/trajectory/<name>/{records [R], limits [T][2], time [T]}
@@ -16,7 +20,7 @@ Writers (one file per call sequence: create, datasets, attributes, close):
   config               /readdy/config/{particle_types, registered_reactions, general}
                        (IOUtils.cpp:58-176: what readdy.Trajectory reads for type names, reactions,
                        kBT, box and periodicity)
-Readers (tests and tools): read_u64, read_f64, read_records, read_string, read_reactions.
+Readers (tests and tools): read_u64, read_f64, read_records, read_string, read_reactions, read_vlen.
 """
 import ctypes
 import json
@@ -63,6 +67,10 @@ def lib():
                                             _u64p, _i32p, _u64p, _u64p, _f64p, _u16p, _u64p, _u64p]
         h.rdy_h5_series_f64.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
                                         _f64p]
+        h.rdy_h5_vlen_series.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.c_uint64, _u64p, ctypes.c_void_p, _u64p]
+        h.rdy_h5_read_vlen.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, _u64p,
+                                       ctypes.c_uint64, ctypes.c_void_p, _u64p, _u64p]
         h.rdy_h5_attribute.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p]
         h.rdy_h5_close.argtypes = [ctypes.c_int64]
         h.rdy_h5_read_u64.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64, _u64p, _u64p]
@@ -114,6 +122,31 @@ class File:
         d = np.ascontiguousarray(data, dtype=np.float64).reshape(len(t), -1)
         self._check(lib().rdy_h5_series_f64(self.f, name.encode(), len(t), d.shape[1], _p(t, _u64p), _p(d, _f64p)),
                     name)
+
+    def _vlen(self, name, dataset, elem, time, frames, with_time):
+        dt, width = _VLEN[elem]
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        if len(frames) != len(t):
+            raise ValueError("one frame per time point")
+        sizes = np.array([len(f) for f in frames], dtype=np.uint64)
+        data = (np.ascontiguousarray(np.concatenate([np.asarray(f, dtype=dt).reshape(-1, width) for f in frames]))
+                if int(sizes.sum()) else np.zeros((1, width), dt))
+        self._check(lib().rdy_h5_vlen_series(self.f, name.encode(), dataset.encode(), elem, len(t), _p(sizes, _u64p),
+                                             data.ctypes.data_as(ctypes.c_void_p), _p(t, _u64p) if with_time else None),
+                    f"{name}/{dataset}")
+
+    def positions(self, name, time, frames):
+        """frames: one [n][3] array of positions per entry of time."""
+        self._vlen(name, "data", 0, time, frames, True)
+
+    def forces(self, name, time, frames):
+        self._vlen(name, "data", 0, time, frames, True)
+
+    def particles(self, name, time, frames):
+        """frames: one (type ids, ids, positions [n][3]) tuple per entry of time."""
+        self._vlen(name, "types", 1, time, [f[0] for f in frames], False)
+        self._vlen(name, "ids", 2, time, [f[1] for f in frames], False)
+        self._vlen(name, "positions", 0, time, [f[2] for f in frames], True)
 
     def flat_trajectory(self, name, time, frames):
         """frames: one (ids, type ids, flavors, positions [n][3]) tuple per entry of time."""
@@ -262,6 +295,26 @@ def read_reactions(path, name):
     out["products"], out["types_from"], out["position"] = pr[:n.value], tf[:n.value], pos[:n.value]
     ends = np.cumsum(sizes[:nf.value].astype(np.int64))
     return t, [out[b - int(s):b] for b, s in zip(ends, sizes[:nf.value])]
+
+
+_VLEN = {0: (np.float64, 3), 1: (np.uint16, 1), 2: (np.uint64, 1)}
+
+
+def read_vlen(path, dataset, elem):
+    """The frames of a variable-length series (elem 0: {x, y, z} rows, 1: uint16, 2: uint64)."""
+    dt, width = _VLEN[elem]
+    nf, n = ctypes.c_uint64(), ctypes.c_uint64()
+    one = np.zeros(1, np.uint64)
+    lib().rdy_h5_read_vlen(path.encode(), dataset.encode(), elem, 0, _p(one, _u64p), 0, None, ctypes.byref(nf),
+                           ctypes.byref(n))
+    sizes = np.zeros(max(1, nf.value), np.uint64)
+    out = np.zeros((max(1, n.value), width), dt)
+    if lib().rdy_h5_read_vlen(path.encode(), dataset.encode(), elem, len(sizes), _p(sizes, _u64p), n.value,
+                              out.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nf), ctypes.byref(n)):
+        raise OSError(f"cannot read {dataset} of {path}")
+    ends = np.cumsum(sizes[:nf.value].astype(np.int64))
+    frames = [out[b - int(s):b] for b, s in zip(ends, sizes[:nf.value])]
+    return frames if width == 3 else [f.ravel() for f in frames]
 
 
 def read_trajectory(path, name):

@@ -8,6 +8,10 @@
 //   /readdy/observables/<name>/bin_centers [B], distribution [T][B] double, time [T]
 //                                                   (RadialDistribution.cpp:158-170)
 //   /readdy/observables/<name>/data [T][B] double, time [T]     (HistogramAlongAxis.cpp:83-96)
+//   /readdy/observables/<name>/data [T] vlen of {x, y, z} double, time [T]    (Positions.cpp:71-86,
+//                                                   Forces.cpp:80-91)
+//   /readdy/observables/<name>/{types vlen u16, ids vlen u64, positions vlen {x,y,z}}, time [T]
+//                                                   (Particles.cpp:62-92)
 //   /readdy/trajectory/<name>/records [R] {id u64, typeId u16, flavor u8, pos f64[3]},
 //                             limits [T][2] uint64 (a frame's [begin, end) in records), time [T]
 //                                                   (FlatTrajectory, Trajectory.cpp:112-158; Types.cpp:79-88)
@@ -32,6 +36,10 @@
 //                                  const uint64_t *time, const double *centers, const double *dist);
 //   int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, const uint64_t *time,
 //                         const double *data);                    data [T][width]
+//   int rdy_h5_vlen_series(int64_t f, const char *name, const char *dataset, int elem, uint64_t T,
+//                          const uint64_t *frame_sizes, const void *data, const uint64_t *time);
+//       elem 0: {x, y, z} doubles, 1: uint16, 2: uint64; the frames' elements back to back in data;
+//       time [T] is written beside the dataset when not null
 //   int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time,
 //                              const uint64_t *frame_sizes, const uint64_t *ids, const uint16_t *types,
 //                              const uint8_t *flavors, const double *pos);          pos [R][3]
@@ -57,11 +65,14 @@
 //                             uint64_t cap, uint64_t *reaction_id, int32_t *type, uint64_t *educts,
 //                             uint64_t *products, double *position, uint16_t *types_from, uint64_t *n_frames,
 //                             uint64_t *n);   *n_frames, *n = frames and records in the file
+//   int rdy_h5_read_vlen(const char *path, const char *dataset, int elem, uint64_t cap_frames,
+//                        uint64_t *frame_sizes, uint64_t cap, void *out, uint64_t *n_frames, uint64_t *n);
 //   int rdy_h5_read_string(const char *path, const char *dataset, uint64_t cap, char *out);
 //       a scalar string dataset into out (NUL-terminated; -1 when it needs more than cap bytes)
 #include <hdf5.h>
 
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -136,6 +147,20 @@ int write_compound(hid_t g, const char *name, hid_t type, hsize_t n, const void 
     if (ds >= 0) H5Dclose(ds);
     H5Sclose(space);
     return rc;
+}
+
+// element type of a variable-length series: {x, y, z} doubles (Types.cpp:57-64), uint16, uint64
+hid_t vlen_elem_type(int elem, size_t *size) {
+    if (elem == 0) {
+        hid_t t = H5Tcreate(H5T_COMPOUND, 3 * sizeof(double));
+        H5Tinsert(t, "x", 0, H5T_NATIVE_DOUBLE);
+        H5Tinsert(t, "y", sizeof(double), H5T_NATIVE_DOUBLE);
+        H5Tinsert(t, "z", 2 * sizeof(double), H5T_NATIVE_DOUBLE);
+        *size = 3 * sizeof(double);
+        return t;
+    }
+    *size = elem == 1 ? 2 : 8;
+    return H5Tcopy(elem == 1 ? H5T_NATIVE_UINT16 : H5T_NATIVE_UINT64);
 }
 
 // one reaction record in memory (the fields the reference's ReactionRecord type writes, Types.cpp:66-77)
@@ -263,6 +288,34 @@ int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, c
     hsize_t dt[1] = {T}, dd[2] = {T, width};
     int rc = write_f64(g, "data", 2, dd, data);
     if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_vlen_series(int64_t f, const char *name, const char *dataset, int elem, uint64_t T,
+                       const uint64_t *frame_sizes, const void *data, const uint64_t *time) {
+    if (elem < 0 || elem > 2) return -1;
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    size_t esz = 0;
+    hid_t et = vlen_elem_type(elem, &esz);
+    hid_t vt = H5Tvlen_create(et);
+    std::vector<hvl_t> frames(T);
+    const char *p = (const char *)data;
+    for (uint64_t t = 0; t < T; ++t) {
+        frames[t].len = frame_sizes[t];
+        frames[t].p = frame_sizes[t] ? (void *)p : nullptr;
+        p += frame_sizes[t] * esz;
+    }
+    hsize_t dt[1] = {T};
+    hid_t space = H5Screate_simple(1, dt, nullptr);
+    hid_t ds = H5Dcreate2(g, dataset, vt, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    int rc = ds < 0 ? -1 : (H5Dwrite(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) < 0 ? -1 : 0);
+    if (ds >= 0) H5Dclose(ds);
+    H5Sclose(space);
+    H5Tclose(vt);
+    H5Tclose(et);
+    if (rc == 0 && time) rc = write_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -544,6 +597,45 @@ int rdy_h5_read_reactions(const char *path, const char *dataset, uint64_t cap_fr
         }
         H5Tclose(vt);
         H5Tclose(ct);
+        H5Sclose(space);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return rc;
+}
+
+int rdy_h5_read_vlen(const char *path, const char *dataset, int elem, uint64_t cap_frames, uint64_t *frame_sizes,
+                     uint64_t cap, void *out, uint64_t *n_frames, uint64_t *n) {
+    if (elem < 0 || elem > 2) return -1;
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return -1;
+    int rc = -1;
+    hid_t ds = H5Dopen2(f, dataset, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t space = H5Dget_space(ds);
+        const hssize_t T = H5Sget_simple_extent_npoints(space);
+        size_t esz = 0;
+        hid_t et = vlen_elem_type(elem, &esz);
+        hid_t vt = H5Tvlen_create(et);
+        std::vector<hvl_t> frames(T > 0 ? (size_t)T : 0);
+        if (T >= 0 && H5Dread(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) >= 0) {
+            uint64_t total = 0;
+            for (const hvl_t &v : frames) total += v.len;
+            *n_frames = (uint64_t)T;
+            *n = total;
+            if ((uint64_t)T <= cap_frames && total <= cap) {
+                char *o = (char *)out;
+                for (size_t t = 0; t < frames.size(); ++t) {
+                    frame_sizes[t] = frames[t].len;
+                    if (frames[t].len) std::memcpy(o, frames[t].p, frames[t].len * esz);
+                    o += frames[t].len * esz;
+                }
+                rc = 0;
+            }
+            H5Dvlen_reclaim(vt, space, H5P_DEFAULT, frames.data());
+        }
+        H5Tclose(vt);
+        H5Tclose(et);
         H5Sclose(space);
         H5Dclose(ds);
     }

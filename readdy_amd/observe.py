@@ -12,6 +12,8 @@ configuration group that readdy.Trajectory reads.
   reactions             Reactions: the last step's reaction records
   radial_distribution   RadialDistribution (RadialDistribution.cpp:75-124), on the device
   histogram_along_axis  HistogramAlongAxis (CPUObservables.cpp:85-136), on the device
+  positions, forces     Positions / Forces: every particle's position or force (ordered by id)
+  particles             Particles: every particle's type, id and position
   trajectory            FlatTrajectory: every particle's id, type and position /readdy/trajectory/<name>
 """
 import math
@@ -54,6 +56,15 @@ class Recorder:
         self._add("histogram", name, stride, borders=np.asarray(borders, dtype=np.float64), types=list(types),
                   axis=int(axis))
 
+    def positions(self, stride, name="positions"):
+        self._add("positions", name, stride)
+
+    def forces(self, stride, name="forces"):
+        self._add("forces", name, stride)
+
+    def particles(self, stride, name="particles"):
+        self._add("particles", name, stride)
+
     def trajectory(self, stride, name=""):
         self._add("trajectory", name, stride)
 
@@ -80,6 +91,14 @@ class Recorder:
                 v = e.radial_distribution(r, a["borders"], a["frm"], a["to"], a["density"])[1]
             elif k == "histogram":
                 v = e.histogram_along_axis(r, a["borders"], a["types"], a["axis"]).astype(np.float64)
+            elif k == "positions":
+                v = e.get_particles(r)[0]
+            elif k == "forces":
+                f, h = e.get_forces(r)
+                v = f[np.argsort(h, kind="stable")]
+            elif k == "particles":
+                pos, types, handles = e.get_particles(r)
+                v = (types, handles, pos)
             else:  # trajectory: (handles, types, positions)
                 pos, types, handles = e.get_particles(r)
                 v = (handles, types, pos)
@@ -93,6 +112,8 @@ class Recorder:
     def run(self, handler, dt, n_steps, graph=True):
         """n_steps fused steps of the replica's engine, observables evaluated where due (and at the
         recorder's first step 0)."""
+        if self._evaluated_at is None and any(o["kind"] == "forces" for o in self._obs):
+            self.e.calculate_forces()  # the loop forms the initial forces before step 0's observables
         self._evaluate(self.t)
         end = self.t + int(n_steps)
         while self.t < end:
@@ -131,6 +152,12 @@ class Recorder:
                     f.radial_distribution(o["name"], t, (b[:-1] + b[1:]) / 2, np.asarray(v).reshape(len(t), -1))
                 elif k == "histogram":
                     f.histogram_along_axis(o["name"], t, np.asarray(v).reshape(len(t), -1))
+                elif k == "positions":
+                    f.positions(o["name"], t, v)
+                elif k == "forces":
+                    f.forces(o["name"], t, v)
+                elif k == "particles":
+                    f.particles(o["name"], t, v)
                 else:  # topology particles would carry flavor 1 (TOPOLOGY)
                     flav = np.array([1 if n in topology_types else 0 for n in type_names], dtype=np.uint8)
                     f.flat_trajectory(o["name"], t, [(h, ty, flav[np.asarray(ty, dtype=np.int64)], x)

@@ -138,3 +138,33 @@ def test_reactions_observable_round_trip(tmp_path):
         for needle in ('DATASET "records"', 'H5T_VLEN', 'H5T_STD_U64LE "reaction_id"', 'H5T_STD_I32LE "reaction_type"',
                        '"educts"', '"products"', '"position"', '"types_from"'):
             assert needle in out, needle
+
+
+def test_positions_forces_particles_round_trip(tmp_path):
+    """Positions / Forces (Positions.cpp:71-86, Forces.cpp:80-91): a variable-length list of {x, y, z}
+    per time point; Particles (Particles.cpp:62-92): types, ids and positions lists."""
+    rng = np.random.default_rng(8)
+    sizes = (3, 0, 5)
+    pos = [rng.normal(size=(n, 3)) for n in sizes]
+    types = [rng.integers(0, 9, n).astype(np.uint16) for n in sizes]
+    ids = [rng.integers(0, 1 << 50, n).astype(np.uint64) for n in sizes]
+    path = str(tmp_path / "p.h5")
+    with h5io.File(path) as f:
+        f.positions("positions", [0, 5, 10], pos)
+        f.forces("forces", [0, 5, 10], pos[::-1])
+        f.particles("particles", [0, 5, 10], list(zip(types, ids, pos)))
+    got = h5io.read_vlen(path, "/readdy/observables/positions/data", 0)
+    assert [g.shape for g in got] == [(3, 3), (0, 3), (5, 3)] and all(np.array_equal(a, b) for a, b in zip(got, pos))
+    assert all(np.array_equal(a, b) for a, b in zip(h5io.read_vlen(path, "/readdy/observables/forces/data", 0),
+                                                   pos[::-1]))
+    g = "/readdy/observables/particles/"
+    assert all(np.array_equal(a, b) for a, b in zip(h5io.read_vlen(path, g + "types", 1), types))
+    assert all(np.array_equal(a, b) for a, b in zip(h5io.read_vlen(path, g + "ids", 2), ids))
+    assert all(np.array_equal(a, b) for a, b in zip(h5io.read_vlen(path, g + "positions", 0), pos))
+    assert list(h5io.read_u64(path, g + "time")) == [0, 5, 10]
+    exe = _h5dump()
+    if exe:
+        out = subprocess.run([exe, "-H", path], capture_output=True, text=True, check=True).stdout
+        flat = "".join(out.split())
+        assert 'H5T_VLEN{H5T_COMPOUND{H5T_IEEE_F64LE"x";H5T_IEEE_F64LE"y";H5T_IEEE_F64LE"z";}}' in flat
+        assert 'H5T_VLEN{H5T_STD_U16LE}' in flat and 'H5T_VLEN{H5T_STD_U64LE}' in flat

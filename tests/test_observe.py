@@ -93,6 +93,9 @@ def test_recorder_on_the_device(tmp_path):
     rec.reaction_counts(1)
     rec.reactions(1)
     rec.trajectory(4)
+    rec.positions(4)
+    rec.forces(10)
+    rec.particles(20)
     borders = np.linspace(0.0, 4.0, 41)
     rec.radial_distribution(5, borders, [0], [0, 1, 2], n / box ** 3)
     hb = np.linspace(-box / 2, box / 2, 26)
@@ -120,6 +123,13 @@ def test_recorder_on_the_device(tmp_path):
     assert np.array_equal(rdf[-1], e.radial_distribution(r, borders, [0], [0, 1, 2], n / box ** 3)[1])
     _, hist = rec.series("histogram_along_axis")
     assert np.array_equal(hist[-1], e.histogram_along_axis(r, hb, [2], 0).astype(np.float64))
+    _, ps = rec.series("positions")
+    assert all(np.array_equal(p, fr[2]) for p, fr in zip(ps, frames))  # positions equal the trajectory's
+    tf_, fs = rec.series("forces")
+    f_now, h_now = e.get_forces(r)
+    assert list(tf_) == [0, 10, 20] and np.array_equal(fs[-1], f_now[np.argsort(h_now)])
+    _, parts = rec.series("particles")
+    assert np.array_equal(parts[-1][1], hf) and np.array_equal(parts[-1][2], xf)
     path = str(tmp_path / "run.h5")
     rec.write(path, ["A", "B", "C"], [box] * 3, diffusion=[1.0, 1.0, 0.5], reactions=rxs)
     ft, fframes = h5io.read_trajectory(path, "")
@@ -128,3 +138,5 @@ def test_recorder_on_the_device(tmp_path):
     assert [len(x) for x in rrec] == [len(x) for x in recs]
     assert np.array_equal(h5io.read_f64(path, "/readdy/observables/radial_distribution/distribution").reshape(5, -1),
                           np.array(rdf))
+    assert np.array_equal(h5io.read_vlen(path, "/readdy/observables/forces/data", 0)[-1], fs[-1])
+    assert np.array_equal(h5io.read_vlen(path, "/readdy/observables/particles/ids", 2)[-1], hf)
