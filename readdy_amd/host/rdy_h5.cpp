@@ -78,6 +78,11 @@
 
 namespace {
 
+// failures are reported through the return codes: HDF5's own error-stack printing stays off
+struct QuietErrors {
+    QuietErrors() { H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr); }
+} quiet_errors;
+
 hid_t ensure_group(hid_t parent, const char *name) {
     if (H5Lexists(parent, name, H5P_DEFAULT) > 0) return H5Gopen2(parent, name, H5P_DEFAULT);
     return H5Gcreate2(parent, name, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
