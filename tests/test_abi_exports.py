@@ -38,3 +38,13 @@ def test_create_rejects_invalid_context_without_gpu_work():
     h = ctypes.c_void_p()
     assert lib.rdy_engine_create(ctypes.byref(ctx), None, ctypes.byref(h)) == -1
     assert b"n_types" in lib.rdy_last_error()
+
+
+def test_python_binding_declares_every_abi_entry_point():
+    """readdy_amd/_lib.py gives every entry point of include/readdy_amd.h its ctypes signature."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    header = open(os.path.join(root, "include", "readdy_amd.h")).read()
+    binding = open(os.path.join(root, "readdy_amd", "_lib.py")).read()
+    declared = set(re.findall(r"\b(rdy_[a-z0-9_]+)\s*\(", header))
+    bound = set(re.findall(r'"(rdy_[a-z0-9_]+)"', binding))
+    assert declared and declared == bound, (sorted(declared - bound), sorted(bound - declared))
