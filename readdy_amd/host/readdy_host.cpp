@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
+#include <tuple>
 
 #include "../../include/readdy_amd.h"
 
@@ -91,7 +93,7 @@ static void check_rate(scalar rate) {
 void ReactionRegistry::addConversion(const std::string &name, const std::string &from, const std::string &to,
                                      scalar rate) {
     check_rate(rate);
-    _reactions.push_back({name, ReactionType::Conversion, {_types->idOf(from)}, {_types->idOf(to)}, rate});
+    append({name, ReactionType::Conversion, {_types->idOf(from)}, {_types->idOf(to)}, rate});
 }
 
 void ReactionRegistry::addEnzymatic(const std::string &name, const std::string &catalyst, const std::string &from,
@@ -100,7 +102,7 @@ void ReactionRegistry::addEnzymatic(const std::string &name, const std::string &
     ReactionDesc r{name, ReactionType::Enzymatic, {_types->idOf(from), _types->idOf(catalyst)},
                    {_types->idOf(to), _types->idOf(catalyst)}, rate};  // Enzymatic.h:56-57
     r.eductDistance = eductDistance;
-    _reactions.push_back(r);
+    append(r);
 }
 
 // weights normalised to sum 1 (Reaction.h:137-160)
@@ -116,7 +118,7 @@ void ReactionRegistry::addFission(const std::string &name, const std::string &fr
     ReactionDesc r{name, ReactionType::Fission, {_types->idOf(from)}, {_types->idOf(to1), _types->idOf(to2)}, rate};
     r.productDistance = productDistance;
     std::tie(r.weight1, r.weight2) = normalised(w1, w2);
-    _reactions.push_back(r);
+    append(r);
 }
 
 void ReactionRegistry::addFusion(const std::string &name, const std::string &from1, const std::string &from2,
@@ -125,12 +127,105 @@ void ReactionRegistry::addFusion(const std::string &name, const std::string &fro
     ReactionDesc r{name, ReactionType::Fusion, {_types->idOf(from1), _types->idOf(from2)}, {_types->idOf(to)}, rate};
     r.eductDistance = eductDistance;
     std::tie(r.weight1, r.weight2) = normalised(w1, w2);
-    _reactions.push_back(r);
+    append(r);
 }
 
 void ReactionRegistry::addDecay(const std::string &name, const std::string &type, scalar rate) {
     check_rate(rate);
-    _reactions.push_back({name, ReactionType::Decay, {_types->idOf(type)}, {}, rate});
+    append({name, ReactionType::Decay, {_types->idOf(type)}, {}, rate});
+}
+
+void ReactionRegistry::append(ReactionDesc r) {
+    for (const auto &q : _reactions)
+        if (q.name == r.name) throw std::invalid_argument("A reaction with the name " + r.name + " exists already");
+    _reactions.push_back(std::move(r));
+}
+
+namespace {
+std::string trimmed(const std::string &s) {
+    const auto b = s.find_first_not_of(" \t\n\r"), e = s.find_last_not_of(" \t\n\r");
+    return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+}
+
+// "X +(r) Y" -> (X, Y, r); "X + Y" -> (X, Y, "") when the side carries no distance
+std::tuple<std::string, std::string, std::string> split_plus(const std::string &side, std::size_t plus, bool distance) {
+    const std::string first = trimmed(side.substr(0, plus));
+    if (!distance) return {first, trimmed(side.substr(plus + 1)), std::string()};
+    const auto open = side.find('('), close = side.find(')');
+    if (open == std::string::npos || close == std::string::npos || close < open)
+        throw std::invalid_argument("The side \"" + side + "\" did not contain a pair of parentheses '(*)'");
+    return {first, trimmed(side.substr(close + 1)), trimmed(side.substr(open + 1, close - open - 1))};
+}
+
+scalar number(const std::string &s, const std::string &what) {
+    std::size_t used = 0;
+    const std::string t = trimmed(s);
+    scalar v = 0;
+    try {
+        v = std::stod(t, &used);
+    } catch (const std::exception &) {
+        used = 0;
+    }
+    if (used == 0) throw std::invalid_argument("The " + what + " \"" + s + "\" is not a number");
+    return v;
+}
+}  // namespace
+
+std::size_t ReactionRegistry::add(const std::string &descriptor, scalar rate) {
+    const std::string arrow = "->";
+    const auto ap = descriptor.find(arrow);
+    if (ap == std::string::npos)
+        throw std::invalid_argument("the descriptor must contain an arrow (\"->\") to indicate lhs and rhs.");
+    if (descriptor.find(arrow, ap + 1) != std::string::npos)
+        throw std::invalid_argument("the descriptor must not contain more than one arrow (\"->\").");
+    std::string lhs = trimmed(descriptor.substr(0, ap)), rhs = trimmed(descriptor.substr(ap + arrow.size()));
+    const auto colon = lhs.find(':');
+    if (colon == std::string::npos)
+        throw std::invalid_argument("The descriptor did not contain a colon ':' to specify the end of the name.");
+    const std::string name = trimmed(lhs.substr(0, colon));
+    lhs = trimmed(lhs.substr(colon + 1));
+    scalar w1 = 0.5, w2 = 0.5;
+    if (const auto lb = rhs.find('['); lb != std::string::npos) {
+        const auto rb = rhs.rfind(']');
+        if (rb == std::string::npos || rb < lb)
+            throw std::invalid_argument("The term \"" + rhs + "\" did not contain a pair of brackets '[*]'");
+        const std::string inner = rhs.substr(lb + 1, rb - lb - 1);
+        const auto comma = inner.find(',');
+        if (comma == std::string::npos)
+            throw std::invalid_argument("The term \"" + rhs + "\" did not contain a comma ','");
+        w1 = number(inner.substr(0, comma), "weight");
+        w2 = number(inner.substr(comma + 1), "weight");
+        rhs = trimmed(rhs.substr(0, lb));
+    }
+    std::string e1, e2, radius;
+    const auto lplus = lhs.find('+');
+    const bool two_educts = lplus != std::string::npos;
+    if (two_educts) std::tie(e1, e2, radius) = split_plus(lhs, lplus, true);
+    else e1 = lhs;
+    if (rhs.empty()) {
+        if (two_educts) throw std::invalid_argument("Left hand side (\"" + lhs + "\") did not contain one educt");
+        addDecay(name, e1, rate);
+        return _reactions.size() - 1;
+    }
+    const auto rplus = rhs.find('+');
+    if (rplus == std::string::npos) {
+        if (two_educts) addFusion(name, e1, e2, rhs, rate, number(radius, "educt distance"), w1, w2);
+        else addConversion(name, e1, rhs, rate);
+        return _reactions.size() - 1;
+    }
+    if (!two_educts) {
+        std::string p1, p2, distance;
+        std::tie(p1, p2, distance) = split_plus(rhs, rplus, true);
+        addFission(name, e1, p1, p2, rate, number(distance, "product distance"), w1, w2);
+        return _reactions.size() - 1;
+    }
+    std::string p1, p2, unused;
+    std::tie(p1, p2, unused) = split_plus(rhs, rplus, false);
+    if (e2 != p2)
+        throw std::invalid_argument("In enzymatic reaction, educt2 (\"" + e2 + "\") and product2 (\"" + p2 +
+                                    "\") have to be equal");
+    addEnzymatic(name, e2, e1, p1, rate, number(radius, "educt distance"));
+    return _reactions.size() - 1;
 }
 
 std::size_t ReactionRegistry::idOf(const std::string &name) const {

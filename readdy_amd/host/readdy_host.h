@@ -134,10 +134,17 @@ public:
                    const std::string &to, scalar rate, scalar eductDistance, scalar weight1 = .5,
                    scalar weight2 = .5);
     void addDecay(const std::string &name, const std::string &type, scalar rate);
+    // the descriptor form, ReactionRegistry::add (ReactionRegistry.cpp:154-305):
+    //   "name: A ->"                    decay        "name: A -> B"              conversion
+    //   "name: A +(r) B -> C [w1, w2]"  fusion       "name: A -> B +(d) C [w1, w2]"  fission
+    //   "name: A +(r) C -> B + C"       enzymatic (C the catalyst); weights optional (0.5, 0.5)
+    // returns the reaction's index; malformed descriptors throw std::invalid_argument
+    std::size_t add(const std::string &descriptor, scalar rate);
     const std::vector<ReactionDesc> &reactions() const { return _reactions; }
     std::size_t idOf(const std::string &name) const;
 
 private:
+    void append(ReactionDesc r);  // rejects a name that exists already (ReactionRegistry.cpp:62-63)
     const ParticleTypeRegistry *_types;
     std::vector<ReactionDesc> _reactions;
 };

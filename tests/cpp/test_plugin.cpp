@@ -504,9 +504,52 @@ static void test_graph_ops() {
     std::printf("graph operations: passed\n");
 }
 
+// the reaction descriptor form on the reference's own cases (readdy/test/TestContext.cpp:222-345)
+static void test_descriptors() {
+    using RT = model::ReactionType;
+    model::Context c;
+    for (const char *t : {"A", "B", "C"}) c.particleTypes().add(t, 1.);
+    auto &reg = c.reactions();
+    const auto A = c.particleTypes().idOf("A"), B = c.particleTypes().idOf("B"), C = c.particleTypes().idOf("C");
+    auto get = [&](const char *d, double rate) { return reg.reactions().at(reg.add(d, rate)); };
+    auto r = get("mydecay:A->", 2.);
+    CHECK(r.type == RT::Decay && r.educts == std::vector<ParticleTypeId>{A} && r.products.empty() && r.rate == 2.,
+          "decay descriptor");
+    r = get("myconv: A -> B", 3.);
+    CHECK(r.type == RT::Conversion && r.educts[0] == A && r.products[0] == B && r.rate == 3., "conversion descriptor");
+    r = get("myfus: B +(1.2) B -> C [0.5, 0.5]", 4.);
+    CHECK(r.type == RT::Fusion && r.educts == (std::vector<ParticleTypeId>{B, B}) && r.products[0] == C &&
+              r.eductDistance == 1.2 && r.weight1 == 0.5 && r.weight2 == 0.5 && r.rate == 4.,
+          "fusion descriptor");
+    r = get("myfiss: B -> C +(3.0) B [0.1, 0.9]", 5.);
+    CHECK(r.type == RT::Fission && r.educts[0] == B && r.products == (std::vector<ParticleTypeId>{C, B}) &&
+              r.productDistance == 3.0 && r.weight1 == 0.1 && r.weight2 == 0.9 && r.rate == 5.,
+          "fission descriptor");
+    r = get("myenz:A +(1.5) C -> B + C", 6.);
+    CHECK(r.type == RT::Enzymatic && r.educts == (std::vector<ParticleTypeId>{A, C}) &&
+              r.products == (std::vector<ParticleTypeId>{B, C}) && r.eductDistance == 1.5 && r.rate == 6.,
+          "enzymatic descriptor");
+    r = get("eat:B +(1) A -> A + A", 7.);
+    CHECK(r.type == RT::Enzymatic && r.educts == (std::vector<ParticleTypeId>{B, A}) &&
+              r.products == (std::vector<ParticleTypeId>{A, A}) && r.eductDistance == 1. && r.rate == 7.,
+          "predator-prey descriptor");
+    for (const char *bad : {"myinvalid: + A -> B", "noarrow: A B", " : Noname ->", "weights: A + A -> A [0.1, ]",
+                            "blub: A (3)+ A -> B", "eat: A +(1) B -> A + A", "two: A -> B -> C", "mydecay: B ->"}) {
+        bool threw = false;
+        try {
+            reg.add(bad, 1.);
+        } catch (const std::invalid_argument &) {
+            threw = true;
+        }
+        CHECK(threw, "descriptor \"%s\" must throw std::invalid_argument", bad);
+    }
+    std::printf("reaction descriptors: passed\n");
+}
+
 int main(int argc, char **argv) {
     if (argc > 1 && std::string(argv[1]) == "--host-only") {
         test_graph_ops();
+        test_descriptors();
         std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
         return failures ? 1 : 0;
     }

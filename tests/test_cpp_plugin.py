@@ -1,7 +1,8 @@
 """The C++ boundary: the MI355X kernel plugin loaded through the host core's KernelProvider.
 
 CPU tests check the plugin's C-linkage entry points, the provider's loading and error paths, and the
-host core's graph operations that BreakBonds uses (edge removal, component split);
+host core's graph operations that BreakBonds uses (edge removal, component split) and its reaction
+descriptor parser on the reference's cases (TestContext.cpp:222-345);
 the GPU test runs tests/cpp/test_plugin, which drives the C0 system and a bead-chain topology
 through Context / StateModel / ActionFactory / SimulationLoop and checks them against the FP64
 oracle (forces, energies, five UncontrolledApproximation steps event for event, conservation
