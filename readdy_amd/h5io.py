@@ -12,6 +12,8 @@ Writers (one file per call sequence: create, datasets, attributes, close):
                        (Positions.cpp:71-86, Forces.cpp:80-91)
   particles            /readdy/observables/<name>/{types, ids, positions [T] (variable-length), time}
                        (Particles.cpp:62-92)
+  topologies           /readdy/observables/<name>/{particles, limitsParticles [T][2], edges [E][2],
+                       limitsEdges [T][2], types [T] (variable-length), time} (Topologies.cpp:60-170)
   reactions            /readdy/observables/<name>/{records [T] (variable-length lists of reaction
                        records), time [T]} (Reactions.cpp:72-86; record fields as in Types.cpp:66-77)
   flat_trajectory      /readdy/trajectory/<name>/{records [R], limits [T][2], time [T]}
@@ -67,6 +69,8 @@ def lib():
                                             _u64p, _i32p, _u64p, _u64p, _f64p, _u16p, _u64p, _u64p]
         h.rdy_h5_series_f64.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
                                         _f64p]
+        h.rdy_h5_u64_table.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
+                                       ctypes.c_uint32, _u64p]
         h.rdy_h5_vlen_series.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                          ctypes.c_uint64, _u64p, ctypes.c_void_p, _u64p]
         h.rdy_h5_read_vlen.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_uint64, _u64p,
@@ -147,6 +151,36 @@ class File:
         self._vlen(name, "types", 1, time, [f[0] for f in frames], False)
         self._vlen(name, "ids", 2, time, [f[1] for f in frames], False)
         self._vlen(name, "positions", 0, time, [f[2] for f in frames], True)
+
+    def topologies(self, name, time, frames):
+        """frames: per time point a list of topologies (type id, particle indices, edges as vertex pairs).
+        The layout of Topologies::append (Topologies.cpp:136-170): per topology its particle count then its
+        particles, and an edge row (edge count, 0) then its edges; limits are each frame's [begin, end)."""
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        if len(frames) != len(t):
+            raise ValueError("one frame per time point")
+        parts, edges, lp, le, types = [], [], [], [], []
+        for tops in frames:
+            p0, e0 = len(parts), len(edges)
+            for ty, idx, ed in tops:
+                parts.append(len(idx))
+                parts.extend(int(i) for i in idx)
+                edges.append((len(ed), 0))
+                edges.extend((int(a), int(b)) for a, b in ed)
+            lp.append((p0, len(parts)))
+            le.append((e0, len(edges)))
+            types.append(np.array([ty for ty, _, _ in tops], dtype=np.uint16))
+
+        def table(dataset, rows, cols):
+            a = np.ascontiguousarray(np.asarray(rows, dtype=np.uint64).reshape(-1, cols) if len(rows) else
+                                     np.zeros((0, cols), np.uint64))
+            self._check(lib().rdy_h5_u64_table(self.f, name.encode(), dataset.encode(), a.shape[0], cols,
+                                               _p(a, _u64p)), f"{name}/{dataset}")
+        table("particles", parts, 1)
+        table("limitsParticles", lp, 2)
+        table("edges", edges, 2)
+        table("limitsEdges", le, 2)
+        self._vlen(name, "types", 1, time, types, True)
 
     def flat_trajectory(self, name, time, frames):
         """frames: one (ids, type ids, flavors, positions [n][3]) tuple per entry of time."""
@@ -315,6 +349,30 @@ def read_vlen(path, dataset, elem):
     ends = np.cumsum(sizes[:nf.value].astype(np.int64))
     frames = [out[b - int(s):b] for b, s in zip(ends, sizes[:nf.value])]
     return frames if width == 3 else [f.ravel() for f in frames]
+
+
+def read_topologies(path, name):
+    """(time, frames) of /readdy/observables/<name>, frames as written by File.topologies."""
+    g = f"/readdy/observables/{name}/"
+    t = read_u64(path, g + "time")
+    parts = read_u64(path, g + "particles")
+    lp = read_u64(path, g + "limitsParticles").reshape(-1, 2)
+    edges = read_u64(path, g + "edges").reshape(-1, 2)
+    le = read_u64(path, g + "limitsEdges").reshape(-1, 2)
+    types = read_vlen(path, g + "types", 1)
+    frames = []
+    for k in range(len(t)):
+        tops, p, e = [], int(lp[k, 0]), int(le[k, 0])
+        for ty in types[k]:
+            n = int(parts[p])
+            idx = parts[p + 1:p + 1 + n]
+            ne = int(edges[e, 0])
+            tops.append((int(ty), idx, [tuple(int(v) for v in row) for row in edges[e + 1:e + 1 + ne]]))
+            p += 1 + n
+            e += 1 + ne
+        assert p == lp[k, 1] and e == le[k, 1]
+        frames.append(tops)
+    return t, frames
 
 
 def read_trajectory(path, name):

@@ -40,6 +40,8 @@
 //                          const uint64_t *frame_sizes, const void *data, const uint64_t *time);
 //       elem 0: {x, y, z} doubles, 1: uint16, 2: uint64; the frames' elements back to back in data;
 //       time [T] is written beside the dataset when not null
+//   int rdy_h5_u64_table(int64_t f, const char *name, const char *dataset, uint64_t rows, uint32_t cols,
+//                        const uint64_t *data);        /readdy/observables/<name>/<dataset>, [rows] or [rows][cols]
 //   int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64_t *time,
 //                              const uint64_t *frame_sizes, const uint64_t *ids, const uint16_t *types,
 //                              const uint8_t *flavors, const double *pos);          pos [R][3]
@@ -282,6 +284,16 @@ int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t
     int rc = write_f64(g, "bin_centers", 1, dc, centers);
     if (rc == 0) rc = write_f64(g, "distribution", 2, dd, dist);
     if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_u64_table(int64_t f, const char *name, const char *dataset, uint64_t rows, uint32_t cols,
+                     const uint64_t *data) {
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    hsize_t d[2] = {rows, cols};
+    const int rc = write_u64(g, dataset, cols > 1 ? 2 : 1, d, data);
     H5Gclose(g);
     return rc;
 }

@@ -168,3 +168,24 @@ def test_positions_forces_particles_round_trip(tmp_path):
         flat = "".join(out.split())
         assert 'H5T_VLEN{H5T_COMPOUND{H5T_IEEE_F64LE"x";H5T_IEEE_F64LE"y";H5T_IEEE_F64LE"z";}}' in flat
         assert 'H5T_VLEN{H5T_STD_U16LE}' in flat and 'H5T_VLEN{H5T_STD_U64LE}' in flat
+
+
+def test_topologies_observable_round_trip(tmp_path):
+    """Topologies::append (Topologies.cpp:136-170): particles with a count prefix per topology, edge rows
+    with an (edge count, 0) prefix, [begin, end) limits per frame, the topology types per frame."""
+    frames = [[(0, [0, 1, 2], [(0, 1), (1, 2)])],
+              [(0, [0, 1], [(0, 1)]), (0, [2], [])],        # a broken bond: two topologies
+              []]                                           # no topologies at all
+    path = str(tmp_path / "tops.h5")
+    with h5io.File(path) as f:
+        f.topologies("topologies", [0, 10, 20], frames)
+    t, got = h5io.read_topologies(path, "topologies")
+    assert list(t) == [0, 10, 20]
+    for want, have in zip(frames, got):
+        assert [(ty, list(idx), ed) for ty, idx, ed in want] == [(ty, list(idx), ed) for ty, idx, ed in have]
+    g = "/readdy/observables/topologies/"
+    assert list(h5io.read_u64(path, g + "particles")) == [3, 0, 1, 2, 2, 0, 1, 1, 2]
+    assert h5io.read_u64(path, g + "limitsParticles").reshape(-1, 2).tolist() == [[0, 4], [4, 9], [9, 9]]
+    assert h5io.read_u64(path, g + "edges").reshape(-1, 2).tolist() == [[2, 0], [0, 1], [1, 2], [1, 0], [0, 1],
+                                                                          [0, 0]]
+    assert h5io.read_u64(path, g + "limitsEdges").reshape(-1, 2).tolist() == [[0, 3], [3, 6], [6, 6]]
