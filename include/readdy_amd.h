@@ -28,7 +28,8 @@ extern "C" {
 #define RDY_ERR_HIP (-2)      /* HIP runtime error                                               */
 #define RDY_ERR_CAPACITY (-3) /* a device pool (cells, bins, events, particles) is exhausted     */
 #define RDY_ERR_STATE (-4)    /* call out of order (std::logic_error in the reference)           */
-#define RDY_ERR_NUMERIC (-5)  /* non-finite position produced by the integrator                  */
+#define RDY_ERR_NUMERIC (-5)  /* non-finite position, or a force beyond the fixed-point range of the
+                                 force sums (1.68e7 per pair or particle)                            */
 
 /* pair potentials: include/readdy/model/potentials/PotentialsOrder2.h:54-323 */
 #define RDY_POT_HARMONIC_REPULSION 0       /* params: force_constant, interaction_distance          */
@@ -144,6 +145,14 @@ int rdy_update_neighbor_list(rdy_engine *e);
 int rdy_calculate_forces(rdy_engine *e);
 /* total potential energy per replica of the last force evaluation (StateModel::energy) */
 int rdy_get_energy(rdy_engine *e, uint32_t replica, double *energy);
+/* Context::recordVirial (include/readdy/model/Context.h:168-173): on = 1 makes every force evaluation
+ * also form the virial of the pair potentials, as CPUCalculateForces::calculateOrder2 does
+ * (kernels/cpu/src/actions/CPUCalculateForces.cpp:214-268): sum over unordered pairs of -x_ij (x) f_i,
+ * x_ij = shortestDifference(x_i, x_j), f_i the pair force on i.  Off by default. */
+int rdy_record_virial(rdy_engine *e, int on);
+/* the virial of the replica's last force evaluation, row-major 3x3 (StateModel::virial, read by the
+ * Virial observable, kernels/cpu/src/observables/CPUObservables.cpp:241-245) */
+int rdy_get_virial(rdy_engine *e, uint32_t replica, double *virial9);
 /* CPUEulerBDIntegrator::perform (kernels/cpu/src/actions/CPUEulerBDIntegrator.cpp:51-103).
  * noise: NULL for the Philox stream, else a device-independent host array [max_handle][3]
  * of N(0,1) draws indexed by handle (single-replica engines only; injected-noise parity). */

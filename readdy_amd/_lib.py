@@ -83,6 +83,8 @@ SIGNATURES = {
     "rdy_update_neighbor_list": (_int, [_vp]),
     "rdy_calculate_forces": (_int, [_vp]),
     "rdy_get_energy": (_int, [_vp, _u32, _dp]),
+    "rdy_record_virial": (_int, [_vp, _int]),
+    "rdy_get_virial": (_int, [_vp, _u32, _dp]),
     "rdy_integrate": (_int, [_vp, _d, _dp, _u64]),
     "rdy_react": (_int, [_vp, _int, _d]),
     "rdy_get_reaction_counts": (_int, [_vp, _u32, _u64p]),

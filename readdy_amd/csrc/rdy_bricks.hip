@@ -685,7 +685,7 @@ __device__ __forceinline__ void frame_min_image(const Frame &F, float &dx, float
 // the pair sum reading neighbours from the bins (slow path for bricks that do not fit in LDS)
 __device__ __forceinline__ void global_forces(const Grid &cur, const RepGeom &geo, uint32_t cap, int radius,
                                               const Tables &T, uint32_t s, const State &me, long long &fx,
-                                              long long &fy, long long &fz, long long &e) {
+                                              long long &fy, long long &fz, long long &e, long long *vir = nullptr) {
     const PairIdx *prow = T.pidx + (size_t)tw_type(me.tw) * T.n_types;
     const uint32_t cl = cell_of_slot(cur, s, cap) - geo.cell_base;
     for_each_stencil_slot(geo, cur, cap, radius, cl, [&](uint32_t s2) {
@@ -704,6 +704,7 @@ __device__ __forceinline__ void global_forces(const Grid &cur, const RepGeom &ge
         fy += py;
         fz += pz;
         e += pe;
+        if (vir) pair_virial(T.pots, pi, dx, dy, dz, d2, vir);
     });
 }
 
@@ -719,6 +720,24 @@ __device__ __forceinline__ void block_add_energy(ReplicaDev *reps, uint32_t r, l
         if (t != 0) atomicAdd(&reps[r].energy, (unsigned long long)t);
     }
     __syncthreads();  // s_e may be written again by a following call
+}
+
+// block-wide sums of the nine fixed-point virial components, one atomic each per block
+__device__ __forceinline__ void block_add_virial(unsigned long long *virial, uint32_t r, const long long v[9]) {
+    __shared__ long long s_v[9][BRICK_THREADS / 64];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        long long x = v[c];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (__lane_id() == 0) s_v[c][threadIdx.x / 64] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        long long t = 0;
+        for (int w = 0; w < BRICK_THREADS / 64; ++w) t += s_v[threadIdx.x][w];
+        if (t != 0) atomicAdd(&virial[9ull * r + threadIdx.x], (unsigned long long)t);
+    }
+    __syncthreads();
 }
 
 // one particle: bonded force, optional force output, integration and re-binning
@@ -737,6 +756,8 @@ __device__ __forceinline__ void finish_particle(const Grid &nxt, const ReplicaDe
         fb[1] = 0;
         fb[2] = 0;
     }
+    if (do_forces && (fixed_saturated(fx) || fixed_saturated(fy) || fixed_saturated(fz)))
+        atomicAdd(&err[ERR_FORCE_RANGE], 1u);
     const double Fx = (double)fx * FORCE_INV, Fy = (double)fy * FORCE_INV, Fz = (double)fz * FORCE_INV;
     if (MODE != MODE_REBIN && force_out && do_forces) {
         force_out[3ull * s] = Fx;
@@ -845,13 +866,15 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                                                          int radius, double dt,
                                                          const double *noise, double *force_out, uint32_t max_states,
                                                          uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
-                                                         uint32_t *err) {
+                                                         uint32_t *err, unsigned long long *virial) {
     extern __shared__ __align__(16) char smem[];
     __shared__ ForceWave wave_lds[BRICK_WAVES];
     const uint32_t bidx = brick0 + xcd_block(blockIdx.x, gridDim.x);
     const BrickDev B = bricks[bidx];
     const ReplicaDev &R = reps[B.r];
     if (!R.in_use) return;
+    if (MODE != MODE_FORCES) virial = nullptr;
+    long long vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const RepGeom geo = load_geom(R);
     const bool forces_on = MODE == MODE_FORCES || (MODE == MODE_STEP && R.active);
     if (!forces_on) {
@@ -901,10 +924,12 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
             const double d2 = dist2(ddx, ddy, ddz);
             long long px, py, pz, pe;
             if (!pair_fixed(ft.pots, pi, ddx, ddy, ddz, d2, px, py, pz, pe)) return;
+            if (fixed_saturated(px) || fixed_saturated(py) || fixed_saturated(pz)) atomicAdd(&err[ERR_FORCE_RANGE], 1u);
             atomicAdd(&W.acc[0][ol], (unsigned long long)px);
             atomicAdd(&W.acc[1][ol], (unsigned long long)py);
             atomicAdd(&W.acc[2][ol], (unsigned long long)pz);
             e_blk += pe;
+            if (virial) pair_virial(ft.pots, pi, ddx, ddy, ddz, d2, vir);
         };
         auto rounds = [&]() {
             for (uint32_t base = threadIdx.x & ~63u; base < n_inner; base += BRICK_THREADS) {  // wave-uniform
@@ -953,6 +978,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
         };
         rounds();
         block_add_energy(reps, B.r, e_blk);
+        if (virial) block_add_virial(virial, B.r, vir);
     } else {
         list_slow_brick(slow, bidx);
     }
@@ -963,22 +989,27 @@ template <int MODE>
 __global__ __launch_bounds__(BRICK_THREADS) void k_brick_slow(Grid cur, Grid nxt, ReplicaDev *reps, Tables T,
                                                               Topo topo, const BrickDev *bricks, uint32_t cap,
                                                               int radius, double dt, const double *noise,
-                                                              double *force_out, uint32_t *slow, uint32_t *err) {
+                                                              double *force_out, uint32_t *slow, uint32_t *err,
+                                                              unsigned long long *virial) {
     const uint32_t n = slow_count(slow);
     if (n == 0) return;  // grid-uniform
+    if (MODE != MODE_FORCES) virial = nullptr;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {  // block-uniform
         const BrickDev B = bricks[slow[2 + i]];
         const ReplicaDev &R = reps[B.r];
         const RepGeom geo = load_geom(R);
         long long e_blk = 0;
+        long long vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for_each_own_record(cur, B, geo, cap, [&](uint32_t s, const State &me) {
             if (me.tw & TW_DEAD) return;
             long long fx = 0, fy = 0, fz = 0, e = 0;
-            if (!(me.tw & TW_OUTSIDE)) global_forces(cur, geo, cap, radius, T, s, me, fx, fy, fz, e);
+            if (!(me.tw & TW_OUTSIDE))
+                global_forces(cur, geo, cap, radius, T, s, me, fx, fy, fz, e, virial ? vir : nullptr);
             e_blk += e;
             finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, s, me, true, fx, fy, fz, err);
         });
         block_add_energy(reps, B.r, e_blk);
+        if (virial) block_add_virial(virial, B.r, vir);
     }
     release_slow_list(slow);
 }

@@ -266,12 +266,20 @@ constexpr double FORCE_INV = 1.0 / 4294967296.0;
 constexpr double ENERGY_SCALE = 16777216.0;        // 2^24
 constexpr double ENERGY_INV = 1.0 / 16777216.0;
 
+// Each fixed-point term is clamped to +-2^56 (a pair force of 1.68e7 at 2^-32), so that sums of up to
+// 128 terms never wrap in int64; a clamped term (or a NaN, which the clamp turns into -2^56) is
+// detected by fixed_saturated and reported as a numeric error instead of being summed silently.
+constexpr float FIXED_LIM = 72057594037927936.0f;  // 2^56
+constexpr long long FIXED_LIM_I = 72057594037927936ll;
+
 __device__ __forceinline__ long long to_fixed(float v, float scale) {
     float s = v * scale;
-    const float lim = 9.0e18f;
-    s = fminf(fmaxf(s, -lim), lim);
+    s = fminf(fmaxf(s, -FIXED_LIM), FIXED_LIM);
     return (long long)s;
 }
+
+// a fixed-point force term or sum at the clamp (a pair force beyond the range)
+__device__ __forceinline__ bool fixed_saturated(long long v) { return v >= FIXED_LIM_I || v <= -FIXED_LIM_I; }
 
 // The force on a particle from one neighbour, summed over the potentials of their ordered type
 // pair, in 2^-32 fixed point (fx, fy, fz), and the pair energy in 2^-24 fixed point (e).  dx, dy, dz
@@ -298,9 +306,30 @@ __device__ __forceinline__ bool pair_fixed(const PotDev *pots, const PairIdx &pi
     return true;
 }
 
+// The virial of one pair as CPUCalculateForces.cpp:247-249 forms it, -x (x) f, with x the displacement
+// neighbour - particle and f the force on the particle summed over the pair's potentials (FP32 pair math,
+// as pair_fixed), each of the nine products added to v in 2^-24 fixed point.  Seen from the other side
+// x and f both change sign exactly, so the product is the same integers: summing both sides and halving
+// counts every unordered pair once.
+__device__ __forceinline__ void pair_virial(const PotDev *pots, const PairIdx &pi, double dx, double dy, double dz,
+                                            double d2, long long v[9]) {
+    if (!(d2 < pi.maxcut2)) return;
+    float f[3] = {0.f, 0.f, 0.f};
+    const float x[3] = {(float)dx, (float)dy, (float)dz};
+    const float d2f = (float)d2;
+    for (uint32_t p = 0; p < pi.count; ++p) {
+        const PotDev pot = pots[pi.start + p];
+        if (d2 < pot.cut2) (void)pair_force_energy(pot, d2f, x[0], x[1], x[2], f[0], f[1], f[2]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) v[3 * a + b] += to_fixed(-x[a] * f[b], 16777216.0f);
+}
+
 __device__ __forceinline__ long long to_fixed_d(double v, double scale) {
     double s = v * scale;
-    const double lim = 9.0e18;
+    const double lim = 72057594037927936.0;  // 2^56, as to_fixed
     s = fmin(fmax(s, -lim), lim);
     return (long long)s;
 }

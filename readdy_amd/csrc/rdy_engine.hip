@@ -102,6 +102,9 @@ struct rdy_engine {
     bool recording = false;
     uint32_t break_calls = 0;  // Philox counter of rdy_break_bonds
     unsigned long long *rx_step = nullptr, *rx_total = nullptr, *e_bonded = nullptr;
+    unsigned long long *virial = nullptr;  // [max_reps][9] 2^-24 fixed point, both sides of every pair
+    bool record_virial = false;            // Context::recordVirial (Context.h:168-173)
+    bool virial_valid = false;             // the last force evaluation formed the virial
     uint32_t *err = nullptr;
     double *force_out = nullptr;
     // fused step (rdy_step.hip): the walk's per-brick force rows (2^-32 fixed point), the records
@@ -134,6 +137,7 @@ struct rdy_engine {
     long long *fb = nullptr;
     uint32_t *loc = nullptr;
     uint64_t topo_cap = 0, topo_used = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> free_topo;  // released (start, length) ranges of loc / fb
     std::vector<BondTerm> terms_h[3];
     BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
     uint64_t d_terms_cap[3] = {0, 0, 0};
@@ -198,6 +202,10 @@ int check_errors(rdy_engine *e) {
     if (!any) return RDY_OK;
     HIP_TRY(hipMemsetAsync(e->err, 0, sizeof(uint32_t) * N_ERR, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (h[ERR_FORCE_RANGE])
+        return set_err(RDY_ERR_NUMERIC, "a force beyond the fixed-point range (|F| >= 1.68e7 per pair or particle, " +
+                                            std::to_string(h[ERR_FORCE_RANGE]) +
+                                            " terms): particles nearly coincide; reduce the time step");
     if (h[ERR_NONFINITE])
         return set_err(RDY_ERR_NUMERIC, "integrator produced non-finite positions (" + std::to_string(h[ERR_NONFINITE]) +
                                             " components); reduce the time step");
@@ -384,6 +392,78 @@ int upload_tables(rdy_engine *e, double dt) {
     return RDY_OK;
 }
 
+// returns [start, start + len) to a free list of ranges, merging neighbours
+void release_range(std::vector<std::pair<uint64_t, uint64_t>> &fl, uint64_t start, uint64_t len) {
+    if (len == 0) return;
+    fl.push_back({start, len});
+    std::sort(fl.begin(), fl.end());
+    std::vector<std::pair<uint64_t, uint64_t>> merged;
+    for (auto &fc : fl) {
+        if (!merged.empty() && merged.back().first + merged.back().second == fc.first) merged.back().second += fc.second;
+        else merged.push_back(fc);
+    }
+    fl.swap(merged);
+}
+
+// first-fit range of len from a free list; false if none is long enough
+bool take_range(std::vector<std::pair<uint64_t, uint64_t>> &fl, uint64_t len, uint64_t *start) {
+    size_t f = 0;
+    while (f < fl.size() && fl[f].second < len) ++f;
+    if (f == fl.size()) return false;
+    *start = fl[f].first;
+    fl[f].first += len;
+    fl[f].second -= len;
+    if (fl[f].second == 0) fl.erase(fl.begin() + f);
+    return true;
+}
+
+// uploads the host list of bonded terms of one kind (device array grown as needed)
+int upload_terms(rdy_engine *e, int kind) {
+    const std::vector<BondTerm> &v = e->terms_h[kind];
+    if (v.size() > e->d_terms_cap[kind]) {
+        const uint64_t cap = std::max<uint64_t>(v.size(), 2 * e->d_terms_cap[kind]);
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        if (e->d_terms[kind]) HIP_TRY(hipFree(e->d_terms[kind]));
+        e->d_terms[kind] = nullptr;
+        e->d_terms_cap[kind] = 0;
+        HIP_TRY(dalloc(&e->d_terms[kind], cap));
+        e->d_terms_cap[kind] = cap;
+    }
+    if (!v.empty())
+        HIP_TRY(hipMemcpyAsync(e->d_terms[kind], v.data(), v.size() * sizeof(BondTerm), hipMemcpyHostToDevice,
+                               e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->version++;
+    return RDY_OK;
+}
+
+// removes every bonded term of replica r (host lists and device arrays)
+int drop_terms(rdy_engine *e, uint32_t r) {
+    for (int kind = 0; kind < 3; ++kind) {
+        std::vector<BondTerm> &v = e->terms_h[kind];
+        const size_t n0 = v.size();
+        v.erase(std::remove_if(v.begin(), v.end(), [r](const BondTerm &t) { return t.r == r; }), v.end());
+        if (v.size() != n0) {
+            int rc = upload_terms(e, kind);
+            if (rc) return rc;
+        }
+    }
+    return RDY_OK;
+}
+
+// gives replica r's range of topology particle slots (loc / fb) back to the pool
+void release_topo(rdy_engine *e, HostReplica &H) {
+    if (H.n_topo == 0) return;
+    release_range(e->free_topo, H.topo_base, H.n_topo);
+    // a free range at the end of the used part shrinks it
+    if (!e->free_topo.empty() && e->free_topo.back().first + e->free_topo.back().second == e->topo_used) {
+        e->topo_used = e->free_topo.back().first;
+        e->free_topo.pop_back();
+    }
+    H.topo_base = 0;
+    H.n_topo = 0;
+}
+
 int alloc_pools(rdy_engine *e) {
     if (e->pools_ready) return RDY_OK;
     e->ovf_cap = (uint32_t)std::max<uint64_t>(4096, e->max_particles / 8);
@@ -414,6 +494,8 @@ int alloc_pools(rdy_engine *e) {
     HIP_TRY(hipMemsetAsync(e->rx_total, 0, (size_t)e->max_reps * nrx * 8, e->stream));
     HIP_TRY(dalloc(&e->e_bonded, e->max_reps));
     HIP_TRY(hipMemsetAsync(e->e_bonded, 0, e->max_reps * 8, e->stream));
+    HIP_TRY(dalloc(&e->virial, 9ull * e->max_reps));
+    HIP_TRY(hipMemsetAsync(e->virial, 0, 9ull * e->max_reps * 8, e->stream));
     HIP_TRY(dalloc(&e->err, N_ERR));
     HIP_TRY(hipMemsetAsync(e->err, 0, N_ERR * sizeof(uint32_t), e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -806,19 +888,21 @@ void launch_step(rdy_engine *e, int handler, double dt, int last) {
 
 void launch_forces(rdy_engine *e, double *force_out) {
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    unsigned long long *vir = e->record_virial ? e->virial : nullptr;
     hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 0, e->e_bonded,
-                       (uint32_t *)nullptr);
+                       (uint32_t *)nullptr, vir);
     launch_bonded(e);
     const int c = e->cur;
     for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
         hipLaunchKernelGGL((k_brick<MODE_FORCES, decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
                            e->stream, e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks,
                            b0, e->cap, e->radius, 0.0, (const double *)nullptr, force_out, e->max_states, e->max_halo,
-                           (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
+                           (uint32_t)e->brick_smem, slow_list(e, 0), e->err, vir);
     });
     hipLaunchKernelGGL(k_brick_slow<MODE_FORCES>, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream,
                        e->grid(c), e->grid(c), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap,
-                       e->radius, 0.0, (const double *)nullptr, force_out, slow_list(e, 0), e->err);
+                       e->radius, 0.0, (const double *)nullptr, force_out, slow_list(e, 0), e->err, vir);
+    e->virial_valid = e->record_virial;
 }
 
 // force_out and the energies after a fused run are formed when first asked for
@@ -959,7 +1043,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->wl[b]);
         (void)hipFree(e->ovf_cell[b]);
     }
-    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->err,
+    void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_rowmax, e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
@@ -1014,13 +1098,8 @@ int rdy_replica_add(rdy_engine *e, const rdy_replica_desc *d, uint32_t *replica_
     while (r < e->max_reps && e->reps_h[r].in_use) ++r;
     if (r == e->max_reps) return set_err(RDY_ERR_CAPACITY, "replica table full");
     // first-fit cell range
-    size_t f = 0;
-    while (f < e->free_cells.size() && e->free_cells[f].second < cells) ++f;
-    if (f == e->free_cells.size()) return set_err(RDY_ERR_CAPACITY, "cell pool exhausted");
-    uint64_t base = e->free_cells[f].first;
-    e->free_cells[f].first += cells;
-    e->free_cells[f].second -= cells;
-    if (e->free_cells[f].second == 0) e->free_cells.erase(e->free_cells.begin() + f);
+    uint64_t base = 0;
+    if (!take_range(e->free_cells, cells, &base)) return set_err(RDY_ERR_CAPACITY, "cell pool exhausted");
     HostReplica &H = e->reps_h[r];
     H = HostReplica{};
     H.in_use = true;
@@ -1072,14 +1151,10 @@ int rdy_replica_remove(rdy_engine *e, uint32_t r) {
     for (int b = 0; b < 2; ++b)
         hipLaunchKernelGGL(k_release_cells, dim3(blocks_for(H.n_cells, 256)), dim3(256), 0, e->stream, e->grid(b),
                            (uint32_t)H.cell_base, (uint32_t)H.n_cells);
-    e->free_cells.push_back({H.cell_base, H.n_cells});
-    std::sort(e->free_cells.begin(), e->free_cells.end());
-    std::vector<std::pair<uint64_t, uint64_t>> merged;
-    for (auto &fc : e->free_cells) {
-        if (!merged.empty() && merged.back().first + merged.back().second == fc.first) merged.back().second += fc.second;
-        else merged.push_back(fc);
-    }
-    e->free_cells.swap(merged);
+    release_range(e->free_cells, H.cell_base, H.n_cells);
+    // the replica's bonded terms and topology slots go with it (a later replica may reuse slot r)
+    if ((rc = drop_terms(e, r))) return rc;
+    release_topo(e, H);
     H.in_use = false;
     e->version++;
     e->wl_valid[0] = e->wl_valid[1] = false;
@@ -1112,7 +1187,14 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
                            (uint32_t)e->max_particles);
     }
     H.has_particles = true;
-    if (n_topo) {
+    // the bonded terms name the replaced particles' handles: the caller sets them again
+    if ((rc = drop_terms(e, r))) return rc;
+    release_topo(e, H);
+    uint64_t tbase = 0;
+    if (n_topo && take_range(e->free_topo, n_topo, &tbase)) {
+        H.topo_base = (uint32_t)tbase;
+        H.n_topo = n_topo;
+    } else if (n_topo) {
         if (e->topo_used + n_topo > e->topo_cap) {
             uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + n_topo);
             long long *fb;
@@ -1218,9 +1300,11 @@ static int set_terms(rdy_engine *e, uint32_t r, int kind, uint64_t n, const uint
                      const double *p0, const uint32_t *mult) {
     int rc = check_replica(e, r);
     if (rc) return rc;
+    if (n && (!idx || !k || !p0)) return set_err(RDY_ERR_INVALID, "null bonded term arrays");
     const HostReplica &H = e->reps_h[r];
-    std::vector<BondTerm> &v = e->terms_h[kind];
-    v.erase(std::remove_if(v.begin(), v.end(), [r](const BondTerm &t) { return t.r == r; }), v.end());
+    // every index is checked before anything changes: a rejected call leaves the terms as they were
+    std::vector<BondTerm> fresh;
+    fresh.reserve(n);
     for (uint64_t i = 0; i < n; ++i) {
         BondTerm t{};
         t.r = r;
@@ -1231,20 +1315,22 @@ static int set_terms(rdy_engine *e, uint32_t r, int kind, uint64_t n, const uint
         t.k = k[i];
         t.p0 = p0[i];
         t.mult = mult ? mult[i] : 0;
-        v.push_back(t);
+        fresh.push_back(t);
     }
-    if (v.size() > e->d_terms_cap[kind]) {
-        HIP_TRY(hipStreamSynchronize(e->stream));
-        if (e->d_terms[kind]) HIP_TRY(hipFree(e->d_terms[kind]));
-        e->d_terms_cap[kind] = std::max<uint64_t>(v.size(), 2 * e->d_terms_cap[kind]);
-        HIP_TRY(dalloc(&e->d_terms[kind], e->d_terms_cap[kind]));
+    if (kind == 0) {
+        // a bond is an unordered edge: (min, max) handles, and every configuration of one edge in one
+        // consecutive run (BreakBonds sums an edge's configurations, Actions.h:291-341); the harmonic
+        // bond's force and energy do not depend on the order of its ends
+        for (BondTerm &t : fresh)
+            if (t.h[0] > t.h[1]) std::swap(t.h[0], t.h[1]);
+        std::stable_sort(fresh.begin(), fresh.end(), [](const BondTerm &a, const BondTerm &b) {
+            return a.h[0] != b.h[0] ? a.h[0] < b.h[0] : a.h[1] < b.h[1];
+        });
     }
-    if (!v.empty())
-        HIP_TRY(hipMemcpyAsync(e->d_terms[kind], v.data(), v.size() * sizeof(BondTerm), hipMemcpyHostToDevice,
-                               e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    e->version++;
-    return RDY_OK;
+    std::vector<BondTerm> &v = e->terms_h[kind];
+    v.erase(std::remove_if(v.begin(), v.end(), [r](const BondTerm &t) { return t.r == r; }), v.end());
+    v.insert(v.end(), fresh.begin(), fresh.end());
+    return upload_terms(e, kind);
 }
 
 int rdy_set_bonds(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *idx2, const double *k, const double *length) {
@@ -1270,7 +1356,7 @@ int rdy_update_neighbor_list(rdy_engine *e) {
     hipLaunchKernelGGL((k_brick<MODE_REBIN, false>), dim3(n_bricks(e)), dim3(BRICK_THREADS), 0, e->stream,
                        e->grid(c), e->grid(n), e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, 0u, e->cap,
                        e->radius, 0.0, (const double *)nullptr, (double *)nullptr, e->max_states, e->max_halo, 0u,
-                       slow_list(e, 0), e->err);
+                       slow_list(e, 0), e->err, (unsigned long long *)nullptr);
     launch_clear_bricks(e, c, n);
     e->wl_valid[n] = false;
     e->cur = n;
@@ -1301,6 +1387,30 @@ int rdy_get_energy(rdy_engine *e, uint32_t r, double *energy) {
     return RDY_OK;
 }
 
+int rdy_record_virial(rdy_engine *e, int on) {
+    if (!e) return set_err(RDY_ERR_INVALID, "null engine");
+    e->record_virial = on != 0;
+    if (e->record_virial && !e->virial_valid) e->export_valid = false;  // formed with the next export
+    return RDY_OK;
+}
+
+int rdy_get_virial(rdy_engine *e, uint32_t r, double *virial9) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (!virial9) return set_err(RDY_ERR_INVALID, "null virial array");
+    if (!e->record_virial) return set_err(RDY_ERR_STATE, "the virial is not being recorded (rdy_record_virial)");
+    if ((rc = ensure_export(e))) return rc;
+    if (!e->virial_valid) {  // switched on after the last force evaluation of a separate-action sequence
+        launch_forces(e, e->force_out);
+        if ((rc = check_errors(e))) return rc;
+    }
+    unsigned long long v[9];
+    HIP_TRY(hipMemcpyAsync(v, e->virial + 9ull * r, sizeof v, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    for (int c = 0; c < 9; ++c) virial9[c] = 0.5 * (double)(long long)v[c] * ENERGY_INV;
+    return RDY_OK;
+}
+
 int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_handles) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
     int rc = upload_tables(e, dt);
@@ -1324,7 +1434,7 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
     if ((rc = ensure_export(e))) return rc;
     ensure_wl(e);
     hipLaunchKernelGGL(k_advance, dim3(blocks_for(nr, 64)), dim3(64), 0, e->stream, e->reps, nr, 1, e->e_bonded,
-                       (uint32_t *)nullptr);
+                       (uint32_t *)nullptr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(k_integrate, dim3(wl_blocks(e)), dim3(e->block), 0, e->stream, e->grid(e->cur), e->reps, e->T,
                        e->cap, dt, dn, (const double *)e->force_out, (uint32_t)e->max_particles, e->err);
     return check_errors(e);

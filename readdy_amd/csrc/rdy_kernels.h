@@ -81,7 +81,7 @@ static_assert(sizeof(RxRecord) == 64, "RxRecord layout");
 
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
-             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, N_ERR = 8 };
+             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, ERR_FORCE_RANGE = 8, N_ERR = 9 };
 
 constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
 constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer

@@ -166,7 +166,7 @@ __device__ __forceinline__ uint32_t bin_insert(const Grid &g, const RepGeom &geo
 // mode 0: zero the energy accumulators (a force evaluation follows); mode 1: advance the Philox
 // step of replicas with budget left (separate integrator action); mode 2: both (fused step)
 __global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned long long *e_bonded,
-                          uint32_t *edit_count) {
+                          uint32_t *edit_count, unsigned long long *virial) {
     uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r == 0 && edit_count) *edit_count = 0u;
     if (r >= n_reps) return;
@@ -174,6 +174,8 @@ __global__ void k_advance(ReplicaDev *reps, uint32_t n_reps, int mode, unsigned 
     if (mode != 1) {
         R.energy = 0ull;
         e_bonded[r] = 0ull;
+        if (virial)
+            for (int c = 0; c < 9; ++c) virial[9ull * r + c] = 0ull;
     }
     if (mode >= 1) {
         if (R.in_use && R.steps_left > 0) {

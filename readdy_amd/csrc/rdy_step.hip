@@ -43,6 +43,7 @@ __device__ __forceinline__ uint32_t advance_particle(const Grid &nxt, const Repl
             fy += fb[1];
             fz += fb[2];
         }
+        if (fixed_saturated(fx) || fixed_saturated(fy) || fixed_saturated(fz)) atomicAdd(&err[ERR_FORCE_RANGE], 1u);
         const double Fv[3] = {(double)fx * FORCE_INV, (double)fy * FORCE_INV, (double)fz * FORCE_INV};
         const double D = T.diffusion[tw_type(me.tw)];
         double xi[3];
@@ -344,6 +345,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
         const double d2 = dist2(ddx, ddy, ddz);
         long long px, py, pz, pe;
         if (pair_fixed<false>(ft.pots, ft.pidx[t1 * nt + t2], ddx, ddy, ddz, d2, px, py, pz, pe)) {
+            if (fixed_saturated(px) || fixed_saturated(py) || fixed_saturated(pz)) atomicAdd(&err[ERR_FORCE_RANGE], 1u);
             atomicAdd(&W.acc[0][ol], (unsigned long long)px);
             atomicAdd(&W.acc[1][ol], (unsigned long long)py);
             atomicAdd(&W.acc[2][ol], (unsigned long long)pz);

@@ -214,6 +214,16 @@ class Engine:
         L.check(self._lib.rdy_get_energy(self._h, r, ctypes.byref(e)))
         return e.value
 
+    def record_virial(self, on=True):
+        """Context::recordVirial: later force evaluations also form the pair virial."""
+        L.check(self._lib.rdy_record_virial(self._h, 1 if on else 0))
+
+    def virial(self, r=0):
+        """3x3 virial of the last force evaluation: sum over unordered pairs of -x_ij (x) f_i."""
+        v = np.zeros(9)
+        L.check(self._lib.rdy_get_virial(self._h, r, _ptr(v, _dp)))
+        return v.reshape(3, 3)
+
     def integrate(self, dt, noise=None):
         if noise is None:
             L.check(self._lib.rdy_integrate(self._h, dt, None, 0))

@@ -109,3 +109,66 @@ def test_replicas_and_argument_errors():
         e.break_bonds(1.0, [(0, 0, 1.0, 0.0)])  # BreakConfig: the rate must be positive
     with pytest.raises(L.EngineError):
         e.break_bonds(1.0, [(0, 5, 1.0, 1.0)])  # no such type
+
+
+def test_reversed_and_scattered_configurations_are_one_edge():
+    """An edge listed as (a, b) and (b, a), not consecutively, is one edge: its two configurations add
+    up and it comes back once, as (replica, min, max)."""
+    rng = np.random.default_rng(5)
+    n = 300
+    pos, types, bonds = _dimers(n, rng, types=(0,))
+    length = np.linalg.norm(((pos[1::2] - pos[0::2]) + BOX / 2) % BOX - BOX / 2, axis=1)
+    l0 = length - np.sqrt(0.6)                                  # 0.6 per configuration (k = 1)
+    both = np.concatenate([bonds, bonds[:, ::-1]])              # every edge twice, the second reversed
+    e, (r,) = _engine(1, pos, types, both, 1.0, np.concatenate([l0, l0]))
+    got = e.break_bonds(1.0, [(0, 0, 1.0, 1e10)])
+    assert len(got) == n
+    assert {tuple(int(v) for v in row) for row in got} == {(r, int(i), int(j)) for i, j in bonds}
+
+
+def test_rejected_set_bonds_leaves_the_terms_unchanged():
+    """A set_bonds call with an index outside the topology particles is refused before anything changes:
+    the following force evaluation equals the one before it, bit for bit."""
+    rng = np.random.default_rng(6)
+    pos, types, bonds = _dimers(200, rng, types=(0,))
+    e, (r,) = _engine(1, pos, types, bonds, 5.0, 1.0)
+    e.calculate_forces()
+    f0, h0 = e.get_forces(r)
+    e0 = e.energy(r)
+    bad = bonds.copy()
+    bad[-1, 1] = 10 ** 6
+    with pytest.raises(L.EngineError):
+        e.set_bonds(r, bad, 5.0, 1.0)
+    e.calculate_forces()
+    f1, h1 = e.get_forces(r)
+    assert np.array_equal(h0, h1) and np.array_equal(f0, f1) and e.energy(r) == e0
+
+
+def test_removed_bonded_replica_leaves_no_terms_behind():
+    """Removing a bonded replica drops its bonded terms: a replica added into the freed slot without bonds
+    does not inherit them, and the forces and energies of the other replicas stay as they were."""
+    rng = np.random.default_rng(7)
+    pos, types, bonds = _dimers(300, rng, types=(0,))
+    e, rs = _engine(1, pos, types, bonds, 5.0, 1.0, replicas=2)
+    e.calculate_forces()
+    before = [e.get_forces(q) for q in rs]
+    en_before = [e.energy(q) for q in rs]
+    e.remove_replica(rs[1])
+    r_new = e.add_replica([BOX] * 3, (1, 1, 1), seed=99, max_particles=2 * len(pos))
+    assert r_new == rs[1]
+    e.set_particles(r_new, pos, types)  # same particles, no bonds
+    e.calculate_forces()
+    f0, h0 = e.get_forces(rs[0])
+    assert np.array_equal(h0, before[0][1]) and np.array_equal(f0, before[0][0]) and e.energy(rs[0]) == en_before[0]
+    f_new, _ = e.get_forces(r_new)
+    assert np.all(f_new == 0.0) and e.energy(r_new) == 0.0  # no pair potentials, no bonds
+    # the freed topology range is reused: add/remove cycles of bonded replicas leave replica 0 alone
+    e.remove_replica(r_new)
+    for c in range(10):
+        q = e.add_replica([BOX] * 3, (1, 1, 1), seed=c, max_particles=2 * len(pos))
+        e.set_particles(q, pos, types)
+        e.set_bonds(q, bonds, 5.0, 1.0)
+        e.remove_replica(q)
+    e.calculate_forces()
+    f0b, _ = e.get_forces(rs[0])
+    assert np.array_equal(f0b, before[0][0])

@@ -240,6 +240,101 @@ def test_free_diffusion_injected_noise_bit_exact():
     assert np.array_equal(p, op[np.argsort(ids)])
 
 
+def test_drift_with_forces_at_kbt_not_one():
+    """EulerBD with pair forces at kBT = 1.7 and unequal D, injected noise, five integrate / NL / forces
+    steps against the oracle.  A dropped 1/kBT or D, or a flipped drift, moves a particle by about
+    |F| D dt (1 - 1/kBT) ~ 1e-3 per step here, far above the 1e-7 the FP32 pair forces allow."""
+    n, Lb, kbt, dt = 1500, 12.0, 1.7, 0.01
+    D = [1.0, 0.4]
+    rng = np.random.default_rng(41)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.integers(0, 2, n).astype(np.uint32)
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(2) for b in range(a, 2)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    e, r = _engine(2, [Lb] * 3, (1, 1, 1), pots, diffusion=D, kbt=kbt, n=n)
+    e.set_particles(r, pos, types)
+    o = _oracle(2, [Lb] * 3, [1, 1, 1], pots, diffusion=D, kbt=kbt)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(n))
+    o.nl_setup(o.max_cutoff())
+    e.calculate_forces()
+    o.calculate_forces()
+    f_ref = _forces_sorted(o)
+    assert np.abs(f_ref).max() > 1.0  # the drift term is exercised
+    for t in range(5):
+        noise = rng.normal(size=(n, 3))
+        e.integrate(dt, noise)
+        e.update_neighbor_list()
+        e.calculate_forces()
+        o.euler_bd(dt, noise.ravel())
+        o.nl_fill()
+        o.calculate_forces()
+    p, _, _ = e.get_particles(r)
+    op, _, ids, _ = o.particles()
+    d = p - op[np.argsort(ids)]
+    d -= Lb * np.round(d / Lb)
+    err = np.abs(d).max()
+    print(f"drift at kBT={kbt}: max |dx| = {err:.2e}", flush=True)
+    assert err < 1e-7
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_virial_vs_oracle(fused):
+    """Context::recordVirial: the device virial (sum over pairs of -x_ij (x) f_i, CPUCalculateForces.cpp:
+    247-249) against the FP64 oracle's within 1e-5 relative (Frobenius), for a separate force evaluation
+    and for the state after fused steps (formed on demand)."""
+    n, Lb = 4000, 20.0
+    rng = np.random.default_rng(42)
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = rng.integers(0, 3, n).astype(np.uint32)
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    e, r = _engine(3, [Lb] * 3, (1, 1, 1), pots, n=n, seed=9)
+    e.set_particles(r, pos, types)
+    e.record_virial(True)
+    if fused:
+        e.run(L.HANDLER_NONE, 0.005, 4, graph=True)
+        pos, types, hs = e.get_particles(r)
+        order = np.argsort(hs)
+        pos, types = pos[order], types[order]
+    else:
+        e.calculate_forces()
+    w = e.virial(r)
+    o = _oracle(3, [Lb] * 3, [1, 1, 1], pots)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    o.nl_setup(o.max_cutoff())
+    _, w_ref = o.calculate_forces(virial=True)
+    rel = np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref)
+    print(f"virial rel err {rel:.2e}", flush=True)
+    assert rel < 1e-5 and np.linalg.norm(w_ref) > 1.0
+    e.record_virial(False)
+    with pytest.raises(L.EngineError):
+        e.virial(r)
+
+
+def test_force_beyond_fixed_point_range_is_a_numeric_error():
+    """A near-coincident Lennard-Jones pair (r = 0.002 sigma: |F| ~ 1e36) gives a force beyond the 2^-32
+    fixed-point range of the force sums.  The force action and the fused step report RDY_ERR_NUMERIC
+    (the reference would carry inf/NaN on), every kernel ends normally, and the engine keeps working."""
+    lj = (L.POT_LENNARD_JONES, 0, 0, [12.0, 6.0, 2.5, 1.0, 1.0, 1.0])
+    rng = np.random.default_rng(43)
+    n, Lb = 200, 12.0
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    pos[1] = pos[0] + np.array([0.002, 0.0, 0.0])
+    types = np.zeros(n, dtype=np.uint32)
+    e, r = _engine(1, [Lb] * 3, (1, 1, 1), [lj], n=n)
+    e.set_particles(r, pos, types)
+    with pytest.raises(L.EngineError, match="error -5"):
+        e.calculate_forces()
+    with pytest.raises(L.EngineError, match="error -5"):
+        e.run(L.HANDLER_NONE, 1e-6, 2)
+    # a clean state afterwards: the engine is usable, the forces finite and below the range
+    good = np.array([[x, y, z] for x in range(-5, 6, 2) for y in range(-5, 6, 2) for z in range(-5, 6, 2)], float)
+    e.set_particles(r, good, np.zeros(len(good), dtype=np.uint32))
+    e.calculate_forces()
+    f, _ = e.get_forces(r)
+    assert np.all(np.isfinite(f)) and len(f) == len(good)
+
+
 def test_philox_noise_matches_oracle_stream():
     """Philox-driven BD: identical integer draws; positions agree to transcendental rounding."""
     n, Lb = 500, 10.0

@@ -292,6 +292,87 @@ def test_euler_bd_closed_form_with_injected_noise():
         np.testing.assert_array_equal(got[list(ids).index(i)], want)
 
 
+def test_euler_bd_drift_term_at_kbt_not_one():
+    """x += sqrt(2 D dt) xi + F D dt / kBT (CPUEulerBDIntegrator.cpp:63-73) with a nonzero force at kBT = 1.7.
+
+    The force is the harmonic repulsion's closed form k (r - d) / r * x_ij (PotentialsOrder2.h:83-91),
+    written out here, so a dropped 1/kBT, a flipped drift sign or a missing D in the drift fails."""
+    box, pbc = [10.0, 10.0, 10.0], [1, 1, 1]
+    kbt, D, dt, k, d = 1.7, [1.0, 0.4], 0.01, 10.0, 1.0
+    s = po.System(2, box, pbc, kbt=kbt, diffusion=D)
+    pos = np.array([[0.1, 0.2, 0.3], [0.5, -0.1, 0.55]])  # 0.56 apart: inside d = 1
+    s.add_particles(pos, [0, 1], ids=[0, 1])
+    s.add_pair_potential(po.POT_HARMONIC_REPULSION, 0, 1, [k, d])
+    s.nl_setup(1.0)
+    s.calculate_forces()
+    x01 = pos[1] - pos[0]
+    r = math.sqrt(x01 @ x01)
+    f0 = k * (r - d) / r * x01  # force on particle 0 (pushed away from 1)
+    forces = np.array([f0, -f0])
+    assert f0[0] < 0  # repulsive: particle 0 moves away from particle 1 along x
+    noise = np.array([[0.3, -1.1, 0.7], [-0.4, 0.9, 0.2]])
+    s.euler_bd(dt, noise.ravel())
+    got, _, ids, _ = s.particles()
+    for i in range(2):
+        want = pos[i] + noise[i] * math.sqrt(2 * D[i] * dt) + forces[i] * D[i] * dt / kbt
+        np.testing.assert_allclose(got[list(ids).index(i)], want, rtol=0, atol=1e-14)
+        wrong = pos[i] + noise[i] * math.sqrt(2 * D[i] * dt) + forces[i] * D[i] * dt  # without 1/kBT
+        assert np.abs(got[list(ids).index(i)] - wrong).max() > 1e-3
+
+
+def test_virial_known_answer_sign():
+    """Virial -x_ij (x) f_i per pair (CPUCalculateForces.cpp:247-249): two particles 1.8 apart along z under
+    harmonic repulsion k = 1, d = 2 (the scene of TestStateModel.cpp:57-100: F = -0.2 z on the first)
+    give W_zz = -(1.8)(-0.2) = +0.36 and zero elsewhere, counted once for the pair."""
+    s = po.System(1, [10.0] * 3, [1, 1, 1])
+    s.add_pair_potential(po.POT_HARMONIC_REPULSION, 0, 0, [1.0, 2.0])
+    s.add_particles([[0.0, 0.0, 0.0], [0.0, 0.0, 1.8]], [0, 0], ids=[0, 1])
+    s.nl_setup(s.max_cutoff())
+    e, w = s.calculate_forces(virial=True)
+    want = np.zeros((3, 3))
+    want[2, 2] = 0.36
+    np.testing.assert_allclose(w, want, rtol=1e-12, atol=1e-15)
+    assert e == pytest.approx(0.02, rel=1e-12)
+
+
+def test_virial_equals_pair_sum_brute_force():
+    """The oracle's virial over a periodic fluid equals sum over unordered pairs of -x_ij (x) f_i, formed here
+    from all O(N^2) minimum-image pairs with the pinned pair forces: a double count, a dropped off-diagonal
+    term or a transposed product (the WIPH pieces make x and f non-parallel only through summation, so the
+    check also compares the full matrix) fails."""
+    rng = np.random.default_rng(11)
+    box, pbc = [6.0, 7.0, 8.0], [1, 1, 1]
+    n = 120
+    pos = (rng.random((n, 3)) - 0.5) * np.array(box)
+    types = rng.integers(0, 2, n)
+    hr, wi = [10.0, 1.0], [10.0, 1.0, 0.5, 2.0]
+    s = po.System(2, box, pbc)
+    for a in range(2):
+        for b in range(a, 2):
+            s.add_pair_potential(po.POT_HARMONIC_REPULSION, a, b, hr)
+    s.add_pair_potential(po.POT_WEAK_INTERACTION, 0, 0, wi)
+    s.add_particles(pos, types, ids=np.arange(n))
+    s.nl_setup(s.max_cutoff())
+    _, w = s.calculate_forces(virial=True)
+    W = np.zeros((3, 3))
+    pairs = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            x = po.shortest_difference(pos[i], pos[j], box, pbc)
+            d2 = x @ x
+            pots = [(po.POT_HARMONIC_REPULSION, hr, 1.0)]
+            if types[i] == 0 and types[j] == 0:
+                pots.append((po.POT_WEAK_INTERACTION, wi, 2.0))
+            for kind, prm, rc in pots:
+                if d2 < rc * rc:
+                    fi, _ = po.pair_force_energy(kind, prm, x)
+                    W += np.outer(-x, fi)
+                    pairs += 1
+    assert pairs > 100
+    np.testing.assert_allclose(w, W, rtol=1e-10, atol=1e-10)
+    assert np.abs(W).max() > 1.0
+
+
 def test_fusion_and_fission_placement():
     g = GOLDEN["fusion_placement"]
     s = po.System(3, g["box"], g["pbc"])
