@@ -242,6 +242,16 @@ int rdy_rdf_counts(rdy_engine *e, uint32_t replica, const double *borders, uint3
 int rdy_histogram_along_axis(rdy_engine *e, uint32_t replica, const double *borders, uint32_t n_borders,
                              const uint32_t *types, uint32_t n_types, uint32_t axis, uint64_t *counts);
 
+/* The same histogram as rdy_histogram_along_axis, computed without allocating: no hipMalloc, hipFree or
+ * other allocating routine is called.  No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (HIP graph capture around it, tight serving loops, harnesses that forbid it).
+ * Its workspace (borders, species mask, counts: 66,048 bytes) is a static device array, one per GPU, held
+ * until the process ends; 2 <= n_borders <= 4097.  Every call on one GPU shares that GPU's workspace and
+ * leaves its contents behind (the counts are cleared at the start of each call): calls that overlap on
+ * one GPU (several host threads, or several streams) race and must be serialised by the caller. */
+int rdy_histogram_along_axis_noalloc(rdy_engine *e, uint32_t replica, const double *borders, uint32_t n_borders,
+                                     const uint32_t *types, uint32_t n_types, uint32_t axis, uint64_t *counts);
+
 /* kernel timing of the dominant (fused force/integrate/rebin) kernel, measured with HIP
  * events on the engine's stream: average microseconds over the launches since the last reset */
 int rdy_profile_reset(rdy_engine *e);

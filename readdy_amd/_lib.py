@@ -103,6 +103,7 @@ SIGNATURES = {
     "rdy_count_types": (_int, [_vp, _u64p, _u32]),
     "rdy_rdf_counts": (_int, [_vp, _u32, _dp, _u32, _u32p, _u32, _u32p, _u32, _u64p, _u64p]),
     "rdy_histogram_along_axis": (_int, [_vp, _u32, _dp, _u32, _u32p, _u32, _u32, _u64p]),
+    "rdy_histogram_along_axis_noalloc": (_int, [_vp, _u32, _dp, _u32, _u32p, _u32, _u32, _u64p]),
     "rdy_profile_reset": (_int, [_vp]),
     "rdy_profile_get": (_int, [_vp, _dp, _u64p]),
 }

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -43,6 +44,41 @@ hipError_t dalloc(T **p, size_t n) {
     *p = nullptr;
     if (n == 0) n = 1;
     return hipMalloc((void **)p, n * sizeof(T));
+}
+
+// Workspace of rdy_histogram_along_axis_noalloc, one per GPU for the life of the process: the bin
+// borders (up to RDF_MAX_BORDERS doubles), the species mask (MAX_TYPES bits) and the counts of one
+// call.  Host code never passes the array's name to a kernel or a copy: it uses the device address
+// that hipGetSymbolAddress returns for the current GPU (g_noalloc_ptr), taken once per device when the
+// first engine is created there.
+constexpr size_t NOALLOC_OFF_BORDERS = 0;
+constexpr size_t NOALLOC_OFF_MASK = 33024;   // 8 * 4097 rounded up to 256
+constexpr size_t NOALLOC_OFF_COUNTS = 33280; // + 4 * (MAX_TYPES / 32), rounded up to 256
+constexpr size_t NOALLOC_BYTES = 66048;      // + 8 * (RDF_MAX_BORDERS - 1)
+static_assert(NOALLOC_OFF_MASK >= 8 * RDF_MAX_BORDERS && NOALLOC_OFF_COUNTS >= NOALLOC_OFF_MASK + 4 * (MAX_TYPES / 32) &&
+                  NOALLOC_BYTES >= NOALLOC_OFF_COUNTS + 8 * (RDF_MAX_BORDERS - 1),
+              "no-alloc workspace layout");
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// the current device's address of the workspace (device must be current)
+int init_noalloc_workspace(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return set_err(RDY_ERR_INVALID, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return RDY_OK;
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_err(RDY_ERR_HIP, "hipGetSymbolAddress returned no address for the workspace");
+    g_noalloc_ptr[device] = p;
+    return RDY_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 }  // namespace
@@ -986,8 +1022,11 @@ int rdy_engine_create(const rdy_context_desc *ctx, const rdy_engine_config *cfg,
         }
     }
     if (ctx->n_reactions > 255) return set_err(RDY_ERR_INVALID, "at most 255 reactions");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (int rw = init_noalloc_workspace(dev)) return rw;
     rdy_engine *e = new rdy_engine();
-    HIP_TRY(hipGetDevice(&e->device));
+    e->device = dev;
     HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     e->n_types = ctx->n_types;
     e->kbt = ctx->kbt;
@@ -1838,6 +1877,40 @@ int rdy_histogram_along_axis(rdy_engine *e, uint32_t r, const double *borders, u
                        e->grid(e->cur), r, (const double *)base, n_borders, (const uint32_t *)(base + off_m),
                        (int)axis, (unsigned long long *)(base + off_c), (uint32_t)e->max_particles);
     HIP_TRY(hipMemcpyAsync(counts, base + off_c, 8 * nbins, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return check_errors(e);
+}
+
+int rdy_histogram_along_axis_noalloc(rdy_engine *e, uint32_t r, const double *borders, uint32_t n_borders,
+                                     const uint32_t *types, uint32_t n_types, uint32_t axis, uint64_t *counts) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (!borders || !counts || axis > 2) return set_err(RDY_ERR_INVALID, "histogram: bin borders, an axis 0..2 and an output array required");
+    if (n_borders < 2 || n_borders > RDF_MAX_BORDERS)
+        return set_err(RDY_ERR_INVALID, "histogram (no-alloc): 2 to 4097 bin borders (the workspace holds at most "
+                                        "4097 borders and 4096 counts)");
+    for (uint32_t b = 0; b + 1 < n_borders; ++b)
+        if (!(borders[b] <= borders[b + 1])) return set_err(RDY_ERR_INVALID, "bin borders must ascend");
+    uint32_t mask[MAX_TYPES / 32] = {};
+    const uint32_t words = (e->n_types + 31) / 32;
+    for (uint32_t k = 0; k < n_types; ++k) {
+        if (!types || types[k] >= e->n_types) return set_err(RDY_ERR_INVALID, "histogram: no such species");
+        mask[types[k] >> 5] |= 1u << (types[k] & 31u);
+    }
+    char *base = (char *)g_noalloc_ptr[e->device];
+    if (!base) return set_err(RDY_ERR_STATE, "no workspace on this device");
+    const uint32_t nbins = n_borders - 1;
+    double *d_b = (double *)(base + NOALLOC_OFF_BORDERS);
+    uint32_t *d_m = (uint32_t *)(base + NOALLOC_OFF_MASK);
+    unsigned long long *d_c = (unsigned long long *)(base + NOALLOC_OFF_COUNTS);
+    HIP_TRY(hipMemcpyAsync(d_b, borders, 8 * (size_t)n_borders, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d_m, mask, 4 * (size_t)words, hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(k_zero_u64, dim3(blocks_for(nbins, 256)), dim3(256), 0, e->stream, d_c, nbins);
+    ensure_wl(e);
+    hipLaunchKernelGGL(k_hist_axis, dim3(blocks_for(e->max_particles, 256)), dim3(256), 0, e->stream,
+                       e->grid(e->cur), r, (const double *)d_b, n_borders, (const uint32_t *)d_m, (int)axis, d_c,
+                       (uint32_t)e->max_particles);
+    HIP_TRY(hipMemcpyAsync(counts, d_c, 8 * (size_t)nbins, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return check_errors(e);
 }

@@ -324,6 +324,15 @@ class Engine:
                                                    int(axis), _ptr(c, _u64p)))
         return c
 
+    def histogram_along_axis_noalloc(self, replica, borders, types, axis):
+        """The same counts from the entry point that allocates nothing (static per-GPU workspace)."""
+        b = np.ascontiguousarray(borders, dtype=np.float64)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        c = np.zeros(max(len(b) - 1, 0), dtype=np.uint64)
+        L.check(self._lib.rdy_histogram_along_axis_noalloc(self._h, replica, _ptr(b, _dp), len(b), _ptr(t, _u32p),
+                                                           len(t), int(axis), _ptr(c, _u64p)))
+        return c
+
     def radial_distribution(self, replica, borders, from_types, to_types, particle_to_density):
         """(bin centers, distribution) as RadialDistribution::evaluate forms them
         (RadialDistribution.cpp:105-123, 136-144): counts / (4/3 pi (r_hi^3 - r_lo^3) n_from density)"""
