@@ -34,19 +34,7 @@ PMC_PROFILE = "brick_step_c1_1M_pmc.md"
 WALK_KERNEL = "k_brick_step<true> (pair forces, reaction candidates and the next move of one brick of cells)"
 
 
-def c1_system(n=1_000_000, L=200.0, seed=1):
-    rng = np.random.default_rng(seed)
-    pos = (rng.random((n, 3)) - 0.5) * L
-    types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
-    return pos, types
-
-
-def c1_chemistry(L):
-    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
-    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
-    rxs = [dict(kind=L.RX_FUSION, educts=[0, 1], products=[2], rate=0.1, educt_distance=1.0),
-           dict(kind=L.RX_FISSION, educts=[2], products=[0, 1], rate=0.05, product_distance=1.0)]
-    return pots, rxs
+from readdy_amd.workloads import c1_chemistry, c1_system  # noqa: E402
 
 
 def traffic_from_profiles():
@@ -58,12 +46,28 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(n_steps=5):
-    """FP64 oracle on the same 1M-particle system, a few steps (bounded sample), on the host cores
-    this process may use (at most 16: forces, integration and the candidate search split over
-    OpenMP threads, as the reference's CPU kernel splits them over its thread pool)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "unknown")
+    except OSError:
+        return "unknown"
+
+
+def _timed_loop(o, n_steps, dt, mode, step0=0):
+    """(particle-steps/s, ms per phase and step) of n_steps full loop steps of oracle system o"""
+    o.phase_times(reset=True)
+    t0 = time.perf_counter()
+    ps = 0
+    for t in range(n_steps):
+        ps += o.n_active
+        o.loop_step(dt, mode, step0 + t)
+    el = time.perf_counter() - t0
+    return ps / el, el, {k: v / n_steps * 1e3 for k, v in o.phase_times().items()}
+
+
+def _c1_oracle(threads):
     from oracle import pyoracle as po
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     pos, types = c1_system()
     o = po.System(3, [200.0] * 3, [1, 1, 1], diffusion=[1.0, 1.0, 0.5])
     o.set_seed(1)
@@ -75,28 +79,96 @@ def cpu_baseline(n_steps=5):
     o.add_reaction(po.RX_FISSION, [2], [0, 1], 0.05, 0.0, 1.0)
     o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
     o.set_threads(threads)
+    o.set_cas_fill(True)
     o.nl_setup(o.max_cutoff())
     o.calculate_forces()
-    t0 = time.perf_counter()
-    ps = 0
-    for t in range(n_steps):
-        ps += o.n_active
-        o.loop_step(0.005, po.UA_KEYED, t)
-    dt = time.perf_counter() - t0
-    model = "unknown"
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), model)
-    except OSError:
-        pass
-    phases = {k: v / n_steps * 1e3 for k, v in o.phase_times().items()}
-    return {"value": ps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+    return o
+
+
+def cpu_baseline(n_steps=5):
+    """The FP64 oracle, this repository's restatement of the reference CPU kernel, timed on the host cores
+    this process may use (at most 16): forces, integration and the candidate search split over OpenMP
+    threads by static partitions, the cell list filled from all threads by atomic exchange, as the
+    reference's CPU kernel does over its thread pool.  The headline figure is 5 full loop steps of the
+    1M-particle workload on 1 x cores threads; beside it the reference's default pool of 4 x cores threads,
+    the plain-method system in full (1000 steps), a sample of the bead-chain system and a sample of the
+    batched ensemble run replica after replica."""
+    from oracle import pyoracle as po
+    from readdy_amd import ensemble as E
+    from readdy_amd import workloads as W
+    from readdy_amd import _lib as L
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    o = _c1_oracle(cores)
+    rate, el, phases = _timed_loop(o, n_steps, 0.005, po.UA_KEYED)
+    del o
+    o = _c1_oracle(4 * cores)
+    rate4, el4, _ = _timed_loop(o, n_steps, 0.005, po.UA_KEYED)
+    del o
+    # the plain-method system (2000 particles, box 20, UA, dt 0.01) in full: 1000 steps on one thread (on
+    # this size more threads only add their overhead)
+    rng = np.random.default_rng(0)
+    c0 = po.System(3, [20.0] * 3, [1, 1, 1], diffusion=[1.0, 1.0, 0.5])
+    for a in range(3):
+        for b in range(a, 3):
+            c0.add_pair_potential(po.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0])
+    c0.add_reaction(po.RX_FUSION, [0, 1], [2], 0.1, 1.0)
+    c0.add_reaction(po.RX_FISSION, [2], [0, 1], 0.05, 0.0, 1.0)
+    c0.add_particles((rng.random((2000, 3)) - 0.5) * 20.0, np.repeat(np.arange(2, dtype=np.int32), 1000),
+                     ids=np.arange(2000))
+    c0.nl_setup(c0.max_cutoff())
+    c0.calculate_forces()
+    rate_c0, el_c0, ph_c0 = _timed_loop(c0, 1000, 0.01, po.UA_KEYED)
+    # the bead-chain system: 10^4 chains of 50 beads + 5 10^5 crowders (box 160), 3 loop steps at
+    # dt = 2e-4 (the configuration's 1e-3 blows the chains up), Gillespie
+    pos, types, bonds, angles, dihedrals = W.system(10000, 50, 500000, 160.0, seed=2)
+    c2 = po.System(3, [160.0] * 3, [1, 1, 1], diffusion=[0.5, 1.0, 1.0])
+    for kind, a, b, p in W.potentials(L):
+        c2.add_pair_potential(kind, a, b, p)
+    for x in W.reactions(L, 0.01):
+        c2.add_reaction(x["kind"], x["educts"], x["products"], x["rate"], x["educt_distance"], 0.0)
+    c2.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    for b in bonds:
+        c2.add_bond(int(b[0]), int(b[1]), 100.0, 1.0)
+    for a in angles:
+        c2.add_angle(int(a[0]), int(a[1]), int(a[2]), 10.0, np.pi)
+    for d in dihedrals:
+        c2.add_dihedral(int(d[0]), int(d[1]), int(d[2]), int(d[3]), 1.0, 3, 0.0)
+    c2.set_threads(cores)
+    c2.set_cas_fill(True)
+    c2.nl_setup(c2.max_cutoff())
+    c2.calculate_forces()
+    rate_c2, el_c2, _ = _timed_loop(c2, 3, 2e-4, po.GILLESPIE_KEYED)
+    del c2
+    # the batched ensemble, replica after replica: 8 replicas evenly spaced through the 4096, 100 steps each
+    n, box = E.sample(4096)
+    ps3, el3 = 0, 0.0
+    for r in range(0, 4096, 512):
+        o3 = po.System(3, [box[r]] * 3, [1, 1, 1], diffusion=E.DIFFUSION)
+        for kind, a, b, p in E.chemistry()[0]:
+            o3.add_pair_potential(kind, a, b, p)
+        for x in E.chemistry()[1]:
+            o3.add_reaction(x["kind"], x["educts"], x["products"], x["rate"], x.get("educt_distance", 0.0),
+                            x.get("product_distance", 0.0))
+        p3, t3 = E.initial_state(n[r], box[r], r)
+        o3.add_particles(p3, t3.astype(np.int32), ids=np.arange(len(t3)))
+        o3.nl_setup(o3.max_cutoff())
+        o3.calculate_forces()
+        rr, ee, _ = _timed_loop(o3, 100, E.DT, po.UA_KEYED)
+        ps3 += rr * ee
+        el3 += ee
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"{n_steps} full steps of the 1M-particle workload (integrate, neighbour list, UA reactions, "
-                      f"neighbour list, forces) by the FP64 C oracle on {threads} OpenMP threads; {dt:.1f} s",
-            "phase_ms_per_step": phases, "cpu_model": model, "nproc": os.cpu_count()}
+                      f"neighbour list, forces) by the FP64 C oracle on {cores} OpenMP threads; {el:.1f} s",
+            "phase_ms_per_step": phases, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "threads_4x_cores": {"threads": 4 * cores, "value": rate4, "seconds": el4},
+            "plain_system_full": {"particles": 2000, "steps": 1000, "threads": 1, "value": rate_c0,
+                                  "seconds": el_c0, "phase_ms_per_step": ph_c0},
+            "bead_chain_sample": {"particles": int(len(types)), "steps": 3, "dt": 2e-4, "threads": cores,
+                                  "value": rate_c2, "seconds": el_c2},
+            "ensemble_sample": {"replicas": 8, "steps": 100, "threads": 1, "value": ps3 / el3, "seconds": el3}}
 
 
-def ensemble_bench(n_replicas, steps):
+def ensemble_bench(n_replicas, steps, plain_steps=20):
     """Batched replica ensemble (4096 replicas of the A+B<->C system, 500..50000 particles each at
     density 0.25) against the plain one-engine-per-replica path; cell size and brick shape from the
     autotuner."""
@@ -108,16 +180,27 @@ def ensemble_bench(n_replicas, steps):
     rate, lat = b.timed(steps, chunk=10, warmup=10)
     particles = int(b.particles())
     b.close()
-    ids = list(range(0, n_replicas, max(1, n_replicas // 32)))
+    # the plain path: one engine per replica, each replica's step kernels launched once per step on its
+    # engine's stream, the replicas interleaved, one synchronisation per run (256 replicas evenly spaced
+    # through the sample, 20 steps)
+    ids = list(range(0, n_replicas, max(1, n_replicas // 256)))
     p = E.Plain(n, box, nl_cutoff=best["nl_cutoff"], replica_ids=ids)
-    plain = p.timed(3)
+    plain = p.timed(plain_steps)
+    old_ids = ids[::max(1, len(ids) // 32)]
+    p.engines = [p.engines[ids.index(i)] for i in old_ids]
+    plain_per_call = p.timed_per_call(3)
     p.close()
     return {"replicas": int(n_replicas), "particles": particles,
             "batched_particle_steps_per_s": rate, "plain_particle_steps_per_s": plain, "speedup": rate / plain,
+            "plain_per_call_particle_steps_per_s": plain_per_call, "speedup_vs_per_call": rate / plain_per_call,
             "step_ms_p50": lat["p50_ms"], "step_ms_p99": lat["p99_ms"],
             "batched_roofline_frac": rate * B_ALG / (HBM_PEAK_GBS * 1e9),
             "steps": steps, "autotune": {"chosen": best, "tried": tried},
-            "plain_sample": f"{len(ids)} replicas evenly spaced through the sample, one engine each, 3 steps",
+            "plain_sample": f"{len(ids)} replicas evenly spaced through the sample, one engine each, a run of "
+                            f"{plain_steps} steps enqueued step by step across the engines, one synchronisation",
+            "plain_per_call_sample": f"{len(old_ids)} of them, 3 steps, one whole run (first walk, step, last copy, "
+                                     f"host synchronisation) per replica per step",
+            "sampled": f"{steps} of the configuration's 5000 steps",
             "latency": "per-step wall time of the batched loop over chunks of 10 graph-replayed steps"}
 
 
@@ -157,6 +240,7 @@ def serving_bench(n_requests, step_scale):
         srv.close()
     summary.update({"trace": f"first {n_requests} arrivals of Poisson(200/s); steps log-uniform [1000, 100000] "
                              f"x {step_scale}", "hbm_budget_gib": 32,
+                    "sampled": f"step counts scaled by {step_scale} so the trace finishes in seconds",
                     "roofline_frac": summary["particle_steps_per_s"] * B_ALG / (HBM_PEAK_GBS * 1e9)})
     return summary
 

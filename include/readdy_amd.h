@@ -200,6 +200,35 @@ typedef struct {
 int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_pair *pairs, uint64_t cap,
                     uint32_t *broken, uint64_t *n_broken);
 
+/* ---- spatial topology reactions of the topology-particle enzymatic kind, "T1(p1) + (p2) -> T2(p3) + (p4)":
+ *      SpatialTopologyReaction.h:64-236 (STRMode::TP_ENZYMATIC), evaluated as
+ *      CPUEvaluateTopologyReactions.cpp:88-475 does.  A particle of species type1 in a topology of type
+ *      top_type1 and a particle of no topology of species type2 closer than radius react with probability
+ *      1 - exp(-rate dt); at most one reaction per topology and per free particle and step (performEvents
+ *      with eventsDependent); the topology particle becomes type_to1, the free one type_to2, the topology
+ *      top_type_to1.  Topology types are the caller's integers.  The bonded terms stay as they are: when
+ *      type_to1 changes the topology's bonded configuration, the caller sets the terms again. */
+typedef struct {
+    uint32_t type1, top_type1;   /* the topology particle's species and its topology's type */
+    uint32_t type2;              /* the free particle's species                               */
+    uint32_t type_to1, top_type_to1, type_to2;
+    double rate, radius;
+} rdy_spatial_reaction;
+/* the engine's spatial topology reactions (TopologyRegistry::addSpatialReaction, TopologyRegistry.cpp:195-223);
+ * radius <= the engine's neighbour-list cutoff, at most 64 reactions */
+int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_reaction *reactions);
+/* the replica's topologies: their types [n_topologies] and, for each topology particle (handles 0 ..
+ * n_topo - 1 of the last upload), the index of its topology [n_topo] (StateModel::addTopology) */
+int rdy_set_topologies(rdy_engine *e, uint32_t replica, uint32_t n_topologies, const uint32_t *topology_types,
+                       const uint32_t *topology_of);
+/* the replica's topology types as the topology reactions left them; *n = the number of topologies */
+int rdy_get_topology_types(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *types, uint32_t *n);
+/* EvaluateTopologyReactions (Actions.h, CPUEvaluateTopologyReactions::perform) on the current state; the
+ * fused loop (rdy_run, rdy_run_graph) performs it in every step after the reactions (SimulationLoop.h:286-287) */
+int rdy_evaluate_topology_reactions(rdy_engine *e, double dt);
+/* per spatial reaction: performed in the last evaluation (totals = 0) or since the replica was added (1) */
+int rdy_get_topology_reaction_counts(rdy_engine *e, uint32_t replica, int totals, uint64_t *counts);
+
 /* ---- the simulation loop: replaces api::SimulationLoop::run's body
  *      (include/readdy/api/SimulationLoop.h:256-306): per step integrate, neighbour list,
  *      reactions, neighbour list, forces -- fused into shared launches over all replicas.
@@ -207,6 +236,15 @@ int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_
  * and replays it. */
 int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps);
 int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps);
+/* The same run taken a few steps at a time, with no host synchronisation in between: rdy_run_begin
+ * enqueues the run's first walk, each rdy_run_enqueue(k) enqueues the next k fused steps (the run's last
+ * step copies instead of moving, as in rdy_run), and rdy_run_end waits for the device and reports errors.
+ * Several engines can interleave their steps this way on their own streams.  While a run is open the
+ * engine's other calls that change state are refused (RDY_ERR_STATE).  No reference counterpart
+ * (SimulationLoop::run performs one whole run, SimulationLoop.h:256-306). */
+int rdy_run_begin(rdy_engine *e, int handler, double dt, uint64_t n_steps);
+int rdy_run_enqueue(rdy_engine *e, uint64_t k);
+int rdy_run_end(rdy_engine *e);
 /* per-replica step budget for serving: a replica stops stepping after this many steps */
 int rdy_replica_set_steps(rdy_engine *e, uint32_t replica, uint64_t steps);
 int rdy_replica_steps_done(rdy_engine *e, uint32_t replica, uint64_t *steps);

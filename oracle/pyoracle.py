@@ -59,6 +59,7 @@ def lib():
         L.or_set_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_id_counter.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
         L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_set_cas_fill.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_phase_times.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
         L.or_phase_times.restype = None
         L.or_histogram_along_axis.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
@@ -119,6 +120,13 @@ def lib():
         L.or_dihedral_force_energy.restype = ctypes.c_double
         L.or_dihedral_force_energy.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp,
                                                _dp, _ip, _dp, _dp, _dp, _dp]
+        L.or_add_spatial_reaction.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
+        L.or_set_topologies.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _ip, ctypes.c_size_t, _ip]
+        L.or_topology_reactions.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, _u64p]
+        L.or_topology_types.restype = _ip
+        L.or_topology_types.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+        L.or_topology_reaction_counts.restype = _u64p
+        L.or_topology_reaction_counts.argtypes = [ctypes.c_void_p, _ip]
         L.or_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -289,6 +297,11 @@ class System:
         """OpenMP threads for forces, integration and the keyed candidate search (default 1)"""
         lib().or_set_threads(self._h, int(n))
 
+    def set_cas_fill(self, on=True):
+        """Fill the cell list from all threads by atomic exchange, as the reference does (timing only:
+        the order within a cell varies from run to run)"""
+        lib().or_set_cas_fill(self._h, 1 if on else 0)
+
     def add_particles(self, pos, types, ids=None):
         pos, pp = _d(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
         types, tp = _i(types)
@@ -391,6 +404,33 @@ class System:
     def reaction_counts(self):
         n = ctypes.c_int()
         p = lib().or_reaction_counts(self._h, ctypes.byref(n))
+        return np.array([p[i] for i in range(n.value)], dtype=np.uint64)
+
+    # ---- spatial topology reactions of the topology-particle enzymatic kind ----
+    def add_spatial_reaction(self, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius):
+        """T1(type1) + (type2) -> T2(type_to1) + (type_to2), topology types as integers"""
+        _check(lib().or_add_spatial_reaction(self._h, type1, top_type1, type2, type_to1, top_type_to1, type_to2,
+                                             rate, radius))
+
+    def set_topologies(self, topology_of_id, top_types):
+        """topology_of_id[id] = topology index of the particle with that id (-1: none)"""
+        a, ap = _i(np.asarray(topology_of_id, dtype=np.int32))
+        b, bp = _i(np.asarray(top_types, dtype=np.int32))
+        _check(lib().or_set_topologies(self._h, a.size, ap, b.size, bp))
+
+    def topology_reactions(self, dt, step=0):
+        n = ctypes.c_uint64()
+        _check(lib().or_topology_reactions(self._h, dt, step, ctypes.byref(n)))
+        return n.value
+
+    def topology_types(self):
+        n = ctypes.c_size_t()
+        p = lib().or_topology_types(self._h, ctypes.byref(n))
+        return np.array([p[i] for i in range(n.value)], dtype=np.int64)
+
+    def topology_reaction_counts(self):
+        n = ctypes.c_int()
+        p = lib().or_topology_reaction_counts(self._h, ctypes.byref(n))
         return np.array([p[i] for i in range(n.value)], dtype=np.uint64)
 
     def loop_step(self, dt, mode, step, noise_by_id=None):

@@ -106,6 +106,28 @@ static void normals3(const uint32_t x[4], double n[3]) {
     n[2] = r1 * cos(two_pi * v3);
 }
 
+/* the Brownian-dynamics noise: three N(0,1) draws by Box-Muller in FP32 from one Philox block, the
+ * transform the HIP engine uses: 24-bit uniforms v_k = ((x_k >> 8) + 1/2) 2^-24, r = sqrt(-2 ln v),
+ * angle 2 pi v.  The logarithm, sine and cosine are formed in FP64 and rounded to FP32, so each result
+ * lies within an FP32 unit of the exact value, as the device's FP32 functions do; the two sides agree to
+ * a few FP32 units, not bit for bit. */
+static void bd_normals3(const uint32_t x[4], double n[3]) {
+    const float s24 = 5.9604644775390625e-08f; /* 2^-24 */
+    const float v0 = ((float)(x[0] >> 8) + 0.5f) * s24;
+    const float v1 = ((float)(x[1] >> 8) + 0.5f) * s24;
+    const float v2 = ((float)(x[2] >> 8) + 0.5f) * s24;
+    const float v3 = ((float)(x[3] >> 8) + 0.5f) * s24;
+    const float r0 = sqrtf(-2.0f * (float)log((double)v0));
+    const float r1 = sqrtf(-2.0f * (float)log((double)v2));
+    const double pi = 3.14159265358979323846;
+    const float a1 = 2.0f * v1, a3 = 2.0f * v3;
+    const float s1 = (float)sin(pi * (double)a1), c1 = (float)cos(pi * (double)a1);
+    const float c3 = (float)cos(pi * (double)a3);
+    n[0] = (double)(r0 * c1);
+    n[1] = (double)(r0 * s1);
+    n[2] = (double)(r1 * c3);
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Periodic boundary helpers: include/readdy/common/boundary_condition_operations.h:53-91.
  * --------------------------------------------------------------------------------------------- */
@@ -425,6 +447,14 @@ typedef struct {
     int mult;
 } bonded_t;
 
+#define OR_MAX_STR 64
+/* a topology-particle enzymatic spatial topology reaction, "T1(p1) + (p2) -> T2(p3) + (p4)"
+ * (SpatialTopologyReaction.h:64-236, STRMode::TP_ENZYMATIC) */
+typedef struct {
+    int type1, top_type1, type2, type_to1, top_type_to1, type_to2;
+    double rate, radius, radius2;
+} str_t;
+
 typedef struct {
     int n_educts;
     size_t idx1, idx2;
@@ -471,6 +501,16 @@ struct or_system {
     size_t n_recs, cap_recs;
     double phase_s[4]; /* or_loop_step wall time: integrator, neighborList (both fills), reactions, forces */
     int threads; /* > 1: forces, integration and the candidate search on that many OpenMP threads */
+    int cas_fill; /* fill the cell list from all threads by atomic head exchange (timing baseline) */
+    /* topologies for spatial topology reactions: the topology of each particle id (-1: none) and the
+     * type of each topology */
+    int *topology_of;
+    size_t topology_of_len;
+    int *top_types;
+    size_t n_tops;
+    str_t str[OR_MAX_STR];
+    int n_str;
+    uint64_t str_counts[OR_MAX_STR];
 };
 
 or_system *or_create(int n_types, const double *box, const int *pbc, double kbt, const double *diffusion) {
@@ -493,6 +533,8 @@ or_system *or_create(int n_types, const double *box, const int *pbc, double kbt,
 
 void or_destroy(or_system *s) {
     if (!s) return;
+    free(s->topology_of);
+    free(s->top_types);
     free(s->e);
     free(s->blanks);
     free(s->bonds);
@@ -553,6 +595,11 @@ int or_set_seed(or_system *s, uint64_t seed) {
  * before use).  The literal Gillespie path depends on list order and gathers on one thread. */
 int or_set_threads(or_system *s, int n) {
     s->threads = n < 1 ? 1 : n;
+    return 0;
+}
+
+int or_set_cas_fill(or_system *s, int on) {
+    s->cas_fill = on != 0;
     return 0;
 }
 
@@ -789,6 +836,19 @@ int or_nl_fill(or_system *s) {
     s->head = (size_t *)calloc(s->n_cells, sizeof(size_t));
     s->list = (size_t *)calloc(s->n + 1, sizeof(size_t));
     s->list_len = s->n + 1;
+    if (s->cas_fill && s->threads > 1) {
+        /* CellLinkedList.cpp:171-228: every thread pushes its particles onto the cell heads by atomic
+         * exchange; the order within a cell then depends on the thread interleaving (the pair set does
+         * not), so this fill serves the timing baseline, and the parity checks use the serial fill */
+#pragma omp parallel for schedule(static) num_threads(s->threads)
+        for (size_t i = 0; i < s->n; ++i) {
+            const entry_t *en = &s->e[i];
+            if (en->deactivated || !in_box(s, en->pos)) continue;
+            size_t c = cidx(s, cell_coord(s, en->pos, 0), cell_coord(s, en->pos, 1), cell_coord(s, en->pos, 2));
+            s->list[i + 1] = __atomic_exchange_n(&s->head[c], i + 1, __ATOMIC_RELAXED);
+        }
+        return 0;
+    }
     for (size_t i = 0; i < s->n; ++i) {
         const entry_t *en = &s->e[i];
         if (en->deactivated || !in_box(s, en->pos)) continue;
@@ -876,6 +936,8 @@ double or_max_cutoff(const or_system *s) {
         int k = s->rx[i].kind;
         if ((k == OR_RX_FUSION || k == OR_RX_ENZYMATIC) && s->rx[i].educt_distance > m) m = s->rx[i].educt_distance;
     }
+    for (int i = 0; i < s->n_str; ++i) /* spatial topology reaction radii (Context.cpp:126-144) */
+        if (s->str[i].radius > m) m = s->str[i].radius;
     return m;
 }
 
@@ -990,7 +1052,7 @@ int or_euler_bd(or_system *s, double dt, const double *noise_by_id, size_t noise
         } else {
             uint32_t x[4];
             philox_draw(s->seed, (uint32_t)en->id, step, 0, 0, x);
-            normals3(x, xi);
+            bd_normals3(x, xi);
         }
         double sc = sqrt(2. * D * dt);
         for (int d = 0; d < 3; ++d) {
@@ -1528,4 +1590,160 @@ void or_phase_times(or_system *s, double *out4, int reset) {
         out4[k] = s->phase_s[k];
         if (reset) s->phase_s[k] = 0.0;
     }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Spatial topology reactions of the topology-particle enzymatic kind
+ * (kernels/cpu/src/actions/CPUEvaluateTopologyReactions.cpp:88-475, SpatialTopologyReaction.h:64-236).
+ *   gatherEvents (:200-367): every particle of a topology, with every neighbour of the cell list that
+ *     belongs to no topology, for each reaction registered under (its type, its topology's type, the
+ *     neighbour's type) whose radius exceeds the distance (distSquared < radius^2), is a candidate of
+ *     rate `rate`; a reaction is found from the topology particle's side only, as the registry keys it.
+ *   performEvents (algorithm.h:62-136) with shouldEval = 1 - exp(-rate dt) and eventsDependent
+ *     (:455-464): events of one topology depend on each other, and so do topology-particle events that
+ *     share the free particle.  As for the Gillespie handler, the selection is realised by
+ *     Efraimidis-Spirakis keys -ln(u) / rate over the candidates whose Bernoulli draw succeeds (Philox
+ *     counter (topology particle id, step, 6 | r << 8, free particle id)), greedily in key order.
+ *   handleTopologyParticleReaction (:369-404): the topology particle takes type_to1, the free particle
+ *     type_to2, the topology the type top_type_to1.  (The reference also writes the topology's index into
+ *     the free particle's record, which only matters for later topology-topology reactions; free
+ *     particles here keep no topology.)
+ * --------------------------------------------------------------------------------------------- */
+int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
+                            int type_to2, double rate, double radius) {
+    if (s->n_str >= OR_MAX_STR) return fail("too many spatial topology reactions");
+    if (type1 < 0 || type1 >= s->n_types || type2 < 0 || type2 >= s->n_types || type_to1 < 0 ||
+        type_to1 >= s->n_types || type_to2 < 0 || type_to2 >= s->n_types)
+        return fail("spatial topology reaction species out of range");
+    if (!(rate > 0)) return fail("The rate of an structural topology reaction should always be positive");
+    str_t *r = &s->str[s->n_str++];
+    r->type1 = type1;
+    r->top_type1 = top_type1;
+    r->type2 = type2;
+    r->type_to1 = type_to1;
+    r->top_type_to1 = top_type_to1;
+    r->type_to2 = type_to2;
+    r->rate = rate;
+    r->radius = radius;
+    r->radius2 = radius * radius;
+    return 0;
+}
+
+int or_set_topologies(or_system *s, size_t n_ids, const int *topology_of_id, size_t n_tops, const int *top_types) {
+    free(s->topology_of);
+    free(s->top_types);
+    s->topology_of = (int *)malloc((n_ids ? n_ids : 1) * sizeof(int));
+    s->top_types = (int *)malloc((n_tops ? n_tops : 1) * sizeof(int));
+    for (size_t i = 0; i < n_ids; ++i) {
+        if (topology_of_id[i] >= (int)n_tops) return fail("particle assigned to a topology that does not exist");
+        s->topology_of[i] = topology_of_id[i];
+    }
+    for (size_t k = 0; k < n_tops; ++k) s->top_types[k] = top_types[k];
+    s->topology_of_len = n_ids;
+    s->n_tops = n_tops;
+    return 0;
+}
+
+const int *or_topology_types(const or_system *s, size_t *n) {
+    if (n) *n = s->n_tops;
+    return s->top_types;
+}
+
+const uint64_t *or_topology_reaction_counts(const or_system *s, int *n) {
+    if (n) *n = s->n_str;
+    return s->str_counts;
+}
+
+static int topology_of_entry(const or_system *s, const entry_t *e) {
+    return e->id < s->topology_of_len ? s->topology_of[e->id] : -1;
+}
+
+typedef struct {
+    size_t p, q; /* slots: topology particle, free particle */
+    int top, r;
+    double gkey;
+    uint64_t ip, iq;
+} tevent_t;
+
+typedef struct {
+    tevent_t *ev;
+    size_t n, cap;
+    double dt;
+    uint32_t step;
+} tev_ud;
+
+static void str_visit(or_system *s, size_t p, size_t q, void *ud) {
+    tev_ud *u = (tev_ud *)ud;
+    const entry_t *ep = &s->e[p], *eq = &s->e[q];
+    if (eq->deactivated) return;
+    const int top = topology_of_entry(s, ep);
+    if (top < 0 || topology_of_entry(s, eq) >= 0) return;
+    double d2 = -1.0;
+    for (int r = 0; r < s->n_str; ++r) {
+        const str_t *x = &s->str[r];
+        if (x->type1 != ep->type || x->top_type1 != s->top_types[top] || x->type2 != eq->type) continue;
+        if (d2 < 0) d2 = or_dist2(ep->pos, eq->pos, s->box, s->pbc);
+        if (!(d2 < x->radius2)) continue;
+        uint32_t w[4];
+        philox_draw(s->seed, (uint32_t)ep->id, u->step, 6u | ((uint32_t)r << 8), (uint32_t)eq->id, w);
+        if (!(or_u53(w[0], w[1]) < accept_p(x->rate, u->dt))) continue;
+        if (u->n == u->cap) {
+            u->cap = u->cap ? 2 * u->cap : 256;
+            u->ev = (tevent_t *)realloc(u->ev, u->cap * sizeof(tevent_t));
+        }
+        tevent_t *e = &u->ev[u->n++];
+        e->p = p;
+        e->q = q;
+        e->top = top;
+        e->r = r;
+        e->gkey = -log(or_u53(w[2], w[3]) + 0x1p-54) / x->rate;
+        e->ip = ep->id;
+        e->iq = eq->id;
+    }
+}
+
+static int cmp_tev(const void *a, const void *b) {
+    const tevent_t *x = (const tevent_t *)a, *y = (const tevent_t *)b;
+    if (x->gkey != y->gkey) return x->gkey < y->gkey ? -1 : 1;
+    if (x->ip != y->ip) return x->ip < y->ip ? -1 : 1;
+    if (x->iq != y->iq) return x->iq < y->iq ? -1 : 1;
+    return x->r - y->r;
+}
+
+int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_performed) {
+    memset(s->str_counts, 0, sizeof s->str_counts); /* CPUEvaluateTopologyReactions.cpp:92-94 */
+    if (n_performed) *n_performed = 0;
+    if (!s->n_str || !s->n_tops) return 0;
+    if (!s->nl_set) return fail("topology reactions need a neighbour list");
+    tev_ud u;
+    memset(&u, 0, sizeof u);
+    u.dt = dt;
+    u.step = step;
+    for (size_t c = 0; c < s->n_cells; ++c)
+        for (size_t p1 = s->head[c]; p1 != 0; p1 = s->list[p1]) {
+            const entry_t *ep = &s->e[p1 - 1];
+            if (ep->deactivated || topology_of_entry(s, ep) < 0) continue;
+            for_each_neighbor(s, p1 - 1, c, str_visit, &u);
+        }
+    qsort(u.ev, u.n, sizeof(tevent_t), cmp_tev);
+    unsigned char *top_used = (unsigned char *)calloc(s->n_tops, 1);
+    unsigned char *p_used = (unsigned char *)calloc(s->n ? s->n : 1, 1);
+    uint64_t done = 0;
+    for (size_t i = 0; i < u.n; ++i) {
+        const tevent_t *e = &u.ev[i];
+        if (top_used[e->top] || p_used[e->q]) continue;
+        top_used[e->top] = 1;
+        p_used[e->q] = 1;
+        const str_t *x = &s->str[e->r];
+        s->e[e->p].type = x->type_to1;
+        s->e[e->q].type = x->type_to2;
+        s->top_types[e->top] = x->top_type_to1;
+        s->str_counts[e->r]++;
+        ++done;
+    }
+    free(top_used);
+    free(p_used);
+    free(u.ev);
+    if (n_performed) *n_performed = done;
+    return 0;
 }

@@ -73,6 +73,9 @@ int or_add_reaction(or_system *s, int kind, int educt1, int educt2, int product1
 int or_set_seed(or_system *s, uint64_t seed);
 int or_set_id_counter(or_system *s, uint64_t next_id);
 int or_set_threads(or_system *s, int n);
+/* on = 1: with more than one thread, the cell list is filled from all threads by atomic head exchange
+ * (CellLinkedList.cpp:171-228); the order within a cell then varies from run to run (timing only) */
+int or_set_cas_fill(or_system *s, int on);
 /* accumulated wall seconds of or_loop_step's phases: integrator, neighbour list, reactions, forces */
 void or_phase_times(or_system *s, double *out4, int reset);
 /* CPUHistogramAlongAxis::evaluate (kernels/cpu/src/observables/CPUObservables.cpp:85-136): each live
@@ -138,6 +141,16 @@ int or_perform_events_abstract(int n, const int64_t *p1, const int64_t *p2, cons
                                uint8_t *performed);
 
 const char *or_last_error(void);
+
+/* ---- spatial topology reactions of the topology-particle enzymatic kind
+ *      (kernels/cpu/src/actions/CPUEvaluateTopologyReactions.cpp:88-475) ---- */
+int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
+                            int type_to2, double rate, double radius);
+/* topology_of_id[id] = the topology of the particle with that id (-1: none), for ids below n_ids */
+int or_set_topologies(or_system *s, size_t n_ids, const int *topology_of_id, size_t n_tops, const int *top_types);
+int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_performed);
+const int *or_topology_types(const or_system *s, size_t *n);
+const uint64_t *or_topology_reaction_counts(const or_system *s, int *n);
 
 #ifdef __cplusplus
 }

@@ -3,8 +3,9 @@
 The HIP engine (libreaddy_amd.so, C ABI in include/readdy_amd.h) implements ReaDDy's per-step
 particle pipeline: cell-linked neighbour list, pair and bonded forces, Euler Brownian dynamics
 and the UncontrolledApproximation / Gillespie reaction handlers, for one system or thousands of
-replicas stepped together.  `readdy_amd.api` mirrors the `readdy` Python API so that
-`ReactionDiffusionSystem(...).simulation(kernel="MI355X")` is a drop-in.
+replicas stepped together.  `Engine` drives it from Python; the C++ host core and the "MI355X" kernel
+plugin (readdy_amd/host) carry the reference's Kernel / StateModel / ActionFactory / ObservableFactory
+object contract.
 """
 from .engine import Engine, bin_capacity_for, device_info  # noqa: F401
 

@@ -29,6 +29,12 @@ class PairPotential(ctypes.Structure):
                 ("params", ctypes.c_double * 8)]
 
 
+class SpatialReaction(ctypes.Structure):
+    _fields_ = [("type1", ctypes.c_uint32), ("top_type1", ctypes.c_uint32), ("type2", ctypes.c_uint32),
+                ("type_to1", ctypes.c_uint32), ("top_type_to1", ctypes.c_uint32), ("type_to2", ctypes.c_uint32),
+                ("rate", ctypes.c_double), ("radius", ctypes.c_double)]
+
+
 class Reaction(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("educts", ctypes.c_uint32 * 2), ("products", ctypes.c_uint32 * 2),
                 ("rate", ctypes.c_double), ("educt_distance", ctypes.c_double),
@@ -94,6 +100,14 @@ SIGNATURES = {
     "rdy_get_reaction_records": (_int, [_vp, _u32, _u64, _vp, _u64p]),
     "rdy_run": (_int, [_vp, _int, _d, _u64]),
     "rdy_run_graph": (_int, [_vp, _int, _d, _u64]),
+    "rdy_set_spatial_reactions": (_int, [_vp, _u32, _vp]),
+    "rdy_set_topologies": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
+    "rdy_get_topology_types": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
+    "rdy_evaluate_topology_reactions": (_int, [_vp, _d]),
+    "rdy_get_topology_reaction_counts": (_int, [_vp, _u32, _int, _u64p]),
+    "rdy_run_begin": (_int, [_vp, _int, _d, _u64]),
+    "rdy_run_enqueue": (_int, [_vp, _u64]),
+    "rdy_run_end": (_int, [_vp]),
     "rdy_replica_set_steps": (_int, [_vp, _u32, _u64]),
     "rdy_replica_steps_done": (_int, [_vp, _u32, _u64p]),
     "rdy_replica_set_handler": (_int, [_vp, _u32, _int]),

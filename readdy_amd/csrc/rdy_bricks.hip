@@ -533,7 +533,6 @@ template <bool ALL_HALO, class Keep, class Eval>
 __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
                                               uint32_t hc, const float4 &p, float reach, Keep &&keep, Eval &&eval) {
     const uint32_t lane = __lane_id();
-    const unsigned long long lt = lanemask_lt();
     Walk<ALL_HALO> w;
     w.q = w.qe = 0;
     bool more = false;
@@ -556,7 +555,7 @@ __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *
 #pragma unroll
         for (uint32_t i = 0; i < BATCH; ++i) {
             const unsigned long long mask = __ballot(k[i]);
-            if (k[i]) ring[(pos + (uint32_t)__popcll(mask & lt)) & (RING - 1)] = (lane << 16) | (q0 + i);
+            if (k[i]) ring[(pos + rank_in_mask(mask)) & (RING - 1)] = (lane << 16) | (q0 + i);
             pos += (uint32_t)__popcll(mask);
         }
         tail = pos;
@@ -768,30 +767,11 @@ __device__ __forceinline__ void finish_particle(const Grid &nxt, const ReplicaDe
     State nw = me;
     nw.tw &= ~TW_NEW;
     if (MODE == MODE_STEP && R.active) {
-        // CPUEulerBDIntegrator.cpp:63-73
-        const double D = T.diffusion[tw_type(me.tw)];
         double xi[3];
-        if (noise) {
-            xi[0] = noise[3ull * me.h];
-            xi[1] = noise[3ull * me.h + 1];
-            xi[2] = noise[3ull * me.h + 2];
-        } else {
-            uint32_t x4[4];
-            philox(me.h, R.step, 0u, 0u, R.seed, x4);
-            normals3(x4, xi);
-        }
-        const double sc = sqrt(2. * D * dt);
-        const double kbt = T.kbt;
+        bd_noise(noise, me.h, R.step, R.seed, xi);
         const double Fv[3] = {Fx, Fy, Fz};
         double p[3] = {me.x, me.y, me.z};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double det = Fv[d] * D * dt / kbt;
-            double rnd = xi[d] * sc;
-            p[d] += rnd + det;
-            p[d] = wrap(p[d], geo.box[d], geo.pbc[d]);
-            if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
-        }
+        em_move(T, tw_type(me.tw), dt, Fv, xi, geo.box, geo.pbc, p, err);
         nw.x = p[0];
         nw.y = p[1];
         nw.z = p[2];
@@ -963,7 +943,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                     const uint32_t tw2 = __float_as_uint(pq.w);
                     float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
                     if constexpr (!AH) frame_min_image(F, dx, dy, dz);
-                    const float d2p = dx * dx + dy * dy + dz * dz;
+                    const float d2p = prefilter_d2(dx, dy, dz);
                     const float thr = thr_row[tw_type(tw2)];
                     return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
                 };
@@ -1169,7 +1149,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
                 const uint32_t t2 = tw_type(tw2);
                 float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
                 if constexpr (!AH) frame_min_image(F, dx, dy, dz);
-                const float d2p = dx * dx + dy * dy + dz * dz;
+                const float d2p = prefilter_d2(dx, dy, dz);
                 const float thr = thr_row[t2];
                 const uint32_t sq = m.slot[q];
                 return ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & ((t2 > t) | ((t2 == t) & (sq > sj))) & (d2p < thr);

@@ -54,6 +54,7 @@ struct ReplicaDev {
     uint64_t steps_left;   // remaining step budget (serving)
     uint64_t steps_done;
     uint32_t topo_base, n_topo;
+    uint32_t tt_base, n_tt;  // this replica's topologies: their types at tt_base .. tt_base + n_tt
     int32_t handler;       // reaction handler of this replica (-1: the one the step is launched with)
     uint32_t integ;        // fused step: the walk of this step also integrates the next one
     unsigned long long energy;  // two's-complement int64, fixed point 2^-24
@@ -130,6 +131,27 @@ __device__ __forceinline__ void normals3(const uint32_t x[4], double n[3]) {
     n[0] = r0 * c1;
     n[1] = r0 * s1;
     n[2] = r1 * c3;
+}
+
+// three N(0,1) draws for the Brownian-dynamics noise by Box-Muller in FP32 from one Philox block:
+// 24-bit uniforms v_k = ((x_k >> 8) + 1/2) 2^-24 (exact in FP32), r = sqrt(-2 ln v), angle 2 pi v.
+// The reference draws its noise from std::normal_distribution (RandomProvider.h:64-69); any exact
+// N(0,1) sampler is equivalent, and an FP32 draw moves a particle by sqrt(2 D dt) times a few 2^-24,
+// far below every other error of the step, for a fifth of the instructions of the FP64 transform.
+__device__ __forceinline__ void bd_normals3(const uint32_t x[4], double n[3]) {
+    const float s24 = 5.9604644775390625e-08f;  // 2^-24
+    const float v0 = ((float)(x[0] >> 8) + 0.5f) * s24;
+    const float v1 = ((float)(x[1] >> 8) + 0.5f) * s24;
+    const float v2 = ((float)(x[2] >> 8) + 0.5f) * s24;
+    const float v3 = ((float)(x[3] >> 8) + 0.5f) * s24;
+    const float r0 = sqrtf(-2.0f * logf(v0));
+    const float r1 = sqrtf(-2.0f * logf(v2));
+    float s1, c1;
+    sincospif(2.0f * v1, &s1, &c1);
+    const float c3 = cospif(2.0f * v3);
+    n[0] = (double)(r0 * c1);
+    n[1] = (double)(r0 * s1);
+    n[2] = (double)(r1 * c3);
 }
 
 // ---- periodic boundary arithmetic (reference semantics, FP64, no FMA) --------------------------

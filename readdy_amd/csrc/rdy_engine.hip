@@ -1,6 +1,7 @@
 // Host side of the MI355X ReaDDy engine: device pools, replica admission, the per-step launch
 // sequence (eager and hipGraph), and the extern "C" ABI declared in include/readdy_amd.h.
 #include "rdy_observe.hip"
+#include "rdy_topo.hip"
 
 #include "../../include/readdy_amd.h"
 
@@ -35,6 +36,7 @@ struct HostReplica {
     bool in_use = false;
     uint64_t cell_base = 0, n_cells = 0;
     uint32_t topo_base = 0, n_topo = 0;
+    uint32_t tt_base = 0, n_tt = 0;  // topologies (their types) for spatial topology reactions
     bool has_particles = false;  // particles were uploaded since the replica was added
     ReplicaDev dev{};
 };
@@ -165,6 +167,7 @@ struct rdy_engine {
     RxPairIdx *d_o2idx = nullptr;
     float2 *d_rowmax = nullptr;
     double *d_D = nullptr;
+    double *d_bd_scale = nullptr;
     uint8_t *d_flavor = nullptr;
     std::vector<RxDev> rx_h;
     double tables_dt = -1.0;
@@ -174,6 +177,18 @@ struct rdy_engine {
     uint32_t *loc = nullptr;
     uint64_t topo_cap = 0, topo_used = 0;
     std::vector<std::pair<uint64_t, uint64_t>> free_topo;  // released (start, length) ranges of loc / fb
+    uint32_t *topo_of = nullptr;     // [topo_cap] topology index (within the replica) of each topology particle
+    uint32_t *topo_owner = nullptr;  // [topo_cap] replica of each topology particle slot, UINT32_MAX if unused
+    uint32_t *loc_cur = nullptr;     // [topo_cap] fused step: topology particles' slots in the buffer it reads
+    // spatial topology reactions (topology-particle enzymatic) and the topology types they read and write
+    std::vector<rdy_spatial_reaction> sr_in;
+    SpatialDev *d_sr = nullptr;
+    uint32_t *ttype = nullptr;
+    uint64_t tt_cap = 0, tt_used = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> free_tt;
+    TopoEvent *tev = nullptr;
+    uint32_t *tev_count = nullptr;
+    unsigned long long *str_step = nullptr, *str_total = nullptr;
     std::vector<BondTerm> terms_h[3];
     BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
     uint64_t d_terms_cap[3] = {0, 0, 0};
@@ -188,6 +203,11 @@ struct rdy_engine {
     uint64_t g_version = ~0ull;
     int g_handler = -1;
     double g_dt = -1.0;
+    // a run taken a few steps at a time (rdy_run_begin / rdy_run_enqueue / rdy_run_end)
+    bool in_session = false;
+    uint64_t session_left = 0;
+    int session_handler = 0;
+    double session_dt = 0.0;
     // profiling of the fused kernel
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
@@ -402,6 +422,11 @@ int upload_tables(rdy_engine *e, double dt) {
     if ((rc = up(&e->d_o2list, o2list))) return rc;
     if ((rc = up(&e->d_rowmax, rowmax))) return rc;
     if ((rc = up(&e->d_D, e->D))) return rc;
+    {  // sqrt(2 D dt) per species, formed as CPUEulerBDIntegrator.cpp:63-73 forms it (FP64, correctly rounded)
+        std::vector<double> sc(nt);
+        for (uint32_t t = 0; t < nt; ++t) sc[t] = std::sqrt(2. * e->D[t] * dt);
+        if ((rc = up(&e->d_bd_scale, sc))) return rc;
+    }
     if (!e->d_flavor) {
         if ((rc = up(&e->d_flavor, e->topo_flavor))) return rc;
     }
@@ -413,6 +438,7 @@ int upload_tables(rdy_engine *e, double dt) {
     e->T.o2idx = e->d_o2idx;
     e->T.o2list = e->d_o2list;
     e->T.diffusion = e->d_D;
+    e->T.bd_scale = e->d_bd_scale;
     e->T.rowmax = e->d_rowmax;
     e->T.n_types = nt;
     e->T.n_rx = e->n_rx;
@@ -423,6 +449,23 @@ int upload_tables(rdy_engine *e, double dt) {
     // configure_bricks reserved (from the registered potentials, before these tables existed)
     if (e->brick_smem && table_bytes(std::min(nt, MAX_STAGED_TYPES), e->T.n_pots) > e->tables_lds)
         return set_err(RDY_ERR_STATE, "type tables exceed the LDS reserved for them");
+    if (!e->sr_in.empty()) {  // spatial topology reactions: acceptance 1 - exp(-rate dt) of this time step
+        std::vector<SpatialDev> sr;
+        for (const auto &x : e->sr_in) {
+            SpatialDev d{};
+            d.type1 = x.type1;
+            d.top_type1 = x.top_type1;
+            d.type2 = x.type2;
+            d.type_to1 = x.type_to1;
+            d.top_type_to1 = x.top_type_to1;
+            d.type_to2 = x.type_to2;
+            d.rate = x.rate;
+            d.p_accept = 1 - std::exp(-x.rate * dt);
+            d.radius2 = x.radius * x.radius;
+            sr.push_back(d);
+        }
+        if ((rc = up(&e->d_sr, sr))) return rc;
+    }
     e->tables_dt = dt;
     e->version++;
     return RDY_OK;
@@ -489,7 +532,12 @@ int drop_terms(rdy_engine *e, uint32_t r) {
 
 // gives replica r's range of topology particle slots (loc / fb) back to the pool
 void release_topo(rdy_engine *e, HostReplica &H) {
+    if (H.n_tt) release_range(e->free_tt, H.tt_base, H.n_tt);
+    H.tt_base = 0;
+    H.n_tt = 0;
     if (H.n_topo == 0) return;
+    if (e->topo_owner)
+        (void)hipMemsetAsync(e->topo_owner + H.topo_base, 0xFF, H.n_topo * sizeof(uint32_t), e->stream);
     release_range(e->free_topo, H.topo_base, H.n_topo);
     // a free range at the end of the used part shrinks it
     if (!e->free_topo.empty() && e->free_topo.back().first + e->free_topo.back().second == e->topo_used) {
@@ -708,6 +756,37 @@ uint32_t *slow_list(const rdy_engine *e, int which) { return e->d_slow + (size_t
 // the record pool handed to the resolve kernels (null unless reactions are being recorded)
 RxRecord *rec_ptr(const rdy_engine *e) { return e->recording ? e->recs : nullptr; }
 
+// the spatial topology reaction tables of the engine
+TopoTables topo_tables(const rdy_engine *e) {
+    TopoTables t{};
+    t.sr = e->d_sr;
+    t.n_sr = (uint32_t)e->sr_in.size();
+    t.topo_of = e->topo_of;
+    t.topo_owner = e->topo_owner;
+    t.ttype = e->ttype;
+    t.ev = e->tev;
+    t.ev_count = e->tev_count;
+    t.ev_cap = TOPO_MAX_EVENTS;
+    t.cnt_step = e->str_step;
+    t.cnt_total = e->str_total;
+    return t;
+}
+
+// true when topology reactions can happen (reactions registered and some replica has topologies)
+bool topo_reactions_on(const rdy_engine *e) { return !e->sr_in.empty() && e->tt_used > 0 && e->tev; }
+
+// the topology reactions of one step (search, then the greedy resolve); edits: the fused step's list
+void launch_topo_reactions(rdy_engine *e, int require_active, Edit *edits, uint32_t *edit_count) {
+    const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
+    const uint32_t n_slots = (uint32_t)e->topo_used;
+    const TopoTables tt = topo_tables(e);
+    const uint32_t *slots = edits ? e->loc_cur : e->loc;  // the fused step's walk has moved loc on
+    hipLaunchKernelGGL(k_topo_search, dim3(blocks_for(n_slots, 256)), dim3(256), 0, e->stream, e->grid(e->cur),
+                       (const ReplicaDev *)e->reps, slots, tt, n_slots, e->cap, e->radius, require_active, e->err);
+    hipLaunchKernelGGL(k_topo_resolve, dim3(nr), dim3(256), 0, e->stream, e->grid(e->cur), e->reps, tt, require_active,
+                       edits, edit_count, e->edit_cap, e->err);
+}
+
 // blocks of the bin-reading kernels (they loop over the listed bricks; none listed: they just exit)
 uint32_t slow_blocks(const rdy_engine *e) { return std::min<uint32_t>(n_bricks(e), 256u); }
 
@@ -895,18 +974,24 @@ void launch_step(rdy_engine *e, int handler, double dt, int last) {
     const int c = e->cur, n = 1 - c;
     const uint64_t nc = std::max<uint64_t>(e->cells_hwm, 1);
     const uint64_t nfb = e->fb ? 3 * e->topo_used : 0;
-    const uint64_t nthreads = std::max<uint64_t>(std::max<uint64_t>(nr, nc), nfb);
+    const uint64_t nloc = topo_reactions_on(e) ? e->topo_used : 0;
+    const uint64_t nthreads = std::max<uint64_t>(std::max<uint64_t>(std::max<uint64_t>(nr, nc), nfb), nloc);
     hipLaunchKernelGGL(k_step_begin, dim3(blocks_for(nthreads, 256)), dim3(256), 0, e->stream, e->reps, nr, last,
-                       e->grid(n), (uint32_t)nc, e->fb, (uint32_t)nfb, e->edit_count, e->e_bonded);
+                       e->grid(n), (uint32_t)nc, e->fb, (uint32_t)nfb, e->edit_count, e->e_bonded,
+                       (const uint32_t *)e->loc, e->loc_cur, (uint32_t)nloc);
     launch_bonded(e);
     const int h = (handler != RDY_HANDLER_NONE && e->n_rx > 0) ? handler : RDY_HANDLER_NONE;
     launch_walk(e, 0, h, dt, h != RDY_HANDLER_NONE);
-    if (h != RDY_HANDLER_NONE) {
+    const bool topo_rx = topo_reactions_on(e);
+    if (h != RDY_HANDLER_NONE || topo_rx) {
+        if (h != RDY_HANDLER_NONE)
         hipLaunchKernelGGL(k_resolve_step, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->grid(n),
                            e->reps, e->T, e->topo(), (const BrickDev *)e->d_bricks, e->cap, e->radius, h, dt,
                            e->events, e->ev_count, slow_list(e, 0), e->rx_step, e->rx_total,
                            (uint32_t)e->max_particles, e->edits, e->edit_count, e->edit_cap, rec_ptr(e),
                            e->recording ? e->rec_count : nullptr, e->err);
+        // SimulationLoop.h:286-287: the topology reactions follow the reactions, on their result
+        if (topo_rx) launch_topo_reactions(e, 1, e->edits, e->edit_count);
         hipLaunchKernelGGL(k_fix_sites, dim3(FIX_BLOCKS), dim3(256), 0, e->stream, e->grid(c),
                            (const ReplicaDev *)e->reps, e->T, e->cap, e->radius, (const Edit *)e->edits,
                            (const uint32_t *)e->edit_count, e->edit_cap, e->aff_bits, e->aff_list, e->edit_count + 1,
@@ -1084,7 +1169,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_D, e->d_flavor, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1237,18 +1322,31 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
         if (e->topo_used + n_topo > e->topo_cap) {
             uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + n_topo);
             long long *fb;
-            uint32_t *loc;
+            uint32_t *loc, *tof, *town, *lcur;
             HIP_TRY(dalloc(&fb, 3 * ncap));
             HIP_TRY(dalloc(&loc, ncap));
+            HIP_TRY(dalloc(&lcur, ncap));
+            HIP_TRY(dalloc(&tof, ncap));
+            HIP_TRY(dalloc(&town, ncap));
             HIP_TRY(hipMemsetAsync(fb, 0, 3 * ncap * sizeof(long long), e->stream));
+            HIP_TRY(hipMemsetAsync(tof, 0xFF, ncap * sizeof(uint32_t), e->stream));
+            HIP_TRY(hipMemsetAsync(town, 0xFF, ncap * sizeof(uint32_t), e->stream));
             if (e->loc) {
                 HIP_TRY(hipMemcpyAsync(loc, e->loc, e->topo_used * 4, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipMemcpyAsync(tof, e->topo_of, e->topo_used * 4, hipMemcpyDeviceToDevice, e->stream));
+                HIP_TRY(hipMemcpyAsync(town, e->topo_owner, e->topo_used * 4, hipMemcpyDeviceToDevice, e->stream));
                 HIP_TRY(hipStreamSynchronize(e->stream));
                 HIP_TRY(hipFree(e->loc));
                 HIP_TRY(hipFree(e->fb));
+                HIP_TRY(hipFree(e->topo_of));
+                HIP_TRY(hipFree(e->topo_owner));
+                HIP_TRY(hipFree(e->loc_cur));
             }
+            e->loc_cur = lcur;
             e->fb = fb;
             e->loc = loc;
+            e->topo_of = tof;
+            e->topo_owner = town;
             e->topo_cap = ncap;
         }
         H.topo_base = (uint32_t)e->topo_used;
@@ -1258,6 +1356,16 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
         H.topo_base = 0;
         H.n_topo = 0;
     }
+    if (H.n_topo) {  // the new range belongs to replica r; its particles have no topology yet
+        std::vector<uint32_t> own(H.n_topo, r), none(H.n_topo, 0xFFFFFFFFu);
+        HIP_TRY(hipMemcpyAsync(e->topo_owner + H.topo_base, own.data(), H.n_topo * 4, hipMemcpyHostToDevice,
+                               e->stream));
+        HIP_TRY(hipMemcpyAsync(e->topo_of + H.topo_base, none.data(), H.n_topo * 4, hipMemcpyHostToDevice,
+                               e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, tt_base), H.tt_base))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, n_tt), H.n_tt))) return rc;
     uint32_t hn = (uint32_t)n;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, topo_base), H.topo_base))) return rc;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, n_topo), H.n_topo))) return rc;
@@ -1573,6 +1681,7 @@ int rdy_get_reaction_totals(rdy_engine *e, uint32_t r, uint64_t *counts) {
 
 int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
     if (int rb = ensure_bricks(e)) return rb;
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
@@ -1593,10 +1702,49 @@ int rdy_run(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     return check_errors(e);
 }
 
+int rdy_run_begin(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if (int rb = ensure_bricks(e)) return rb;
+    if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    if (n_steps == 0) return set_err(RDY_ERR_INVALID, "a run needs at least one step");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    if ((rc = ensure_step_pools(e))) return rc;
+    e->export_valid = false;
+    launch_prologue(e, dt);
+    e->in_session = true;
+    e->session_left = n_steps;
+    e->session_handler = handler;
+    e->session_dt = dt;
+    return RDY_OK;
+}
+
+int rdy_run_enqueue(rdy_engine *e, uint64_t k) {
+    if (!e || !e->in_session) return set_err(RDY_ERR_STATE, "no open run (rdy_run_begin)");
+    if (k > e->session_left) return set_err(RDY_ERR_INVALID, "more steps than the run has left");
+    for (uint64_t s = 0; s < k; ++s) {
+        const bool last = --e->session_left == 0;
+        launch_step(e, e->session_handler, e->session_dt, last ? 1 : 0);
+        if (last) launch_epilogue(e);
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return set_err(RDY_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(le));
+    return RDY_OK;
+}
+
+int rdy_run_end(rdy_engine *e) {
+    if (!e || !e->in_session) return set_err(RDY_ERR_STATE, "no open run (rdy_run_begin)");
+    if (e->session_left) return set_err(RDY_ERR_STATE, "the run has steps left to enqueue");
+    e->in_session = false;
+    return check_errors(e);
+}
+
 int rdy_run_graph(rdy_engine *e, int handler, double dt, uint64_t n_steps) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
     if (int rb = ensure_bricks(e)) return rb;
     if (handler < 0 || handler > RDY_HANDLER_NONE) return set_err(RDY_ERR_INVALID, "unknown reaction handler");
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
     int rc = upload_tables(e, dt);
     if (rc) return rc;
     if (n_steps == 0) return RDY_OK;
@@ -1913,6 +2061,129 @@ int rdy_histogram_along_axis_noalloc(rdy_engine *e, uint32_t r, const double *bo
     HIP_TRY(hipMemcpyAsync(counts, d_c, 8 * (size_t)nbins, hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     return check_errors(e);
+}
+
+int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_reaction *sr) {
+    if (!e) return set_err(RDY_ERR_INVALID, "null engine");
+    if (n && !sr) return set_err(RDY_ERR_INVALID, "null spatial reactions");
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if (n > 64) return set_err(RDY_ERR_INVALID, "at most 64 spatial topology reactions");
+    for (uint32_t i = 0; i < n; ++i) {
+        const rdy_spatial_reaction &x = sr[i];
+        if (x.type1 >= e->n_types || x.type2 >= e->n_types || x.type_to1 >= e->n_types || x.type_to2 >= e->n_types)
+            return set_err(RDY_ERR_INVALID, "spatial topology reaction species out of range");
+        if (!e->topo_flavor[x.type1] || !e->topo_flavor[x.type_to1])
+            return set_err(RDY_ERR_INVALID, "the topology side of a topology-particle reaction must stay a topology species");
+        if (e->topo_flavor[x.type2] || e->topo_flavor[x.type_to2])
+            return set_err(RDY_ERR_INVALID, "the particle side of a topology-particle reaction must stay a normal species");
+        if (!(x.rate > 0))
+            return set_err(RDY_ERR_INVALID, "The rate of an structural topology reaction should always be positive");
+        if (!(x.radius > 0) || x.radius > e->nl_cutoff)
+            return set_err(RDY_ERR_INVALID, "a spatial topology reaction radius must be positive and at most the engine's "
+                                            "neighbour-list cutoff (Context::calculateMaxCutoff includes these radii)");
+    }
+    if (!e->pools_ready) return set_err(RDY_ERR_STATE, "add a replica first");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->sr_in.assign(sr, sr + n);
+    if (n && !e->tev) {
+        const size_t nr = e->max_reps;
+        HIP_TRY(dalloc(&e->tev, nr * TOPO_MAX_EVENTS));
+        HIP_TRY(dalloc(&e->tev_count, nr));
+        HIP_TRY(hipMemsetAsync(e->tev_count, 0, nr * sizeof(uint32_t), e->stream));
+    }
+    if (e->str_step) HIP_TRY(hipFree(e->str_step));
+    if (e->str_total) HIP_TRY(hipFree(e->str_total));
+    e->str_step = e->str_total = nullptr;
+    const size_t cn = (size_t)e->max_reps * std::max<uint32_t>(n, 1);
+    HIP_TRY(dalloc(&e->str_step, cn));
+    HIP_TRY(dalloc(&e->str_total, cn));
+    HIP_TRY(hipMemsetAsync(e->str_step, 0, cn * 8, e->stream));
+    HIP_TRY(hipMemsetAsync(e->str_total, 0, cn * 8, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->tables_dt = -1.0;  // the acceptance probabilities go up with the next tables
+    e->version++;
+    return RDY_OK;
+}
+
+int rdy_set_topologies(rdy_engine *e, uint32_t r, uint32_t n_topologies, const uint32_t *topology_types,
+                       const uint32_t *topology_of) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    HostReplica &H = e->reps_h[r];
+    if (n_topologies && !topology_types) return set_err(RDY_ERR_INVALID, "null topology types");
+    if (H.n_topo && !topology_of) return set_err(RDY_ERR_INVALID, "null topology assignment");
+    for (uint32_t h = 0; h < H.n_topo; ++h)
+        if (topology_of[h] >= n_topologies) return set_err(RDY_ERR_INVALID, "a topology particle's topology does not exist");
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (H.n_tt) release_range(e->free_tt, H.tt_base, H.n_tt);
+    H.tt_base = 0;
+    H.n_tt = 0;
+    if (n_topologies) {
+        uint64_t base = 0;
+        if (!take_range(e->free_tt, n_topologies, &base)) {
+            if (e->tt_used + n_topologies > e->tt_cap) {
+                const uint64_t ncap = std::max<uint64_t>(2 * e->tt_cap, e->tt_used + n_topologies);
+                uint32_t *nt;
+                HIP_TRY(dalloc(&nt, ncap));
+                if (e->ttype) {
+                    HIP_TRY(hipMemcpyAsync(nt, e->ttype, e->tt_used * 4, hipMemcpyDeviceToDevice, e->stream));
+                    HIP_TRY(hipStreamSynchronize(e->stream));
+                    HIP_TRY(hipFree(e->ttype));
+                }
+                e->ttype = nt;
+                e->tt_cap = ncap;
+            }
+            base = e->tt_used;
+            e->tt_used += n_topologies;
+        }
+        H.tt_base = (uint32_t)base;
+        H.n_tt = n_topologies;
+        HIP_TRY(hipMemcpyAsync(e->ttype + base, topology_types, n_topologies * 4ull, hipMemcpyHostToDevice, e->stream));
+    }
+    if (H.n_topo)
+        HIP_TRY(hipMemcpyAsync(e->topo_of + H.topo_base, topology_of, H.n_topo * 4ull, hipMemcpyHostToDevice, e->stream));
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, tt_base), H.tt_base))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, n_tt), H.n_tt))) return rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->version++;
+    return RDY_OK;
+}
+
+int rdy_get_topology_types(rdy_engine *e, uint32_t r, uint32_t cap, uint32_t *types, uint32_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    const HostReplica &H = e->reps_h[r];
+    if (n) *n = H.n_tt;
+    const uint32_t m = std::min(cap, H.n_tt);
+    if (m && types) {
+        HIP_TRY(hipMemcpyAsync(types, e->ttype + H.tt_base, m * 4ull, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return RDY_OK;
+}
+
+int rdy_evaluate_topology_reactions(rdy_engine *e, double dt) {
+    if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    int rc = upload_tables(e, dt);
+    if (rc) return rc;
+    if (!topo_reactions_on(e)) return RDY_OK;
+    launch_topo_reactions(e, 0, nullptr, nullptr);
+    e->export_valid = false;  // species changed: forces and energies of the state are formed anew
+    return check_errors(e);
+}
+
+int rdy_get_topology_reaction_counts(rdy_engine *e, uint32_t r, int totals, uint64_t *counts) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    const size_t n = e->sr_in.size();
+    if (!n) return RDY_OK;
+    if (!counts) return set_err(RDY_ERR_INVALID, "null counts");
+    HIP_TRY(hipMemcpyAsync(counts, (totals ? e->str_total : e->str_step) + (size_t)r * n, n * 8,
+                           hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
 }
 
 int rdy_profile_reset(rdy_engine *e) {

@@ -27,6 +27,7 @@ struct Tables {
     const RxPairIdx *o2idx;  // [n_types * n_types]
     const uint32_t *o2list;
     const double *diffusion; // [n_types]
+    const double *bd_scale;  // [n_types] sqrt(2 D dt) of the tables' time step
     const float2 *rowmax;    // [n_types] largest FP32 squared cutoff of the type's potentials (x) and of
                              // its order-2 reactions (y); -1 where it has none
     uint32_t n_types;

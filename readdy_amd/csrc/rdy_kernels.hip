@@ -95,6 +95,17 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << __lane_id()) - 1ull;
 }
 
+// set bits of a wave mask in the lanes below this one (v_mbcnt_lo / v_mbcnt_hi)
+__device__ __forceinline__ uint32_t rank_in_mask(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// squared length of an FP32 local-frame displacement for the conservative prefilters: fused
+// multiply-adds round no worse than the separate products and sums the prefilter margin allows for
+__device__ __forceinline__ float prefilter_d2(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
 // wave-aggregated append: returns the position for lanes with want == true
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
     unsigned long long mask = __ballot(want);
@@ -316,6 +327,42 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_wl_write(Grid g, uint32_t cap,
     }
 }
 
+// One Euler-Maruyama move (CPUEulerBDIntegrator.cpp:63-73): per axis x += sqrt(2 D dt) xi + F D dt / kBT,
+// then fixPosition.  sqrt(2 D dt) comes from the tables; the drift keeps the reference's operation order
+// (F D dt, then / kBT, skipped when kBT is 1, which leaves the value unchanged).  Every move of the
+// engine (separate integrator, fused step, fix-up) goes through this function, so they agree bit for bit.
+__device__ __forceinline__ void em_move(const Tables &T, uint32_t type, double dt, const double F[3],
+                                        const double xi[3], const double *box, const int32_t *pbc, double p[3],
+                                        uint32_t *err) {
+    const double D = T.diffusion[type];
+    const double sc = T.bd_scale[type];
+    const double kbt = T.kbt;
+    const bool unit_kbt = kbt == 1.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double det = F[d] * D * dt;
+        if (!unit_kbt) det = det / kbt;
+        const double rnd = xi[d] * sc;
+        p[d] += rnd + det;
+        p[d] = wrap(p[d], box[d], pbc[d]);
+        if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
+    }
+}
+
+// the Brownian-dynamics noise of handle h at Philox step `step`: the injected buffer when given,
+// else the counter (h, step, 0, 0) of the replica's key
+__device__ __forceinline__ void bd_noise(const double *noise, uint32_t h, uint32_t step, uint64_t seed, double xi[3]) {
+    if (noise) {
+        xi[0] = noise[3ull * h];
+        xi[1] = noise[3ull * h + 1];
+        xi[2] = noise[3ull * h + 2];
+    } else {
+        uint32_t x4[4];
+        philox(h, step, 0u, 0u, seed, x4);
+        bd_normals3(x4, xi);
+    }
+}
+
 // separate integrator action: in place on the current buffer, forces from force_out
 __global__ __launch_bounds__(256) void k_integrate(Grid cur, ReplicaDev *reps, Tables T, uint32_t cap, double dt,
                                                     const double *noise, const double *force_in, uint32_t max_wl,
@@ -329,27 +376,12 @@ __global__ __launch_bounds__(256) void k_integrate(Grid cur, ReplicaDev *reps, T
     if (me.tw & TW_DEAD) return;
     const ReplicaDev &R = reps[tw_replica(me.tw)];
     if (!R.active) return;
-    const double D = T.diffusion[tw_type(me.tw)];
     double xi[3];
-    if (noise) {
-        xi[0] = noise[3ull * me.h];
-        xi[1] = noise[3ull * me.h + 1];
-        xi[2] = noise[3ull * me.h + 2];
-    } else {
-        uint32_t x4[4];
-        philox(me.h, R.step, 0u, 0u, R.seed, x4);
-        normals3(x4, xi);
-    }
-    const double sc = sqrt(2. * D * dt);
+    bd_noise(noise, me.h, R.step, R.seed, xi);
+    const double F[3] = {force_in ? force_in[3ull * s] : 0.0, force_in ? force_in[3ull * s + 1] : 0.0,
+                         force_in ? force_in[3ull * s + 2] : 0.0};
     double p[3] = {me.x, me.y, me.z};
-    for (int d = 0; d < 3; ++d) {
-        double F = force_in ? force_in[3ull * s + d] : 0.0;
-        double det = F * D * dt / T.kbt;
-        double rnd = xi[d] * sc;
-        p[d] += rnd + det;
-        p[d] = wrap(p[d], R.box[d], R.pbc[d]);
-        if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
-    }
+    em_move(T, tw_type(me.tw), dt, F, xi, R.box, R.pbc, p, err);
     me.x = p[0];
     me.y = p[1];
     me.z = p[2];

@@ -45,22 +45,10 @@ __device__ __forceinline__ uint32_t advance_particle(const Grid &nxt, const Repl
         }
         if (fixed_saturated(fx) || fixed_saturated(fy) || fixed_saturated(fz)) atomicAdd(&err[ERR_FORCE_RANGE], 1u);
         const double Fv[3] = {(double)fx * FORCE_INV, (double)fy * FORCE_INV, (double)fz * FORCE_INV};
-        const double D = T.diffusion[tw_type(me.tw)];
         double xi[3];
-        uint32_t x4[4];
-        philox(me.h, step, 0u, 0u, R.seed, x4);
-        normals3(x4, xi);
-        const double sc = sqrt(2. * D * dt);
-        const double kbt = T.kbt;
+        bd_noise(nullptr, me.h, step, R.seed, xi);
         double p[3] = {me.x, me.y, me.z};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double det = Fv[d] * D * dt / kbt;
-            double rnd = xi[d] * sc;
-            p[d] += rnd + det;
-            p[d] = wrap(p[d], geo.box[d], geo.pbc[d]);
-            if (!isfinite(p[d])) atomicAdd(&err[ERR_NONFINITE], 1u);
-        }
+        em_move(T, tw_type(me.tw), dt, Fv, xi, geo.box, geo.pbc, p, err);
         nw.x = p[0];
         nw.y = p[1];
         nw.z = p[2];
@@ -73,9 +61,13 @@ __device__ __forceinline__ uint32_t advance_particle(const Grid &nxt, const Repl
 // Step t's bookkeeping before its walk: replicas with budget left take the step (Philox step t) and
 // integrate step t+1 in the same walk unless this is their last step or the launch's (last);
 // clears buffer `clear` (the walk's target), the change lists and the bonded forces.
+// With topology reactions on, it also keeps each topology particle's slot in the buffer the walk reads
+// (loc_cur): the walk's moves point loc at the other buffer, and the topology reactions run after it.
 __global__ void k_step_begin(ReplicaDev *reps, uint32_t n_reps, int last, Grid clear, uint32_t n_cells,
-                             long long *fb, uint32_t n_fb, uint32_t *counters, unsigned long long *e_bonded) {
+                             long long *fb, uint32_t n_fb, uint32_t *counters, unsigned long long *e_bonded,
+                             const uint32_t *loc, uint32_t *loc_cur, uint32_t n_loc) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_loc) loc_cur[i] = loc[i];
     if (i < n_reps) {
         ReplicaDev &R = reps[i];
         R.energy = 0ull;
@@ -411,7 +403,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
             const uint32_t tw2 = __float_as_uint(pq.w);
             float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
             if constexpr (!AH) frame_min_image(F, dx, dy, dz);
-            const float d2p = dx * dx + dy * dy + dz * dz;
+            const float d2p = prefilter_d2(dx, dy, dz);
             const float thr = thr_row[tw_type(tw2)];
             return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
         };

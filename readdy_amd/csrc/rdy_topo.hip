@@ -1,0 +1,236 @@
+// Spatial topology reactions of the topology-particle enzymatic kind, "T1(p1) + (p2) -> T2(p3) + (p4)"
+// (SpatialTopologyReaction.h:64-236; CPUEvaluateTopologyReactions.cpp:88-475).
+//
+//   k_topo_search   one thread per topology particle: every live particle of no topology within the
+//                   reaction's radius in the particle's stencil (distSquared < radius^2), for each reaction
+//                   registered under (its species, its topology's type, the partner's species), draws the
+//                   acceptance 1 - exp(-rate dt) (Philox counter (topology particle handle, step,
+//                   6 | reaction << 8, partner handle)) and appends the accepted candidates with their
+//                   Efraimidis-Spirakis keys -ln(u) / rate to the replica's segment
+//   k_topo_resolve  one workgroup per replica: the greedy pass over the accepted candidates in key order
+//                   (performEvents, algorithm.h:62-136, with eventsDependent, :455-464: events of one
+//                   topology depend on each other, and so do events sharing the free particle), then the
+//                   winners' type changes: the topology particle takes p3, the free particle p4, the
+//                   topology the type T2 (handleTopologyParticleReaction, :369-404)
+//
+// In the fused step both run after the step's reactions, on the post-reaction state, and list the
+// records they retype for the fix-up (their moves, and those of their neighbours whose pair force
+// changed, are redone).  Particles keep their handles.  The bonded terms are not reconfigured here: a
+// reaction that changes a topology particle's species keeps that particle's bonded parameters (the host
+// core reconfigures the topology when the species change alters them).
+#pragma once
+
+#include "rdy_step.hip"
+
+namespace rdy {
+
+struct SpatialDev {
+    uint32_t type1, top_type1, type2, type_to1, top_type_to1, type_to2;
+    double rate, p_accept, radius2;
+};
+
+struct TopoEvent {
+    unsigned long long key;
+    uint32_t s_top, s_free;  // bin slots of the topology particle and of the free particle
+    uint32_t h_top, h_free;  // their handles
+    uint32_t top;            // topology index within the replica
+    uint32_t sr;             // reaction index
+};
+static_assert(sizeof(TopoEvent) == 32, "TopoEvent layout");
+
+struct TopoTables {
+    const SpatialDev *sr;
+    uint32_t n_sr;
+    const uint32_t *topo_of;     // [topology particle slot] topology index within its replica
+    const uint32_t *topo_owner;  // [topology particle slot] replica, UINT32_MAX when unused
+    uint32_t *ttype;             // topology types; replica r's at reps[r].tt_base
+    TopoEvent *ev;               // [replica][ev_cap]
+    uint32_t *ev_count;          // [replica]
+    uint32_t ev_cap;
+    unsigned long long *cnt_step, *cnt_total;  // [replica][n_sr]: last evaluation, since the replica was added
+};
+
+constexpr uint32_t TOPO_MAX_EVENTS = 1024;  // accepted candidates per replica per evaluation (LDS resident)
+constexpr uint32_t TOPO_HASH = 4096;        // LDS hash-set entries for the used topologies and partners
+
+// slots: each topology particle's slot in cur (loc, or its copy from the start of a fused step)
+__global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev *reps, const uint32_t *slots,
+                                                     TopoTables TT, uint32_t n_slots, uint32_t cap, int radius,
+                                                     int require_active, uint32_t *err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const uint32_t r = TT.topo_owner[i];
+    if (r == 0xFFFFFFFFu) return;
+    const ReplicaDev &R = reps[r];
+    if (!R.in_use || (require_active && !R.active)) return;
+    if (i < R.topo_base || i - R.topo_base >= R.n_topo) return;
+    const uint32_t k = TT.topo_of[i];
+    if (k >= R.n_tt) return;
+    const uint32_t slot = slots[i];
+    const State me = cur.bins[slot];
+    if (me.tw & (TW_DEAD | TW_OUTSIDE)) return;
+    const uint32_t t = tw_type(me.tw), T = TT.ttype[R.tt_base + k];
+    bool any = false;
+    for (uint32_t j = 0; j < TT.n_sr; ++j) any = any || (TT.sr[j].type1 == t && TT.sr[j].top_type1 == T);
+    if (!any) return;
+    const RepGeom geo = load_geom(R);
+    for_each_stencil_slot(geo, cur, cap, radius, cell_of_slot(cur, slot, cap) - geo.cell_base, [&](uint32_t s2) {
+        if (s2 == slot) return;
+        const State o = cur.bins[s2];
+        if (o.tw & (TW_DEAD | TW_OUTSIDE | TW_TOPO)) return;
+        const uint32_t t2 = tw_type(o.tw);
+        // distSquared(entry.pos, neighbor.pos) (CPUEvaluateTopologyReactions.cpp:278)
+        const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
+        const double dy = min_image(o.y - me.y, geo.box[1], geo.pbc[1]);
+        const double dz = min_image(o.z - me.z, geo.box[2], geo.pbc[2]);
+        const double d2 = dist2(dx, dy, dz);
+        for (uint32_t j = 0; j < TT.n_sr; ++j) {
+            const SpatialDev X = TT.sr[j];
+            if (X.type1 != t || X.top_type1 != T || X.type2 != t2 || !(d2 < X.radius2)) continue;
+            uint32_t x4[4];
+            philox(me.h, R.step, 6u | (j << 8), o.h, R.seed, x4);
+            if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
+            const uint32_t p = atomicAdd(&TT.ev_count[r], 1u);
+            if (p >= TT.ev_cap) {
+                atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
+                continue;
+            }
+            TopoEvent ev;
+            ev.key = event_key(1, x4, X.rate);
+            ev.s_top = slot;
+            ev.s_free = s2;
+            ev.h_top = me.h;
+            ev.h_free = o.h;
+            ev.top = k;
+            ev.sr = j;
+            TT.ev[(size_t)r * TT.ev_cap + p] = ev;
+        }
+    });
+}
+
+// inserts v into an LDS open-addressing set; false if it was there already
+__device__ __forceinline__ bool hash_insert(uint32_t *set, uint32_t v) {
+    uint32_t h = (v * 2654435761u) & (TOPO_HASH - 1);
+    for (;;) {
+        const uint32_t cur = set[h];
+        if (cur == v) return false;
+        if (cur == 0xFFFFFFFFu) {
+            set[h] = v;
+            return true;
+        }
+        h = (h + 1) & (TOPO_HASH - 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps, TopoTables TT, int require_active,
+                                                      Edit *edits, uint32_t *edit_count, uint32_t edit_cap,
+                                                      uint32_t *err) {
+    __shared__ unsigned long long skey[TOPO_MAX_EVENTS];
+    __shared__ uint32_t order[TOPO_MAX_EVENTS];
+    __shared__ uint8_t won[TOPO_MAX_EVENTS];
+    __shared__ uint32_t used_top[TOPO_HASH], used_free[TOPO_HASH];
+    const uint32_t r = blockIdx.x, tid = threadIdx.x;
+    ReplicaDev &R = reps[r];
+    if (!R.in_use || (require_active && !R.active)) {
+        if (tid == 0) TT.ev_count[r] = 0u;
+        return;
+    }
+    if (tid < TT.n_sr) TT.cnt_step[(size_t)r * TT.n_sr + tid] = 0ull;
+    uint32_t ne = TT.ev_count[r];
+    ne = ne < TT.ev_cap ? ne : TT.ev_cap;
+    if (ne > TOPO_MAX_EVENTS) {
+        if (tid == 0) atomicAdd(&err[ERR_EVENT_LDS], 1u);
+        ne = TOPO_MAX_EVENTS;
+    }
+    __syncthreads();  // every thread has read the count before thread 0 clears it below
+    if (ne == 0) {
+        if (tid == 0) TT.ev_count[r] = 0u;
+        return;
+    }
+    const TopoEvent *ev = TT.ev + (size_t)r * TT.ev_cap;
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        skey[k] = ev[k].key;
+        won[k] = 0;
+    }
+    for (uint32_t k = tid; k < TOPO_HASH; k += blockDim.x) {
+        used_top[k] = 0xFFFFFFFFu;
+        used_free[k] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // rank of each event under (key, topology particle handle, free particle handle, reaction): a strict
+    // total order on distinct candidates, so the ranks are a permutation
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        const TopoEvent a = ev[k];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < ne; ++j) {
+            const unsigned long long kj = skey[j];
+            if (kj < a.key) {
+                ++rank;
+            } else if (kj == a.key && j != k) {
+                const TopoEvent b = ev[j];
+                if (b.h_top != a.h_top ? b.h_top < a.h_top : (b.h_free != a.h_free ? b.h_free < a.h_free : b.sr < a.sr))
+                    ++rank;
+            }
+        }
+        order[rank] = k;
+    }
+    __syncthreads();
+    if (tid == 0) {  // the greedy pass in key order (few accepted events per step)
+        for (uint32_t q = 0; q < ne; ++q) {
+            const TopoEvent a = ev[order[q]];
+            bool top_free = true, partner_free = true;
+            {  // probe without inserting: an event is performed only when both are unused
+                uint32_t h = (a.top * 2654435761u) & (TOPO_HASH - 1);
+                for (;;) {
+                    const uint32_t c = used_top[h];
+                    if (c == a.top) {
+                        top_free = false;
+                        break;
+                    }
+                    if (c == 0xFFFFFFFFu) break;
+                    h = (h + 1) & (TOPO_HASH - 1);
+                }
+                h = (a.s_free * 2654435761u) & (TOPO_HASH - 1);
+                for (;;) {
+                    const uint32_t c = used_free[h];
+                    if (c == a.s_free) {
+                        partner_free = false;
+                        break;
+                    }
+                    if (c == 0xFFFFFFFFu) break;
+                    h = (h + 1) & (TOPO_HASH - 1);
+                }
+            }
+            if (!top_free || !partner_free) continue;
+            hash_insert(used_top, a.top);
+            hash_insert(used_free, a.s_free);
+            won[order[q]] = 1;
+        }
+    }
+    __syncthreads();
+    const bool fix = edits != nullptr;
+    auto edit = [&](uint32_t slot, uint32_t old_type, uint32_t handle) {
+        const uint32_t p = atomicAdd(edit_count, 1u);
+        if (p < edit_cap) edits[p] = Edit{slot, EDIT_RETYPED | (old_type << 2), r, handle};
+        else atomicAdd(&err[ERR_EDIT_OVERFLOW], 1u);
+    };
+    for (uint32_t k = tid; k < ne; k += blockDim.x) {
+        if (!won[k]) continue;
+        const TopoEvent a = ev[k];
+        const SpatialDev X = TT.sr[a.sr];
+        const uint32_t tw1 = cur.bins[a.s_top].tw, tw2 = cur.bins[a.s_free].tw;
+        // a record the step's reactions changed already carries its change for the fix-up (TW_NEW), which
+        // redoes its move on the final state; every other retyped record is listed once here
+        const uint32_t n1 = fix && !(tw1 & TW_NEW) ? TW_NEW : 0u, n2 = fix && !(tw2 & TW_NEW) ? TW_NEW : 0u;
+        cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
+        cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
+        TT.ttype[R.tt_base + a.top] = X.top_type_to1;
+        atomicAdd(&TT.cnt_step[(size_t)r * TT.n_sr + a.sr], 1ull);
+        atomicAdd(&TT.cnt_total[(size_t)r * TT.n_sr + a.sr], 1ull);
+        if (n1) edit(a.s_top, tw_type(tw1), a.h_top);
+        if (n2) edit(a.s_free, tw_type(tw2), a.h_free);
+    }
+    if (tid == 0) TT.ev_count[r] = 0u;
+}
+
+}  // namespace rdy

@@ -265,6 +265,49 @@ class Engine:
         fn = self._lib.rdy_run_graph if graph else self._lib.rdy_run
         L.check(fn(self._h, handler, dt, int(n_steps)))
 
+    # ---- spatial topology reactions (topology-particle enzymatic) ----
+    def set_spatial_reactions(self, reactions):
+        """reactions: dicts with type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius
+        ("T1(type1) + (type2) -> T2(type_to1) + (type_to2)")"""
+        arr = (L.SpatialReaction * max(len(reactions), 1))()
+        for i, x in enumerate(reactions):
+            for f in ("type1", "top_type1", "type2", "type_to1", "top_type_to1", "type_to2", "rate", "radius"):
+                setattr(arr[i], f, x[f])
+        self._sr, self._n_sr = arr, len(reactions)
+        L.check(self._lib.rdy_set_spatial_reactions(self._h, len(reactions), ctypes.cast(arr, ctypes.c_void_p)))
+
+    def set_topologies(self, r, topology_types, topology_of):
+        """topology_types[k]: type of topology k; topology_of[h]: topology of topology particle h"""
+        tt = np.ascontiguousarray(topology_types, dtype=np.uint32)
+        to = np.ascontiguousarray(topology_of, dtype=np.uint32)
+        L.check(self._lib.rdy_set_topologies(self._h, r, len(tt), _ptr(tt, _u32p), _ptr(to, _u32p)))
+
+    def topology_types(self, r=0):
+        n = ctypes.c_uint32()
+        L.check(self._lib.rdy_get_topology_types(self._h, r, 0, None, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        L.check(self._lib.rdy_get_topology_types(self._h, r, n.value, _ptr(out, _u32p), ctypes.byref(n)))
+        return out[:n.value]
+
+    def evaluate_topology_reactions(self, dt):
+        L.check(self._lib.rdy_evaluate_topology_reactions(self._h, dt))
+
+    def topology_reaction_counts(self, r=0, totals=False):
+        n = getattr(self, "_n_sr", 0)
+        c = np.zeros(max(n, 1), dtype=np.uint64)
+        L.check(self._lib.rdy_get_topology_reaction_counts(self._h, r, 1 if totals else 0, _ptr(c, _u64p)))
+        return c[:n]
+
+    def run_begin(self, handler, dt, n_steps):
+        """Opens a run of n_steps taken a few steps at a time (no host synchronisation in between)."""
+        L.check(self._lib.rdy_run_begin(self._h, handler, dt, int(n_steps)))
+
+    def run_enqueue(self, k=1):
+        L.check(self._lib.rdy_run_enqueue(self._h, int(k)))
+
+    def run_end(self):
+        L.check(self._lib.rdy_run_end(self._h))
+
     def set_steps(self, r, steps):
         L.check(self._lib.rdy_replica_set_steps(self._h, r, int(steps)))
 

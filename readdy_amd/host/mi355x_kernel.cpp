@@ -51,6 +51,7 @@ public:
     void setTime(scalar t) override { _time = t; }
     GraphTopology *addTopology(const std::string &type, const std::vector<Particle> &particles) override;
     std::vector<GraphTopology *> getTopologies() override {
+        syncTopologies();
         std::vector<GraphTopology *> v;
         for (auto &t : _topologies) v.push_back(t.get());
         return v;
@@ -59,19 +60,37 @@ public:
     std::vector<uint64_t> reactionCounts() const override;
     std::vector<ReactionRecord> reactionRecords() const override;
     void clear() override;
+    std::array<scalar, 9> virial() const override;
+    std::vector<unsigned long> particlesPerType() const override;
+    std::vector<uint64_t> histogramAlongAxis(const std::vector<scalar> &borders, const std::vector<ParticleTypeId> &types,
+                                             unsigned axis) const override;
+    std::vector<uint64_t> rdfCounts(const std::vector<scalar> &borders, const std::vector<ParticleTypeId> &from,
+                                    const std::vector<ParticleTypeId> &to, uint64_t *nFrom) const override;
 
     // uploads the pending particles (topology particles first) if the device copy is stale
     void flush();
     // BreakBonds: the device picks the edges that break; their topologies lose the edges, split into
     // their connected components and the bonded terms are configured again
     void breakBonds(scalar dt, const BreakConfig &config);
+    // EvaluateTopologyReactions on the device; the topologies' types and their particles' species are read
+    // back, and their bonded terms set again when a reaction changed a species they depend on
+    void evaluateTopologyReactions(scalar dt);
+    // whether a spatial topology reaction changes a species that the bonded configuration names: the
+    // device loop keeps the bonded terms as they are, so such a context runs step by step
+    bool reactionsChangeBondedTerms() const;
+    // after a device loop: the topology types and species are read back when next asked for
+    void markTopologiesStale();
 
 private:
+    void configureSpatialReactions();
+    void uploadTopologyTypes();
+    void syncTopologies();
     MI355XKernel *_k;
     scalar _time = 0;
     std::vector<Particle> _topoParticles, _freeParticles;  // pending host copies (before the first flush)
     std::vector<std::unique_ptr<GraphTopology>> _topologies;
     bool _dirty = false, _uploaded = false, _topoDirty = false;
+    bool _spatialSet = false, _topoStale = false;
 };
 
 class MI355XKernel : public Kernel {
@@ -92,22 +111,29 @@ public:
     bool runFused(const std::string &scheduler, scalar dt, std::size_t steps) override {
         engine();
         _state.flush();
+        if (_state.reactionsChangeBondedTerms()) return false;
         const int h = scheduler == "Gillespie" ? RDY_HANDLER_GILLESPIE
                       : scheduler == "UncontrolledApproximation" ? RDY_HANDLER_UNCONTROLLED_APPROXIMATION
                                                                  : RDY_HANDLER_NONE;
         syncRecording();
         ok(rdy_run_graph(_engine, h, dt, steps), "run");
+        _state.markTopologiesStale();
         return true;
     }
 
     // switches the engine's reaction records on once the context asks for them (after the first
-    // upload: the engine keeps them per replica)
+    // upload: the engine keeps them per replica), and its virial with Context::recordVirial
     void syncRecording() {
         if (_hasReplica && context().recordReactionsWithPositions() != _recording) {
             ok(rdy_record_reactions(_engine, context().recordReactionsWithPositions() ? 1 : 0), "record reactions");
             _recording = context().recordReactionsWithPositions();
         }
+        if (_engine && context().recordVirial() != _virial) {
+            ok(rdy_record_virial(_engine, context().recordVirial() ? 1 : 0), "record virial");
+            _virial = context().recordVirial();
+        }
     }
+    TopologyActionFactory *getTopologyActionFactory() override { return &_taf; }
 
     // the engine and its single replica, created from the context on first use
     rdy_engine *engine() {
@@ -160,6 +186,9 @@ public:
         cd.reactions = rxs.data();
         rdy_engine_config cfg{};
         cfg.nl_cutoff = _nlCutoff;
+        // the engine's own default covers the pair potentials and order-2 reactions; the spatial topology
+        // reactions' radii widen it as Context::calculateMaxCutoff does
+        if (_nlCutoff == 0 && !c.topologyRegistry().spatialReactions().empty()) cfg.nl_cutoff = c.calculateMaxCutoff();
         cfg.max_replicas = 1;
         ok(rdy_engine_create(&cd, &cfg, &_engine), "engine");
         return _engine;
@@ -189,6 +218,7 @@ public:
 
 private:
     MI355XStateModel _state;
+    TopologyActionFactory _taf{&_state};
 
 public:
     class Actions : public ActionFactory {
@@ -218,6 +248,7 @@ private:
     uint32_t _replica = 0;
     bool _hasReplica = false;
     bool _recording = false;
+    bool _virial = false;
     uint64_t _seed = 0;
     scalar _nlCutoff = 0;  // 0: Context::calculateMaxCutoff
 
@@ -278,7 +309,106 @@ void MI355XStateModel::flush() {
     _freeParticles.clear();
     _dirty = false;
     _uploaded = true;
+    configureSpatialReactions();
     if (_topoDirty) configureTopologies();
+}
+
+void MI355XStateModel::markTopologiesStale() {
+    _topoStale = !_k->context().topologyRegistry().spatialReactions().empty();
+}
+
+// the context's spatial topology reactions, set on the engine once (TopologyRegistry.cpp:195-223)
+void MI355XStateModel::configureSpatialReactions() {
+    if (_spatialSet) return;
+    std::vector<rdy_spatial_reaction> v;
+    for (const auto &r : _k->context().topologyRegistry().spatialReactions()) {
+        if (r.mode != STRMode::TP_ENZYMATIC)
+            throw std::invalid_argument("spatial topology reaction \"" + r.name +
+                                        "\": the MI355X kernel performs the topology-particle enzymatic kind "
+                                        "(T1(p1) + (p2) -> T2(p3) + (p4)) only");
+        rdy_spatial_reaction q{};
+        q.type1 = r.type1;
+        q.top_type1 = (uint32_t)r.topType1;
+        q.type2 = r.type2;
+        q.type_to1 = r.typeTo1;
+        q.top_type_to1 = (uint32_t)r.topTypeTo1;
+        q.type_to2 = r.typeTo2;
+        q.rate = r.rate;
+        q.radius = r.radius;
+        v.push_back(q);
+    }
+    if (!v.empty()) ok(rdy_set_spatial_reactions(_k->engine(), (uint32_t)v.size(), v.data()), "spatial reactions");
+    _spatialSet = true;
+}
+
+// the topologies' types and each topology particle's topology, for the device's topology reactions
+void MI355XStateModel::uploadTopologyTypes() {
+    const auto &reg = _k->context().topologyRegistry();
+    if (reg.spatialReactions().empty() || !_uploaded) return;
+    std::vector<uint32_t> types, of(_topoParticles.size(), 0);
+    for (std::size_t t = 0; t < _topologies.size(); ++t) {
+        types.push_back((uint32_t)reg.idOf(_topologies[t]->type()));
+        for (auto i : _topologies[t]->particleIndices()) of.at(i) = (uint32_t)t;
+    }
+    ok(rdy_set_topologies(_k->engine(), _k->replicaId(), (uint32_t)types.size(), types.data(), of.data()),
+       "topologies");
+}
+
+void MI355XStateModel::syncTopologies() {
+    if (!_topoStale) return;
+    _topoStale = false;
+    const auto &reg = _k->context().topologyRegistry();
+    std::vector<uint32_t> types(_topologies.size() + 1);
+    uint32_t n = 0;
+    ok(rdy_get_topology_types(_k->engine(), _k->replicaId(), (uint32_t)types.size(), types.data(), &n),
+       "topology types");
+    if (n != _topologies.size()) throw std::logic_error("the device holds a different number of topologies");
+    for (std::size_t t = 0; t < n; ++t) _topologies[t]->setType(reg.nameOf(types[t]));
+    // the topology particles are the device handles 0 .. n_topo - 1 (they are uploaded first)
+    for (const auto &p : getParticles())
+        if (p.id() < _topoParticles.size()) _topoParticles[p.id()] = Particle(p.pos(), p.type(), _topoParticles[p.id()].id());
+}
+
+bool MI355XStateModel::reactionsChangeBondedTerms() const {
+    const auto &reg = _k->context().topologyRegistry();
+    auto named = [&](ParticleTypeId t) {
+        for (const auto &kv : reg.bonds)
+            if (std::get<0>(kv.first) == t || std::get<1>(kv.first) == t) return true;
+        for (const auto &kv : reg.angles)
+            if (std::get<0>(kv.first) == t || std::get<1>(kv.first) == t || std::get<2>(kv.first) == t) return true;
+        for (const auto &kv : reg.torsions)
+            if (std::get<0>(kv.first) == t || std::get<1>(kv.first) == t || std::get<2>(kv.first) == t ||
+                std::get<3>(kv.first) == t)
+                return true;
+        return false;
+    };
+    for (const auto &r : reg.spatialReactions())
+        if (r.typeTo1 != r.type1 && (named(r.type1) || named(r.typeTo1))) return true;
+    return false;
+}
+
+// CPUEvaluateTopologyReactions::perform (CPUEvaluateTopologyReactions.cpp:88-475), spatial reactions only
+void MI355XStateModel::evaluateTopologyReactions(scalar dt) {
+    flush();
+    const auto &sr = _k->context().topologyRegistry().spatialReactions();
+    if (sr.empty() || !_k->hasReplica() || _topologies.empty()) return;
+    if (_topoDirty) configureTopologies();
+    ok(rdy_evaluate_topology_reactions(_k->engine(), dt), "topology reactions");
+    std::vector<uint64_t> c(sr.size(), 0);
+    ok(rdy_get_topology_reaction_counts(_k->engine(), _k->replicaId(), 0, c.data()), "topology reaction counts");
+    bool any = false, retyped = false;
+    for (std::size_t i = 0; i < sr.size(); ++i)
+        if (c[i]) {
+            any = true;
+            retyped = retyped || sr[i].typeTo1 != sr[i].type1;
+        }
+    if (!any) return;
+    _topoStale = true;
+    syncTopologies();
+    if (retyped) {  // GraphTopology::configure after the reaction (TopologyReactionActions.h)
+        _topoDirty = true;
+        configureTopologies();
+    }
 }
 
 // GraphTopology::configure (GraphTopology.cpp:59-143) for every topology, flattened per replica
@@ -331,6 +461,7 @@ void MI355XStateModel::configureTopologies() {
     ok(rdy_set_bonds(e, r, bk.size(), b2.data(), bk.data(), bl.data()), "bonds");
     ok(rdy_set_angles(e, r, ak.size(), a3.data(), ak.data(), at.data()), "angles");
     ok(rdy_set_dihedrals(e, r, dk.size(), d4.data(), dk.data(), mult.data(), dp.data()), "dihedrals");
+    uploadTopologyTypes();
     _topoDirty = false;
 }
 
@@ -395,6 +526,59 @@ scalar MI355XStateModel::energy() const {
     double en = 0;
     ok(rdy_get_energy(const_cast<MI355XKernel *>(_k)->engine(), _k->replicaId(), &en), "energy");
     return en;
+}
+
+std::array<scalar, 9> MI355XStateModel::virial() const {
+    std::array<scalar, 9> v{};
+    if (!_k->hasReplica()) return v;
+    auto *k = const_cast<MI355XKernel *>(_k);
+    k->syncRecording();
+    ok(rdy_get_virial(k->engine(), _k->replicaId(), v.data()), "virial");
+    return v;
+}
+
+std::vector<unsigned long> MI355XStateModel::particlesPerType() const {
+    const auto nt = _k->context().particleTypes().nTypes();
+    std::vector<unsigned long> out(nt, 0);
+    if (!_k->hasReplica()) {
+        for (const auto &p : getParticles()) ++out[p.type()];
+        return out;
+    }
+    const uint32_t r = _k->replicaId();
+    std::vector<uint64_t> c((std::size_t)(r + 1) * nt, 0);
+    ok(rdy_count_types(const_cast<MI355XKernel *>(_k)->engine(), c.data(), r + 1), "particles per type");
+    for (std::size_t t = 0; t < nt; ++t) out[t] = (unsigned long)c[(std::size_t)r * nt + t];
+    return out;
+}
+
+std::vector<uint64_t> MI355XStateModel::histogramAlongAxis(const std::vector<scalar> &borders,
+                                                           const std::vector<ParticleTypeId> &types,
+                                                           unsigned axis) const {
+    if (borders.size() < 2) throw std::invalid_argument("histogram needs at least two bin borders");
+    std::vector<uint64_t> c(borders.size() - 1, 0);
+    if (!_k->hasReplica()) return c;
+    std::vector<uint32_t> t(types.begin(), types.end());
+    ok(rdy_histogram_along_axis(const_cast<MI355XKernel *>(_k)->engine(), _k->replicaId(), borders.data(),
+                                (uint32_t)borders.size(), t.data(), (uint32_t)t.size(), axis, c.data()),
+       "histogram along axis");
+    return c;
+}
+
+std::vector<uint64_t> MI355XStateModel::rdfCounts(const std::vector<scalar> &borders,
+                                                  const std::vector<ParticleTypeId> &from,
+                                                  const std::vector<ParticleTypeId> &to, uint64_t *nFrom) const {
+    if (borders.size() < 2) throw std::invalid_argument("radial distribution needs at least two bin borders");
+    std::vector<uint64_t> c(borders.size() - 1, 0);
+    uint64_t nf = 0;
+    if (_k->hasReplica()) {
+        std::vector<uint32_t> f(from.begin(), from.end()), t(to.begin(), to.end());
+        ok(rdy_rdf_counts(const_cast<MI355XKernel *>(_k)->engine(), _k->replicaId(), borders.data(),
+                          (uint32_t)borders.size(), f.data(), (uint32_t)f.size(), t.data(), (uint32_t)t.size(),
+                          c.data(), &nf),
+           "radial distribution");
+    }
+    if (nFrom) *nFrom = nf;
+    return c;
 }
 
 std::vector<uint64_t> MI355XStateModel::reactionCounts() const {
@@ -483,7 +667,7 @@ void MI355XStateModel::clear() {
     _topoParticles.clear();
     _freeParticles.clear();
     _topologies.clear();
-    _dirty = _topoDirty = false;
+    _dirty = _topoDirty = _topoStale = false;
     _time = 0;
 }
 
@@ -583,8 +767,10 @@ std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::detailedBalance(
     throw std::invalid_argument("DetailedBalance reaction handler is not available on the MI355X kernel");
 }
 
+// EvaluateTopologyReactions (Actions.h:343-360): the spatial topology reactions of the topology-particle
+// enzymatic kind on the device; the context's structural reactions are not registered on this kernel
 std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::evaluateTopologyReactions(scalar dt) {
-    return step_action(dt, [](scalar) {});  // no topology reactions are registered in this context
+    return step_action(dt, [k = _k](scalar h) { k->_state.evaluateTopologyReactions(h); });
 }
 
 std::unique_ptr<Action> MI355XKernel::Actions::initializeKernel() {

@@ -255,12 +255,155 @@ void TopologyRegistry::configureTorsionPotential(const std::string &t1, const st
     if (std::make_tuple(a, b, c, d) != std::make_tuple(d, c, b, a)) torsions[{d, c, b, a}].push_back(cfg);
 }
 
+// ---- TopologyRegistry: topology types and spatial reactions ----------------------------------------
+std::size_t TopologyRegistry::addType(const std::string &name) {
+    if (name.empty()) throw std::invalid_argument("a topology type needs a name");
+    for (const auto &t : _topologyTypes)
+        if (t == name) throw std::invalid_argument("the topology type \"" + name + "\" exists already");
+    _topologyTypes.push_back(name);
+    return _topologyTypes.size() - 1;
+}
+
+bool TopologyRegistry::hasType(const std::string &name) const {
+    return std::find(_topologyTypes.begin(), _topologyTypes.end(), name) != _topologyTypes.end();
+}
+
+std::size_t TopologyRegistry::idOf(const std::string &name) const {
+    for (std::size_t i = 0; i < _topologyTypes.size(); ++i)
+        if (_topologyTypes[i] == name) return i;
+    throw std::invalid_argument("unknown topology type \"" + name + "\"");
+}
+
+const std::string &TopologyRegistry::nameOf(std::size_t id) const {
+    if (id >= _topologyTypes.size()) throw std::invalid_argument("no topology type with id " + std::to_string(id));
+    return _topologyTypes[id];
+}
+
+bool TopologyRegistry::isSpatialReactionType(ParticleTypeId t) const {
+    for (const auto &r : _spatial)
+        if (r.type1 == t || r.type2 == t) return true;
+    return false;
+}
+
+namespace {
+std::string trim(const std::string &s) {
+    const auto a = s.find_first_not_of(" \t\n\r"), b = s.find_last_not_of(" \t\n\r");
+    return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// one term "T(p)" or "(p)": (particle type, topology type or "")
+std::pair<std::string, std::string> str_term(const std::string &s) {
+    const auto open = s.find('('), close = s.find(')', open == std::string::npos ? 0 : open);
+    if (open == std::string::npos || close == std::string::npos)
+        throw std::invalid_argument("The term \"" + s + "\" did not contain a particle type.");
+    return {trim(s.substr(open + 1, close - open - 1)), trim(s.substr(0, open))};
+}
+}  // namespace
+
+void TopologyRegistry::addSpatialReaction(const std::string &descriptor, scalar rate, scalar radius) {
+    // STRParser::parse_descriptor (SpatialTopologyReaction.cpp:81-236)
+    const std::string arrow = "->";
+    const auto ap = descriptor.find(arrow);
+    if (ap == std::string::npos)
+        throw std::invalid_argument("the descriptor must contain an arrow (\"->\") to indicate lhs and rhs.");
+    if (descriptor.find(arrow, ap + 1) != std::string::npos)
+        throw std::invalid_argument("the descriptor must not contain more than one arrow (\"->\").");
+    std::string lhs = trim(descriptor.substr(0, ap));
+    const std::string rhs = trim(descriptor.substr(ap + arrow.size()));
+    const auto colon = lhs.find(':');
+    if (colon == std::string::npos)
+        throw std::invalid_argument("The descriptor did not contain a colon ':' to specify the end of the name.");
+    SpatialReactionDesc d{};
+    d.name = trim(lhs.substr(0, colon));
+    lhs = trim(lhs.substr(colon + 1));
+    const auto lp = lhs.find('+');
+    if (lp == std::string::npos)
+        throw std::invalid_argument("The left hand side of the topology reaction did not contain a '+'.");
+    const auto e1 = str_term(lhs.substr(0, lp)), e2 = str_term(lhs.substr(lp + 1));
+    std::pair<std::string, std::string> p1, p2;
+    const auto rp = rhs.find('+');
+    const bool fusion = rp == std::string::npos;
+    if (fusion) {  // "T(p1--p2)", an option block after it is not part of the term
+        const auto f = str_term(rhs);
+        const auto bond = f.first.find("--");
+        if (bond == std::string::npos)
+            throw std::invalid_argument("The right-hand side was of fusion type but there was no bond \"--\" defined.");
+        p1 = {trim(f.first.substr(0, bond)), f.second};
+        p2 = {trim(f.first.substr(bond + 2)), ""};
+    } else {
+        const auto r1 = str_term(rhs.substr(0, rp)), r2 = str_term(rhs.substr(rp + 1));
+        p1 = r1;
+        p2 = r2;
+    }
+    d.type1 = _types->idOf(e1.first);
+    d.type2 = _types->idOf(e2.first);
+    d.typeTo1 = _types->idOf(p1.first);
+    d.typeTo2 = _types->idOf(p2.first);
+    if (e2.second.empty()) {  // the topology-particle case
+        d.mode = fusion ? STRMode::TP_FUSION : STRMode::TP_ENZYMATIC;
+        d.topType1 = idOf(e1.second);
+        d.topTypeTo1 = idOf(p1.second);
+        d.topType2 = d.topTypeTo2 = EmptyTopologyId;
+    } else {
+        d.topType1 = idOf(e1.second);
+        d.topType2 = idOf(e2.second);
+        d.topTypeTo1 = idOf(p1.second);
+        if (fusion) {
+            d.mode = STRMode::TT_FUSION;
+            d.topTypeTo2 = EmptyTopologyId;
+            // "[self=true]" (with an optional distance) allows a topology to fuse with itself
+            const auto ob = rhs.find('[');
+            const auto cb = ob == std::string::npos ? std::string::npos : rhs.find(']', ob);
+            if (cb != std::string::npos) {
+                std::string opts;
+                for (char ch : rhs.substr(ob + 1, cb - ob - 1))
+                    if (ch != ' ' && ch != '\t') opts += ch;
+                if (opts.find("self=true") == std::string::npos)
+                    throw std::runtime_error("Option \"self=true\" is missing. This option is currently required for "
+                                             "all valid options.");
+                d.mode = STRMode::TT_FUSION_ALLOW_SELF;
+            }
+        } else {
+            d.mode = STRMode::TT_ENZYMATIC;
+            d.topTypeTo2 = idOf(p2.second);
+        }
+    }
+    d.rate = rate;
+    d.radius = radius;
+    // TopologyRegistry::validateSpatialReaction (TopologyRegistry.cpp:238-293)
+    if (radius <= 0)
+        throw std::invalid_argument("The radius of an structural topology reaction(" + d.name +
+                                    ") should always be positive");
+    const auto &i1 = _types->infoOf(d.type1), &i2 = _types->infoOf(d.type2);
+    const auto &o1 = _types->infoOf(d.typeTo1), &o2 = _types->infoOf(d.typeTo2);
+    if (!i1.topologyFlavor && !i2.topologyFlavor)
+        throw std::invalid_argument("At least one of the educt types (" + i1.name + " and " + i2.name +
+                                    ") of topology reaction " + d.name + " need to be topology flavored!");
+    for (const auto &r : _spatial)
+        if (r.name == d.name)
+            throw std::invalid_argument("An structural topology reaction with the same name (" + d.name +
+                                        ") was already registered!");
+    const bool isFusion = d.mode == STRMode::TP_FUSION || d.mode == STRMode::TT_FUSION ||
+                          d.mode == STRMode::TT_FUSION_ALLOW_SELF;
+    if (isFusion) {
+        if (!o1.topologyFlavor || !o2.topologyFlavor)
+            throw std::invalid_argument("One or both of the target types (" + o1.name + " and " + o2.name +
+                                        ") of topology reaction " + d.name +
+                                        " did not have the topology flavor and is_fusion was set to true!");
+    } else if (i1.topologyFlavor != o1.topologyFlavor || i2.topologyFlavor != o2.topologyFlavor) {
+        throw std::invalid_argument("if is_fusion is false, flavors need to remain same (topology reaction " +
+                                    d.name + ")");
+    }
+    _spatial.push_back(d);
+}
+
 // ---- Context -------------------------------------------------------------------------------------
 Context::Context() = default;
 
 Context::Context(const Context &o)
     : _types(o._types), _potentials(o._potentials), _reactions(o._reactions), _topologies(o._topologies),
-      _box(o._box), _pbc(o._pbc), _kbt(o._kbt), _recordReactions(o._recordReactions) {
+      _box(o._box), _pbc(o._pbc), _kbt(o._kbt), _recordReactions(o._recordReactions),
+      _recordVirial(o._recordVirial) {
     rebind();
 }
 
@@ -274,6 +417,7 @@ Context &Context::operator=(const Context &o) {
     _pbc = o._pbc;
     _kbt = o._kbt;
     _recordReactions = o._recordReactions;
+    _recordVirial = o._recordVirial;
     rebind();
     return *this;
 }
@@ -290,6 +434,7 @@ scalar Context::calculateMaxCutoff() const {
     for (const auto &p : _potentials.potentialsOrder2()) c = std::max(c, p.cutoff());
     for (const auto &r : _reactions.reactions())
         if (r.educts.size() == 2) c = std::max(c, r.eductDistance);
+    for (const auto &r : _topologies.spatialReactions()) c = std::max(c, r.radius);  // Context.cpp:126-144
     return c;
 }
 
@@ -377,6 +522,285 @@ void GraphTopology::findNTuples(
                 if (n2 != n && n2 < n) triple(n2, v, n);
         }
     }
+}
+
+// ---- StateModel observable inputs: kernels without them ------------------------------------------
+std::array<scalar, 9> StateModel::virial() const { throw std::logic_error("this kernel provides no virial"); }
+std::vector<unsigned long> StateModel::particlesPerType() const {
+    throw std::logic_error("this kernel provides no particle counts per type");
+}
+std::vector<uint64_t> StateModel::histogramAlongAxis(const std::vector<scalar> &, const std::vector<ParticleTypeId> &,
+                                                     unsigned) const {
+    throw std::logic_error("this kernel provides no histogram along an axis");
+}
+std::vector<uint64_t> StateModel::rdfCounts(const std::vector<scalar> &, const std::vector<ParticleTypeId> &,
+                                            const std::vector<ParticleTypeId> &, uint64_t *) const {
+    throw std::logic_error("this kernel provides no radial distribution");
+}
+void StateModel::changeParticleType(ParticleId, ParticleTypeId) {
+    throw std::logic_error("this kernel cannot change a particle's species");
+}
+
+// ---- topology reaction actions (TopologyReactionActionFactory.h:55-101) ------------------------------
+namespace {
+
+class ChangeParticleType : public TopologyReactionAction {
+public:
+    ChangeParticleType(GraphTopology *t, StateModel *state, std::size_t vertex, ParticleTypeId to)
+        : TopologyReactionAction(t), _state(state), _vertex(vertex), _to(to) {
+        if (vertex >= t->nParticles()) throw std::invalid_argument("no such vertex in the topology");
+    }
+    void perform() override {
+        const ParticleId id = _topology->particleIndices()[_vertex];
+        bool found = false;
+        for (const auto &p : _state->getParticles())
+            if (p.id() == id) {
+                _from = p.type();
+                found = true;
+                break;
+            }
+        if (!found) throw std::logic_error("the topology's particle is not in the state model");
+        _state->changeParticleType(id, _to);
+        _done = true;
+    }
+    void undo() override {
+        if (_done) _state->changeParticleType(_topology->particleIndices()[_vertex], _from);
+        _done = false;
+    }
+
+private:
+    StateModel *_state;
+    std::size_t _vertex;
+    ParticleTypeId _to, _from = 0;
+    bool _done = false;
+};
+
+class EdgeAction : public TopologyReactionAction {
+public:
+    EdgeAction(GraphTopology *t, std::size_t v1, std::size_t v2, bool add)
+        : TopologyReactionAction(t), _v1(v1), _v2(v2), _add(add) {}
+    void perform() override {
+        if (_add) {
+            _changed = !_topology->hasEdge(_v1, _v2);
+            _topology->addEdge(_v1, _v2);
+        } else {
+            _changed = _topology->removeEdge(_v1, _v2);
+            if (!_changed) throw std::invalid_argument("the topology has no such edge");
+        }
+    }
+    void undo() override {
+        if (!_changed) return;
+        if (_add) _topology->removeEdge(_v1, _v2);
+        else _topology->addEdge(_v1, _v2);
+        _changed = false;
+    }
+
+private:
+    std::size_t _v1, _v2;
+    bool _add, _changed = false;
+};
+
+class ChangeTopologyType : public TopologyReactionAction {
+public:
+    ChangeTopologyType(GraphTopology *t, std::string to) : TopologyReactionAction(t), _to(std::move(to)) {}
+    void perform() override {
+        _from = _topology->type();
+        _topology->setType(_to);
+    }
+    void undo() override { _topology->setType(_from); }
+
+private:
+    std::string _to, _from;
+};
+
+}  // namespace
+
+TopologyActionFactory::ActionPtr TopologyActionFactory::createChangeParticleType(GraphTopology *topology,
+                                                                                 std::size_t vertex,
+                                                                                 ParticleTypeId typeTo) const {
+    return std::make_unique<ChangeParticleType>(topology, _state, vertex, typeTo);
+}
+TopologyActionFactory::ActionPtr TopologyActionFactory::createAddEdge(GraphTopology *topology, std::size_t v1,
+                                                                      std::size_t v2) const {
+    return std::make_unique<EdgeAction>(topology, v1, v2, true);
+}
+TopologyActionFactory::ActionPtr TopologyActionFactory::createRemoveEdge(GraphTopology *topology, std::size_t v1,
+                                                                         std::size_t v2) const {
+    return std::make_unique<EdgeAction>(topology, v1, v2, false);
+}
+TopologyActionFactory::ActionPtr TopologyActionFactory::createChangeTopologyType(GraphTopology *topology,
+                                                                                 const std::string &typeTo) const {
+    return std::make_unique<ChangeTopologyType>(topology, typeTo);
+}
+
+// ---- observables (readdy/main/model/observables/*.cpp; kernels/cpu/src/observables/CPUObservables.cpp)
+namespace observables {
+
+void Energy::evaluate() { result = kernel->stateModel().energy(); }
+
+void Virial::evaluate() { result = kernel->stateModel().virial(); }
+
+void NParticles::evaluate() {
+    const auto per = kernel->stateModel().particlesPerType();
+    result.clear();
+    if (typesToCount.empty()) {  // CPUNParticles::evaluate: the total when no types are listed
+        unsigned long n = 0;
+        for (auto c : per) n += c;
+        result.push_back(n);
+    } else {
+        for (auto t : typesToCount) result.push_back(t < per.size() ? per[t] : 0ul);
+    }
+}
+
+namespace {
+bool listed(const std::vector<ParticleTypeId> &types, ParticleTypeId t) {
+    return types.empty() || std::find(types.begin(), types.end(), t) != types.end();
+}
+}  // namespace
+
+void Forces::evaluate() {
+    result.clear();
+    const auto ps = kernel->stateModel().getParticles();
+    const auto fs = kernel->stateModel().getParticleForces();
+    for (std::size_t i = 0; i < ps.size(); ++i)
+        if (listed(typesToCount, ps[i].type())) result.push_back(fs[i]);
+}
+
+void Positions::evaluate() {
+    result.clear();
+    for (const auto &p : kernel->stateModel().getParticles())
+        if (listed(typesToCount, p.type())) result.push_back(p.pos());
+}
+
+void HistogramAlongAxis::evaluate() {
+    const auto c = kernel->stateModel().histogramAlongAxis(binBorders, typesToCount, axis);
+    result.assign(c.begin(), c.end());
+}
+
+RadialDistribution::RadialDistribution(Kernel *k, Stride s, std::vector<scalar> borders,
+                                       std::vector<ParticleTypeId> from, std::vector<ParticleTypeId> to,
+                                       scalar particleToDensity)
+    : Observable(k, s), binBorders(std::move(borders)), typeCountFrom(std::move(from)), typeCountTo(std::move(to)),
+      particleToDensity(particleToDensity) {
+    if (binBorders.size() < 2) throw std::invalid_argument("radial distribution needs at least two bin borders");
+    auto &centers = std::get<0>(result);
+    for (std::size_t b = 0; b + 1 < binBorders.size(); ++b) centers.push_back(.5 * (binBorders[b] + binBorders[b + 1]));
+    std::get<1>(result).assign(centers.size(), 0.0);
+}
+
+void RadialDistribution::evaluate() {
+    uint64_t nFrom = 0;
+    const auto counts = kernel->stateModel().rdfCounts(binBorders, typeCountFrom, typeCountTo, &nFrom);
+    auto &dist = std::get<1>(result);
+    // RadialDistribution.cpp:105-123
+    const scalar pi = 3.14159265358979323846;
+    for (std::size_t b = 0; b < counts.size(); ++b) {
+        const scalar lo = binBorders[b], hi = binBorders[b + 1];
+        dist[b] = (scalar)counts[b] /
+                  (4. / 3. * pi * (std::pow(hi, 3.) - std::pow(lo, 3.)) * (scalar)nFrom * particleToDensity);
+    }
+}
+
+void Particles::evaluate() {
+    auto &types = std::get<0>(result);
+    auto &ids = std::get<1>(result);
+    auto &pos = std::get<2>(result);
+    types.clear();
+    ids.clear();
+    pos.clear();
+    for (const auto &p : kernel->stateModel().getParticles()) {
+        types.push_back(p.type());
+        ids.push_back(p.id());
+        pos.push_back(p.pos());
+    }
+}
+
+void Reactions::evaluate() {
+    if (!kernel->context().recordReactionsWithPositions())
+        throw std::logic_error("the Reactions observable needs Context::recordReactionsWithPositions");
+    result = kernel->stateModel().reactionRecords();
+}
+
+void ReactionCounts::evaluate() {
+    const auto c = kernel->stateModel().reactionCounts();
+    result.clear();
+    for (std::size_t i = 0; i < c.size(); ++i) result[i] = (std::size_t)c[i];
+}
+
+std::vector<ParticleTypeId> ObservableFactory::ids(const std::vector<std::string> &names) const {
+    std::vector<ParticleTypeId> out;
+    for (const auto &n : names) out.push_back(kernel->context().particleTypes().idOf(n));
+    return out;
+}
+
+namespace {
+template <class T, class CB>
+std::unique_ptr<T> with_callback(std::unique_ptr<T> o, CB cb) {
+    o->callback() = std::move(cb);
+    return o;
+}
+}  // namespace
+
+std::unique_ptr<Energy> ObservableFactory::energy(Stride stride, ObsCallback<Energy> cb) const {
+    return with_callback(std::make_unique<Energy>(kernel, stride), std::move(cb));
+}
+std::unique_ptr<Virial> ObservableFactory::virial(Stride stride, ObsCallback<Virial> cb) const {
+    kernel->context().recordVirial() = true;  // as CPUObservableFactory::virial does (CPUObservableFactory.cpp:128)
+    return with_callback(std::make_unique<Virial>(kernel, stride), std::move(cb));
+}
+std::unique_ptr<HistogramAlongAxis> ObservableFactory::histogramAlongAxis(Stride stride, std::vector<scalar> borders,
+                                                                          std::vector<std::string> types, unsigned axis,
+                                                                          ObsCallback<HistogramAlongAxis> cb) const {
+    if (axis > 2) throw std::invalid_argument("histogram axis must be 0, 1 or 2");
+    return with_callback(std::make_unique<HistogramAlongAxis>(kernel, stride, std::move(borders), ids(types), axis),
+                         std::move(cb));
+}
+std::unique_ptr<NParticles> ObservableFactory::nParticles(Stride stride, std::vector<std::string> types,
+                                                          ObsCallback<NParticles> cb) const {
+    return with_callback(std::make_unique<NParticles>(kernel, stride, ids(types)), std::move(cb));
+}
+std::unique_ptr<Forces> ObservableFactory::forces(Stride stride, std::vector<std::string> types,
+                                                  ObsCallback<Forces> cb) const {
+    return with_callback(std::make_unique<Forces>(kernel, stride, ids(types)), std::move(cb));
+}
+std::unique_ptr<Positions> ObservableFactory::positions(Stride stride, std::vector<std::string> types,
+                                                        ObsCallback<Positions> cb) const {
+    return with_callback(std::make_unique<Positions>(kernel, stride, ids(types)), std::move(cb));
+}
+std::unique_ptr<RadialDistribution> ObservableFactory::radialDistribution(Stride stride, std::vector<scalar> borders,
+                                                                          std::vector<std::string> from,
+                                                                          std::vector<std::string> to,
+                                                                          scalar particleToDensity,
+                                                                          ObsCallback<RadialDistribution> cb) const {
+    return with_callback(
+        std::make_unique<RadialDistribution>(kernel, stride, std::move(borders), ids(from), ids(to), particleToDensity),
+        std::move(cb));
+}
+std::unique_ptr<Particles> ObservableFactory::particles(Stride stride, ObsCallback<Particles> cb) const {
+    return with_callback(std::make_unique<Particles>(kernel, stride), std::move(cb));
+}
+std::unique_ptr<Reactions> ObservableFactory::reactions(Stride stride, ObsCallback<Reactions> cb) const {
+    kernel->context().recordReactionsWithPositions() = true;
+    return with_callback(std::make_unique<Reactions>(kernel, stride), std::move(cb));
+}
+std::unique_ptr<ReactionCounts> ObservableFactory::reactionCounts(Stride stride, ObsCallback<ReactionCounts> cb) const {
+    return with_callback(std::make_unique<ReactionCounts>(kernel, stride), std::move(cb));
+}
+
+}  // namespace observables
+
+// ---- Kernel observable bookkeeping --------------------------------------------------------------
+void Kernel::connectObservable(observables::ObservableBase *o) {
+    if (!o) throw std::invalid_argument("null observable");
+    if (std::find(_observables.begin(), _observables.end(), o) == _observables.end()) _observables.push_back(o);
+}
+
+void Kernel::disconnectObservable(observables::ObservableBase *o) {
+    _observables.erase(std::remove(_observables.begin(), _observables.end(), o), _observables.end());
+}
+
+void Kernel::evaluateObservables(observables::TimeStep t) {
+    for (auto *o : _observables) o->call(t);
 }
 
 }  // namespace model
@@ -469,28 +893,73 @@ void SimulationLoop::run(std::size_t steps) {
     auto &state = _kernel->stateModel();
     const bool reactions = _reactions && !_kernel->context().reactions().reactions().empty();
     const std::string scheduler = reactions ? _scheduler : std::string("none");
-    if (_callbacks.empty() && _forces && _kernel->runFused(scheduler, _dt, steps)) {
-        state.setTime(state.time() + _dt * (scalar)steps);
-        return;
+    const auto &obs = _kernel->connectedObservables();
+    const observables::TimeStep t0 = _t;
+    // the device loop performs the registered topology reactions in every step
+    const bool fusable = _topologyReactions || _kernel->context().topologyRegistry().spatialReactions().empty();
+    if (_callbacks.empty() && _forces && fusable) {
+        if (obs.empty()) {
+            if (_kernel->runFused(scheduler, _dt, steps)) {
+                state.setTime(state.time() + _dt * (scalar)steps);
+                _t += steps;
+                return;
+            }
+        } else {
+            // the loop evaluates the observables before the first step (after the initial forces) and
+            // after every step (SimulationLoop.h:270-300); each observable decides by its stride, so the
+            // device runs fused chunks that end where the next observable is due
+            auto next_due = [&](observables::TimeStep t) {
+                observables::TimeStep n = t0 + steps;
+                for (auto *o : obs) {
+                    const auto s = o->stride();
+                    n = std::min<observables::TimeStep>(n, s == 0 ? t + 1 : (t / s + 1) * s);
+                }
+                return n;
+            };
+            observables::TimeStep t = t0;
+            bool fused = true, started = false;
+            while (t < t0 + steps) {
+                if (!started) {
+                    actions.calculateForces()->perform();  // the loop's runForces() before its first evaluation
+                    _kernel->evaluateObservables(t);
+                    started = true;
+                }
+                const observables::TimeStep n = next_due(t);
+                if (!_kernel->runFused(scheduler, _dt, n - t)) {
+                    fused = false;
+                    break;
+                }
+                state.setTime(state.time() + _dt * (scalar)(n - t));
+                t = n;
+                _t = t;
+                _kernel->evaluateObservables(t);
+            }
+            if (fused) return;
+            steps -= (std::size_t)(t - t0);  // nothing ran fused: the separate actions below take all steps
+        }
     }
     auto integrator = actions.eulerBDIntegrator(_dt);
     auto forces = actions.calculateForces();
     auto nl = actions.updateNeighborList();
-    std::unique_ptr<model::TimeStepDependentAction> react;
+    std::unique_ptr<model::TimeStepDependentAction> react, topoReact;
     if (reactions)
         react = scheduler == "Gillespie" ? actions.gillespie(_dt) : actions.uncontrolledApproximation(_dt);
+    if (_topologyReactions && !_kernel->context().topologyRegistry().spatialReactions().empty())
+        topoReact = actions.evaluateTopologyReactions(_dt);
     nl->perform();
     if (_forces) forces->perform();
+    _kernel->evaluateObservables(_t);
     for (auto &cb : _callbacks) cb(0);
     for (std::size_t t = 0; t < steps; ++t) {
         integrator->perform();
         nl->perform();
-        if (react) {
-            react->perform();
-            nl->perform();
-        }
+        if (react) react->perform();
+        if (topoReact) topoReact->perform();  // SimulationLoop.h:286-287
+        if (react || topoReact) nl->perform();
         if (_forces) forces->perform();
         state.setTime(state.time() + _dt);
+        ++_t;
+        _kernel->evaluateObservables(_t);
         for (auto &cb : _callbacks) cb(t + 1);
     }
 }

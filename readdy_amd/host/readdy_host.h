@@ -153,9 +153,36 @@ struct BondConfiguration { scalar forceConstant, length; };                     
 struct AngleConfiguration { scalar forceConstant, equilibriumAngle; };
 struct TorsionConfiguration { scalar forceConstant; unsigned multiplicity; scalar phi_0; };
 
+enum class STRMode { TT_ENZYMATIC, TT_FUSION, TT_FUSION_ALLOW_SELF, TP_ENZYMATIC, TP_FUSION };  // SpatialTopologyReaction.h:64-67
+
+// a spatial topology reaction as STRParser reads it (SpatialTopologyReaction.cpp:81-220); the second
+// topology type is EmptyTopologyId for the topology-particle kinds
+struct SpatialReactionDesc {
+    std::string name;
+    STRMode mode;
+    ParticleTypeId type1, type2, typeTo1, typeTo2;
+    std::size_t topType1, topType2, topTypeTo1, topTypeTo2;
+    scalar rate, radius;
+    bool isTopologyParticle() const { return mode == STRMode::TP_ENZYMATIC || mode == STRMode::TP_FUSION; }
+};
+
 class TopologyRegistry {
 public:
+    static constexpr std::size_t EmptyTopologyId = static_cast<std::size_t>(-1);
     explicit TopologyRegistry(const ParticleTypeRegistry *types) : _types(types) {}
+    // topology types (TopologyRegistry::addType); ids in registration order
+    std::size_t addType(const std::string &name);
+    std::size_t idOf(const std::string &name) const;  // throws std::invalid_argument for an unknown type
+    bool hasType(const std::string &name) const;
+    const std::string &nameOf(std::size_t id) const;
+    // TopologyRegistry::addSpatialReaction(descriptor, rate, radius) (TopologyRegistry.cpp:204-223):
+    //   "name: T1(p1) + (p2) -> T2(p3) + (p4)"  topology-particle enzymatic
+    //   "name: T1(p1) + (p2) -> T2(p3--p4)"     topology-particle fusion
+    //   "name: T1(p1) + T2(p2) -> T3(p3) + T4(p4)" / "... -> T3(p3--p4) [self=true]"  topology-topology kinds
+    // Malformed descriptors, unknown types and a non-positive rate throw std::invalid_argument.
+    void addSpatialReaction(const std::string &descriptor, scalar rate, scalar radius);
+    const std::vector<SpatialReactionDesc> &spatialReactions() const { return _spatial; }
+    bool isSpatialReactionType(ParticleTypeId t) const;
     void rebind(const ParticleTypeRegistry *types) { _types = types; }  // after a Context copy
     void configureBondPotential(const std::string &t1, const std::string &t2, BondConfiguration cfg);
     void configureAnglePotential(const std::string &t1, const std::string &t2, const std::string &t3,
@@ -171,6 +198,8 @@ public:
 
 private:
     const ParticleTypeRegistry *_types;
+    std::vector<std::string> _topologyTypes;
+    std::vector<SpatialReactionDesc> _spatial;
 };
 
 class Context {
@@ -195,6 +224,9 @@ public:
     // Context::recordReactionsWithPositions (Context.h): keep each reaction step's records
     bool &recordReactionsWithPositions() { return _recordReactions; }
     bool recordReactionsWithPositions() const { return _recordReactions; }
+    // Context::recordVirial (Context.h:168-173): force evaluations also form the pair virial
+    bool &recordVirial() { return _recordVirial; }
+    bool recordVirial() const { return _recordVirial; }
     // Context::calculateMaxCutoff (Context.cpp:126-144): pair potentials and order-2 educt distances
     scalar calculateMaxCutoff() const;
 
@@ -208,6 +240,7 @@ private:
     std::array<bool, 3> _pbc{{true, true, true}};
     scalar _kbt = 1;
     bool _recordReactions = false;
+    bool _recordVirial = false;
 };
 
 class Particle {  // Particle.h:60-120
@@ -239,6 +272,7 @@ public:
     std::vector<GraphTopology> connectedComponents() const;
     const std::vector<std::size_t> &particleIndices() const { return _particles; }
     const std::string &type() const { return _type; }
+    void setType(std::string type) { _type = std::move(type); }
     // visits every edge, 3-path and 4-path once, in the reference's enumeration order
     void findNTuples(const std::function<void(std::size_t, std::size_t)> &pair,
                      const std::function<void(std::size_t, std::size_t, std::size_t)> &triple,
@@ -324,7 +358,207 @@ public:
     // of the last reaction handler call, when the context records them (empty otherwise)
     virtual std::vector<ReactionRecord> reactionRecords() const = 0;
     virtual void clear() = 0;
+    // observable inputs (the reference's observables read them from the state model): the pair virial
+    // of the last force evaluation (row-major, needs Context::recordVirial), particles per type, and the
+    // integer counts of HistogramAlongAxis (CPUObservables.cpp:85-136) and of the radial distribution's
+    // pair loop (RadialDistribution.cpp:75-103, with the number of "from" particles).  A kernel that does
+    // not provide one throws std::logic_error.
+    virtual std::array<scalar, 9> virial() const;
+    virtual std::vector<unsigned long> particlesPerType() const;
+    virtual std::vector<uint64_t> histogramAlongAxis(const std::vector<scalar> &borders,
+                                                     const std::vector<ParticleTypeId> &types, unsigned axis) const;
+    virtual std::vector<uint64_t> rdfCounts(const std::vector<scalar> &borders, const std::vector<ParticleTypeId> &from,
+                                            const std::vector<ParticleTypeId> &to, uint64_t *nFrom) const;
+    // changes the species of the particle with this id (TopologyReactionActionFactory::createChangeParticleType
+    // acts through it); a kernel that cannot throws std::logic_error
+    virtual void changeParticleType(ParticleId id, ParticleTypeId type);
 };
+
+class Kernel;
+
+// ---- topology actions (TopologyActionFactory.h; TopologyReactionActionFactory.h:55-101) --------
+// The bonded potentials of topologies are evaluated inside the kernel's force action (the device sums
+// every topology's bonds, angles and dihedrals with the pair forces), so the potential half of the
+// reference factory has no separate actions here; the reaction half edits a topology: its graph, its
+// type and its particles' species.  The graph edits take effect with the next configureTopologies().
+class TopologyReactionAction : public Action {
+public:
+    explicit TopologyReactionAction(GraphTopology *topology) : _topology(topology) {
+        if (!topology) throw std::invalid_argument("topology reaction action without a topology");
+    }
+    virtual void undo() = 0;  // TopologyReactionAction::undo (reverts the change when a reaction is rolled back)
+
+protected:
+    GraphTopology *_topology;
+};
+
+class TopologyActionFactory {
+public:
+    explicit TopologyActionFactory(StateModel *state) : _state(state) {}
+    using ActionPtr = std::unique_ptr<TopologyReactionAction>;
+    // vertex indices are positions in GraphTopology::particleIndices()
+    ActionPtr createChangeParticleType(GraphTopology *topology, std::size_t vertex, ParticleTypeId typeTo) const;
+    ActionPtr createAddEdge(GraphTopology *topology, std::size_t v1, std::size_t v2) const;
+    ActionPtr createRemoveEdge(GraphTopology *topology, std::size_t v1, std::size_t v2) const;
+    ActionPtr createChangeTopologyType(GraphTopology *topology, const std::string &typeTo) const;
+
+private:
+    StateModel *_state;
+};
+
+// ---- observables (ObservableFactory.h:75-178, Observable.h:85-205) -----------------------------
+namespace observables {
+
+using Stride = std::size_t;
+using TimeStep = std::size_t;
+using Matrix33 = std::array<scalar, 9>;  // row-major
+
+// An observable is evaluated at time step t when the stride divides t (stride 0: every step) and it
+// was not evaluated at t already (Observable.h:144-147); after each evaluation its callback receives
+// the result.
+class ObservableBase {
+public:
+    ObservableBase(Kernel *kernel, Stride stride) : kernel(kernel), _stride(stride) {}
+    virtual ~ObservableBase() = default;
+    ObservableBase(const ObservableBase &) = delete;
+    ObservableBase &operator=(const ObservableBase &) = delete;
+    Stride stride() const { return _stride; }
+    TimeStep currentTimeStep() const { return t_current; }
+    bool shouldEvaluate(TimeStep t) const { return (t_current != t || firstCall) && (_stride == 0 || t % _stride == 0); }
+    void call(TimeStep t) {
+        if (!shouldEvaluate(t)) return;
+        firstCall = false;
+        t_current = t;
+        evaluate();
+        fire();
+    }
+    virtual void evaluate() = 0;
+    virtual std::string type() const = 0;
+
+protected:
+    virtual void fire() = 0;
+    Kernel *kernel;
+    Stride _stride;
+    TimeStep t_current = 0;
+    bool firstCall = true;
+};
+
+template <class Result>
+class Observable : public ObservableBase {
+public:
+    using result_type = Result;
+    using callback_type = std::function<void(const Result &)>;
+    Observable(Kernel *kernel, Stride stride) : ObservableBase(kernel, stride) {}
+    const Result &getResult() const { return result; }
+    callback_type &callback() { return _callback; }
+
+protected:
+    void fire() override {
+        if (_callback) _callback(result);
+    }
+    Result result{};
+    callback_type _callback;
+};
+
+// The observables a kernel provides; each evaluate() reads the device state through the C ABI.
+struct Energy : Observable<scalar> {
+    using Observable::Observable;
+    void evaluate() override;
+    std::string type() const override { return "Energy"; }
+};
+struct Virial : Observable<Matrix33> {  // needs Context::recordVirial, which the factory switches on
+    using Observable::Observable;
+    void evaluate() override;
+    std::string type() const override { return "Virial"; }
+};
+struct NParticles : Observable<std::vector<unsigned long>> {  // total (no types) or per listed type
+    NParticles(Kernel *k, Stride s, std::vector<ParticleTypeId> types) : Observable(k, s), typesToCount(std::move(types)) {}
+    void evaluate() override;
+    std::string type() const override { return "NParticles"; }
+    std::vector<ParticleTypeId> typesToCount;
+};
+struct Forces : Observable<std::vector<Vec3>> {  // of the listed types (all when none), particle order
+    Forces(Kernel *k, Stride s, std::vector<ParticleTypeId> types) : Observable(k, s), typesToCount(std::move(types)) {}
+    void evaluate() override;
+    std::string type() const override { return "Forces"; }
+    std::vector<ParticleTypeId> typesToCount;
+};
+struct Positions : Observable<std::vector<Vec3>> {
+    Positions(Kernel *k, Stride s, std::vector<ParticleTypeId> types) : Observable(k, s), typesToCount(std::move(types)) {}
+    void evaluate() override;
+    std::string type() const override { return "Positions"; }
+    std::vector<ParticleTypeId> typesToCount;
+};
+struct HistogramAlongAxis : Observable<std::vector<scalar>> {
+    HistogramAlongAxis(Kernel *k, Stride s, std::vector<scalar> borders, std::vector<ParticleTypeId> types,
+                       unsigned axis)
+        : Observable(k, s), binBorders(std::move(borders)), typesToCount(std::move(types)), axis(axis) {
+        result.assign(binBorders.size() > 1 ? binBorders.size() - 1 : 0, 0.0);
+    }
+    void evaluate() override;
+    std::string type() const override { return "HistogramAlongAxis"; }
+    std::vector<scalar> binBorders;
+    std::vector<ParticleTypeId> typesToCount;
+    unsigned axis;
+};
+// (bin centers, distribution) as RadialDistribution.cpp:105-123 normalises the pair counts
+struct RadialDistribution : Observable<std::pair<std::vector<scalar>, std::vector<scalar>>> {
+    RadialDistribution(Kernel *k, Stride s, std::vector<scalar> borders, std::vector<ParticleTypeId> from,
+                       std::vector<ParticleTypeId> to, scalar particleToDensity);
+    void evaluate() override;
+    std::string type() const override { return "RadialDistribution"; }
+    std::vector<scalar> binBorders;
+    std::vector<ParticleTypeId> typeCountFrom, typeCountTo;
+    scalar particleToDensity;
+};
+// (types, ids, positions) of every particle (Particles.cpp:62-92)
+struct Particles : Observable<std::tuple<std::vector<ParticleTypeId>, std::vector<ParticleId>, std::vector<Vec3>>> {
+    using Observable::Observable;
+    void evaluate() override;
+    std::string type() const override { return "Particles"; }
+};
+struct Reactions : Observable<std::vector<ReactionRecord>> {  // needs recordReactionsWithPositions
+    using Observable::Observable;
+    void evaluate() override;
+    std::string type() const override { return "Reactions"; }
+};
+struct ReactionCounts : Observable<std::map<std::size_t, std::size_t>> {  // reaction index -> count, last step
+    using Observable::Observable;
+    void evaluate() override;
+    std::string type() const override { return "ReactionCounts"; }
+};
+
+class ObservableFactory {
+public:
+    explicit ObservableFactory(Kernel *kernel) : kernel(kernel) {}
+    template <class T>
+    using ObsCallback = std::function<void(const typename T::result_type &)>;
+    std::unique_ptr<Energy> energy(Stride stride, ObsCallback<Energy> callback = {}) const;
+    std::unique_ptr<Virial> virial(Stride stride, ObsCallback<Virial> callback = {}) const;
+    std::unique_ptr<HistogramAlongAxis> histogramAlongAxis(Stride stride, std::vector<scalar> binBorders,
+                                                           std::vector<std::string> typesToCount, unsigned axis,
+                                                           ObsCallback<HistogramAlongAxis> callback = {}) const;
+    std::unique_ptr<NParticles> nParticles(Stride stride, std::vector<std::string> typesToCount = {},
+                                           ObsCallback<NParticles> callback = {}) const;
+    std::unique_ptr<Forces> forces(Stride stride, std::vector<std::string> typesToCount = {},
+                                   ObsCallback<Forces> callback = {}) const;
+    std::unique_ptr<Positions> positions(Stride stride, std::vector<std::string> typesToCount = {},
+                                         ObsCallback<Positions> callback = {}) const;
+    std::unique_ptr<RadialDistribution> radialDistribution(Stride stride, std::vector<scalar> binBorders,
+                                                           std::vector<std::string> typeCountFrom,
+                                                           std::vector<std::string> typeCountTo,
+                                                           scalar particleToDensity,
+                                                           ObsCallback<RadialDistribution> callback = {}) const;
+    std::unique_ptr<Particles> particles(Stride stride, ObsCallback<Particles> callback = {}) const;
+    std::unique_ptr<Reactions> reactions(Stride stride, ObsCallback<Reactions> callback = {}) const;
+    std::unique_ptr<ReactionCounts> reactionCounts(Stride stride, ObsCallback<ReactionCounts> callback = {}) const;
+
+private:
+    std::vector<ParticleTypeId> ids(const std::vector<std::string> &names) const;
+    Kernel *kernel;
+};
+
+}  // namespace observables
 
 class Kernel {  // Kernel.h:71-235
 public:
@@ -347,10 +581,22 @@ public:
         (void)scheduler, (void)dt, (void)steps;
         return false;
     }
+    // Kernel::observe (Kernel.h:193-199): the factory of this kernel's observables
+    observables::ObservableFactory &observe() { return _observe; }
+    // Kernel::connectObservable / evaluateObservables: the loop evaluates the connected observables at
+    // the start and after every step (each decides by its stride)
+    void connectObservable(observables::ObservableBase *o);
+    void disconnectObservable(observables::ObservableBase *o);
+    void evaluateObservables(observables::TimeStep t);
+    const std::vector<observables::ObservableBase *> &connectedObservables() const { return _observables; }
+    // Kernel::getTopologyActionFactory (Kernel.h:163-165): nullptr when the kernel has no topologies
+    virtual TopologyActionFactory *getTopologyActionFactory() { return nullptr; }
 
 protected:
     std::string _name;
     Context _context;
+    observables::ObservableFactory _observe{this};
+    std::vector<observables::ObservableBase *> _observables;
 };
 
 }  // namespace model
@@ -388,6 +634,8 @@ private:
 
 namespace api {
 
+namespace observables = model::observables;
+
 // SimulationLoop.h:256-306: integrate, neighbour list, reactions, neighbour list, forces per step
 class SimulationLoop {
 public:
@@ -395,17 +643,20 @@ public:
     void useReactionScheduler(const std::string &name);  // "UncontrolledApproximation" | "Gillespie"
     void evaluateForces(bool v) { _forces = v; }
     void evaluateReactions(bool v) { _reactions = v; }
+    void evaluateTopologyReactions(bool v) { _topologyReactions = v; }  // SimulationLoop.h:185-190
     // called after every step with the step index (observables); none registered: the loop hands the
     // whole run to the device-fused path (rdy_run), which performs the same actions in the same order
     void addCallback(std::function<void(std::size_t)> cb) { _callbacks.push_back(std::move(cb)); }
     void run(std::size_t steps);
     scalar timeStep() const { return _dt; }
+    std::size_t currentStep() const { return _t; }  // time steps taken by this loop so far
 
 private:
+    std::size_t _t = 0;
     model::Kernel *_kernel;
     scalar _dt;
     std::string _scheduler = "Gillespie";  // the reference default when supported (SimulationLoop.h:104-109)
-    bool _forces = true, _reactions = true;
+    bool _forces = true, _reactions = true, _topologyReactions = true;
     std::vector<std::function<void(std::size_t)>> _callbacks;
 };
 

@@ -1,4 +1,11 @@
-"""The bead-chain + crowder system with enzymatic binding, at any size.
+"""Synthetic inputs of the benchmark and test configurations.
+
+c1_system / c1_chemistry: the single large A+B<->C system (periodic 200^3 box, 1,000,000 particles,
+40 / 40 / 20 % A / B / C uniform random, seed 1; harmonic repulsion k=10 d=1 on all pairs, A-A weak
+interaction k=10 d0=1 depth 0.5 cutoff 2, fusion A+B->C rate 0.1 radius 1, fission C->A+B rate 0.05
+distance 1).
+
+system / potentials / reactions: the bead-chain + crowder system with enzymatic binding, at any size.
 
 Chains are 3-D random walks of unit steps (a step closer than 0.8 to the bead two or three back is
 drawn again, otherwise Lennard-Jones would blow up); crowders sit on random, jittered sites of a
@@ -80,6 +87,21 @@ def system(n_chains, beads, n_crowders, box, seed):
     angles = np.stack([idx[:, :-2], idx[:, 1:-1], idx[:, 2:]], axis=2).reshape(-1, 3)
     dihedrals = np.stack([idx[:, :-3], idx[:, 1:-2], idx[:, 2:-1], idx[:, 3:]], axis=2).reshape(-1, 4)
     return pos, types, bonds, angles, dihedrals
+
+
+def c1_system(n=1_000_000, L=200.0, seed=1):
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((n, 3)) - 0.5) * L
+    types = rng.choice(np.array([0, 1, 2], dtype=np.uint32), n, p=[0.4, 0.4, 0.2])
+    return pos, types
+
+
+def c1_chemistry(L):
+    pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]
+    pots.append((L.POT_WEAK_INTERACTION, 0, 0, [10.0, 1.0, 0.5, 2.0]))
+    rxs = [dict(kind=L.RX_FUSION, educts=[0, 1], products=[2], rate=0.1, educt_distance=1.0),
+           dict(kind=L.RX_FISSION, educts=[2], products=[0, 1], rate=0.05, product_distance=1.0)]
+    return pots, rxs
 
 
 def potentials(L):

@@ -546,10 +546,276 @@ static void test_descriptors() {
     std::printf("reaction descriptors: passed\n");
 }
 
+// spatial topology reaction descriptors on the reference's cases (readdy/test/TestContext.cpp:100-160)
+// and the validation of TopologyRegistry::validateSpatialReaction (TopologyRegistry.cpp:238-293)
+static void test_spatial_descriptors() {
+    using M = model::STRMode;
+    model::Context c;
+    for (const char *t : {"A", "B", "C"}) c.particleTypes().addTopologyType(t, 1.);
+    c.particleTypes().add("P", 1.);
+    c.particleTypes().add("Q", 1.);
+    auto &reg = c.topologyRegistry();
+    CHECK(reg.addType("T") == 0 && reg.addType("TT") == 1 && reg.nameOf(1) == "TT", "topology types in order");
+    reg.addSpatialReaction("MyFusionReaction: T(A)+T(B) -> TT(B--A)", 10, 20);
+    reg.addSpatialReaction("MySelfFusionReaction: T(A)+T(B) -> TT(B--A) [self=true]", 10, 20);
+    reg.addSpatialReaction("MyTPFusion: T(A)+(P) -> TT(B--B)", 10, 20);
+    reg.addSpatialReaction("MyEnzymaticTT: T(A)+T(B) -> TT(B) + TT(A)", 10, 20);
+    reg.addSpatialReaction("MyEnzymaticTP: T(A) + (P) -> TT(B) + (Q)", 10, 20);
+    const auto &sr = reg.spatialReactions();
+    CHECK(sr.size() == 5, "five spatial reactions");
+    const auto id = [&](const char *n) { return c.particleTypes().idOf(n); };
+    CHECK(reg.isSpatialReactionType(id("A")) && !reg.isSpatialReactionType(id("C")), "spatial reaction types");
+    const auto by = [&](const std::string &n) -> const model::SpatialReactionDesc & {
+        for (const auto &r : sr)
+            if (r.name == n) return r;
+        throw std::invalid_argument("no reaction " + n);
+    };
+    const auto &tp = by("MyEnzymaticTP");
+    CHECK(tp.mode == M::TP_ENZYMATIC && tp.type1 == id("A") && tp.type2 == id("P") && tp.topType1 == 0 &&
+              tp.topTypeTo1 == 1 && tp.typeTo1 == id("B") && tp.typeTo2 == id("Q") && tp.rate == 10 &&
+              tp.radius == 20 && tp.isTopologyParticle(),
+          "TP_ENZYMATIC descriptor");
+    const auto &tt = by("MyEnzymaticTT");
+    CHECK(tt.mode == M::TT_ENZYMATIC && tt.type1 == id("A") && tt.type2 == id("B") && tt.topType1 == 0 &&
+              tt.topType2 == 0 && tt.topTypeTo1 == 1 && tt.topTypeTo2 == 1 && tt.typeTo1 == id("B") &&
+              tt.typeTo2 == id("A") && !tt.isTopologyParticle(),
+          "TT_ENZYMATIC descriptor");
+    const auto &tf = by("MyFusionReaction");
+    CHECK(tf.mode == M::TT_FUSION && tf.typeTo1 == id("B") && tf.typeTo2 == id("A") && tf.topTypeTo1 == 1,
+          "TT_FUSION descriptor");
+    CHECK(by("MySelfFusionReaction").mode == M::TT_FUSION_ALLOW_SELF, "TT_FUSION_ALLOW_SELF descriptor");
+    const auto &pf = by("MyTPFusion");
+    CHECK(pf.mode == M::TP_FUSION && pf.typeTo1 == id("B") && pf.typeTo2 == id("B") && pf.topTypeTo1 == 1,
+          "TP_FUSION descriptor");
+    for (const char *bad : {"noarrow: T(A) + (P)", "two: T(A) + (P) -> T(A) + (P) -> T(A) + (P)",
+                            "nocolon T(A) + (P) -> T(B) + (Q)", "noplus: T(A) -> T(B) + (Q)",
+                            "nobond: T(A) + T(B) -> TT(A)", "unknowntop: X(A) + (P) -> T(B) + (Q)",
+                            "unknowntype: T(Z) + (P) -> T(B) + (Q)", "MyEnzymaticTP: T(A) + (P) -> TT(B) + (Q)",
+                            "flavor: T(A) + (P) -> T(P) + (Q)", "notopology: T(P) + (Q) -> T(Q) + (P)",
+                            "fusionflavor: T(A) + (P) -> TT(B--P)"}) {
+        bool threw = false;
+        try {
+            reg.addSpatialReaction(bad, 1., 1.);
+        } catch (const std::invalid_argument &) {
+            threw = true;
+        }
+        CHECK(threw, "spatial descriptor \"%s\" must throw std::invalid_argument", bad);
+    }
+    bool threw = false;
+    try {
+        reg.addSpatialReaction("radius: T(A) + (P) -> T(B) + (Q)", 1., 0.);
+    } catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw, "a non-positive radius throws");
+    threw = false;
+    try {
+        reg.addSpatialReaction("opt: T(A) + T(B) -> TT(A--B) [distance=3]", 1., 1.);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw, "an option block without self=true throws");
+    CHECK(c.calculateMaxCutoff() == 20, "Context::calculateMaxCutoff includes the spatial radii");
+    std::printf("spatial topology reaction descriptors: passed\n");
+}
+
+// A chain scene for the topology-particle enzymatic reaction through the plugin: 10 chains of 4 beads
+// (type "Chain", head first), an X next to the head of chains 0-4 and 5 more X far from every chain.
+// bind: Chain(head) + (X) -> Bound(head2) + (Y); each reaction turns one Chain into Bound and one X into Y,
+// and a Bound topology matches no reaction, so #Bound == #Y at any time.
+static plugin::KernelProvider::kernel_ptr tr_kernel(bool retype) {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {40., 40., 40.};
+    c.particleTypes().addTopologyType("head", 0.1);
+    c.particleTypes().addTopologyType("bead", 0.1);
+    c.particleTypes().addTopologyType("head2", 0.1);
+    c.particleTypes().add("X", 1.);
+    c.particleTypes().add("Y", 1.);
+    auto &reg = c.topologyRegistry();
+    reg.addType("Chain");
+    reg.addType("Bound");
+    reg.configureBondPotential("head", "bead", {10., 1.});
+    reg.configureBondPotential("bead", "bead", {10., 1.});
+    reg.configureBondPotential("head2", "bead", {20., 1.5});
+    c.potentials().addHarmonicRepulsion("X", "bead", 10., 1.);
+    reg.addSpatialReaction(retype ? "bind: Chain(head) + (X) -> Bound(head2) + (Y)"
+                                  : "bind: Chain(head) + (X) -> Bound(head) + (Y)",
+                           1e10, 1.5);
+    auto &sm = k->stateModel();
+    const auto head = c.particleTypes().idOf("head"), bead = c.particleTypes().idOf("bead");
+    std::vector<model::Particle> xs;
+    for (int t = 0; t < 10; ++t) {
+        const double y = -15. + 3. * t;
+        std::vector<model::Particle> ps;
+        for (int i = 0; i < 4; ++i) ps.emplace_back(-5. + i, y, 0., i ? bead : head);
+        auto *top = sm.addTopology("Chain", ps);
+        for (int i = 0; i < 3; ++i) top->addEdge(i, i + 1);
+        xs.emplace_back(t < 5 ? -5.8 : 12., t < 5 ? y : -15. + 6. * (t - 5), t < 5 ? 0.3 : 10., c.particleTypes().idOf("X"));
+    }
+    sm.addParticles(xs);
+    return k;
+}
+
+static void tr_counts(model::Kernel &k, int &bound, int &y, int &x, int &h2) {
+    bound = y = x = h2 = 0;
+    for (auto *t : k.stateModel().getTopologies()) bound += t->type() == "Bound";
+    const auto &pt = k.context().particleTypes();
+    for (const auto &p : k.stateModel().getParticles()) {
+        y += p.type() == pt.idOf("Y");
+        x += p.type() == pt.idOf("X");
+        h2 += p.type() == pt.idOf("head2");
+    }
+}
+
+static void test_topology_reactions() {
+    for (int retype = 0; retype < 2; ++retype) {
+        auto k = tr_kernel(retype != 0);
+        k->actions().calculateForces()->perform();
+        k->actions().evaluateTopologyReactions(0.01)->perform();
+        int bound, y, x, h2;
+        tr_counts(*k, bound, y, x, h2);
+        // every X next to a head is within the radius, the rate makes the reaction sure, and no other pair
+        // is within reach: chains 0-4 react exactly once each
+        CHECK(bound == 5 && y == 5 && x == 5 && h2 == (retype ? 5 : 0),
+              "one evaluation: Bound %d, Y %d, X %d, head2 %d (retype %d)", bound, y, x, h2, retype);
+        const auto tops = k->stateModel().getTopologies();
+        for (int t = 0; t < 10; ++t)
+            CHECK(tops[t]->type() == (t < 5 ? "Bound" : "Chain"), "topology %d type %s", t, tops[t]->type().c_str());
+        if (retype) {
+            // the retyped heads carry the head2 bond (k 20, length 1.5): stretched by 0.5 it holds 20 * 0.25
+            const double e = k->stateModel().energy();
+            k->actions().calculateForces()->perform();
+            const double e2 = k->stateModel().energy();
+            CHECK(std::fabs(e2 - 5 * 20. * 0.25) < 1e-6 * 5 * 20 * 0.25 + 1e-9 && e2 > e,
+                  "bonded terms set again after the retyping: energy %.9g (before %.9g)", e2, e);
+        }
+        // a loop of 20 steps: fused on the device when the bonded terms stay, step by step otherwise
+        api::SimulationLoop loop(k.get(), 0.01);
+        loop.run(20);
+        tr_counts(*k, bound, y, x, h2);
+        CHECK(bound == y && x + y == 10 && bound >= 5 && h2 == (retype ? bound : 0),
+              "after 20 steps: Bound %d, Y %d, X %d, head2 %d (retype %d)", bound, y, x, h2, retype);
+        std::printf("topology reactions (retype %d): Bound %d, Y %d after 20 steps\n", retype, bound, y);
+    }
+}
+
+// Kernel::observe(): observables connected to the kernel and evaluated by the SimulationLoop on the
+// device-fused path in chunks that end where one is due (strides 0, 2, 5 and 10 over 10 steps), checked
+// against the oracle on the state they saw: energy, virial, particles per type, the radial distribution
+// and the histogram along an axis; conservation of A + C and B + C at every step.
+static void test_observables() {
+    const double L = 12.0;
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    c0_context(k->context(), L);
+    const System s = uniform(400, 400, L, 9);
+    std::vector<model::Particle> ps;
+    for (size_t i = 0; i < s.types.size(); ++i)
+        ps.emplace_back(s.pos[3 * i], s.pos[3 * i + 1], s.pos[3 * i + 2], (ParticleTypeId)s.types[i]);
+    k->actions().addParticles(ps)->perform();
+    auto &f = k->observe();
+    std::vector<std::size_t> e_steps, n_steps;
+    std::vector<double> energies;
+    std::vector<std::vector<unsigned long>> counts;
+    model::observables::Matrix33 last_virial{};
+    std::vector<double> last_rdf;
+    std::vector<double> last_hist;
+    auto en = f.energy(2, [&](const double &e) { energies.push_back(e); });
+    auto np = f.nParticles(0, {"A", "B", "C"}, [&](const std::vector<unsigned long> &c) { counts.push_back(c); });
+    auto vir = f.virial(5, [&](const model::observables::Matrix33 &v) { last_virial = v; });
+    std::vector<double> borders;
+    for (int b = 0; b <= 20; ++b) borders.push_back(0.1 * b);
+    auto rdf = f.radialDistribution(10, borders, {"A"}, {"B", "C"}, 800.0 / (L * L * L),
+                                    [&](const std::pair<std::vector<double>, std::vector<double>> &r) { last_rdf = r.second; });
+    std::vector<double> hb;
+    for (int b = 0; b <= 24; ++b) hb.push_back(-6.0 + 0.5 * b);
+    auto hist = f.histogramAlongAxis(5, hb, {"A", "C"}, 1, [&](const std::vector<double> &h) { last_hist = h; });
+    CHECK(k->context().recordVirial(), "the virial observable switches recordVirial on");
+    for (model::observables::ObservableBase *o :
+         std::initializer_list<model::observables::ObservableBase *>{en.get(), np.get(), vir.get(), rdf.get(), hist.get()})
+        k->connectObservable(o);
+    api::SimulationLoop loop(k.get(), 0.01);
+    loop.useReactionScheduler("UncontrolledApproximation");
+    loop.run(10);
+    CHECK(loop.currentStep() == 10, "loop step %zu", loop.currentStep());
+    CHECK(energies.size() == 6, "energy evaluated %zu times (t = 0, 2, ..., 10)", energies.size());
+    CHECK(counts.size() == 11, "particle counts evaluated %zu times (every step)", counts.size());
+    for (const auto &c : counts) {
+        CHECK(c.size() == 3 && c[0] + c[2] == 400 && c[1] + c[2] == 400, "A + C and B + C conserved (%lu %lu %lu)",
+              c[0], c[1], c[2]);
+    }
+    CHECK(en->currentTimeStep() == 10 && rdf->currentTimeStep() == 10, "last evaluations at t = 10");
+    // the oracle on the state the t = 10 evaluations saw
+    const auto parts = k->stateModel().getParticles();
+    std::vector<double> pos;
+    std::vector<int> types;
+    std::vector<uint64_t> ids;
+    for (const auto &p : parts) {
+        pos.insert(pos.end(), {p.pos().x, p.pos().y, p.pos().z});
+        types.push_back(p.type());
+        ids.push_back(p.id());
+    }
+    const double box[3] = {L, L, L};
+    const int pbc[3] = {1, 1, 1};
+    const double D[3] = {1.0, 1.0, 0.5};
+    or_system *o = or_create(3, box, pbc, 1.0, D);
+    const double hr[2] = {10.0, 1.0};
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) or_add_pair_potential(o, OR_POT_HARMONIC_REPULSION, a, b, hr);
+    or_add_particles(o, (int)types.size(), pos.data(), types.data(), ids.data());
+    or_nl_setup(o, 1.0, 1);
+    double e_ref = 0, w_ref[9];
+    or_calculate_forces(o, &e_ref, w_ref);
+    CHECK(std::fabs(energies.back() - e_ref) <= 1e-5 * std::fabs(e_ref), "energy %.10g vs %.10g", energies.back(), e_ref);
+    double dn = 0, dd = 0;
+    for (int c = 0; c < 9; ++c) {
+        dn += (last_virial[c] - w_ref[c]) * (last_virial[c] - w_ref[c]);
+        dd += w_ref[c] * w_ref[c];
+    }
+    CHECK(dd > 0 && std::sqrt(dn / dd) < 1e-5, "virial relative error %.3g", std::sqrt(dn / std::max(dd, 1e-300)));
+    const int from[1] = {0}, to[2] = {1, 2};
+    std::vector<uint64_t> rc(borders.size() - 1);
+    uint64_t nfrom = 0;
+    or_rdf_counts(o, borders.data(), (int)borders.size(), from, 1, to, 2, rc.data(), &nfrom);
+    CHECK(last_rdf.size() == rc.size(), "rdf bins");
+    for (size_t b = 0; b < rc.size() && b < last_rdf.size(); ++b) {
+        const double lo = borders[b], hi = borders[b + 1];
+        const double want = (double)rc[b] / (4. / 3. * M_PI * (std::pow(hi, 3.) - std::pow(lo, 3.)) * (double)nfrom *
+                                             (800.0 / (L * L * L)));
+        CHECK(last_rdf[b] == want, "rdf bin %zu: %.10g vs %.10g", b, last_rdf[b], want);
+    }
+    const int htypes[2] = {0, 2};
+    std::vector<uint64_t> hc(hb.size() - 1);
+    or_histogram_along_axis(o, hb.data(), (int)hb.size(), htypes, 2, 1, hc.data());
+    for (size_t b = 0; b < hc.size() && b < last_hist.size(); ++b)
+        CHECK(last_hist[b] == (double)hc[b], "histogram bin %zu: %g vs %llu", b, last_hist[b], (unsigned long long)hc[b]);
+    or_destroy(o);
+    // the topology action factory: edits of a topology's graph and type, undone in reverse
+    CHECK(k->getTopologyActionFactory() != nullptr, "the kernel supports topologies");
+    model::GraphTopology g("T", {0, 1, 2});
+    auto *taf = k->getTopologyActionFactory();
+    auto a1 = taf->createAddEdge(&g, 0, 1);
+    auto a2 = taf->createChangeTopologyType(&g, "U");
+    a1->perform();
+    a2->perform();
+    CHECK(g.hasEdge(0, 1) && g.type() == "U", "edge added and type changed");
+    a2->undo();
+    a1->undo();
+    CHECK(!g.hasEdge(0, 1) && g.type() == "T", "both undone");
+    bool threw = false;
+    try {
+        taf->createRemoveEdge(&g, 0, 2)->perform();
+    } catch (const std::invalid_argument &) {
+        threw = true;
+    }
+    CHECK(threw, "removing a missing edge throws");
+}
+
 int main(int argc, char **argv) {
     if (argc > 1 && std::string(argv[1]) == "--host-only") {
         test_graph_ops();
         test_descriptors();
+        test_spatial_descriptors();
         std::printf(failures ? "FAILED (%d)\n" : "ALL PASSED\n", failures);
         return failures ? 1 : 0;
     }
@@ -562,8 +828,10 @@ int main(int argc, char **argv) {
         test_provider_and_errors(argv[1]);
         if (!provider_only) {
             test_c0_forces_and_loop(argv[1]);
+            test_observables();
             test_topology();
             test_break_bonds();
+            test_topology_reactions();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

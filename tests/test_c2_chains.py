@@ -6,17 +6,22 @@ positions within the FP32-pair-force bound), and the fused run path reproduces t
 bit for bit (the fix-up after bindings touches beads, whose moves carry bonded forces).  Full size
 (10^4 chains of 50 beads, 5 10^5 crowders, box 160): the forces of the initial state against the
 oracle, then invariants of a run (species counts conserved where the chemistry conserves them,
-positions in the box, energy and forces finite).  Bond lengths are no invariant of this system: where
-three beads of a chain pass through a line, the cosine-dihedral force grows as 1/sin(theta) up to the
-reference's clamps (TorsionPotential.cpp:53-159), and the FP64 oracle itself flings such chains apart
-at dt = 1e-3 (bead forces up to 1e12 within 20 steps on 40 chains of 50 beads).
+positions in the box, energy and forces finite).
+
+The configuration's own time step, dt = 1e-3, is numerically unstable for these chains: where three
+beads pass through a line the cosine-dihedral force grows as 1/sin(theta) up to the reference's clamps
+(TorsionPotential.cpp:53-159), a bead jumps by F D dt, bonds and crowder overlaps follow, and the FP64
+oracle itself reaches forces of 1e18 within two steps at full size (1e7 at 5e-4; stable at 2e-4 for 20
+steps).  The device's force sums are 64-bit fixed point, so it stops such a run with RDY_ERR_NUMERIC
+(a force beyond 1.68e7) instead of carrying inf or wrapped sums on; the runs below use dt = 2e-4, and
+one test shows that at dt = 1e-3 the device reports the blow-up that the oracle shows from the same state.
 """
 import numpy as np
 import pytest
 
 from oracle import pyoracle as po
 
-import c2_system as C2
+from readdy_amd import workloads as C2
 
 pytestmark = pytest.mark.gpu
 
@@ -96,7 +101,7 @@ def test_reduced_chains_match_oracle_event_for_event():
 
 
 def test_reduced_chains_fused_run_equals_separate_actions():
-    box, dt, rate, steps = 12.0, 0.001, 50.0, 20
+    box, dt, rate, steps = 12.0, 2e-4, 250.0, 20
     sysm = C2.system(12, 10, 300, box, seed=3)
     e1, r1 = _gpu(*sysm, box, rate)
     e1.run(1, dt, steps)
@@ -116,7 +121,7 @@ def test_reduced_chains_fused_run_equals_separate_actions():
 
 
 def test_full_size_forces_and_invariants():
-    box, dt = 160.0, 0.001
+    box, dt = 160.0, 2e-4
     print("building the 10^6-particle chain system", flush=True)
     pos, types, bonds, angles, dihedrals = C2.system(10000, 50, 500000, box, seed=2)
     n_beads = int((types == 0).sum())
@@ -139,3 +144,23 @@ def test_full_size_forces_and_invariants():
     assert np.isfinite(e.energy(r))
     f, _ = e.get_forces(r)
     assert np.all(np.isfinite(f))
+
+
+def test_full_size_at_the_configured_time_step_reports_the_blow_up():
+    """dt = 1e-3: the FP64 oracle's forces pass 1.68e7 within five steps from the initial state, and the
+    device's run of those steps stops with RDY_ERR_NUMERIC (every kernel ends normally)."""
+    box, dt, steps = 160.0, 1e-3, 5
+    pos, types, bonds, angles, dihedrals = C2.system(10000, 50, 500000, box, seed=2)
+    o = _oracle(pos, types, bonds, angles, dihedrals, box, 0.01, 2)
+    o.set_threads(max(1, min(16, len(__import__("os").sched_getaffinity(0)))))
+    o.calculate_forces()
+    peak = 0.0
+    for t in range(steps):
+        o.loop_step(dt, po.GILLESPIE_KEYED, t)
+        peak = max(peak, float(np.abs(o.particles()[3]).max()))
+    print(f"oracle: largest force within {steps} steps {peak:.2e}", flush=True)
+    assert peak > 1.68e7
+    del o
+    e, r = _gpu(pos, types, bonds, angles, dihedrals, box, 0.01, seed=2)
+    with pytest.raises(L.EngineError, match="error -5"):
+        e.run(1, dt, steps, graph=True)

@@ -336,7 +336,10 @@ def test_force_beyond_fixed_point_range_is_a_numeric_error():
 
 
 def test_philox_noise_matches_oracle_stream():
-    """Philox-driven BD: identical integer draws; positions agree to transcendental rounding."""
+    """Philox-driven BD: identical integer draws, the same FP32 Box-Muller transform on both sides.  The
+    device's FP32 logarithm, sine and cosine and the oracle's correctly rounded ones may differ by an FP32
+    unit, so each draw agrees to a few 2^-24 relative: at most 5 steps x sqrt(2 D dt) = 0.14 x 5 x 3e-7
+    here, 2e-6 with margin; a different transform or counter would differ by O(0.1)."""
     n, Lb = 500, 10.0
     rng = np.random.default_rng(5)
     pos = (rng.random((n, 3)) - 0.5) * Lb
@@ -354,7 +357,8 @@ def test_philox_noise_matches_oracle_stream():
     op, _, ids, _ = o.particles()
     d = p - op[np.argsort(ids)]
     d -= Lb * np.round(d / Lb)
-    assert np.abs(d).max() < 1e-12
+    print(f"Philox BD stream: max |dx| = {np.abs(d).max():.2e}", flush=True)
+    assert np.abs(d).max() < 2e-6
 
 
 def _c0_chemistry():

@@ -605,3 +605,103 @@ def test_histogram_along_axis_equals_numpy():
             want = np.bincount(u - 1, minlength=len(borders) - 1).astype(np.uint64)
             assert np.array_equal(got, want)
             assert got[-1] == 0
+
+
+def test_cas_filled_cell_list_has_the_serial_pair_set():
+    """The timing baseline's cell list, filled from all threads by atomic head exchange as
+    CellLinkedList.cpp:171-228 fills it, holds the same pairs as the serial fill (only the order within
+    a cell may differ), and its forces agree to rounding."""
+    rng = np.random.default_rng(21)
+    n, box = 20000, [30.0, 30.0, 30.0]
+    pos = (rng.random((n, 3)) - 0.5) * 30.0
+    out = []
+    for cas in (False, True):
+        s = po.System(1, box, [1, 1, 1])
+        s.add_pair_potential(po.POT_HARMONIC_REPULSION, 0, 0, [10.0, 1.5])
+        s.add_particles(pos, np.zeros(n, dtype=np.int32), ids=np.arange(n))
+        s.set_threads(8)
+        s.set_cas_fill(cas)
+        s.nl_setup(1.5)
+        pairs = s.nl_pairs(1.5)
+        e = s.calculate_forces()
+        out.append((set(map(tuple, pairs.tolist())), e, s.particles()[3]))
+    assert out[0][0] == out[1][0] and len(out[0][0]) > 10000
+    assert out[1][1] == pytest.approx(out[0][1], rel=1e-12)
+    np.testing.assert_allclose(out[1][2], out[0][2], rtol=1e-12, atol=1e-12)
+
+
+def _tp_system(pos, types, topology_of, top_types, rate=1e10, radius=1.2, to1=0, box=20.0):
+    """Species 0 bead (topology), 1 X_free, 2 X_bound, 3 bead'; reaction T0(0) + (1) -> T1(to1) + (2)."""
+    s = po.System(4, [box] * 3, [1, 1, 1])
+    s.add_spatial_reaction(0, 0, 1, to1, 1, 2, rate, radius)
+    s.add_particles(np.asarray(pos, float), np.asarray(types, np.int32), ids=np.arange(len(types)))
+    s.set_topologies(topology_of, top_types)
+    s.nl_setup(s.max_cutoff())
+    return s
+
+
+def _types_by_id(s):
+    _, t, ids, _ = s.particles()
+    return t[np.argsort(ids)]
+
+
+def test_topology_particle_reactions_scene():
+    """rate 1e10 (acceptance 1 in FP64): topology 0 (beads 0, 1) has one free particle near both beads, so
+    exactly one of its two candidate events fires (events of one topology depend on each other,
+    CPUEvaluateTopologyReactions.cpp:455-464); topology 1 (bead 2) binds its neighbour; a free particle
+    beyond the radius, and one of a species with no reaction, stay; both topologies change type, so a
+    second evaluation finds nothing (the reaction needs topology type 0)."""
+    pos = [[0, 0, 0], [1, 0, 0], [6, 0, 0], [0.5, 0.5, 0], [6.5, 0, 0], [0, 3, 0], [6, 0.5, 0]]
+    types = [0, 0, 0, 1, 1, 1, 2]
+    s = _tp_system(pos, types, [0, 0, 1, -1, -1, -1, -1], [0, 0])
+    assert s.max_cutoff() == 1.2
+    assert s.topology_reactions(1.0, step=0) == 2
+    assert list(_types_by_id(s)) == [0, 0, 0, 2, 2, 1, 2]
+    assert list(s.topology_types()) == [1, 1] and list(s.topology_reaction_counts()) == [2]
+    assert s.topology_reactions(1.0, step=1) == 0 and list(s.topology_reaction_counts()) == [0]
+
+
+def test_topology_particle_reactions_share_the_free_particle():
+    """One free particle within reach of two topologies reacts once: topology-particle events that share
+    the free particle depend on each other; the topology particle takes type_to1, the free one type_to2."""
+    pos = [[0, 0, 0], [1.5, 0, 0], [0.75, 0, 0]]
+    s = _tp_system(pos, [0, 0, 1], [0, 1, -1], [0, 0], to1=3)
+    assert s.topology_reactions(1.0, step=0) == 1
+    t = _types_by_id(s)
+    assert t[2] == 2 and sorted(t[:2].tolist()) == [0, 3]
+    tt = s.topology_types()
+    assert sorted(tt.tolist()) == [0, 1] and tt[int(np.argmax(t[:2] == 3))] == 1
+
+
+def test_topology_particle_reaction_gate_is_strict():
+    """distSquared < radius^2: a free particle at exactly the radius (1.25, exact in FP64) does not react,
+    one just inside does."""
+    for d, want in [(1.25, 0), (1.25 - 1e-12, 1)]:
+        s = _tp_system([[0, 0, 0], [d, 0, 0]], [0, 1], [0, -1], [0], radius=1.25)
+        assert s.topology_reactions(1.0) == want
+
+
+def test_topology_keyed_selection_matches_literal_in_distribution():
+    """The keyed selection with the topology dependencies (all events of one topology conflict; events
+    sharing a free particle conflict) follows performEvents' law: events as (topology token, free particle
+    token) pairs in the abstract performEvents of the oracle, chi-square against the literal algorithm.
+    Topology 10 has three candidate partners, free particle 1 is shared by topologies 10 and 11."""
+    p1 = [10, 10, 10, 11, 12]
+    p2 = [1, 2, 3, 1, 3]
+    rate = [1.0, 2.5, 0.7, 1.8, 3.0]
+    acc = [0.5, 0.4, 0.8, 0.6, 0.3]
+    trials = 20000
+    counts = {}
+    for mode in (po.GILLESPIE_LITERAL, po.GILLESPIE_KEYED):
+        c = {}
+        for t in range(trials):
+            key = tuple(po.perform_events_abstract(p1, p2, rate, acc, mode, seed=7 + mode, trial=t).tolist())
+            c[key] = c.get(key, 0) + 1
+        counts[mode] = c
+    keys = sorted(set(counts[po.GILLESPIE_LITERAL]) | set(counts[po.GILLESPIE_KEYED]))
+    a = np.array([counts[po.GILLESPIE_LITERAL].get(k, 0) for k in keys], float)
+    b = np.array([counts[po.GILLESPIE_KEYED].get(k, 0) for k in keys], float)
+    mask = (a + b) >= 20
+    chi2 = np.sum((a[mask] - b[mask]) ** 2 / (a[mask] + b[mask]))
+    from scipy.stats import chi2 as chi2dist
+    assert chi2dist.sf(chi2, mask.sum() - 1) > 1e-3, (chi2, keys, a, b)
