@@ -306,7 +306,7 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
     };
     auto put = [&](uint32_t j, uint32_t hc, uint32_t s, const State &st) {
         if (s == 0xFFFFFFFFu) {
-            m.P[j] = make_float4(0.f, 0.f, 0.f, __uint_as_float(TW_DEAD));
+            m.P[j] = make_float4(__uint_as_float(0x7FC00000u), 0.f, 0.f, __uint_as_float(TW_DEAD));
             m.slot[j] = 0xFFFFFFFFu;
             return;
         }
@@ -323,7 +323,10 @@ __device__ uint32_t stage_brick(const Grid &g, const BrickDev &B, const RepGeom 
             }
         }
         m.slot[j] = s;
-        const float lx = (float)((st.x - F.org[0]) + shift[0]);
+        // a dead or outside record is never a neighbour: its NaN position fails every distance test
+        const float nan = __uint_as_float(0x7FC00000u);
+        const bool gone = (st.tw & (TW_DEAD | TW_OUTSIDE)) != 0u;
+        const float lx = gone ? nan : (float)((st.x - F.org[0]) + shift[0]);
         const float ly = (float)((st.y - F.org[1]) + shift[1]);
         const float lz = (float)((st.z - F.org[2]) + shift[2]);
         m.P[j] = make_float4(lx, ly, lz, __uint_as_float(st.tw));
@@ -524,14 +527,17 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Streams candidate pairs through the wave's ring: every active lane walks the stencil of its own
 // staged record (halo cell hc, local position p, reach), taking up to BATCH candidates of its
-// current run per iteration (independent LDS reads), and offers each to keep(q); survivors are
+// current run per iteration (independent LDS reads), and offers each to keep(q, P[q]); survivors are
 // appended as (lane << 16 | q) and evaluated by eval(entry) 64 at a time with all lanes of the
-// wave.  The last, partial batch too runs eval on every lane, idle lanes with NO_ENTRY (so that eval
-// may read other lanes' registers).  Must be called by all 64 lanes of a wave (wave-uniform control
-// flow around it).
+// wave.  A lane's survivors of one iteration take consecutive ring places after those of the lower
+// lanes (one prefix over the lanes' survivor counts per iteration).  The last, partial batch too runs
+// eval on every lane, idle lanes with NO_ENTRY (so that eval may read other lanes' registers).  Must
+// be called by all 64 lanes of a wave (wave-uniform control flow around it).
 template <bool ALL_HALO, class Keep, class Eval>
-__device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, uint32_t *ring, bool active,
-                                              uint32_t hc, const float4 &p, float reach, Keep &&keep, Eval &&eval) {
+__device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *hs, const float4 *P, uint32_t *ring,
+                                              bool active, uint32_t hc, const float4 &p, float reach, Keep &&keep,
+                                              Eval &&eval) {
+    static_assert(BATCH <= 7, "survivor counts are ranked as three bit planes");
     const uint32_t lane = __lane_id();
     Walk<ALL_HALO> w;
     w.q = w.qe = 0;
@@ -548,17 +554,22 @@ __device__ __forceinline__ void pair_pipeline(const WalkCtx &c, const uint32_t *
         }
         // candidates q0 .. q0 + nq - 1; reads past the batch repeat its last record (masked out)
         const uint32_t last = nq ? q0 + nq - 1u : q0;
-        bool k[BATCH];
+        float4 pq[BATCH];  // the batch's records first (independent LDS reads), then the tests
 #pragma unroll
-        for (uint32_t i = 0; i < BATCH; ++i) k[i] = keep(min(q0 + i, last)) & (i < nq);
-        uint32_t pos = tail;
+        for (uint32_t i = 0; i < BATCH; ++i) pq[i] = P[min(q0 + i, last)];
+        uint32_t bits = 0;
 #pragma unroll
-        for (uint32_t i = 0; i < BATCH; ++i) {
-            const unsigned long long mask = __ballot(k[i]);
-            if (k[i]) ring[(pos + rank_in_mask(mask)) & (RING - 1)] = (lane << 16) | (q0 + i);
-            pos += (uint32_t)__popcll(mask);
+        for (uint32_t i = 0; i < BATCH; ++i) bits |= (keep(min(q0 + i, last), pq[i]) & (i < nq)) ? (1u << i) : 0u;
+        const uint32_t cnt = (uint32_t)__popc(bits);
+        const unsigned long long b0 = __ballot(cnt & 1u), b1 = __ballot(cnt & 2u), b2 = __ballot(cnt & 4u);
+        uint32_t pos = tail + rank_in_mask(b0) + 2u * rank_in_mask(b1) + 4u * rank_in_mask(b2);
+        tail += (uint32_t)__popcll(b0) + 2u * (uint32_t)__popcll(b1) + 4u * (uint32_t)__popcll(b2);
+        while (bits) {  // at most BATCH passes, usually one
+            const uint32_t i = (uint32_t)__ffs(bits) - 1u;
+            bits &= bits - 1u;
+            ring[pos & (RING - 1)] = (lane << 16) | (q0 + i);
+            ++pos;
         }
-        tail = pos;
         while (tail - head >= 64) {
             wave_lds_sync();
             eval(ring[(head + lane) & (RING - 1)]);
@@ -938,16 +949,15 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick(Grid cur, Grid nxt, Rep
                 const bool active = valid && !(me.tw & TW_OUTSIDE) && reach >= 0.f;
                 const float *thr_row = ft.thr + t * nt;
                 // prefilter, branch-free: one LDS read of the candidate, one of the threshold
-                auto keep = [&](uint32_t q) -> bool {
-                    const float4 pq = m.P[q];
+                auto keep = [&](uint32_t q, const float4 &pq) -> bool {
                     const uint32_t tw2 = __float_as_uint(pq.w);
                     float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
                     if constexpr (!AH) frame_min_image(F, dx, dy, dz);
                     const float d2p = prefilter_d2(dx, dy, dz);
                     const float thr = thr_row[tw_type(tw2)];
-                    return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
+                    return (q != j) & (d2p < thr);  // dead / outside records hold NaN positions
                 };
-                pair_pipeline<AH>(wc, m.hstart, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
+                pair_pipeline<AH>(wc, m.hstart, m.P, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
                 wave_lds_sync();
                 if (valid)
                     finish_particle<MODE>(nxt, R, geo, T, topo, cap, dt, noise, force_out, sj, me, true,
@@ -1143,8 +1153,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
             const float *thr_row = st.thr + t * nt;
             // each unordered pair once: the lower type finds it; equal types: the lower bin slot
             // (which also excludes the particle itself); branch-free
-            auto keep = [&](uint32_t q) -> bool {
-                const float4 pq = m.P[q];
+            auto keep = [&](uint32_t q, const float4 &pq) -> bool {
                 const uint32_t tw2 = __float_as_uint(pq.w);
                 const uint32_t t2 = tw_type(tw2);
                 float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
@@ -1154,7 +1163,7 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
                 const uint32_t sq = m.slot[q];
                 return ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & ((t2 > t) | ((t2 == t) & (sq > sj))) & (d2p < thr);
             };
-            pair_pipeline<AH>(wc, m.hstart, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
+            pair_pipeline<AH>(wc, m.hstart, m.P, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep, eval);
             wave_lds_sync();
         }
     };

@@ -94,13 +94,13 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        // the 64-bit products give both halves from one multiply (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
         c0 = n0;
-        c1 = lo1;
+        c1 = (uint32_t)p1;
         c2 = n2;
-        c3 = lo0;
+        c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -215,16 +215,26 @@ __device__ __forceinline__ float ipowf(float b, int32_t e) {
     return r;
 }
 
+// r = |x| and inv_r = 1/r of a pair (0 and 0 for coincident particles) from the hardware reciprocal
+// square root (1 ulp): every potential of the pair shares them
+struct PairDist {
+    float r, inv_r;
+};
+__device__ __forceinline__ PairDist pair_dist(float d2) {
+    const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    return PairDist{d2 * inv_r, inv_r};
+}
+
 // adds the force on "self" to (fx, fy, fz) and returns the pair energy (PotentialsOrder2.h:54-323)
-__device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, float x, float y, float z,
-                                                   float &fx, float &fy, float &fz) {
+__device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, const PairDist &pd, float x, float y,
+                                                   float z, float &fx, float &fy, float &fz) {
     switch (pt.kind) {
     case POT_HR: {  // E = k/2 (r - d)^2, F = k (r - d)/r x  (PotentialsOrder2.h:72-91)
         const float k = pt.p[0], d = pt.p[1];
-        float r = sqrtf(d2);
+        const float r = pd.r;
         float dr = r - d;
         if (d2 > 0.0f) {
-            float fac = k * dr / r;
+            float fac = k * dr * pd.inv_r;
             fx += fac * x;
             fy += fac * y;
             fz += fac * z;
@@ -233,7 +243,7 @@ __device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, f
     }
     case POT_WEAK: {  // three harmonic pieces (PotentialsOrder2.h:131-183)
         const float k = pt.p[0], d0 = pt.p[1], h = pt.p[2], rc = pt.p[3], c = pt.p[4];  // c = (2/(rc-d0))^2
-        float r = sqrtf(d2);
+        const float r = pd.r;
         float e, fac;
         if (r < d0) {
             e = 0.5f * k * (r - d0) * (r - d0) - h;
@@ -246,7 +256,7 @@ __device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, f
             fac = h * c * (rc - r);
         }
         if (r > 0.0f && fac != 0.0f) {
-            float q = fac / r;
+            float q = fac * pd.inv_r;
             fx += q * x;
             fy += q * y;
             fz += q * z;
@@ -255,7 +265,7 @@ __device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, f
     }
     case POT_LJ: {  // E = k[(s/r)^m - (s/r)^n] (- shift), F = -k/s^2 (m (s/r)^(m+2) - n (s/r)^(n+2)) x
         const float k = pt.p[0], sigma = pt.p[1], eshift = pt.p[2];
-        float inv_r = 1.0f / sqrtf(d2);
+        const float inv_r = pd.inv_r;
         float q = sigma * inv_r;
         float qm = ipowf(q, pt.ia), qn = ipowf(q, pt.ib), q2 = q * q;
         float fac = -k / (sigma * sigma) * ((float)pt.ia * qm * q2 - (float)pt.ib * qn * q2);
@@ -266,8 +276,7 @@ __device__ __forceinline__ float pair_force_energy(const PotDev &pt, float d2, f
     }
     case POT_SE: {  // E = C e^(-kr)/r + D (s/r)^n  (PotentialsOrder2.h:299-315)
         const float C = pt.p[0], kappa = pt.p[1], D = pt.p[2], sigma = pt.p[3];
-        float r = sqrtf(d2);
-        float inv_r = 1.0f / r;
+        const float r = pd.r, inv_r = pd.inv_r;
         float ex = expf(-kappa * r);
         float qn = ipowf(sigma * inv_r, pt.ia);
         float ff = C * ex * (kappa * inv_r + inv_r * inv_r) + D * (float)pt.ia / sigma * qn * (sigma * inv_r);
@@ -317,9 +326,10 @@ __device__ __forceinline__ bool pair_fixed(const PotDev *pots, const PairIdx &pi
     if (!(d2 < pi.maxcut2)) return false;
     float pfx = 0.f, pfy = 0.f, pfz = 0.f, pe = 0.f;
     const float xf = (float)dx, yf = (float)dy, zf = (float)dz, d2f = (float)d2;
+    const PairDist pd = pair_dist(d2f);
     for (uint32_t p = 0; p < pi.count; ++p) {
         const PotDev pot = pots[pi.start + p];
-        if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, xf, yf, zf, pfx, pfy, pfz);
+        if (d2 < pot.cut2) pe += pair_force_energy(pot, d2f, pd, xf, yf, zf, pfx, pfy, pfz);
     }
     fx = to_fixed(pfx, 4294967296.0f);
     fy = to_fixed(pfy, 4294967296.0f);
@@ -339,9 +349,10 @@ __device__ __forceinline__ void pair_virial(const PotDev *pots, const PairIdx &p
     float f[3] = {0.f, 0.f, 0.f};
     const float x[3] = {(float)dx, (float)dy, (float)dz};
     const float d2f = (float)d2;
+    const PairDist pd = pair_dist(d2f);
     for (uint32_t p = 0; p < pi.count; ++p) {
         const PotDev pot = pots[pi.start + p];
-        if (d2 < pot.cut2) (void)pair_force_energy(pot, d2f, x[0], x[1], x[2], f[0], f[1], f[2]);
+        if (d2 < pot.cut2) (void)pair_force_energy(pot, d2f, pd, x[0], x[1], x[2], f[0], f[1], f[2]);
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a)

@@ -148,8 +148,6 @@ struct rdy_engine {
     // fused step (rdy_step.hip): the walk's per-brick force rows (2^-32 fixed point), the records
     // changed by reactions, the records moved again and the copies withdrawn; force_out and the
     // energies are formed on demand after a run (export_valid)
-    long long *gstage = nullptr;
-    uint64_t gstage_rows = 0;
     Edit *edits = nullptr;
     uint32_t *edit_count = nullptr;  // [3]: changed records, records to move again, copies to withdraw
     uint32_t edit_cap = 0;
@@ -593,15 +591,6 @@ int alloc_pools(rdy_engine *e) {
 uint32_t n_bricks(const rdy_engine *e);
 
 int ensure_step_pools(rdy_engine *e) {
-    // force rows of the walk: max_states per brick (a brick's own records are among its staged ones)
-    const uint64_t rows = (uint64_t)n_bricks(e) * e->max_states;
-    if (rows > e->gstage_rows) {
-        HIP_TRY(hipStreamSynchronize(e->stream));
-        if (e->gstage) HIP_TRY(hipFree(e->gstage));
-        e->gstage = nullptr;
-        HIP_TRY(dalloc(&e->gstage, 3 * rows));
-        e->gstage_rows = rows;
-    }
     if (e->edits) return RDY_OK;
     const uint64_t slots = e->max_cells * e->cap + e->ovf_cap;
     HIP_TRY(dalloc(&e->aff_bits, slots / 32 + 1));
@@ -938,7 +927,7 @@ void launch_walk(rdy_engine *e, int prologue, int handler, double dt, bool slow_
         hipLaunchKernelGGL((k_brick_step<decltype(ah)::value>), dim3(nb), dim3(BRICK_THREADS), e->brick_smem,
                            e->stream, e->grid(c), e->grid(n), prologue, e->reps, e->T, e->topo(),
                            (const BrickDev *)e->d_bricks, b0, e->cap, e->radius, handler, dt, e->events, e->ev_count,
-                           e->gstage, e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
+                           e->max_states, e->max_halo, (uint32_t)e->brick_smem, slow_list(e, 0), e->err);
     });
     if (e->prof && !prologue && e->prof_used < e->prof_ev.size()) {
         (void)hipEventRecord(e->prof_ev[e->prof_used].second, e->stream);
@@ -1168,7 +1157,7 @@ int rdy_engine_destroy(rdy_engine *e) {
         (void)hipFree(e->ovf_cell[b]);
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
-                    e->force_out, e->gstage, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
+                    e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
                     e->d_rowmax, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)

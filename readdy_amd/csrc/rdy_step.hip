@@ -222,19 +222,18 @@ __device__ void order_inner_by_reach(const BrickSmem &m, uint32_t n_inner, const
     __syncthreads();
 }
 
-// The walk of one brick over cur = x'(t): pair forces, stored per slot when a fix-up may need them
-// (gforce[3 s ..], 2^-32 fixed point); step t's reaction candidates (order 1 and 2, the draws of
-// k_brick_search); and each particle moved to step t+1 (or copied) into nxt, fwd[s] = its new slot.
-// prologue: the first walk of a run (no candidates; moves replicas with steps left, Philox step
-// R.step + 1).  Replicas neither searching nor moving are copied without a walk.
+// The walk of one brick over cur = x'(t): the pair forces of the brick's own records (2^-32 fixed
+// point, summed in LDS per wave), step t's reaction candidates (order 1 and 2, the draws of
+// k_brick_search), and each particle moved to step t+1 (or copied) into nxt as soon as its force is
+// complete.  prologue: the first walk of a run (no candidates; moves replicas with steps left, Philox
+// step R.step + 1).  Replicas neither searching nor moving are copied without a walk.  Four waves per
+// SIMD are asked of the register allocator (the move inside the walk loop would otherwise take the
+// kernel to three).
 template <bool AH>
-__global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt, int prologue, ReplicaDev *reps,
-                                                              Tables T, Topo topo, const BrickDev *bricks,
-                                                              uint32_t brick0, uint32_t cap, int radius, int handler,
-                                                              double dt, Event *events, uint32_t *ev_count,
-                                                              long long *gstage, uint32_t max_states,
-                                                              uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow,
-                                                              uint32_t *err) {
+__global__ __launch_bounds__(BRICK_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_brick_step(
+    Grid cur, Grid nxt, int prologue, ReplicaDev *reps, Tables T, Topo topo, const BrickDev *bricks, uint32_t brick0,
+    uint32_t cap, int radius, int handler, double dt, Event *events, uint32_t *ev_count, uint32_t max_states,
+    uint32_t max_halo, uint32_t smem_bytes, uint32_t *slow, uint32_t *err) {
     extern __shared__ __align__(16) char smem[];
     __shared__ ForceWave wave_lds[BRICK_WAVES];
     const bool staged_types = T.n_types <= MAX_STAGED_TYPES;
@@ -308,7 +307,6 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
             atomicAdd(&err[ERR_EVENT_OVERFLOW], 1u);
         }
     };
-    long long *gs = gstage + 3ull * max_states * bidx;  // n_inner <= max_states rows of this brick
     ForceWave &W = wave_lds[threadIdx.x / 64];
     const uint32_t lane = __lane_id();
     // a surviving pair (entry = owner lane << 16 | staged neighbour): the exact FP64 gates, the
@@ -398,33 +396,21 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_step(Grid cur, Grid nxt
         const bool active = valid && !(me.tw & TW_OUTSIDE) && reach >= 0.f;
         const float *thr_row = thr_tab + t * nt;
         // prefilter, branch-free: one LDS read of the candidate, one of the threshold
-        auto keep = [&](uint32_t q) -> bool {
-            const float4 pq = m.P[q];
+        auto keep = [&](uint32_t q, const float4 &pq) -> bool {
             const uint32_t tw2 = __float_as_uint(pq.w);
             float dx = pq.x - pj.x, dy = pq.y - pj.y, dz = pq.z - pj.z;
             if constexpr (!AH) frame_min_image(F, dx, dy, dz);
             const float d2p = prefilter_d2(dx, dy, dz);
             const float thr = thr_row[tw_type(tw2)];
-            return (q != j) & ((tw2 & (TW_DEAD | TW_OUTSIDE)) == 0u) & (d2p < thr);
+            return (q != j) & (d2p < thr);  // dead / outside records hold NaN positions
         };
-        pair_pipeline<AH>(wc, m.hstart, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep,
+        pair_pipeline<AH>(wc, m.hstart, m.P, m.rings + (threadIdx.x / 64) * RING, active, m.hc_of[j], pj, reach, keep,
                           eval);
         wave_lds_sync();
-        if (valid) {  // the brick's own staging rows, written and read back by this thread only
-            gs[3ull * ii] = (long long)W.acc[0][lane];
-            gs[3ull * ii + 1] = (long long)W.acc[1][lane];
-            gs[3ull * ii + 2] = (long long)W.acc[2][lane];
-        }
+        // the move with the finished pair force, from the owner's registers
+        const long long fx = (long long)W.acc[0][lane], fy = (long long)W.acc[1][lane], fz = (long long)W.acc[2][lane];
         wave_lds_sync();
-    }
-    // the moves, after the walk (the walk's registers are free by now): thread tid owns the inner
-    // records tid, tid + 256, ... in both loops
-    for (uint32_t ii = threadIdx.x; ii < n_inner; ii += BRICK_THREADS) {
-        const uint32_t j = m.inner[ii];
-        if (__float_as_uint(m.P[j].w) & TW_DEAD) continue;
-        const State me = cur.bins[m.slot[j]];
-        advance_particle(nxt, R, geo, T, topo, cap, dt, step_next, integ, me, gs[3ull * ii], gs[3ull * ii + 1],
-                         gs[3ull * ii + 2], err);
+        if (valid) advance_particle(nxt, R, geo, T, topo, cap, dt, step_next, integ, me, fx, fy, fz, err);
     }
 }
 
