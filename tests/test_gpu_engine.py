@@ -335,6 +335,30 @@ def test_force_beyond_fixed_point_range_is_a_numeric_error():
     assert np.all(np.isfinite(f)) and len(f) == len(good)
 
 
+def test_more_accepted_events_than_a_step_holds_is_a_capacity_error():
+    """A step accepts at most 1024 reaction events per replica (the resolve workgroup keeps them in LDS).
+    2000 particles of a species that decays at rate 1e10 all fire in one step: the separate reaction action
+    and the fused step report RDY_ERR_CAPACITY naming the event limit, every kernel ends normally, and the
+    engine keeps working on a state within the limit."""
+    rng = np.random.default_rng(47)
+    n, Lb = 2000, 30.0
+    pos = (rng.random((n, 3)) - 0.5) * Lb
+    types = np.zeros(n, dtype=np.uint32)
+    decay = [dict(kind=L.RX_DECAY, educts=[0], products=[], rate=1e10)]
+    e, r = _engine(2, [Lb] * 3, (1, 1, 1), [], rxs=decay, n=n)
+    e.set_particles(r, pos, types)
+    with pytest.raises(L.EngineError, match="error -3.*(event|1024)"):
+        e.react(L.HANDLER_UA, 0.01)
+    e.set_particles(r, pos, types)
+    with pytest.raises(L.EngineError, match="error -3.*(event|1024)"):
+        e.run(L.HANDLER_UA, 0.01, 2)
+    # within the limit: 500 decaying particles and 1500 of an inert species all behave
+    types[:1500] = 1
+    e.set_particles(r, pos, types)
+    e.run(L.HANDLER_UA, 0.01, 2)
+    assert list(e.count_types(1)[0]) == [0, 1500]
+
+
 def test_philox_noise_matches_oracle_stream():
     """Philox-driven BD: identical integer draws, the same FP32 Box-Muller transform on both sides.  The
     device's FP32 logarithm, sine and cosine and the oracle's correctly rounded ones may differ by an FP32
