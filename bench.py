@@ -257,6 +257,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-rdf", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the engine's own brick for the 1M system")
     ap.add_argument("--serving-requests", type=int, default=200)
     ap.add_argument("--serving-step-scale", type=float, default=0.1)
     ap.add_argument("--ensemble-replicas", type=int, default=4096)
@@ -285,11 +286,26 @@ def main():
 
     pos, types = c1_system()
     pots, rxs = c1_chemistry(L)
-    eng = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs)
-    r = eng.add_replica([200.0] * 3, (1, 1, 1), seed=1, max_particles=len(types))
-    eng.set_particles(r, pos, types)
     dt = 0.005
     handler = L.HANDLER_UA
+
+    def make_engine(cand):
+        eng = Engine(3, [1.0, 1.0, 0.5], potentials=pots, reactions=rxs, cell_radius=cand["cell_radius"],
+                     brick=tuple(cand["brick"]))
+        r = eng.add_replica([200.0] * 3, (1, 1, 1), seed=1, max_particles=len(types))
+        eng.set_particles(r, pos, types)
+        return eng, r
+
+    tuned = None
+    chosen = {"cell_radius": 1, "brick": [0, 0, 0]}
+    if not args.no_autotune:
+        from readdy_amd.tune import autotune_system, brick_candidates
+        best, tried = autotune_system(make_engine, handler, dt, brick_candidates(len(types) / 200.0 ** 3, 2.0),
+                                      steps=30, warmup=5)
+        chosen = {"cell_radius": best["cell_radius"], "brick": best["brick"]}
+        tuned = {"chosen": best, "tried": tried, "steps": 30}
+        print(f"bench: autotuned C1 geometry {chosen}", file=sys.stderr, flush=True)
+    eng, r = make_engine(chosen)
     graph = not args.eager
     eng.run(handler, dt, args.warmup, graph=graph)
     eng.synchronize()
@@ -324,6 +340,7 @@ def main():
                      "kernel": WALK_KERNEL, "algorithmic_bytes_per_particle": B_WALK,
                      "avg_kernel_us": avg_us, "launches": launches},
         "step_roofline_frac": (value * B_ALG / 1e9) / HBM_PEAK_GBS,
+        "geometry": {"cell_radius": chosen["cell_radius"], "brick": chosen["brick"], "autotune": tuned},
         "device": device_info()["name"],
     }
     if not args.no_rdf:
