@@ -48,6 +48,10 @@ extern "C" {
 #define RDY_HANDLER_UNCONTROLLED_APPROXIMATION 0
 #define RDY_HANDLER_GILLESPIE 1
 #define RDY_HANDLER_NONE 2
+/* DetailedBalance (include/readdy/model/actions/Actions.h:157-176, SCPUReactionImpls.cpp:407-555): the
+ * Gillespie selection, then each selected event of a reversible pair a Metropolis move on its energy
+ * change.  rdy_react only (not a step handler of rdy_run / rdy_run_graph / rdy_run_begin). */
+#define RDY_HANDLER_DETAILED_BALANCE 3
 
 typedef struct rdy_engine rdy_engine;
 
@@ -157,7 +161,8 @@ int rdy_get_virial(rdy_engine *e, uint32_t replica, double *virial9);
  * noise: NULL for the Philox stream, else a device-independent host array [max_handle][3]
  * of N(0,1) draws indexed by handle (single-replica engines only; injected-noise parity). */
 int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_handles);
-/* CPUUncontrolledApproximation::perform / CPUGillespie::perform */
+/* CPUUncontrolledApproximation::perform / CPUGillespie::perform / SCPUDetailedBalance::perform
+ * (handler RDY_HANDLER_DETAILED_BALANCE; the reversible pairs are those rdy_reversible_reactions lists) */
 int rdy_react(rdy_engine *e, int handler, double dt);
 /* reaction counts of the last handler call, [n_reactions] (StateModel::reactionCounts) */
 int rdy_get_reaction_counts(rdy_engine *e, uint32_t replica, uint64_t *counts);
@@ -183,6 +188,36 @@ int rdy_record_reactions(rdy_engine *e, int on);
  * cap are not written) */
 int rdy_get_reaction_records(rdy_engine *e, uint32_t replica, uint64_t cap, rdy_reaction_record *out,
                              uint64_t *n);
+
+/* ReversibleReactionConfig (include/readdy/model/actions/DetailedBalance.h:64-139) as
+ * DetailedBalance::searchReversibleReactions pairs them (readdy/main/model/Actions.cpp:432-519):
+ * forward / backward index into rdy_context_desc.reactions (for FusionFission the forward reaction is the
+ * fusion), lhs / rhs the species of the two sides (second entry 0 for a single species).  Host computation,
+ * no device needed; total_volume is the box volume.  Errors as the reference throws them (ambiguous
+ * or inconsistent pairs) return RDY_ERR_INVALID with the reference's message.  Writes at most cap rows;
+ * *n = the number of pairs. */
+#define RDY_REVERSIBLE_FUSION_FISSION 0
+#define RDY_REVERSIBLE_CONVERSION_CONVERSION 1
+#define RDY_REVERSIBLE_ENZYMATIC_ENZYMATIC 2
+typedef struct {
+    int32_t kind;
+    uint32_t forward, backward;
+    uint32_t n_lhs, n_rhs;
+    uint32_t lhs[2], rhs[2];
+    double reaction_radius;
+    double lhs_interaction_radius, rhs_interaction_radius;
+    double lhs_interaction_volume, rhs_interaction_volume;
+    double eff_lhs_interaction_volume, eff_rhs_interaction_volume;
+    double eff_lhs_reaction_volume, eff_rhs_reaction_volume;
+    double acceptance_prefactor;
+    double total_volume;
+    double kbt;
+    double equilibrium_constant;
+    double macro_forward_rate, macro_backward_rate;
+    double micro_forward_rate, micro_backward_rate;
+} rdy_reversible_reaction;
+int rdy_reversible_reactions(const rdy_context_desc *ctx, double total_volume, uint64_t cap,
+                             rdy_reversible_reaction *out, uint64_t *n);
 
 /* BreakBonds (include/readdy/model/actions/Actions.h:218-341) with its BreakConfig (Actions.h:190-211):
  * a bond edge (the consecutive bond terms of one replica on one handle pair, i.e. all bond

@@ -127,6 +127,9 @@ def lib():
         L.or_topology_types.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_topology_reaction_counts.restype = _u64p
         L.or_topology_reaction_counts.argtypes = [ctypes.c_void_p, _ip]
+        L.or_reversible_configs.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _dp, ctypes.POINTER(ctypes.c_size_t)]
+        L.or_fission_table.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.or_detailed_balance.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, _u64p]
         L.or_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -383,6 +386,38 @@ class System:
     def reactions(self, mode, dt, step=0):
         n = ctypes.c_uint64()
         _check(lib().or_reactions(self._h, mode, dt, step, ctypes.byref(n)))
+        return n.value
+
+    REV_FIELDS = ("kind", "forward", "backward", "lhs0", "lhs1", "rhs0", "rhs1", "reaction_radius", "lhs_radius",
+                  "rhs_radius", "lhs_volume", "rhs_volume", "eff_lhs_interaction", "eff_rhs_interaction",
+                  "eff_lhs_reaction", "eff_rhs_reaction", "acceptance_prefactor", "total_volume",
+                  "equilibrium_constant", "macro_forward", "macro_backward")
+
+    def reversible_configs(self):
+        """DetailedBalance's reversible pairs and their configuration, one dict per pair"""
+        n = ctypes.c_size_t()
+        _check(lib().or_reversible_configs(self._h, 0, None, ctypes.byref(n)))
+        rows = np.zeros((max(n.value, 1), len(self.REV_FIELDS)))
+        _check(lib().or_reversible_configs(self._h, n.value, rows.ctypes.data_as(_dp), ctypes.byref(n)))
+        out = []
+        for r in rows[:n.value]:
+            d = dict(zip(self.REV_FIELDS, r.tolist()))
+            for k in ("kind", "forward", "backward", "lhs0", "lhs1", "rhs0", "rhs1"):
+                d[k] = int(d[k])
+            out.append(d)
+        return out
+
+    def fission_table(self, k):
+        radii, cdf = np.zeros(1001), np.zeros(1001)
+        rc = lib().or_fission_table(self._h, k, radii.ctypes.data_as(_dp), cdf.ctypes.data_as(_dp))
+        if rc < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return radii[:rc], cdf[:rc]
+
+    def detailed_balance(self, dt, step=0):
+        """one DetailedBalance handler call; returns the number of performed events"""
+        n = ctypes.c_uint64()
+        _check(lib().or_detailed_balance(self._h, dt, step, ctypes.byref(n)))
         return n.value
 
     # or_reaction_record (readdy_oracle.h)

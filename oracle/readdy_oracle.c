@@ -15,6 +15,8 @@
  *   order-1 reaction r of particle h        ctr = (h, t, 1 | r << 8, 0)
  *   order-2 reaction r of pair (lo, hi)     ctr = (lo, t, 2 | r << 8, hi)   (lo < hi ids)
  *   fission direction of particle h         ctr = (h, t, 3, 0)
+ *   DetailedBalance fission distance of h   ctr = (h, t, 10, 0)        u53(x0, x1)
+ *   DetailedBalance Metropolis, reaction r  ctr = (lo, t, 11 | r << 8, hi)   u53(x0, x1), lo <= hi ids
  *   uniform from words (a, b)               u53(a, b) = ((a >> 5) * 2^26 + (b >> 6)) / 2^53
  *   normals (Box-Muller)                    v_k = (x_k + 0.5) / 2^32;
  *        n0 = sqrt(-2 ln v0) cos(2 pi v1), n1 = sqrt(-2 ln v0) sin(2 pi v1), n2 = sqrt(-2 ln v2) cos(2 pi v3)
@@ -34,6 +36,12 @@
  *   performReaction      reference known answers (kernels/cpu/test/TestReactions.cpp, readdy/test/TestReactions.cpp)
  *   UA / Gillespie       deterministic reference scenes (kernels/cpu/test/TestReactions.cpp:250-354) and
  *                        the statistical equivalence of literal and keyed Gillespie selection
+ *   reversible configs   reference known answers (readdy/test/TestDetailedBalance.cpp: volumes, prefactor,
+ *                        rates, pairs found and rejected) and closed forms (free-particle volumes, the
+ *                        harmonic-repulsion CDF against direct quadrature)
+ *   DetailedBalance      closed forms: acceptance 1 without potentials, the Metropolis ratio of single
+ *                        conversions against exp(-dE/kT) of a two-particle scene, the rejected move's id
+ *                        bookkeeping; the equilibrium statistics of a conversion pair
  */
 #include "readdy_oracle.h"
 
@@ -511,6 +519,10 @@ struct or_system {
     str_t str[OR_MAX_STR];
     int n_str;
     uint64_t str_counts[OR_MAX_STR];
+    /* DetailedBalance: reversible pairs, built on the first call */
+    struct rev_s *rev;
+    int n_rev, rev_ready;
+    int rev_of_rx[OR_MAX_RX];
 };
 
 or_system *or_create(int n_types, const double *box, const int *pbc, double kbt, const double *diffusion) {
@@ -544,6 +556,7 @@ void or_destroy(or_system *s) {
     free(s->head);
     free(s->list);
     free(s->recs);
+    free(s->rev);
     free(s);
 }
 
@@ -1536,6 +1549,535 @@ int or_reactions(or_system *s, int mode, double dt, uint32_t step, uint64_t *n_p
         }
     }
     apply_update(s, &up);
+    free(u.ev);
+    if (n_performed) *n_performed = done;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * DetailedBalance.
+ *   reversible pairs: DetailedBalance::searchReversibleReactions (readdy/main/model/Actions.cpp:432-519)
+ *   their configuration: ReversibleReactionConfig (Actions.cpp:67-367); the integrals of
+ *     4 pi r^2 exp(-U(r)/kBT) by adaptive Simpson quadrature to a relative 1e-13 (the reference uses
+ *     adaptive Gauss-Kronrod, common/integration.h; the values agree to the quadrature accuracy)
+ *   event loop: SCPUDetailedBalance::perform (kernels/singlecpu/src/actions/SCPUReactionImpls.cpp:407-555)
+ *     with the keyed Gillespie selection: the events that pass their Bernoulli draw, in key order;
+ *     each one not sharing a particle with an earlier evaluated event is evaluated on the state the
+ *     earlier ones left (accepted or not, it blocks the later events that share a particle)
+ *   proposal: performReversibleReactionEvent (SCPUReactionImpls.cpp:579-743); rollback with fresh
+ *     ids: ParticleBackup / generateBackwardUpdate (SCPUReactionImpls.cpp:362-405, 557-577)
+ *   energy: calculateEnergies (SCPUReactionImpls.cpp:745-781), the pair potentials only; here the
+ *     change is summed directly over the particles that leave and arrive, brute force over all
+ *     particles inside the box, each potential gated by d^2 < cutoff^2 as the force pass gates it
+ * --------------------------------------------------------------------------------------------- */
+#define OR_FISSION_POINTS 1000
+
+typedef struct rev_s {
+    int kind; /* 0 FusionFission, 1 ConversionConversion, 2 EnzymaticEnzymatic */
+    int fwd, bwd, n_lhs, n_rhs;
+    int lhs[2], rhs[2];
+    double reaction_radius, lhs_radius, rhs_radius, lhs_volume, rhs_volume;
+    double eff_lhs_int, eff_rhs_int, eff_lhs_rx, eff_rhs_rx;
+    double prefactor, total_volume, K, kf, kb;
+    double radii[OR_FISSION_POINTS + 1], cdf[OR_FISSION_POINTS + 1];
+} rev_t;
+
+/* U(r) of the potentials between types a and b, as calculateEnergy((r, 0, 0)) sums them (no gate) */
+static double rev_u(const or_system *s, int a, int b, double r) {
+    double x[3] = {r, 0., 0.}, f[3] = {0., 0., 0.}, u = 0.;
+    for (int k = 0; k < s->n_pots; ++k) {
+        const pot_t *pt = &s->pots[k];
+        if ((pt->t1 == a && pt->t2 == b) || (pt->t1 == b && pt->t2 == a)) u += or_pair_force_energy(pt->kind, pt->p, x, f);
+    }
+    return u;
+}
+
+typedef struct {
+    const or_system *s;
+    int a, b, any;
+} rev_f;
+
+static double rev_integrand(const rev_f *f, double r) {
+    const double u = f->any ? rev_u(f->s, f->a, f->b, r) : 0.;
+    return 4. * 3.14159265358979323846 * r * r * exp(-1. * u / f->s->kbt);
+}
+
+static double simpson_rec(const rev_f *f, double a, double b, double fa, double fm, double fb, double whole, double tol,
+                          int depth) {
+    const double m = .5 * (a + b), lm = .5 * (a + m), rm = .5 * (m + b);
+    const double flm = rev_integrand(f, lm), frm = rev_integrand(f, rm);
+    const double left = (m - a) / 6. * (fa + 4. * flm + fm), right = (b - m) / 6. * (fm + 4. * frm + fb);
+    const double d = left + right - whole;
+    if (depth <= 0 || fabs(d) <= 15. * tol) return left + right + d / 15.;
+    return simpson_rec(f, a, m, fa, flm, fm, left, .5 * tol, depth - 1) +
+           simpson_rec(f, m, b, fm, frm, fb, right, .5 * tol, depth - 1);
+}
+
+static double rev_integrate(const rev_f *f, double a, double b) {
+    if (!(a < b)) return 0.;
+    /* a first pass on 64 panels fixes the scale of the absolute tolerance */
+    double coarse = 0.;
+    const int np = 64;
+    for (int i = 0; i < np; ++i) {
+        const double x0 = a + (b - a) * i / np, x1 = a + (b - a) * (i + 1) / np;
+        coarse += (x1 - x0) / 6. * (rev_integrand(f, x0) + 4. * rev_integrand(f, .5 * (x0 + x1)) + rev_integrand(f, x1));
+    }
+    const double tol = 1e-13 * fabs(coarse) / np;
+    double total = 0.;
+    for (int i = 0; i < np; ++i) {
+        const double x0 = a + (b - a) * i / np, x1 = a + (b - a) * (i + 1) / np;
+        const double f0 = rev_integrand(f, x0), fm = rev_integrand(f, .5 * (x0 + x1)), f1 = rev_integrand(f, x1);
+        total += simpson_rec(f, x0, x1, f0, fm, f1, (x1 - x0) / 6. * (f0 + 4. * fm + f1), tol, 40);
+    }
+    return total;
+}
+
+static int n_educts_of(int kind) { return (kind == OR_RX_FUSION || kind == OR_RX_ENZYMATIC) ? 2 : 1; }
+static int n_products_of(int kind) {
+    return kind == OR_RX_DECAY ? 0 : ((kind == OR_RX_FISSION || kind == OR_RX_ENZYMATIC) ? 2 : 1);
+}
+
+static double pot_cutoff_of_pair(const or_system *s, int a, int b, int *any) {
+    double m = 0.;
+    *any = 0;
+    for (int k = 0; k < s->n_pots; ++k) {
+        const pot_t *pt = &s->pots[k];
+        if ((pt->t1 == a && pt->t2 == b) || (pt->t1 == b && pt->t2 == a)) {
+            const double c = sqrt(pt->cutoff2);
+            if (!*any || c > m) m = c;
+            *any = 1;
+        }
+    }
+    return m;
+}
+
+static int rev_configure(const or_system *s, int fwd, int bwd, rev_t *c) {
+    memset(c, 0, sizeof *c);
+    const rx_t *F = &s->rx[fwd], *B = &s->rx[bwd];
+    int nl = n_educts_of(F->kind), nr = n_educts_of(B->kind);
+    if (nl == 2 && nr == 1) c->kind = 0;
+    else if (nl == 1 && nr == 2) { /* the fusion is the forward reaction */
+        const rx_t *t = F;
+        F = B;
+        B = t;
+        int ti = fwd;
+        fwd = bwd;
+        bwd = ti;
+        nl = 2;
+        nr = 1;
+        c->kind = 0;
+    } else if (nl == 1 && nr == 1) c->kind = 1;
+    else if (nl == 2 && nr == 2) c->kind = 2;
+    else return fail("Type of reversible reaction cannot be determined");
+    c->fwd = fwd;
+    c->bwd = bwd;
+    c->n_lhs = nl;
+    c->n_rhs = nr;
+    /* Reaction::educts() is a two-entry array; an order-1 reaction's second entry is 0 there, which the
+     * species comparison below sees as well */
+    c->lhs[0] = F->educts[0];
+    c->lhs[1] = nl == 2 ? F->educts[1] : 0;
+    c->rhs[0] = B->educts[0];
+    c->rhs[1] = nr == 2 ? B->educts[1] : 0;
+    int bp[2] = {B->products[0], n_products_of(B->kind) == 2 ? B->products[1] : 0};
+    int fp[2] = {F->products[0], n_products_of(F->kind) == 2 ? F->products[1] : 0};
+    if (!(c->lhs[0] == bp[0] && c->lhs[1] == bp[1]) || !(c->rhs[0] == fp[0] && c->rhs[1] == fp[1])) {
+        if (!(c->kind == 0 && c->lhs[0] == bp[1] && c->lhs[1] == bp[0] && fp[0] == c->rhs[0]))
+            return fail("The given reactions are not reversible with respect to particle species");
+    }
+    if (c->kind == 0) {
+        if (F->educts[0] == bp[0] && F->educts[1] == bp[1]) {
+            if (!(F->w1 == B->w1 && F->w2 == B->w2)) return fail("The given reactions are not reversible with respect to weights");
+        } else if (F->educts[0] == bp[1] && F->educts[1] == bp[0]) {
+            if (!(F->w1 == B->w2 && F->w2 == B->w1)) return fail("The given reactions are not reversible with respect to weights");
+        }
+    }
+    if (c->kind == 0) {
+        if (F->educt_distance != B->product_distance) return fail("Reaction radii forward and backward must be equal");
+        c->reaction_radius = F->educt_distance;
+    } else if (c->kind == 2) {
+        if (F->educt_distance != B->educt_distance) return fail("Reaction radii forward and backward must be equal");
+        c->reaction_radius = F->educt_distance;
+    }
+    rev_f fl = {s, c->lhs[0], c->lhs[1], 0}, fr = {s, c->rhs[0], c->rhs[1], 0};
+    if (c->kind != 1) c->lhs_radius = pot_cutoff_of_pair(s, c->lhs[0], c->lhs[1], &fl.any);
+    if (c->kind == 2) c->rhs_radius = pot_cutoff_of_pair(s, c->rhs[0], c->rhs[1], &fr.any);
+    const double pi = 3.14159265358979323846;
+    if (fl.any) c->lhs_volume = 4. / 3. * pi * pow(c->lhs_radius, 3);
+    if (fr.any) c->rhs_volume = 4. / 3. * pi * pow(c->rhs_radius, 3);
+    c->eff_lhs_int = rev_integrate(&fl, 0., c->lhs_radius);
+    c->eff_rhs_int = rev_integrate(&fr, 0., c->rhs_radius);
+    c->eff_lhs_rx = rev_integrate(&fl, 0., c->reaction_radius);
+    c->eff_rhs_rx = rev_integrate(&fr, 0., c->reaction_radius);
+    if (c->kind == 0) {
+        const double dr = c->reaction_radius / (double)OR_FISSION_POINTS;
+        double r = 0.;
+        for (int i = 0; i <= OR_FISSION_POINTS; ++i) {
+            c->radii[i] = r;
+            r += dr;
+        }
+        c->cdf[0] = 0.;
+        for (int i = 1; i <= OR_FISSION_POINTS; ++i) c->cdf[i] = c->cdf[i - 1] + rev_integrate(&fl, c->radii[i - 1], c->radii[i]);
+        const double last = c->cdf[OR_FISSION_POINTS];
+        if (last >= 1e-16) {
+            for (int i = 0; i <= OR_FISSION_POINTS; ++i) c->cdf[i] /= last;
+        } else {
+            for (int i = 0; i < OR_FISSION_POINTS; ++i) c->cdf[i] = 0.;
+            c->cdf[OR_FISSION_POINTS] = 1.;
+        }
+    }
+    c->prefactor = c->kind == 2 ? c->eff_lhs_rx / c->eff_rhs_rx : 1.;
+    c->total_volume = s->box[0] * s->box[1] * s->box[2];
+    const double kf = F->rate, kb = B->rate, V = c->total_volume;
+    if (c->kind == 0) {
+        c->K = (kb / kf) * (V - c->lhs_volume + c->eff_lhs_int) / c->eff_lhs_rx;
+        c->kb = kb;
+        c->kf = c->kb * V / c->K;
+    } else if (c->kind == 1) {
+        c->kf = kf;
+        c->kb = kb;
+        c->K = c->kb / c->kf;
+    } else {
+        c->kf = kf * V * c->eff_lhs_rx / (V - c->lhs_volume + c->eff_lhs_int);
+        c->kb = kb * V * c->eff_rhs_rx / (V - c->rhs_volume + c->eff_rhs_int);
+        c->K = c->kb / c->kf;
+    }
+    return 0;
+}
+
+static int rev_build(or_system *s) {
+    free(s->rev);
+    s->rev = NULL;
+    s->n_rev = 0;
+    for (int i = 0; i < OR_MAX_RX; ++i) s->rev_of_rx[i] = -1;
+    int cap = 0;
+    int o1[OR_MAX_RX], o2[OR_MAX_RX], n1 = 0, n2 = 0;
+    for (int i = 0; i < s->n_rx; ++i) {
+        if (n_educts_of(s->rx[i].kind) == 1) o1[n1++] = i;
+        else o2[n2++] = i;
+    }
+    int pairs[2 * OR_MAX_RX * OR_MAX_RX / 8][2];
+    int np = 0;
+    const int max_pairs = (int)(sizeof pairs / sizeof pairs[0]);
+    /* FusionFission: (fusion, fission) with the fission's product distance = the fusion's educt distance */
+    for (int a = 0; a < n2; ++a)
+        for (int b = 0; b < n1; ++b) {
+            const rx_t *r2 = &s->rx[o2[a]], *r1 = &s->rx[o1[b]];
+            if (!(n_products_of(r2->kind) == 1 && n_products_of(r1->kind) == 2 && r1->product_distance == r2->educt_distance))
+                continue;
+            const int same = r2->educts[0] == r1->products[0] && r2->educts[1] == r1->products[1] && r1->w1 == r2->w1 &&
+                             r1->w2 == r2->w2;
+            /* Actions.cpp:453-455 compare educts()[0] with products()[1] twice */
+            const int flipped = r2->educts[0] == r1->products[1] && r1->w1 == r2->w2 && r1->w2 == r2->w1;
+            if (r1->educts[0] == r2->products[0] && (same || flipped)) {
+                if (np >= max_pairs) return fail("too many reversible pairs");
+                pairs[np][0] = o2[a];
+                pairs[np][1] = o1[b];
+                ++np;
+            }
+        }
+    for (int i = 0; i < n1; ++i)
+        for (int j = i + 1; j < n1; ++j) {
+            const rx_t *ri = &s->rx[o1[i]], *rj = &s->rx[o1[j]];
+            if (n_products_of(ri->kind) == 1 && n_products_of(rj->kind) == 1 && ri->educts[0] == rj->products[0] &&
+                ri->products[0] == rj->educts[0]) {
+                if (np >= max_pairs) return fail("too many reversible pairs");
+                pairs[np][0] = o1[i];
+                pairs[np][1] = o1[j];
+                ++np;
+            }
+        }
+    for (int i = 0; i < n2; ++i)
+        for (int j = i + 1; j < n2; ++j) {
+            const rx_t *ri = &s->rx[o2[i]], *rj = &s->rx[o2[j]];
+            if (n_products_of(ri->kind) == 2 && n_products_of(rj->kind) == 2 && ri->educt_distance == rj->educt_distance &&
+                ri->educts[0] == rj->products[0] && ri->educts[1] == rj->products[1] && ri->products[0] == rj->educts[0] &&
+                ri->products[1] == rj->educts[1]) {
+                if (np >= max_pairs) return fail("too many reversible pairs");
+                pairs[np][0] = o2[i];
+                pairs[np][1] = o2[j];
+                ++np;
+            }
+        }
+    cap = np > 0 ? np : 1;
+    s->rev = (rev_t *)calloc((size_t)cap, sizeof(rev_t));
+    for (int k = 0; k < np; ++k)
+        if (rev_configure(s, pairs[k][0], pairs[k][1], &s->rev[k])) return -1;
+    s->n_rev = np;
+    /* equivalentReversibleReactions (DetailedBalance.h:144-148) */
+    for (int i = 0; i < np; ++i)
+        for (int j = i + 1; j < np; ++j) {
+            const rev_t *x = &s->rev[i], *y = &s->rev[j];
+            const int ll = x->lhs[0] == y->lhs[0] && x->lhs[1] == y->lhs[1], rr = x->rhs[0] == y->rhs[0] && x->rhs[1] == y->rhs[1];
+            const int lr = x->lhs[0] == y->rhs[0] && x->lhs[1] == y->rhs[1], rl = x->rhs[0] == y->lhs[0] && x->rhs[1] == y->lhs[1];
+            if ((ll && rr) || (lr && rl))
+                return fail("Duplicate found in reversible reactions. Registered reactions might be ambiguous with respect to reversibility.");
+        }
+    for (int k = 0; k < np; ++k) {
+        s->rev_of_rx[s->rev[k].fwd] = k;
+        s->rev_of_rx[s->rev[k].bwd] = k;
+    }
+    s->rev_ready = 1;
+    return 0;
+}
+
+int or_reversible_configs(or_system *s, size_t cap, double *rows, size_t *n) {
+    if (rev_build(s)) return -1;
+    for (int k = 0; k < s->n_rev && (size_t)k < cap; ++k) {
+        const rev_t *c = &s->rev[k];
+        double *o = rows + (size_t)k * OR_REV_ROW;
+        const double v[OR_REV_ROW] = {c->kind, c->fwd, c->bwd, c->lhs[0], c->lhs[1], c->rhs[0], c->rhs[1],
+                                      c->reaction_radius, c->lhs_radius, c->rhs_radius, c->lhs_volume, c->rhs_volume,
+                                      c->eff_lhs_int, c->eff_rhs_int, c->eff_lhs_rx, c->eff_rhs_rx, c->prefactor,
+                                      c->total_volume, c->K, c->kf, c->kb};
+        memcpy(o, v, sizeof v);
+    }
+    *n = (size_t)s->n_rev;
+    return 0;
+}
+
+int or_fission_table(or_system *s, int k, double *radii, double *cdf) {
+    if (!s->rev_ready && rev_build(s)) return -1;
+    if (k < 0 || k >= s->n_rev || s->rev[k].kind != 0) return fail("not a FusionFission pair");
+    memcpy(radii, s->rev[k].radii, sizeof s->rev[k].radii);
+    memcpy(cdf, s->rev[k].cdf, sizeof s->rev[k].cdf);
+    return OR_FISSION_POINTS + 1;
+}
+
+/* pair energy of a particle of type t at pos against every other particle inside the box (skipping
+ * entries x0 and x1), each potential gated by d^2 < cutoff^2 */
+static double db_env_energy(const or_system *s, const double *pos, int t, size_t x0, size_t x1) {
+    if (!in_box(s, pos)) return 0.;
+    double sum = 0.;
+    for (size_t i = 0; i < s->n; ++i) {
+        const entry_t *o = &s->e[i];
+        if (o->deactivated || i == x0 || i == x1 || !in_box(s, o->pos)) continue;
+        double x[3], f[3] = {0., 0., 0.};
+        or_shortest_difference(pos, o->pos, s->box, s->pbc, x);
+        const double d2 = dot3(x, x);
+        for (int k = 0; k < s->n_pots; ++k) {
+            const pot_t *pt = &s->pots[k];
+            if (!((pt->t1 == t && pt->t2 == o->type) || (pt->t1 == o->type && pt->t2 == t))) continue;
+            if (d2 < pt->cutoff2) sum += or_pair_force_energy(pt->kind, pt->p, x, f);
+        }
+    }
+    return sum;
+}
+
+/* the gated pair energy of two positions of types a and b (both inside the box) */
+static double db_pair_energy(const or_system *s, const double *p, int a, const double *q, int b) {
+    if (!in_box(s, p) || !in_box(s, q)) return 0.;
+    double x[3], f[3] = {0., 0., 0.}, u = 0.;
+    or_shortest_difference(p, q, s->box, s->pbc, x);
+    const double d2 = dot3(x, x);
+    for (int k = 0; k < s->n_pots; ++k) {
+        const pot_t *pt = &s->pots[k];
+        if (!((pt->t1 == a && pt->t2 == b) || (pt->t1 == b && pt->t2 == a))) continue;
+        if (d2 < pt->cutoff2) u += or_pair_force_energy(pt->kind, pt->p, x, f);
+    }
+    return u;
+}
+
+/* gated U at distance r between types a and b (the mutual interaction dU_int of a FusionFission move) */
+static double db_u_at(const or_system *s, int a, int b, double r) {
+    double x[3] = {r, 0., 0.}, f[3] = {0., 0., 0.}, u = 0.;
+    for (int k = 0; k < s->n_pots; ++k) {
+        const pot_t *pt = &s->pots[k];
+        if (!((pt->t1 == a && pt->t2 == b) || (pt->t1 == b && pt->t2 == a))) continue;
+        if (r * r < pt->cutoff2) u += or_pair_force_energy(pt->kind, pt->p, x, f);
+    }
+    return u;
+}
+
+static or_reaction_record *db_record(or_system *s) {
+    if (s->n_recs == s->cap_recs) {
+        s->cap_recs = s->cap_recs ? 2 * s->cap_recs : 64;
+        s->recs = (or_reaction_record *)realloc(s->recs, s->cap_recs * sizeof(or_reaction_record));
+    }
+    or_reaction_record *rec = &s->recs[s->n_recs++];
+    memset(rec, 0, sizeof *rec);
+    return rec;
+}
+
+/* one evaluated event; returns 1 when it was performed (accepted) */
+static int db_event(or_system *s, const event_t *ev, uint32_t step) {
+    const int k = s->rev_of_rx[ev->rx];
+    if (k < 0) { /* not reversible: performed directly (SCPUReactionImpls.cpp:525-548) */
+        update_t up;
+        memset(&up, 0, sizeof up);
+        perform_reaction(s, ev, &up, step);
+        apply_update(s, &up);
+        return 1;
+    }
+    const rx_t *r = &s->rx[ev->rx];
+    const rev_t *c = &s->rev[k];
+    entry_t *e1 = &s->e[ev->idx1], *e2 = &s->e[ev->idx2];
+    size_t educt = ev->idx1; /* the particle that converts (EnzymaticEnzymatic: not the catalyst) */
+    double dE = 0., du_int = 0., fdist = 0., n3[3] = {0., 0., 0.};
+    double q1[3], q2[3];
+    int n_fwd = 1, n_back = 1;
+    switch (r->kind) {
+    case OR_RX_FISSION: {
+        uint32_t x[4];
+        philox_draw(s->seed, (uint32_t)e1->id, step, 3, 0, x);
+        normals3(x, n3);
+        const double nn = sqrt(dot3(n3, n3));
+        for (int d = 0; d < 3; ++d) n3[d] /= nn;
+        philox_draw(s->seed, (uint32_t)e1->id, step, 10, 0, x);
+        const double u = or_u53(x[0], x[1]);
+        int lo = 0, hi = OR_FISSION_POINTS + 1; /* lower_bound, DetailedBalance.h:133-138 */
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (c->cdf[mid] < u) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo > OR_FISSION_POINTS) lo = OR_FISSION_POINTS;
+        fdist = c->radii[lo];
+        du_int = db_u_at(s, r->products[0], r->products[1], fdist);
+        for (int d = 0; d < 3; ++d) {
+            q1[d] = e1->pos[d] - r->w2 * fdist * n3[d]; /* products[1] */
+            q2[d] = e1->pos[d] + r->w1 * fdist * n3[d]; /* products[0] */
+        }
+        or_fix_position(q1, s->box, s->pbc);
+        or_fix_position(q2, s->box, s->pbc);
+        dE = db_env_energy(s, q1, r->products[1], ev->idx1, ev->idx1) + db_env_energy(s, q2, r->products[0], ev->idx1, ev->idx1) +
+             db_pair_energy(s, q1, r->products[1], q2, r->products[0]) - db_env_energy(s, e1->pos, e1->type, ev->idx1, ev->idx1);
+        n_fwd = 2;
+        break;
+    }
+    case OR_RX_FUSION: {
+        double diff[3];
+        or_shortest_difference(e1->pos, e2->pos, s->box, s->pbc, diff);
+        du_int = -db_u_at(s, r->educts[0], r->educts[1], sqrt(dot3(diff, diff)));
+        const double w = (r->educts[0] == e1->type) ? r->w1 : r->w2;
+        for (int d = 0; d < 3; ++d) q1[d] = e1->pos[d] + w * diff[d];
+        or_fix_position(q1, s->box, s->pbc);
+        dE = db_env_energy(s, q1, r->products[0], ev->idx1, ev->idx2) - db_env_energy(s, e1->pos, e1->type, ev->idx1, ev->idx2) -
+             db_env_energy(s, e2->pos, e2->type, ev->idx1, ev->idx2) - db_pair_energy(s, e1->pos, e1->type, e2->pos, e2->type);
+        n_back = 2;
+        break;
+    }
+    case OR_RX_ENZYMATIC:
+        educt = (e1->type == r->educts[1]) ? ev->idx2 : ev->idx1;
+        /* fall through */
+    default: {
+        const entry_t *ed = &s->e[educt];
+        dE = db_env_energy(s, ed->pos, r->products[0], educt, educt) - db_env_energy(s, ed->pos, ed->type, educt, educt);
+        break;
+    }
+    }
+    const double boltzmann = exp(-1. / s->kbt * (dE - du_int));
+    const double pref = (c->kind == 2 && r->educts[0] == c->lhs[0] && r->educts[1] == c->lhs[1]) ? c->prefactor : 1.;
+    double acceptance = pref * boltzmann;
+    if (!(acceptance < 1.)) acceptance = 1.; /* std::min(1., x) */
+    const uint64_t lo = e1->id < e2->id ? e1->id : e2->id, hi = e1->id < e2->id ? e2->id : e1->id;
+    uint32_t x[4];
+    philox_draw(s->seed, (uint32_t)lo, step, 11u | ((uint32_t)ev->rx << 8), (uint32_t)hi, x);
+    const int accept = or_u53(x[0], x[1]) < acceptance;
+    const uint64_t base = s->next_id;
+    s->next_id += (uint64_t)(n_fwd + n_back); /* products, then the backup particles */
+    if (!accept) {
+        if (r->kind == OR_RX_FUSION) {
+            e1->id = base + 1;
+            e2->id = base + 2;
+        } else {
+            s->e[educt].id = base + (uint64_t)n_fwd;
+        }
+        return 0;
+    }
+    or_reaction_record *rec = s->record ? db_record(s) : NULL;
+    if (rec) {
+        rec->reaction = (uint32_t)ev->rx;
+        rec->type = r->kind;
+    }
+    switch (r->kind) {
+    case OR_RX_FISSION: {
+        if (rec) {
+            for (int d = 0; d < 3; ++d) rec->where[d] = e1->pos[d];
+            or_fix_position(rec->where, s->box, s->pbc);
+            rec->educts[0] = rec->educts[1] = e1->id;
+            rec->types_from[0] = rec->types_from[1] = (uint32_t)e1->type;
+        }
+        update_t up;
+        memset(&up, 0, sizeof up);
+        entry_t p1, p2;
+        memset(&p1, 0, sizeof p1);
+        memset(&p2, 0, sizeof p2);
+        memcpy(p1.pos, q1, sizeof q1);
+        p1.type = r->products[1];
+        p1.id = base;
+        memcpy(p2.pos, q2, sizeof q2);
+        p2.type = r->products[0];
+        p2.id = base + 1;
+        upd_new(&up, &p1);
+        upd_new(&up, &p2);
+        upd_removed(&up, ev->idx1);
+        apply_update(s, &up);
+        break;
+    }
+    case OR_RX_FUSION: {
+        if (rec) {
+            for (int d = 0; d < 3; ++d) rec->where[d] = (e1->pos[d] + e2->pos[d]) / 2.;
+            or_fix_position(rec->where, s->box, s->pbc);
+            rec->educts[0] = e1->id;
+            rec->educts[1] = e2->id;
+            rec->types_from[0] = (uint32_t)e1->type;
+            rec->types_from[1] = (uint32_t)e2->type;
+        }
+        update_t up;
+        memset(&up, 0, sizeof up);
+        entry_t p;
+        memset(&p, 0, sizeof p);
+        memcpy(p.pos, q1, sizeof q1);
+        p.type = r->products[0];
+        p.id = base;
+        upd_new(&up, &p);
+        upd_removed(&up, ev->idx1);
+        upd_removed(&up, ev->idx2);
+        apply_update(s, &up);
+        break;
+    }
+    default: {
+        entry_t *ed = &s->e[educt];
+        if (rec) {
+            const entry_t *other = (educt == ev->idx1) ? e2 : e1; /* the catalyst, or the educt itself */
+            if (r->kind == OR_RX_CONVERSION) other = ed;
+            for (int d = 0; d < 3; ++d) rec->where[d] = (ed->pos[d] + other->pos[d]) / 2.;
+            or_fix_position(rec->where, s->box, s->pbc);
+            rec->educts[0] = ed->id;
+            rec->educts[1] = other->id;
+            rec->types_from[0] = (uint32_t)ed->type;
+            rec->types_from[1] = (uint32_t)other->type;
+        }
+        ed->type = r->products[0];
+        ed->id = base;
+        break;
+    }
+    }
+    s->counts[ev->rx]++;
+    return 1;
+}
+
+int or_detailed_balance(or_system *s, double dt, uint32_t step, uint64_t *n_performed) {
+    memset(s->counts, 0, sizeof s->counts);
+    s->n_recs = 0;
+    if (n_performed) *n_performed = 0;
+    if (!s->rev_ready && rev_build(s)) return -1;
+    ev_ud u;
+    memset(&u, 0, sizeof u);
+    u.dt = dt;
+    u.step = step;
+    u.draw = 1;
+    gather(s, &u);
+    qsort(u.ev, u.n, sizeof(event_t), cmp_gkey);
+    uint64_t done = 0;
+    for (size_t i = 0; i < u.n; ++i) {
+        event_t *ev = &u.ev[i];
+        if (ev->blocked) continue;
+        done += (uint64_t)db_event(s, ev, step);
+        for (size_t j = i + 1; j < u.n; ++j)
+            if (shares(&u.ev[j], ev)) u.ev[j].blocked = 1;
+    }
     free(u.ev);
     if (n_performed) *n_performed = done;
     return 0;

@@ -140,6 +140,19 @@ int or_perform_events_abstract(int n, const int64_t *p1, const int64_t *p2, cons
                                const double *accept_p, int mode, uint64_t seed, uint32_t trial,
                                uint8_t *performed);
 
+/* ---- DetailedBalance (kernels/singlecpu/src/actions/SCPUReactionImpls.cpp:407-813) ----
+ * or_reversible_configs: the reversible pairs (Actions.cpp:432-519) and their ReversibleReactionConfig
+ * (Actions.cpp:67-367), total volume = box volume, one row of OR_REV_ROW doubles each:
+ * {kind, forward, backward, lhs0, lhs1, rhs0, rhs1, reaction_radius, lhs_radius, rhs_radius, lhs_volume,
+ *  rhs_volume, eff_lhs_interaction, eff_rhs_interaction, eff_lhs_reaction, eff_rhs_reaction,
+ *  acceptance_prefactor, total_volume, equilibrium_constant, macro_forward, macro_backward}
+ * (order-1 sides list the species and 0).  or_fission_table: radii and cumulative probabilities of pair
+ * k (FusionFission), returns their count.  or_detailed_balance: one handler call. */
+#define OR_REV_ROW 21
+int or_reversible_configs(or_system *s, size_t cap, double *rows, size_t *n);
+int or_fission_table(or_system *s, int k, double *radii, double *cdf);
+int or_detailed_balance(or_system *s, double dt, uint32_t step, uint64_t *n_performed);
+
 const char *or_last_error(void);
 
 /* ---- spatial topology reactions of the topology-particle enzymatic kind

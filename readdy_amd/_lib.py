@@ -22,6 +22,7 @@ RX_DECAY = 4
 HANDLER_UA = 0
 HANDLER_GILLESPIE = 1
 HANDLER_NONE = 2
+HANDLER_DETAILED_BALANCE = 3
 
 
 class PairPotential(ctypes.Structure):
@@ -60,6 +61,20 @@ class ReplicaDesc(ctypes.Structure):
                 ("max_particles", ctypes.c_uint64)]
 
 
+class ReversibleReaction(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("forward", ctypes.c_uint32), ("backward", ctypes.c_uint32),
+                ("n_lhs", ctypes.c_uint32), ("n_rhs", ctypes.c_uint32), ("lhs", ctypes.c_uint32 * 2),
+                ("rhs", ctypes.c_uint32 * 2), ("reaction_radius", ctypes.c_double),
+                ("lhs_interaction_radius", ctypes.c_double), ("rhs_interaction_radius", ctypes.c_double),
+                ("lhs_interaction_volume", ctypes.c_double), ("rhs_interaction_volume", ctypes.c_double),
+                ("eff_lhs_interaction_volume", ctypes.c_double), ("eff_rhs_interaction_volume", ctypes.c_double),
+                ("eff_lhs_reaction_volume", ctypes.c_double), ("eff_rhs_reaction_volume", ctypes.c_double),
+                ("acceptance_prefactor", ctypes.c_double), ("total_volume", ctypes.c_double),
+                ("kbt", ctypes.c_double), ("equilibrium_constant", ctypes.c_double),
+                ("macro_forward_rate", ctypes.c_double), ("macro_backward_rate", ctypes.c_double),
+                ("micro_forward_rate", ctypes.c_double), ("micro_backward_rate", ctypes.c_double)]
+
+
 # every symbol the header declares, with (restype, argtypes)
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -93,6 +108,8 @@ SIGNATURES = {
     "rdy_get_virial": (_int, [_vp, _u32, _dp]),
     "rdy_integrate": (_int, [_vp, _d, _dp, _u64]),
     "rdy_react": (_int, [_vp, _int, _d]),
+    "rdy_reversible_reactions": (_int, [ctypes.POINTER(ContextDesc), _d, _u64, ctypes.POINTER(ReversibleReaction),
+                                        _u64p]),
     "rdy_get_reaction_counts": (_int, [_vp, _u32, _u64p]),
     "rdy_get_reaction_totals": (_int, [_vp, _u32, _u64p]),
     "rdy_record_reactions": (_int, [_vp, _int]),

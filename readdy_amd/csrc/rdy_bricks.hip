@@ -1040,8 +1040,10 @@ __global__ __launch_bounds__(BRICK_THREADS) void k_brick_search(Grid cur, Grid o
                        (uint32_t)(B.o[2] + lz)] = 0u;
         }
     }
-    // a launch without reactions (handler > 1) covers every replica; else a replica's own choice wins
-    if (handler <= 1 && R.handler >= 0) handler = R.handler;
+    // a launch without reactions (handler 2) covers every replica; DetailedBalance searches every replica
+    // with the Gillespie keys; else a replica's own choice wins
+    if (handler == HANDLER_DETAILED_BALANCE) handler = 1;
+    else if (handler <= 1 && R.handler >= 0) handler = R.handler;
     if (handler > 1 || !R.in_use || !R.active) return;
     const uint32_t nt = T.n_types;
     if (!T.has_o2 && T.o1idx[nt] == 0) return;  // no reactions

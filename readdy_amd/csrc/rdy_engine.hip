@@ -162,6 +162,13 @@ struct rdy_engine {
     PairIdx *d_pidx = nullptr;
     RxDev *d_rx = nullptr;
     uint32_t *d_o1idx = nullptr, *d_o1list = nullptr, *d_o2list = nullptr;
+    // DetailedBalance tables, built on the handler's first call (they do not depend on the time step)
+    DbPot *d_dbpots = nullptr;
+    uint32_t *d_dbpidx = nullptr, *d_dbtable = nullptr;
+    int32_t *d_dbkind = nullptr;
+    double *d_dbpref = nullptr, *d_dbradii = nullptr, *d_dbcdf = nullptr;
+    bool db_ready = false;
+    DbTables db{};
     RxPairIdx *d_o2idx = nullptr;
     float2 *d_rowmax = nullptr;
     double *d_D = nullptr;
@@ -1158,7 +1165,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1576,14 +1583,190 @@ int rdy_integrate(rdy_engine *e, double dt, const double *noise, uint64_t noise_
     return check_errors(e);
 }
 
+// the reactions and potentials of a context as the reversible-pair search reads them (weights normalised
+// as Fusion.h:61-68 and Fission.h:63-70 do)
+static void rev_inputs(const rdy_reaction *rx, uint32_t n_rx, const rdy_pair_potential *pots, uint32_t n_pots,
+                       std::vector<rdy_rev::Reaction> &R, std::vector<rdy_rev::PairPotential> &P) {
+    R.clear();
+    P.clear();
+    for (uint32_t i = 0; i < n_rx; ++i) {
+        const rdy_reaction &x = rx[i];
+        rdy_rev::Reaction r{};
+        r.kind = (int)x.kind;
+        for (int j = 0; j < 2; ++j) {
+            r.educts[j] = j < r.n_educts() ? x.educts[j] : 0u;
+            r.products[j] = j < r.n_products() ? x.products[j] : 0u;
+        }
+        r.rate = x.rate;
+        r.educt_distance = x.educt_distance;
+        r.product_distance = x.product_distance;
+        double w1 = x.weight1, w2 = x.weight2;
+        if (x.kind == RDY_RX_FUSION || x.kind == RDY_RX_FISSION) {
+            const double sum = w1 + w2;
+            if (sum != 1) {
+                w1 /= sum;
+                w2 /= sum;
+            }
+        }
+        r.weight1 = w1;
+        r.weight2 = w2;
+        R.push_back(r);
+    }
+    for (uint32_t i = 0; i < n_pots; ++i) {
+        rdy_rev::PairPotential q{};
+        q.kind = (int)pots[i].kind;
+        q.type1 = pots[i].type1;
+        q.type2 = pots[i].type2;
+        std::memcpy(q.params, pots[i].params, sizeof q.params);
+        P.push_back(q);
+    }
+}
+
+int rdy_reversible_reactions(const rdy_context_desc *ctx, double total_volume, uint64_t cap,
+                             rdy_reversible_reaction *out, uint64_t *n) {
+    if (!ctx || !n) return set_err(RDY_ERR_INVALID, "null argument");
+    if ((ctx->n_reactions && !ctx->reactions) || (ctx->n_potentials && !ctx->potentials))
+        return set_err(RDY_ERR_INVALID, "null reaction or potential list");
+    *n = 0;
+    std::vector<rdy_rev::Reaction> R;
+    std::vector<rdy_rev::PairPotential> P;
+    rev_inputs(ctx->reactions, ctx->n_reactions, ctx->potentials, ctx->n_potentials, R, P);
+    std::vector<rdy_rev::Config> cfg;
+    try {
+        cfg = rdy_rev::search(R, P, ctx->kbt, total_volume);
+    } catch (const std::exception &x) {
+        return set_err(RDY_ERR_INVALID, x.what());
+    }
+    for (size_t i = 0; i < cfg.size() && i < cap && out; ++i) {
+        const rdy_rev::Config &c = cfg[i];
+        rdy_reversible_reaction &o = out[i];
+        o.kind = (int32_t)c.kind;
+        o.forward = c.forward;
+        o.backward = c.backward;
+        o.n_lhs = (uint32_t)c.n_lhs;
+        o.n_rhs = (uint32_t)c.n_rhs;
+        for (int j = 0; j < 2; ++j) {
+            o.lhs[j] = c.lhs[j];
+            o.rhs[j] = c.rhs[j];
+        }
+        o.reaction_radius = c.reaction_radius;
+        o.lhs_interaction_radius = c.lhs_radius;
+        o.rhs_interaction_radius = c.rhs_radius;
+        o.lhs_interaction_volume = c.lhs_volume;
+        o.rhs_interaction_volume = c.rhs_volume;
+        o.eff_lhs_interaction_volume = c.eff_lhs_interaction;
+        o.eff_rhs_interaction_volume = c.eff_rhs_interaction;
+        o.eff_lhs_reaction_volume = c.eff_lhs_reaction;
+        o.eff_rhs_reaction_volume = c.eff_rhs_reaction;
+        o.acceptance_prefactor = c.acceptance_prefactor;
+        o.total_volume = c.total_volume;
+        o.kbt = c.kbt;
+        o.equilibrium_constant = c.equilibrium_constant;
+        o.macro_forward_rate = c.macro_forward;
+        o.macro_backward_rate = c.macro_backward;
+        o.micro_forward_rate = c.micro_forward;
+        o.micro_backward_rate = c.micro_backward;
+    }
+    *n = cfg.size();
+    return RDY_OK;
+}
+
+// DetailedBalance device tables: FP64 pair potentials per ordered type pair, and per reaction its
+// reversible pair's kind, Metropolis prefactor and fission-distance table
+static int ensure_db_tables(rdy_engine *e) {
+    if (e->db_ready) return RDY_OK;
+    std::vector<rdy_rev::Reaction> R;
+    std::vector<rdy_rev::PairPotential> P;
+    rev_inputs(e->rx_in.data(), e->n_rx, e->pots_in.data(), (uint32_t)e->pots_in.size(), R, P);
+    std::vector<rdy_rev::Config> cfg;
+    try {
+        cfg = rdy_rev::search(R, P, e->kbt, 0.0);  // the volume enters only the inferred macro rates
+    } catch (const std::exception &x) {
+        return set_err(RDY_ERR_INVALID, x.what());
+    }
+    const uint32_t nt = e->n_types;
+    std::vector<std::vector<uint32_t>> lists((size_t)nt * nt);
+    for (uint32_t k = 0; k < (uint32_t)P.size(); ++k) {
+        lists[(size_t)P[k].type1 * nt + P[k].type2].push_back(k);
+        if (P[k].type1 != P[k].type2) lists[(size_t)P[k].type2 * nt + P[k].type1].push_back(k);
+    }
+    std::vector<DbPot> pots;
+    std::vector<uint32_t> pidx((size_t)nt * nt);
+    for (size_t k = 0; k < lists.size(); ++k) {
+        pidx[k] = (uint32_t)pots.size() | ((uint32_t)lists[k].size() << 16);
+        for (uint32_t id : lists[k]) {
+            DbPot d{};
+            const double c = rdy_rev::cutoff_of(P[id]);
+            d.cut2 = c * c;
+            std::memcpy(d.q, P[id].params, sizeof d.q);
+            d.kind = P[id].kind;
+            pots.push_back(d);
+        }
+    }
+    if (pots.size() > 65535) return set_err(RDY_ERR_INVALID, "too many pair potentials");
+    std::vector<int32_t> kind(std::max<uint32_t>(e->n_rx, 1), -1);
+    std::vector<double> pref(kind.size(), 1.0), radii, cdf;
+    std::vector<uint32_t> table(kind.size(), 0u);
+    for (const rdy_rev::Config &c : cfg) {
+        uint32_t t0 = 0;
+        if (c.kind == rdy_rev::FUSION_FISSION) {
+            t0 = (uint32_t)radii.size();
+            radii.insert(radii.end(), c.fission_radii.begin(), c.fission_radii.end());
+            cdf.insert(cdf.end(), c.fission_cdf.begin(), c.fission_cdf.end());
+        }
+        for (uint32_t rx : {c.forward, c.backward}) {
+            kind[rx] = (int32_t)c.kind;
+            table[rx] = t0;
+            // SCPUReactionImpls.cpp:482-489: the prefactor for educts == lhs types, in order; the
+            // backward branch repeats the same test and is never taken
+            if (c.kind == rdy_rev::ENZYMATIC_ENZYMATIC && R[rx].educts[0] == c.lhs[0] && R[rx].educts[1] == c.lhs[1])
+                pref[rx] = c.acceptance_prefactor;
+        }
+    }
+    if (radii.empty()) {
+        radii.push_back(0.0);
+        cdf.push_back(1.0);
+    }
+    auto up = [&](auto **dst, const auto &v) -> int {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+        if (*dst) HIP_TRY(hipFree(*dst));
+        HIP_TRY(dalloc(dst, std::max<size_t>(v.size(), 1)));
+        if (!v.empty()) HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return RDY_OK;
+    };
+    int rc;
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if ((rc = up(&e->d_dbpots, pots))) return rc;
+    if ((rc = up(&e->d_dbpidx, pidx))) return rc;
+    if ((rc = up(&e->d_dbkind, kind))) return rc;
+    if ((rc = up(&e->d_dbpref, pref))) return rc;
+    if ((rc = up(&e->d_dbtable, table))) return rc;
+    if ((rc = up(&e->d_dbradii, radii))) return rc;
+    if ((rc = up(&e->d_dbcdf, cdf))) return rc;
+    e->db.pots = e->d_dbpots;
+    e->db.pidx = e->d_dbpidx;
+    e->db.kind = e->d_dbkind;
+    e->db.prefactor = e->d_dbpref;
+    e->db.table = e->d_dbtable;
+    e->db.radii = e->d_dbradii;
+    e->db.cdf = e->d_dbcdf;
+    e->db.n_table = (uint32_t)(rdy_rev::FISSION_POINTS + 1);
+    e->db.radius = e->radius;
+    e->db_ready = true;
+    return RDY_OK;
+}
+
 int rdy_react(rdy_engine *e, int handler, double dt) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
     if (int rb = ensure_bricks(e)) return rb;
-    if (handler != RDY_HANDLER_UNCONTROLLED_APPROXIMATION && handler != RDY_HANDLER_GILLESPIE)
+    if (handler != RDY_HANDLER_UNCONTROLLED_APPROXIMATION && handler != RDY_HANDLER_GILLESPIE &&
+        handler != RDY_HANDLER_DETAILED_BALANCE)
         return set_err(RDY_ERR_INVALID, "unknown reaction handler");
     int rc = upload_tables(e, dt);
     if (rc) return rc;
     if (e->n_rx == 0) return RDY_OK;
+    const bool db = handler == RDY_HANDLER_DETAILED_BALANCE;
+    if (db && (rc = ensure_db_tables(e))) return rc;
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
     const int c = e->cur;
     for_brick_groups(e, [&](auto ah, uint32_t nb, uint32_t b0) {
@@ -1595,10 +1778,15 @@ int rdy_react(rdy_engine *e, int handler, double dt) {
     hipLaunchKernelGGL(k_brick_search_slow, dim3(slow_blocks(e)), dim3(BRICK_THREADS), 0, e->stream, e->grid(c),
                        e->reps, e->T, (const BrickDev *)e->d_bricks, e->cap, e->radius, handler, e->events,
                        e->ev_count, slow_list(e, 1), e->err);
-    hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T, e->cap,
-                       e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles,
-                       (Edit *)nullptr, (uint32_t *)nullptr, 0u, rec_ptr(e), e->recording ? e->rec_count : nullptr,
-                       e->err);
+    if (db)
+        hipLaunchKernelGGL(k_detailed_balance, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps,
+                           e->T, e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, (uint32_t)e->max_particles,
+                           rec_ptr(e), e->recording ? e->rec_count : nullptr, e->err, e->db);
+    else
+        hipLaunchKernelGGL(k_resolve, dim3(nr), dim3(RESOLVE_THREADS), 0, e->stream, e->grid(c), e->reps, e->T,
+                           e->cap, e->events, e->ev_count, e->rx_step, e->rx_total, 0, (uint32_t)e->max_particles,
+                           (Edit *)nullptr, (uint32_t *)nullptr, 0u, rec_ptr(e),
+                           e->recording ? e->rec_count : nullptr, e->err);
     return check_errors(e);
 }
 

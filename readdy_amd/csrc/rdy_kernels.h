@@ -80,6 +80,28 @@ struct RxRecord {
 };
 static_assert(sizeof(RxRecord) == 64, "RxRecord layout");
 
+// Tables of the DetailedBalance handler.  The pair potentials again, with their FP64 parameters (the
+// rdy_pair_potential parameters and the squared cutoff), listed per ordered type pair as for the
+// force tables; per reaction the kind of its reversible pair (-1: none), its Metropolis prefactor and,
+// for a FusionFission pair, the first entry of the pair's fission-distance table (radii and
+// cumulative probabilities, n_table entries).
+struct DbPot {
+    double cut2;
+    double q[8];
+    int32_t kind, pad;
+};
+
+struct DbTables {
+    const DbPot *pots;
+    const uint32_t *pidx;   // [n_types * n_types]: start | count << 16
+    const int32_t *kind;    // [n_rx]
+    const double *prefactor;
+    const uint32_t *table;
+    const double *radii, *cdf;
+    uint32_t n_table;
+    int32_t radius;         // cell stencil radius of the grid
+};
+
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
              ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, ERR_FORCE_RANGE = 8, N_ERR = 9 };
@@ -87,6 +109,8 @@ enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, 
 constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
 constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer
 constexpr int MODE_REBIN = 2;   // re-bin only (neighbour-list update)
+
+constexpr int HANDLER_DETAILED_BALANCE = 3;  // RDY_HANDLER_DETAILED_BALANCE: Gillespie search, sequential resolve
 
 constexpr int RESOLVE_THREADS = 256;
 constexpr int RESOLVE_MAX_EVENTS = 1024;  // accepted events per replica per step (LDS resident)

@@ -1,5 +1,6 @@
 // Kernels of the MI355X ReaDDy engine (see rdy_kernels.h for the data layout).
 #include "rdy_kernels.h"
+#include "rdy_reversible.h"
 
 namespace rdy {
 
@@ -625,7 +626,8 @@ __device__ __forceinline__ void search_one(const Grid &cur, ReplicaDev *reps, co
     const uint32_t t = tw_type(me.tw);
     const uint32_t step = R.step;
     const uint64_t seed = R.seed;
-    if (handler <= 1 && R.handler >= 0) handler = R.handler;  // the replica's own handler
+    if (handler == HANDLER_DETAILED_BALANCE) handler = 1;  // Gillespie keys for every replica
+    else if (handler <= 1 && R.handler >= 0) handler = R.handler;  // the replica's own handler
 
     auto emit = [&](uint32_t rx, uint32_t s1, uint32_t s2, uint32_t h1, uint32_t h2, unsigned long long key,
                     uint32_t ne) {
@@ -755,12 +757,232 @@ __device__ __forceinline__ int axis_elem(int k, int i, int D, int p, int r) {
     return max(0, i - r) + k;
 }
 
-// the reactions of replica r = blockIdx.x (one workgroup of RESOLVE_THREADS)
+// -------------------------------------------------------------------------------------------------
+// DetailedBalance (SCPUReactionImpls.cpp:407-555): the events the Gillespie selection keeps are
+// performed one after another in key order, each on the state the previous ones left.  An event
+// outside every reversible pair is performed as the Gillespie handler performs it.  An event of a
+// reversible pair is a Metropolis move: its products are proposed (a fission at a distance drawn
+// from the pair's fission-distance table), the workgroup sums the change of the pair-potential
+// energy, and the move is accepted with probability
+//   min(1, prefactor exp(-(dE - dU_int) / kBT))
+// where dU_int is the educts' (fusion, negated) or products' (fission) mutual interaction energy for
+// FusionFission and zero otherwise, and prefactor is the acceptance prefactor for the forward
+// reaction of an EnzymaticEnzymatic pair and one otherwise (the reference tests the forward educt
+// order twice, so the backward reaction gets one).  A rejected move puts the educts back as new
+// particles: their positions and types are kept and they get the ids after the products' ids.
+// Energies are the FP64 pair potentials gated by d^2 < cutoff^2, between particles inside the box.
+// Random numbers: the fission direction on stream 3 (as every fission), the fission distance on
+// stream 10, the Metropolis uniform on stream 11 | reaction << 8 keyed by the educts' handles.
+// -------------------------------------------------------------------------------------------------
+template <class Perform>
+__device__ __forceinline__ void detailed_balance_events(const Grid &cur, const ReplicaDev &R, const Tables &T,
+                                                        const DbTables &D, uint32_t cap, uint32_t ne,
+                                                        const uint8_t *status, const uint32_t *sidx,
+                                                        const uint32_t *prefix, const Event *ev, uint32_t hbase,
+                                                        const RepGeom &geo, RxRecord *rrec, double *red,
+                                                        Perform &perform) {
+    const int tid = threadIdx.x;
+    const uint32_t nt = T.n_types;
+    auto pair_u = [&](uint32_t ta, uint32_t tb, double d2) -> double {
+        const uint32_t pi = D.pidx[(size_t)ta * nt + tb];
+        double u = 0.0;
+        for (uint32_t p = pi & 0xFFFFu, pe = (pi & 0xFFFFu) + (pi >> 16); p < pe; ++p) {
+            const DbPot &P = D.pots[p];
+            if (d2 < P.cut2) u += rdy_rev::pair_energy_fp64(P.kind, P.q, sqrt(d2));
+        }
+        return u;
+    };
+    auto inside = [&](const double *q) {
+        return in_box_axis(q[0], geo.box[0]) && in_box_axis(q[1], geo.box[1]) && in_box_axis(q[2], geo.box[2]);
+    };
+    const int rad = D.radius;
+    const uint32_t side = 2u * (uint32_t)rad + 1u, S = side * side * side;  // stencil cells per center
+    for (uint32_t k = 0; k < ne; ++k) {
+        if (status[k] != 1) {
+            if (rrec && tid == 0) rrec[k].valid = 0u;
+            continue;
+        }
+        const Event e = ev[sidx[k]];
+        const RxDev X = T.rx[e.rx];
+        const int32_t rk = D.kind[e.rx];
+        if (rk < 0) {
+            if (tid == 0) perform(k, e, X.product_distance);
+            __syncthreads();
+            continue;
+        }
+        const State a = cur.bins[e.s1], b = cur.bins[e.s2];
+        // centers 0, 1: educts that leave; 2, 3: proposed products
+        double cp[4][3];
+        uint32_t ct[4];
+        bool cv[4] = {false, false, false, false};
+        uint32_t ex0 = e.s1, ex1 = e.s1, educt_slot = e.s1;
+        double fdist = X.product_distance, du_int = 0.0;
+        auto set = [&](int c, double x, double y, double z, uint32_t t, bool in) {
+            cp[c][0] = x;
+            cp[c][1] = y;
+            cp[c][2] = z;
+            ct[c] = t;
+            cv[c] = in;
+        };
+        switch (X.kind) {
+        case RX_FISSION: {  // SCPUReactionImpls.cpp:600-633
+            uint32_t x4[4];
+            double nv[3];
+            philox(a.h, R.step, 3u, 0u, R.seed, x4);
+            normals3(x4, nv);
+            const double nrm = sqrt(dist2(nv[0], nv[1], nv[2]));
+            for (int d = 0; d < 3; ++d) nv[d] /= nrm;
+            philox(a.h, R.step, 10u, 0u, R.seed, x4);
+            const double u = u53(x4[0], x4[1]);
+            const double *cdf = D.cdf + D.table[e.rx];
+            uint32_t lo = 0, hi = D.n_table;  // lower_bound (DetailedBalance.h:133-138)
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (cdf[mid] < u) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo >= D.n_table) lo = D.n_table - 1;
+            fdist = D.radii[D.table[e.rx] + lo];
+            du_int = pair_u(X.products[0], X.products[1], fdist * fdist);
+            set(0, a.x, a.y, a.z, tw_type(a.tw), !(a.tw & TW_OUTSIDE));
+            double q2[3], q1[3];
+            const double p[3] = {a.x, a.y, a.z};
+            for (int d = 0; d < 3; ++d) {
+                q2[d] = wrap(p[d] - X.w2 * fdist * nv[d], geo.box[d], geo.pbc[d]);
+                q1[d] = wrap(p[d] + X.w1 * fdist * nv[d], geo.box[d], geo.pbc[d]);
+            }
+            set(2, q2[0], q2[1], q2[2], X.products[1], inside(q2));
+            set(3, q1[0], q1[1], q1[2], X.products[0], inside(q1));
+            break;
+        }
+        case RX_FUSION: {  // SCPUReactionImpls.cpp:634-681
+            const double dx = min_image(b.x - a.x, geo.box[0], geo.pbc[0]);
+            const double dy = min_image(b.y - a.y, geo.box[1], geo.pbc[1]);
+            const double dz = min_image(b.z - a.z, geo.box[2], geo.pbc[2]);
+            du_int = -pair_u(X.educts[0], X.educts[1], dist2(dx, dy, dz));
+            const double w = (X.educts[0] == tw_type(a.tw)) ? X.w1 : X.w2;
+            double q[3] = {wrap(a.x + w * dx, geo.box[0], geo.pbc[0]), wrap(a.y + w * dy, geo.box[1], geo.pbc[1]),
+                           wrap(a.z + w * dz, geo.box[2], geo.pbc[2])};
+            set(0, a.x, a.y, a.z, tw_type(a.tw), !(a.tw & TW_OUTSIDE));
+            set(1, b.x, b.y, b.z, tw_type(b.tw), !(b.tw & TW_OUTSIDE));
+            set(2, q[0], q[1], q[2], X.products[0], inside(q));
+            ex1 = e.s2;
+            break;
+        }
+        case RX_ENZYMATIC: {  // SCPUReactionImpls.cpp:703-735: the educt that is not the catalyst converts
+            const bool a_cat = tw_type(a.tw) == X.educts[1];
+            const State &c = a_cat ? b : a;
+            educt_slot = ex0 = ex1 = a_cat ? e.s2 : e.s1;
+            set(0, c.x, c.y, c.z, tw_type(c.tw), !(c.tw & TW_OUTSIDE));
+            set(2, c.x, c.y, c.z, X.products[0], !(c.tw & TW_OUTSIDE));
+            break;
+        }
+        default: {  // RX_CONVERSION, SCPUReactionImpls.cpp:683-702
+            set(0, a.x, a.y, a.z, tw_type(a.tw), !(a.tw & TW_OUTSIDE));
+            set(2, a.x, a.y, a.z, X.products[0], !(a.tw & TW_OUTSIDE));
+            break;
+        }
+        }
+        // energy change: each center against every other particle inside the box within the pair cutoffs
+        // (the leaving educts excluded), plus the educts' and the products' mutual pair once
+        double acc = 0.0;
+        for (uint32_t w = tid; w < 4u * S; w += blockDim.x) {
+            const uint32_t c = w / S, j = w - c * S;
+            if (!cv[c]) continue;
+            int ci[3], na[3];
+            for (int d = 0; d < 3; ++d) {
+                ci[d] = cell_coord(cp[c][d], geo.box[d], geo.cs[d], geo.dims[d]);
+                na[d] = axis_count(ci[d], geo.dims[d], geo.pbc[d], rad);
+            }
+            if (j >= (uint32_t)(na[0] * na[1] * na[2])) continue;
+            const int jz = (int)(j % (uint32_t)na[2]), jy = (int)((j / (uint32_t)na[2]) % (uint32_t)na[1]),
+                      jx = (int)(j / ((uint32_t)na[2] * (uint32_t)na[1]));
+            const uint32_t cell =
+                geo.cell_base +
+                ((uint32_t)axis_elem(jx, ci[0], geo.dims[0], geo.pbc[0], rad) * (uint32_t)geo.dims[1] +
+                 (uint32_t)axis_elem(jy, ci[1], geo.dims[1], geo.pbc[1], rad)) *
+                    (uint32_t)geo.dims[2] +
+                (uint32_t)axis_elem(jz, ci[2], geo.dims[2], geo.pbc[2], rad);
+            const double sign = c < 2 ? -1.0 : 1.0;
+            auto visit = [&](uint32_t slot) {
+                if (slot == ex0 || slot == ex1) return;
+                const State o = cur.bins[slot];
+                if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
+                const double dx = min_image(o.x - cp[c][0], geo.box[0], geo.pbc[0]);
+                const double dy = min_image(o.y - cp[c][1], geo.box[1], geo.pbc[1]);
+                const double dz = min_image(o.z - cp[c][2], geo.box[2], geo.pbc[2]);
+                const double u = pair_u(ct[c], tw_type(o.tw), dist2(dx, dy, dz));
+                if (u != 0.0) acc += sign * u;
+            };
+            const uint32_t n = cur.counts[cell], m = n < cap ? n : cap;
+            for (uint32_t q = 0; q < m; ++q) visit(cell * cap + q);
+            if (n > cap) {
+                uint32_t no = *cur.ovf_count;
+                no = no < cur.ovf_cap ? no : cur.ovf_cap;
+                for (uint32_t o = 0; o < no; ++o)
+                    if (cur.ovf_cell[o] == cell) visit(cur.nbin + o);
+            }
+        }
+        if (tid == 0) {
+            for (int g = 0; g < 4; g += 2)
+                if (cv[g] && cv[g + 1]) {
+                    const double dx = min_image(cp[g + 1][0] - cp[g][0], geo.box[0], geo.pbc[0]);
+                    const double dy = min_image(cp[g + 1][1] - cp[g][1], geo.box[1], geo.pbc[1]);
+                    const double dz = min_image(cp[g + 1][2] - cp[g][2], geo.box[2], geo.pbc[2]);
+                    acc += (g == 0 ? -1.0 : 1.0) * pair_u(ct[g], ct[g + 1], dist2(dx, dy, dz));
+                }
+        }
+        red[tid] = acc;
+        __syncthreads();
+        for (uint32_t h = blockDim.x >> 1; h > 0; h >>= 1) {
+            if ((uint32_t)tid < h) red[tid] += red[tid + h];
+            __syncthreads();
+        }
+        const double dE = red[0];
+        const double boltzmann = exp(-1.0 / T.kbt * (dE - du_int));
+        const double acceptance = fmin(1.0, D.prefactor[e.rx] * boltzmann);
+        uint32_t x4[4];
+        const uint32_t lo = e.h1 < e.h2 ? e.h1 : e.h2, hi = e.h1 < e.h2 ? e.h2 : e.h1;
+        philox(lo, R.step, 11u | (e.rx << 8), hi, R.seed, x4);
+        const bool accept = u53(x4[0], x4[1]) < acceptance;
+        __syncthreads();  // red[0] read by every thread before the next event writes red
+        if (tid == 0) {
+            if (accept) {
+                perform(k, e, fdist);
+                if (rrec) {  // the reversible path fills no products (SCPUReactionImpls.cpp:579-742)
+                    RxRecord &rc = rrec[k];
+                    rc.products[0] = rc.products[1] = 0u;
+                    if (X.kind == RX_ENZYMATIC && educt_slot == e.s2) {  // educt first, then the catalyst
+                        rc.educts[0] = b.h;
+                        rc.educts[1] = a.h;
+                        rc.types_from[0] = tw_type(b.tw);
+                        rc.types_from[1] = tw_type(a.tw);
+                    }
+                }
+            } else {
+                if (rrec) rrec[k].valid = 0u;
+                const uint32_t h0 = hbase + prefix[k] + (X.kind == RX_FISSION ? 2u : 1u);
+                if (X.kind == RX_FUSION) {
+                    cur.bins[e.s1].h = h0;
+                    cur.bins[e.s2].h = h0 + 1u;
+                } else {
+                    cur.bins[educt_slot].h = h0;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// the reactions of replica r = blockIdx.x (one workgroup of RESOLVE_THREADS).  DB: the DetailedBalance
+// handler (tables D), which performs the selected events one after another.
+template <bool DB = false>
 __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *reps, const Tables &T, uint32_t cap,
                                                 Event *events, uint32_t *ev_count, unsigned long long *rx_step,
                                                 unsigned long long *rx_total, int require_active, uint32_t max_wl,
                                                 Edit *edits, uint32_t *edit_count, uint32_t edit_cap,
-                                                RxRecord *recs, uint32_t *rec_count, uint32_t *err) {
+                                                RxRecord *recs, uint32_t *rec_count, uint32_t *err,
+                                                const DbTables D = DbTables{}) {
     constexpr int E = RESOLVE_MAX_EVENTS;
     constexpr int HT = 4 * E;  // hash table entries (power of two, > 2 slots per event)
     __shared__ unsigned long long skey[E];
@@ -913,8 +1135,12 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
     for (uint32_t k = tid; k < ne; k += blockDim.x) {
         uint32_t c = 0;
         if (status[k] == 1) {
-            const int kind = T.rx[ev[sidx[k]].rx].kind;
+            const uint32_t rx = ev[sidx[k]].rx;
+            const int kind = T.rx[rx].kind;
             c = (kind == RX_FISSION) ? 2u : (kind == RX_DECAY ? 0u : 1u);
+            // a reversible event takes ids for its products and then for the educts' backup particles,
+            // whether or not it is accepted (SCPUReactionImpls.cpp:447-461, 557-577)
+            if (DB && D.kind[rx] >= 0) c = (kind == RX_FISSION || kind == RX_FUSION) ? 3u : 2u;
         }
         prefix[k] = c;
     }
@@ -945,12 +1171,8 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
         else atomicAdd(&err[ERR_EDIT_OVERFLOW], 1u);
     };
     RxRecord *rrec = recs ? recs + (size_t)r * E : nullptr;
-    for (uint32_t k = tid; k < ne; k += blockDim.x) {
-        if (status[k] != 1) {
-            if (rrec) rrec[k].valid = 0u;
-            continue;
-        }
-        const Event e = ev[sidx[k]];
+    // performs sorted event k (fission products fdist apart)
+    auto perform = [&](uint32_t k, const Event &e, double fdist) {
         const RxDev X = T.rx[e.rx];
         atomicAdd(&rxcount[e.rx & 255u], 1u);
         State a = cur.bins[e.s1];
@@ -1024,8 +1246,8 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
             State p2, p1;
             double q2[3], q1[3];
             for (int d = 0; d < 3; ++d) {
-                q2[d] = wrap(p[d] - X.w2 * X.product_distance * nv[d], geo.box[d], geo.pbc[d]);
-                q1[d] = wrap(p[d] + X.w1 * X.product_distance * nv[d], geo.box[d], geo.pbc[d]);
+                q2[d] = wrap(p[d] - X.w2 * fdist * nv[d], geo.box[d], geo.pbc[d]);
+                q1[d] = wrap(p[d] + X.w1 * fdist * nv[d], geo.box[d], geo.pbc[d]);
             }
             p2.x = q2[0]; p2.y = q2[1]; p2.z = q2[2];
             p2.tw = make_tw(X.products[1], tw_new, r);
@@ -1059,6 +1281,19 @@ __device__ __forceinline__ void resolve_replica(const Grid &cur, ReplicaDev *rep
             break;
         }
         }
+    };
+    if constexpr (!DB) {
+        for (uint32_t k = tid; k < ne; k += blockDim.x) {
+            if (status[k] != 1) {
+                if (rrec) rrec[k].valid = 0u;
+                continue;
+            }
+            const Event e = ev[sidx[k]];
+            perform(k, e, T.rx[e.rx].product_distance);
+        }
+    } else {
+        detailed_balance_events(cur, R, T, D, cap, ne, status, sidx, prefix, ev, hbase, geo, rrec,
+                                reinterpret_cast<double *>(skey), perform);
     }
     __syncthreads();
     if (tid < (int)T.n_rx && tid < 256) {
@@ -1080,6 +1315,18 @@ __global__ __launch_bounds__(RESOLVE_THREADS) void k_resolve(Grid cur, ReplicaDe
                                                              uint32_t *rec_count, uint32_t *err) {
     resolve_replica(cur, reps, T, cap, events, ev_count, rx_step, rx_total, require_active, max_wl, edits, edit_count,
                     edit_cap, recs, rec_count, err);
+}
+
+// SCPUDetailedBalance::perform's event loop, one workgroup per replica (the candidates come from the
+// search kernels with the Gillespie keys)
+__global__ __launch_bounds__(RESOLVE_THREADS) void k_detailed_balance(Grid cur, ReplicaDev *reps, Tables T,
+                                                                      uint32_t cap, Event *events, uint32_t *ev_count,
+                                                                      unsigned long long *rx_step,
+                                                                      unsigned long long *rx_total, uint32_t max_wl,
+                                                                      RxRecord *recs, uint32_t *rec_count,
+                                                                      uint32_t *err, DbTables D) {
+    resolve_replica<true>(cur, reps, T, cap, events, ev_count, rx_step, rx_total, 0, max_wl, (Edit *)nullptr,
+                          (uint32_t *)nullptr, 0u, recs, rec_count, err, D);
 }
 
 // marks every particle of replica r dead in the current buffer

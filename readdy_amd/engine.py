@@ -27,6 +27,66 @@ def bin_capacity_for(particles_per_cell):
     return max(8, (c + 3) // 4 * 4)
 
 
+def _potential_array(potentials):
+    a = (L.PairPotential * max(len(potentials), 1))()
+    for i, (kind, t1, t2, params) in enumerate(potentials):
+        a[i].kind = kind
+        a[i].type1 = t1
+        a[i].type2 = t2
+        for k, v in enumerate(params):
+            a[i].params[k] = float(v)
+    return a
+
+
+def _reaction_array(reactions):
+    a = (L.Reaction * max(len(reactions), 1))()
+    for i, r in enumerate(reactions):
+        x = a[i]
+        x.kind = r["kind"]
+        ed = list(r.get("educts", [])) + [0, 0]
+        pr = list(r.get("products", [])) + [0, 0]
+        x.educts[0], x.educts[1] = ed[0], ed[1]
+        x.products[0], x.products[1] = pr[0], pr[1]
+        x.rate = r["rate"]
+        x.educt_distance = r.get("educt_distance", 0.0)
+        x.product_distance = r.get("product_distance", 0.0)
+        x.weight1 = r.get("weight1", 0.5)
+        x.weight2 = r.get("weight2", 0.5)
+    return a
+
+
+REVERSIBLE_KINDS = {0: "FusionFission", 1: "ConversionConversion", 2: "EnzymaticEnzymatic"}
+
+
+def reversible_reactions(n_types, potentials=(), reactions=(), kbt=1.0, total_volume=1.0):
+    """DetailedBalance's reversible pairs of a chemistry (rdy_reversible_reactions; host computation, no
+    GPU needed): one dict per pair with the ReversibleReactionConfig values
+    (include/readdy/model/actions/DetailedBalance.h:64-139)."""
+    lib = L.load()
+    pots, rx = _potential_array(potentials), _reaction_array(reactions)
+    diff = np.ones(max(int(n_types), 1))
+    ctx = L.ContextDesc()
+    ctx.n_types = int(n_types)
+    ctx.kbt = kbt
+    ctx.diffusion = _ptr(diff, _dp)
+    ctx.topology_flavor = None
+    ctx.n_potentials = len(potentials)
+    ctx.potentials = pots
+    ctx.n_reactions = len(reactions)
+    ctx.reactions = rx
+    n = ctypes.c_uint64()
+    L.check(lib.rdy_reversible_reactions(ctypes.byref(ctx), float(total_volume), 0, None, ctypes.byref(n)))
+    out = (L.ReversibleReaction * max(n.value, 1))()
+    L.check(lib.rdy_reversible_reactions(ctypes.byref(ctx), float(total_volume), n.value, out, ctypes.byref(n)))
+    res = []
+    for c in out[:n.value]:
+        d = {f: getattr(c, f) for f, _ in L.ReversibleReaction._fields_}
+        d["lhs"], d["rhs"] = list(c.lhs), list(c.rhs)
+        d["kind_name"] = REVERSIBLE_KINDS[c.kind]
+        res.append(d)
+    return res
+
+
 class Engine:
     def __init__(self, n_types, diffusion, kbt=1.0, potentials=(), reactions=(), topology_flavor=None,
                  nl_cutoff=0.0, cell_radius=1, bin_capacity=0, max_replicas=1, max_cells=0, max_particles=0,
@@ -35,28 +95,9 @@ class Engine:
         self.n_types = int(n_types)
         self._diff = np.ascontiguousarray(diffusion, dtype=np.float64)
         assert self._diff.size == self.n_types
-        self._pots = (L.PairPotential * max(len(potentials), 1))()
-        for i, (kind, t1, t2, params) in enumerate(potentials):
-            self._pots[i].kind = kind
-            self._pots[i].type1 = t1
-            self._pots[i].type2 = t2
-            for k, v in enumerate(params):
-                self._pots[i].params[k] = float(v)
-        self._rx = (L.Reaction * max(len(reactions), 1))()
-        self.reaction_kinds = []
-        for i, r in enumerate(reactions):
-            x = self._rx[i]
-            x.kind = r["kind"]
-            ed = list(r.get("educts", [])) + [0, 0]
-            pr = list(r.get("products", [])) + [0, 0]
-            x.educts[0], x.educts[1] = ed[0], ed[1]
-            x.products[0], x.products[1] = pr[0], pr[1]
-            x.rate = r["rate"]
-            x.educt_distance = r.get("educt_distance", 0.0)
-            x.product_distance = r.get("product_distance", 0.0)
-            x.weight1 = r.get("weight1", 0.5)
-            x.weight2 = r.get("weight2", 0.5)
-            self.reaction_kinds.append(r["kind"])
+        self._pots = _potential_array(potentials)
+        self._rx = _reaction_array(reactions)
+        self.reaction_kinds = [r["kind"] for r in reactions]
         self.n_reactions = len(reactions)
         self._flavor = None
         if topology_flavor is not None:

@@ -93,6 +93,57 @@ private:
     bool _spatialSet = false, _topoStale = false;
 };
 
+// the context as the engine's C ABI describes it (arrays owned here)
+struct ContextDesc {
+    std::vector<double> D;
+    std::vector<uint8_t> flavor;
+    std::vector<rdy_pair_potential> pots;
+    std::vector<rdy_reaction> rxs;
+    rdy_context_desc cd{};
+    explicit ContextDesc(const Context &c) {
+        const auto &types = c.particleTypes().types();
+        if (types.empty()) throw std::logic_error("the context has no particle types");
+        for (const auto &t : types) {
+            D.push_back(t.diffusionConstant);
+            flavor.push_back(t.topologyFlavor ? 1 : 0);
+        }
+        for (const auto &p : c.potentials().potentialsOrder2()) {
+            rdy_pair_potential q{};
+            q.kind = p.kind;
+            q.type1 = p.type1;
+            q.type2 = p.type2;
+            for (int i = 0; i < 8; ++i) q.params[i] = p.params[i];
+            pots.push_back(q);
+        }
+        for (const auto &r : c.reactions().reactions()) {
+            rdy_reaction q{};
+            switch (r.type) {
+            case ReactionType::Conversion: q.kind = RDY_RX_CONVERSION; break;
+            case ReactionType::Fusion: q.kind = RDY_RX_FUSION; break;
+            case ReactionType::Fission: q.kind = RDY_RX_FISSION; break;
+            case ReactionType::Enzymatic: q.kind = RDY_RX_ENZYMATIC; break;
+            case ReactionType::Decay: q.kind = RDY_RX_DECAY; break;
+            }
+            for (std::size_t i = 0; i < r.educts.size() && i < 2; ++i) q.educts[i] = r.educts[i];
+            for (std::size_t i = 0; i < r.products.size() && i < 2; ++i) q.products[i] = r.products[i];
+            q.rate = r.rate;
+            q.educt_distance = r.eductDistance;
+            q.product_distance = r.productDistance;
+            q.weight1 = r.weight1;
+            q.weight2 = r.weight2;
+            rxs.push_back(q);
+        }
+        cd.n_types = (uint32_t)types.size();
+        cd.kbt = c.kBT();
+        cd.diffusion = D.data();
+        cd.topology_flavor = flavor.data();
+        cd.n_potentials = (uint32_t)pots.size();
+        cd.potentials = pots.data();
+        cd.n_reactions = (uint32_t)rxs.size();
+        cd.reactions = rxs.data();
+    }
+};
+
 class MI355XKernel : public Kernel {
 public:
     MI355XKernel() : Kernel("MI355X"), _state(this), _actions(this) {
@@ -112,6 +163,7 @@ public:
         engine();
         _state.flush();
         if (_state.reactionsChangeBondedTerms()) return false;
+        if (scheduler == "DetailedBalance") return false;  // a separate action (rdy_react only)
         const int h = scheduler == "Gillespie" ? RDY_HANDLER_GILLESPIE
                       : scheduler == "UncontrolledApproximation" ? RDY_HANDLER_UNCONTROLLED_APPROXIMATION
                                                                  : RDY_HANDLER_NONE;
@@ -139,51 +191,8 @@ public:
     rdy_engine *engine() {
         if (_engine) return _engine;
         const Context &c = context();
-        const auto &types = c.particleTypes().types();
-        if (types.empty()) throw std::logic_error("the context has no particle types");
-        std::vector<double> D;
-        std::vector<uint8_t> flavor;
-        for (const auto &t : types) {
-            D.push_back(t.diffusionConstant);
-            flavor.push_back(t.topologyFlavor ? 1 : 0);
-        }
-        std::vector<rdy_pair_potential> pots;
-        for (const auto &p : c.potentials().potentialsOrder2()) {
-            rdy_pair_potential q{};
-            q.kind = p.kind;
-            q.type1 = p.type1;
-            q.type2 = p.type2;
-            for (int i = 0; i < 8; ++i) q.params[i] = p.params[i];
-            pots.push_back(q);
-        }
-        std::vector<rdy_reaction> rxs;
-        for (const auto &r : c.reactions().reactions()) {
-            rdy_reaction q{};
-            switch (r.type) {
-            case ReactionType::Conversion: q.kind = RDY_RX_CONVERSION; break;
-            case ReactionType::Fusion: q.kind = RDY_RX_FUSION; break;
-            case ReactionType::Fission: q.kind = RDY_RX_FISSION; break;
-            case ReactionType::Enzymatic: q.kind = RDY_RX_ENZYMATIC; break;
-            case ReactionType::Decay: q.kind = RDY_RX_DECAY; break;
-            }
-            for (std::size_t i = 0; i < r.educts.size() && i < 2; ++i) q.educts[i] = r.educts[i];
-            for (std::size_t i = 0; i < r.products.size() && i < 2; ++i) q.products[i] = r.products[i];
-            q.rate = r.rate;
-            q.educt_distance = r.eductDistance;
-            q.product_distance = r.productDistance;
-            q.weight1 = r.weight1;
-            q.weight2 = r.weight2;
-            rxs.push_back(q);
-        }
-        rdy_context_desc cd{};
-        cd.n_types = (uint32_t)types.size();
-        cd.kbt = c.kBT();
-        cd.diffusion = D.data();
-        cd.topology_flavor = flavor.data();
-        cd.n_potentials = (uint32_t)pots.size();
-        cd.potentials = pots.data();
-        cd.n_reactions = (uint32_t)rxs.size();
-        cd.reactions = rxs.data();
+        ContextDesc desc(c);
+        const rdy_context_desc &cd = desc.cd;
         rdy_engine_config cfg{};
         cfg.nl_cutoff = _nlCutoff;
         // the engine's own default covers the pair potentials and order-2 reactions; the spatial topology
@@ -233,7 +242,7 @@ public:
         std::unique_ptr<Action> evaluateCompartments() override;
         std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) override;
         std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) override;
-        std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) override;
+        std::unique_ptr<DetailedBalance> detailedBalance(scalar dt) override;
         std::unique_ptr<TimeStepDependentAction> breakBonds(scalar dt, BreakConfig config) override;
         std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) override;
         std::unique_ptr<Action> initializeKernel() override;
@@ -253,6 +262,7 @@ private:
     scalar _nlCutoff = 0;  // 0: Context::calculateMaxCutoff
 
     friend class MI355XStateModel;
+    friend class MI355XDetailedBalance;
 };
 
 // ---- state model -----------------------------------------------------------------------------------
@@ -758,13 +768,75 @@ std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::gillespie(scalar
     });
 }
 
-// the CPU kernel does not provide DetailedBalance either (SCPU-only in the reference)
 std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::breakBonds(scalar dt, BreakConfig config) {
     return step_action(dt, [k = _k, config](scalar h) { k->_state.breakBonds(h, config); });
 }
 
-std::unique_ptr<TimeStepDependentAction> MI355XKernel::Actions::detailedBalance(scalar) {
-    throw std::invalid_argument("DetailedBalance reaction handler is not available on the MI355X kernel");
+// SCPUDetailedBalance (kernels/singlecpu/src/actions/SCPUReactionImpls.cpp:407-813) on the device; the
+// reversible pairs come from rdy_reversible_reactions when the action is created
+class MI355XDetailedBalance : public DetailedBalance {
+public:
+    MI355XDetailedBalance(MI355XKernel *k, scalar dt) : DetailedBalance(dt), _k(k) {
+        const Context &c = k->context();
+        ContextDesc desc(c);
+        const auto box = c.boxSize();
+        const double volume = box[0] * box[1] * box[2];
+        uint64_t n = 0;
+        if (rdy_reversible_reactions(&desc.cd, volume, 0, nullptr, &n) != RDY_OK)
+            throw std::logic_error(rdy_last_error());
+        std::vector<rdy_reversible_reaction> rows(n);
+        if (n && rdy_reversible_reactions(&desc.cd, volume, n, rows.data(), &n) != RDY_OK)
+            throw std::logic_error(rdy_last_error());
+        const auto &rx = c.reactions().reactions();
+        for (const auto &r : rows) {
+            ReversibleReactionConfig q;
+            q.forwardId = r.forward;
+            q.backwardId = r.backward;
+            q.forwardName = rx.at(r.forward).name;
+            q.backwardName = rx.at(r.backward).name;
+            q.reversibleType = r.kind == RDY_REVERSIBLE_FUSION_FISSION          ? ReversibleType::FusionFission
+                               : r.kind == RDY_REVERSIBLE_CONVERSION_CONVERSION ? ReversibleType::ConversionConversion
+                                                                                : ReversibleType::EnzymaticEnzymatic;
+            q.numberLhsTypes = (std::uint8_t)r.n_lhs;
+            q.numberRhsTypes = (std::uint8_t)r.n_rhs;
+            for (int i = 0; i < 2; ++i) {
+                q.lhsTypes[i] = r.lhs[i];
+                q.rhsTypes[i] = r.rhs[i];
+                q.lhsNames[i] = i < (int)r.n_lhs ? c.particleTypes().nameOf(r.lhs[i]) : std::string();
+                q.rhsNames[i] = i < (int)r.n_rhs ? c.particleTypes().nameOf(r.rhs[i]) : std::string();
+            }
+            q.microForwardRate = r.micro_forward_rate;
+            q.microBackwardRate = r.micro_backward_rate;
+            q.reactionRadius = r.reaction_radius;
+            q.lhsInteractionRadius = r.lhs_interaction_radius;
+            q.rhsInteractionRadius = r.rhs_interaction_radius;
+            q.lhsInteractionVolume = r.lhs_interaction_volume;
+            q.rhsInteractionVolume = r.rhs_interaction_volume;
+            q.effectiveLhsInteractionVolume = r.eff_lhs_interaction_volume;
+            q.effectiveRhsInteractionVolume = r.eff_rhs_interaction_volume;
+            q.effectiveLhsReactionVolume = r.eff_lhs_reaction_volume;
+            q.effectiveRhsReactionVolume = r.eff_rhs_reaction_volume;
+            q.totalVolume = r.total_volume;
+            q.kbt = r.kbt;
+            q.acceptancePrefactor = r.acceptance_prefactor;
+            q.equilibriumConstant = r.equilibrium_constant;
+            q.macroForwardRate = r.macro_forward_rate;
+            q.macroBackwardRate = r.macro_backward_rate;
+            _reversibleReactions.push_back(q);
+        }
+    }
+    void perform() override {
+        _k->_state.flush();
+        _k->syncRecording();
+        ok(rdy_react(_k->engine(), RDY_HANDLER_DETAILED_BALANCE, _dt), "detailed balance");
+    }
+
+private:
+    MI355XKernel *_k;
+};
+
+std::unique_ptr<DetailedBalance> MI355XKernel::Actions::detailedBalance(scalar dt) {
+    return std::make_unique<MI355XDetailedBalance>(_k, dt);
 }
 
 // EvaluateTopologyReactions (Actions.h:343-360): the spatial topology reactions of the topology-particle

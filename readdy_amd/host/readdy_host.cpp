@@ -493,6 +493,28 @@ std::vector<GraphTopology> GraphTopology::connectedComponents() const {
     return out;
 }
 
+std::string DetailedBalance::describe() const {  // Actions.cpp:521-545
+    std::string s;
+    if (_reversibleReactions.empty()) return s;
+    s += "- reversible reactions handled subject to detailed-balance:\n";
+    for (const auto &r : _reversibleReactions) {
+        s += "     * -> \"" + r.forwardName + "\" and <- \"" + r.backwardName + "\", with types: ";
+        switch (r.reversibleType) {
+        case ReversibleType::FusionFission:
+            s += r.lhsNames[0] + " + " + r.lhsNames[1] + " <--> " + r.rhsNames[0];
+            break;
+        case ReversibleType::ConversionConversion:
+            s += r.lhsNames[0] + " <--> " + r.rhsNames[0];
+            break;
+        case ReversibleType::EnzymaticEnzymatic:
+            s += r.lhsNames[0] + " + " + r.lhsNames[1] + " <--> " + r.rhsNames[0] + " + " + r.rhsNames[1];
+            break;
+        }
+        s += "\n";
+    }
+    return s;
+}
+
 void BreakConfig::addBreakablePair(ParticleTypeId type1, ParticleTypeId type2, scalar thresholdEnergy, scalar rate) {
     if (!(rate > 0.))  // Actions.h:193-195
         throw std::logic_error("Rate of breaking a bond must be positive, was " + std::to_string(rate));
@@ -883,8 +905,9 @@ SimulationLoop::SimulationLoop(model::Kernel *kernel, scalar timeStep) : _kernel
 }
 
 void SimulationLoop::useReactionScheduler(const std::string &name) {
-    if (name != "UncontrolledApproximation" && name != "Gillespie")
-        throw std::invalid_argument("unknown reaction scheduler \"" + name + "\"");
+    if (name != "UncontrolledApproximation" && name != "Gillespie" && name != "DetailedBalance")
+        throw std::invalid_argument("Requested reaction scheduler " + name + " is not available.");
+    if (name == "DetailedBalance") (void)_kernel->actions().detailedBalance(_dt);  // its pairs are checked now
     _scheduler = name;
 }
 
@@ -943,7 +966,10 @@ void SimulationLoop::run(std::size_t steps) {
     auto nl = actions.updateNeighborList();
     std::unique_ptr<model::TimeStepDependentAction> react, topoReact;
     if (reactions)
-        react = scheduler == "Gillespie" ? actions.gillespie(_dt) : actions.uncontrolledApproximation(_dt);
+        react = scheduler == "Gillespie"         ? actions.gillespie(_dt)
+                : scheduler == "DetailedBalance" ? std::unique_ptr<model::TimeStepDependentAction>(
+                                                       actions.detailedBalance(_dt))
+                                                 : actions.uncontrolledApproximation(_dt);
     if (_topologyReactions && !_kernel->context().topologyRegistry().spatialReactions().empty())
         topoReact = actions.evaluateTopologyReactions(_dt);
     nl->perform();

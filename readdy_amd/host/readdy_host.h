@@ -301,6 +301,39 @@ protected:
     scalar _dt;
 };
 
+// ---- DetailedBalance (Actions.h:157-176, DetailedBalance.h:49-148) ----------------------------
+enum class ReversibleType { FusionFission, ConversionConversion, EnzymaticEnzymatic };
+
+// ReversibleReactionConfig (DetailedBalance.h:64-139): a pair of reactions that undo each other and the
+// values derived from it; ids are reaction indices, the FusionFission forward reaction is the fusion
+struct ReversibleReactionConfig {
+    std::size_t forwardId = 0, backwardId = 0;
+    std::string forwardName, backwardName;
+    ReversibleType reversibleType = ReversibleType::ConversionConversion;
+    std::uint8_t numberLhsTypes = 0, numberRhsTypes = 0;
+    std::array<ParticleTypeId, 2> lhsTypes{{0, 0}}, rhsTypes{{0, 0}};
+    std::array<std::string, 2> lhsNames, rhsNames;
+    scalar microForwardRate = 0, microBackwardRate = 0, reactionRadius = 0;
+    scalar lhsInteractionRadius = 0, rhsInteractionRadius = 0, lhsInteractionVolume = 0, rhsInteractionVolume = 0;
+    scalar effectiveLhsInteractionVolume = 0, effectiveRhsInteractionVolume = 0;
+    scalar effectiveLhsReactionVolume = 0, effectiveRhsReactionVolume = 0;
+    scalar totalVolume = 0, kbt = 0, acceptancePrefactor = 1;
+    scalar equilibriumConstant = 0, macroForwardRate = 0, macroBackwardRate = 0;
+};
+
+// the DetailedBalance reaction handler: the reversible pairs are searched when the action is created
+// (DetailedBalance::searchReversibleReactions, Actions.cpp:432-519), which throws std::logic_error for
+// ambiguous or inconsistent pairs
+class DetailedBalance : public TimeStepDependentAction {
+public:
+    using TimeStepDependentAction::TimeStepDependentAction;
+    const std::vector<ReversibleReactionConfig> &reversibleReactions() const { return _reversibleReactions; }
+    std::string describe() const;  // Actions.cpp:369-430: one block per pair
+
+protected:
+    std::vector<ReversibleReactionConfig> _reversibleReactions;
+};
+
 // BreakConfig (Actions.h:190-211): per unordered pair of particle types, the energy above which a bond
 // may break and the rate at which it then breaks
 class BreakConfig {
@@ -328,7 +361,7 @@ public:
     virtual std::unique_ptr<Action> evaluateCompartments() = 0;
     virtual std::unique_ptr<TimeStepDependentAction> uncontrolledApproximation(scalar dt) = 0;
     virtual std::unique_ptr<TimeStepDependentAction> gillespie(scalar dt) = 0;
-    virtual std::unique_ptr<TimeStepDependentAction> detailedBalance(scalar dt) = 0;
+    virtual std::unique_ptr<DetailedBalance> detailedBalance(scalar dt) = 0;
     // ActionFactory::breakBonds (Actions.h:218-341)
     virtual std::unique_ptr<TimeStepDependentAction> breakBonds(scalar dt, BreakConfig config) = 0;
     virtual std::unique_ptr<TimeStepDependentAction> evaluateTopologyReactions(scalar dt) = 0;
@@ -640,7 +673,8 @@ namespace observables = model::observables;
 class SimulationLoop {
 public:
     SimulationLoop(model::Kernel *kernel, scalar timeStep);
-    void useReactionScheduler(const std::string &name);  // "UncontrolledApproximation" | "Gillespie"
+    // "UncontrolledApproximation" | "Gillespie" | "DetailedBalance" (ActionFactory.h:80-91)
+    void useReactionScheduler(const std::string &name);
     void evaluateForces(bool v) { _forces = v; }
     void evaluateReactions(bool v) { _reactions = v; }
     void evaluateTopologyReactions(bool v) { _topologyReactions = v; }  // SimulationLoop.h:185-190

@@ -118,14 +118,93 @@ static void test_provider_and_errors(const std::string &dir) {
     auto k = prov.create("MI355X");
     CHECK(k->name() == "MI355X", "kernel name");
     CHECK(k->supportsGillespie(), "MI355X supports Gillespie");
-    threw = false;
+    std::printf("provider: ok\n");
+}
+
+static bool close_rel(double a, double b, double eps) { return std::fabs(a - b) <= eps * std::max(std::fabs(a), std::fabs(b)); }
+
+// DetailedBalance's reversible pairs through the plugin (host computation): the reference's known
+// answers (readdy/test/TestDetailedBalance.cpp:65-91, 322-327) and the scheduler names
+static void test_detailed_balance_configs() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    const double L = 21.544346900318832;
+    c.boxSize() = {L, L, L};
+    c.particleTypes().add("A", 1.);
+    c.particleTypes().add("B", 1.);
+    c.particleTypes().add("C", 1.);
+    c.potentials().addHarmonicRepulsion("A", "B", 2., 3.);
+    c.reactions().add("fusion: A +(5) B -> C", 1.);
+    c.reactions().add("fission: C -> A +(5) B", 2.);
+    auto db = k->actions().detailedBalance(1.);
+    CHECK(db->reversibleReactions().size() == 1, "one reversible pair");
+    if (db->reversibleReactions().size() == 1) {
+        const auto &r = db->reversibleReactions()[0];
+        CHECK(r.reversibleType == model::ReversibleType::FusionFission, "FusionFission");
+        CHECK(r.forwardName == "fusion" && r.backwardName == "fission", "names %s %s", r.forwardName.c_str(),
+              r.backwardName.c_str());
+        CHECK(close_rel(r.totalVolume, 10000., 1e-4), "total volume %g", r.totalVolume);
+        CHECK(r.lhsInteractionRadius == 3. && r.reactionRadius == 5., "radii");
+        CHECK(close_rel(r.lhsInteractionVolume, 113.09733, 1e-4), "lhs volume %g", r.lhsInteractionVolume);
+        CHECK(close_rel(r.effectiveLhsInteractionVolume, 68.09910, 1e-4), "eff lhs int %g", r.effectiveLhsInteractionVolume);
+        CHECK(close_rel(r.effectiveLhsReactionVolume, 478.60057, 1e-4), "eff lhs rx %g", r.effectiveLhsReactionVolume);
+        CHECK(close_rel(r.equilibriumConstant, 41.60045, 1e-4), "K %g", r.equilibriumConstant);
+        CHECK(close_rel(r.macroForwardRate, 480.76392, 1e-4), "kf %g", r.macroForwardRate);
+        CHECK(r.macroBackwardRate == 2. && r.acceptancePrefactor == 1., "kb, prefactor");
+    }
+    CHECK(db->describe().find("A + B <--> C") != std::string::npos, "describe: %s", db->describe().c_str());
+    api::SimulationLoop loop(k.get(), 0.1);
+    loop.useReactionScheduler("DetailedBalance");
+    bool threw = false;
     try {
-        k->actions().detailedBalance(0.1);
+        loop.useReactionScheduler("NoSuchScheduler");
     } catch (const std::invalid_argument &) {
         threw = true;
     }
-    CHECK(threw, "DetailedBalance must throw");
-    std::printf("provider: ok\n");
+    CHECK(threw, "an unknown scheduler throws std::invalid_argument");
+    c.reactions().add("anotherFusion: A +(5) B -> C", 1.);
+    threw = false;
+    try {
+        (void)k->actions().detailedBalance(1.);
+    } catch (const std::logic_error &) {
+        threw = true;
+    }
+    CHECK(threw, "duplicate reversible pairs throw std::logic_error");
+    std::printf("detailed balance configs: ok\n");
+}
+
+// DetailedBalance on the device through the plugin: the reference's integration scenes
+// (TestDetailedBalance.cpp:379-462): a conversion A -> B next to 500 D particles is accepted when A
+// repels D and rejected when B does
+static void test_detailed_balance_scenes() {
+    for (int accept = 0; accept < 2; ++accept) {
+        auto k = plugin::KernelProvider::getInstance().create("MI355X");
+        auto &c = k->context();
+        c.boxSize() = {12., 12., 12.};
+        c.particleTypes().add("A", 1.);
+        c.particleTypes().add("B", 1.);
+        c.particleTypes().add("D", 0.);
+        c.reactions().addConversion("convAB", "A", "B", 1e15);
+        c.reactions().addConversion("convBA", "B", "A", 1e-15);
+        c.potentials().addHarmonicRepulsion(accept ? "A" : "B", "D", 10., 2.);
+        std::mt19937 gen(5);
+        std::normal_distribution<double> nd(0., 1.);
+        std::vector<model::Particle> ps;
+        ps.emplace_back(0., 0., 0., c.particleTypes().idOf("A"));
+        for (int i = 0; i < 500; ++i) ps.emplace_back(nd(gen), nd(gen), nd(gen), c.particleTypes().idOf("D"));
+        k->actions().addParticles(ps)->perform();
+        auto db = k->actions().detailedBalance(0.1);
+        db->perform();
+        std::size_t nA = 0, nB = 0, nD = 0;
+        for (const auto &p : k->stateModel().getParticles()) {
+            nA += p.type() == c.particleTypes().idOf("A");
+            nB += p.type() == c.particleTypes().idOf("B");
+            nD += p.type() == c.particleTypes().idOf("D");
+        }
+        CHECK(nD == 500 && nA + nB == 1, "conservation");
+        CHECK(nB == (std::size_t)accept, "conversion %s", accept ? "accepted" : "rejected");
+    }
+    std::printf("detailed balance scenes: ok\n");
 }
 
 // forces and energy of the C0 system against the oracle; then five loop steps event for event
@@ -826,7 +905,9 @@ int main(int argc, char **argv) {
     const bool provider_only = argc > 2 && std::string(argv[2]) == "--provider-only";
     try {
         test_provider_and_errors(argv[1]);
+        test_detailed_balance_configs();
         if (!provider_only) {
+            test_detailed_balance_scenes();
             test_c0_forces_and_loop(argv[1]);
             test_observables();
             test_topology();
