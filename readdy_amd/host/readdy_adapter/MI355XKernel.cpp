@@ -4,7 +4,7 @@
 // ReaDDy's dependencies; see the Makefile beside this file.  It mirrors readdy_amd/host/mi355x_kernel.cpp, the
 // plugin of this repository's own host core, which is built and tested here.
 //
-// Scope: particles, pair potentials, reactions (UncontrolledApproximation, Gillespie), the integrator, the
+// Scope: particles, pair potentials, reactions (UncontrolledApproximation, Gillespie, DetailedBalance), the integrator, the
 // neighbour list, the device-fused observables and checkpoints.  Topologies are not offered through this
 // adapter (getTopologyActionFactory() returns null, so Kernel::supportsTopologies() is false and ReaDDy's API
 // refuses topology calls up front); the host core of this repository carries them.
@@ -464,6 +464,26 @@ private:
     MI355XKernel *_k;
 };
 
+// SCPUDetailedBalance::perform (kernels/singlecpu/src/actions/SCPUReactionImpls.cpp:407-555) on the device.
+// The reversible pairs the base class lists are the reference's own search (Actions.cpp:432-519, which also
+// throws for ambiguous pairs); the device pairs the same reactions in registration order, so for an
+// EnzymaticEnzymatic pair the reaction that gets the acceptance prefactor is the one registered first,
+// where the reference's search walks hash maps keyed by particle type.
+class DetailedBalance : public rm::actions::reactions::DetailedBalance {
+public:
+    DetailedBalance(MI355XKernel *k, scalar dt) : rm::actions::reactions::DetailedBalance(dt), _k(k) {
+        searchReversibleReactions(k->context());
+    }
+    void perform() override {
+        _k->state().flush();
+        _k->syncRecording();
+        if (_k->hasReplica()) ok(rdy_react(_k->engine(), RDY_HANDLER_DETAILED_BALANCE, timeStep()), "detailed balance");
+    }
+
+private:
+    MI355XKernel *_k;
+};
+
 class EvaluateTopologyReactions : public rm::actions::top::EvaluateTopologyReactions {
 public:
     using rm::actions::top::EvaluateTopologyReactions::EvaluateTopologyReactions;
@@ -536,9 +556,8 @@ public:
     std::unique_ptr<rm::actions::reactions::Gillespie> gillespie(scalar dt) const override {
         return std::make_unique<React<rm::actions::reactions::Gillespie, RDY_HANDLER_GILLESPIE>>(_k, dt);
     }
-    std::unique_ptr<rm::actions::reactions::DetailedBalance> detailedBalance(scalar) const override {
-        // as ReaDDy's CPU kernel (CPUActionFactory.cpp:108)
-        throw std::invalid_argument("DetailedBalance reaction handler is not available on the MI355X kernel");
+    std::unique_ptr<rm::actions::reactions::DetailedBalance> detailedBalance(scalar dt) const override {
+        return std::make_unique<DetailedBalance>(_k, dt);
     }
     std::unique_ptr<rm::actions::top::EvaluateTopologyReactions> evaluateTopologyReactions(scalar dt) const override {
         return std::make_unique<EvaluateTopologyReactions>(dt);
