@@ -22,6 +22,9 @@ Writers (one file per call sequence: create, datasets, attributes, close):
   config               /readdy/config/{particle_types, registered_reactions, general}
                        (IOUtils.cpp:58-176: what readdy.Trajectory reads for type names, reactions,
                        kBT, box and periodicity)
+Every time-indexed dataset is chunked and unlimited along time (chunk rows: File.set_chunk, the
+reference's flushStride), and a writer called again for the same observable appends its frames, so a
+recorder can flush every flushStride frames (Observable.h:157-199).
 Readers (tests and tools): read_u64, read_f64, read_records, read_string, read_reactions, read_vlen.
 """
 import ctypes
@@ -50,6 +53,7 @@ def lib():
         h = ctypes.CDLL(os.path.join(_HERE, "libreaddy_h5.so"))
         h.rdy_h5_create.restype = ctypes.c_int64
         h.rdy_h5_create.argtypes = [ctypes.c_char_p]
+        h.rdy_h5_set_chunk.argtypes = [ctypes.c_int64, ctypes.c_uint64]
         for fn in ("rdy_h5_n_particles", "rdy_h5_reaction_counts"):
             getattr(h, fn).argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
                                        _u64p]
@@ -92,11 +96,17 @@ def _p(a, t):
 class File:
     """One HDF5 file being written; every write raises on failure."""
 
-    def __init__(self, path):
+    def __init__(self, path, chunk=64):
         self.path = path
         self.f = lib().rdy_h5_create(path.encode())
         if self.f < 0:
             raise OSError(f"cannot create {path}")
+        self._top_totals = {}  # topologies observable -> (particle rows, edge rows) written so far
+        self.set_chunk(chunk)
+
+    def set_chunk(self, rows):
+        """chunk rows (time points) of the datasets created from now on: the reference's flushStride"""
+        self._check(lib().rdy_h5_set_chunk(self.f, int(rows)), "chunk size")
 
     def _check(self, rc, what):
         if rc:
@@ -160,6 +170,7 @@ class File:
         if len(frames) != len(t):
             raise ValueError("one frame per time point")
         parts, edges, lp, le, types = [], [], [], [], []
+        done_p, done_e = self._top_totals.get(name, (0, 0))  # rows of earlier appends of this observable
         for tops in frames:
             p0, e0 = len(parts), len(edges)
             for ty, idx, ed in tops:
@@ -167,8 +178,8 @@ class File:
                 parts.extend(int(i) for i in idx)
                 edges.append((len(ed), 0))
                 edges.extend((int(a), int(b)) for a, b in ed)
-            lp.append((p0, len(parts)))
-            le.append((e0, len(edges)))
+            lp.append((done_p + p0, done_p + len(parts)))
+            le.append((done_e + e0, done_e + len(edges)))
             types.append(np.array([ty for ty, _, _ in tops], dtype=np.uint16))
 
         def table(dataset, rows, cols):
@@ -181,6 +192,7 @@ class File:
         table("edges", edges, 2)
         table("limitsEdges", le, 2)
         self._vlen(name, "types", 1, time, types, True)
+        self._top_totals[name] = (done_p + len(parts), done_e + len(edges))
 
     def flat_trajectory(self, name, time, frames):
         """frames: one (ids, type ids, flavors, positions [n][3]) tuple per entry of time."""

@@ -26,8 +26,16 @@
 //                                                   (IOUtils.cpp:58-176, as readdy.Trajectory reads them)
 // plus string attributes on /readdy/config describing the request.
 //
+// Every time-indexed dataset is chunked and unlimited along time, as the reference's observables create
+// them with their flushStride as the chunk size (Observable.h:157-199, TimeSeriesWriter.h:64-66): a
+// writer called again for the same observable appends its frames (a flat trajectory's limits continue
+// from the records already written), so a recorder can flush every flushStride frames instead of
+// holding the whole series in memory.
+//
 // C ABI (all return 0 on success, -1 on an HDF5 error):
 //   int64_t rdy_h5_create(const char *path);                    file handle (> 0) or -1
+//   int rdy_h5_set_chunk(int64_t f, uint64_t rows);   chunk rows (time points) of datasets created from
+//                                                      now on in this file (the flushStride; default 64)
 //   int rdy_h5_n_particles(int64_t f, const char *name, uint64_t T, uint32_t n_types,
 //                          const uint64_t *time, const uint64_t *counts);
 //   int rdy_h5_reaction_counts(int64_t f, const char *name, uint64_t T, uint32_t n_reactions,
@@ -73,12 +81,93 @@
 //       a scalar string dataset into out (NUL-terminated; -1 when it needs more than cap bytes)
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 namespace {
+
+std::mutex g_chunk_mu;
+std::map<hid_t, hsize_t> g_chunk;  // chunk rows per open file
+
+hsize_t chunk_rows(hid_t g) {
+    const hid_t f = H5Iget_file_id(g);
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    auto it = g_chunk.find(f);
+    const hsize_t r = it == g_chunk.end() ? 64 : it->second;
+    H5Fclose(f);  // H5Iget_file_id took a reference
+    return r;
+}
+
+// Appends dims[0] rows (each of shape dims[1 .. rank-1]) to the dataset `name` of group g, creating it
+// chunked and unlimited along axis 0 when absent; *prev (if not null) = its row count before.
+int append(hid_t g, const char *name, hid_t ftype, hid_t mtype, int rank, const hsize_t *dims, const void *data,
+           hsize_t *prev = nullptr) {
+    hsize_t cur[4] = {0, 0, 0, 0}, nd[4], start[4] = {0, 0, 0, 0};
+    hid_t ds;
+    if (H5Lexists(g, name, H5P_DEFAULT) > 0) {
+        ds = H5Dopen2(g, name, H5P_DEFAULT);
+        if (ds < 0) return -1;
+        hid_t sp = H5Dget_space(ds);
+        const int r = H5Sget_simple_extent_ndims(sp);
+        if (r != rank) {
+            H5Sclose(sp);
+            H5Dclose(ds);
+            return -1;
+        }
+        H5Sget_simple_extent_dims(sp, cur, nullptr);
+        H5Sclose(sp);
+        for (int a = 1; a < rank; ++a)
+            if (cur[a] != dims[a]) {
+                H5Dclose(ds);
+                return -1;
+            }
+    } else {
+        hsize_t zero[4], maxd[4], chunk[4];
+        for (int a = 0; a < rank; ++a) {
+            zero[a] = a == 0 ? 0 : dims[a];
+            maxd[a] = a == 0 ? H5S_UNLIMITED : dims[a];
+            chunk[a] = a == 0 ? std::max<hsize_t>(chunk_rows(g), 1) : std::max<hsize_t>(dims[a], 1);
+        }
+        hid_t sp = H5Screate_simple(rank, zero, maxd);
+        hid_t dcpl = H5Pcreate(H5P_DATASET_CREATE);
+        H5Pset_chunk(dcpl, rank, chunk);
+        ds = H5Dcreate2(g, name, ftype, sp, H5P_DEFAULT, dcpl, H5P_DEFAULT);
+        H5Pclose(dcpl);
+        H5Sclose(sp);
+        if (ds < 0) return -1;
+        for (int a = 1; a < rank; ++a) cur[a] = dims[a];
+    }
+    if (prev) *prev = cur[0];
+    int rc = 0;
+    if (dims[0] > 0) {
+        for (int a = 0; a < rank; ++a) nd[a] = a == 0 ? cur[0] + dims[0] : dims[a];
+        rc = H5Dset_extent(ds, nd) < 0 ? -1 : 0;
+        if (rc == 0) {
+            hid_t fs = H5Dget_space(ds);
+            start[0] = cur[0];
+            H5Sselect_hyperslab(fs, H5S_SELECT_SET, start, nullptr, dims, nullptr);
+            hid_t ms = H5Screate_simple(rank, dims, nullptr);
+            rc = H5Dwrite(ds, mtype, ms, fs, H5P_DEFAULT, data) < 0 ? -1 : 0;
+            H5Sclose(ms);
+            H5Sclose(fs);
+        }
+    }
+    H5Dclose(ds);
+    return rc;
+}
+
+int append_u64(hid_t g, const char *name, int rank, const hsize_t *dims, const uint64_t *data, hsize_t *prev = nullptr) {
+    return append(g, name, H5T_STD_U64LE, H5T_NATIVE_UINT64, rank, dims, data, prev);
+}
+
+int append_f64(hid_t g, const char *name, int rank, const hsize_t *dims, const double *data) {
+    return append(g, name, H5T_IEEE_F64LE, H5T_NATIVE_DOUBLE, rank, dims, data);
+}
 
 // failures are reported through the return codes: HDF5's own error-stack printing stays off
 struct QuietErrors {
@@ -225,19 +314,6 @@ int write_f64(hid_t g, const char *name, int rank, const hsize_t *dims, const do
     return rc;
 }
 
-int write_u64(hid_t g, const char *name, int rank, const hsize_t *dims, const uint64_t *data) {
-    hid_t space = H5Screate_simple(rank, dims, nullptr);
-    if (space < 0) return -1;
-    hid_t ds = H5Dcreate2(g, name, H5T_STD_U64LE, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    int rc = -1;
-    if (ds >= 0) {
-        rc = H5Dwrite(ds, H5T_NATIVE_UINT64, H5S_ALL, H5S_ALL, H5P_DEFAULT, data) < 0 ? -1 : 0;
-        H5Dclose(ds);
-    }
-    H5Sclose(space);
-    return rc;
-}
-
 }  // namespace
 
 extern "C" {
@@ -247,13 +323,20 @@ int64_t rdy_h5_create(const char *path) {
     return f < 0 ? -1 : (int64_t)f;
 }
 
+int rdy_h5_set_chunk(int64_t f, uint64_t rows) {
+    if (f < 0 || rows == 0) return -1;
+    std::lock_guard<std::mutex> lk(g_chunk_mu);
+    g_chunk[(hid_t)f] = (hsize_t)rows;
+    return 0;
+}
+
 int rdy_h5_n_particles(int64_t f, const char *name, uint64_t T, uint32_t n_types, const uint64_t *time,
                        const uint64_t *counts) {
     hid_t g = observable_group((hid_t)f, name);
     if (g < 0) return -1;
     hsize_t dt[1] = {T}, dd[2] = {T, n_types};
-    int rc = write_u64(g, "time", 1, dt, time);
-    if (rc == 0) rc = write_u64(g, "data", 2, dd, counts);
+    int rc = append_u64(g, "time", 1, dt, time);
+    if (rc == 0) rc = append_u64(g, "data", 2, dd, counts);
     H5Gclose(g);
     return rc;
 }
@@ -263,13 +346,13 @@ int rdy_h5_reaction_counts(int64_t f, const char *name, uint64_t T, uint32_t n_r
     hid_t g = observable_group((hid_t)f, name);
     if (g < 0) return -1;
     hsize_t dt[1] = {T};
-    int rc = write_u64(g, "time", 1, dt, time);
+    int rc = append_u64(g, "time", 1, dt, time);
     hid_t c = rc == 0 ? ensure_group(g, "counts") : -1;
     if (c < 0) rc = -1;
-    std::vector<uint64_t> col(T);
+    std::vector<uint64_t> col(std::max<uint64_t>(T, 1));
     for (uint32_t r = 0; rc == 0 && r < n_reactions; ++r) {
         for (uint64_t t = 0; t < T; ++t) col[t] = counts[t * n_reactions + r];
-        rc = write_u64(c, std::to_string(r).c_str(), 1, dt, col.data());
+        rc = append_u64(c, std::to_string(r).c_str(), 1, dt, col.data());
     }
     if (c >= 0) H5Gclose(c);
     H5Gclose(g);
@@ -281,9 +364,9 @@ int rdy_h5_radial_distribution(int64_t f, const char *name, uint64_t T, uint32_t
     hid_t g = observable_group((hid_t)f, name);
     if (g < 0) return -1;
     hsize_t dt[1] = {T}, dc[1] = {n_bins}, dd[2] = {T, n_bins};
-    int rc = write_f64(g, "bin_centers", 1, dc, centers);
-    if (rc == 0) rc = write_f64(g, "distribution", 2, dd, dist);
-    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    int rc = H5Lexists(g, "bin_centers", H5P_DEFAULT) > 0 ? 0 : write_f64(g, "bin_centers", 1, dc, centers);
+    if (rc == 0) rc = append_f64(g, "distribution", 2, dd, dist);
+    if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -293,7 +376,7 @@ int rdy_h5_u64_table(int64_t f, const char *name, const char *dataset, uint64_t 
     hid_t g = observable_group((hid_t)f, name);
     if (g < 0) return -1;
     hsize_t d[2] = {rows, cols};
-    const int rc = write_u64(g, dataset, cols > 1 ? 2 : 1, d, data);
+    const int rc = append_u64(g, dataset, cols > 1 ? 2 : 1, d, data);
     H5Gclose(g);
     return rc;
 }
@@ -303,8 +386,8 @@ int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, c
     hid_t g = observable_group((hid_t)f, name);
     if (g < 0) return -1;
     hsize_t dt[1] = {T}, dd[2] = {T, width};
-    int rc = write_f64(g, "data", 2, dd, data);
-    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    int rc = append_f64(g, "data", 2, dd, data);
+    if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -325,14 +408,10 @@ int rdy_h5_vlen_series(int64_t f, const char *name, const char *dataset, int ele
         p += frame_sizes[t] * esz;
     }
     hsize_t dt[1] = {T};
-    hid_t space = H5Screate_simple(1, dt, nullptr);
-    hid_t ds = H5Dcreate2(g, dataset, vt, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    int rc = ds < 0 ? -1 : (H5Dwrite(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) < 0 ? -1 : 0);
-    if (ds >= 0) H5Dclose(ds);
-    H5Sclose(space);
+    int rc = append(g, dataset, vt, vt, 1, dt, frames.data());
     H5Tclose(vt);
     H5Tclose(et);
-    if (rc == 0 && time) rc = write_u64(g, "time", 1, dt, time);
+    if (rc == 0 && time) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -367,17 +446,14 @@ int rdy_h5_flat_trajectory(int64_t f, const char *name, uint64_t T, const uint64
     }
     hid_t mem, file;
     record_types(&mem, &file);
-    hsize_t dr[1] = {total};
-    hid_t space = H5Screate_simple(1, dr, nullptr);
-    hid_t ds = H5Dcreate2(g, "records", file, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    int rc = ds < 0 ? -1 : (H5Dwrite(ds, mem, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0 ? -1 : 0);
-    if (ds >= 0) H5Dclose(ds);
-    H5Sclose(space);
+    hsize_t dr[1] = {total}, before = 0;
+    int rc = append(g, "records", file, mem, 1, dr, rec.data(), &before);
     H5Tclose(mem);
     H5Tclose(file);
+    for (auto &l : limits) l += before;  // frames appended after earlier ones continue their records
     hsize_t dl[2] = {T, 2}, dt[1] = {T};
-    if (rc == 0) rc = write_u64(g, "limits", 2, dl, limits.data());
-    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    if (rc == 0) rc = append_u64(g, "limits", 2, dl, limits.data());
+    if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -411,14 +487,10 @@ int rdy_h5_reactions(int64_t f, const char *name, uint64_t T, const uint64_t *ti
     hid_t ct = reaction_record_type();
     hid_t vt = H5Tvlen_create(ct);
     hsize_t dt[1] = {T};
-    hid_t space = H5Screate_simple(1, dt, nullptr);
-    hid_t ds = H5Dcreate2(g, "records", vt, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
-    int rc = ds < 0 ? -1 : (H5Dwrite(ds, vt, H5S_ALL, H5S_ALL, H5P_DEFAULT, frames.data()) < 0 ? -1 : 0);
-    if (ds >= 0) H5Dclose(ds);
-    H5Sclose(space);
+    int rc = append(g, "records", vt, vt, 1, dt, frames.data());
     H5Tclose(vt);
     H5Tclose(ct);
-    if (rc == 0) rc = write_u64(g, "time", 1, dt, time);
+    if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;
 }
@@ -505,7 +577,13 @@ int rdy_h5_attribute(int64_t f, const char *key, const char *value) {
     return rc;
 }
 
-int rdy_h5_close(int64_t f) { return H5Fclose((hid_t)f) < 0 ? -1 : 0; }
+int rdy_h5_close(int64_t f) {
+    {
+        std::lock_guard<std::mutex> lk(g_chunk_mu);
+        g_chunk.erase((hid_t)f);
+    }
+    return H5Fclose((hid_t)f) < 0 ? -1 : 0;
+}
 
 int rdy_h5_read_u64(const char *path, const char *dataset, uint64_t cap, uint64_t *out, uint64_t *n) {
     hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);

@@ -5,7 +5,9 @@ each step t whose number is a multiple of the observable's stride (SimulationLoo
 Observable.h:144-147: stride 0 means every step).  Recorder does the same around the engine's fused
 steps: it runs the replica in chunks that end where the next observable is due, evaluates the due ones
 on the device state, and keeps the series on the host; write() puts them into an HDF5 file with the
-configuration group that readdy.Trajectory reads.
+configuration group that readdy.Trajectory reads.  open() / close() instead stream the series into a
+file: every observable's frames are appended each flush_stride evaluations and leave host memory, as
+the reference's observables flush their buffered frames every flushStride (Observable.h:157-199).
 
   n_particles           NParticles: particles per type                       /readdy/observables/<name>
   reaction_counts       ReactionCounts: events per reaction of the last step  (its handler call)
@@ -30,6 +32,9 @@ class Recorder:
         self.t = 0                 # steps run through this recorder
         self._obs = []             # dicts: kind, name, stride, args, time, values
         self._evaluated_at = None  # the step whose observables were evaluated last
+        self._file = None          # h5io.File while streaming (open() .. close())
+        self._flush_stride = 0
+        self._flavors = None
 
     def _add(self, kind, name, stride, **args):
         if any(o["name"] == name and o["group"] == _group(kind) for o in self._obs):
@@ -104,6 +109,8 @@ class Recorder:
                 v = (handles, types, pos)
             o["time"].append(t)
             o["values"].append(v)
+            if self._file is not None and len(o["time"]) >= self._flush_stride:
+                self._flush(o)
 
     def _next_due(self, t):
         nxt = [t + 1 if o["stride"] == 0 else (t // o["stride"] + 1) * o["stride"] for o in self._obs]
@@ -130,38 +137,75 @@ class Recorder:
         raise KeyError(name)
 
     # ------------------------------------------------------------------------------------ output
-    def write(self, path, type_names, box, pbc=(True, True, True), kbt=1.0, diffusion=None, reactions=(),
-              topology_types=()):
-        """One HDF5 file: the recorded series plus /readdy/config (types, reactions, general).
+    def open(self, path, type_names, box, pbc=(True, True, True), kbt=1.0, diffusion=None, reactions=(),
+             topology_types=(), flush_stride=64):
+        """Streams the series into one HDF5 file: /readdy/config now, then every observable's frames in
+        appends of flush_stride evaluations (also the datasets' chunk rows); close() writes the rest.
         reactions: the engine's reaction dicts (kind, educts, products, rate, ...) with a "name"."""
+        if self._file is not None:
+            raise RuntimeError("the recorder already streams into a file")
+        if int(flush_stride) < 1:
+            raise ValueError("flush_stride must be at least 1")
         diffusion = [0.0] * len(type_names) if diffusion is None else list(diffusion)
-        with h5io.File(path) as f:
+        f = h5io.File(path, chunk=int(flush_stride))
+        try:
             f.config([(n, d, "TOPOLOGY" if n in topology_types else "NORMAL") for n, d in zip(type_names, diffusion)],
                      box, pbc, kbt, [dict(rx, id=i, name=rx.get("name", f"reaction_{i}")) for i, rx in
                                      enumerate(reactions)])
+        except Exception:
+            f.close()
+            raise
+        self._file, self._flush_stride = f, int(flush_stride)
+        # topology particles would carry flavor 1 (TOPOLOGY)
+        self._flavors = np.array([1 if n in topology_types else 0 for n in type_names], dtype=np.uint8)
+        for o in self._obs:  # frames recorded before open() go first
+            if o["time"]:
+                self._flush(o)
+
+    def _flush(self, o):
+        """appends the buffered frames of observable o to the open file and drops them"""
+        f, k, t, v = self._file, o["kind"], o["time"], o["values"]
+        if k == "n_particles":
+            f.n_particles(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
+        elif k == "reaction_counts":
+            f.reaction_counts(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
+        elif k == "reactions":
+            f.reactions(o["name"], t, v)
+        elif k == "rdf":
+            b = o["args"]["borders"]
+            f.radial_distribution(o["name"], t, (b[:-1] + b[1:]) / 2, np.asarray(v).reshape(len(t), -1))
+        elif k == "histogram":
+            f.histogram_along_axis(o["name"], t, np.asarray(v).reshape(len(t), -1))
+        elif k == "positions":
+            f.positions(o["name"], t, v)
+        elif k == "forces":
+            f.forces(o["name"], t, v)
+        elif k == "particles":
+            f.particles(o["name"], t, v)
+        else:
+            flav = self._flavors
+            f.flat_trajectory(o["name"], t, [(h, ty, flav[np.asarray(ty, dtype=np.int64)], x) for h, ty, x in v])
+        o["time"], o["values"] = [], []
+
+    def close(self):
+        """Appends the frames still buffered and closes the file open() started."""
+        if self._file is None:
+            return
+        try:
             for o in self._obs:
-                k, t, v = o["kind"], o["time"], o["values"]
-                if k == "n_particles":
-                    f.n_particles(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
-                elif k == "reaction_counts":
-                    f.reaction_counts(o["name"], t, np.asarray(v, dtype=np.uint64).reshape(len(t), -1))
-                elif k == "reactions":
-                    f.reactions(o["name"], t, v)
-                elif k == "rdf":
-                    b = o["args"]["borders"]
-                    f.radial_distribution(o["name"], t, (b[:-1] + b[1:]) / 2, np.asarray(v).reshape(len(t), -1))
-                elif k == "histogram":
-                    f.histogram_along_axis(o["name"], t, np.asarray(v).reshape(len(t), -1))
-                elif k == "positions":
-                    f.positions(o["name"], t, v)
-                elif k == "forces":
-                    f.forces(o["name"], t, v)
-                elif k == "particles":
-                    f.particles(o["name"], t, v)
-                else:  # topology particles would carry flavor 1 (TOPOLOGY)
-                    flav = np.array([1 if n in topology_types else 0 for n in type_names], dtype=np.uint8)
-                    f.flat_trajectory(o["name"], t, [(h, ty, flav[np.asarray(ty, dtype=np.int64)], x)
-                                                     for h, ty, x in v])
+                if o["time"]:
+                    self._flush(o)
+        finally:
+            f, self._file = self._file, None
+            f.close()
+
+    def write(self, path, type_names, box, pbc=(True, True, True), kbt=1.0, diffusion=None, reactions=(),
+              topology_types=()):
+        """One HDF5 file: the recorded series plus /readdy/config (types, reactions, general).  The frames
+        written leave the recorder, as after a stream's close()."""
+        n = max([len(o["time"]) for o in self._obs] + [1])
+        self.open(path, type_names, box, pbc, kbt, diffusion, reactions, topology_types, flush_stride=n)
+        self.close()
 
 
 def _group(kind):

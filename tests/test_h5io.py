@@ -189,3 +189,44 @@ def test_topologies_observable_round_trip(tmp_path):
     assert h5io.read_u64(path, g + "edges").reshape(-1, 2).tolist() == [[2, 0], [0, 1], [1, 2], [1, 0], [0, 1],
                                                                           [0, 0]]
     assert h5io.read_u64(path, g + "limitsEdges").reshape(-1, 2).tolist() == [[0, 3], [3, 6], [6, 6]]
+
+
+def test_writers_append_to_chunked_series(tmp_path):
+    """A writer called again for one observable appends (the flat trajectory's limits and the topologies'
+    limits continue after the earlier frames); every time-indexed dataset is chunked with the file's
+    chunk rows and unlimited along time, as the reference creates them with its flushStride."""
+    path = str(tmp_path / "app.h5")
+    rng = np.random.default_rng(8)
+    frames = _frames(rng, [3, 0, 2, 4, 1])
+    tops = [[(0, [0, 1], [(0, 1)])], [(1, [2, 3, 4], [(0, 1), (1, 2)]), (0, [5], [])], [],
+            [(2, [6, 7], [(0, 1)])]]
+    with h5io.File(path, chunk=4) as f:
+        f.n_particles("n", [0, 1, 2], np.arange(6).reshape(3, 2))
+        f.n_particles("n", [3, 4], np.arange(6, 10).reshape(2, 2))
+        f.reaction_counts("rc", [0], [[1, 2]])
+        f.reaction_counts("rc", [1, 2], [[3, 4], [5, 6]])
+        f.flat_trajectory("traj", [0, 1], frames[:2])
+        f.flat_trajectory("traj", [2, 3, 4], frames[2:])
+        f.topologies("tops", [0, 1], tops[:2])
+        f.topologies("tops", [2, 3], tops[2:])
+        f.positions("pos", [0], [frames[0][3]])
+        f.positions("pos", [1, 2], [frames[2][3], frames[3][3]])
+    assert list(h5io.read_u64(path, "/readdy/observables/n/time")) == [0, 1, 2, 3, 4]
+    assert list(h5io.read_u64(path, "/readdy/observables/n/data")) == list(range(10))
+    assert list(h5io.read_u64(path, "/readdy/observables/rc/counts/1")) == [2, 4, 6]
+    t, got = h5io.read_trajectory(path, "traj")
+    assert list(t) == [0, 1, 2, 3, 4]
+    for (ids, ty, fl, pos), want in zip(got, frames):
+        assert np.array_equal(ids, want[0]) and np.array_equal(pos, np.asarray(want[3]).reshape(-1, 3))
+    tt, tf = h5io.read_topologies(path, "tops")
+    assert list(tt) == [0, 1, 2, 3]
+    for got_f, want_f in zip(tf, tops):
+        assert [(ty, list(map(int, idx)), ed) for ty, idx, ed in got_f] == \
+               [(ty, idx, [tuple(e) for e in ed]) for ty, idx, ed in want_f]
+    pv = h5io.read_vlen(path, "/readdy/observables/pos/data", 0)
+    assert len(pv) == 3 and np.array_equal(pv[2], np.asarray(frames[3][3]).reshape(-1, 3))
+    exe = _h5dump()
+    if exe:
+        out = subprocess.run([exe, "-p", "-H", "-d", "/readdy/observables/n/data", path], capture_output=True,
+                             text=True, check=True).stdout
+        assert "CHUNKED ( 4, 2 )" in out and "H5S_UNLIMITED" in out, out

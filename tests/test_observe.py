@@ -140,3 +140,33 @@ def test_recorder_on_the_device(tmp_path):
                           np.array(rdf))
     assert np.array_equal(h5io.read_vlen(path, "/readdy/observables/forces/data", 0)[-1], fs[-1])
     assert np.array_equal(h5io.read_vlen(path, "/readdy/observables/particles/ids", 2)[-1], hf)
+
+
+def test_streaming_appends_every_flush_stride(tmp_path):
+    """open() streams: no observable holds flush_stride frames in memory, and the file after close() equals
+    the one write() makes from a recorder that kept everything"""
+    def recorder():
+        r = Recorder(_Steps())
+        r.n_particles(1)
+        r.reaction_counts(2)
+        return r
+
+    path_s, path_w = str(tmp_path / "stream.h5"), str(tmp_path / "write.h5")
+    rx = [dict(kind=0, educts=[0], products=[1], rate=1.0, name="conv")]
+    rs = recorder()
+    rs.open(path_s, ["A", "B"], [10.0] * 3, reactions=rx, flush_stride=3)
+    for _ in range(5):
+        rs.run(0, 0.1, 4)
+        assert all(len(o["time"]) < 3 for o in rs._obs)
+    rs.close()
+    rw = recorder()
+    for _ in range(5):
+        rw.run(0, 0.1, 4)
+    rw.write(path_w, ["A", "B"], [10.0] * 3, reactions=rx)
+    for ds in ("n_particles/time", "n_particles/data", "reaction_counts/time", "reaction_counts/counts/0"):
+        a = h5io.read_u64(path_s, "/readdy/observables/" + ds)
+        b = h5io.read_u64(path_w, "/readdy/observables/" + ds)
+        assert np.array_equal(a, b), ds
+    assert list(h5io.read_u64(path_s, "/readdy/observables/n_particles/time")) == list(range(21))
+    with pytest.raises(ValueError):
+        recorder().open(str(tmp_path / "x.h5"), ["A", "B"], [10.0] * 3, flush_stride=0)
