@@ -253,11 +253,22 @@ typedef struct {
  * radius <= the engine's neighbour-list cutoff, at most 64 reactions */
 int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_reaction *reactions);
 /* the replica's topologies: their types [n_topologies] and, for each topology particle (handles 0 ..
- * n_topo - 1 of the last upload), the index of its topology [n_topo] (StateModel::addTopology) */
+ * n_topo - 1 of the last upload), the index of its topology [n_topo] (StateModel::addTopology), or
+ * UINT32_MAX for one that left its topology (a structural reaction made it a normal particle) */
 int rdy_set_topologies(rdy_engine *e, uint32_t replica, uint32_t n_topologies, const uint32_t *topology_types,
                        const uint32_t *topology_of);
 /* the replica's topology types as the topology reactions left them; *n = the number of topologies */
 int rdy_get_topology_types(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *types, uint32_t *n);
+/* per topology, 1 if a spatial topology reaction changed it in the last rdy_evaluate_topology_reactions call:
+ * the structural reactions of the same evaluation skip those topologies (one topology reaction per topology
+ * and evaluation, eventsDependent in CPUEvaluateTopologyReactions.cpp:455-464); *n = the number of topologies */
+int rdy_get_topology_reacted(rdy_engine *e, uint32_t replica, uint32_t cap, uint8_t *reacted, uint32_t *n);
+/* TopologyReactionActionFactory::createChangeParticleType (TopologyReactionActions.cpp:40-60) for the recipes
+ * of structural topology reactions: topology particles (handles below the replica's topology count) take
+ * the given species; a species of normal flavour takes the particle out of the bonded-term bookkeeping (the
+ * caller drops it from its topology).  Forces and energies are formed anew by the next force evaluation. */
+int rdy_set_particle_types(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *handles,
+                           const uint32_t *types);
 /* EvaluateTopologyReactions (Actions.h, CPUEvaluateTopologyReactions::perform) on the current state; the
  * fused loop (rdy_run, rdy_run_graph) performs it in every step after the reactions (SimulationLoop.h:286-287) */
 int rdy_evaluate_topology_reactions(rdy_engine *e, double dt);

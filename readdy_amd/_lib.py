@@ -108,6 +108,8 @@ SIGNATURES = {
     "rdy_get_virial": (_int, [_vp, _u32, _dp]),
     "rdy_integrate": (_int, [_vp, _d, _dp, _u64]),
     "rdy_react": (_int, [_vp, _int, _d]),
+    "rdy_get_topology_reacted": (_int, [_vp, _u32, _u32, ctypes.POINTER(ctypes.c_uint8), _u32p]),
+    "rdy_set_particle_types": (_int, [_vp, _u32, _u64, _u32p, _u32p]),
     "rdy_reversible_reactions": (_int, [ctypes.POINTER(ContextDesc), _d, _u64, ctypes.POINTER(ReversibleReaction),
                                         _u64p]),
     "rdy_get_reaction_counts": (_int, [_vp, _u32, _u64p]),

@@ -189,6 +189,8 @@ struct rdy_engine {
     std::vector<rdy_spatial_reaction> sr_in;
     SpatialDev *d_sr = nullptr;
     uint32_t *ttype = nullptr;
+    uint32_t *treact = nullptr;      // [tt_cap] serial of the evaluation each topology last reacted in
+    uint32_t topo_serial = 0;        // serial of the last separate topology-reaction evaluation
     uint64_t tt_cap = 0, tt_used = 0;
     std::vector<std::pair<uint64_t, uint64_t>> free_tt;
     TopoEvent *tev = nullptr;
@@ -760,6 +762,8 @@ TopoTables topo_tables(const rdy_engine *e) {
     t.topo_of = e->topo_of;
     t.topo_owner = e->topo_owner;
     t.ttype = e->ttype;
+    t.treact = e->treact;
+    t.serial = 0;
     t.ev = e->tev;
     t.ev_count = e->tev_count;
     t.ev_cap = TOPO_MAX_EVENTS;
@@ -772,10 +776,11 @@ TopoTables topo_tables(const rdy_engine *e) {
 bool topo_reactions_on(const rdy_engine *e) { return !e->sr_in.empty() && e->tt_used > 0 && e->tev; }
 
 // the topology reactions of one step (search, then the greedy resolve); edits: the fused step's list
-void launch_topo_reactions(rdy_engine *e, int require_active, Edit *edits, uint32_t *edit_count) {
+void launch_topo_reactions(rdy_engine *e, int require_active, Edit *edits, uint32_t *edit_count, uint32_t serial = 0) {
     const uint32_t nr = std::max<uint32_t>(e->n_reps_used, 1);
     const uint32_t n_slots = (uint32_t)e->topo_used;
-    const TopoTables tt = topo_tables(e);
+    TopoTables tt = topo_tables(e);
+    tt.serial = serial;
     const uint32_t *slots = edits ? e->loc_cur : e->loc;  // the fused step's walk has moved loc on
     hipLaunchKernelGGL(k_topo_search, dim3(blocks_for(n_slots, 256)), dim3(256), 0, e->stream, e->grid(e->cur),
                        (const ReplicaDev *)e->reps, slots, tt, n_slots, e->cap, e->radius, require_active, e->err);
@@ -1165,7 +1170,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -2291,7 +2296,8 @@ int rdy_set_topologies(rdy_engine *e, uint32_t r, uint32_t n_topologies, const u
     if (n_topologies && !topology_types) return set_err(RDY_ERR_INVALID, "null topology types");
     if (H.n_topo && !topology_of) return set_err(RDY_ERR_INVALID, "null topology assignment");
     for (uint32_t h = 0; h < H.n_topo; ++h)
-        if (topology_of[h] >= n_topologies) return set_err(RDY_ERR_INVALID, "a topology particle's topology does not exist");
+        if (topology_of[h] != 0xFFFFFFFFu && topology_of[h] >= n_topologies)
+            return set_err(RDY_ERR_INVALID, "a topology particle's topology does not exist");
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (H.n_tt) release_range(e->free_tt, H.tt_base, H.n_tt);
     H.tt_base = 0;
@@ -2309,6 +2315,15 @@ int rdy_set_topologies(rdy_engine *e, uint32_t r, uint32_t n_topologies, const u
                     HIP_TRY(hipFree(e->ttype));
                 }
                 e->ttype = nt;
+                uint32_t *nr_;
+                HIP_TRY(dalloc(&nr_, ncap));
+                HIP_TRY(hipMemsetAsync(nr_, 0, ncap * 4, e->stream));
+                if (e->treact) {
+                    HIP_TRY(hipMemcpyAsync(nr_, e->treact, e->tt_used * 4, hipMemcpyDeviceToDevice, e->stream));
+                    HIP_TRY(hipStreamSynchronize(e->stream));
+                    HIP_TRY(hipFree(e->treact));
+                }
+                e->treact = nr_;
                 e->tt_cap = ncap;
             }
             base = e->tt_used;
@@ -2346,9 +2361,56 @@ int rdy_evaluate_topology_reactions(rdy_engine *e, double dt) {
     int rc = upload_tables(e, dt);
     if (rc) return rc;
     if (!topo_reactions_on(e)) return RDY_OK;
-    launch_topo_reactions(e, 0, nullptr, nullptr);
+    if (++e->topo_serial == 0) ++e->topo_serial;  // 0 marks evaluations that are not recorded
+    launch_topo_reactions(e, 0, nullptr, nullptr, e->topo_serial);
     e->export_valid = false;  // species changed: forces and energies of the state are formed anew
     return check_errors(e);
+}
+
+int rdy_get_topology_reacted(rdy_engine *e, uint32_t r, uint32_t cap, uint8_t *reacted, uint32_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    const HostReplica &H = e->reps_h[r];
+    if (n) *n = H.n_tt;
+    const uint32_t m = std::min(cap, H.n_tt);
+    if (m && reacted) {
+        std::vector<uint32_t> v(m);
+        HIP_TRY(hipMemcpyAsync(v.data(), e->treact + H.tt_base, m * 4ull, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (uint32_t k = 0; k < m; ++k) reacted[k] = e->topo_serial != 0 && v[k] == e->topo_serial ? 1 : 0;
+    }
+    return RDY_OK;
+}
+
+int rdy_set_particle_types(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *handles, const uint32_t *types) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if (n == 0) return RDY_OK;
+    if (!handles || !types) return set_err(RDY_ERR_INVALID, "null handles or types");
+    const HostReplica &H = e->reps_h[r];
+    for (uint64_t i = 0; i < n; ++i) {
+        if (handles[i] >= H.n_topo)
+            return set_err(RDY_ERR_INVALID, "only topology particles (handles below the replica's topology count) "
+                                            "can change species here");
+        if (types[i] >= e->n_types) return set_err(RDY_ERR_INVALID, "particle type out of range");
+    }
+    if ((rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0))) return rc;
+    if ((rc = ensure_scratch(e, (size_t)(2 * n + 1) * sizeof(uint32_t)))) return rc;
+    uint32_t *d = (uint32_t *)e->scratch;
+    HIP_TRY(hipMemcpyAsync(d, handles, n * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(d + n, types, n * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(d + 2 * n, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_set_topo_types, dim3(blocks_for((uint32_t)n, 256)), dim3(256), 0, e->stream,
+                       e->grid(e->cur), (const ReplicaDev *)e->reps, r, (const uint32_t *)e->loc, (const uint32_t *)d,
+                       (const uint32_t *)(d + n), (const uint8_t *)e->d_flavor, (uint32_t)n, d + 2 * n);
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, d + 2 * n, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (bad) return set_err(RDY_ERR_STATE, "a topology particle's record was not found");
+    e->export_valid = false;  // species changed: forces and energies are formed anew
+    e->version++;
+    return RDY_OK;
 }
 
 int rdy_get_topology_reaction_counts(rdy_engine *e, uint32_t r, int totals, uint64_t *counts) {

@@ -44,6 +44,8 @@ struct TopoTables {
     const uint32_t *topo_of;     // [topology particle slot] topology index within its replica
     const uint32_t *topo_owner;  // [topology particle slot] replica, UINT32_MAX when unused
     uint32_t *ttype;             // topology types; replica r's at reps[r].tt_base
+    uint32_t *treact;            // per topology (as ttype): the serial of the evaluation it last reacted in
+    uint32_t serial;             // this evaluation's serial (0: not recorded, the fused step)
     TopoEvent *ev;               // [replica][ev_cap]
     uint32_t *ev_count;          // [replica]
     uint32_t ev_cap;
@@ -225,12 +227,32 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
         cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
         TT.ttype[R.tt_base + a.top] = X.top_type_to1;
+        if (TT.serial) TT.treact[R.tt_base + a.top] = TT.serial;
         atomicAdd(&TT.cnt_step[(size_t)r * TT.n_sr + a.sr], 1ull);
         atomicAdd(&TT.cnt_total[(size_t)r * TT.n_sr + a.sr], 1ull);
         if (n1) edit(a.s_top, tw_type(tw1), a.h_top);
         if (n2) edit(a.s_free, tw_type(tw2), a.h_free);
     }
     if (tid == 0) TT.ev_count[r] = 0u;
+}
+
+// sets the species of topology particles of replica r (by handle, found through loc) and their
+// topology flag by the new species' flavour; a handle whose record is not live at loc counts in *bad
+__global__ void k_set_topo_types(Grid cur, const ReplicaDev *reps, uint32_t r, const uint32_t *loc,
+                                 const uint32_t *handles, const uint32_t *types, const uint8_t *flavor, uint32_t n,
+                                 uint32_t *bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ReplicaDev &R = reps[r];
+    const uint32_t h = handles[i];
+    const uint32_t s = loc[R.topo_base + h];
+    const State st = cur.bins[s];
+    if (st.h != h || (st.tw & TW_DEAD) || tw_replica(st.tw) != r) {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    const uint32_t t = types[i];
+    cur.bins[s].tw = make_tw(t, (flavor[t] ? TW_TOPO : 0u) | (st.tw & TW_OUTSIDE), r);
 }
 
 }  // namespace rdy

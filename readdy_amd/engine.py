@@ -333,6 +333,21 @@ class Engine:
     def evaluate_topology_reactions(self, dt):
         L.check(self._lib.rdy_evaluate_topology_reactions(self._h, dt))
 
+    def topology_reacted(self, r=0):
+        """per topology, whether the last evaluate_topology_reactions changed it"""
+        n = ctypes.c_uint32()
+        L.check(self._lib.rdy_get_topology_reacted(self._h, r, 0, None, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint8)
+        L.check(self._lib.rdy_get_topology_reacted(self._h, r, n.value, out.ctypes.data_as(ctypes.POINTER(
+            ctypes.c_uint8)), ctypes.byref(n)))
+        return out[:n.value].astype(bool)
+
+    def set_particle_types(self, r, handles, types):
+        """new species of topology particles (handles below the replica's topology count)"""
+        h = np.ascontiguousarray(handles, dtype=np.uint32)
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        L.check(self._lib.rdy_set_particle_types(self._h, r, len(h), _ptr(h, _u32p), _ptr(t, _u32p)))
+
     def topology_reaction_counts(self, r=0, totals=False):
         n = getattr(self, "_n_sr", 0)
         c = np.zeros(max(n, 1), dtype=np.uint64)

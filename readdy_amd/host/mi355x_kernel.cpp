@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -80,17 +81,22 @@ public:
     bool reactionsChangeBondedTerms() const;
     // after a device loop: the topology types and species are read back when next asked for
     void markTopologiesStale();
+    // TopologyReactionActionFactory::createChangeParticleType on the device (topology particles only)
+    void changeParticleType(ParticleId id, ParticleTypeId type) override;
 
 private:
     void configureSpatialReactions();
     void uploadTopologyTypes();
     void syncTopologies();
+    void structuralReactions(scalar dt, const std::vector<uint8_t> &reacted);
+    void spatialReactions(scalar dt);
     MI355XKernel *_k;
     scalar _time = 0;
     std::vector<Particle> _topoParticles, _freeParticles;  // pending host copies (before the first flush)
     std::vector<std::unique_ptr<GraphTopology>> _topologies;
     bool _dirty = false, _uploaded = false, _topoDirty = false;
     bool _spatialSet = false, _topoStale = false;
+    uint64_t _structuralCalls = 0;  // keys the host draws of the structural reactions
 };
 
 // the context as the engine's C ABI describes it (arrays owned here)
@@ -164,6 +170,7 @@ public:
         _state.flush();
         if (_state.reactionsChangeBondedTerms()) return false;
         if (scheduler == "DetailedBalance") return false;  // a separate action (rdy_react only)
+        if (context().topologyRegistry().hasStructuralReactions()) return false;  // host recipes
         const int h = scheduler == "Gillespie" ? RDY_HANDLER_GILLESPIE
                       : scheduler == "UncontrolledApproximation" ? RDY_HANDLER_UNCONTROLLED_APPROXIMATION
                                                                  : RDY_HANDLER_NONE;
@@ -220,6 +227,7 @@ public:
     uint32_t replicaId() const { return _replica; }
     bool hasReplica() const { return _hasReplica; }
     void setSeed(uint64_t s) { _seed = s; }
+    uint64_t seed() const { return _seed; }
     void setNeighborListCutoff(scalar c) {
         if (_engine) throw std::logic_error("the neighbour-list cutoff is fixed once the engine exists");
         _nlCutoff = c;
@@ -355,7 +363,7 @@ void MI355XStateModel::configureSpatialReactions() {
 void MI355XStateModel::uploadTopologyTypes() {
     const auto &reg = _k->context().topologyRegistry();
     if (reg.spatialReactions().empty() || !_uploaded) return;
-    std::vector<uint32_t> types, of(_topoParticles.size(), 0);
+    std::vector<uint32_t> types, of(_topoParticles.size(), 0xFFFFFFFFu);  // UINT32_MAX: left its topology
     for (std::size_t t = 0; t < _topologies.size(); ++t) {
         types.push_back((uint32_t)reg.idOf(_topologies[t]->type()));
         for (auto i : _topologies[t]->particleIndices()) of.at(i) = (uint32_t)t;
@@ -397,12 +405,96 @@ bool MI355XStateModel::reactionsChangeBondedTerms() const {
     return false;
 }
 
-// CPUEvaluateTopologyReactions::perform (CPUEvaluateTopologyReactions.cpp:88-475), spatial reactions only
+// CPUEvaluateTopologyReactions::perform (CPUEvaluateTopologyReactions.cpp:88-475): the spatial reactions on
+// the device, then the structural reactions of the topologies they left alone
 void MI355XStateModel::evaluateTopologyReactions(scalar dt) {
     flush();
-    const auto &sr = _k->context().topologyRegistry().spatialReactions();
-    if (sr.empty() || !_k->hasReplica() || _topologies.empty()) return;
+    const auto &reg = _k->context().topologyRegistry();
+    const auto &sr = reg.spatialReactions();
+    if (!_k->hasReplica() || _topologies.empty()) return;
     if (_topoDirty) configureTopologies();
+    std::vector<uint8_t> reacted(_topologies.size(), 0);
+    if (!sr.empty()) {
+        spatialReactions(dt);
+        uint32_t n = 0;
+        ok(rdy_get_topology_reacted(_k->engine(), _k->replicaId(), (uint32_t)reacted.size(), reacted.data(), &n),
+           "topology reactions");
+    }
+    if (reg.hasStructuralReactions()) structuralReactions(dt, reacted);
+}
+
+// The structural reactions (CPUEvaluateTopologyReactions.cpp:88-220, executeStructuralReaction in
+// actions/Utils.h:52-80).  Each topology that no spatial reaction changed in this evaluation draws, for each
+// structural reaction of its type with rate r = rate(topology) > 0, the acceptance 1 - exp(-r dt) and an
+// Efraimidis-Spirakis key -ln(u) / r; the accepted reaction with the smallest key is performed (one topology
+// reaction per topology and evaluation, as performEvents' eventsDependent rule gives), topologies in key
+// order.  A topology that fell apart is replaced by its components (children of one normal-flavour particle
+// leave the topologies), one left as a single normal-flavour particle leaves them as well.  The reference
+// draws structural and spatial events from one performEvents list; here the spatial ones go first, which
+// differs only when both kinds are accepted for one topology in one step.
+void MI355XStateModel::structuralReactions(scalar dt, const std::vector<uint8_t> &reacted) {
+    const auto &reg = _k->context().topologyRegistry();
+    std::mt19937_64 gen(_k->seed() ^ (0x9E3779B97F4A7C15ull * ++_structuralCalls));
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    struct Ev {
+        double key;
+        std::size_t top, rx;
+    };
+    std::vector<Ev> events;
+    for (std::size_t ti = 0; ti < _topologies.size(); ++ti) {
+        GraphTopology &t = *_topologies[ti];
+        const auto &rs = reg.structuralReactionsOf(t.type());
+        if (rs.empty() || (ti < reacted.size() && reacted[ti])) continue;
+        t.updateReactionRates(rs);
+        Ev best{0.0, ti, SIZE_MAX};
+        for (std::size_t j = 0; j < rs.size(); ++j) {
+            const double r = t.rates()[j];
+            const double u = uni(gen), v = uni(gen);
+            if (!(r > 0) || !(u < 1.0 - std::exp(-r * dt))) continue;
+            const double key = -std::log(v + 0x1p-54) / r;
+            if (best.rx == SIZE_MAX || key < best.key) best = {key, ti, j};
+        }
+        if (best.rx != SIZE_MAX) events.push_back(best);
+    }
+    if (events.empty()) return;
+    std::sort(events.begin(), events.end(), [](const Ev &a, const Ev &b) { return a.key < b.key; });
+    std::vector<char> gone(_topologies.size(), 0);
+    std::vector<std::unique_ptr<GraphTopology>> appended;
+    for (const Ev &ev : events) {
+        GraphTopology &t = *_topologies[ev.top];
+        const auto &rx = reg.structuralReactionsOf(t.type()).at(ev.rx);
+        auto children = rx.execute(t, _k);
+        if (!children.empty()) {
+            gone[ev.top] = 1;
+            for (auto &c : children)
+                if (!c.isNormalParticle(*_k)) appended.push_back(std::make_unique<GraphTopology>(std::move(c)));
+        } else if (t.isNormalParticle(*_k)) {
+            gone[ev.top] = 1;
+        }
+    }
+    std::vector<std::unique_ptr<GraphTopology>> next;
+    for (std::size_t ti = 0; ti < _topologies.size(); ++ti)
+        if (!gone[ti]) next.push_back(std::move(_topologies[ti]));
+    for (auto &t : appended) next.push_back(std::move(t));
+    _topologies = std::move(next);
+    _topoDirty = true;
+    configureTopologies();
+}
+
+void MI355XStateModel::changeParticleType(ParticleId id, ParticleTypeId type) {
+    flush();
+    if (id >= _topoParticles.size())
+        throw std::invalid_argument("the MI355X kernel changes the species of topology particles only");
+    const uint32_t h = (uint32_t)id, t = (uint32_t)type;
+    ok(rdy_set_particle_types(_k->engine(), _k->replicaId(), 1, &h, &t), "change particle type");
+    syncTopologies();
+    const Particle &old = _topoParticles[id];
+    _topoParticles[id] = Particle(old.pos(), type, old.id());
+}
+
+// the spatial pass on the device, its species and topology types read back
+void MI355XStateModel::spatialReactions(scalar dt) {
+    const auto &sr = _k->context().topologyRegistry().spatialReactions();
     ok(rdy_evaluate_topology_reactions(_k->engine(), dt), "topology reactions");
     std::vector<uint64_t> c(sr.size(), 0);
     ok(rdy_get_topology_reaction_counts(_k->engine(), _k->replicaId(), 0, c.data()), "topology reaction counts");

@@ -20,8 +20,14 @@ namespace model {
 
 // ---- ParticleTypeRegistry (ParticleTypeRegistry.h:60-199) --------------------------------------
 void ParticleTypeRegistry::add(const std::string &name, scalar diffusionConstant, bool topologyFlavor) {
-    if (name.empty() || name.find_first_of(" \t\n") != std::string::npos)
-        throw std::invalid_argument("particle type name must be non-empty and contain no whitespace");
+    // validateTypeName (readdy/main/model/Utils.cpp:171-190): none of the descriptor tokens, no leading space
+    // (inner spaces are allowed, "Topology A"); an empty name is refused as well
+    for (const char *cs : {"[", "]", "(", ")", "->", "--", ":", "+"})
+        if (name.find(cs) != std::string::npos)
+            throw std::invalid_argument("Encountered invalid character sequence \"" + std::string(cs) +
+                                        "\" in type name \"" + name + "\"");
+    if (name.empty() || name[0] == ' ')
+        throw std::invalid_argument("Type name \"" + name + "\" is empty or contained leading/trailing whitespaces.");
     if (_byName.count(name)) throw std::invalid_argument("particle type \"" + name + "\" already exists");
     if (diffusionConstant < 0) throw std::invalid_argument("diffusion constant must be non-negative");
     const auto id = static_cast<ParticleTypeId>(_types.size());
@@ -493,6 +499,151 @@ std::vector<GraphTopology> GraphTopology::connectedComponents() const {
     return out;
 }
 
+bool GraphTopology::isConnected() const {
+    if (_particles.size() <= 1) return true;
+    return connectedComponents().size() == 1;
+}
+
+void GraphTopology::updateReactionRates(const std::vector<StructuralTopologyReaction> &reactions) {
+    _rates.clear();
+    for (const auto &r : reactions) _rates.push_back(r.rate(*this));
+}
+
+bool GraphTopology::isNormalParticle(const Kernel &kernel) const {
+    if (_particles.size() != 1) return false;
+    const ParticleId id = _particles[0];
+    for (const auto &p : kernel.stateModel().getParticles())
+        if (p.id() == id) return !kernel.context().particleTypes().infoOf(p.type()).topologyFlavor;
+    throw std::logic_error("a topology's particle is not in the state model");
+}
+
+// ---- structural topology reactions --------------------------------------------------------------------
+std::size_t StructuralTopologyReaction::counter = 0;
+
+StructuralTopologyReaction::StructuralTopologyReaction(std::string name, reaction_function reaction, rate_function rate)
+    : _name(std::move(name)), _reaction(std::move(reaction)), _rate(std::move(rate)), _id(counter++) {}
+
+StructuralTopologyReaction::StructuralTopologyReaction(std::string name, reaction_function reaction, scalar rate)
+    : StructuralTopologyReaction(std::move(name), std::move(reaction),
+                                 [rate](const GraphTopology &) -> scalar { return rate; }) {}
+
+Recipe &Recipe::changeParticleType(std::size_t vertex, ParticleTypeId to) {
+    Step st{};
+    st.kind = Step::ChangeParticleType;
+    st.v1 = vertex;
+    st.type = to;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::changeParticleType(std::size_t vertex, const std::string &to) {
+    Step st{};
+    st.kind = Step::ChangeParticleType;
+    st.v1 = vertex;
+    st.name = to;
+    st.byName = true;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::addEdge(std::size_t v1, std::size_t v2) {
+    Step st{};
+    st.kind = Step::AddEdge;
+    st.v1 = v1;
+    st.v2 = v2;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::removeEdge(std::size_t v1, std::size_t v2) {
+    Step st{};
+    st.kind = Step::RemoveEdge;
+    st.v1 = v1;
+    st.v2 = v2;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::separateVertex(std::size_t vertex) {  // Recipe.cpp:56-62: one edge removal per neighbour
+    for (std::size_t n : _topology->neighborsOf(vertex)) removeEdge(vertex, n);
+    return *this;
+}
+Recipe &Recipe::changeTopologyType(const std::string &type) {
+    Step st{};
+    st.kind = Step::ChangeTopologyType;
+    st.name = type;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::changeParticlePosition(std::size_t vertex, const Vec3 &pos) {
+    Step st{};
+    st.kind = Step::ChangeParticlePosition;
+    st.v1 = vertex;
+    st.pos = pos;
+    _steps.push_back(st);
+    return *this;
+}
+Recipe &Recipe::appendNewParticle(const std::vector<std::size_t> &neighbors, const std::string &type, const Vec3 &pos) {
+    Step st{};
+    st.kind = Step::AppendNewParticle;
+    st.neighbors = neighbors;
+    st.name = type;
+    st.pos = pos;
+    _steps.push_back(st);
+    return *this;
+}
+
+std::vector<GraphTopology> StructuralTopologyReaction::execute(GraphTopology &topology, Kernel *kernel) const {
+    const auto &types = kernel->context().particleTypes();
+    const auto &registry = kernel->context().topologyRegistry();
+    Recipe recipe = operations(topology);
+    const auto &steps = recipe.steps();
+    if (steps.empty()) return {};
+    TopologyActionFactory *taf = kernel->getTopologyActionFactory();
+    if (!taf) throw std::logic_error("the kernel has no topology action factory");
+    std::vector<TopologyActionFactory::ActionPtr> actions;
+    for (const auto &st : steps) {
+        switch (st.kind) {
+        case Recipe::Step::ChangeParticleType:
+            actions.push_back(taf->createChangeParticleType(&topology, st.v1, st.byName ? types.idOf(st.name) : st.type));
+            break;
+        case Recipe::Step::AddEdge: actions.push_back(taf->createAddEdge(&topology, st.v1, st.v2)); break;
+        case Recipe::Step::RemoveEdge:
+        case Recipe::Step::SeparateVertex: actions.push_back(taf->createRemoveEdge(&topology, st.v1, st.v2)); break;
+        case Recipe::Step::ChangeTopologyType: actions.push_back(taf->createChangeTopologyType(&topology, st.name)); break;
+        case Recipe::Step::ChangeParticlePosition:
+        case Recipe::Step::AppendNewParticle:
+            throw std::invalid_argument("structural reaction \"" + _name +
+                                        "\": moving or appending particles is not available on this kernel");
+        }
+    }
+    for (auto &a : actions) a->perform();
+    if (expects_connected_after_reaction()) {
+        bool valid = topology.isConnected();
+        for (std::size_t id : topology.particleIndices())
+            for (const auto &p : kernel->stateModel().getParticles())
+                if (p.id() == id && !types.infoOf(p.type()).topologyFlavor) valid = false;
+        if (!valid)
+            throw TopologyReactionException("The topology was invalid after the reaction, see previous warning messages.");
+        kernel->stateModel().configureTopologies();
+        topology.updateReactionRates(registry.structuralReactionsOf(topology.type()));
+        return {};
+    }
+    if (!topology.isConnected()) return topology.connectedComponents();
+    if (!topology.isNormalParticle(*kernel)) {
+        kernel->stateModel().configureTopologies();
+        topology.updateReactionRates(registry.structuralReactionsOf(topology.type()));
+    }
+    return {};
+}
+
+void TopologyRegistry::addStructuralReaction(const std::string &type, StructuralTopologyReaction reaction) {
+    if (!hasType(type)) throw std::invalid_argument("unknown topology type \"" + type + "\"");
+    _structural[type].push_back(std::move(reaction));
+}
+
+const std::vector<StructuralTopologyReaction> &TopologyRegistry::structuralReactionsOf(const std::string &type) const {
+    static const std::vector<StructuralTopologyReaction> none;
+    auto it = _structural.find(type);
+    return it == _structural.end() ? none : it->second;
+}
+
 std::string DetailedBalance::describe() const {  // Actions.cpp:521-545
     std::string s;
     if (_reversibleReactions.empty()) return s;
@@ -919,7 +1070,9 @@ void SimulationLoop::run(std::size_t steps) {
     const auto &obs = _kernel->connectedObservables();
     const observables::TimeStep t0 = _t;
     // the device loop performs the registered topology reactions in every step
-    const bool fusable = _topologyReactions || _kernel->context().topologyRegistry().spatialReactions().empty();
+    const auto &treg = _kernel->context().topologyRegistry();
+    const bool fusable = (_topologyReactions || treg.spatialReactions().empty()) &&
+                         !(_topologyReactions && treg.hasStructuralReactions());
     if (_callbacks.empty() && _forces && fusable) {
         if (obs.empty()) {
             if (_kernel->runFused(scheduler, _dt, steps)) {
@@ -970,7 +1123,7 @@ void SimulationLoop::run(std::size_t steps) {
                 : scheduler == "DetailedBalance" ? std::unique_ptr<model::TimeStepDependentAction>(
                                                        actions.detailedBalance(_dt))
                                                  : actions.uncontrolledApproximation(_dt);
-    if (_topologyReactions && !_kernel->context().topologyRegistry().spatialReactions().empty())
+    if (_topologyReactions && (!treg.spatialReactions().empty() || treg.hasStructuralReactions()))
         topoReact = actions.evaluateTopologyReactions(_dt);
     nl->perform();
     if (_forces) forces->perform();

@@ -166,6 +166,8 @@ struct SpatialReactionDesc {
     bool isTopologyParticle() const { return mode == STRMode::TP_ENZYMATIC || mode == STRMode::TP_FUSION; }
 };
 
+class StructuralTopologyReaction;
+
 class TopologyRegistry {
 public:
     static constexpr std::size_t EmptyTopologyId = static_cast<std::size_t>(-1);
@@ -183,6 +185,10 @@ public:
     void addSpatialReaction(const std::string &descriptor, scalar rate, scalar radius);
     const std::vector<SpatialReactionDesc> &spatialReactions() const { return _spatial; }
     bool isSpatialReactionType(ParticleTypeId t) const;
+    // TopologyRegistry::addStructuralReaction (TopologyRegistry.h:108-122): a reaction of one topology type
+    void addStructuralReaction(const std::string &type, StructuralTopologyReaction reaction);
+    const std::vector<StructuralTopologyReaction> &structuralReactionsOf(const std::string &type) const;
+    bool hasStructuralReactions() const { return !_structural.empty(); }
     void rebind(const ParticleTypeRegistry *types) { _types = types; }  // after a Context copy
     void configureBondPotential(const std::string &t1, const std::string &t2, BondConfiguration cfg);
     void configureAnglePotential(const std::string &t1, const std::string &t2, const std::string &t3,
@@ -200,6 +206,7 @@ private:
     const ParticleTypeRegistry *_types;
     std::vector<std::string> _topologyTypes;
     std::vector<SpatialReactionDesc> _spatial;
+    std::map<std::string, std::vector<StructuralTopologyReaction>> _structural;
 };
 
 class Context {
@@ -257,6 +264,8 @@ private:
     ParticleId _id;
 };
 
+class Kernel;
+
 // A linear or branched topology: its particles and graph edges (GraphTopology.h; the graph
 // library's findNTuples enumerates its bonds, angles and dihedrals, Graph_detail.h:39-78)
 class GraphTopology {
@@ -273,6 +282,13 @@ public:
     const std::vector<std::size_t> &particleIndices() const { return _particles; }
     const std::string &type() const { return _type; }
     void setType(std::string type) { _type = std::move(type); }
+    const std::vector<std::size_t> &neighborsOf(std::size_t v) const { return _neighbors.at(v); }
+    bool isConnected() const;  // Graph::isConnected (a topology of one vertex is connected)
+    // the structural reactions' rates for this topology (GraphTopology::updateReactionRates)
+    void updateReactionRates(const std::vector<StructuralTopologyReaction> &reactions);
+    const std::vector<scalar> &rates() const { return _rates; }
+    // one particle of normal flavour: no longer a topology (GraphTopology::isNormalParticle)
+    bool isNormalParticle(const Kernel &kernel) const;
     // visits every edge, 3-path and 4-path once, in the reference's enumeration order
     void findNTuples(const std::function<void(std::size_t, std::size_t)> &pair,
                      const std::function<void(std::size_t, std::size_t, std::size_t)> &triple,
@@ -282,6 +298,79 @@ private:
     std::string _type;
     std::vector<std::size_t> _particles;
     std::vector<std::vector<std::size_t>> _neighbors;
+    std::vector<scalar> _rates;
+};
+
+// ---- structural topology reactions (StructuralTopologyReaction.h:58-220, Recipe.h:40-132) -------
+// thrown when a reaction leaves a topology that was to stay connected disconnected or with particles
+// of normal flavour (TopologyReactionException.h)
+class TopologyReactionException : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
+};
+
+// the operations a structural reaction applies to its topology, in order; vertices are indices into
+// the topology's particleIndices()
+class Recipe {
+public:
+    struct Step {
+        enum Kind { ChangeParticleType, AddEdge, RemoveEdge, SeparateVertex, ChangeTopologyType, ChangeParticlePosition,
+                    AppendNewParticle };
+        Kind kind;
+        std::size_t v1 = 0, v2 = 0;
+        ParticleTypeId type = 0;
+        std::string name;  // a species name to resolve, or the new topology type
+        bool byName = false;
+        Vec3 pos;
+        std::vector<std::size_t> neighbors;
+    };
+    explicit Recipe(GraphTopology &topology) : _topology(&topology) {}
+    Recipe &changeParticleType(std::size_t vertex, ParticleTypeId to);
+    Recipe &changeParticleType(std::size_t vertex, const std::string &to);
+    Recipe &addEdge(std::size_t v1, std::size_t v2);
+    Recipe &removeEdge(std::size_t v1, std::size_t v2);
+    Recipe &separateVertex(std::size_t vertex);  // removes every edge of the vertex (Recipe.cpp:56-62)
+    Recipe &changeTopologyType(const std::string &type);
+    // in the reference's recipe; a kernel that cannot move or create particles in a topology throws when
+    // the reaction executes
+    Recipe &changeParticlePosition(std::size_t vertex, const Vec3 &pos);
+    Recipe &appendNewParticle(const std::vector<std::size_t> &neighbors, const std::string &type, const Vec3 &pos);
+    const std::vector<Step> &steps() const { return _steps; }
+    GraphTopology &topology() { return *_topology; }
+    const GraphTopology &topology() const { return *_topology; }
+
+private:
+    GraphTopology *_topology;
+    std::vector<Step> _steps;
+};
+
+class StructuralTopologyReaction {
+public:
+    using reaction_function = std::function<Recipe(GraphTopology &)>;
+    using rate_function = std::function<scalar(const GraphTopology &)>;
+    StructuralTopologyReaction(std::string name, reaction_function reaction, rate_function rate);
+    StructuralTopologyReaction(std::string name, reaction_function reaction, scalar rate);
+    scalar rate(const GraphTopology &topology) const { return _rate(topology); }
+    Recipe operations(GraphTopology &topology) const { return _reaction(topology); }
+    // Mode (StructuralTopologyReaction.h:58-70): by default the topology may fall apart into children
+    bool expects_connected_after_reaction() const { return _expectConnected; }
+    void expect_connected_after_reaction() { _expectConnected = true; }
+    bool creates_child_topologies_after_reaction() const { return !_expectConnected; }
+    void create_child_topologies_after_reaction() { _expectConnected = false; }
+    const std::string &name() const { return _name; }
+    std::size_t id() const { return _id; }
+    // StructuralTopologyReaction::execute (StructuralTopologyReaction.cpp:61-131): the recipe through the
+    // kernel's topology action factory, then the post-reaction checks.  Returns the connected components
+    // when the graph fell apart (create-children mode); otherwise empty, the topology reconfigured.
+    std::vector<GraphTopology> execute(GraphTopology &topology, Kernel *kernel) const;
+
+private:
+    static std::size_t counter;
+    std::string _name;
+    reaction_function _reaction;
+    rate_function _rate;
+    bool _expectConnected = false;
+    std::size_t _id;
 };
 
 // ---- actions -----------------------------------------------------------------------------------

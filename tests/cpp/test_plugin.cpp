@@ -173,6 +173,155 @@ static void test_detailed_balance_configs() {
     std::printf("detailed balance configs: ok\n");
 }
 
+// Structural topology reactions through the plugin: the reference's scenes (readdy/test/TestTopologyReactions.cpp:
+// 82-420) with the recipes executed on the device's topology particles
+static void structural_context(model::Context &c) {
+    c.boxSize() = {10., 10., 10.};
+    c.particleTypes().add("Topology A", 1.0, true);
+    c.particleTypes().add("Topology B", 1.0, true);
+    c.particleTypes().add("Topology Invalid Type", 1.0, true);
+    c.particleTypes().add("A", 1.0);
+    c.topologyRegistry().configureBondPotential("Topology A", "Topology A", {10, 10});
+    c.topologyRegistry().configureBondPotential("Topology A", "Topology B", {10, 10});
+    c.topologyRegistry().configureBondPotential("Topology B", "Topology B", {10, 10});
+}
+
+static model::GraphTopology *small_topology(model::Kernel *k, const std::string &type) {
+    const auto a = k->context().particleTypes().idOf("Topology A");
+    auto *t = k->stateModel().addTopology(type, {model::Particle(0, 0, 0, a), model::Particle(0, 0, 0, a),
+                                                 model::Particle(0, 0, 0, a)});
+    t->addEdge(0, 1);
+    t->addEdge(1, 2);
+    return t;
+}
+
+static ParticleTypeId type_of(model::Kernel *k, ParticleId id) {
+    for (const auto &p : k->stateModel().getParticles())
+        if (p.id() == id) return p.type();
+    return (ParticleTypeId)-1;
+}
+
+static void test_structural_reactions() {
+    using model::Recipe;
+    using model::StructuralTopologyReaction;
+    {  // mode flags (:96-114)
+        StructuralTopologyReaction r("r", [](model::GraphTopology &t) { return Recipe(t); }, 0.);
+        r.expect_connected_after_reaction();
+        CHECK(r.expects_connected_after_reaction() && !r.creates_child_topologies_after_reaction(), "mode flags");
+        r.create_child_topologies_after_reaction();
+        CHECK(!r.expects_connected_after_reaction() && r.creates_child_topologies_after_reaction(), "mode flags");
+    }
+    {  // change a particle type, constant and function rates (:116-162)
+        auto k = plugin::KernelProvider::getInstance().create("MI355X");
+        auto &c = k->context();
+        structural_context(c);
+        c.topologyRegistry().addType("Reactive Topology Type");
+        auto *top = small_topology(k.get(), "Reactive Topology Type");
+        const auto B = c.particleTypes().idOf("Topology B"), A = c.particleTypes().idOf("Topology A");
+        StructuralTopologyReaction r1("r", [&](model::GraphTopology &t) { return Recipe(t).changeParticleType(0, B); }, 5.);
+        r1.expect_connected_after_reaction();
+        StructuralTopologyReaction r2("r2", [&](model::GraphTopology &t) { return Recipe(t).changeParticleType(0, A); },
+                                      [](const model::GraphTopology &) { return 15.; });
+        r2.expect_connected_after_reaction();
+        c.topologyRegistry().addStructuralReaction("Reactive Topology Type", r1);
+        c.topologyRegistry().addStructuralReaction("Reactive Topology Type", r2);
+        k->actions().updateNeighborList()->perform();  // uploads (the particles coincide: no forces)
+        const auto &rs = c.topologyRegistry().structuralReactionsOf("Reactive Topology Type");
+        top->updateReactionRates(rs);
+        CHECK(top->rates().size() == 2 && top->rates()[0] == 5 && top->rates()[1] == 15, "rates 5 and 15");
+        CHECK(rs.at(0).execute(*top, k.get()).empty(), "in-place reaction returns no children");
+        CHECK(type_of(k.get(), top->particleIndices()[0]) == B, "vertex 0 became Topology B");
+        CHECK(rs.at(1).execute(*top, k.get()).empty(), "in-place reaction returns no children");
+        CHECK(type_of(k.get(), top->particleIndices()[0]) == A, "vertex 0 is Topology A again");
+    }
+    {  // add an edge, remove an edge into children (:163-245)
+        auto k = plugin::KernelProvider::getInstance().create("MI355X");
+        auto &c = k->context();
+        structural_context(c);
+        c.topologyRegistry().addType("TA");
+        auto *top = small_topology(k.get(), "TA");
+        StructuralTopologyReaction add("add", [](model::GraphTopology &t) { return Recipe(t).addEdge(0, 2); }, 5.);
+        add.expect_connected_after_reaction();
+        StructuralTopologyReaction rem("rem", [](model::GraphTopology &t) { return Recipe(t).removeEdge(0, 1); }, 5.);
+        k->actions().updateNeighborList()->perform();
+        CHECK(add.execute(*top, k.get()).empty() && top->hasEdge(0, 2), "edge 0--2 added");
+        CHECK(top->removeEdge(0, 2), "edge removed again");
+        const auto parts = top->particleIndices();
+        auto children = rem.execute(*top, k.get());
+        CHECK(children.size() == 2, "two children after removing the edge");
+        if (children.size() == 2) {
+            CHECK(children[0].nParticles() == 1 && children[1].nParticles() == 2, "children sizes 1 and 2");
+            CHECK(children[1].hasEdge(0, 1) && children[1].neighborsOf(0).size() == 1, "the second child keeps one edge");
+            CHECK(children[0].particleIndices()[0] == parts[0] && children[1].particleIndices()[0] == parts[1] &&
+                      children[1].particleIndices()[1] == parts[2],
+                  "children keep their particles");
+        }
+    }
+    for (int decay = 0; decay < 2; ++decay) {  // a 50-chain split up, with and without decay (:247-420)
+        auto k = plugin::KernelProvider::getInstance().create("MI355X");
+        auto &c = k->context();
+        structural_context(c);
+        c.topologyRegistry().addType("TA");
+        const auto A = c.particleTypes().idOf("Topology A");
+        std::vector<model::Particle> ps;
+        for (int i = 0; i < 50; ++i) ps.emplace_back(-5 + i * 10. / 50., 0, 0, A);
+        auto *top = k->stateModel().addTopology("TA", ps);
+        for (int i = 0; i + 1 < 50; ++i) top->addEdge(i, i + 1);
+        auto gen = std::make_shared<std::mt19937>(11 + decay);
+        StructuralTopologyReaction split(
+            "r",
+            [gen](model::GraphTopology &t) {
+                Recipe recipe(t);
+                const auto n = t.nParticles();
+                if (n > 1) {  // a random edge of the chain: vertices are kept in chain order
+                    std::vector<std::pair<std::size_t, std::size_t>> edges;
+                    for (std::size_t v = 0; v < n; ++v)
+                        for (std::size_t w : t.neighborsOf(v))
+                            if (v < w) edges.emplace_back(v, w);
+                    if (!edges.empty()) {
+                        const auto e = edges[std::uniform_int_distribution<std::size_t>(0, edges.size() - 1)(*gen)];
+                        recipe.removeEdge(e.first, e.second);
+                    }
+                }
+                return recipe;
+            },
+            [](const model::GraphTopology &t) { return t.nParticles() > 1 ? t.nParticles() / 50. : 0.; });
+        c.topologyRegistry().addStructuralReaction("TA", split);
+        if (decay) {
+            const auto normal = c.particleTypes().idOf("A");
+            StructuralTopologyReaction dec(
+                "r2",
+                [normal](model::GraphTopology &t) {
+                    if (t.nParticles() != 1) throw std::logic_error("decay of a topology with more than one particle");
+                    return Recipe(t).changeParticleType(0, normal);
+                },
+                [](const model::GraphTopology &t) { return t.nParticles() > 1 ? 0. : 1.; });
+            c.topologyRegistry().addStructuralReaction("TA", dec);
+        }
+        auto integrator = k->actions().eulerBDIntegrator(1.0);
+        auto forces = k->actions().calculateForces();
+        auto topReactions = k->actions().evaluateTopologyReactions(1.0);
+        forces->perform();
+        for (int t = 1; t < 500; ++t) {
+            integrator->perform();
+            topReactions->perform();
+            forces->perform();
+        }
+        const auto tops = k->stateModel().getTopologies();
+        std::size_t nA = 0;
+        for (const auto &p : k->stateModel().getParticles()) nA += p.type() == c.particleTypes().idOf("A");
+        if (decay) {
+            CHECK(tops.empty(), "every single particle decayed out of the topologies (%zu left)", tops.size());
+            CHECK(nA == 50, "50 normal particles (%zu)", nA);
+        } else {
+            bool single = tops.size() == 50;
+            for (auto *t : tops) single = single && t->nParticles() == 1;
+            CHECK(single, "the chain split into 50 one-particle topologies (%zu topologies)", tops.size());
+        }
+    }
+    std::printf("structural topology reactions: ok\n");
+}
+
 // DetailedBalance on the device through the plugin: the reference's integration scenes
 // (TestDetailedBalance.cpp:379-462): a conversion A -> B next to 500 D particles is accepted when A
 // repels D and rejected when B does
@@ -908,6 +1057,7 @@ int main(int argc, char **argv) {
         test_detailed_balance_configs();
         if (!provider_only) {
             test_detailed_balance_scenes();
+            test_structural_reactions();
             test_c0_forces_and_loop(argv[1]);
             test_observables();
             test_topology();

@@ -47,6 +47,30 @@ def test_scene_matches_the_oracle():
     assert list(e.topology_reaction_counts(r)) == [0] and list(e.topology_reaction_counts(r, totals=True)) == [2]
 
 
+def test_reacted_topologies_and_topology_particle_species():
+    """the topologies a spatial evaluation changed (the structural reactions of the same evaluation skip them),
+    only for that evaluation; topology particles take new species by handle, their topology flag from the
+    species' flavour (a normal species takes a particle out of the topology reactions)"""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [6, 0, 0], [0.5, 0.5, 0], [0, 3, 0]], float)
+    e, r = _engine(20.0)
+    e.set_particles(r, pos, np.array([0, 0, 0, 1, 1], np.uint32))
+    e.set_spatial_reactions(_sr())
+    e.set_topologies(r, [0, 0, 0], [0, 1, 2])
+    e.evaluate_topology_reactions(1.0)
+    reacted = e.topology_reacted(r)
+    assert reacted.sum() == 1 and list(e.topology_types(r)[reacted]) == [1]
+    e.evaluate_topology_reactions(1.0)  # nothing left to react
+    assert not e.topology_reacted(r).any()
+    e.set_particle_types(r, [2], [3])  # a topology species
+    e.set_particle_types(r, [1], [1])  # a normal species: now a free partner for topology 0 or 2
+    _, t, _ = e.get_particles(r)
+    assert t[2] == 3 and t[1] == 1
+    with pytest.raises(L.EngineError, match="topology particles"):
+        e.set_particle_types(r, [3], [0])  # a free particle
+    with pytest.raises(L.EngineError, match="type out of range"):
+        e.set_particle_types(r, [0], [9])
+
+
 def test_shared_free_particle_and_strict_gate():
     e, r = _engine(20.0, radius=1.25, nl_cutoff=1.5)
     # topology particles first (handles 0-2), then the free ones (3: shared by topologies 0 and 1, 4: exactly
