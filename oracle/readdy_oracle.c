@@ -1599,7 +1599,10 @@ typedef struct {
 
 static double rev_integrand(const rev_f *f, double r) {
     const double u = f->any ? rev_u(f->s, f->a, f->b, r) : 0.;
-    return 4. * 3.14159265358979323846 * r * r * exp(-1. * u / f->s->kbt);
+    const double v = 4. * 3.14159265358979323846 * r * r * exp(-1. * u / f->s->kbt);
+    /* r = 0 under a repulsive core (LJ, SE: inf - inf): the integrand's limit there is 0; the reference's
+     * Gauss-Kronrod nodes never reach the end points */
+    return isfinite(v) ? v : 0.;
 }
 
 static double simpson_rec(const rev_f *f, double a, double b, double fa, double fm, double fb, double whole, double tol,
@@ -1608,12 +1611,18 @@ static double simpson_rec(const rev_f *f, double a, double b, double fa, double 
     const double flm = rev_integrand(f, lm), frm = rev_integrand(f, rm);
     const double left = (m - a) / 6. * (fa + 4. * flm + fm), right = (b - m) / 6. * (fm + 4. * frm + fb);
     const double d = left + right - whole;
-    if (depth <= 0 || fabs(d) <= 15. * tol) return left + right + d / 15.;
+    /* converged, or the difference is at the rounding level of the panel (a steep wall of the potential
+     * would otherwise recurse to the depth limit everywhere around it) */
+    if (depth <= 0 || !(fabs(d) > 15. * tol) || fabs(d) <= 64. * 2.220446049250313e-16 * fabs(left + right) ||
+        fabs(d) < 1e-290)  /* subnormal values carry no relative precision */
+        return left + right + d / 15.;
     return simpson_rec(f, a, m, fa, flm, fm, left, .5 * tol, depth - 1) +
            simpson_rec(f, m, b, fm, frm, fb, right, .5 * tol, depth - 1);
 }
 
-static double rev_integrate(const rev_f *f, double a, double b) {
+/* the integral over [a, b] to an absolute tolerance of 1e-13 times scale (scale <= 0: the integral's own
+ * coarse estimate) */
+static double rev_integrate_scaled(const rev_f *f, double a, double b, double scale) {
     if (!(a < b)) return 0.;
     /* a first pass on 64 panels fixes the scale of the absolute tolerance */
     double coarse = 0.;
@@ -1622,7 +1631,7 @@ static double rev_integrate(const rev_f *f, double a, double b) {
         const double x0 = a + (b - a) * i / np, x1 = a + (b - a) * (i + 1) / np;
         coarse += (x1 - x0) / 6. * (rev_integrand(f, x0) + 4. * rev_integrand(f, .5 * (x0 + x1)) + rev_integrand(f, x1));
     }
-    const double tol = 1e-13 * fabs(coarse) / np;
+    const double tol = 1e-13 * (scale > 0. ? scale : fabs(coarse)) / np;
     double total = 0.;
     for (int i = 0; i < np; ++i) {
         const double x0 = a + (b - a) * i / np, x1 = a + (b - a) * (i + 1) / np;
@@ -1631,6 +1640,8 @@ static double rev_integrate(const rev_f *f, double a, double b) {
     }
     return total;
 }
+
+static double rev_integrate(const rev_f *f, double a, double b) { return rev_integrate_scaled(f, a, b, 0.); }
 
 static int n_educts_of(int kind) { return (kind == OR_RX_FUSION || kind == OR_RX_ENZYMATIC) ? 2 : 1; }
 static int n_products_of(int kind) {
@@ -1716,8 +1727,12 @@ static int rev_configure(const or_system *s, int fwd, int bwd, rev_t *c) {
             c->radii[i] = r;
             r += dr;
         }
+        /* cumulative panels, each to a tolerance relative to the whole range (the reference integrates
+         * each cumulative value from 0 to a relative 1e-12) */
+        const double whole = c->eff_lhs_rx;
         c->cdf[0] = 0.;
-        for (int i = 1; i <= OR_FISSION_POINTS; ++i) c->cdf[i] = c->cdf[i - 1] + rev_integrate(&fl, c->radii[i - 1], c->radii[i]);
+        for (int i = 1; i <= OR_FISSION_POINTS; ++i)
+            c->cdf[i] = c->cdf[i - 1] + rev_integrate_scaled(&fl, c->radii[i - 1], c->radii[i], whole);
         const double last = c->cdf[OR_FISSION_POINTS];
         if (last >= 1e-16) {
             for (int i = 0; i <= OR_FISSION_POINTS; ++i) c->cdf[i] /= last;

@@ -149,3 +149,46 @@ def test_detailed_balance_is_not_a_step_handler():
         e.set_handler(r, L.HANDLER_DETAILED_BALANCE)
     e.react(L.HANDLER_DETAILED_BALANCE, 0.1)
     assert e.n_particles(r) == 1
+
+
+# the pair potentials of every kind in the Metropolis energies (LJ: m, n, cutoff, shift, epsilon, sigma; SE:
+# C, kappa, D, sigma, exponent, cutoff; WIPH: k, d0, depth, cutoff), strengths depending on the type pair so
+# that every move changes the energy, on a jittered lattice so no pair overlaps
+POTENTIAL_CASES = {
+    "weak_interaction": (L.POT_WEAK_INTERACTION, lambda s: [10., 1.0, 0.4 + 0.2 * s, 1.6]),
+    "lennard_jones": (L.POT_LENNARD_JONES, lambda s: [12., 6., 1.6, 1., 0.3 + 0.1 * s, 0.8]),
+    "lennard_jones_unshifted": (L.POT_LENNARD_JONES, lambda s: [12., 6., 1.6, 0., 0.3 + 0.1 * s, 0.8]),
+    "screened_electrostatics": (L.POT_SCREENED_ELECTROSTATICS, lambda s: [-0.5 - 0.25 * s, 1.0, 0.5, 0.8, 6., 1.6]),
+}
+
+
+@pytest.mark.parametrize("case", list(POTENTIAL_CASES))
+def test_every_potential_kind_in_the_metropolis_energy(case):
+    kind, params = POTENTIAL_CASES[case]
+    pots = [(kind, a, b, params(a + b)) for a in range(4) for b in range(a, 4)]
+    rxs = [fusion(A, B, C, 50., 1.4), fission(C, A, B, 30., 1.4), conversion(A, D, 20.), conversion(D, A, 20.),
+           enzymatic(B, C, D, 40., 1.4), enzymatic(D, C, B, 40., 1.4)]
+    m, a0, Lb = 8, 1.25, 10.0
+    g = np.array([(i, j, k) for i in range(m) for j in range(m) for k in range(m)], float) * a0 - Lb / 2 + 0.1
+    rng = np.random.default_rng(3)
+    pos = g + rng.uniform(-0.15, 0.15, g.shape)
+    ty = rng.integers(0, 4, len(pos)).astype(np.uint32)
+    n = len(pos)
+    e = Engine(4, [1.] * 4, potentials=pots, reactions=rxs)
+    r = e.add_replica([Lb] * 3, (1, 1, 1), seed=13, max_particles=4 * n)
+    e.set_particles(r, pos, ty)
+    e.record_reactions(True)
+    e.react(L.HANDLER_DETAILED_BALANCE, 0.05)
+    s = oracle_system(4, [Lb] * 3, pots, rxs)
+    s.add_particles(pos, ty.astype(np.int32), ids=np.arange(n))
+    s.set_id_counter(n)
+    s.set_seed(13)
+    s.nl_setup(s.max_cutoff())
+    s.set_record_reactions(True)
+    done = s.detailed_balance(0.05, NO_INTEGRATION)
+    _compare(e, r, s, Lb, 1e-12)
+    assert np.array_equal(e.reaction_counts(r), s.reaction_counts()[:len(rxs)])
+    _compare_records(e, r, s)
+    # some moves were rejected (ids of rejected moves' educts advanced) and some accepted
+    _, _, ids, _ = s.particles()
+    assert done > 10 and int(ids.max()) > n
