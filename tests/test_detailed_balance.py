@@ -352,3 +352,26 @@ def test_oracle_without_potentials_equals_gillespie():
         out.append((n, s.reaction_counts().tolist(), np.bincount(t, minlength=4).tolist()))
     assert out[0] == out[1]
     assert out[0][0] > 30
+
+
+@pytest.mark.parametrize("kind,params", [
+    (L.POT_WEAK_INTERACTION, [10., 1.0, 0.8, 1.6]),
+    (L.POT_LENNARD_JONES, [12., 6., 1.6, 1., 0.5, 0.8]),
+    (L.POT_LENNARD_JONES, [12., 6., 1.6, 0., 0.5, 0.8]),
+    (L.POT_SCREENED_ELECTROSTATICS, [-1.0, 1.0, 0.5, 0.8, 6., 1.6]),
+])
+def test_configs_of_every_potential_kind(kind, params):
+    """the effective volumes of each potential kind: the C ABI (adaptive Gauss-Kronrod) against the oracle
+    (adaptive Simpson) and scipy's quadrature of 4 pi r^2 exp(-U(r)) with the oracle's U"""
+    pots = [(kind, A, B, params), (kind, B, C, params)]
+    rxs = [fusion(A, B, C, 1., 1.4), fission(C, A, B, 2., 1.4), enzymatic(A, C, D, 1., 1.4),
+           enzymatic(D, C, A, 2., 1.4)]
+    prod, orc = both(4, [10.] * 3, pots, rxs)
+    assert [p["kind"] for p in prod] == [0, 2] == [o["kind"] for o in orc]
+    u = lambda r: po.pair_force_energy(kind, params, [r, 0., 0.])[1]
+    f = lambda r: 4 * math.pi * r * r * math.exp(-u(r)) if r > 0 else 0.
+    want = si.quad(f, 1e-9, 1.4, epsabs=0, epsrel=1e-12, limit=500)[0]
+    for p, o in zip(prod, orc):
+        assert approx(p["eff_lhs_reaction_volume"], o["eff_lhs_reaction"], 1e-9)
+        assert approx(p["eff_lhs_interaction_volume"], o["eff_lhs_interaction"], 1e-9)
+    assert approx(prod[0]["eff_lhs_reaction_volume"], want, 1e-8)
