@@ -36,39 +36,43 @@ class Recorder:
         self._flush_stride = 0
         self._flavors = None
 
-    def _add(self, kind, name, stride, **args):
+    def _add(self, kind, name, stride, callback=None, **args):
         if any(o["name"] == name and o["group"] == _group(kind) for o in self._obs):
             raise ValueError(f"an observable named {name!r} is already recorded")
         if kind == "reactions":
             self.e.record_reactions(True)
         self._obs.append(dict(kind=kind, name=name, group=_group(kind), stride=int(stride), args=args, time=[],
-                              values=[]))
+                              values=[], callback=callback))
 
-    def n_particles(self, stride, name="n_particles"):
-        self._add("n_particles", name, stride)
+    def n_particles(self, stride, name="n_particles", types=None, callback=None):
+        """particles per type; types: the listed type ids only, in that order, or (an empty list) their total
+        (CPUNParticles::evaluate); None: every type.  callback(value) runs at each evaluation."""
+        self._add("n_particles", name, stride, callback, types=None if types is None else list(types))
 
-    def reaction_counts(self, stride, name="reaction_counts"):
-        self._add("reaction_counts", name, stride)
+    def reaction_counts(self, stride, name="reaction_counts", callback=None):
+        self._add("reaction_counts", name, stride, callback)
 
-    def reactions(self, stride, name="reactions"):
-        self._add("reactions", name, stride)
+    def reactions(self, stride, name="reactions", callback=None):
+        self._add("reactions", name, stride, callback)
 
-    def radial_distribution(self, stride, borders, from_types, to_types, density, name="radial_distribution"):
-        self._add("rdf", name, stride, borders=np.asarray(borders, dtype=np.float64), frm=list(from_types),
+    def radial_distribution(self, stride, borders, from_types, to_types, density, name="radial_distribution",
+                            callback=None):
+        self._add("rdf", name, stride, callback, borders=np.asarray(borders, dtype=np.float64), frm=list(from_types),
                   to=list(to_types), density=float(density))
 
-    def histogram_along_axis(self, stride, borders, types, axis, name="histogram_along_axis"):
-        self._add("histogram", name, stride, borders=np.asarray(borders, dtype=np.float64), types=list(types),
-                  axis=int(axis))
+    def histogram_along_axis(self, stride, borders, types, axis, name="histogram_along_axis", callback=None):
+        self._add("histogram", name, stride, callback, borders=np.asarray(borders, dtype=np.float64),
+                  types=list(types), axis=int(axis))
 
-    def positions(self, stride, name="positions"):
-        self._add("positions", name, stride)
+    def positions(self, stride, name="positions", types=None, callback=None):
+        """positions ordered by id; types: only particles of these type ids"""
+        self._add("positions", name, stride, callback, types=None if types is None else list(types))
 
-    def forces(self, stride, name="forces"):
-        self._add("forces", name, stride)
+    def forces(self, stride, name="forces", types=None, callback=None):
+        self._add("forces", name, stride, callback, types=None if types is None else list(types))
 
-    def particles(self, stride, name="particles"):
-        self._add("particles", name, stride)
+    def particles(self, stride, name="particles", callback=None):
+        self._add("particles", name, stride, callback)
 
     def trajectory(self, stride, name=""):
         self._add("trajectory", name, stride)
@@ -88,6 +92,8 @@ class Recorder:
             k, a = o["kind"], o["args"]
             if k == "n_particles":
                 v = e.count_types(r + 1)[r].copy()
+                if a.get("types") is not None:  # the listed types, or their total when none is listed
+                    v = v[a["types"]] if a["types"] else np.array([v.sum()], dtype=v.dtype)
             elif k == "reaction_counts":
                 v = e.reaction_counts(r).copy()
             elif k == "reactions":
@@ -97,10 +103,13 @@ class Recorder:
             elif k == "histogram":
                 v = e.histogram_along_axis(r, a["borders"], a["types"], a["axis"]).astype(np.float64)
             elif k == "positions":
-                v = e.get_particles(r)[0]
+                pos, types, _ = e.get_particles(r)
+                v = pos if a.get("types") is None else pos[np.isin(types, a["types"])]
             elif k == "forces":
                 f, h = e.get_forces(r)
                 v = f[np.argsort(h, kind="stable")]
+                if a.get("types") is not None:
+                    v = v[np.isin(e.get_particles(r)[1], a["types"])]
             elif k == "particles":
                 pos, types, handles = e.get_particles(r)
                 v = (types, handles, pos)
@@ -109,6 +118,8 @@ class Recorder:
                 v = (handles, types, pos)
             o["time"].append(t)
             o["values"].append(v)
+            if o["callback"] is not None:
+                o["callback"](v)
             if self._file is not None and len(o["time"]) >= self._flush_stride:
                 self._flush(o)
 
