@@ -181,6 +181,8 @@ __device__ __forceinline__ void place_tables(char *base, const Tables &T, const 
 // Orders the brick's own records so that those of the longest-reach types come first (a stable
 // partition of m.inner): a wave then holds particles of like stencil reach, and the short-reach waves
 // finish early instead of idling behind their long-reach lanes.  tmp: n_inner halves of free LDS.
+// block_exclusive_scan keeps one sum per wave of a SCAN_THREADS block.
+static_assert(BRICK_THREADS == SCAN_THREADS, "order_inner_by_reach scans a brick block with block_exclusive_scan");
 __device__ void order_inner_by_reach(const BrickSmem &m, uint32_t n_inner, const float *reach_tab, uint32_t nt,
                                      uint16_t *tmp) {
     __shared__ float s_rmax;
