@@ -336,10 +336,16 @@ class _Observables:  # registry/observables.py:58-240, recorded by observe.Recor
                                                         axis=axis, callback=callback,
                                                         name=_save_name(save, "histogram_along_axis"))))
 
+    def energy(self, stride, callback=None, save="default"):
+        self._list.append(("energy", dict(stride=stride, callback=callback, name=_save_name(save, "energy"))))
+
+    def virial(self, stride, callback=None, save="default"):
+        self._list.append(("virial", dict(stride=stride, callback=callback, name=_save_name(save, "virial"))))
+
     def _unsupported(self, *args, **kwargs):
         raise NotImplementedError("this observable is not available on the MI355X kernel")
 
-    energy = virial = pressure = topologies = _unsupported
+    pressure = topologies = _unsupported
 
 
 def _save_name(save, default):
@@ -467,6 +473,10 @@ class Simulation:
                 rec.forces(stride, name=name, types=a["types"], callback=cb)
             elif kind == "particles":
                 rec.particles(stride, name=name, callback=cb)
+            elif kind == "energy":
+                rec.energy(stride, name=name, callback=cb)
+            elif kind == "virial":
+                rec.virial(stride, name=name, callback=cb)
             elif kind == "radial_distribution":
                 rec.radial_distribution(stride, a["borders"], a["from_types"], a["to_types"], a["density"], name=name,
                                         callback=cb)

@@ -73,6 +73,8 @@ def lib():
                                             _u64p, _i32p, _u64p, _u64p, _f64p, _u16p, _u64p, _u64p]
         h.rdy_h5_series_f64.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32, _u64p,
                                         _f64p]
+        h.rdy_h5_series_f64_nd.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.POINTER(ctypes.c_uint32), _u64p, _f64p]
         h.rdy_h5_u64_table.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64,
                                        ctypes.c_uint32, _u64p]
         h.rdy_h5_vlen_series.argtypes = [ctypes.c_int64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
@@ -136,6 +138,21 @@ class File:
         d = np.ascontiguousarray(data, dtype=np.float64).reshape(len(t), -1)
         self._check(lib().rdy_h5_series_f64(self.f, name.encode(), len(t), d.shape[1], _p(t, _u64p), _p(d, _f64p)),
                     name)
+
+    def energy(self, name, time, values):
+        """Energy (Energy.cpp:69-71): data [T], one value per frame"""
+        self._series_nd(name, time, np.asarray(values, dtype=np.float64).reshape(len(time)))
+
+    def virial(self, name, time, values):
+        """Virial (Virial.cpp:76-78): data [T][3][3]"""
+        self._series_nd(name, time, np.asarray(values, dtype=np.float64).reshape(len(time), 3, 3))
+
+    def _series_nd(self, name, time, data):
+        t = np.ascontiguousarray(time, dtype=np.uint64)
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        dims = (ctypes.c_uint32 * 3)(*(list(d.shape[1:]) + [0, 0, 0])[:3])
+        self._check(lib().rdy_h5_series_f64_nd(self.f, name.encode(), len(t), d.ndim - 1, dims, _p(t, _u64p),
+                                               _p(d, _f64p)), name)
 
     def _vlen(self, name, dataset, elem, time, frames, with_time):
         dt, width = _VLEN[elem]

@@ -44,6 +44,10 @@
 //                                  const uint64_t *time, const double *centers, const double *dist);
 //   int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, const uint64_t *time,
 //                         const double *data);                    data [T][width]
+//   int rdy_h5_series_f64_nd(int64_t f, const char *name, uint64_t T, uint32_t ndims, const uint32_t *dims,
+//                            const uint64_t *time, const double *data);
+//       data [T][dims[0]]..[dims[ndims-1]], ndims <= 3: Energy's [T] (ndims 0, Energy.cpp:69-71) and
+//       Virial's [T][3][3] (Virial.cpp:76-78)
 //   int rdy_h5_vlen_series(int64_t f, const char *name, const char *dataset, int elem, uint64_t T,
 //                          const uint64_t *frame_sizes, const void *data, const uint64_t *time);
 //       elem 0: {x, y, z} doubles, 1: uint16, 2: uint64; the frames' elements back to back in data;
@@ -387,6 +391,19 @@ int rdy_h5_series_f64(int64_t f, const char *name, uint64_t T, uint32_t width, c
     if (g < 0) return -1;
     hsize_t dt[1] = {T}, dd[2] = {T, width};
     int rc = append_f64(g, "data", 2, dd, data);
+    if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
+    H5Gclose(g);
+    return rc;
+}
+
+int rdy_h5_series_f64_nd(int64_t f, const char *name, uint64_t T, uint32_t ndims, const uint32_t *dims,
+                         const uint64_t *time, const double *data) {
+    if (ndims > 3 || (ndims && !dims)) return -1;
+    hid_t g = observable_group((hid_t)f, name);
+    if (g < 0) return -1;
+    hsize_t dt[1] = {T}, dd[4] = {T, 0, 0, 0};
+    for (uint32_t a = 0; a < ndims; ++a) dd[a + 1] = dims[a];
+    int rc = append_f64(g, "data", 1 + (int)ndims, dd, data);
     if (rc == 0) rc = append_u64(g, "time", 1, dt, time);
     H5Gclose(g);
     return rc;

@@ -14,6 +14,8 @@ the reference's observables flush their buffered frames every flushStride (Obser
   reactions             Reactions: the last step's reaction records
   radial_distribution   RadialDistribution (RadialDistribution.cpp:75-124), on the device
   histogram_along_axis  HistogramAlongAxis (CPUObservables.cpp:85-136), on the device
+  energy                Energy: the potential energy of the state (pair and bonded terms)
+  virial                Virial: the 3x3 pair virial (Context::recordVirial is switched on)
   positions, forces     Positions / Forces: every particle's position or force (ordered by id)
   particles             Particles: every particle's type, id and position
   trajectory            FlatTrajectory: every particle's id, type and position /readdy/trajectory/<name>
@@ -64,6 +66,13 @@ class Recorder:
         self._add("histogram", name, stride, callback, borders=np.asarray(borders, dtype=np.float64),
                   types=list(types), axis=int(axis))
 
+    def energy(self, stride, name="energy", callback=None):
+        self._add("energy", name, stride, callback)
+
+    def virial(self, stride, name="virial", callback=None):
+        self.e.record_virial(True)
+        self._add("virial", name, stride, callback)
+
     def positions(self, stride, name="positions", types=None, callback=None):
         """positions ordered by id; types: only particles of these type ids"""
         self._add("positions", name, stride, callback, types=None if types is None else list(types))
@@ -102,6 +111,10 @@ class Recorder:
                 v = e.radial_distribution(r, a["borders"], a["frm"], a["to"], a["density"])[1]
             elif k == "histogram":
                 v = e.histogram_along_axis(r, a["borders"], a["types"], a["axis"]).astype(np.float64)
+            elif k == "energy":
+                v = e.energy(r)
+            elif k == "virial":
+                v = e.virial(r)
             elif k == "positions":
                 pos, types, _ = e.get_particles(r)
                 v = pos if a.get("types") is None else pos[np.isin(types, a["types"])]
@@ -130,7 +143,7 @@ class Recorder:
     def run(self, handler, dt, n_steps, graph=True):
         """n_steps fused steps of the replica's engine, observables evaluated where due (and at the
         recorder's first step 0)."""
-        if self._evaluated_at is None and any(o["kind"] == "forces" for o in self._obs):
+        if self._evaluated_at is None and any(o["kind"] in ("forces", "energy", "virial") for o in self._obs):
             self.e.calculate_forces()  # the loop forms the initial forces before step 0's observables
         self._evaluate(self.t)
         end = self.t + int(n_steps)
@@ -187,6 +200,10 @@ class Recorder:
             f.radial_distribution(o["name"], t, (b[:-1] + b[1:]) / 2, np.asarray(v).reshape(len(t), -1))
         elif k == "histogram":
             f.histogram_along_axis(o["name"], t, np.asarray(v).reshape(len(t), -1))
+        elif k == "energy":
+            f.energy(o["name"], t, v)
+        elif k == "virial":
+            f.virial(o["name"], t, v)
         elif k == "positions":
             f.positions(o["name"], t, v)
         elif k == "forces":

@@ -149,17 +149,27 @@ def test_simulation_through_the_api_equals_the_engine(tmp_path, handler):
     seen = []
     sim.observe.number_of_particles(5, ["A", "C"], callback=lambda v: seen.append(v.tolist()))
     sim.observe.reaction_counts(5)
+    energies, virials = [], []
+    sim.observe.energy(5, callback=energies.append)
+    sim.observe.virial(5, callback=virials.append)
     sim.record_trajectory(10)
     sim.run(20, 0.01)
 
     e = Engine(3, [1., 0.5, 0.2], potentials=s.potentials._pots, reactions=s.reactions._rx)
     r = e.add_replica([12.] * 3, (1, 1, 1), seed=9, max_particles=3200)
     e.set_particles(r, np.vstack([pa, pb]), np.array([0] * 400 + [1] * 400, dtype=np.uint32))
+    e.record_virial(True)
+    e.calculate_forces()
     h = L.HANDLER_UA if handler == "UncontrolledApproximation" else L.HANDLER_GILLESPIE
     counts = [e.count_types(1)[0][[0, 2]].tolist()]
+    want_e, want_v = [e.energy(r)], [e.virial(r)]
     for _ in range(4):
         e.run(h, 0.01, 5)
         counts.append(e.count_types(1)[0][[0, 2]].tolist())
+        want_e.append(e.energy(r))
+        want_v.append(e.virial(r))
+    assert energies == want_e and all(np.array_equal(a, b) for a, b in zip(virials, want_v))
+    assert want_e[0] > 0 and abs(want_v[0]).max() > 0  # overlapping A-B pairs repel
     pos, types, _ = e.get_particles(r)
     names = ["A", "B", "C"]
     got = sim.current_particles()
@@ -171,6 +181,8 @@ def test_simulation_through_the_api_equals_the_engine(tmp_path, handler):
     path = str(tmp_path / "api.h5")
     assert h5io.read_u64(path, "/readdy/observables/n_particles/time").tolist() == [0, 5, 10, 15, 20]
     assert h5io.read_u64(path, "/readdy/observables/n_particles/data").reshape(5, 2).tolist() == counts
+    assert h5io.read_f64(path, "/readdy/observables/energy/data").tolist() == want_e
+    assert np.array_equal(h5io.read_f64(path, "/readdy/observables/virial/data").reshape(5, 3, 3), np.array(want_v))
     t, frames = h5io.read_trajectory(path, "")
     assert t.tolist() == [0, 10, 20] and np.array_equal(frames[-1][3], pos)
 

@@ -230,3 +230,26 @@ def test_writers_append_to_chunked_series(tmp_path):
         out = subprocess.run([exe, "-p", "-H", "-d", "/readdy/observables/n/data", path], capture_output=True,
                              text=True, check=True).stdout
         assert "CHUNKED ( 4, 2 )" in out and "H5S_UNLIMITED" in out, out
+
+
+def test_energy_and_virial_series(tmp_path):
+    """Energy's data is one value per frame ([T], Energy.cpp:69-71), Virial's a 3x3 matrix per frame
+    ([T][3][3], Virial.cpp:76-78); both append like the other series"""
+    path = str(tmp_path / "ev.h5")
+    v = np.arange(27, dtype=np.float64).reshape(3, 3, 3) * 0.5
+    with h5io.File(path, chunk=2) as f:
+        f.energy("energy", [0, 5], [1.5, -2.25])
+        f.energy("energy", [10], [3.0])
+        f.virial("virial", [0], v[:1])
+        f.virial("virial", [5, 10], v[1:])
+    assert list(h5io.read_u64(path, "/readdy/observables/energy/time")) == [0, 5, 10]
+    assert list(h5io.read_f64(path, "/readdy/observables/energy/data")) == [1.5, -2.25, 3.0]
+    assert np.array_equal(h5io.read_f64(path, "/readdy/observables/virial/data"), v.ravel())
+    exe = _h5dump()
+    if exe:
+        out = subprocess.run([exe, "-H", "-d", "/readdy/observables/energy/data", path], capture_output=True,
+                             text=True, check=True).stdout
+        assert "( 3 ) / ( H5S_UNLIMITED )" in out, out
+        out = subprocess.run([exe, "-H", "-d", "/readdy/observables/virial/data", path], capture_output=True,
+                             text=True, check=True).stdout
+        assert "( 3, 3, 3 ) / ( H5S_UNLIMITED, 3, 3 )" in out, out
