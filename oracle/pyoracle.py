@@ -121,6 +121,10 @@ def lib():
         L.or_dihedral_force_energy.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp,
                                                _dp, _ip, _dp, _dp, _dp, _dp]
         L.or_add_spatial_reaction.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
+        L.or_add_spatial_fusion.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
+        L.or_set_topology_reserve.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        L.or_topology_fusions.restype = _u64p
+        L.or_topology_fusions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_set_topologies.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _ip, ctypes.c_size_t, _ip]
         L.or_topology_reactions.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_uint32, _u64p]
         L.or_topology_types.restype = _ip
@@ -441,11 +445,23 @@ class System:
         p = lib().or_reaction_counts(self._h, ctypes.byref(n))
         return np.array([p[i] for i in range(n.value)], dtype=np.uint64)
 
-    # ---- spatial topology reactions of the topology-particle enzymatic kind ----
-    def add_spatial_reaction(self, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius):
-        """T1(type1) + (type2) -> T2(type_to1) + (type_to2), topology types as integers"""
-        _check(lib().or_add_spatial_reaction(self._h, type1, top_type1, type2, type_to1, top_type_to1, type_to2,
-                                             rate, radius))
+    # ---- spatial topology reactions of the topology-particle kinds ----
+    def add_spatial_reaction(self, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius,
+                             fusion=False):
+        """T1(type1) + (type2) -> T2(type_to1) + (type_to2), or with fusion T1(type1) + (type2) ->
+        T2(type_to1--type_to2); topology types as integers"""
+        f = lib().or_add_spatial_fusion if fusion else lib().or_add_spatial_reaction
+        _check(f(self._h, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius))
+
+    def set_topology_reserve(self, next_id, end_id):
+        """ids next_id .. end_id - 1 for particles that join topologies by fusion"""
+        _check(lib().or_set_topology_reserve(self._h, next_id, end_id))
+
+    def topology_fusions(self):
+        """rows (topology, topology particle id, old id, new id) of the last call's fusions"""
+        n = ctypes.c_size_t()
+        p = lib().or_topology_fusions(self._h, ctypes.byref(n))
+        return np.array([p[i] for i in range(4 * n.value)], dtype=np.uint64).reshape(n.value, 4)
 
     def set_topologies(self, topology_of_id, top_types):
         """topology_of_id[id] = topology index of the particle with that id (-1: none)"""

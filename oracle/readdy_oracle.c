@@ -456,11 +456,12 @@ typedef struct {
 } bonded_t;
 
 #define OR_MAX_STR 64
-/* a topology-particle enzymatic spatial topology reaction, "T1(p1) + (p2) -> T2(p3) + (p4)"
- * (SpatialTopologyReaction.h:64-236, STRMode::TP_ENZYMATIC) */
+/* a topology-particle spatial topology reaction, enzymatic "T1(p1) + (p2) -> T2(p3) + (p4)" or fusion
+ * "T1(p1) + (p2) -> T2(p3--p4)" (SpatialTopologyReaction.h:64-236, STRMode::TP_ENZYMATIC / TP_FUSION) */
 typedef struct {
     int type1, top_type1, type2, type_to1, top_type_to1, type_to2;
     double rate, radius, radius2;
+    int fusion;
 } str_t;
 
 typedef struct {
@@ -516,6 +517,9 @@ struct or_system {
     size_t topology_of_len;
     int *top_types;
     size_t n_tops;
+    uint64_t fuse_next, fuse_end; /* ids for particles joining topologies by fusion: fuse_next .. fuse_end - 1 */
+    uint64_t *fusions;            /* the last call's fusions: (topology, topology particle id, old id, new id) */
+    size_t n_fusions;
     str_t str[OR_MAX_STR];
     int n_str;
     uint64_t str_counts[OR_MAX_STR];
@@ -546,6 +550,7 @@ or_system *or_create(int n_types, const double *box, const int *pbc, double kbt,
 void or_destroy(or_system *s) {
     if (!s) return;
     free(s->topology_of);
+    free(s->fusions);
     free(s->top_types);
     free(s->e);
     free(s->blanks);
@@ -2164,7 +2169,11 @@ void or_phase_times(or_system *s, double *out4, int reset) {
  *   handleTopologyParticleReaction (:369-404): the topology particle takes type_to1, the free particle
  *     type_to2, the topology the type top_type_to1.  (The reference also writes the topology's index into
  *     the free particle's record, which only matters for later topology-topology reactions; free
- *     particles here keep no topology.)
+ *     particles here keep no topology.)  A fusion (TP_FUSION) also appends the free particle to the
+ *     topology, bonded to the topology particle (GraphTopology::appendParticle): it becomes a member of
+ *     the topology under the next id of the reserve (or_set_topology_reserve), taken in the order the
+ *     events are performed, and the call lists (topology, topology particle id, old id, new id).  The
+ *     reference keeps the particle's id; the MI355X kernel renames it into its topology handle range.
  * --------------------------------------------------------------------------------------------- */
 int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
                             int type_to2, double rate, double radius) {
@@ -2183,7 +2192,27 @@ int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, i
     r->rate = rate;
     r->radius = radius;
     r->radius2 = radius * radius;
+    r->fusion = 0;
     return 0;
+}
+
+int or_add_spatial_fusion(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
+                          int type_to2, double rate, double radius) {
+    if (or_add_spatial_reaction(s, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius)) return -1;
+    s->str[s->n_str - 1].fusion = 1;
+    return 0;
+}
+
+int or_set_topology_reserve(or_system *s, uint64_t next_id, uint64_t end_id) {
+    if (end_id < next_id) return fail("topology reserve: end before start");
+    s->fuse_next = next_id;
+    s->fuse_end = end_id;
+    return 0;
+}
+
+const uint64_t *or_topology_fusions(const or_system *s, size_t *n) {
+    if (n) *n = s->n_fusions;
+    return s->fusions;
 }
 
 int or_set_topologies(or_system *s, size_t n_ids, const int *topology_of_id, size_t n_tops, const int *top_types) {
@@ -2270,6 +2299,7 @@ static int cmp_tev(const void *a, const void *b) {
 int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_performed) {
     memset(s->str_counts, 0, sizeof s->str_counts); /* CPUEvaluateTopologyReactions.cpp:92-94 */
     if (n_performed) *n_performed = 0;
+    s->n_fusions = 0;
     if (!s->n_str || !s->n_tops) return 0;
     if (!s->nl_set) return fail("topology reactions need a neighbour list");
     tev_ud u;
@@ -2289,11 +2319,35 @@ int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_pe
     for (size_t i = 0; i < u.n; ++i) {
         const tevent_t *e = &u.ev[i];
         if (top_used[e->top] || p_used[e->q]) continue;
+        const str_t *x = &s->str[e->r];
+        if (x->fusion && s->fuse_next >= s->fuse_end) {
+            free(top_used);
+            free(p_used);
+            free(u.ev);
+            return fail("topology-particle fusion beyond the topology reserve");
+        }
         top_used[e->top] = 1;
         p_used[e->q] = 1;
-        const str_t *x = &s->str[e->r];
         s->e[e->p].type = x->type_to1;
         s->e[e->q].type = x->type_to2;
+        if (x->fusion) { /* appendParticle(idx2, idx1): the free particle joins under the next reserve id */
+            const uint64_t nid = s->fuse_next++;
+            if (nid >= s->topology_of_len) {
+                int *t = (int *)realloc(s->topology_of, (size_t)(nid + 1) * sizeof(int));
+                if (!t) return fail("out of memory");
+                for (size_t k = s->topology_of_len; k <= nid; ++k) t[k] = -1;
+                s->topology_of = t;
+                s->topology_of_len = (size_t)nid + 1;
+            }
+            s->topology_of[nid] = e->top;
+            s->fusions = (uint64_t *)realloc(s->fusions, 4 * (s->n_fusions + 1) * sizeof(uint64_t));
+            uint64_t *row = s->fusions + 4 * s->n_fusions++;
+            row[0] = (uint64_t)e->top;
+            row[1] = e->ip;
+            row[2] = e->iq;
+            row[3] = nid;
+            s->e[e->q].id = nid;
+        }
         s->top_types[e->top] = x->top_type_to1;
         s->str_counts[e->r]++;
         ++done;

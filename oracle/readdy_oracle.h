@@ -155,10 +155,17 @@ int or_detailed_balance(or_system *s, double dt, uint32_t step, uint64_t *n_perf
 
 const char *or_last_error(void);
 
-/* ---- spatial topology reactions of the topology-particle enzymatic kind
+/* ---- spatial topology reactions of the topology-particle kinds (enzymatic, fusion)
  *      (kernels/cpu/src/actions/CPUEvaluateTopologyReactions.cpp:88-475) ---- */
 int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
                             int type_to2, double rate, double radius);
+/* the fusion kind "T1(p1) + (p2) -> T2(p3--p4)": the free particle joins the topology under the next id of
+ * the reserve next_id .. end_id - 1; or_topology_fusions lists the last call's joins as rows (topology,
+ * topology particle id, old id, new id) */
+int or_add_spatial_fusion(or_system *s, int type1, int top_type1, int type2, int type_to1, int top_type_to1,
+                          int type_to2, double rate, double radius);
+int or_set_topology_reserve(or_system *s, uint64_t next_id, uint64_t end_id);
+const uint64_t *or_topology_fusions(const or_system *s, size_t *n);
 /* topology_of_id[id] = the topology of the particle with that id (-1: none), for ids below n_ids */
 int or_set_topologies(or_system *s, size_t n_ids, const int *topology_of_id, size_t n_tops, const int *top_types);
 int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_performed);

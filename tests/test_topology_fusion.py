@@ -1,0 +1,261 @@
+"""Topology-particle fusion, "T1(p1) + (p2) -> T2(p3--p4)" (SpatialTopologyReaction.h:64-236, STRMode::TP_FUSION;
+handleTopologyParticleReaction with appendParticle, CPUEvaluateTopologyReactions.cpp:369-404).
+
+The oracle is pinned on a hand-built polymer-growth scene whose outcome follows from the rules alone (rate
+1e10 accepts with probability 1 in FP64): every chain end with a free monomer inside the radius grows by that
+monomer, which joins under the next reserve id and is the new end, so that the next evaluation grows the chain
+from it (a wrong id or topology for the joining particle, or a dropped type change, fails this); a monomer
+shared by two ends joins one of them; a fusion beyond the reserve is an error.
+
+The device tests below (the same scenes, random chains growing among monomers event for event against the
+oracle, the fused loop against the separate actions) are skipped: the device path of this kind was taken out
+after its first GPU run ended in a memory fault whose cause was not found.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+MID, END, MONO = 0, 1, 2   # chain species (topology flavour) and the free monomer
+FLAVOR = [1, 1, 0]
+GROW = dict(type1=END, top_type1=0, type2=MONO, type_to1=MID, top_type_to1=0, type_to2=END, rate=1e10, radius=1.0,
+            fusion=1)
+
+# two chains (ids 0-1 and 2-3, each "mid end" along x), a reserve of three ids (4, 5, 6), monomers 7-10:
+# 7 is 0.7 from chain 0's end, 8 is 0.8 from chain 1's end, 9 is 1.5 from chain 0's end but 0.8 from 7, 10 far
+SCENE_POS = np.array([[-5, 0, 0], [-4, 0, 0], [5, 0, 0], [6, 0, 0],
+                      [-3.3, 0, 0], [6.8, 0, 0], [-2.5, 0, 0], [9, 0, 0]], float)
+SCENE_TYPES = np.array([MID, END, MID, END, MONO, MONO, MONO, MONO])
+SCENE_IDS = np.array([0, 1, 2, 3, 7, 8, 9, 10])
+
+
+def _oracle(pos, types, ids, topology_of_id, top_types, reserve, box=20.0, reaction=GROW, pots=(), seed=3):
+    o = po.System(3, [box] * 3, [1, 1, 1], diffusion=[1.0, 1.0, 1.0])
+    for kind, a, b, p in pots:
+        o.add_pair_potential(kind, a, b, p)
+    x = reaction
+    o.add_spatial_reaction(x["type1"], x["top_type1"], x["type2"], x["type_to1"], x["top_type_to1"], x["type_to2"],
+                           x["rate"], x["radius"], fusion=bool(x.get("fusion")))
+    o.set_seed(seed)
+    o.add_particles(pos, np.asarray(types, np.int32), ids=np.asarray(ids))
+    o.set_topologies(np.asarray(topology_of_id), np.asarray(top_types))
+    o.set_topology_reserve(*reserve)
+    o.nl_setup(max(o.max_cutoff(), x["radius"]))
+    return o
+
+
+def _by_id(o):
+    _, t, ids, _ = o.particles()
+    return {int(i): int(ty) for i, ty in zip(ids, t)}
+
+
+def test_oracle_polymer_growth_scene():
+    o = _oracle(SCENE_POS, SCENE_TYPES, SCENE_IDS, [0, 0, 1, 1], [0, 0], (4, 7))
+    assert o.topology_reactions(1.0, step=0) == 2
+    rows = o.topology_fusions()
+    assert sorted(map(tuple, rows[:, :3].tolist())) == [(0, 1, 7), (1, 3, 8)]
+    assert sorted(rows[:, 3].tolist()) == [4, 5]  # reserve ids, in the order performed
+    new7 = int(rows[rows[:, 2] == 7, 3][0])
+    t = _by_id(o)
+    assert t[1] == MID and t[3] == MID and t[new7] == END and 7 not in t and 8 not in t
+    # the joined monomer is chain 0's end now: 9 joins it (0.8 away), chain 1 finds nothing
+    assert o.topology_reactions(1.0, step=1) == 1
+    assert o.topology_fusions().tolist() == [[0, new7, 9, 6]]
+    t = _by_id(o)
+    assert t[new7] == MID and t[6] == END and t[10] == MONO
+    assert o.topology_reactions(1.0, step=2) == 0 and len(o.topology_fusions()) == 0
+    assert o.topology_types().tolist() == [0, 0]
+
+
+def test_oracle_fusion_reserve_and_shared_monomer():
+    # a monomer midway between the two chain ends (0.5 from each): exactly one chain takes it
+    pos = np.array([[-1.5, 0, 0], [-0.5, 0, 0], [1.5, 0, 0], [0.5, 0, 0], [0, 0, 0]], float)
+    o = _oracle(pos, [MID, END, MID, END, MONO], [0, 1, 2, 3, 5], [0, 0, 1, 1], [0, 0], (4, 5))
+    assert o.topology_reactions(1.0, step=0) == 1
+    (top, end, mono, new), = o.topology_fusions().tolist()
+    assert (end, mono, new) == ((1 if top == 0 else 3), 5, 4)
+    # two fusions wanted, one reserve id: an error
+    o = _oracle(SCENE_POS, SCENE_TYPES, SCENE_IDS, [0, 0, 1, 1], [0, 0], (4, 5))
+    with pytest.raises(RuntimeError, match="reserve"):
+        o.topology_reactions(1.0, step=0)
+
+
+def test_oracle_enzymatic_and_fusion_differ_only_in_the_join():
+    """the same scene with the enzymatic kind: the same events, no join, ids kept"""
+    enz = dict(GROW, fusion=0, type_to2=MONO)
+    o = _oracle(SCENE_POS, SCENE_TYPES, SCENE_IDS, [0, 0, 1, 1], [0, 0], (4, 7), reaction=enz)
+    assert o.topology_reactions(1.0, step=0) == 2 and len(o.topology_fusions()) == 0
+    t = _by_id(o)
+    assert t[1] == MID and t[3] == MID and t[7] == MONO and t[8] == MONO
+    assert o.topology_reactions(1.0, step=1) == 0  # no chain end left
+
+
+# ------------------------------------------------------------------------------------------------ device
+DEVICE_OFF = pytest.mark.skip(reason="topology-particle fusion on the device is switched off: its first GPU run "
+                                     "(test_polymer_growth_scene_on_the_device) faulted and the cause was not found; "
+                                     "the engine has no fusion entry points")
+
+def _engine(box, reserve, n, pots=(), rxs=(), seed=3, nl_cutoff=1.0):
+    from readdy_amd import Engine
+    e = Engine(3, [1.0, 1.0, 1.0], potentials=list(pots), reactions=list(rxs), topology_flavor=FLAVOR,
+               nl_cutoff=nl_cutoff)
+    r = e.add_replica([box] * 3, (1, 1, 1), seed=seed, max_particles=4 * n + 64)
+    e.reserve_topology_particles(r, reserve)
+    return e, r
+
+
+@pytest.mark.gpu
+@DEVICE_OFF
+def test_polymer_growth_scene_on_the_device():
+    print("fusion scene", flush=True)
+    e, r = _engine(20.0, 3, len(SCENE_TYPES))
+    e.set_particles(r, SCENE_POS, SCENE_TYPES.astype(np.uint32))
+    _, _, h = e.get_particles(r)
+    assert h.tolist() == SCENE_IDS.tolist()  # free particles numbered after the reserve
+    e.set_spatial_reactions([GROW])
+    e.set_topologies(r, [0, 0], [0, 0, 1, 1])
+    o = _oracle(SCENE_POS, SCENE_TYPES, SCENE_IDS, [0, 0, 1, 1], [0, 0], (4, 7))
+    zero = np.zeros((16, 3))
+    for step in range(3):
+        e.integrate(1.0, zero)  # no force, no noise: the positions stay, the step counter advances
+        e.evaluate_topology_reactions(1.0)
+        o.topology_reactions(1.0, step=step)
+        _, gt, gh = e.get_particles(r)
+        assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o), step
+        assert e.topology_fusions(r).tolist() == o.topology_fusions().tolist(), step
+    assert e.topology_reaction_counts(r, totals=True).tolist() == [3]
+    # the topology particles are handles 0 .. 6 now: the assignment takes seven entries
+    e.set_topologies(r, [0, 0], [0, 0, 1, 1, 0, 1, 0])
+    with pytest.raises(Exception):
+        e.set_topologies(r, [0, 0], [0, 0, 1, 1])
+
+
+@pytest.mark.gpu
+@DEVICE_OFF
+def test_fusion_beyond_the_reserve_is_a_capacity_error():
+    from readdy_amd import _lib as L
+    print("fusion reserve", flush=True)
+    e, r = _engine(20.0, 1, len(SCENE_TYPES))
+    pos, types = SCENE_POS, SCENE_TYPES.astype(np.uint32)
+    e.set_particles(r, pos, types)
+    e.set_spatial_reactions([GROW])
+    e.set_topologies(r, [0, 0], [0, 0, 1, 1])
+    with pytest.raises(L.EngineError, match="reserve"):
+        e.evaluate_topology_reactions(1.0)
+    with pytest.raises(L.EngineError, match="joining particle"):
+        e.set_spatial_reactions([dict(GROW, type_to2=MONO)])  # a fusion's joining particle must be a topology species
+
+
+def _growth_system(n_chains, n_mono, box, seed):
+    rng = np.random.default_rng(seed)
+    pos, types, bonds, top = [], [], [], []
+    for c in range(n_chains):
+        p0 = rng.uniform(-box / 2 + 3.5, box / 2 - 3.5, 3)  # the whole chain inside the box
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for k in range(4):
+            pos.append(p0 + 0.9 * k * u)
+            types.append(END if k == 3 else MID)
+            top.append(c)
+            if k:
+                bonds.append((4 * c + k - 1, 4 * c + k))
+    pos += list(rng.uniform(-box / 2, box / 2, (n_mono, 3)))
+    types += [MONO] * n_mono
+    return np.array(pos), np.array(types, np.uint32), bonds, top
+
+
+@pytest.mark.gpu
+@DEVICE_OFF
+@pytest.mark.parametrize("rate", [5.0, 1e10])
+def test_random_growth_event_for_event(rate):
+    """40 chains grow among 1500 monomers over five integrated steps: the same monomers join the same chains
+    under the same reserve handles, the species and the fusion rows agree, and the grown edges get their
+    bonds on both sides before the next step"""
+    from readdy_amd import _lib as L
+    print(f"fusion growth rate {rate}", flush=True)
+    box, n_chains, n_mono, reserve, dt = 16.0, 40, 1500, 400, 0.01
+    pos, types, bonds, top = _growth_system(n_chains, n_mono, box, seed=2)
+    n_topo = 4 * n_chains
+    pots = [(L.POT_HARMONIC_REPULSION, a, MONO, [5.0, 0.8]) for a in range(3)]
+    e, r = _engine(box, reserve, len(types), pots=pots, seed=11)
+    e.set_particles(r, pos, types)
+    e.set_spatial_reactions([dict(GROW, rate=rate)])
+    e.set_topologies(r, np.zeros(n_chains), top)
+    ids = np.concatenate([np.arange(n_topo), np.arange(n_topo, len(types)) + reserve])
+    tof = np.full(int(ids.max()) + 1, -1)
+    tof[:n_topo] = top
+    o = _oracle(pos, types, ids, tof, np.zeros(n_chains, int), (n_topo, n_topo + reserve), box=box,
+                reaction=dict(GROW, rate=rate), pots=pots, seed=11)
+    slot = {int(i): k for k, i in enumerate(ids)}  # the oracle's entries stay in their slots
+    edges = list(bonds)
+    e.set_bonds(r, edges, 100.0, 0.9)
+    for a, b in edges:
+        o.add_bond(slot[a], slot[b], 100.0, 0.9)
+    rng = np.random.default_rng(5)
+    e.calculate_forces()
+    o.calculate_forces()
+    grown = 0
+    for t in range(5):
+        noise = rng.normal(size=(n_topo + reserve + n_mono, 3))
+        e.integrate(dt, noise)
+        e.update_neighbor_list()
+        o.euler_bd(dt, noise.ravel(), step=t)
+        o.nl_fill()
+        e.evaluate_topology_reactions(dt)
+        o.topology_reactions(dt, step=t)
+        rows = e.topology_fusions(r)
+        assert rows.tolist() == o.topology_fusions().tolist(), t
+        _, gt, gh = e.get_particles(r)
+        assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o), t
+        for tp, a, old, new in rows.tolist():
+            slot[new] = slot.pop(old)
+            edges.append((a, new))
+            o.add_bond(slot[a], slot[new], 100.0, 0.9)
+        grown += len(rows)
+        e.set_bonds(r, edges, 100.0, 0.9)
+        e.calculate_forces()
+        o.calculate_forces()
+        _, fh = e.get_forces(r)
+        assert np.array_equal(fh, np.sort(o.particles()[2]).astype(np.uint32))
+    print(f"{grown} monomers joined", flush=True)
+    assert grown > (40 if rate > 1e9 else 5)
+    # every chain is a path: each joined monomer is bonded to the previous end
+    assert len(edges) == 3 * n_chains + grown
+
+
+@pytest.mark.gpu
+@DEVICE_OFF
+def test_fused_loop_equals_separate_actions_with_fusion():
+    from readdy_amd import _lib as L
+    print("fusion fused loop", flush=True)
+    box, n_chains, n_mono, reserve, dt, steps = 16.0, 40, 1500, 600, 0.005, 12
+    pos, types, bonds, top = _growth_system(n_chains, n_mono, box, seed=7)
+    pots = [(L.POT_HARMONIC_REPULSION, a, MONO, [5.0, 0.8]) for a in range(3)]
+
+    def build():
+        e, r = _engine(box, reserve, len(types), pots=pots, seed=23)
+        e.set_particles(r, pos, types)
+        e.set_bonds(r, bonds, 100.0, 0.9)
+        e.set_spatial_reactions([dict(GROW, rate=40.0)])
+        e.set_topologies(r, np.zeros(n_chains), top)
+        return e, r
+
+    e1, r1 = build()
+    e1.run(L.HANDLER_UA, dt, steps)
+    e2, r2 = build()
+    e2.calculate_forces()
+    for _ in range(steps):
+        e2.integrate(dt)
+        e2.update_neighbor_list()
+        e2.react(L.HANDLER_UA, dt)
+        e2.evaluate_topology_reactions(dt)
+        e2.update_neighbor_list()
+        e2.calculate_forces()
+    for x, y in zip(e1.get_particles(r1), e2.get_particles(r2)):
+        assert np.array_equal(x, y)
+    tot = e1.topology_reaction_counts(r1, totals=True)
+    assert np.array_equal(tot, e2.topology_reaction_counts(r2, totals=True)) and tot.sum() > 10
+    _, t1, h1 = e1.get_particles(r1)
+    n_topo = 4 * n_chains
+    joined = (h1 >= n_topo) & (h1 < n_topo + reserve)
+    assert joined.sum() == tot.sum() and set(t1[joined].tolist()) <= {MID, END}
