@@ -42,6 +42,13 @@
  *   DetailedBalance      closed forms: acceptance 1 without potentials, the Metropolis ratio of single
  *                        conversions against exp(-dE/kT) of a two-particle scene, the rejected move's id
  *                        bookkeeping; the equilibrium statistics of a conversion pair
+ *   virial               two-particle known answer (sign and the once-per-pair count) + O(N^2) brute force
+ *   topology reactions   enzymatic kind: hand-built scenes (one event per topology and per free particle,
+ *                        the strict radius gate) and a chi-square test of the keyed selection against the
+ *                        literal performEvents; fusion kind: polymer-growth scenes whose outcome follows
+ *                        from the rules (the joined particle is the next end), the reserve limit
+ *   break bonds          reference scenes (TestBreakingBonds.cpp) through the energy threshold
+ *   RDF / histograms     two-particle known answer, numpy brute-force pair counts and direct binning
  */
 #include "readdy_oracle.h"
 
