@@ -201,7 +201,8 @@ def ensemble_bench(n_replicas, steps, plain_steps=20):
             "plain_per_call_sample": f"{len(old_ids)} of them, 3 steps, one whole run (first walk, step, last copy, "
                                      f"host synchronisation) per replica per step",
             "sampled": f"{steps} of the configuration's 5000 steps",
-            "latency": "per-step wall time of the batched loop over chunks of 10 graph-replayed steps"}
+            "latency": "per-step wall time of the batched loop: one open run of the sampled steps, enqueued and "
+                       "synchronised in chunks of 10"}
 
 
 def rdf_bench(eng, r, pos, types, sample=10000):

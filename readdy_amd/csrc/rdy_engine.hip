@@ -976,7 +976,7 @@ void launch_step(rdy_engine *e, int handler, double dt, int last) {
     const uint64_t nc = std::max<uint64_t>(e->cells_hwm, 1);
     const uint64_t nfb = e->fb ? 3 * e->topo_used : 0;
     const uint64_t nloc = topo_reactions_on(e) ? e->topo_used : 0;
-    const uint64_t nthreads = std::max<uint64_t>(std::max<uint64_t>(std::max<uint64_t>(nr, nc), nfb), nloc);
+    const uint64_t nthreads = std::max<uint64_t>(std::max<uint64_t>(std::max<uint64_t>(nr, (nc + 3) / 4), nfb), nloc);
     hipLaunchKernelGGL(k_step_begin, dim3(blocks_for(nthreads, 256)), dim3(256), 0, e->stream, e->reps, nr, last,
                        e->grid(n), (uint32_t)nc, e->fb, (uint32_t)nfb, e->edit_count, e->e_bonded,
                        (const uint32_t *)e->loc, e->loc_cur, (uint32_t)nloc);

@@ -60,7 +60,8 @@ __device__ __forceinline__ uint32_t advance_particle(const Grid &nxt, const Repl
 
 // Step t's bookkeeping before its walk: replicas with budget left take the step (Philox step t) and
 // integrate step t+1 in the same walk unless this is their last step or the launch's (last);
-// clears buffer `clear` (the walk's target), the change lists and the bonded forces.
+// clears buffer `clear` (the walk's target; thread i its cell counters 4i .. 4i + 3), the change lists
+// and the bonded forces.
 // With topology reactions on, it also keeps each topology particle's slot in the buffer the walk reads
 // (loc_cur): the walk's moves point loc at the other buffer, and the topology reactions run after it.
 __global__ void k_step_begin(ReplicaDev *reps, uint32_t n_reps, int last, Grid clear, uint32_t n_cells,
@@ -83,7 +84,10 @@ __global__ void k_step_begin(ReplicaDev *reps, uint32_t n_reps, int last, Grid c
             R.integ = 0u;
         }
     }
-    if (i < n_cells) clear.counts[i] = 0u;
+    const uint32_t c4 = 4u * i;  // the counters array starts 256-byte aligned (hipMalloc)
+    if (c4 + 3u < n_cells) reinterpret_cast<uint4 *>(clear.counts)[i] = make_uint4(0u, 0u, 0u, 0u);
+    else
+        for (uint32_t c = c4; c < n_cells; ++c) clear.counts[c] = 0u;
     if (i < n_fb) fb[i] = 0;
     if (i == 0) {
         *clear.wl_count = 0u;

@@ -92,18 +92,24 @@ class Batched:
         self.engine.run(L.HANDLER_UA, DT, steps, graph=graph)
 
     def timed(self, steps, chunk=10, warmup=10):
-        """(particle-steps/s, per-step wall-time percentiles over chunks of `chunk` steps)."""
+        """(particle-steps/s, per-step wall-time percentiles over chunks of `chunk` steps).  The steps are one
+        open run (its first walk and last copy once, as a serving loop keeps its run open), enqueued and
+        synchronised chunk by chunk for the latencies."""
         self.run(warmup)
         n0 = self.particles()
         per_step = []
+        e = self.engine
         t_all = time.perf_counter()
+        e.run_begin(L.HANDLER_UA, DT, steps)
         done = 0
         while done < steps:
             k = min(chunk, steps - done)
             t0 = time.perf_counter()
-            self.run(k)  # returns synchronised
+            e.run_enqueue(k)
+            e.synchronize()
             per_step.append((time.perf_counter() - t0) / k)
             done += k
+        e.run_end()
         elapsed = time.perf_counter() - t_all
         n1 = self.particles()
         rate = 0.5 * (n0 + n1) * steps / elapsed
@@ -182,12 +188,19 @@ def half_round_brick(cell):
     return (e, e, e)
 
 
+def brick_candidates(cell):
+    """The engine's one-round choice, the half-round cube and the two shapes next to it: how bricks tile
+    the uneven replica grids (the last brick of an axis is clipped) changes the best shape by a cell."""
+    e = half_round_brick(cell)[0]
+    return [(0, 0, 0), (e, e, e), (e, e, e + 1), (e + 1, e + 1, e + 1)]
+
+
 def autotune(n, box, cells=(1.0, 1.5, 2.0), steps=20):
-    """Times short batched runs of each (cell size, brick shape) candidate, the brick being the engine's
-    one-round choice or half_round_brick; returns the fastest with all timings."""
+    """Times short batched runs of each (cell size, brick shape) candidate (brick_candidates); returns the
+    fastest with all timings."""
     results = []
     for cutoff in cells:
-        for brick in ((0, 0, 0), half_round_brick(cutoff)):
+        for brick in brick_candidates(cutoff):
             b = Batched(n, box, nl_cutoff=cutoff, brick=brick)
             try:
                 rate, _ = b.timed(steps, chunk=steps, warmup=5)

@@ -48,3 +48,11 @@ def test_autotune_candidates_give_one_trajectory():
     for x in res[1:]:
         p, t, h = x["state"]
         assert np.array_equal(h, h0) and np.array_equal(t, t0) and np.array_equal(p, p0), x
+
+
+def test_ensemble_brick_candidates_around_the_half_round_cube():
+    from readdy_amd.ensemble import brick_candidates, half_round_brick
+    for cell in (1.0, 1.5, 2.0):
+        e = half_round_brick(cell)[0]
+        c = brick_candidates(cell)
+        assert c[0] == (0, 0, 0) and (e, e, e) in c and (e, e, e + 1) in c and (e + 1,) * 3 in c
