@@ -235,28 +235,45 @@ typedef struct {
 int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_pair *pairs, uint64_t cap,
                     uint32_t *broken, uint64_t *n_broken);
 
-/* ---- spatial topology reactions of the topology-particle enzymatic kind, "T1(p1) + (p2) -> T2(p3) + (p4)":
- *      SpatialTopologyReaction.h:64-236 (STRMode::TP_ENZYMATIC), evaluated as
- *      CPUEvaluateTopologyReactions.cpp:88-475 does.  A particle of species type1 in a topology of type
- *      top_type1 and a particle of no topology of species type2 closer than radius react with probability
- *      1 - exp(-rate dt); at most one reaction per topology and per free particle and step (performEvents
- *      with eventsDependent); the topology particle becomes type_to1, the free one type_to2, the topology
- *      top_type_to1.  Topology types are the caller's integers.  The bonded terms stay as they are: when
- *      type_to1 changes the topology's bonded configuration, the caller sets the terms again. */
+/* ---- spatial topology reactions of the topology-particle kinds: enzymatic "T1(p1) + (p2) -> T2(p3) + (p4)"
+ *      and fusion "T1(p1) + (p2) -> T2(p3--p4)": SpatialTopologyReaction.h:64-236 (STRMode::TP_ENZYMATIC,
+ *      TP_FUSION), evaluated as CPUEvaluateTopologyReactions.cpp:88-475 does.  A particle of species type1 in
+ *      a topology of type top_type1 and a particle of no topology of species type2 closer than radius react
+ *      with probability 1 - exp(-rate dt); at most one reaction per topology and per free particle and step
+ *      (performEvents with eventsDependent); the topology particle becomes type_to1, the free one type_to2,
+ *      the topology top_type_to1.  Fusion (handleTopologyParticleReaction, :369-404, appendParticle): the free
+ *      particle joins the topology, bonded to the topology particle; it takes the next handle of the
+ *      replica's topology reserve (rdy_reserve_topology_particles), and rdy_get_topology_fusions lists the
+ *      joins.  Topology types are the caller's integers.  The bonded terms stay as they are: the caller
+ *      sets them again when a reaction changes the topology's bonded configuration (every fusion does). */
 typedef struct {
     uint32_t type1, top_type1;   /* the topology particle's species and its topology's type */
     uint32_t type2;              /* the free particle's species                               */
     uint32_t type_to1, top_type_to1, type_to2;
     double rate, radius;
+    uint32_t fusion;             /* 0: TP_ENZYMATIC, 1: TP_FUSION (type_to2 a topology species) */
+    uint32_t reserved;           /* 0 */
 } rdy_spatial_reaction;
 /* the engine's spatial topology reactions (TopologyRegistry::addSpatialReaction, TopologyRegistry.cpp:195-223);
  * radius <= the engine's neighbour-list cutoff, at most 64 reactions */
 int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_reaction *reactions);
 /* the replica's topologies: their types [n_topologies] and, for each topology particle (handles 0 ..
- * n_topo - 1 of the last upload), the index of its topology [n_topo] (StateModel::addTopology), or
- * UINT32_MAX for one that left its topology (a structural reaction made it a normal particle) */
+ * n_topo - 1: the last upload's and those fusions added), the index of its topology [n_topo]
+ * (StateModel::addTopology), or UINT32_MAX for one that left its topology (a structural reaction made it a
+ * normal particle) */
 int rdy_set_topologies(rdy_engine *e, uint32_t replica, uint32_t n_topologies, const uint32_t *topology_types,
                        const uint32_t *topology_of);
+/* Topology-particle fusion needs handles for the particles that join topologies.  Called before
+ * rdy_set_particles, this reserves `n` handles after the upload's topology particles: the upload then
+ * numbers its topology particles 0 .. n_topo - 1 and its free particles from n_topo + n on, and each
+ * fusion gives the joining particle the next reserved handle (its id changes; ReaDDy keeps the id).  A
+ * fusion beyond the reserve is a capacity error.  The reserve holds for the replica's next uploads. */
+int rdy_reserve_topology_particles(rdy_engine *e, uint32_t replica, uint32_t n);
+/* the fusions of the last rdy_evaluate_topology_reactions call (or fused step) of the replica, in the order
+ * they were performed: rows [n][4] of (topology index, topology particle's handle, the joining particle's
+ * handle before, its handle now); *n = their count.  The replica's topology particles are then handles
+ * 0 .. n_topo - 1 with n_topo grown by the fusions (rdy_set_topologies takes that many entries). */
+int rdy_get_topology_fusions(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *rows, uint32_t *n);
 /* the replica's topology types as the topology reactions left them; *n = the number of topologies */
 int rdy_get_topology_types(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *types, uint32_t *n);
 /* per topology, 1 if a spatial topology reaction changed it in the last rdy_evaluate_topology_reactions call:

@@ -33,7 +33,8 @@ class PairPotential(ctypes.Structure):
 class SpatialReaction(ctypes.Structure):
     _fields_ = [("type1", ctypes.c_uint32), ("top_type1", ctypes.c_uint32), ("type2", ctypes.c_uint32),
                 ("type_to1", ctypes.c_uint32), ("top_type_to1", ctypes.c_uint32), ("type_to2", ctypes.c_uint32),
-                ("rate", ctypes.c_double), ("radius", ctypes.c_double)]
+                ("rate", ctypes.c_double), ("radius", ctypes.c_double), ("fusion", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class Reaction(ctypes.Structure):
@@ -121,6 +122,8 @@ SIGNATURES = {
     "rdy_run_graph": (_int, [_vp, _int, _d, _u64]),
     "rdy_set_spatial_reactions": (_int, [_vp, _u32, _vp]),
     "rdy_set_topologies": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
+    "rdy_reserve_topology_particles": (_int, [_vp, _u32, _u32]),
+    "rdy_get_topology_fusions": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
     "rdy_get_topology_types": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
     "rdy_evaluate_topology_reactions": (_int, [_vp, _d]),
     "rdy_get_topology_reaction_counts": (_int, [_vp, _u32, _int, _u64p]),

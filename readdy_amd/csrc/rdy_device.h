@@ -58,6 +58,8 @@ struct ReplicaDev {
     int32_t handler;       // reaction handler of this replica (-1: the one the step is launched with)
     uint32_t integ;        // fused step: the walk of this step also integrates the next one
     unsigned long long energy;  // two's-complement int64, fixed point 2^-24
+    uint32_t topo_range;   // topology handles of the replica: n_topo in use, the rest reserved for fusions
+    uint32_t topo_pad;
 };
 
 // ---- potentials and reaction tables -----------------------------------------------------------

@@ -36,6 +36,8 @@ struct HostReplica {
     bool in_use = false;
     uint64_t cell_base = 0, n_cells = 0;
     uint32_t topo_base = 0, n_topo = 0;
+    uint32_t topo_range = 0;    // topology handle range claimed at upload: n_topo of the upload + the reserve
+    uint32_t topo_reserve = 0;  // handles reserved for topology-particle fusions (rdy_reserve_topology_particles)
     uint32_t tt_base = 0, n_tt = 0;  // topologies (their types) for spatial topology reactions
     bool has_particles = false;  // particles were uploaded since the replica was added
     ReplicaDev dev{};
@@ -195,6 +197,7 @@ struct rdy_engine {
     std::vector<std::pair<uint64_t, uint64_t>> free_tt;
     TopoEvent *tev = nullptr;
     uint32_t *tev_count = nullptr;
+    uint32_t *tfus = nullptr, *tfus_count = nullptr;  // topology-particle fusions of the last evaluation
     unsigned long long *str_step = nullptr, *str_total = nullptr;
     std::vector<BondTerm> terms_h[3];
     BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
@@ -281,6 +284,9 @@ int check_errors(rdy_engine *e) {
     if (h[ERR_LDS_LAYOUT]) return set_err(RDY_ERR_STATE, "brick kernels: LDS layout exceeds the reserved size");
     if (h[ERR_EVENT_LDS]) m += "more than " + std::to_string(RESOLVE_MAX_EVENTS) + " accepted events in one replica step; ";
     if (h[ERR_WL_OVERFLOW]) m += "particle pool exhausted: raise max_particles; ";
+    if (h[ERR_TOPO_RESERVE])
+        m += "topology-particle fusions beyond the replica's topology reserve (" + std::to_string(h[ERR_TOPO_RESERVE]) +
+             " skipped): raise it with rdy_reserve_topology_particles; ";
     if (h[ERR_EDIT_OVERFLOW])
         m += "changed-record list of the fused step overflowed (" + std::to_string(h[ERR_EDIT_OVERFLOW]) +
              "): too many reactions per step, reduce the time step; ";
@@ -469,6 +475,7 @@ int upload_tables(rdy_engine *e, double dt) {
             d.rate = x.rate;
             d.p_accept = 1 - std::exp(-x.rate * dt);
             d.radius2 = x.radius * x.radius;
+            d.fusion = x.fusion;
             sr.push_back(d);
         }
         if ((rc = up(&e->d_sr, sr))) return rc;
@@ -542,10 +549,10 @@ void release_topo(rdy_engine *e, HostReplica &H) {
     if (H.n_tt) release_range(e->free_tt, H.tt_base, H.n_tt);
     H.tt_base = 0;
     H.n_tt = 0;
-    if (H.n_topo == 0) return;
+    if (H.topo_range == 0) return;
     if (e->topo_owner)
-        (void)hipMemsetAsync(e->topo_owner + H.topo_base, 0xFF, H.n_topo * sizeof(uint32_t), e->stream);
-    release_range(e->free_topo, H.topo_base, H.n_topo);
+        (void)hipMemsetAsync(e->topo_owner + H.topo_base, 0xFF, H.topo_range * sizeof(uint32_t), e->stream);
+    release_range(e->free_topo, H.topo_base, H.topo_range);
     // a free range at the end of the used part shrinks it
     if (!e->free_topo.empty() && e->free_topo.back().first + e->free_topo.back().second == e->topo_used) {
         e->topo_used = e->free_topo.back().first;
@@ -553,6 +560,7 @@ void release_topo(rdy_engine *e, HostReplica &H) {
     }
     H.topo_base = 0;
     H.n_topo = 0;
+    H.topo_range = 0;
 }
 
 int alloc_pools(rdy_engine *e) {
@@ -648,6 +656,17 @@ template <class T>
 int get_field(rdy_engine *e, uint32_t r, size_t offset, T *v) {
     HIP_TRY(hipMemcpyAsync(v, (char *)(e->reps + r) + offset, sizeof(T), hipMemcpyDeviceToHost, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    return RDY_OK;
+}
+
+// the replica's topology particle count as the device's fusions left it
+int sync_topo_count(rdy_engine *e, uint32_t r) {
+    HostReplica &H = e->reps_h[r];
+    if (H.topo_range <= H.n_topo) return RDY_OK;  // no reserve: fusions cannot change the count
+    uint32_t n = 0;
+    int rc = get_field(e, r, offsetof(ReplicaDev, n_topo), &n);
+    if (rc) return rc;
+    H.n_topo = n;
     return RDY_OK;
 }
 
@@ -760,6 +779,9 @@ TopoTables topo_tables(const rdy_engine *e) {
     t.sr = e->d_sr;
     t.n_sr = (uint32_t)e->sr_in.size();
     t.topo_of = e->topo_of;
+    t.loc = e->loc;
+    t.fus = e->tfus;
+    t.fus_count = e->tfus_count;
     t.topo_owner = e->topo_owner;
     t.ttype = e->ttype;
     t.treact = e->treact;
@@ -1027,11 +1049,16 @@ void launch_forces(rdy_engine *e, double *force_out) {
     e->virial_valid = e->record_virial;
 }
 
-// force_out and the energies after a fused run are formed when first asked for
+// force_out and the energies after a fused run are formed when first asked for.  The type tables and
+// the brick tables (with the bin-reading kernel's brick list) exist only once an action has built them:
+// an export or an integration straight after an upload builds them here first (the tables' time step
+// only sets reaction probabilities, which a force evaluation does not read).
 int ensure_export(rdy_engine *e) {
     if (e->export_valid) return RDY_OK;
-    int rc = ensure_force_out(e);
+    int rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0);
     if (rc) return rc;
+    if ((rc = ensure_bricks(e))) return rc;
+    if ((rc = ensure_force_out(e))) return rc;
     launch_forces(e, e->force_out);
     e->export_valid = true;
     return RDY_OK;
@@ -1170,7 +1197,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->tfus, e->tfus_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1315,13 +1342,18 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
     // the bonded terms name the replaced particles' handles: the caller sets them again
     if ((rc = drop_terms(e, r))) return rc;
     release_topo(e, H);
+    // the topology handles: the upload's topology particles, then the reserve for fusions
+    const uint32_t spare = n_topo ? H.topo_reserve : 0u;
+    if ((uint64_t)n + spare > 0xFFFFFFF0ull) return set_err(RDY_ERR_INVALID, "particle handles beyond 32 bits");
+    const uint32_t range = n_topo + spare;
     uint64_t tbase = 0;
-    if (n_topo && take_range(e->free_topo, n_topo, &tbase)) {
+    if (range && take_range(e->free_topo, range, &tbase)) {
         H.topo_base = (uint32_t)tbase;
         H.n_topo = n_topo;
-    } else if (n_topo) {
-        if (e->topo_used + n_topo > e->topo_cap) {
-            uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + n_topo);
+        H.topo_range = range;
+    } else if (range) {
+        if (e->topo_used + range > e->topo_cap) {
+            uint64_t ncap = std::max<uint64_t>(2 * e->topo_cap, e->topo_used + range);
             long long *fb;
             uint32_t *loc, *tof, *town, *lcur;
             HIP_TRY(dalloc(&fb, 3 * ncap));
@@ -1352,24 +1384,27 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
         }
         H.topo_base = (uint32_t)e->topo_used;
         H.n_topo = n_topo;
-        e->topo_used += n_topo;
+        H.topo_range = range;
+        e->topo_used += range;
     } else {
         H.topo_base = 0;
         H.n_topo = 0;
+        H.topo_range = 0;
     }
-    if (H.n_topo) {  // the new range belongs to replica r; its particles have no topology yet
-        std::vector<uint32_t> own(H.n_topo, r), none(H.n_topo, 0xFFFFFFFFu);
-        HIP_TRY(hipMemcpyAsync(e->topo_owner + H.topo_base, own.data(), H.n_topo * 4, hipMemcpyHostToDevice,
+    if (H.topo_range) {  // the new range belongs to replica r; its particles have no topology yet
+        std::vector<uint32_t> own(H.topo_range, r), none(H.topo_range, 0xFFFFFFFFu);
+        HIP_TRY(hipMemcpyAsync(e->topo_owner + H.topo_base, own.data(), H.topo_range * 4, hipMemcpyHostToDevice,
                                e->stream));
-        HIP_TRY(hipMemcpyAsync(e->topo_of + H.topo_base, none.data(), H.n_topo * 4, hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(e->topo_of + H.topo_base, none.data(), H.topo_range * 4, hipMemcpyHostToDevice,
                                e->stream));
         HIP_TRY(hipStreamSynchronize(e->stream));
     }
     if ((rc = put_field(e, r, offsetof(ReplicaDev, tt_base), H.tt_base))) return rc;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, n_tt), H.n_tt))) return rc;
-    uint32_t hn = (uint32_t)n;
+    uint32_t hn = (uint32_t)n + spare;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, topo_base), H.topo_base))) return rc;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, n_topo), H.n_topo))) return rc;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, topo_range), H.topo_range))) return rc;
     if ((rc = put_field(e, r, offsetof(ReplicaDev, handle_next), hn))) return rc;
     if (n) {
         size_t bytes = n * (3 * sizeof(double) + sizeof(uint32_t));
@@ -1383,7 +1418,7 @@ int rdy_set_particles(rdy_engine *e, uint32_t r, uint64_t n, const double *pos, 
             HIP_TRY(hipMemcpyAsync(e->d_flavor, e->topo_flavor.data(), e->n_types, hipMemcpyHostToDevice, e->stream));
         }
         hipLaunchKernelGGL(k_upload, dim3(blocks_for(n, 256)), dim3(256), 0, e->stream, e->grid(e->cur), e->reps, r,
-                           (uint32_t)n, dpos, dty, (const uint8_t *)e->d_flavor, e->topo(), e->cap,
+                           (uint32_t)n, n_topo, spare, dpos, dty, (const uint8_t *)e->d_flavor, e->topo(), e->cap,
                            (uint32_t)e->max_particles, e->err);
     }
     e->wl_valid[e->cur] = false;
@@ -1449,6 +1484,7 @@ static int set_terms(rdy_engine *e, uint32_t r, int kind, uint64_t n, const uint
     int rc = check_replica(e, r);
     if (rc) return rc;
     if (n && (!idx || !k || !p0)) return set_err(RDY_ERR_INVALID, "null bonded term arrays");
+    if ((rc = sync_topo_count(e, r))) return rc;
     const HostReplica &H = e->reps_h[r];
     // every index is checked before anything changes: a rejected call leaves the terms as they were
     std::vector<BondTerm> fresh;
@@ -2254,10 +2290,16 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         const rdy_spatial_reaction &x = sr[i];
         if (x.type1 >= e->n_types || x.type2 >= e->n_types || x.type_to1 >= e->n_types || x.type_to2 >= e->n_types)
             return set_err(RDY_ERR_INVALID, "spatial topology reaction species out of range");
+        if (x.fusion > 1 || x.reserved) return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion is 0 or 1");
         if (!e->topo_flavor[x.type1] || !e->topo_flavor[x.type_to1])
             return set_err(RDY_ERR_INVALID, "the topology side of a topology-particle reaction must stay a topology species");
-        if (e->topo_flavor[x.type2] || e->topo_flavor[x.type_to2])
-            return set_err(RDY_ERR_INVALID, "the particle side of a topology-particle reaction must stay a normal species");
+        if (e->topo_flavor[x.type2])
+            return set_err(RDY_ERR_INVALID, "the particle side of a topology-particle reaction must be a normal species");
+        if (!x.fusion && e->topo_flavor[x.type_to2])
+            return set_err(RDY_ERR_INVALID, "the particle side of a topology-particle enzymatic reaction must stay a "
+                                            "normal species");
+        if (x.fusion && !e->topo_flavor[x.type_to2])
+            return set_err(RDY_ERR_INVALID, "a topology-particle fusion's joining particle must become a topology species");
         if (!(x.rate > 0))
             return set_err(RDY_ERR_INVALID, "The rate of an structural topology reaction should always be positive");
         if (!(x.radius > 0) || x.radius > e->nl_cutoff)
@@ -2272,6 +2314,14 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         HIP_TRY(dalloc(&e->tev, nr * TOPO_MAX_EVENTS));
         HIP_TRY(dalloc(&e->tev_count, nr));
         HIP_TRY(hipMemsetAsync(e->tev_count, 0, nr * sizeof(uint32_t), e->stream));
+    }
+    bool fusion = false;
+    for (uint32_t i = 0; i < n; ++i) fusion = fusion || sr[i].fusion;
+    if (fusion && !e->tfus) {
+        const size_t nr = e->max_reps;
+        HIP_TRY(dalloc(&e->tfus, nr * TOPO_MAX_EVENTS * 4));
+        HIP_TRY(dalloc(&e->tfus_count, nr));
+        HIP_TRY(hipMemsetAsync(e->tfus_count, 0, nr * sizeof(uint32_t), e->stream));
     }
     if (e->str_step) HIP_TRY(hipFree(e->str_step));
     if (e->str_total) HIP_TRY(hipFree(e->str_total));
@@ -2292,6 +2342,7 @@ int rdy_set_topologies(rdy_engine *e, uint32_t r, uint32_t n_topologies, const u
     int rc = check_replica(e, r);
     if (rc) return rc;
     if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if ((rc = sync_topo_count(e, r))) return rc;
     HostReplica &H = e->reps_h[r];
     if (n_topologies && !topology_types) return set_err(RDY_ERR_INVALID, "null topology types");
     if (H.n_topo && !topology_of) return set_err(RDY_ERR_INVALID, "null topology assignment");
@@ -2355,6 +2406,31 @@ int rdy_get_topology_types(rdy_engine *e, uint32_t r, uint32_t cap, uint32_t *ty
     return RDY_OK;
 }
 
+int rdy_reserve_topology_particles(rdy_engine *e, uint32_t r, uint32_t n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (n > (1u << 30)) return set_err(RDY_ERR_INVALID, "topology reserve beyond 2^30 handles");
+    e->reps_h[r].topo_reserve = n;
+    return RDY_OK;
+}
+
+int rdy_get_topology_fusions(rdy_engine *e, uint32_t r, uint32_t cap, uint32_t *rows, uint32_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    uint32_t m = 0;
+    if (e->tfus_count) {
+        HIP_TRY(hipMemcpyAsync(&m, e->tfus_count + r, 4, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    if (n) *n = m;
+    const uint32_t k = std::min(cap, m);
+    if (k && rows) {
+        HIP_TRY(hipMemcpyAsync(rows, e->tfus + 4ull * r * TOPO_MAX_EVENTS, 16ull * k, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return sync_topo_count(e, r);
+}
+
 int rdy_evaluate_topology_reactions(rdy_engine *e, double dt) {
     if (!e || !e->pools_ready) return set_err(RDY_ERR_STATE, "engine has no replicas");
     if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
@@ -2388,6 +2464,7 @@ int rdy_set_particle_types(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t
     if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
     if (n == 0) return RDY_OK;
     if (!handles || !types) return set_err(RDY_ERR_INVALID, "null handles or types");
+    if ((rc = sync_topo_count(e, r))) return rc;
     const HostReplica &H = e->reps_h[r];
     for (uint64_t i = 0; i < n; ++i) {
         if (handles[i] >= H.n_topo)

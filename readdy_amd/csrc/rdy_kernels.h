@@ -104,7 +104,8 @@ struct DbTables {
 
 // error flags, all counters
 enum : int { ERR_BIN_OVERFLOW = 0, ERR_EVENT_OVERFLOW = 1, ERR_WL_OVERFLOW = 2, ERR_NONFINITE = 3, ERR_EVENT_LDS = 4,
-             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, ERR_FORCE_RANGE = 8, N_ERR = 9 };
+             ERR_LDS_SEARCH = 5, ERR_LDS_LAYOUT = 6, ERR_EDIT_OVERFLOW = 7, ERR_FORCE_RANGE = 8, ERR_TOPO_RESERVE = 9,
+             N_ERR = 10 };
 
 constexpr int MODE_FORCES = 0;  // forces + energy of the current buffer
 constexpr int MODE_STEP = 1;    // forces, Euler-Maruyama step, re-bin into the other buffer

@@ -1355,8 +1355,10 @@ __global__ void k_release_cells(Grid g, uint32_t base, uint32_t n) {
 // -------------------------------------------------------------------------------------------------
 // upload, gather, observables, test exports
 // -------------------------------------------------------------------------------------------------
-__global__ void k_upload(Grid g, ReplicaDev *reps, uint32_t r, uint32_t n, const double *pos, const uint32_t *types,
-                         const uint8_t *topo_flavor, Topo topo, uint32_t cap, uint32_t max_wl, uint32_t *err) {
+// handles: the topology particles (the first n_topo) 0 .. n_topo - 1, the free ones from n_topo + spare on
+__global__ void k_upload(Grid g, ReplicaDev *reps, uint32_t r, uint32_t n, uint32_t n_topo, uint32_t spare,
+                         const double *pos, const uint32_t *types, const uint8_t *topo_flavor, Topo topo, uint32_t cap,
+                         uint32_t max_wl, uint32_t *err) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const ReplicaDev &R = reps[r];
@@ -1368,7 +1370,7 @@ __global__ void k_upload(Grid g, ReplicaDev *reps, uint32_t r, uint32_t n, const
     const uint32_t t = types[i];
     const bool is_topo = topo_flavor && topo_flavor[t];
     st.tw = make_tw(t, is_topo ? TW_TOPO : 0u, r);
-    st.h = i;
+    st.h = i < n_topo ? i : i + spare;
     uint32_t slot = bin_insert(g, geo, cap, st, err, false, max_wl);
     if (is_topo && topo.loc && slot != 0xFFFFFFFFu) topo.loc[R.topo_base + i] = slot;
 }

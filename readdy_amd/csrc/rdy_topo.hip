@@ -1,5 +1,5 @@
-// Spatial topology reactions of the topology-particle enzymatic kind, "T1(p1) + (p2) -> T2(p3) + (p4)"
-// (SpatialTopologyReaction.h:64-236; CPUEvaluateTopologyReactions.cpp:88-475).
+// Spatial topology reactions of the topology-particle kinds, enzymatic "T1(p1) + (p2) -> T2(p3) + (p4)" and
+// fusion "T1(p1) + (p2) -> T2(p3--p4)" (SpatialTopologyReaction.h:64-236; CPUEvaluateTopologyReactions.cpp:88-475).
 //
 //   k_topo_search   one thread per topology particle: every live particle of no topology within the
 //                   reaction's radius in the particle's stencil (distSquared < radius^2), for each reaction
@@ -11,11 +11,14 @@
 //                   (performEvents, algorithm.h:62-136, with eventsDependent, :455-464: events of one
 //                   topology depend on each other, and so do events sharing the free particle), then the
 //                   winners' type changes: the topology particle takes p3, the free particle p4, the
-//                   topology the type T2 (handleTopologyParticleReaction, :369-404)
+//                   topology the type T2 (handleTopologyParticleReaction, :369-404); a fusion's free
+//                   particle joins the topology under the next handle of the replica's topology reserve
+//                   (allocated in key order) and is listed for the host, which adds the vertex and its
+//                   edge to the graph (appendParticle) and sets the bonded terms again
 //
 // In the fused step both run after the step's reactions, on the post-reaction state, and list the
 // records they retype for the fix-up (their moves, and those of their neighbours whose pair force
-// changed, are redone).  Particles keep their handles.  The bonded terms are not reconfigured here: a
+// changed, are redone).  Particles keep their handles, but for a fusion's joining particle.  The bonded terms are not reconfigured here: a
 // reaction that changes a topology particle's species keeps that particle's bonded parameters (the host
 // core reconfigures the topology when the species change alters them).
 #pragma once
@@ -27,6 +30,7 @@ namespace rdy {
 struct SpatialDev {
     uint32_t type1, top_type1, type2, type_to1, top_type_to1, type_to2;
     double rate, p_accept, radius2;
+    uint32_t fusion, pad;
 };
 
 struct TopoEvent {
@@ -41,7 +45,10 @@ static_assert(sizeof(TopoEvent) == 32, "TopoEvent layout");
 struct TopoTables {
     const SpatialDev *sr;
     uint32_t n_sr;
-    const uint32_t *topo_of;     // [topology particle slot] topology index within its replica
+    uint32_t *topo_of;           // [topology particle slot] topology index within its replica
+    uint32_t *loc;               // [topology particle slot] the record's bin slot (written for joining particles)
+    uint32_t *fus;               // [replica][TOPO_MAX_EVENTS][4] the last evaluation's fusions
+    uint32_t *fus_count;         // [replica]
     const uint32_t *topo_owner;  // [topology particle slot] replica, UINT32_MAX when unused
     uint32_t *ttype;             // topology types; replica r's at reps[r].tt_base
     uint32_t *treact;            // per topology (as ttype): the serial of the evaluation it last reacted in
@@ -131,6 +138,8 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
     __shared__ uint32_t order[TOPO_MAX_EVENTS];
     __shared__ uint8_t won[TOPO_MAX_EVENTS];
     __shared__ uint32_t used_top[TOPO_HASH], used_free[TOPO_HASH];
+    __shared__ uint32_t fidx[TOPO_MAX_EVENTS];  // a won fusion's number: its joining particle's reserve handle
+    __shared__ uint32_t s_nf;                    // fusions performed
     const uint32_t r = blockIdx.x, tid = threadIdx.x;
     ReplicaDev &R = reps[r];
     if (!R.in_use || (require_active && !R.active)) {
@@ -138,6 +147,7 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         return;
     }
     if (tid < TT.n_sr) TT.cnt_step[(size_t)r * TT.n_sr + tid] = 0ull;
+    if (tid == 0 && TT.fus_count) TT.fus_count[r] = 0u;
     uint32_t ne = TT.ev_count[r];
     ne = ne < TT.ev_cap ? ne : TT.ev_cap;
     if (ne > TOPO_MAX_EVENTS) {
@@ -177,7 +187,9 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         order[rank] = k;
     }
     __syncthreads();
+    const uint32_t n_topo0 = R.n_topo, reserve = R.topo_range > n_topo0 ? R.topo_range - n_topo0 : 0u;
     if (tid == 0) {  // the greedy pass in key order (few accepted events per step)
+        uint32_t nf = 0;
         for (uint32_t q = 0; q < ne; ++q) {
             const TopoEvent a = ev[order[q]];
             bool top_free = true, partner_free = true;
@@ -204,10 +216,18 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
                 }
             }
             if (!top_free || !partner_free) continue;
+            if (TT.sr[a.sr].fusion) {  // the joining particle takes the next reserve handle, in key order
+                if (nf >= reserve || !TT.fus) {
+                    atomicAdd(&err[ERR_TOPO_RESERVE], 1u);
+                    continue;
+                }
+                fidx[order[q]] = nf++;
+            }
             hash_insert(used_top, a.top);
             hash_insert(used_free, a.s_free);
             won[order[q]] = 1;
         }
+        s_nf = nf;
     }
     __syncthreads();
     const bool fix = edits != nullptr;
@@ -225,15 +245,34 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         // redoes its move on the final state; every other retyped record is listed once here
         const uint32_t n1 = fix && !(tw1 & TW_NEW) ? TW_NEW : 0u, n2 = fix && !(tw2 & TW_NEW) ? TW_NEW : 0u;
         cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
-        cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
+        if (X.fusion) {  // appendParticle: the free particle joins topology a.top under a reserve handle
+            const uint32_t hn = n_topo0 + fidx[k], ti = R.topo_base + hn;
+            cur.bins[a.s_free].tw = make_tw(X.type_to2, TW_TOPO | (tw2 & (TW_OUTSIDE | TW_NEW)), r) | n2;
+            cur.bins[a.s_free].h = hn;
+            TT.loc[ti] = a.s_free;
+            TT.topo_of[ti] = a.top;
+            uint32_t *row = TT.fus + 4ull * ((size_t)r * TOPO_MAX_EVENTS + fidx[k]);
+            row[0] = a.top;
+            row[1] = a.h_top;
+            row[2] = a.h_free;
+            row[3] = hn;
+        } else {
+            cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
+        }
         TT.ttype[R.tt_base + a.top] = X.top_type_to1;
         if (TT.serial) TT.treact[R.tt_base + a.top] = TT.serial;
         atomicAdd(&TT.cnt_step[(size_t)r * TT.n_sr + a.sr], 1ull);
         atomicAdd(&TT.cnt_total[(size_t)r * TT.n_sr + a.sr], 1ull);
         if (n1) edit(a.s_top, tw_type(tw1), a.h_top);
-        if (n2) edit(a.s_free, tw_type(tw2), a.h_free);
+        if (n2) edit(a.s_free, tw_type(tw2), a.h_free);  // (its moved copy carries the handle it had)
     }
-    if (tid == 0) TT.ev_count[r] = 0u;
+    if (tid == 0) {
+        TT.ev_count[r] = 0u;
+        if (s_nf) {  // every thread read n_topo0 before the greedy pass's barrier
+            R.n_topo = n_topo0 + s_nf;
+            TT.fus_count[r] = s_nf;
+        }
+    }
 }
 
 // sets the species of topology particles of replica r (by handle, found through loc) and their

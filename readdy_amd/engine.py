@@ -306,14 +306,16 @@ class Engine:
         fn = self._lib.rdy_run_graph if graph else self._lib.rdy_run
         L.check(fn(self._h, handler, dt, int(n_steps)))
 
-    # ---- spatial topology reactions (topology-particle enzymatic) ----
+    # ---- spatial topology reactions (topology-particle enzymatic and fusion) ----
     def set_spatial_reactions(self, reactions):
-        """reactions: dicts with type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius
-        ("T1(type1) + (type2) -> T2(type_to1) + (type_to2)")"""
+        """reactions: dicts with type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius and
+        optionally fusion ("T1(type1) + (type2) -> T2(type_to1) + (type_to2)", with fusion=1
+        "T1(type1) + (type2) -> T2(type_to1--type_to2)")"""
         arr = (L.SpatialReaction * max(len(reactions), 1))()
         for i, x in enumerate(reactions):
             for f in ("type1", "top_type1", "type2", "type_to1", "top_type_to1", "type_to2", "rate", "radius"):
                 setattr(arr[i], f, x[f])
+            arr[i].fusion = int(x.get("fusion", 0))
         self._sr, self._n_sr = arr, len(reactions)
         L.check(self._lib.rdy_set_spatial_reactions(self._h, len(reactions), ctypes.cast(arr, ctypes.c_void_p)))
 
@@ -328,6 +330,19 @@ class Engine:
         L.check(self._lib.rdy_get_topology_types(self._h, r, 0, None, ctypes.byref(n)))
         out = np.zeros(max(n.value, 1), dtype=np.uint32)
         L.check(self._lib.rdy_get_topology_types(self._h, r, n.value, _ptr(out, _u32p), ctypes.byref(n)))
+        return out[:n.value]
+
+    def reserve_topology_particles(self, r, n):
+        """handles for particles that join topologies by fusion; takes effect with the next set_particles"""
+        L.check(self._lib.rdy_reserve_topology_particles(self._h, r, int(n)))
+
+    def topology_fusions(self, r=0):
+        """rows (topology, topology particle handle, joining particle's old handle, its new handle) of the last
+        evaluation's fusions, in the order they were performed"""
+        n = ctypes.c_uint32()
+        L.check(self._lib.rdy_get_topology_fusions(self._h, r, 0, None, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 4), dtype=np.uint32)
+        L.check(self._lib.rdy_get_topology_fusions(self._h, r, n.value, _ptr(out, _u32p), ctypes.byref(n)))
         return out[:n.value]
 
     def evaluate_topology_reactions(self, dt):

@@ -323,6 +323,14 @@ void MI355XStateModel::flush() {
     for (const auto &p : all) put(p);
     for (const auto &p : _freeParticles) put(p);
     _k->replica(types.size());
+    // topology-particle fusion: the joining particles take handles of a reserve after the topology particles
+    // (at most every free particle of this upload joins once, and some made by reactions later)
+    bool fusion = false;
+    for (const auto &r : _k->context().topologyRegistry().spatialReactions()) fusion = fusion || r.mode == STRMode::TP_FUSION;
+    if (fusion && !_uploaded)
+        ok(rdy_reserve_topology_particles(_k->engine(), _k->replicaId(),
+                                          (uint32_t)std::max<std::size_t>(1024, 2 * _freeParticles.size())),
+           "topology reserve");
     ok(rdy_set_particles(_k->engine(), _k->replicaId(), types.size(), pos.data(), types.data()), "set particles");
     _freeParticles.clear();
     _dirty = false;
@@ -340,10 +348,10 @@ void MI355XStateModel::configureSpatialReactions() {
     if (_spatialSet) return;
     std::vector<rdy_spatial_reaction> v;
     for (const auto &r : _k->context().topologyRegistry().spatialReactions()) {
-        if (r.mode != STRMode::TP_ENZYMATIC)
+        if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION)
             throw std::invalid_argument("spatial topology reaction \"" + r.name +
-                                        "\": the MI355X kernel performs the topology-particle enzymatic kind "
-                                        "(T1(p1) + (p2) -> T2(p3) + (p4)) only");
+                                        "\": the MI355X kernel performs the topology-particle kinds "
+                                        "(T1(p1) + (p2) -> T2(p3) + (p4) and T1(p1) + (p2) -> T2(p3--p4)) only");
         rdy_spatial_reaction q{};
         q.type1 = r.type1;
         q.top_type1 = (uint32_t)r.topType1;
@@ -353,6 +361,7 @@ void MI355XStateModel::configureSpatialReactions() {
         q.type_to2 = r.typeTo2;
         q.rate = r.rate;
         q.radius = r.radius;
+        q.fusion = r.mode == STRMode::TP_FUSION ? 1u : 0u;
         v.push_back(q);
     }
     if (!v.empty()) ok(rdy_set_spatial_reactions(_k->engine(), (uint32_t)v.size(), v.data()), "spatial reactions");
@@ -400,8 +409,10 @@ bool MI355XStateModel::reactionsChangeBondedTerms() const {
                 return true;
         return false;
     };
-    for (const auto &r : reg.spatialReactions())
+    for (const auto &r : reg.spatialReactions()) {
+        if (r.mode == STRMode::TP_FUSION) return true;  // every fusion adds an edge
         if (r.typeTo1 != r.type1 && (named(r.type1) || named(r.typeTo1))) return true;
+    }
     return false;
 }
 
@@ -505,6 +516,22 @@ void MI355XStateModel::spatialReactions(scalar dt) {
             retyped = retyped || sr[i].typeTo1 != sr[i].type1;
         }
     if (!any) return;
+    // fusions: the joining particle is the topology particle of the next reserve handle
+    // (handleTopologyParticleReaction: appendParticle(idx2, idx1), CPUEvaluateTopologyReactions.cpp:369-404)
+    uint32_t nf = 0;
+    ok(rdy_get_topology_fusions(_k->engine(), _k->replicaId(), 0, nullptr, &nf), "topology fusions");
+    if (nf) {
+        std::vector<uint32_t> rows(4ull * nf);
+        ok(rdy_get_topology_fusions(_k->engine(), _k->replicaId(), nf, rows.data(), &nf), "topology fusions");
+        for (uint32_t i = 0; i < nf; ++i) {
+            const uint32_t top = rows[4 * i], h_top = rows[4 * i + 1], h_new = rows[4 * i + 3];
+            if (h_new != _topoParticles.size() || top >= _topologies.size())
+                throw std::logic_error("topology fusion rows out of order");
+            _topoParticles.emplace_back(0., 0., 0., 0, h_new);  // species and position from the device below
+            _topologies[top]->appendParticle(h_new, h_top);
+        }
+        retyped = true;
+    }
     _topoStale = true;
     syncTopologies();
     if (retyped) {  // GraphTopology::configure after the reaction (TopologyReactionActions.h)

@@ -453,6 +453,17 @@ void GraphTopology::addEdge(std::size_t v1, std::size_t v2) {
     _neighbors[v2].push_back(v1);
 }
 
+void GraphTopology::appendParticle(std::size_t particle, std::size_t counterpart) {
+    const auto it = std::find(_particles.begin(), _particles.end(), counterpart);
+    if (it == _particles.end()) throw std::invalid_argument("appendParticle: the counterpart is not in this topology");
+    if (std::find(_particles.begin(), _particles.end(), particle) != _particles.end())
+        throw std::invalid_argument("appendParticle: the particle is in this topology already");
+    const std::size_t v = (std::size_t)(it - _particles.begin());
+    _particles.push_back(particle);
+    _neighbors.emplace_back();
+    addEdge(v, _particles.size() - 1);
+}
+
 bool GraphTopology::hasEdge(std::size_t v1, std::size_t v2) const {
     if (v1 >= _particles.size() || v2 >= _particles.size()) return false;
     return std::find(_neighbors[v1].begin(), _neighbors[v1].end(), v2) != _neighbors[v1].end();

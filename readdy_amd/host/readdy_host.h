@@ -273,6 +273,9 @@ public:
     GraphTopology(std::string type, std::vector<std::size_t> particleIndices)
         : _type(std::move(type)), _particles(std::move(particleIndices)), _neighbors(_particles.size()) {}
     void addEdge(std::size_t v1, std::size_t v2);  // vertex indices into particleIndices()
+    // GraphTopology::appendParticle (GraphTopology.cpp): a new vertex for particle `particle`, bonded to the
+    // vertex of particle `counterpart`
+    void appendParticle(std::size_t particle, std::size_t counterpart);
     bool removeEdge(std::size_t v1, std::size_t v2);  // false when there is no such edge
     bool hasEdge(std::size_t v1, std::size_t v2) const;
     std::size_t nParticles() const { return _particles.size(); }

@@ -928,6 +928,68 @@ static void test_topology_reactions() {
     }
 }
 
+// Topology-particle fusion through the plugin ("grow: Polymer(end) + (mono) -> Polymer(mid--end)", rate 1e10):
+// five chains mid-mid-end, each with a monomer 1.1 from its end and a second one 1.1 beyond it (2.2 from the
+// end, outside the radius 1.2).  One evaluation grows every chain by its first monomer (the end becomes mid,
+// the monomer the new end, under a topology handle after the 15 chain particles), the next by the second;
+// the new edges carry the mid-end bond (k 10, length 1), stretched by 0.1: 10 * 0.01 per join.
+static void test_topology_fusion() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {40., 40., 40.};
+    c.particleTypes().addTopologyType("mid", 0.1);
+    c.particleTypes().addTopologyType("end", 0.1);
+    c.particleTypes().add("mono", 1.);
+    auto &reg = c.topologyRegistry();
+    reg.addType("Polymer");
+    reg.configureBondPotential("mid", "mid", {10., 1.});
+    reg.configureBondPotential("mid", "end", {10., 1.});
+    reg.addSpatialReaction("grow: Polymer(end) + (mono) -> Polymer(mid--end)", 1e10, 1.2);
+    const auto mid = c.particleTypes().idOf("mid"), end = c.particleTypes().idOf("end"),
+               mono = c.particleTypes().idOf("mono");
+    auto &sm = k->stateModel();
+    std::vector<model::Particle> monos;
+    for (int t = 0; t < 5; ++t) {
+        const double y = -10. + 4. * t;
+        std::vector<model::Particle> ps;
+        for (int i = 0; i < 3; ++i) ps.emplace_back((double)i, y, 0., i == 2 ? end : mid);
+        auto *top = sm.addTopology("Polymer", ps);
+        top->addEdge(0, 1);
+        top->addEdge(1, 2);
+        monos.emplace_back(3.1, y, 0., mono);
+        monos.emplace_back(4.2, y, 0., mono);
+    }
+    sm.addParticles(monos);
+    k->actions().calculateForces()->perform();
+    CHECK(std::fabs(k->stateModel().energy()) < 1e-9, "chains at rest: energy %.3g", k->stateModel().energy());
+    for (int round = 1; round <= 2; ++round) {
+        k->actions().evaluateTopologyReactions(0.01)->perform();
+        int n_mid = 0, n_end = 0, n_mono = 0;
+        std::size_t max_topo_id = 0;
+        for (const auto &p : sm.getParticles()) {
+            n_mid += p.type() == mid;
+            n_end += p.type() == end;
+            n_mono += p.type() == mono;
+            if (p.type() != mono) max_topo_id = std::max<std::size_t>(max_topo_id, p.id());
+        }
+        CHECK(n_mid == 5 * (2 + round) && n_end == 5 && n_mono == 10 - 5 * round,
+              "fusion round %d: mid %d end %d mono %d", round, n_mid, n_end, n_mono);
+        CHECK(max_topo_id == 15u + 5u * round - 1u, "round %d: joined particles take handles up to %zu", round,
+              max_topo_id);
+        for (auto *t : sm.getTopologies())
+            CHECK(t->nParticles() == (std::size_t)(3 + round) && t->isConnected() && t->type() == "Polymer",
+                  "round %d: a chain of %zu particles", round, t->nParticles());
+        k->actions().calculateForces()->perform();
+        const double e = k->stateModel().energy();
+        CHECK(std::fabs(e - 5 * round * 10. * 0.01) < 1e-6, "round %d: bond energy %.9g", round, e);
+    }
+    api::SimulationLoop loop(k.get(), 0.001);
+    loop.run(10);
+    CHECK(sm.getParticles().size() == 25u && sm.getTopologies().size() == 5u, "after 10 steps: %zu particles",
+          sm.getParticles().size());
+    std::printf("topology fusion: 5 chains grew by two monomers each\n");
+}
+
 // Kernel::observe(): observables connected to the kernel and evaluated by the SimulationLoop on the
 // device-fused path in chunks that end where one is due (strides 0, 2, 5 and 10 over 10 steps), checked
 // against the oracle on the state they saw: energy, virial, particles per type, the radial distribution
@@ -1063,6 +1125,7 @@ int main(int argc, char **argv) {
             test_topology();
             test_break_bonds();
             test_topology_reactions();
+            test_topology_fusion();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

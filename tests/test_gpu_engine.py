@@ -110,6 +110,47 @@ def _force_case(pots, n_types, pos, types, box, pbc=(1, 1, 1), **kw):
     return rel, e_rel, np.linalg.norm(f - f_ref) / pair_scale
 
 
+def test_forces_formed_on_demand_before_any_neighbour_list():
+    """Exports and the separate integrator form the forces on demand when the exported ones are stale; straight
+    after an upload nothing has built the brick tables (nor the bin-reading kernel's brick list) yet, so the
+    on-demand evaluation builds them first.  Switching the virial on right after the upload makes the exported
+    forces stale without a walk; every first call then gives what an explicit force evaluation gives."""
+    print("on-demand forces after upload", flush=True)
+    pots = [(L.POT_HARMONIC_REPULSION, 0, 0, [10.0, 1.0])]
+    rng = np.random.default_rng(31)
+    pos = rng.uniform(-10, 10, (8, 3))
+    pos[1] = pos[0] + [0.6, 0, 0]  # one overlapping pair: a nonzero force
+    types = np.zeros(8, np.uint32)
+    noise = rng.normal(size=(8, 3))
+
+    def fresh(explicit):
+        e, r = _engine(1, [20.0] * 3, (1, 1, 1), pots, n=8)
+        e.set_particles(r, pos, types)
+        e.record_virial(True)
+        if explicit:
+            e.calculate_forces()
+        return e, r
+
+    e, r = fresh(True)
+    f_ref, e_ref, v_ref = e.get_forces(r)[0], e.energy(r), e.virial(r)
+    e.integrate(0.01, noise)
+    x_ref = e.get_particles(r)[0]
+    e.close()
+    assert abs(f_ref).max() > 0 and e_ref > 0
+    for first in ("forces", "energy", "virial", "integrate"):
+        e, r = fresh(False)
+        if first == "forces":
+            assert np.array_equal(e.get_forces(r)[0], f_ref)
+        elif first == "energy":
+            assert e.energy(r) == e_ref
+        elif first == "virial":
+            assert np.array_equal(e.virial(r), v_ref)
+        else:
+            e.integrate(0.01, noise)
+            assert np.array_equal(e.get_particles(r)[0], x_ref)
+        e.close()
+
+
 def test_forces_harmonic_repulsion_c0():
     pos, types = _system_c0like()
     pots = [(L.POT_HARMONIC_REPULSION, a, b, [10.0, 1.0]) for a in range(3) for b in range(a, 3)]

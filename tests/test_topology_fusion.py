@@ -5,11 +5,9 @@ The oracle is pinned on a hand-built polymer-growth scene whose outcome follows 
 1e10 accepts with probability 1 in FP64): every chain end with a free monomer inside the radius grows by that
 monomer, which joins under the next reserve id and is the new end, so that the next evaluation grows the chain
 from it (a wrong id or topology for the joining particle, or a dropped type change, fails this); a monomer
-shared by two ends joins one of them; a fusion beyond the reserve is an error.
-
-The device tests below (the same scenes, random chains growing among monomers event for event against the
-oracle, the fused loop against the separate actions) are skipped: the device path of this kind was taken out
-after its first GPU run ended in a memory fault whose cause was not found.
+shared by two ends joins one of them; a fusion beyond the reserve is an error.  On the device (gpu): the same
+scenes, random chains growing among monomers event for event against the oracle (ids, species, fusion rows,
+the bonds of the grown edges), and the fused loop against the separate actions bit for bit.
 """
 import numpy as np
 import pytest
@@ -91,10 +89,6 @@ def test_oracle_enzymatic_and_fusion_differ_only_in_the_join():
 
 
 # ------------------------------------------------------------------------------------------------ device
-DEVICE_OFF = pytest.mark.skip(reason="topology-particle fusion on the device is switched off: its first GPU run "
-                                     "(test_polymer_growth_scene_on_the_device) faulted and the cause was not found; "
-                                     "the engine has no fusion entry points")
-
 def _engine(box, reserve, n, pots=(), rxs=(), seed=3, nl_cutoff=1.0):
     from readdy_amd import Engine
     e = Engine(3, [1.0, 1.0, 1.0], potentials=list(pots), reactions=list(rxs), topology_flavor=FLAVOR,
@@ -105,7 +99,6 @@ def _engine(box, reserve, n, pots=(), rxs=(), seed=3, nl_cutoff=1.0):
 
 
 @pytest.mark.gpu
-@DEVICE_OFF
 def test_polymer_growth_scene_on_the_device():
     print("fusion scene", flush=True)
     e, r = _engine(20.0, 3, len(SCENE_TYPES))
@@ -131,7 +124,6 @@ def test_polymer_growth_scene_on_the_device():
 
 
 @pytest.mark.gpu
-@DEVICE_OFF
 def test_fusion_beyond_the_reserve_is_a_capacity_error():
     from readdy_amd import _lib as L
     print("fusion reserve", flush=True)
@@ -165,7 +157,6 @@ def _growth_system(n_chains, n_mono, box, seed):
 
 
 @pytest.mark.gpu
-@DEVICE_OFF
 @pytest.mark.parametrize("rate", [5.0, 1e10])
 def test_random_growth_event_for_event(rate):
     """40 chains grow among 1500 monomers over five integrated steps: the same monomers join the same chains
@@ -224,7 +215,6 @@ def test_random_growth_event_for_event(rate):
 
 
 @pytest.mark.gpu
-@DEVICE_OFF
 def test_fused_loop_equals_separate_actions_with_fusion():
     from readdy_amd import _lib as L
     print("fusion fused loop", flush=True)
