@@ -274,6 +274,15 @@ int rdy_reserve_topology_particles(rdy_engine *e, uint32_t replica, uint32_t n);
  * handle before, its handle now); *n = their count.  The replica's topology particles are then handles
  * 0 .. n_topo - 1 with n_topo grown by the fusions (rdy_set_topologies takes that many entries). */
 int rdy_get_topology_fusions(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *rows, uint32_t *n);
+/* Recipe steps of structural topology reactions that move or add topology particles (TopologyReactionActions:
+ * ChangeParticlePosition, AppendParticle; Recipe.h:40-132).  rdy_set_particle_positions moves topology particles
+ * (by handle) and re-bins the engine's particles, as a neighbour-list update does.  rdy_append_topology_particles
+ * adds n particles of topology species at the given positions under the next handles of the replica's topology
+ * reserve (rdy_reserve_topology_particles), returned in handles[n]; the caller then assigns them to topologies
+ * (rdy_set_topologies) and sets the bonded terms.  Beyond the reserve: RDY_ERR_CAPACITY. */
+int rdy_set_particle_positions(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *handles, const double *pos);
+int rdy_append_topology_particles(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *types,
+                                  const double *pos, uint32_t *handles);
 /* the replica's topology types as the topology reactions left them; *n = the number of topologies */
 int rdy_get_topology_types(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *types, uint32_t *n);
 /* per topology, 1 if a spatial topology reaction changed it in the last rdy_evaluate_topology_reactions call:

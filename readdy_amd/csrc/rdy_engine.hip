@@ -2490,6 +2490,75 @@ int rdy_set_particle_types(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t
     return RDY_OK;
 }
 
+int rdy_set_particle_positions(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *handles, const double *pos) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if (n == 0) return RDY_OK;
+    if (!handles || !pos) return set_err(RDY_ERR_INVALID, "null handles or positions");
+    if ((rc = sync_topo_count(e, r))) return rc;
+    const HostReplica &H = e->reps_h[r];
+    for (uint64_t i = 0; i < n; ++i) {
+        if (handles[i] >= H.n_topo)
+            return set_err(RDY_ERR_INVALID, "only topology particles (handles below the replica's topology count) "
+                                            "can be moved here");
+        for (int d = 0; d < 3; ++d)
+            if (!std::isfinite(pos[3 * i + d])) return set_err(RDY_ERR_INVALID, "non-finite particle position");
+    }
+    if ((rc = ensure_scratch(e, (size_t)n * (4 + 24) + 16))) return rc;
+    double *dp = (double *)e->scratch;
+    uint32_t *dh = (uint32_t *)(dp + 3 * n), *bad = dh + n;
+    HIP_TRY(hipMemcpyAsync(dp, pos, n * 24, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(dh, handles, n * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemsetAsync(bad, 0, 4, e->stream));
+    hipLaunchKernelGGL(k_set_topo_positions, dim3(blocks_for((uint32_t)n, 256)), dim3(256), 0, e->stream,
+                       e->grid(e->cur), (const ReplicaDev *)e->reps, r, (const uint32_t *)e->loc, (const uint32_t *)dh,
+                       (const double *)dp, (uint32_t)n, bad);
+    uint32_t hb = 0;
+    HIP_TRY(hipMemcpyAsync(&hb, bad, 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (hb) return set_err(RDY_ERR_STATE, "a topology particle's record was not found");
+    e->export_valid = false;
+    e->version++;
+    return rdy_update_neighbor_list(e);  // the moved records into the cells of their new positions
+}
+
+int rdy_append_topology_particles(rdy_engine *e, uint32_t r, uint64_t n, const uint32_t *types, const double *pos,
+                                  uint32_t *handles) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    if (e->in_session) return set_err(RDY_ERR_STATE, "a run is open: finish it with rdy_run_end");
+    if (n == 0) return RDY_OK;
+    if (!types || !pos || !handles) return set_err(RDY_ERR_INVALID, "null types, positions or handles");
+    if ((rc = sync_topo_count(e, r))) return rc;
+    HostReplica &H = e->reps_h[r];
+    for (uint64_t i = 0; i < n; ++i) {
+        if (types[i] >= e->n_types || !e->topo_flavor[types[i]])
+            return set_err(RDY_ERR_INVALID, "appended particles must be of a topology species");
+        for (int d = 0; d < 3; ++d)
+            if (!std::isfinite(pos[3 * i + d])) return set_err(RDY_ERR_INVALID, "non-finite particle position");
+    }
+    if (H.n_topo + n > H.topo_range)
+        return set_err(RDY_ERR_CAPACITY, "appended particles beyond the replica's topology reserve: raise it with "
+                                         "rdy_reserve_topology_particles before the upload");
+    if ((rc = upload_tables(e, e->tables_dt > 0 ? e->tables_dt : 1.0))) return rc;
+    if ((rc = ensure_scratch(e, (size_t)n * (4 + 24)))) return rc;
+    double *dp = (double *)e->scratch;
+    uint32_t *dt = (uint32_t *)(dp + 3 * n);
+    HIP_TRY(hipMemcpyAsync(dp, pos, n * 24, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(dt, types, n * 4, hipMemcpyHostToDevice, e->stream));
+    const uint32_t h0 = H.n_topo;
+    hipLaunchKernelGGL(k_append_topo, dim3(blocks_for((uint32_t)n, 256)), dim3(256), 0, e->stream, e->grid(e->cur),
+                       e->reps, r, h0, (const uint32_t *)dt, (const double *)dp, (uint32_t)n, e->loc, e->cap, e->err);
+    H.n_topo += (uint32_t)n;
+    if ((rc = put_field(e, r, offsetof(ReplicaDev, n_topo), H.n_topo))) return rc;
+    for (uint64_t i = 0; i < n; ++i) handles[i] = h0 + (uint32_t)i;
+    e->wl_valid[e->cur] = false;
+    e->export_valid = false;
+    e->version++;
+    return check_errors(e);
+}
+
 int rdy_get_topology_reaction_counts(rdy_engine *e, uint32_t r, int totals, uint64_t *counts) {
     int rc = check_replica(e, r);
     if (rc) return rc;

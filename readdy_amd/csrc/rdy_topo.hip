@@ -294,4 +294,40 @@ __global__ void k_set_topo_types(Grid cur, const ReplicaDev *reps, uint32_t r, c
     cur.bins[s].tw = make_tw(t, (flavor[t] ? TW_TOPO : 0u) | (st.tw & TW_OUTSIDE), r);
 }
 
+// moves topology particles of replica r (by handle, found through loc) to new positions in place; the caller
+// re-bins afterwards.  A handle whose record is not live at loc counts in *bad.
+__global__ void k_set_topo_positions(Grid cur, const ReplicaDev *reps, uint32_t r, const uint32_t *loc,
+                                     const uint32_t *handles, const double *pos, uint32_t n, uint32_t *bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ReplicaDev &R = reps[r];
+    const uint32_t h = handles[i];
+    const uint32_t s = loc[R.topo_base + h];
+    const State st = cur.bins[s];
+    if (st.h != h || (st.tw & TW_DEAD) || tw_replica(st.tw) != r || !(st.tw & TW_TOPO)) {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    cur.bins[s].x = pos[3ull * i];
+    cur.bins[s].y = pos[3ull * i + 1];
+    cur.bins[s].z = pos[3ull * i + 2];
+}
+
+// new topology particles of replica r under handles h0, h0 + 1, ...: binned into cur, their slots in loc
+__global__ void k_append_topo(Grid cur, ReplicaDev *reps, uint32_t r, uint32_t h0, const uint32_t *types,
+                              const double *pos, uint32_t n, uint32_t *loc, uint32_t cap, uint32_t *err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ReplicaDev &R = reps[r];
+    const RepGeom geo = load_geom(R);
+    State st;
+    st.x = pos[3ull * i];
+    st.y = pos[3ull * i + 1];
+    st.z = pos[3ull * i + 2];
+    st.tw = make_tw(types[i], TW_TOPO, r);
+    st.h = h0 + i;
+    const uint32_t slot = bin_insert(cur, geo, cap, st, err, false, 0);
+    if (slot != 0xFFFFFFFFu) loc[R.topo_base + h0 + i] = slot;
+}
+
 }  // namespace rdy

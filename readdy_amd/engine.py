@@ -363,6 +363,23 @@ class Engine:
         t = np.ascontiguousarray(types, dtype=np.uint32)
         L.check(self._lib.rdy_set_particle_types(self._h, r, len(h), _ptr(h, _u32p), _ptr(t, _u32p)))
 
+    def set_particle_positions(self, r, handles, pos):
+        """moves topology particles (by handle); the engine re-bins its particles"""
+        h = np.ascontiguousarray(handles, dtype=np.uint32)
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        assert len(h) == len(p)
+        L.check(self._lib.rdy_set_particle_positions(self._h, r, len(h), _ptr(h, _u32p), _ptr(p, _dp)))
+
+    def append_topology_particles(self, r, types, pos):
+        """new topology particles under the next handles of the topology reserve; returns their handles"""
+        t = np.ascontiguousarray(types, dtype=np.uint32)
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        assert len(t) == len(p)
+        h = np.zeros(max(len(t), 1), dtype=np.uint32)
+        L.check(self._lib.rdy_append_topology_particles(self._h, r, len(t), _ptr(t, _u32p), _ptr(p, _dp),
+                                                        _ptr(h, _u32p)))
+        return h[:len(t)]
+
     def topology_reaction_counts(self, r=0, totals=False):
         n = getattr(self, "_n_sr", 0)
         c = np.zeros(max(n, 1), dtype=np.uint64)

@@ -83,6 +83,8 @@ public:
     void markTopologiesStale();
     // TopologyReactionActionFactory::createChangeParticleType on the device (topology particles only)
     void changeParticleType(ParticleId id, ParticleTypeId type) override;
+    void changeParticlePosition(ParticleId id, const Vec3 &pos) override;
+    ParticleId appendTopologyParticle(ParticleTypeId type, const Vec3 &pos) override;
 
 private:
     void configureSpatialReactions();
@@ -323,9 +325,9 @@ void MI355XStateModel::flush() {
     for (const auto &p : all) put(p);
     for (const auto &p : _freeParticles) put(p);
     _k->replica(types.size());
-    // topology-particle fusion: the joining particles take handles of a reserve after the topology particles
-    // (at most every free particle of this upload joins once, and some made by reactions later)
-    bool fusion = false;
+    // topology-particle fusion and the recipes that append particles: the new topology particles take handles
+    // of a reserve after the uploaded ones (every free particle of this upload joining once, and more)
+    bool fusion = _k->context().topologyRegistry().hasStructuralReactions();
     for (const auto &r : _k->context().topologyRegistry().spatialReactions()) fusion = fusion || r.mode == STRMode::TP_FUSION;
     if (fusion && !_uploaded)
         ok(rdy_reserve_topology_particles(_k->engine(), _k->replicaId(),
@@ -501,6 +503,33 @@ void MI355XStateModel::changeParticleType(ParticleId id, ParticleTypeId type) {
     syncTopologies();
     const Particle &old = _topoParticles[id];
     _topoParticles[id] = Particle(old.pos(), type, old.id());
+}
+
+// TopologyReactionActions ChangeParticlePosition: the device record moves and the particles are re-binned
+void MI355XStateModel::changeParticlePosition(ParticleId id, const Vec3 &pos) {
+    flush();
+    if (id >= _topoParticles.size())
+        throw std::invalid_argument("the MI355X kernel moves topology particles only");
+    const uint32_t h = (uint32_t)id;
+    const double p[3] = {pos.x, pos.y, pos.z};
+    ok(rdy_set_particle_positions(_k->engine(), _k->replicaId(), 1, &h, p), "change particle position");
+    const Particle &old = _topoParticles[id];
+    _topoParticles[id] = Particle(pos, old.type(), old.id());
+}
+
+// TopologyReactionActions AppendParticle: a new topology particle under the next reserve handle
+ParticleId MI355XStateModel::appendTopologyParticle(ParticleTypeId type, const Vec3 &pos) {
+    flush();
+    if (!_k->context().particleTypes().infoOf(type).topologyFlavor)
+        throw std::invalid_argument("an appended particle must be of a topology species");
+    const uint32_t t = (uint32_t)type;
+    const double p[3] = {pos.x, pos.y, pos.z};
+    uint32_t h = 0;
+    ok(rdy_append_topology_particles(_k->engine(), _k->replicaId(), 1, &t, p, &h), "append topology particle");
+    if (h != _topoParticles.size()) throw std::logic_error("appended particle handle out of order");
+    _topoParticles.emplace_back(pos, type, h);
+    _topoDirty = true;
+    return h;
 }
 
 // the spatial pass on the device, its species and topology types read back

@@ -453,6 +453,14 @@ void GraphTopology::addEdge(std::size_t v1, std::size_t v2) {
     _neighbors[v2].push_back(v1);
 }
 
+std::size_t GraphTopology::addVertex(std::size_t particle) {
+    if (std::find(_particles.begin(), _particles.end(), particle) != _particles.end())
+        throw std::invalid_argument("addVertex: the particle is in this topology already");
+    _particles.push_back(particle);
+    _neighbors.emplace_back();
+    return _particles.size() - 1;
+}
+
 void GraphTopology::appendParticle(std::size_t particle, std::size_t counterpart) {
     const auto it = std::find(_particles.begin(), _particles.end(), counterpart);
     if (it == _particles.end()) throw std::invalid_argument("appendParticle: the counterpart is not in this topology");
@@ -619,9 +627,11 @@ std::vector<GraphTopology> StructuralTopologyReaction::execute(GraphTopology &to
         case Recipe::Step::SeparateVertex: actions.push_back(taf->createRemoveEdge(&topology, st.v1, st.v2)); break;
         case Recipe::Step::ChangeTopologyType: actions.push_back(taf->createChangeTopologyType(&topology, st.name)); break;
         case Recipe::Step::ChangeParticlePosition:
+            actions.push_back(taf->createChangeParticlePosition(&topology, st.v1, st.pos));
+            break;
         case Recipe::Step::AppendNewParticle:
-            throw std::invalid_argument("structural reaction \"" + _name +
-                                        "\": moving or appending particles is not available on this kernel");
+            actions.push_back(taf->createAppendParticle(&topology, st.neighbors, types.idOf(st.name), st.pos));
+            break;
         }
     }
     for (auto &a : actions) a->perform();
@@ -724,6 +734,12 @@ std::vector<uint64_t> StateModel::rdfCounts(const std::vector<scalar> &, const s
 void StateModel::changeParticleType(ParticleId, ParticleTypeId) {
     throw std::logic_error("this kernel cannot change a particle's species");
 }
+void StateModel::changeParticlePosition(ParticleId, const Vec3 &) {
+    throw std::logic_error("this kernel cannot move a topology particle");
+}
+ParticleId StateModel::appendTopologyParticle(ParticleTypeId, const Vec3 &) {
+    throw std::logic_error("this kernel cannot add topology particles");
+}
 
 // ---- topology reaction actions (TopologyReactionActionFactory.h:55-101) ------------------------------
 namespace {
@@ -784,6 +800,60 @@ private:
     bool _add, _changed = false;
 };
 
+class ChangeParticlePosition : public TopologyReactionAction {
+public:
+    ChangeParticlePosition(GraphTopology *t, StateModel *state, std::size_t vertex, const Vec3 &pos)
+        : TopologyReactionAction(t), _state(state), _vertex(vertex), _to(pos) {
+        if (vertex >= t->nParticles()) throw std::invalid_argument("no such vertex in the topology");
+    }
+    void perform() override {
+        const ParticleId id = _topology->particleIndices()[_vertex];
+        bool found = false;
+        for (const auto &p : _state->getParticles())
+            if (p.id() == id) {
+                _from = p.pos();
+                found = true;
+                break;
+            }
+        if (!found) throw std::logic_error("the topology's particle is not in the state model");
+        _state->changeParticlePosition(id, _to);
+        _done = true;
+    }
+    void undo() override {
+        if (_done) _state->changeParticlePosition(_topology->particleIndices()[_vertex], _from);
+        _done = false;
+    }
+
+private:
+    StateModel *_state;
+    std::size_t _vertex;
+    Vec3 _to, _from{};
+    bool _done = false;
+};
+
+// the new particle stays in the state model on undo (a recipe's step cannot take a particle back)
+class AppendParticle : public TopologyReactionAction {
+public:
+    AppendParticle(GraphTopology *t, StateModel *state, std::vector<std::size_t> neighbors, ParticleTypeId type,
+                   const Vec3 &pos)
+        : TopologyReactionAction(t), _state(state), _neighbors(std::move(neighbors)), _type(type), _pos(pos) {
+        for (std::size_t v : _neighbors)
+            if (v >= t->nParticles()) throw std::invalid_argument("no such vertex in the topology");
+    }
+    void perform() override {
+        const ParticleId id = _state->appendTopologyParticle(_type, _pos);
+        const std::size_t v = _topology->addVertex((std::size_t)id);
+        for (std::size_t n : _neighbors) _topology->addEdge(n, v);
+    }
+    void undo() override { throw std::logic_error("an appended particle cannot be taken back"); }
+
+private:
+    StateModel *_state;
+    std::vector<std::size_t> _neighbors;
+    ParticleTypeId _type;
+    Vec3 _pos;
+};
+
 class ChangeTopologyType : public TopologyReactionAction {
 public:
     ChangeTopologyType(GraphTopology *t, std::string to) : TopologyReactionAction(t), _to(std::move(to)) {}
@@ -811,6 +881,16 @@ TopologyActionFactory::ActionPtr TopologyActionFactory::createAddEdge(GraphTopol
 TopologyActionFactory::ActionPtr TopologyActionFactory::createRemoveEdge(GraphTopology *topology, std::size_t v1,
                                                                          std::size_t v2) const {
     return std::make_unique<EdgeAction>(topology, v1, v2, false);
+}
+TopologyActionFactory::ActionPtr TopologyActionFactory::createChangeParticlePosition(GraphTopology *topology,
+                                                                                     std::size_t vertex,
+                                                                                     const Vec3 &pos) const {
+    return std::make_unique<ChangeParticlePosition>(topology, _state, vertex, pos);
+}
+TopologyActionFactory::ActionPtr TopologyActionFactory::createAppendParticle(GraphTopology *topology,
+                                                                             const std::vector<std::size_t> &neighbors,
+                                                                             ParticleTypeId type, const Vec3 &pos) const {
+    return std::make_unique<AppendParticle>(topology, _state, neighbors, type, pos);
 }
 TopologyActionFactory::ActionPtr TopologyActionFactory::createChangeTopologyType(GraphTopology *topology,
                                                                                  const std::string &typeTo) const {

@@ -276,6 +276,8 @@ public:
     // GraphTopology::appendParticle (GraphTopology.cpp): a new vertex for particle `particle`, bonded to the
     // vertex of particle `counterpart`
     void appendParticle(std::size_t particle, std::size_t counterpart);
+    // a new vertex for particle `particle`, without edges; returns its vertex index
+    std::size_t addVertex(std::size_t particle);
     bool removeEdge(std::size_t v1, std::size_t v2);  // false when there is no such edge
     bool hasEdge(std::size_t v1, std::size_t v2) const;
     std::size_t nParticles() const { return _particles.size(); }
@@ -497,6 +499,10 @@ public:
     // changes the species of the particle with this id (TopologyReactionActionFactory::createChangeParticleType
     // acts through it); a kernel that cannot throws std::logic_error
     virtual void changeParticleType(ParticleId id, ParticleTypeId type);
+    // moves the topology particle with this id (createChangeParticlePosition acts through it)
+    virtual void changeParticlePosition(ParticleId id, const Vec3 &pos);
+    // a new particle of a topology species; returns its id (createAppendParticle acts through it)
+    virtual ParticleId appendTopologyParticle(ParticleTypeId type, const Vec3 &pos);
 };
 
 class Kernel;
@@ -526,6 +532,11 @@ public:
     ActionPtr createAddEdge(GraphTopology *topology, std::size_t v1, std::size_t v2) const;
     ActionPtr createRemoveEdge(GraphTopology *topology, std::size_t v1, std::size_t v2) const;
     ActionPtr createChangeTopologyType(GraphTopology *topology, const std::string &typeTo) const;
+    // TopologyReactionActions ChangeParticlePosition / AppendParticle: the particle of `vertex` moves to pos; a new
+    // particle of `type` at pos joins the topology, bonded to the vertices `neighbors`
+    ActionPtr createChangeParticlePosition(GraphTopology *topology, std::size_t vertex, const Vec3 &pos) const;
+    ActionPtr createAppendParticle(GraphTopology *topology, const std::vector<std::size_t> &neighbors,
+                                   ParticleTypeId type, const Vec3 &pos) const;
 
 private:
     StateModel *_state;

@@ -990,6 +990,78 @@ static void test_topology_fusion() {
     std::printf("topology fusion: 5 chains grew by two monomers each\n");
 }
 
+// Recipe steps that move and append topology particles (Recipe::changeParticlePosition, appendNewParticle): three
+// chains of three beads 1 apart with bonds (k 10, length 1); a structural reaction (rate 1e10 while a chain has
+// fewer than four particles) moves bead 0 by 0.3 away from bead 1 and appends a bead 1.2 beyond bead 2, bonded to
+// it.  One evaluation: four particles per chain, the new ones under topology handles 9..11 (after the nine
+// uploaded ones), the positions as the recipe set them, and the bond energy 10 * 0.3^2 + 10 * 0.2^2 per chain;
+// the next evaluation does nothing (rate 0).
+static void test_structural_move_and_append() {
+    using model::Recipe;
+    using model::StructuralTopologyReaction;
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {30., 30., 30.};
+    c.particleTypes().addTopologyType("bead", 0.1);
+    auto &reg = c.topologyRegistry();
+    reg.addType("Chain");
+    reg.configureBondPotential("bead", "bead", {10., 1.});
+    const auto bead = c.particleTypes().idOf("bead");
+    std::vector<double> ys;
+    StructuralTopologyReaction grow(
+        "grow",
+        [&](model::GraphTopology &t) {
+            const auto ps = k->stateModel().getParticles();
+            Vec3 p0{}, p2{};
+            for (const auto &p : ps) {
+                if (p.id() == t.particleIndices()[0]) p0 = p.pos();
+                if (p.id() == t.particleIndices()[2]) p2 = p.pos();
+            }
+            return Recipe(t).changeParticlePosition(0, Vec3(p0.x - 0.3, p0.y, p0.z))
+                .appendNewParticle({2}, "bead", Vec3(p2.x + 1.2, p2.y, p2.z));
+        },
+        [](const model::GraphTopology &t) { return t.nParticles() < 4 ? 1e10 : 0.; });
+    grow.expect_connected_after_reaction();
+    reg.addStructuralReaction("Chain", grow);
+    auto &sm = k->stateModel();
+    for (int t = 0; t < 3; ++t) {
+        const double y = -6. + 6. * t;
+        ys.push_back(y);
+        std::vector<model::Particle> ps;
+        for (int i = 0; i < 3; ++i) ps.emplace_back((double)i, y, 0., bead);
+        auto *top = sm.addTopology("Chain", ps);
+        top->addEdge(0, 1);
+        top->addEdge(1, 2);
+    }
+    k->actions().calculateForces()->perform();
+    CHECK(std::fabs(sm.energy()) < 1e-9, "chains at rest: energy %.3g", sm.energy());
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    const auto tops = sm.getTopologies();
+    const auto ps = sm.getParticles();
+    CHECK(ps.size() == 12u && tops.size() == 3u, "%zu particles, %zu topologies", ps.size(), tops.size());
+    for (std::size_t t = 0; t < tops.size(); ++t) {
+        const auto &pi = tops[t]->particleIndices();
+        CHECK(pi.size() == 4u && pi[3] >= 9u && pi[3] < 12u && tops[t]->hasEdge(2, 3) && tops[t]->isConnected(),
+              "chain %zu: %zu particles, new handle %zu", t, pi.size(), pi.size() > 3 ? pi[3] : 0);
+        for (const auto &p : ps) {
+            if (p.id() == pi[0])
+                CHECK(std::fabs(p.pos().x + 0.3) < 1e-12 && std::fabs(p.pos().y - ys[t]) < 1e-12, "bead 0 moved to %.3f",
+                      p.pos().x);
+            if (pi.size() > 3 && p.id() == pi[3])
+                CHECK(std::fabs(p.pos().x - 3.2) < 1e-12 && p.type() == bead, "appended bead at %.3f", p.pos().x);
+        }
+    }
+    k->actions().calculateForces()->perform();
+    const double e = sm.energy(), want = 3 * (10. * 0.09 + 10. * 0.04);
+    CHECK(std::fabs(e - want) < 1e-6 * want, "bond energy %.9g, expected %.9g", e, want);
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    CHECK(sm.getParticles().size() == 12u, "rate 0 at four particles: %zu particles", sm.getParticles().size());
+    api::SimulationLoop loop(k.get(), 0.001);
+    loop.run(5);
+    CHECK(sm.getParticles().size() == 12u && sm.getTopologies().size() == 3u, "after 5 steps");
+    std::printf("structural move and append: three chains grew to four beads\n");
+}
+
 // Kernel::observe(): observables connected to the kernel and evaluated by the SimulationLoop on the
 // device-fused path in chunks that end where one is due (strides 0, 2, 5 and 10 over 10 steps), checked
 // against the oracle on the state they saw: energy, virial, particles per type, the radial distribution
@@ -1126,6 +1198,7 @@ int main(int argc, char **argv) {
             test_break_bonds();
             test_topology_reactions();
             test_topology_fusion();
+            test_structural_move_and_append();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

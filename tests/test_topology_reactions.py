@@ -199,3 +199,40 @@ def test_argument_errors():
     e.set_spatial_reactions(_sr())
     with pytest.raises(L.EngineError):
         e.set_topologies(r, [0], [3])  # topology 3 does not exist
+
+
+def test_move_and_append_topology_particles():
+    """Recipe steps on the device: a moved topology particle lands in its new cell (its repulsion with a free
+    particle it now overlaps shows in the forces), appended particles take the reserve's handles after the
+    uploaded topology particles, and the reserve bounds them"""
+    print("move and append", flush=True)
+    pots = [(L.POT_HARMONIC_REPULSION, 0, 1, [10.0, 1.0])]
+    e = Engine(4, [1.0] * 4, potentials=pots, topology_flavor=FLAVOR, nl_cutoff=1.2)
+    r = e.add_replica([20.0] * 3, (1, 1, 1), seed=2, max_particles=256)
+    e.reserve_topology_particles(r, 2)
+    e.set_particles(r, [[0, 0, 0], [1, 0, 0], [5, 5, 5]], [0, 0, 1])  # handles 0, 1 (beads), 4 (free X)
+    e.set_topologies(r, [0], [0, 0])
+    e.calculate_forces()
+    f, h = e.get_forces(r)
+    assert h.tolist() == [0, 1, 4] and abs(f).max() == 0
+    e.set_particle_positions(r, [1], [[5.5, 5, 5]])  # 0.5 from X: overlapping
+    e.calculate_forces()
+    f, h = e.get_forces(r)
+    # harmonic repulsion k (1 - 0.5) = 5 pushes the bead (x 5.5) away from X (x 5.0)
+    assert f[1, 0] == -f[2, 0] and abs(f[1, 0] - 5.0) < 1e-6 and f[0].tolist() == [0, 0, 0]
+    assert np.allclose(e.get_particles(r)[0][1], [5.5, 5, 5], rtol=0, atol=0)
+    new = e.append_topology_particles(r, [3, 0], [[-1, 0, 0], [9, 9, 9]])
+    assert new.tolist() == [2, 3]
+    pos, types, handles = e.get_particles(r)
+    assert handles.tolist() == [0, 1, 2, 3, 4] and types.tolist() == [0, 0, 3, 0, 1]
+    assert np.array_equal(pos[2], [-1, 0, 0])
+    e.set_topologies(r, [0], [0, 0, 0, 0])  # four topology particles now
+    with pytest.raises(L.EngineError, match="reserve"):
+        e.append_topology_particles(r, [0], [[1, 1, 1]])
+    with pytest.raises(L.EngineError, match="topology species"):
+        e.append_topology_particles(r, [1], [[1, 1, 1]])
+    with pytest.raises(L.EngineError, match="topology particles"):
+        e.set_particle_positions(r, [4], [[0, 0, 0]])
+    e.update_neighbor_list()
+    e.calculate_forces()
+    assert e.get_forces(r)[0].shape == (5, 3)
