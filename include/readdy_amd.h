@@ -244,15 +244,22 @@ int rdy_break_bonds(rdy_engine *e, double dt, uint32_t n_pairs, const rdy_break_
  *      the topology top_type_to1.  Fusion (handleTopologyParticleReaction, :369-404, appendParticle): the free
  *      particle joins the topology, bonded to the topology particle; it takes the next handle of the
  *      replica's topology reserve (rdy_reserve_topology_particles), and rdy_get_topology_fusions lists the
- *      joins.  Topology types are the caller's integers.  The bonded terms stay as they are: the caller
- *      sets them again when a reaction changes the topology's bonded configuration (every fusion does). */
+ *      joins.  Topology-topology enzymatic (STRMode::TT_ENZYMATIC, handleTopologyTopologyReaction, :406-475):
+ *      two topology particles of different topologies whose (species, topology type) pairs match the reaction
+ *      in either order react once per pair; each particle and topology takes the product of its side, and
+ *      both topologies count as reacted (eventsDependent).  Topology types are the caller's integers.  The
+ *      bonded terms stay as they are: the caller sets them again when a reaction changes the topology's
+ *      bonded configuration (every fusion does). */
 typedef struct {
     uint32_t type1, top_type1;   /* the topology particle's species and its topology's type */
     uint32_t type2;              /* the free particle's species                               */
     uint32_t type_to1, top_type_to1, type_to2;
     double rate, radius;
     uint32_t fusion;             /* 0: TP_ENZYMATIC, 1: TP_FUSION (type_to2 a topology species) */
-    uint32_t reserved;           /* 0 */
+    uint32_t topology_topology;  /* 1: TT_ENZYMATIC "T1(p1) + T2(p2) -> T3(p3) + T4(p4)": the second particle is
+                                    a topology particle of another topology, of type top_type2, which becomes
+                                    top_type_to2 (fusion 0) */
+    uint32_t top_type2, top_type_to2;
 } rdy_spatial_reaction;
 /* the engine's spatial topology reactions (TopologyRegistry::addSpatialReaction, TopologyRegistry.cpp:195-223);
  * radius <= the engine's neighbour-list cutoff, at most 64 reactions */

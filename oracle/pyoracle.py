@@ -123,6 +123,7 @@ def lib():
         L.or_add_spatial_reaction.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
         L.or_add_spatial_fusion.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
         L.or_set_topology_reserve.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+        L.or_add_spatial_tt.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 8 + [ctypes.c_double] * 2
         L.or_topology_fusions.restype = _u64p
         L.or_topology_fusions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_set_topologies.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _ip, ctypes.c_size_t, _ip]
@@ -452,6 +453,12 @@ class System:
         T2(type_to1--type_to2); topology types as integers"""
         f = lib().or_add_spatial_fusion if fusion else lib().or_add_spatial_reaction
         _check(f(self._h, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius))
+
+    def add_spatial_tt(self, type1, top_type1, type2, top_type2, type_to1, top_type_to1, type_to2, top_type_to2, rate,
+                       radius):
+        """T1(type1) + T2(type2) -> T3(type_to1) + T4(type_to2), both sides in topologies"""
+        _check(lib().or_add_spatial_tt(self._h, type1, top_type1, type2, top_type2, type_to1, top_type_to1, type_to2,
+                                       top_type_to2, rate, radius))
 
     def set_topology_reserve(self, next_id, end_id):
         """ids next_id .. end_id - 1 for particles that join topologies by fusion"""

@@ -469,6 +469,7 @@ typedef struct {
     int type1, top_type1, type2, type_to1, top_type_to1, type_to2;
     double rate, radius, radius2;
     int fusion;
+    int tt, top_type2, top_type_to2; /* topology-topology enzymatic: the second side in another topology */
 } str_t;
 
 typedef struct {
@@ -2200,6 +2201,8 @@ int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, i
     r->radius = radius;
     r->radius2 = radius * radius;
     r->fusion = 0;
+    r->tt = 0;
+    r->top_type2 = r->top_type_to2 = -1;
     return 0;
 }
 
@@ -2207,6 +2210,16 @@ int or_add_spatial_fusion(or_system *s, int type1, int top_type1, int type2, int
                           int type_to2, double rate, double radius) {
     if (or_add_spatial_reaction(s, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius)) return -1;
     s->str[s->n_str - 1].fusion = 1;
+    return 0;
+}
+
+int or_add_spatial_tt(or_system *s, int type1, int top_type1, int type2, int top_type2, int type_to1, int top_type_to1,
+                      int type_to2, int top_type_to2, double rate, double radius) {
+    if (or_add_spatial_reaction(s, type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius)) return -1;
+    str_t *r = &s->str[s->n_str - 1];
+    r->tt = 1;
+    r->top_type2 = top_type2;
+    r->top_type_to2 = top_type_to2;
     return 0;
 }
 
@@ -2252,8 +2265,9 @@ static int topology_of_entry(const or_system *s, const entry_t *e) {
 }
 
 typedef struct {
-    size_t p, q; /* slots: topology particle, free particle */
+    size_t p, q; /* slots: topology particle, free particle (topology-topology: the pair, lower id first) */
     int top, r;
+    int top2, flip; /* topology-topology: the partner's topology; the reaction matched with its sides swapped */
     double gkey;
     uint64_t ip, iq;
 } tevent_t;
@@ -2269,12 +2283,24 @@ static void str_visit(or_system *s, size_t p, size_t q, void *ud) {
     tev_ud *u = (tev_ud *)ud;
     const entry_t *ep = &s->e[p], *eq = &s->e[q];
     if (eq->deactivated) return;
-    const int top = topology_of_entry(s, ep);
-    if (top < 0 || topology_of_entry(s, eq) >= 0) return;
+    const int top = topology_of_entry(s, ep), top2 = topology_of_entry(s, eq);
+    if (top < 0) return;
+    /* a partner in a topology: each pair once, from the lower id, in two different topologies
+     * (CPUEvaluateTopologyReactions.cpp:244-268) */
+    if (top2 >= 0 && (!(ep->id < eq->id) || top2 == top)) return;
     double d2 = -1.0;
     for (int r = 0; r < s->n_str; ++r) {
         const str_t *x = &s->str[r];
-        if (x->type1 != ep->type || x->top_type1 != s->top_types[top] || x->type2 != eq->type) continue;
+        int flip = 0;
+        if (top2 < 0) {
+            if (x->tt || x->type1 != ep->type || x->top_type1 != s->top_types[top] || x->type2 != eq->type) continue;
+        } else { /* the registry keys a topology-topology reaction under both orders of its sides */
+            if (!x->tt) continue;
+            const int T = s->top_types[top], T2 = s->top_types[top2];
+            if (x->type1 == ep->type && x->top_type1 == T && x->type2 == eq->type && x->top_type2 == T2) flip = 0;
+            else if (x->type2 == ep->type && x->top_type2 == T && x->type1 == eq->type && x->top_type1 == T2) flip = 1;
+            else continue;
+        }
         if (d2 < 0) d2 = or_dist2(ep->pos, eq->pos, s->box, s->pbc);
         if (!(d2 < x->radius2)) continue;
         uint32_t w[4];
@@ -2289,6 +2315,8 @@ static void str_visit(or_system *s, size_t p, size_t q, void *ud) {
         e->q = q;
         e->top = top;
         e->r = r;
+        e->top2 = top2;
+        e->flip = flip;
         e->gkey = -log(or_u53(w[2], w[3]) + 0x1p-54) / x->rate;
         e->ip = ep->id;
         e->iq = eq->id;
@@ -2325,8 +2353,19 @@ int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_pe
     uint64_t done = 0;
     for (size_t i = 0; i < u.n; ++i) {
         const tevent_t *e = &u.ev[i];
-        if (top_used[e->top] || p_used[e->q]) continue;
         const str_t *x = &s->str[e->r];
+        if (x->tt) { /* handleTopologyTopologyReaction (:406-475), enzymatic: both topologies used */
+            if (top_used[e->top] || top_used[e->top2]) continue;
+            top_used[e->top] = top_used[e->top2] = 1;
+            s->e[e->p].type = e->flip ? x->type_to2 : x->type_to1;
+            s->e[e->q].type = e->flip ? x->type_to1 : x->type_to2;
+            s->top_types[e->top] = e->flip ? x->top_type_to2 : x->top_type_to1;
+            s->top_types[e->top2] = e->flip ? x->top_type_to1 : x->top_type_to2;
+            s->str_counts[e->r]++;
+            ++done;
+            continue;
+        }
+        if (top_used[e->top] || p_used[e->q]) continue;
         if (x->fusion && s->fuse_next >= s->fuse_end) {
             free(top_used);
             free(p_used);

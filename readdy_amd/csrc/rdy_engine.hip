@@ -476,6 +476,9 @@ int upload_tables(rdy_engine *e, double dt) {
             d.p_accept = 1 - std::exp(-x.rate * dt);
             d.radius2 = x.radius * x.radius;
             d.fusion = x.fusion;
+            d.tt = x.topology_topology;
+            d.top_type2 = x.top_type2;
+            d.top_type_to2 = x.top_type_to2;
             sr.push_back(d);
         }
         if ((rc = up(&e->d_sr, sr))) return rc;
@@ -2290,9 +2293,21 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         const rdy_spatial_reaction &x = sr[i];
         if (x.type1 >= e->n_types || x.type2 >= e->n_types || x.type_to1 >= e->n_types || x.type_to2 >= e->n_types)
             return set_err(RDY_ERR_INVALID, "spatial topology reaction species out of range");
-        if (x.fusion > 1 || x.reserved) return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion is 0 or 1");
+        if (x.fusion > 1 || x.topology_topology > 1 || (x.fusion && x.topology_topology))
+            return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion and topology_topology are 0 or 1, not both");
         if (!e->topo_flavor[x.type1] || !e->topo_flavor[x.type_to1])
             return set_err(RDY_ERR_INVALID, "the topology side of a topology-particle reaction must stay a topology species");
+        if (x.topology_topology) {
+            if (!e->topo_flavor[x.type2] || !e->topo_flavor[x.type_to2])
+                return set_err(RDY_ERR_INVALID, "both sides of a topology-topology reaction must stay topology species");
+            if (!(x.rate > 0))
+                return set_err(RDY_ERR_INVALID, "The rate of an structural topology reaction should always be positive");
+            if (!(x.radius > 0) || x.radius > e->nl_cutoff)
+                return set_err(RDY_ERR_INVALID, "a spatial topology reaction radius must be positive and at most the "
+                                                "engine's neighbour-list cutoff (Context::calculateMaxCutoff includes "
+                                                "these radii)");
+            continue;
+        }
         if (e->topo_flavor[x.type2])
             return set_err(RDY_ERR_INVALID, "the particle side of a topology-particle reaction must be a normal species");
         if (!x.fusion && e->topo_flavor[x.type_to2])

@@ -30,17 +30,22 @@ namespace rdy {
 struct SpatialDev {
     uint32_t type1, top_type1, type2, type_to1, top_type_to1, type_to2;
     double rate, p_accept, radius2;
-    uint32_t fusion, pad;
+    uint32_t fusion, tt;             // tt: topology-topology enzymatic (the partner in another topology)
+    uint32_t top_type2, top_type_to2;
 };
 
 struct TopoEvent {
     unsigned long long key;
-    uint32_t s_top, s_free;  // bin slots of the topology particle and of the free particle
+    uint32_t s_top, s_free;  // bin slots of the topology particle and of the free particle (topology-topology:
+                             // of the particle that found the pair and of its partner)
     uint32_t h_top, h_free;  // their handles
     uint32_t top;            // topology index within the replica
-    uint32_t sr;             // reaction index
+    uint32_t sr;             // reaction index; bit 31: topology-topology pair matched in reverse order
+    uint32_t top2;           // topology-topology: the partner's topology (else UINT32_MAX)
+    uint32_t pad;
 };
-static_assert(sizeof(TopoEvent) == 32, "TopoEvent layout");
+static_assert(sizeof(TopoEvent) == 40, "TopoEvent layout");
+constexpr uint32_t SR_FLIP = 0x80000000u;
 
 struct TopoTables {
     const SpatialDev *sr;
@@ -80,13 +85,25 @@ __global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev 
     if (me.tw & (TW_DEAD | TW_OUTSIDE)) return;
     const uint32_t t = tw_type(me.tw), T = TT.ttype[R.tt_base + k];
     bool any = false;
-    for (uint32_t j = 0; j < TT.n_sr; ++j) any = any || (TT.sr[j].type1 == t && TT.sr[j].top_type1 == T);
+    for (uint32_t j = 0; j < TT.n_sr; ++j) {
+        const SpatialDev &X = TT.sr[j];
+        any = any || (X.type1 == t && X.top_type1 == T) || (X.tt && X.type2 == t && X.top_type2 == T);
+    }
     if (!any) return;
     const RepGeom geo = load_geom(R);
     for_each_stencil_slot(geo, cur, cap, radius, cell_of_slot(cur, slot, cap) - geo.cell_base, [&](uint32_t s2) {
         if (s2 == slot) return;
         const State o = cur.bins[s2];
-        if (o.tw & (TW_DEAD | TW_OUTSIDE | TW_TOPO)) return;
+        if (o.tw & (TW_DEAD | TW_OUTSIDE)) return;
+        // a partner in a topology: the topology-topology kind, each pair once (from the lower handle), the
+        // two particles in different topologies (CPUEvaluateTopologyReactions.cpp:244-268)
+        uint32_t k2 = 0xFFFFFFFFu, T2 = 0xFFFFFFFFu;
+        if (o.tw & TW_TOPO) {
+            if (!(me.h < o.h) || o.h >= R.n_topo) return;
+            k2 = TT.topo_of[R.topo_base + o.h];
+            if (k2 >= R.n_tt || k2 == k) return;
+            T2 = TT.ttype[R.tt_base + k2];
+        }
         const uint32_t t2 = tw_type(o.tw);
         // distSquared(entry.pos, neighbor.pos) (CPUEvaluateTopologyReactions.cpp:278)
         const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
@@ -95,7 +112,16 @@ __global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev 
         const double d2 = dist2(dx, dy, dz);
         for (uint32_t j = 0; j < TT.n_sr; ++j) {
             const SpatialDev X = TT.sr[j];
-            if (X.type1 != t || X.top_type1 != T || X.type2 != t2 || !(d2 < X.radius2)) continue;
+            uint32_t flip = 0;
+            if (k2 == 0xFFFFFFFFu) {
+                if (X.tt || X.type1 != t || X.top_type1 != T || X.type2 != t2) continue;
+            } else {  // the registry holds a topology-topology reaction under both orders of its sides
+                if (!X.tt) continue;
+                if (X.type1 == t && X.top_type1 == T && X.type2 == t2 && X.top_type2 == T2) flip = 0;
+                else if (X.type2 == t && X.top_type2 == T && X.type1 == t2 && X.top_type1 == T2) flip = SR_FLIP;
+                else continue;
+            }
+            if (!(d2 < X.radius2)) continue;
             uint32_t x4[4];
             philox(me.h, R.step, 6u | (j << 8), o.h, R.seed, x4);
             if (!(u53(x4[0], x4[1]) < X.p_accept)) continue;
@@ -111,7 +137,9 @@ __global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev 
             ev.h_top = me.h;
             ev.h_free = o.h;
             ev.top = k;
-            ev.sr = j;
+            ev.sr = j | flip;
+            ev.top2 = k2;
+            ev.pad = 0;
             TT.ev[(size_t)r * TT.ev_cap + p] = ev;
         }
     });
@@ -193,30 +221,22 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         for (uint32_t q = 0; q < ne; ++q) {
             const TopoEvent a = ev[order[q]];
             bool top_free = true, partner_free = true;
-            {  // probe without inserting: an event is performed only when both are unused
-                uint32_t h = (a.top * 2654435761u) & (TOPO_HASH - 1);
+            auto probe = [&](const uint32_t *set, uint32_t v) -> bool {  // v in the set?
+                uint32_t h = (v * 2654435761u) & (TOPO_HASH - 1);
                 for (;;) {
-                    const uint32_t c = used_top[h];
-                    if (c == a.top) {
-                        top_free = false;
-                        break;
-                    }
-                    if (c == 0xFFFFFFFFu) break;
+                    const uint32_t c = set[h];
+                    if (c == v) return true;
+                    if (c == 0xFFFFFFFFu) return false;
                     h = (h + 1) & (TOPO_HASH - 1);
                 }
-                h = (a.s_free * 2654435761u) & (TOPO_HASH - 1);
-                for (;;) {
-                    const uint32_t c = used_free[h];
-                    if (c == a.s_free) {
-                        partner_free = false;
-                        break;
-                    }
-                    if (c == 0xFFFFFFFFu) break;
-                    h = (h + 1) & (TOPO_HASH - 1);
-                }
-            }
+            };
+            // an event is performed only when its topologies (and a topology-particle event's free particle)
+            // are unused (eventsDependent, :455-464)
+            const bool tt = a.top2 != 0xFFFFFFFFu;
+            top_free = !probe(used_top, a.top) && !(tt && probe(used_top, a.top2));
+            partner_free = tt || !probe(used_free, a.s_free);
             if (!top_free || !partner_free) continue;
-            if (TT.sr[a.sr].fusion) {  // the joining particle takes the next reserve handle, in key order
+            if (TT.sr[a.sr & ~SR_FLIP].fusion) {  // the joining particle takes the next reserve handle, in key order
                 if (nf >= reserve || !TT.fus) {
                     atomicAdd(&err[ERR_TOPO_RESERVE], 1u);
                     continue;
@@ -224,7 +244,8 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
                 fidx[order[q]] = nf++;
             }
             hash_insert(used_top, a.top);
-            hash_insert(used_free, a.s_free);
+            if (tt) hash_insert(used_top, a.top2);
+            else hash_insert(used_free, a.s_free);
             won[order[q]] = 1;
         }
         s_nf = nf;
@@ -239,30 +260,40 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
     for (uint32_t k = tid; k < ne; k += blockDim.x) {
         if (!won[k]) continue;
         const TopoEvent a = ev[k];
-        const SpatialDev X = TT.sr[a.sr];
+        const uint32_t sr = a.sr & ~SR_FLIP;
+        const SpatialDev X = TT.sr[sr];
         const uint32_t tw1 = cur.bins[a.s_top].tw, tw2 = cur.bins[a.s_free].tw;
         // a record the step's reactions changed already carries its change for the fix-up (TW_NEW), which
         // redoes its move on the final state; every other retyped record is listed once here
         const uint32_t n1 = fix && !(tw1 & TW_NEW) ? TW_NEW : 0u, n2 = fix && !(tw2 & TW_NEW) ? TW_NEW : 0u;
-        cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
-        if (X.fusion) {  // appendParticle: the free particle joins topology a.top under a reserve handle
-            const uint32_t hn = n_topo0 + fidx[k], ti = R.topo_base + hn;
-            cur.bins[a.s_free].tw = make_tw(X.type_to2, TW_TOPO | (tw2 & (TW_OUTSIDE | TW_NEW)), r) | n2;
-            cur.bins[a.s_free].h = hn;
-            TT.loc[ti] = a.s_free;
-            TT.topo_of[ti] = a.top;
-            uint32_t *row = TT.fus + 4ull * ((size_t)r * TOPO_MAX_EVENTS + fidx[k]);
-            row[0] = a.top;
-            row[1] = a.h_top;
-            row[2] = a.h_free;
-            row[3] = hn;
+        if (X.tt) {  // handleTopologyTopologyReaction (:406-475), enzymatic: each side takes its products
+            const bool flip = (a.sr & SR_FLIP) != 0u;
+            cur.bins[a.s_top].tw = make_tw(flip ? X.type_to2 : X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
+            cur.bins[a.s_free].tw = make_tw(flip ? X.type_to1 : X.type_to2, tw2 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n2;
+            TT.ttype[R.tt_base + a.top] = flip ? X.top_type_to2 : X.top_type_to1;
+            TT.ttype[R.tt_base + a.top2] = flip ? X.top_type_to1 : X.top_type_to2;
+            if (TT.serial) TT.treact[R.tt_base + a.top2] = TT.serial;
         } else {
-            cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
+            cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
+            if (X.fusion) {  // appendParticle: the free particle joins topology a.top under a reserve handle
+                const uint32_t hn = n_topo0 + fidx[k], ti = R.topo_base + hn;
+                cur.bins[a.s_free].tw = make_tw(X.type_to2, TW_TOPO | (tw2 & (TW_OUTSIDE | TW_NEW)), r) | n2;
+                cur.bins[a.s_free].h = hn;
+                TT.loc[ti] = a.s_free;
+                TT.topo_of[ti] = a.top;
+                uint32_t *row = TT.fus + 4ull * ((size_t)r * TOPO_MAX_EVENTS + fidx[k]);
+                row[0] = a.top;
+                row[1] = a.h_top;
+                row[2] = a.h_free;
+                row[3] = hn;
+            } else {
+                cur.bins[a.s_free].tw = make_tw(X.type_to2, tw2 & (TW_OUTSIDE | TW_NEW), r) | n2;
+            }
+            TT.ttype[R.tt_base + a.top] = X.top_type_to1;
         }
-        TT.ttype[R.tt_base + a.top] = X.top_type_to1;
         if (TT.serial) TT.treact[R.tt_base + a.top] = TT.serial;
-        atomicAdd(&TT.cnt_step[(size_t)r * TT.n_sr + a.sr], 1ull);
-        atomicAdd(&TT.cnt_total[(size_t)r * TT.n_sr + a.sr], 1ull);
+        atomicAdd(&TT.cnt_step[(size_t)r * TT.n_sr + sr], 1ull);
+        atomicAdd(&TT.cnt_total[(size_t)r * TT.n_sr + sr], 1ull);
         if (n1) edit(a.s_top, tw_type(tw1), a.h_top);
         if (n2) edit(a.s_free, tw_type(tw2), a.h_free);  // (its moved copy carries the handle it had)
     }

@@ -310,12 +310,16 @@ class Engine:
     def set_spatial_reactions(self, reactions):
         """reactions: dicts with type1, top_type1, type2, type_to1, top_type_to1, type_to2, rate, radius and
         optionally fusion ("T1(type1) + (type2) -> T2(type_to1) + (type_to2)", with fusion=1
-        "T1(type1) + (type2) -> T2(type_to1--type_to2)")"""
+        "T1(type1) + (type2) -> T2(type_to1--type_to2)"), or topology_topology=1 with top_type2 and top_type_to2
+        ("T1(type1) + T3(type2) -> T2(type_to1) + T4(type_to2)")"""
         arr = (L.SpatialReaction * max(len(reactions), 1))()
         for i, x in enumerate(reactions):
             for f in ("type1", "top_type1", "type2", "type_to1", "top_type_to1", "type_to2", "rate", "radius"):
                 setattr(arr[i], f, x[f])
             arr[i].fusion = int(x.get("fusion", 0))
+            arr[i].topology_topology = int(x.get("topology_topology", 0))
+            arr[i].top_type2 = int(x.get("top_type2", 0))
+            arr[i].top_type_to2 = int(x.get("top_type_to2", 0))
         self._sr, self._n_sr = arr, len(reactions)
         L.check(self._lib.rdy_set_spatial_reactions(self._h, len(reactions), ctypes.cast(arr, ctypes.c_void_p)))
 

@@ -350,10 +350,11 @@ void MI355XStateModel::configureSpatialReactions() {
     if (_spatialSet) return;
     std::vector<rdy_spatial_reaction> v;
     for (const auto &r : _k->context().topologyRegistry().spatialReactions()) {
-        if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION)
+        if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION && r.mode != STRMode::TT_ENZYMATIC)
             throw std::invalid_argument("spatial topology reaction \"" + r.name +
                                         "\": the MI355X kernel performs the topology-particle kinds "
-                                        "(T1(p1) + (p2) -> T2(p3) + (p4) and T1(p1) + (p2) -> T2(p3--p4)) only");
+                                        "(T1(p1) + (p2) -> T2(p3) + (p4) and T1(p1) + (p2) -> T2(p3--p4)) and the "
+                                        "topology-topology enzymatic kind (T1(p1) + T2(p2) -> T3(p3) + T4(p4)) only");
         rdy_spatial_reaction q{};
         q.type1 = r.type1;
         q.top_type1 = (uint32_t)r.topType1;
@@ -364,6 +365,11 @@ void MI355XStateModel::configureSpatialReactions() {
         q.rate = r.rate;
         q.radius = r.radius;
         q.fusion = r.mode == STRMode::TP_FUSION ? 1u : 0u;
+        if (r.mode == STRMode::TT_ENZYMATIC) {
+            q.topology_topology = 1u;
+            q.top_type2 = (uint32_t)r.topType2;
+            q.top_type_to2 = (uint32_t)r.topTypeTo2;
+        }
         v.push_back(q);
     }
     if (!v.empty()) ok(rdy_set_spatial_reactions(_k->engine(), (uint32_t)v.size(), v.data()), "spatial reactions");
@@ -414,6 +420,7 @@ bool MI355XStateModel::reactionsChangeBondedTerms() const {
     for (const auto &r : reg.spatialReactions()) {
         if (r.mode == STRMode::TP_FUSION) return true;  // every fusion adds an edge
         if (r.typeTo1 != r.type1 && (named(r.type1) || named(r.typeTo1))) return true;
+        if (r.mode == STRMode::TT_ENZYMATIC && r.typeTo2 != r.type2 && (named(r.type2) || named(r.typeTo2))) return true;
     }
     return false;
 }
@@ -542,7 +549,8 @@ void MI355XStateModel::spatialReactions(scalar dt) {
     for (std::size_t i = 0; i < sr.size(); ++i)
         if (c[i]) {
             any = true;
-            retyped = retyped || sr[i].typeTo1 != sr[i].type1;
+            retyped = retyped || sr[i].typeTo1 != sr[i].type1 ||
+                      (sr[i].mode == STRMode::TT_ENZYMATIC && sr[i].typeTo2 != sr[i].type2);
         }
     if (!any) return;
     // fusions: the joining particle is the topology particle of the next reserve handle

@@ -990,6 +990,49 @@ static void test_topology_fusion() {
     std::printf("topology fusion: 5 chains grew by two monomers each\n");
 }
 
+// Topology-topology enzymatic through the plugin ("swap: Left(end) + Right(end) -> Left2(done) + Right2(done)",
+// rate 1e10): three pairs of two-bead chains whose ends are 0.6 apart react, a fourth pair 3 apart does not;
+// the reacted topologies take the new types, the ends the new species.
+static void test_topology_topology_enzymatic() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {30., 30., 30.};
+    c.particleTypes().addTopologyType("bead", 0.1);
+    c.particleTypes().addTopologyType("end", 0.1);
+    c.particleTypes().addTopologyType("done", 0.1);
+    auto &reg = c.topologyRegistry();
+    for (const char *t : {"Left", "Right", "Left2", "Right2"}) reg.addType(t);
+    reg.configureBondPotential("bead", "end", {10., 1.});
+    reg.configureBondPotential("bead", "done", {10., 1.});
+    reg.addSpatialReaction("swap: Left(end) + Right(end) -> Left2(done) + Right2(done)", 1e10, 1.0);
+    const auto bead = c.particleTypes().idOf("bead"), end = c.particleTypes().idOf("end"),
+               done = c.particleTypes().idOf("done");
+    auto &sm = k->stateModel();
+    for (int p = 0; p < 4; ++p) {
+        const double y = -9. + 6. * p, gap = p < 3 ? 0.6 : 3.0;
+        auto *l = sm.addTopology("Left", {model::Particle(-1., y, 0., bead), model::Particle(0., y, 0., end)});
+        l->addEdge(0, 1);
+        auto *r = sm.addTopology("Right", {model::Particle(gap + 1., y, 0., bead), model::Particle(gap, y, 0., end)});
+        r->addEdge(0, 1);
+    }
+    k->actions().calculateForces()->perform();
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    const auto tops = sm.getTopologies();
+    int left2 = 0, right2 = 0, n_done = 0;
+    for (auto *t : tops) {
+        left2 += t->type() == "Left2";
+        right2 += t->type() == "Right2";
+    }
+    for (const auto &p : sm.getParticles()) n_done += p.type() == done;
+    CHECK(left2 == 3 && right2 == 3 && n_done == 6, "topology-topology: Left2 %d Right2 %d done %d", left2, right2,
+          n_done);
+    CHECK(tops[6]->type() == "Left" && tops[7]->type() == "Right", "the distant pair stays: %s %s",
+          tops[6]->type().c_str(), tops[7]->type().c_str());
+    k->actions().calculateForces()->perform();
+    CHECK(std::fabs(sm.energy()) < 1e-9, "bonds configured again at rest: energy %.3g", sm.energy());
+    std::printf("topology-topology enzymatic: 3 of 4 pairs reacted\n");
+}
+
 // Recipe steps that move and append topology particles (Recipe::changeParticlePosition, appendNewParticle): three
 // chains of three beads 1 apart with bonds (k 10, length 1); a structural reaction (rate 1e10 while a chain has
 // fewer than four particles) moves bead 0 by 0.3 away from bead 1 and appends a bead 1.2 beyond bead 2, bonded to
@@ -1199,6 +1242,7 @@ int main(int argc, char **argv) {
             test_topology_reactions();
             test_topology_fusion();
             test_structural_move_and_append();
+            test_topology_topology_enzymatic();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

@@ -249,3 +249,113 @@ def test_fused_loop_equals_separate_actions_with_fusion():
     n_topo = 4 * n_chains
     joined = (h1 >= n_topo) & (h1 < n_topo + reserve)
     assert joined.sum() == tot.sum() and set(t1[joined].tolist()) <= {MID, END}
+
+
+# ---------------------------------------------------------------------------------- topology-topology enzymatic
+# species 0 A, 1 B, 2 A', 3 B' (all topology species); R1 "T0(A) + T1(B) -> T2(A') + T3(B')", R2 "T4(A) + T4(B) ->
+# T5(A') + T5(B')".  Topologies (one or two particles each, all 0.5 apart in pairs): 0 {A} + 1 {B} react by R1;
+# 2 {B, type 1} + 3 {A, type 0} react by R1 with its sides swapped (the lower id is the B); 4 {A, B} of type 4 is one
+# topology (no reaction within a topology); 5 {A} + 6 {B} of type 4 react by R2; 7 {A} with two B topologies 8, 9
+# on either side reacts once.
+TT_POS = np.array([[0, 0, 0], [0.5, 0, 0], [5, 0, 0], [5.5, 0, 0], [10, 0, 0], [10.5, 0, 0], [12, 5, 0],
+                   [12.5, 5, 0], [-5, 5, 0], [-4.5, 5, 0], [-5.5, 5, 0]], float)
+TT_TYPES = np.array([0, 1, 1, 0, 0, 1, 0, 1, 0, 1, 1])
+TT_TOP_OF = [0, 1, 2, 3, 4, 4, 5, 6, 7, 8, 9]
+TT_TOP_TYPES = [0, 1, 1, 0, 4, 4, 4, 0, 1, 1]
+TT_R1 = dict(type1=0, top_type1=0, type2=1, top_type2=1, type_to1=2, top_type_to1=2, type_to2=3, top_type_to2=3,
+             rate=1e10, radius=1.0, topology_topology=1)
+TT_R2 = dict(TT_R1, top_type1=4, top_type2=4, top_type_to1=5, top_type_to2=5)
+
+
+def _tt_oracle(pos, types, top_of, top_types, reactions, box=30.0, seed=3, pots=()):
+    o = po.System(4, [box] * 3, [1, 1, 1], diffusion=[1.0] * 4)
+    for kind, a, b, p in pots:
+        o.add_pair_potential(kind, a, b, p)
+    for x in reactions:
+        o.add_spatial_tt(x["type1"], x["top_type1"], x["type2"], x["top_type2"], x["type_to1"], x["top_type_to1"],
+                         x["type_to2"], x["top_type_to2"], x["rate"], x["radius"])
+    o.set_seed(seed)
+    o.add_particles(pos, np.asarray(types, np.int32), ids=np.arange(len(types)))
+    o.set_topologies(np.asarray(top_of), np.asarray(top_types))
+    o.nl_setup(max(o.max_cutoff(), max(x["radius"] for x in reactions)))
+    return o
+
+
+def test_oracle_topology_topology_enzymatic_scene():
+    o = _tt_oracle(TT_POS, TT_TYPES, TT_TOP_OF, TT_TOP_TYPES, [TT_R1, TT_R2])
+    assert o.topology_reactions(1.0, step=0) == 4
+    t = _by_id(o)
+    assert [t[i] for i in range(8)] == [2, 3, 3, 2, 0, 1, 2, 3]
+    tt = o.topology_types().tolist()
+    assert tt[:7] == [2, 3, 3, 2, 4, 5, 5]
+    # topology 7 reacted with exactly one of 8 and 9
+    assert t[8] == 2 and sorted([t[9], t[10]]) == [1, 3] and tt[7] == 2 and sorted(tt[8:]) == [1, 3]
+    assert o.topology_reaction_counts().tolist() == [3, 1]
+    assert o.topology_reactions(1.0, step=1) == 0  # no A in a topology of type 0 next to a B in one of type 1
+
+
+@pytest.mark.gpu
+def test_topology_topology_enzymatic_scene_on_the_device():
+    print("tt enzymatic scene", flush=True)
+    from readdy_amd import Engine
+    e = Engine(4, [1.0] * 4, topology_flavor=[1, 1, 1, 1], nl_cutoff=1.0)
+    r = e.add_replica([30.0] * 3, (1, 1, 1), seed=3, max_particles=128)
+    e.set_particles(r, TT_POS, TT_TYPES.astype(np.uint32))
+    e.set_spatial_reactions([TT_R1, TT_R2])
+    e.set_topologies(r, TT_TOP_TYPES, TT_TOP_OF)
+    o = _tt_oracle(TT_POS, TT_TYPES, TT_TOP_OF, TT_TOP_TYPES, [TT_R1, TT_R2])
+    e.integrate(1.0, np.zeros((11, 3)))  # no force, no noise: the step counter advances
+    e.evaluate_topology_reactions(1.0)
+    o.topology_reactions(1.0, step=0)
+    _, gt, gh = e.get_particles(r)
+    assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o)
+    assert e.topology_types(r).tolist() == o.topology_types().tolist()
+    assert e.topology_reaction_counts(r).tolist() == [3, 1]
+    reacted = e.topology_reacted(r).tolist()
+    assert reacted == [True, True, True, True, False, True, True, True] + [t == 3 for t in e.topology_types(r)[8:]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rate", [5.0, 1e10])
+def test_topology_topology_random_chains_event_for_event(rate):
+    """40 chains of four beads (species 0, topology type 0) among each other: bead pairs of different chains
+    within 1.2 react "T0(0) + T0(0) -> T1(2) + T1(2)" (a symmetric reaction); the same pairs react on both
+    sides over four integrated steps, one reaction per topology and step"""
+    from readdy_amd import Engine
+    from readdy_amd import _lib as L
+    print(f"tt chains rate {rate}", flush=True)
+    box, n_chains, dt = 12.0, 40, 0.01
+    pos, types, bonds, top = _growth_system(n_chains, 0, box, seed=8)
+    types = np.zeros(len(types), np.uint32)
+    rx = dict(type1=0, top_type1=0, type2=0, top_type2=0, type_to1=2, top_type_to1=1, type_to2=2, top_type_to2=1,
+              rate=rate, radius=1.2, topology_topology=1)
+    pots = [(L.POT_HARMONIC_REPULSION, 0, 0, [5.0, 0.8])]
+    e = Engine(4, [1.0] * 4, potentials=pots, topology_flavor=[1, 1, 1, 1], nl_cutoff=1.2)
+    r = e.add_replica([box] * 3, (1, 1, 1), seed=17, max_particles=4 * len(types))
+    e.set_particles(r, pos, types)
+    e.set_bonds(r, bonds, 100.0, 0.9)
+    e.set_spatial_reactions([rx])
+    e.set_topologies(r, np.zeros(n_chains), top)
+    o = _tt_oracle(pos, types, top, np.zeros(n_chains, int), [rx], box=box, seed=17, pots=pots)
+    for a, b in bonds:
+        o.add_bond(a, b, 100.0, 0.9)
+    rng = np.random.default_rng(9)
+    e.calculate_forces()
+    o.calculate_forces()
+    total = 0
+    for t in range(4):
+        noise = rng.normal(size=(len(types), 3))
+        e.integrate(dt, noise)
+        e.update_neighbor_list()
+        o.euler_bd(dt, noise.ravel(), step=t)
+        o.nl_fill()
+        e.evaluate_topology_reactions(dt)
+        o.topology_reactions(dt, step=t)
+        _, gt, gh = e.get_particles(r)
+        assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o), t
+        assert e.topology_types(r).tolist() == o.topology_types().tolist(), t
+        total += int(o.topology_reaction_counts().sum())
+        e.calculate_forces()
+        o.calculate_forces()
+    print(f"{total} topology-topology reactions", flush=True)
+    assert total > (5 if rate > 1e9 else 1)
