@@ -258,7 +258,8 @@ typedef struct {
     uint32_t fusion;             /* 0: TP_ENZYMATIC, 1: TP_FUSION (type_to2 a topology species) */
     uint32_t topology_topology;  /* 1: TT_ENZYMATIC "T1(p1) + T2(p2) -> T3(p3) + T4(p4)": the second particle is
                                     a topology particle of another topology, of type top_type2, which becomes
-                                    top_type_to2 (fusion 0) */
+                                    top_type_to2; with fusion 1, TT_FUSION "T1(p1) + T2(p2) -> T3(p3--p4)": the
+                                    second topology merges into the first, which becomes top_type_to1 */
     uint32_t top_type2, top_type_to2;
 } rdy_spatial_reaction;
 /* the engine's spatial topology reactions (TopologyRegistry::addSpatialReaction, TopologyRegistry.cpp:195-223);
@@ -290,6 +291,11 @@ int rdy_get_topology_fusions(rdy_engine *e, uint32_t replica, uint32_t cap, uint
 int rdy_set_particle_positions(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *handles, const double *pos);
 int rdy_append_topology_particles(rdy_engine *e, uint32_t replica, uint64_t n, const uint32_t *types,
                                   const double *pos, uint32_t *handles);
+/* the topology-topology fusions of the last evaluation (or fused step) of the replica, in key order: rows [n][4]
+ * of (surviving topology, absorbed topology, the surviving side's particle, the absorbed side's particle); the
+ * absorbed topology keeps its index with no particles until the caller sets the topologies again
+ * (Topology::appendTopology and the erase of the absorbed one, CPUEvaluateTopologyReactions.cpp:433-437) */
+int rdy_get_topology_merges(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *rows, uint32_t *n);
 /* the replica's topology types as the topology reactions left them; *n = the number of topologies */
 int rdy_get_topology_types(rdy_engine *e, uint32_t replica, uint32_t cap, uint32_t *types, uint32_t *n);
 /* per topology, 1 if a spatial topology reaction changed it in the last rdy_evaluate_topology_reactions call:

@@ -124,6 +124,9 @@ def lib():
         L.or_add_spatial_fusion.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_double] * 2
         L.or_set_topology_reserve.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
         L.or_add_spatial_tt.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 8 + [ctypes.c_double] * 2
+        L.or_add_spatial_tt_fusion.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7 + [ctypes.c_double] * 2
+        L.or_topology_merges.restype = _u64p
+        L.or_topology_merges.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_topology_fusions.restype = _u64p
         L.or_topology_fusions.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_set_topologies.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _ip, ctypes.c_size_t, _ip]
@@ -459,6 +462,17 @@ class System:
         """T1(type1) + T2(type2) -> T3(type_to1) + T4(type_to2), both sides in topologies"""
         _check(lib().or_add_spatial_tt(self._h, type1, top_type1, type2, top_type2, type_to1, top_type_to1, type_to2,
                                        top_type_to2, rate, radius))
+
+    def add_spatial_tt_fusion(self, type1, top_type1, type2, top_type2, type_to1, type_to2, top_type_to, rate, radius):
+        """T1(type1) + T2(type2) -> T3(type_to1--type_to2): the second topology merges into the first"""
+        _check(lib().or_add_spatial_tt_fusion(self._h, type1, top_type1, type2, top_type2, type_to1, type_to2,
+                                              top_type_to, rate, radius))
+
+    def topology_merges(self):
+        """rows (surviving topology, absorbed topology, surviving side's id, absorbed side's id)"""
+        n = ctypes.c_size_t()
+        p = lib().or_topology_merges(self._h, ctypes.byref(n))
+        return np.array([p[i] for i in range(4 * n.value)], dtype=np.uint64).reshape(n.value, 4)
 
     def set_topology_reserve(self, next_id, end_id):
         """ids next_id .. end_id - 1 for particles that join topologies by fusion"""

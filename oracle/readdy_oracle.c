@@ -528,6 +528,8 @@ struct or_system {
     uint64_t fuse_next, fuse_end; /* ids for particles joining topologies by fusion: fuse_next .. fuse_end - 1 */
     uint64_t *fusions;            /* the last call's fusions: (topology, topology particle id, old id, new id) */
     size_t n_fusions;
+    uint64_t *merges;             /* the last call's topology merges: (surviving, absorbed, their particles' ids) */
+    size_t n_merges;
     str_t str[OR_MAX_STR];
     int n_str;
     uint64_t str_counts[OR_MAX_STR];
@@ -559,6 +561,7 @@ void or_destroy(or_system *s) {
     if (!s) return;
     free(s->topology_of);
     free(s->fusions);
+    free(s->merges);
     free(s->top_types);
     free(s->e);
     free(s->blanks);
@@ -2223,6 +2226,20 @@ int or_add_spatial_tt(or_system *s, int type1, int top_type1, int type2, int top
     return 0;
 }
 
+int or_add_spatial_tt_fusion(or_system *s, int type1, int top_type1, int type2, int top_type2, int type_to1,
+                             int type_to2, int top_type_to, double rate, double radius) {
+    if (or_add_spatial_tt(s, type1, top_type1, type2, top_type2, type_to1, top_type_to, type_to2, top_type_to, rate,
+                          radius))
+        return -1;
+    s->str[s->n_str - 1].fusion = 1;
+    return 0;
+}
+
+const uint64_t *or_topology_merges(const or_system *s, size_t *n) {
+    if (n) *n = s->n_merges;
+    return s->merges;
+}
+
 int or_set_topology_reserve(or_system *s, uint64_t next_id, uint64_t end_id) {
     if (end_id < next_id) return fail("topology reserve: end before start");
     s->fuse_next = next_id;
@@ -2335,6 +2352,7 @@ int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_pe
     memset(s->str_counts, 0, sizeof s->str_counts); /* CPUEvaluateTopologyReactions.cpp:92-94 */
     if (n_performed) *n_performed = 0;
     s->n_fusions = 0;
+    s->n_merges = 0;
     if (!s->n_str || !s->n_tops) return 0;
     if (!s->nl_set) return fail("topology reactions need a neighbour list");
     tev_ud u;
@@ -2359,8 +2377,20 @@ int or_topology_reactions(or_system *s, double dt, uint32_t step, uint64_t *n_pe
             top_used[e->top] = top_used[e->top2] = 1;
             s->e[e->p].type = e->flip ? x->type_to2 : x->type_to1;
             s->e[e->q].type = e->flip ? x->type_to1 : x->type_to2;
-            s->top_types[e->top] = e->flip ? x->top_type_to2 : x->top_type_to1;
-            s->top_types[e->top2] = e->flip ? x->top_type_to1 : x->top_type_to2;
+            if (x->fusion) { /* appendTopology into the first side's topology, of type top_type_to1 (:433-437) */
+                s->top_types[e->top] = x->top_type_to1;
+                for (size_t k = 0; k < s->topology_of_len; ++k)
+                    if (s->topology_of[k] == e->top2) s->topology_of[k] = e->top;
+                s->merges = (uint64_t *)realloc(s->merges, 4 * (s->n_merges + 1) * sizeof(uint64_t));
+                uint64_t *row = s->merges + 4 * s->n_merges++;
+                row[0] = (uint64_t)e->top;
+                row[1] = (uint64_t)e->top2;
+                row[2] = e->ip;
+                row[3] = e->iq;
+            } else {
+                s->top_types[e->top] = e->flip ? x->top_type_to2 : x->top_type_to1;
+                s->top_types[e->top2] = e->flip ? x->top_type_to1 : x->top_type_to2;
+            }
             s->str_counts[e->r]++;
             ++done;
             continue;

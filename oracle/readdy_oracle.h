@@ -168,6 +168,11 @@ int or_set_topology_reserve(or_system *s, uint64_t next_id, uint64_t end_id);
 /* the topology-topology enzymatic kind "T1(p1) + T2(p2) -> T3(p3) + T4(p4)" (STRMode::TT_ENZYMATIC) */
 int or_add_spatial_tt(or_system *s, int type1, int top_type1, int type2, int top_type2, int type_to1, int top_type_to1,
                       int type_to2, int top_type_to2, double rate, double radius);
+/* the topology-topology fusion kind "T1(p1) + T2(p2) -> T3(p3--p4)" (STRMode::TT_FUSION): the second topology's
+ * particles join the first, which becomes top_type_to; or_topology_merges lists (surviving, absorbed, ids) */
+int or_add_spatial_tt_fusion(or_system *s, int type1, int top_type1, int type2, int top_type2, int type_to1,
+                             int type_to2, int top_type_to, double rate, double radius);
+const uint64_t *or_topology_merges(const or_system *s, size_t *n);
 const uint64_t *or_topology_fusions(const or_system *s, size_t *n);
 /* topology_of_id[id] = the topology of the particle with that id (-1: none), for ids below n_ids */
 int or_set_topologies(or_system *s, size_t n_ids, const int *topology_of_id, size_t n_tops, const int *top_types);

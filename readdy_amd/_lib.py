@@ -125,6 +125,7 @@ SIGNATURES = {
     "rdy_set_topologies": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
     "rdy_reserve_topology_particles": (_int, [_vp, _u32, _u32]),
     "rdy_get_topology_fusions": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
+    "rdy_get_topology_merges": (_int, [_vp, _u32, _u32, _u32p, _u32p]),
     "rdy_set_particle_positions": (_int, [_vp, _u32, _u64, _u32p, _dp]),
     "rdy_append_topology_particles": (_int, [_vp, _u32, _u64, _u32p, _dp, _u32p]),
     "rdy_get_topology_types": (_int, [_vp, _u32, _u32, _u32p, _u32p]),

@@ -198,6 +198,7 @@ struct rdy_engine {
     TopoEvent *tev = nullptr;
     uint32_t *tev_count = nullptr;
     uint32_t *tfus = nullptr, *tfus_count = nullptr;  // topology-particle fusions of the last evaluation
+    uint32_t *tmrg = nullptr, *tmrg_count = nullptr;  // topology-topology fusions (merges) of the last evaluation
     unsigned long long *str_step = nullptr, *str_total = nullptr;
     std::vector<BondTerm> terms_h[3];
     BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
@@ -785,6 +786,8 @@ TopoTables topo_tables(const rdy_engine *e) {
     t.loc = e->loc;
     t.fus = e->tfus;
     t.fus_count = e->tfus_count;
+    t.mrg = e->tmrg;
+    t.mrg_count = e->tmrg_count;
     t.topo_owner = e->topo_owner;
     t.ttype = e->ttype;
     t.treact = e->treact;
@@ -1200,7 +1203,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->tfus, e->tfus_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->tfus, e->tfus_count, e->tmrg, e->tmrg_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -2293,8 +2296,8 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         const rdy_spatial_reaction &x = sr[i];
         if (x.type1 >= e->n_types || x.type2 >= e->n_types || x.type_to1 >= e->n_types || x.type_to2 >= e->n_types)
             return set_err(RDY_ERR_INVALID, "spatial topology reaction species out of range");
-        if (x.fusion > 1 || x.topology_topology > 1 || (x.fusion && x.topology_topology))
-            return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion and topology_topology are 0 or 1, not both");
+        if (x.fusion > 1 || x.topology_topology > 1)
+            return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion and topology_topology are 0 or 1");
         if (!e->topo_flavor[x.type1] || !e->topo_flavor[x.type_to1])
             return set_err(RDY_ERR_INVALID, "the topology side of a topology-particle reaction must stay a topology species");
         if (x.topology_topology) {
@@ -2330,8 +2333,17 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         HIP_TRY(dalloc(&e->tev_count, nr));
         HIP_TRY(hipMemsetAsync(e->tev_count, 0, nr * sizeof(uint32_t), e->stream));
     }
-    bool fusion = false;
-    for (uint32_t i = 0; i < n; ++i) fusion = fusion || sr[i].fusion;
+    bool fusion = false, merge = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        fusion = fusion || (sr[i].fusion && !sr[i].topology_topology);
+        merge = merge || (sr[i].fusion && sr[i].topology_topology);
+    }
+    if (merge && !e->tmrg) {
+        const size_t nr = e->max_reps;
+        HIP_TRY(dalloc(&e->tmrg, nr * TOPO_MAX_EVENTS * 4));
+        HIP_TRY(dalloc(&e->tmrg_count, nr));
+        HIP_TRY(hipMemsetAsync(e->tmrg_count, 0, nr * sizeof(uint32_t), e->stream));
+    }
     if (fusion && !e->tfus) {
         const size_t nr = e->max_reps;
         HIP_TRY(dalloc(&e->tfus, nr * TOPO_MAX_EVENTS * 4));
@@ -2444,6 +2456,23 @@ int rdy_get_topology_fusions(rdy_engine *e, uint32_t r, uint32_t cap, uint32_t *
         HIP_TRY(hipStreamSynchronize(e->stream));
     }
     return sync_topo_count(e, r);
+}
+
+int rdy_get_topology_merges(rdy_engine *e, uint32_t r, uint32_t cap, uint32_t *rows, uint32_t *n) {
+    int rc = check_replica(e, r);
+    if (rc) return rc;
+    uint32_t m = 0;
+    if (e->tmrg_count) {
+        HIP_TRY(hipMemcpyAsync(&m, e->tmrg_count + r, 4, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    if (n) *n = m;
+    const uint32_t k = std::min(cap, m);
+    if (k && rows) {
+        HIP_TRY(hipMemcpyAsync(rows, e->tmrg + 4ull * r * TOPO_MAX_EVENTS, 16ull * k, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    return RDY_OK;
 }
 
 int rdy_evaluate_topology_reactions(rdy_engine *e, double dt) {

@@ -54,6 +54,8 @@ struct TopoTables {
     uint32_t *loc;               // [topology particle slot] the record's bin slot (written for joining particles)
     uint32_t *fus;               // [replica][TOPO_MAX_EVENTS][4] the last evaluation's fusions
     uint32_t *fus_count;         // [replica]
+    uint32_t *mrg;               // [replica][TOPO_MAX_EVENTS][4] the last evaluation's topology merges
+    uint32_t *mrg_count;         // [replica]
     const uint32_t *topo_owner;  // [topology particle slot] replica, UINT32_MAX when unused
     uint32_t *ttype;             // topology types; replica r's at reps[r].tt_base
     uint32_t *treact;            // per topology (as ttype): the serial of the evaluation it last reacted in
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
     __shared__ uint8_t won[TOPO_MAX_EVENTS];
     __shared__ uint32_t used_top[TOPO_HASH], used_free[TOPO_HASH];
     __shared__ uint32_t fidx[TOPO_MAX_EVENTS];  // a won fusion's number: its joining particle's reserve handle
-    __shared__ uint32_t s_nf;                    // fusions performed
+    __shared__ uint32_t s_nf, s_nm;              // fusions and topology merges performed
     const uint32_t r = blockIdx.x, tid = threadIdx.x;
     ReplicaDev &R = reps[r];
     if (!R.in_use || (require_active && !R.active)) {
@@ -176,6 +178,7 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
     }
     if (tid < TT.n_sr) TT.cnt_step[(size_t)r * TT.n_sr + tid] = 0ull;
     if (tid == 0 && TT.fus_count) TT.fus_count[r] = 0u;
+    if (tid == 0 && TT.mrg_count) TT.mrg_count[r] = 0u;
     uint32_t ne = TT.ev_count[r];
     ne = ne < TT.ev_cap ? ne : TT.ev_cap;
     if (ne > TOPO_MAX_EVENTS) {
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
     __syncthreads();
     const uint32_t n_topo0 = R.n_topo, reserve = R.topo_range > n_topo0 ? R.topo_range - n_topo0 : 0u;
     if (tid == 0) {  // the greedy pass in key order (few accepted events per step)
-        uint32_t nf = 0;
+        uint32_t nf = 0, nm = 0;
         for (uint32_t q = 0; q < ne; ++q) {
             const TopoEvent a = ev[order[q]];
             bool top_free = true, partner_free = true;
@@ -236,12 +239,17 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
             top_free = !probe(used_top, a.top) && !(tt && probe(used_top, a.top2));
             partner_free = tt || !probe(used_free, a.s_free);
             if (!top_free || !partner_free) continue;
-            if (TT.sr[a.sr & ~SR_FLIP].fusion) {  // the joining particle takes the next reserve handle, in key order
-                if (nf >= reserve || !TT.fus) {
-                    atomicAdd(&err[ERR_TOPO_RESERVE], 1u);
-                    continue;
+            if (TT.sr[a.sr & ~SR_FLIP].fusion) {
+                if (tt) {  // a topology merge: its row number, in key order
+                    if (!TT.mrg) continue;
+                    fidx[order[q]] = nm++;
+                } else {  // the joining particle takes the next reserve handle, in key order
+                    if (nf >= reserve || !TT.fus) {
+                        atomicAdd(&err[ERR_TOPO_RESERVE], 1u);
+                        continue;
+                    }
+                    fidx[order[q]] = nf++;
                 }
-                fidx[order[q]] = nf++;
             }
             hash_insert(used_top, a.top);
             if (tt) hash_insert(used_top, a.top2);
@@ -249,6 +257,7 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
             won[order[q]] = 1;
         }
         s_nf = nf;
+        s_nm = nm;
     }
     __syncthreads();
     const bool fix = edits != nullptr;
@@ -266,12 +275,21 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         // a record the step's reactions changed already carries its change for the fix-up (TW_NEW), which
         // redoes its move on the final state; every other retyped record is listed once here
         const uint32_t n1 = fix && !(tw1 & TW_NEW) ? TW_NEW : 0u, n2 = fix && !(tw2 & TW_NEW) ? TW_NEW : 0u;
-        if (X.tt) {  // handleTopologyTopologyReaction (:406-475), enzymatic: each side takes its products
+        if (X.tt) {  // handleTopologyTopologyReaction (:406-475): each side takes its products
             const bool flip = (a.sr & SR_FLIP) != 0u;
             cur.bins[a.s_top].tw = make_tw(flip ? X.type_to2 : X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
             cur.bins[a.s_free].tw = make_tw(flip ? X.type_to1 : X.type_to2, tw2 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n2;
-            TT.ttype[R.tt_base + a.top] = flip ? X.top_type_to2 : X.top_type_to1;
-            TT.ttype[R.tt_base + a.top2] = flip ? X.top_type_to1 : X.top_type_to2;
+            if (X.fusion) {  // the partner's topology merges into this one, which takes top_type_to1 (:433-437)
+                TT.ttype[R.tt_base + a.top] = X.top_type_to1;
+                uint32_t *row = TT.mrg + 4ull * ((size_t)r * TOPO_MAX_EVENTS + fidx[k]);
+                row[0] = a.top;
+                row[1] = a.top2;
+                row[2] = a.h_top;
+                row[3] = a.h_free;
+            } else {
+                TT.ttype[R.tt_base + a.top] = flip ? X.top_type_to2 : X.top_type_to1;
+                TT.ttype[R.tt_base + a.top2] = flip ? X.top_type_to1 : X.top_type_to2;
+            }
             if (TT.serial) TT.treact[R.tt_base + a.top2] = TT.serial;
         } else {
             cur.bins[a.s_top].tw = make_tw(X.type_to1, tw1 & (TW_TOPO | TW_OUTSIDE | TW_NEW), r) | n1;
@@ -297,12 +315,23 @@ __global__ __launch_bounds__(256) void k_topo_resolve(Grid cur, ReplicaDev *reps
         if (n1) edit(a.s_top, tw_type(tw1), a.h_top);
         if (n2) edit(a.s_free, tw_type(tw2), a.h_free);  // (its moved copy carries the handle it had)
     }
+    if (s_nm) {  // the absorbed topologies' particles join the surviving ones (the merge rows are written)
+        __syncthreads();
+        const uint32_t *mrows = TT.mrg + 4ull * (size_t)r * TOPO_MAX_EVENTS;
+        const uint32_t nt = n_topo0 + s_nf;
+        for (uint32_t i = tid; i < nt; i += blockDim.x) {
+            const uint32_t t = TT.topo_of[R.topo_base + i];
+            for (uint32_t m = 0; m < s_nm; ++m)
+                if (t == mrows[4 * m + 1]) TT.topo_of[R.topo_base + i] = mrows[4 * m];
+        }
+    }
     if (tid == 0) {
         TT.ev_count[r] = 0u;
         if (s_nf) {  // every thread read n_topo0 before the greedy pass's barrier
             R.n_topo = n_topo0 + s_nf;
             TT.fus_count[r] = s_nf;
         }
+        if (s_nm) TT.mrg_count[r] = s_nm;
     }
 }
 

@@ -349,6 +349,15 @@ class Engine:
         L.check(self._lib.rdy_get_topology_fusions(self._h, r, n.value, _ptr(out, _u32p), ctypes.byref(n)))
         return out[:n.value]
 
+    def topology_merges(self, r=0):
+        """rows (surviving topology, absorbed topology, surviving side's particle, absorbed side's particle) of the
+        last evaluation's topology-topology fusions, in key order"""
+        n = ctypes.c_uint32()
+        L.check(self._lib.rdy_get_topology_merges(self._h, r, 0, None, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 4), dtype=np.uint32)
+        L.check(self._lib.rdy_get_topology_merges(self._h, r, n.value, _ptr(out, _u32p), ctypes.byref(n)))
+        return out[:n.value]
+
     def evaluate_topology_reactions(self, dt):
         L.check(self._lib.rdy_evaluate_topology_reactions(self._h, dt))
 

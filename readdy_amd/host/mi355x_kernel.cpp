@@ -92,6 +92,7 @@ private:
     void syncTopologies();
     void structuralReactions(scalar dt, const std::vector<uint8_t> &reacted);
     void spatialReactions(scalar dt);
+    std::vector<uint8_t> _spatialReacted;  // per topology (after merges): changed by the last spatial pass
     MI355XKernel *_k;
     scalar _time = 0;
     std::vector<Particle> _topoParticles, _freeParticles;  // pending host copies (before the first flush)
@@ -350,11 +351,13 @@ void MI355XStateModel::configureSpatialReactions() {
     if (_spatialSet) return;
     std::vector<rdy_spatial_reaction> v;
     for (const auto &r : _k->context().topologyRegistry().spatialReactions()) {
-        if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION && r.mode != STRMode::TT_ENZYMATIC)
+        if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION && r.mode != STRMode::TT_ENZYMATIC &&
+            r.mode != STRMode::TT_FUSION)
             throw std::invalid_argument("spatial topology reaction \"" + r.name +
                                         "\": the MI355X kernel performs the topology-particle kinds "
                                         "(T1(p1) + (p2) -> T2(p3) + (p4) and T1(p1) + (p2) -> T2(p3--p4)) and the "
-                                        "topology-topology enzymatic kind (T1(p1) + T2(p2) -> T3(p3) + T4(p4)) only");
+                                        "topology-topology kinds without self-fusion (T1(p1) + T2(p2) -> T3(p3) + "
+                                        "T4(p4), T1(p1) + T2(p2) -> T3(p3--p4)) only");
         rdy_spatial_reaction q{};
         q.type1 = r.type1;
         q.top_type1 = (uint32_t)r.topType1;
@@ -364,8 +367,8 @@ void MI355XStateModel::configureSpatialReactions() {
         q.type_to2 = r.typeTo2;
         q.rate = r.rate;
         q.radius = r.radius;
-        q.fusion = r.mode == STRMode::TP_FUSION ? 1u : 0u;
-        if (r.mode == STRMode::TT_ENZYMATIC) {
+        q.fusion = (r.mode == STRMode::TP_FUSION || r.mode == STRMode::TT_FUSION) ? 1u : 0u;
+        if (r.mode == STRMode::TT_ENZYMATIC || r.mode == STRMode::TT_FUSION) {
             q.topology_topology = 1u;
             q.top_type2 = (uint32_t)r.topType2;
             q.top_type_to2 = (uint32_t)r.topTypeTo2;
@@ -418,7 +421,7 @@ bool MI355XStateModel::reactionsChangeBondedTerms() const {
         return false;
     };
     for (const auto &r : reg.spatialReactions()) {
-        if (r.mode == STRMode::TP_FUSION) return true;  // every fusion adds an edge
+        if (r.mode == STRMode::TP_FUSION || r.mode == STRMode::TT_FUSION) return true;  // every fusion adds an edge
         if (r.typeTo1 != r.type1 && (named(r.type1) || named(r.typeTo1))) return true;
         if (r.mode == STRMode::TT_ENZYMATIC && r.typeTo2 != r.type2 && (named(r.type2) || named(r.typeTo2))) return true;
     }
@@ -436,9 +439,7 @@ void MI355XStateModel::evaluateTopologyReactions(scalar dt) {
     std::vector<uint8_t> reacted(_topologies.size(), 0);
     if (!sr.empty()) {
         spatialReactions(dt);
-        uint32_t n = 0;
-        ok(rdy_get_topology_reacted(_k->engine(), _k->replicaId(), (uint32_t)reacted.size(), reacted.data(), &n),
-           "topology reactions");
+        reacted = _spatialReacted;
     }
     if (reg.hasStructuralReactions()) structuralReactions(dt, reacted);
 }
@@ -543,6 +544,12 @@ ParticleId MI355XStateModel::appendTopologyParticle(ParticleTypeId type, const V
 void MI355XStateModel::spatialReactions(scalar dt) {
     const auto &sr = _k->context().topologyRegistry().spatialReactions();
     ok(rdy_evaluate_topology_reactions(_k->engine(), dt), "topology reactions");
+    // the topologies this pass changed, in the device's present order (merges below remove some)
+    _spatialReacted.assign(_topologies.size(), 0);
+    uint32_t nr = 0;
+    ok(rdy_get_topology_reacted(_k->engine(), _k->replicaId(), (uint32_t)_spatialReacted.size(), _spatialReacted.data(),
+                                &nr),
+       "topology reactions");
     std::vector<uint64_t> c(sr.size(), 0);
     ok(rdy_get_topology_reaction_counts(_k->engine(), _k->replicaId(), 0, c.data()), "topology reaction counts");
     bool any = false, retyped = false;
@@ -571,6 +578,32 @@ void MI355XStateModel::spatialReactions(scalar dt) {
     }
     _topoStale = true;
     syncTopologies();
+    // topology-topology fusions: the absorbed topology's graph joins the surviving one with the new edge, and
+    // the absorbed one leaves the list (appendTopology and erase, CPUEvaluateTopologyReactions.cpp:433-437)
+    uint32_t nm = 0;
+    ok(rdy_get_topology_merges(_k->engine(), _k->replicaId(), 0, nullptr, &nm), "topology merges");
+    if (nm) {
+        std::vector<uint32_t> rows(4ull * nm);
+        ok(rdy_get_topology_merges(_k->engine(), _k->replicaId(), nm, rows.data(), &nm), "topology merges");
+        std::vector<char> gone(_topologies.size(), 0);
+        for (uint32_t i = 0; i < nm; ++i) {
+            const uint32_t a = rows[4 * i], b = rows[4 * i + 1];
+            if (a >= _topologies.size() || b >= _topologies.size() || gone[a] || gone[b])
+                throw std::logic_error("topology merge rows out of range");
+            _topologies[a]->appendTopology(*_topologies[b], rows[4 * i + 3], rows[4 * i + 2]);
+            gone[b] = 1;
+        }
+        std::vector<std::unique_ptr<GraphTopology>> next;
+        std::vector<uint8_t> flags;
+        for (std::size_t t = 0; t < _topologies.size(); ++t)
+            if (!gone[t]) {
+                next.push_back(std::move(_topologies[t]));
+                flags.push_back(t < _spatialReacted.size() ? _spatialReacted[t] : 0);
+            }
+        _topologies = std::move(next);
+        _spatialReacted = std::move(flags);
+        retyped = true;
+    }
     if (retyped) {  // GraphTopology::configure after the reaction (TopologyReactionActions.h)
         _topoDirty = true;
         configureTopologies();

@@ -461,6 +461,21 @@ std::size_t GraphTopology::addVertex(std::size_t particle) {
     return _particles.size() - 1;
 }
 
+void GraphTopology::appendTopology(const GraphTopology &other, std::size_t otherParticle, std::size_t thisParticle) {
+    const auto mine = std::find(_particles.begin(), _particles.end(), thisParticle);
+    const auto theirs = std::find(other._particles.begin(), other._particles.end(), otherParticle);
+    if (mine == _particles.end() || theirs == other._particles.end())
+        throw std::invalid_argument("appendTopology: a particle of the new edge is not in its topology");
+    const std::size_t off = _particles.size(), v = (std::size_t)(mine - _particles.begin()),
+                      w = off + (std::size_t)(theirs - other._particles.begin());
+    for (std::size_t i = 0; i < other._particles.size(); ++i) {
+        _particles.push_back(other._particles[i]);
+        _neighbors.emplace_back();
+        for (std::size_t n : other._neighbors[i]) _neighbors.back().push_back(off + n);
+    }
+    addEdge(v, w);
+}
+
 void GraphTopology::appendParticle(std::size_t particle, std::size_t counterpart) {
     const auto it = std::find(_particles.begin(), _particles.end(), counterpart);
     if (it == _particles.end()) throw std::invalid_argument("appendParticle: the counterpart is not in this topology");

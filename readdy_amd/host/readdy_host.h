@@ -278,6 +278,9 @@ public:
     void appendParticle(std::size_t particle, std::size_t counterpart);
     // a new vertex for particle `particle`, without edges; returns its vertex index
     std::size_t addVertex(std::size_t particle);
+    // GraphTopology::appendTopology: the other topology's vertices and edges join this one, with an edge between
+    // the vertices of particle `thisParticle` (here) and `otherParticle` (there)
+    void appendTopology(const GraphTopology &other, std::size_t otherParticle, std::size_t thisParticle);
     bool removeEdge(std::size_t v1, std::size_t v2);  // false when there is no such edge
     bool hasEdge(std::size_t v1, std::size_t v2) const;
     std::size_t nParticles() const { return _particles.size(); }

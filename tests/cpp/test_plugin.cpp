@@ -1033,6 +1033,49 @@ static void test_topology_topology_enzymatic() {
     std::printf("topology-topology enzymatic: 3 of 4 pairs reacted\n");
 }
 
+// Topology-topology fusion through the plugin ("merge: Left(end) + Right(end) -> Joined(mid--mid)", rate 1e10):
+// three pairs of two-bead chains whose ends are 0.6 apart merge into one four-bead topology each (the distant
+// fourth pair stays two topologies); the new bond (mid-mid, k 10, length 1) is stretched to 0.6: 10 * 0.4^2 each.
+static void test_topology_topology_fusion() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {30., 30., 30.};
+    c.particleTypes().addTopologyType("bead", 0.1);
+    c.particleTypes().addTopologyType("end", 0.1);
+    c.particleTypes().addTopologyType("mid", 0.1);
+    auto &reg = c.topologyRegistry();
+    for (const char *t : {"Left", "Right", "Joined"}) reg.addType(t);
+    reg.configureBondPotential("bead", "end", {10., 1.});
+    reg.configureBondPotential("bead", "mid", {10., 1.});
+    reg.configureBondPotential("mid", "mid", {10., 1.});
+    reg.addSpatialReaction("merge: Left(end) + Right(end) -> Joined(mid--mid)", 1e10, 1.0);
+    const auto bead = c.particleTypes().idOf("bead"), end = c.particleTypes().idOf("end");
+    auto &sm = k->stateModel();
+    for (int p = 0; p < 4; ++p) {
+        const double y = -9. + 6. * p, gap = p < 3 ? 0.6 : 3.0;
+        auto *l = sm.addTopology("Left", {model::Particle(-1., y, 0., bead), model::Particle(0., y, 0., end)});
+        l->addEdge(0, 1);
+        auto *r = sm.addTopology("Right", {model::Particle(gap + 1., y, 0., bead), model::Particle(gap, y, 0., end)});
+        r->addEdge(0, 1);
+    }
+    k->actions().calculateForces()->perform();
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    const auto tops = sm.getTopologies();
+    int joined = 0;
+    for (auto *t : tops)
+        if (t->type() == "Joined") {
+            ++joined;
+            CHECK(t->nParticles() == 4u && t->isConnected(), "a merged topology of %zu particles", t->nParticles());
+        }
+    CHECK(tops.size() == 5u && joined == 3, "topology-topology fusion: %zu topologies, %d joined", tops.size(), joined);
+    k->actions().calculateForces()->perform();
+    const double e = sm.energy(), want = 3 * 10. * 0.16;
+    CHECK(std::fabs(e - want) < 1e-6 * want, "merged bonds: energy %.9g, expected %.9g", e, want);
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    CHECK(sm.getTopologies().size() == 5u, "nothing left to merge: %zu topologies", sm.getTopologies().size());
+    std::printf("topology-topology fusion: 3 of 4 pairs merged\n");
+}
+
 // Recipe steps that move and append topology particles (Recipe::changeParticlePosition, appendNewParticle): three
 // chains of three beads 1 apart with bonds (k 10, length 1); a structural reaction (rate 1e10 while a chain has
 // fewer than four particles) moves bead 0 by 0.3 away from bead 1 and appends a bead 1.2 beyond bead 2, bonded to
@@ -1243,6 +1286,7 @@ int main(int argc, char **argv) {
             test_topology_fusion();
             test_structural_move_and_append();
             test_topology_topology_enzymatic();
+            test_topology_topology_fusion();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());

@@ -359,3 +359,136 @@ def test_topology_topology_random_chains_event_for_event(rate):
         o.calculate_forces()
     print(f"{total} topology-topology reactions", flush=True)
     assert total > (5 if rate > 1e9 else 1)
+
+
+# ----------------------------------------------------------------------------------- topology-topology fusion
+# species 0 A, 1 B, 2 A', 3 B'; M "T0(A) + T1(B) -> T2(A'--B')" merges the B side's topology into the A side's
+# (the side of the lower id survives), of type 2; C "T2(A') + T1(B') -> T2(A') + T1(B)" can only fire between
+# different topologies, so after correct merges it finds nothing (an absorbed particle left in its old topology,
+# of type 1, would react).  Topologies: 0 {A} + 1 {B}; 2 {B} + 3 {A} (reverse order); 4 {A} with 5 {B}, 6 {B}
+# on either side (merges with one).
+TM_POS = np.array([[0, 0, 0], [0.5, 0, 0], [5, 0, 0], [5.5, 0, 0], [-5, 5, 0], [-4.5, 5, 0], [-5.5, 5, 0]], float)
+TM_TYPES = np.array([0, 1, 1, 0, 0, 1, 1])
+TM_TOP_OF = [0, 1, 2, 3, 4, 5, 6]
+TM_TOP_TYPES = [0, 1, 1, 0, 0, 1, 1]
+TM_M = dict(type1=0, top_type1=0, type2=1, top_type2=1, type_to1=2, top_type_to1=2, type_to2=3, top_type_to2=2,
+            rate=1e10, radius=1.0, topology_topology=1, fusion=1)
+TM_C = dict(type1=2, top_type1=2, type2=3, top_type2=1, type_to1=2, top_type_to1=2, type_to2=1, top_type_to2=1,
+            rate=1e10, radius=1.0, topology_topology=1)
+
+
+def _tm_oracle(pos, types, top_of, top_types, reactions, box=30.0, seed=3, pots=()):
+    o = po.System(4, [box] * 3, [1, 1, 1], diffusion=[1.0] * 4)
+    for kind, a, b, p in pots:
+        o.add_pair_potential(kind, a, b, p)
+    for x in reactions:
+        if x.get("fusion"):
+            o.add_spatial_tt_fusion(x["type1"], x["top_type1"], x["type2"], x["top_type2"], x["type_to1"],
+                                    x["type_to2"], x["top_type_to1"], x["rate"], x["radius"])
+        else:
+            o.add_spatial_tt(x["type1"], x["top_type1"], x["type2"], x["top_type2"], x["type_to1"], x["top_type_to1"],
+                             x["type_to2"], x["top_type_to2"], x["rate"], x["radius"])
+    o.set_seed(seed)
+    o.add_particles(pos, np.asarray(types, np.int32), ids=np.arange(len(types)))
+    o.set_topologies(np.asarray(top_of), np.asarray(top_types))
+    o.nl_setup(max(o.max_cutoff(), max(x["radius"] for x in reactions)))
+    return o
+
+
+def test_oracle_topology_topology_fusion_scene():
+    o = _tm_oracle(TM_POS, TM_TYPES, TM_TOP_OF, TM_TOP_TYPES, [TM_M, TM_C])
+    assert o.topology_reactions(1.0, step=0) == 3
+    rows = sorted(map(tuple, o.topology_merges().tolist()))
+    assert rows[:2] == [(0, 1, 0, 1), (2, 3, 2, 3)] and rows[2][:3] in [(4, 5, 4), (4, 6, 4)]
+    t = _by_id(o)
+    assert [t[i] for i in range(5)] == [2, 3, 3, 2, 2]  # the reverse pair: the B (lower id) takes B'
+    tt = o.topology_types().tolist()
+    assert tt[0] == 2 and tt[2] == 2 and tt[4] == 2 and tt[1] == 1 and tt[3] == 0
+    # the check reaction finds no A' and B' in different topologies
+    assert o.topology_reactions(1.0, step=1) == 0
+
+
+@pytest.mark.gpu
+def test_topology_topology_fusion_scene_on_the_device():
+    print("tt fusion scene", flush=True)
+    from readdy_amd import Engine
+    e = Engine(4, [1.0] * 4, topology_flavor=[1, 1, 1, 1], nl_cutoff=1.0)
+    r = e.add_replica([30.0] * 3, (1, 1, 1), seed=3, max_particles=128)
+    e.set_particles(r, TM_POS, TM_TYPES.astype(np.uint32))
+    e.set_spatial_reactions([TM_M, TM_C])
+    e.set_topologies(r, TM_TOP_TYPES, TM_TOP_OF)
+    o = _tm_oracle(TM_POS, TM_TYPES, TM_TOP_OF, TM_TOP_TYPES, [TM_M, TM_C])
+    for step in range(2):
+        e.integrate(1.0, np.zeros((7, 3)))
+        e.evaluate_topology_reactions(1.0)
+        o.topology_reactions(1.0, step=step)
+        _, gt, gh = e.get_particles(r)
+        assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o), step
+        assert e.topology_types(r).tolist() == o.topology_types().tolist(), step
+        assert e.topology_merges(r).tolist() == o.topology_merges().tolist(), step
+    assert e.topology_reaction_counts(r, totals=True).tolist() == [3, 0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rate", [50.0, 1e10])
+def test_topology_topology_fusion_random_chains_event_for_event(rate):
+    """40 two-bead chains (ends of species 0 in topologies of type 0, the other bead species 1) merge pairwise
+    by "T0(0) + T0(0) -> T1(2--2)", then merged chains of type 1 merge on by "T1(0) + T0(0) -> T1(2--2)" and
+    "T1(0) + T1(0) -> T1(2--2)" while free ends meet: the same merges on both sides over four integrated steps,
+    the merged topologies never reacting within themselves"""
+    from readdy_amd import Engine
+    from readdy_amd import _lib as L
+    print(f"tt fusion chains rate {rate}", flush=True)
+    box, n, dt = 8.0, 40, 0.01
+    rng = np.random.default_rng(21)
+    pos, types, top = [], [], []
+    for c in range(n):
+        p0 = rng.uniform(-box / 2 + 1, box / 2 - 1, 3)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pos += [p0, p0 + 0.9 * u]
+        types += [1, 0]
+        top += [c, c]
+    pos, types = np.array(pos), np.array(types, np.uint32)
+    bonds = [(2 * c, 2 * c + 1) for c in range(n)]
+    base = dict(type1=0, type2=0, type_to1=2, type_to2=2, top_type_to1=1, top_type_to2=1, rate=rate, radius=1.2,
+                topology_topology=1, fusion=1)
+    rxs = [dict(base, top_type1=0, top_type2=0), dict(base, top_type1=1, top_type2=0),
+           dict(base, top_type1=1, top_type2=1)]
+    pots = [(L.POT_HARMONIC_REPULSION, 1, 1, [5.0, 0.8])]
+    e = Engine(3, [1.0] * 3, potentials=pots, topology_flavor=[1, 1, 1], nl_cutoff=1.2)
+    r = e.add_replica([box] * 3, (1, 1, 1), seed=29, max_particles=4 * len(types))
+    e.set_particles(r, pos, types)
+    e.set_bonds(r, bonds, 100.0, 0.9)
+    e.set_spatial_reactions(rxs)
+    e.set_topologies(r, np.zeros(n), top)
+    o = po.System(3, [box] * 3, [1, 1, 1], diffusion=[1.0] * 3)
+    o.add_pair_potential(*pots[0])
+    for x in rxs:
+        o.add_spatial_tt_fusion(x["type1"], x["top_type1"], x["type2"], x["top_type2"], 2, 2, 1, rate, 1.2)
+    o.set_seed(29)
+    o.add_particles(pos, types.astype(np.int32), ids=np.arange(len(types)))
+    o.set_topologies(np.asarray(top), np.zeros(n, int))
+    for a, b in bonds:
+        o.add_bond(a, b, 100.0, 0.9)
+    o.nl_setup(max(o.max_cutoff(), 1.2))
+    e.calculate_forces()
+    o.calculate_forces()
+    merged = 0
+    for t in range(4):
+        noise = rng.normal(size=(len(types), 3))
+        e.integrate(dt, noise)
+        e.update_neighbor_list()
+        o.euler_bd(dt, noise.ravel(), step=t)
+        o.nl_fill()
+        e.evaluate_topology_reactions(dt)
+        o.topology_reactions(dt, step=t)
+        _, gt, gh = e.get_particles(r)
+        assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o), t
+        assert e.topology_types(r).tolist() == o.topology_types().tolist(), t
+        assert e.topology_merges(r).tolist() == o.topology_merges().tolist(), t
+        merged += len(o.topology_merges())
+        e.calculate_forces()
+        o.calculate_forces()
+    print(f"{merged} merges", flush=True)
+    assert merged > 3
