@@ -1,26 +1,29 @@
-// Spatial topology reactions of the topology-particle kinds, enzymatic "T1(p1) + (p2) -> T2(p3) + (p4)" and
-// fusion "T1(p1) + (p2) -> T2(p3--p4)" (SpatialTopologyReaction.h:64-236; CPUEvaluateTopologyReactions.cpp:88-475).
+// Spatial topology reactions (SpatialTopologyReaction.h:64-236; CPUEvaluateTopologyReactions.cpp:88-475):
+// topology-particle enzymatic "T1(p1) + (p2) -> T2(p3) + (p4)" and fusion "T1(p1) + (p2) -> T2(p3--p4)",
+// topology-topology enzymatic "T1(p1) + T2(p2) -> T3(p3) + T4(p4)" and fusion "T1(p1) + T2(p2) -> T3(p3--p4)".
 //
-//   k_topo_search   one thread per topology particle: every live particle of no topology within the
-//                   reaction's radius in the particle's stencil (distSquared < radius^2), for each reaction
-//                   registered under (its species, its topology's type, the partner's species), draws the
-//                   acceptance 1 - exp(-rate dt) (Philox counter (topology particle handle, step,
-//                   6 | reaction << 8, partner handle)) and appends the accepted candidates with their
-//                   Efraimidis-Spirakis keys -ln(u) / rate to the replica's segment
+//   k_topo_search   one thread per topology particle: every live partner within the reaction's radius in
+//                   the particle's stencil (distSquared < radius^2) that is a particle of no topology, or a
+//                   topology particle of another topology with a higher handle (each such pair once; a
+//                   topology-topology reaction matches with its sides in either order), for each matching
+//                   reaction draws the acceptance 1 - exp(-rate dt) (Philox counter (topology particle
+//                   handle, step, 6 | reaction << 8, partner handle)) and appends the accepted candidates
+//                   with their Efraimidis-Spirakis keys -ln(u) / rate to the replica's segment
 //   k_topo_resolve  one workgroup per replica: the greedy pass over the accepted candidates in key order
 //                   (performEvents, algorithm.h:62-136, with eventsDependent, :455-464: events of one
-//                   topology depend on each other, and so do events sharing the free particle), then the
-//                   winners' type changes: the topology particle takes p3, the free particle p4, the
-//                   topology the type T2 (handleTopologyParticleReaction, :369-404); a fusion's free
-//                   particle joins the topology under the next handle of the replica's topology reserve
-//                   (allocated in key order) and is listed for the host, which adds the vertex and its
-//                   edge to the graph (appendParticle) and sets the bonded terms again
+//                   topology depend on each other, and so do topology-particle events sharing the free
+//                   particle), then the winners' changes: each particle and topology takes the product of
+//                   its side (handleTopologyParticleReaction, :369-404; handleTopologyTopologyReaction,
+//                   :406-475); a topology-particle fusion's free particle joins the topology under the next
+//                   handle of the replica's topology reserve (allocated in key order) and is listed for the
+//                   host (appendParticle); a topology-topology fusion moves every particle of the absorbed
+//                   topology into the surviving one and is listed for the host (appendTopology)
 //
 // In the fused step both run after the step's reactions, on the post-reaction state, and list the
 // records they retype for the fix-up (their moves, and those of their neighbours whose pair force
-// changed, are redone).  Particles keep their handles, but for a fusion's joining particle.  The bonded terms are not reconfigured here: a
-// reaction that changes a topology particle's species keeps that particle's bonded parameters (the host
-// core reconfigures the topology when the species change alters them).
+// changed, are redone).  Particles keep their handles, but for a topology-particle fusion's joining
+// particle.  The bonded terms are not reconfigured here: the host sets them again when a reaction changes
+// a topology's bonded configuration (every fusion does).
 #pragma once
 
 #include "rdy_step.hip"
