@@ -46,7 +46,10 @@
  *   topology reactions   enzymatic kind: hand-built scenes (one event per topology and per free particle,
  *                        the strict radius gate) and a chi-square test of the keyed selection against the
  *                        literal performEvents; fusion kind: polymer-growth scenes whose outcome follows
- *                        from the rules (the joined particle is the next end), the reserve limit
+ *                        from the rules (the joined particle is the next end), the reserve limit;
+ *                        topology-topology kinds: scenes with pairs in either order, pairs inside one
+ *                        topology, one reaction per topology, and for merges a check reaction that fires
+ *                        only on a particle left in its absorbed topology
  *   break bonds          reference scenes (TestBreakingBonds.cpp) through the energy threshold
  *   RDF / histograms     two-particle known answer, numpy brute-force pair counts and direct binning
  */
