@@ -492,3 +492,56 @@ def test_topology_topology_fusion_random_chains_event_for_event(rate):
         o.calculate_forces()
     print(f"{merged} merges", flush=True)
     assert merged > 3
+
+
+@pytest.mark.gpu
+def test_fused_loop_equals_separate_actions_with_topology_topology_reactions():
+    """two-bead chains with topology-topology enzymatic and fusion reactions in the fused step (after the step's
+    reactions, merges rewriting topology indices on the device) and by separate actions: the same state bit for
+    bit, the same reaction counts"""
+    from readdy_amd import Engine
+    from readdy_amd import _lib as L
+    print("tt fused loop", flush=True)
+    box, n, dt, steps = 8.0, 60, 0.005, 12
+    rng = np.random.default_rng(33)
+    pos, types, top = [], [], []
+    for c in range(n):
+        p0 = rng.uniform(-box / 2 + 1, box / 2 - 1, 3)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pos += [p0, p0 + 0.9 * u]
+        types += [1, 0]
+        top += [c, c]
+    pos, types = np.array(pos), np.array(types, np.uint32)
+    bonds = [(2 * c, 2 * c + 1) for c in range(n)]
+    rxs = [dict(type1=0, top_type1=0, type2=0, top_type2=0, type_to1=2, top_type_to1=1, type_to2=2, top_type_to2=1,
+                rate=30.0, radius=1.2, topology_topology=1, fusion=1),
+           dict(type1=1, top_type1=0, type2=1, top_type2=0, type_to1=1, top_type_to1=2, type_to2=1, top_type_to2=2,
+                rate=30.0, radius=1.2, topology_topology=1)]
+    pots = [(L.POT_HARMONIC_REPULSION, 1, 1, [5.0, 0.8])]
+
+    def build():
+        e = Engine(3, [1.0] * 3, potentials=pots, topology_flavor=[1, 1, 1], nl_cutoff=1.2)
+        r = e.add_replica([box] * 3, (1, 1, 1), seed=41, max_particles=4 * len(types))
+        e.set_particles(r, pos, types)
+        e.set_bonds(r, bonds, 100.0, 0.9)
+        e.set_spatial_reactions(rxs)
+        e.set_topologies(r, np.zeros(n), top)
+        return e, r
+
+    e1, r1 = build()
+    e1.run(L.HANDLER_UA, dt, steps)
+    e2, r2 = build()
+    e2.calculate_forces()
+    for _ in range(steps):
+        e2.integrate(dt)
+        e2.update_neighbor_list()
+        e2.react(L.HANDLER_UA, dt)
+        e2.evaluate_topology_reactions(dt)
+        e2.update_neighbor_list()
+        e2.calculate_forces()
+    for x, y in zip(e1.get_particles(r1), e2.get_particles(r2)):
+        assert np.array_equal(x, y)
+    tot = e1.topology_reaction_counts(r1, totals=True)
+    assert np.array_equal(tot, e2.topology_reaction_counts(r2, totals=True)) and tot[0] > 0 and tot[1] > 0
+    assert np.array_equal(e1.topology_types(r1), e2.topology_types(r2))
