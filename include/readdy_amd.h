@@ -261,6 +261,9 @@ typedef struct {
                                     top_type_to2; with fusion 1, TT_FUSION "T1(p1) + T2(p2) -> T3(p3--p4)": the
                                     second topology merges into the first, which becomes top_type_to1 */
     uint32_t top_type2, top_type_to2;
+    uint32_t self_fusion;        /* topology-topology fusion "[self=true]" (TT_FUSION_ALLOW_SELF): two particles of
+                                    one topology that are not bonded to each other fuse as well, adding the edge */
+    uint32_t reserved;           /* 0 */
 } rdy_spatial_reaction;
 /* the engine's spatial topology reactions (TopologyRegistry::addSpatialReaction, TopologyRegistry.cpp:195-223);
  * radius <= the engine's neighbour-list cutoff, at most 64 reactions */

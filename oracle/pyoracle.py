@@ -125,6 +125,7 @@ def lib():
         L.or_set_topology_reserve.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
         L.or_add_spatial_tt.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 8 + [ctypes.c_double] * 2
         L.or_add_spatial_tt_fusion.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 7 + [ctypes.c_double] * 2
+        L.or_allow_self_fusion.argtypes = [ctypes.c_void_p]
         L.or_topology_merges.restype = _u64p
         L.or_topology_merges.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
         L.or_topology_fusions.restype = _u64p
@@ -467,6 +468,10 @@ class System:
         """T1(type1) + T2(type2) -> T3(type_to1--type_to2): the second topology merges into the first"""
         _check(lib().or_add_spatial_tt_fusion(self._h, type1, top_type1, type2, top_type2, type_to1, type_to2,
                                               top_type_to, rate, radius))
+
+    def allow_self_fusion(self):
+        """the last topology-topology fusion also joins two unbonded particles of one topology"""
+        _check(lib().or_allow_self_fusion(self._h))
 
     def topology_merges(self):
         """rows (surviving topology, absorbed topology, surviving side's id, absorbed side's id)"""

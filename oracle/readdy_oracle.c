@@ -49,7 +49,9 @@
  *                        from the rules (the joined particle is the next end), the reserve limit;
  *                        topology-topology kinds: scenes with pairs in either order, pairs inside one
  *                        topology, one reaction per topology, and for merges a check reaction that fires
- *                        only on a particle left in its absorbed topology
+ *                        only on a particle left in its absorbed topology; self-fusion: a ring scene whose
+ *                        unbonded chain ends join while a bonded pair does not, and the same scene without
+ *                        the flag, where only the two separate topologies merge
  *   break bonds          reference scenes (TestBreakingBonds.cpp) through the energy threshold
  *   RDF / histograms     two-particle known answer, numpy brute-force pair counts and direct binning
  */
@@ -473,6 +475,7 @@ typedef struct {
     double rate, radius, radius2;
     int fusion;
     int tt, top_type2, top_type_to2; /* topology-topology enzymatic: the second side in another topology */
+    int self_fusion; /* topology-topology fusion that also joins two unbonded particles of one topology */
 } str_t;
 
 typedef struct {
@@ -2208,6 +2211,7 @@ int or_add_spatial_reaction(or_system *s, int type1, int top_type1, int type2, i
     r->radius2 = radius * radius;
     r->fusion = 0;
     r->tt = 0;
+    r->self_fusion = 0;
     r->top_type2 = r->top_type_to2 = -1;
     return 0;
 }
@@ -2235,6 +2239,13 @@ int or_add_spatial_tt_fusion(or_system *s, int type1, int top_type1, int type2, 
                           radius))
         return -1;
     s->str[s->n_str - 1].fusion = 1;
+    return 0;
+}
+
+int or_allow_self_fusion(or_system *s) {
+    if (!s->n_str || !s->str[s->n_str - 1].tt || !s->str[s->n_str - 1].fusion)
+        return fail("self fusion applies to a topology-topology fusion");
+    s->str[s->n_str - 1].self_fusion = 1;
     return 0;
 }
 
@@ -2299,15 +2310,27 @@ typedef struct {
     uint32_t step;
 } tev_ud;
 
+/* whether a bond joins the particles of slots p and q (graph distance 1) */
+static int slots_bonded(const or_system *s, size_t p, size_t q) {
+    for (size_t b = 0; b < s->n_bonds; ++b) {
+        const size_t a = s->bonds[b].i[0], c = s->bonds[b].i[1];
+        if ((a == p && c == q) || (a == q && c == p)) return 1;
+    }
+    return 0;
+}
+
 static void str_visit(or_system *s, size_t p, size_t q, void *ud) {
     tev_ud *u = (tev_ud *)ud;
     const entry_t *ep = &s->e[p], *eq = &s->e[q];
     if (eq->deactivated) return;
     const int top = topology_of_entry(s, ep), top2 = topology_of_entry(s, eq);
     if (top < 0) return;
-    /* a partner in a topology: each pair once, from the lower id, in two different topologies
-     * (CPUEvaluateTopologyReactions.cpp:244-268) */
-    if (top2 >= 0 && (!(ep->id < eq->id) || top2 == top)) return;
+    /* a partner in a topology: each pair once, from the lower id (CPUEvaluateTopologyReactions.cpp:244-268); a
+     * pair within one topology reacts only by self-fusion, and only when the two are not bonded: graph distance
+     * above 1, the default minimum distance (:303-318) */
+    if (top2 >= 0 && !(ep->id < eq->id)) return;
+    const int same = top2 >= 0 && top2 == top;
+    int bonded = -1;
     double d2 = -1.0;
     for (int r = 0; r < s->n_str; ++r) {
         const str_t *x = &s->str[r];
@@ -2316,6 +2339,11 @@ static void str_visit(or_system *s, size_t p, size_t q, void *ud) {
             if (x->tt || x->type1 != ep->type || x->top_type1 != s->top_types[top] || x->type2 != eq->type) continue;
         } else { /* the registry keys a topology-topology reaction under both orders of its sides */
             if (!x->tt) continue;
+            if (same) {
+                if (!x->self_fusion) continue;
+                if (bonded < 0) bonded = slots_bonded(s, p, q);
+                if (bonded) continue;
+            }
             const int T = s->top_types[top], T2 = s->top_types[top2];
             if (x->type1 == ep->type && x->top_type1 == T && x->type2 == eq->type && x->top_type2 == T2) flip = 0;
             else if (x->type2 == ep->type && x->top_type2 == T && x->type1 == eq->type && x->top_type1 == T2) flip = 1;

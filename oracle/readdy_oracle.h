@@ -172,6 +172,9 @@ int or_add_spatial_tt(or_system *s, int type1, int top_type1, int type2, int top
  * particles join the first, which becomes top_type_to; or_topology_merges lists (surviving, absorbed, ids) */
 int or_add_spatial_tt_fusion(or_system *s, int type1, int top_type1, int type2, int top_type2, int type_to1,
                              int type_to2, int top_type_to, double rate, double radius);
+/* the last topology-topology fusion also joins two unbonded particles of one topology (STRMode::
+ * TT_FUSION_ALLOW_SELF with the default minimum graph distance); its merge row names the topology twice */
+int or_allow_self_fusion(or_system *s);
 const uint64_t *or_topology_merges(const or_system *s, size_t *n);
 const uint64_t *or_topology_fusions(const or_system *s, size_t *n);
 /* topology_of_id[id] = the topology of the particle with that id (-1: none), for ids below n_ids */

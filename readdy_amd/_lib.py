@@ -35,7 +35,7 @@ class SpatialReaction(ctypes.Structure):
                 ("type_to1", ctypes.c_uint32), ("top_type_to1", ctypes.c_uint32), ("type_to2", ctypes.c_uint32),
                 ("rate", ctypes.c_double), ("radius", ctypes.c_double), ("fusion", ctypes.c_uint32),
                 ("topology_topology", ctypes.c_uint32), ("top_type2", ctypes.c_uint32),
-                ("top_type_to2", ctypes.c_uint32)]
+                ("top_type_to2", ctypes.c_uint32), ("self_fusion", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Reaction(ctypes.Structure):

@@ -199,6 +199,8 @@ struct rdy_engine {
     uint32_t *tev_count = nullptr;
     uint32_t *tfus = nullptr, *tfus_count = nullptr;  // topology-particle fusions of the last evaluation
     uint32_t *tmrg = nullptr, *tmrg_count = nullptr;  // topology-topology fusions (merges) of the last evaluation
+    uint32_t *adj_off = nullptr, *adj = nullptr;       // bond adjacency of the topology particle slots (CSR)
+    uint64_t adj_n = 0, adj_cap = 0, adj_off_cap = 0;
     unsigned long long *str_step = nullptr, *str_total = nullptr;
     std::vector<BondTerm> terms_h[3];
     BondTerm *d_terms[3] = {nullptr, nullptr, nullptr};
@@ -480,6 +482,7 @@ int upload_tables(rdy_engine *e, double dt) {
             d.tt = x.topology_topology;
             d.top_type2 = x.top_type2;
             d.top_type_to2 = x.top_type_to2;
+            d.self_fusion = x.self_fusion;
             sr.push_back(d);
         }
         if ((rc = up(&e->d_sr, sr))) return rc;
@@ -515,6 +518,51 @@ bool take_range(std::vector<std::pair<uint64_t, uint64_t>> &fl, uint64_t len, ui
 }
 
 // uploads the host list of bonded terms of one kind (device array grown as needed)
+// the bonds as adjacency lists of the topology particle slots (global: topo_base + handle), for the
+// self-fusion test of the topology reactions (a bonded pair does not self-fuse)
+int upload_adjacency(rdy_engine *e) {
+    const uint64_t n = e->topo_used;
+    std::vector<uint32_t> off(n + 1, 0), adj;
+    const auto &v = e->terms_h[0];
+    for (const BondTerm &t : v) {
+        const HostReplica &H = e->reps_h[t.r];
+        const uint64_t a = (uint64_t)H.topo_base + t.h[0], b = (uint64_t)H.topo_base + t.h[1];
+        if (a < n && b < n) {
+            ++off[a + 1];
+            ++off[b + 1];
+        }
+    }
+    for (uint64_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    adj.resize(off[n]);
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (const BondTerm &t : v) {
+        const HostReplica &H = e->reps_h[t.r];
+        const uint64_t a = (uint64_t)H.topo_base + t.h[0], b = (uint64_t)H.topo_base + t.h[1];
+        if (a < n && b < n) {
+            adj[fill[a]++] = (uint32_t)b;
+            adj[fill[b]++] = (uint32_t)a;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (off.size() > e->adj_off_cap) {
+        if (e->adj_off) HIP_TRY(hipFree(e->adj_off));
+        e->adj_off = nullptr;
+        e->adj_off_cap = std::max<uint64_t>(off.size(), 2 * e->adj_off_cap);
+        HIP_TRY(dalloc(&e->adj_off, e->adj_off_cap));
+    }
+    if (adj.size() > e->adj_cap || !e->adj) {
+        if (e->adj) HIP_TRY(hipFree(e->adj));
+        e->adj = nullptr;
+        e->adj_cap = std::max<uint64_t>(std::max<uint64_t>(adj.size(), 1), 2 * e->adj_cap);
+        HIP_TRY(dalloc(&e->adj, e->adj_cap));
+    }
+    HIP_TRY(hipMemcpyAsync(e->adj_off, off.data(), off.size() * 4, hipMemcpyHostToDevice, e->stream));
+    if (!adj.empty()) HIP_TRY(hipMemcpyAsync(e->adj, adj.data(), adj.size() * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->adj_n = off.size();
+    return RDY_OK;
+}
+
 int upload_terms(rdy_engine *e, int kind) {
     const std::vector<BondTerm> &v = e->terms_h[kind];
     if (v.size() > e->d_terms_cap[kind]) {
@@ -531,6 +579,7 @@ int upload_terms(rdy_engine *e, int kind) {
                                e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     e->version++;
+    if (kind == 0) return upload_adjacency(e);
     return RDY_OK;
 }
 
@@ -788,6 +837,9 @@ TopoTables topo_tables(const rdy_engine *e) {
     t.fus_count = e->tfus_count;
     t.mrg = e->tmrg;
     t.mrg_count = e->tmrg_count;
+    t.adj_off = e->adj_off;
+    t.adj = e->adj;
+    t.adj_n = (uint32_t)e->adj_n;
     t.topo_owner = e->topo_owner;
     t.ttype = e->ttype;
     t.treact = e->treact;
@@ -1203,7 +1255,7 @@ int rdy_engine_destroy(rdy_engine *e) {
     }
     void *ptrs[] = {e->wl_count, e->ovf_count, e->tile_sums, e->d_bricks, e->d_slow, e->reps, e->events, e->ev_count, e->recs, e->rec_count, e->rx_step, e->rx_total, e->e_bonded, e->virial, e->err,
                     e->force_out, e->edits, e->edit_count, e->aff_bits, e->aff_list, e->wd_list, e->d_pots, e->d_pidx, e->d_rx, e->d_o1idx, e->d_o1list, e->d_o2list, e->d_o2idx,
-                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->tfus, e->tfus_count, e->tmrg, e->tmrg_count, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
+                    e->d_rowmax, e->d_dbpots, e->d_dbpidx, e->d_dbtable, e->d_dbkind, e->d_dbpref, e->d_dbradii, e->d_dbcdf, e->d_D, e->d_bd_scale, e->d_flavor, e->topo_of, e->topo_owner, e->treact, e->loc_cur, e->d_sr, e->ttype, e->tev, e->tev_count, e->tfus, e->tfus_count, e->tmrg, e->tmrg_count, e->adj_off, e->adj, e->str_step, e->str_total, e->fb, e->loc, e->scratch, e->d_noise, e->d_terms[0], e->d_terms[1],
                     e->d_terms[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -2296,8 +2348,10 @@ int rdy_set_spatial_reactions(rdy_engine *e, uint32_t n, const rdy_spatial_react
         const rdy_spatial_reaction &x = sr[i];
         if (x.type1 >= e->n_types || x.type2 >= e->n_types || x.type_to1 >= e->n_types || x.type_to2 >= e->n_types)
             return set_err(RDY_ERR_INVALID, "spatial topology reaction species out of range");
-        if (x.fusion > 1 || x.topology_topology > 1)
-            return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion and topology_topology are 0 or 1");
+        if (x.fusion > 1 || x.topology_topology > 1 || x.self_fusion > 1 || x.reserved ||
+            (x.self_fusion && !(x.fusion && x.topology_topology)))
+            return set_err(RDY_ERR_INVALID, "spatial topology reaction: fusion, topology_topology and self_fusion are 0 "
+                                            "or 1, self_fusion only for a topology-topology fusion");
         if (!e->topo_flavor[x.type1] || !e->topo_flavor[x.type_to1])
             return set_err(RDY_ERR_INVALID, "the topology side of a topology-particle reaction must stay a topology species");
         if (x.topology_topology) {

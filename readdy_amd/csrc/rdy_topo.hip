@@ -35,6 +35,7 @@ struct SpatialDev {
     double rate, p_accept, radius2;
     uint32_t fusion, tt;             // tt: topology-topology enzymatic (the partner in another topology)
     uint32_t top_type2, top_type_to2;
+    uint32_t self_fusion, pad;       // topology-topology fusion within one topology (unbonded pairs)
 };
 
 struct TopoEvent {
@@ -59,6 +60,9 @@ struct TopoTables {
     uint32_t *fus_count;         // [replica]
     uint32_t *mrg;               // [replica][TOPO_MAX_EVENTS][4] the last evaluation's topology merges
     uint32_t *mrg_count;         // [replica]
+    const uint32_t *adj_off;     // [adj_n] bond adjacency of the topology particle slots (CSR offsets)
+    const uint32_t *adj;         // the bonded partners' slots
+    uint32_t adj_n;
     const uint32_t *topo_owner;  // [topology particle slot] replica, UINT32_MAX when unused
     uint32_t *ttype;             // topology types; replica r's at reps[r].tt_base
     uint32_t *treact;            // per topology (as ttype): the serial of the evaluation it last reacted in
@@ -106,9 +110,22 @@ __global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev 
         if (o.tw & TW_TOPO) {
             if (!(me.h < o.h) || o.h >= R.n_topo) return;
             k2 = TT.topo_of[R.topo_base + o.h];
-            if (k2 >= R.n_tt || k2 == k) return;
+            if (k2 >= R.n_tt) return;
             T2 = TT.ttype[R.tt_base + k2];
         }
+        // a pair within one topology reacts only by self-fusion and only when the two are not bonded (graph
+        // distance above 1, CPUEvaluateTopologyReactions.cpp:303-318 with the default minimum distance)
+        const bool same = k2 == k;
+        int bonded = -1;
+        auto is_bonded = [&]() -> bool {
+            if (bonded < 0) {
+                bonded = 0;
+                const uint32_t g = R.topo_base + me.h, go = R.topo_base + o.h;
+                if (TT.adj_off && g + 1 < TT.adj_n)
+                    for (uint32_t a = TT.adj_off[g]; a < TT.adj_off[g + 1]; ++a) bonded |= TT.adj[a] == go ? 1 : 0;
+            }
+            return bonded != 0;
+        };
         const uint32_t t2 = tw_type(o.tw);
         // distSquared(entry.pos, neighbor.pos) (CPUEvaluateTopologyReactions.cpp:278)
         const double dx = min_image(o.x - me.x, geo.box[0], geo.pbc[0]);
@@ -122,6 +139,7 @@ __global__ __launch_bounds__(256) void k_topo_search(Grid cur, const ReplicaDev 
                 if (X.tt || X.type1 != t || X.top_type1 != T || X.type2 != t2) continue;
             } else {  // the registry holds a topology-topology reaction under both orders of its sides
                 if (!X.tt) continue;
+                if (same && (!X.self_fusion || is_bonded())) continue;
                 if (X.type1 == t && X.top_type1 == T && X.type2 == t2 && X.top_type2 == T2) flip = 0;
                 else if (X.type2 == t && X.top_type2 == T && X.type1 == t2 && X.top_type1 == T2) flip = SR_FLIP;
                 else continue;

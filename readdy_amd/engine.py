@@ -320,6 +320,7 @@ class Engine:
             arr[i].topology_topology = int(x.get("topology_topology", 0))
             arr[i].top_type2 = int(x.get("top_type2", 0))
             arr[i].top_type_to2 = int(x.get("top_type_to2", 0))
+            arr[i].self_fusion = int(x.get("self_fusion", 0))
         self._sr, self._n_sr = arr, len(reactions)
         L.check(self._lib.rdy_set_spatial_reactions(self._h, len(reactions), ctypes.cast(arr, ctypes.c_void_p)))
 

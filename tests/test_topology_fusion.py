@@ -545,3 +545,77 @@ def test_fused_loop_equals_separate_actions_with_topology_topology_reactions():
     tot = e1.topology_reaction_counts(r1, totals=True)
     assert np.array_equal(tot, e2.topology_reaction_counts(r2, totals=True)) and tot[0] > 0 and tot[1] > 0
     assert np.array_equal(e1.topology_types(r1), e2.topology_types(r2))
+
+
+# self-fusion: a U-shaped chain of four (handles 0-3, ends 0.9 apart), a bonded pair (4, 5) and two single
+# particles (6, 7) in topologies of their own; every topology has type 0 and species A=0 / B=1 / C=2
+TS_POS = np.array([[0, 0, 0], [0.9, 0, 0], [0.9, 0.9, 0], [0, 0.9, 0], [5, 0, 0], [5.5, 0, 0], [10, 0, 0],
+                   [10.5, 0, 0]], float)
+TS_TYPES = np.array([0, 2, 2, 1, 0, 1, 0, 1])
+TS_TOP_OF = [0, 0, 0, 0, 1, 1, 2, 3]
+TS_BONDS = [(0, 1), (1, 2), (2, 3), (4, 5)]
+TS_R = dict(type1=0, top_type1=0, type2=1, top_type2=0, type_to1=2, top_type_to1=1, type_to2=2, top_type_to2=1,
+            rate=1e10, radius=1.0, topology_topology=1, fusion=1, self_fusion=1)
+
+
+def _ts_oracle(reaction):
+    o = po.System(3, [30.0] * 3, [1, 1, 1], diffusion=[1.0] * 3)
+    o.add_spatial_tt_fusion(0, 0, 1, 0, 2, 2, 1, reaction["rate"], reaction["radius"])
+    if reaction["self_fusion"]:
+        o.allow_self_fusion()
+    o.set_seed(3)
+    o.add_particles(TS_POS, TS_TYPES.astype(np.int32), ids=np.arange(8))
+    o.set_topologies(np.asarray(TS_TOP_OF), np.zeros(4, int))
+    for a, b in TS_BONDS:
+        o.add_bond(a, b, 100.0, 0.9)
+    o.nl_setup(1.0)
+    return o
+
+
+@pytest.mark.parametrize("self_fusion", [1, 0])
+def test_oracle_self_fusion_closes_rings_of_unbonded_ends_only(self_fusion):
+    o = _ts_oracle(dict(TS_R, self_fusion=self_fusion))
+    n = o.topology_reactions(1.0, step=0)
+    rows = sorted(map(tuple, o.topology_merges().tolist()))
+    t = _by_id(o)
+    if self_fusion:  # the chain's ends join (a merge row naming topology 0 twice); the bonded pair does not
+        assert n == 2 and rows == [(0, 0, 0, 3), (2, 3, 6, 7)]
+        assert [t[i] for i in range(8)] == [2, 2, 2, 2, 0, 1, 2, 2]
+        assert o.topology_types().tolist() == [1, 0, 1, 0]
+    else:
+        assert n == 1 and rows == [(2, 3, 6, 7)]
+        assert [t[i] for i in range(8)] == [0, 2, 2, 1, 0, 1, 2, 2]
+        assert o.topology_types().tolist() == [0, 0, 1, 0]
+    with pytest.raises(Exception):
+        po.System(3, [30.0] * 3, [1, 1, 1], diffusion=[1.0] * 3).allow_self_fusion()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("self_fusion", [1, 0])
+def test_self_fusion_scene_on_the_device(self_fusion):
+    print("self fusion scene", self_fusion, flush=True)
+    from readdy_amd import Engine
+    x = dict(TS_R, self_fusion=self_fusion)
+    e = Engine(3, [1.0] * 3, topology_flavor=[1, 1, 1], nl_cutoff=1.0)
+    r = e.add_replica([30.0] * 3, (1, 1, 1), seed=3, max_particles=64)
+    e.set_particles(r, TS_POS, TS_TYPES.astype(np.uint32))
+    e.set_spatial_reactions([x])
+    e.set_topologies(r, [0, 0, 0, 0], TS_TOP_OF)
+    e.set_bonds(r, TS_BONDS, 100.0, 0.9)
+    o = _ts_oracle(x)
+    e.evaluate_topology_reactions(1.0)
+    o.topology_reactions(1.0, step=0)
+    _, gt, gh = e.get_particles(r)
+    assert dict(zip(gh.tolist(), gt.tolist())) == _by_id(o)
+    assert e.topology_types(r).tolist() == o.topology_types().tolist()
+    assert sorted(map(tuple, e.topology_merges(r).tolist())) == sorted(map(tuple, o.topology_merges().tolist()))
+    assert e.topology_reaction_counts(r, totals=True).tolist() == [2 if self_fusion else 1]
+
+
+@pytest.mark.gpu
+def test_self_fusion_needs_a_topology_topology_fusion():
+    print("self fusion validation", flush=True)
+    from readdy_amd import Engine
+    e = Engine(3, [1.0] * 3, topology_flavor=[1, 1, 1], nl_cutoff=1.0)
+    with pytest.raises(Exception):
+        e.set_spatial_reactions([dict(TS_R, fusion=0)])
