@@ -352,12 +352,14 @@ void MI355XStateModel::configureSpatialReactions() {
     std::vector<rdy_spatial_reaction> v;
     for (const auto &r : _k->context().topologyRegistry().spatialReactions()) {
         if (r.mode != STRMode::TP_ENZYMATIC && r.mode != STRMode::TP_FUSION && r.mode != STRMode::TT_ENZYMATIC &&
-            r.mode != STRMode::TT_FUSION)
+            r.mode != STRMode::TT_FUSION && !(r.mode == STRMode::TT_FUSION_ALLOW_SELF && r.minGraphDistance <= 1))
             throw std::invalid_argument("spatial topology reaction \"" + r.name +
                                         "\": the MI355X kernel performs the topology-particle kinds "
                                         "(T1(p1) + (p2) -> T2(p3) + (p4) and T1(p1) + (p2) -> T2(p3--p4)) and the "
-                                        "topology-topology kinds without self-fusion (T1(p1) + T2(p2) -> T3(p3) + "
-                                        "T4(p4), T1(p1) + T2(p2) -> T3(p3--p4)) only");
+                                        "topology-topology kinds (T1(p1) + T2(p2) -> T3(p3) + T4(p4), "
+                                        "T1(p1) + T2(p2) -> T3(p3--p4), with [self=true] at a minimum graph "
+                                        "distance of at most 1) only");
+        const bool selfFusion = r.mode == STRMode::TT_FUSION_ALLOW_SELF;
         rdy_spatial_reaction q{};
         q.type1 = r.type1;
         q.top_type1 = (uint32_t)r.topType1;
@@ -367,8 +369,9 @@ void MI355XStateModel::configureSpatialReactions() {
         q.type_to2 = r.typeTo2;
         q.rate = r.rate;
         q.radius = r.radius;
-        q.fusion = (r.mode == STRMode::TP_FUSION || r.mode == STRMode::TT_FUSION) ? 1u : 0u;
-        if (r.mode == STRMode::TT_ENZYMATIC || r.mode == STRMode::TT_FUSION) {
+        q.fusion = (r.mode == STRMode::TP_FUSION || r.mode == STRMode::TT_FUSION || selfFusion) ? 1u : 0u;
+        q.self_fusion = selfFusion ? 1u : 0u;
+        if (r.mode == STRMode::TT_ENZYMATIC || r.mode == STRMode::TT_FUSION || selfFusion) {
             q.topology_topology = 1u;
             q.top_type2 = (uint32_t)r.topType2;
             q.top_type_to2 = (uint32_t)r.topTypeTo2;
@@ -590,6 +593,14 @@ void MI355XStateModel::spatialReactions(scalar dt) {
             const uint32_t a = rows[4 * i], b = rows[4 * i + 1];
             if (a >= _topologies.size() || b >= _topologies.size() || gone[a] || gone[b])
                 throw std::logic_error("topology merge rows out of range");
+            if (a == b) {  // self-fusion: an edge between two vertices of one graph (the topology stays)
+                const auto &pi = _topologies[a]->particleIndices();
+                const auto v1 = std::find(pi.begin(), pi.end(), (std::size_t)rows[4 * i + 2]);
+                const auto v2 = std::find(pi.begin(), pi.end(), (std::size_t)rows[4 * i + 3]);
+                if (v1 == pi.end() || v2 == pi.end()) throw std::logic_error("self-fusion of particles not in the topology");
+                _topologies[a]->addEdge((std::size_t)(v1 - pi.begin()), (std::size_t)(v2 - pi.begin()));
+                continue;
+            }
             _topologies[a]->appendTopology(*_topologies[b], rows[4 * i + 3], rows[4 * i + 2]);
             gone[b] = 1;
         }

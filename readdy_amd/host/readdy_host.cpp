@@ -368,6 +368,8 @@ void TopologyRegistry::addSpatialReaction(const std::string &descriptor, scalar 
                     throw std::runtime_error("Option \"self=true\" is missing. This option is currently required for "
                                              "all valid options.");
                 d.mode = STRMode::TT_FUSION_ALLOW_SELF;
+                const auto dp = opts.find("distance>");  // get_distance (SpatialTopologyReaction.cpp:262-269)
+                d.minGraphDistance = dp == std::string::npos ? 0u : (unsigned)std::stoi(opts.substr(dp + 9));
             }
         } else {
             d.mode = STRMode::TT_ENZYMATIC;

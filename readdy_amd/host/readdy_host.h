@@ -163,6 +163,7 @@ struct SpatialReactionDesc {
     ParticleTypeId type1, type2, typeTo1, typeTo2;
     std::size_t topType1, topType2, topTypeTo1, topTypeTo2;
     scalar rate, radius;
+    unsigned minGraphDistance = 0;  // self-fusion: pairs at a graph distance up to this do not react ("distance>N")
     bool isTopologyParticle() const { return mode == STRMode::TP_ENZYMATIC || mode == STRMode::TP_FUSION; }
 };
 

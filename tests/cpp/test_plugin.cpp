@@ -1076,6 +1076,65 @@ static void test_topology_topology_fusion() {
     std::printf("topology-topology fusion: 3 of 4 pairs merged\n");
 }
 
+// Topology-topology self-fusion ("[self=true]"): a chain end(0,0,0)-bead(1,0,0)-bead(1,0.9,0)-end(0,0.9,0) and a
+// bonded pair of ends 0.5 apart, bonds k 10, length 1.  One evaluation closes the chain into a ring of four
+// (its ends 0.9 apart are not bonded) and leaves the bonded pair alone: two topologies, the bond energy
+// 10 * 0.1^2 twice plus 10 * 0.5^2; "distance>2" is refused.
+static void test_topology_self_fusion() {
+    auto k = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c = k->context();
+    c.boxSize() = {30., 30., 30.};
+    c.particleTypes().addTopologyType("bead", 0.1);
+    c.particleTypes().addTopologyType("end", 0.1);
+    c.particleTypes().addTopologyType("mid", 0.1);
+    auto &reg = c.topologyRegistry();
+    for (const char *t : {"Chain", "Ring"}) reg.addType(t);
+    for (const char *a : {"bead", "end", "mid"})
+        for (const char *b : {"bead", "end", "mid"}) reg.configureBondPotential(a, b, {10., 1.});
+    reg.addSpatialReaction("ring: Chain(end) + Chain(end) -> Ring(mid--mid) [self=true]", 1e10, 1.0);
+    const auto bead = c.particleTypes().idOf("bead"), end = c.particleTypes().idOf("end");
+    auto &sm = k->stateModel();
+    auto *u = sm.addTopology("Chain", {model::Particle(0., 0., 0., end), model::Particle(1., 0., 0., bead),
+                                       model::Particle(1., 0.9, 0., bead), model::Particle(0., 0.9, 0., end)});
+    u->addEdge(0, 1);
+    u->addEdge(1, 2);
+    u->addEdge(2, 3);
+    auto *p = sm.addTopology("Chain", {model::Particle(5., 0., 0., end), model::Particle(5.5, 0., 0., end)});
+    p->addEdge(0, 1);
+    k->actions().calculateForces()->perform();
+    k->actions().evaluateTopologyReactions(0.01)->perform();
+    const auto tops = sm.getTopologies();
+    int rings = 0;
+    for (auto *t : tops)
+        if (t->type() == "Ring") {
+            ++rings;
+            CHECK(t->nParticles() == 4u, "the ring holds %zu particles", t->nParticles());
+        }
+    CHECK(tops.size() == 2u && rings == 1, "self-fusion: %zu topologies, %d rings", tops.size(), rings);
+    k->actions().calculateForces()->perform();
+    const double e = sm.energy(), want = 2 * 10. * 0.01 + 10. * 0.25;
+    CHECK(std::fabs(e - want) < 1e-6 * want, "ring bonds: energy %.9g, expected %.9g", e, want);
+    auto k2 = plugin::KernelProvider::getInstance().create("MI355X");
+    auto &c2 = k2->context();
+    c2.boxSize() = {30., 30., 30.};
+    c2.particleTypes().addTopologyType("end", 0.1);
+    c2.topologyRegistry().addType("Chain");
+    c2.topologyRegistry().configureBondPotential("end", "end", {10., 1.});
+    c2.topologyRegistry().addSpatialReaction("far: Chain(end) + Chain(end) -> Chain(end--end) [self=true, distance>2]",
+                                             1e10, 1.0);
+    auto *q = k2->stateModel().addTopology("Chain", {model::Particle(0., 0., 0., c2.particleTypes().idOf("end"))});
+    (void)q;
+    bool refused = false;
+    try {
+        k2->actions().calculateForces()->perform();
+        k2->actions().evaluateTopologyReactions(0.01)->perform();
+    } catch (const std::exception &) {
+        refused = true;
+    }
+    CHECK(refused, "a minimum graph distance above 1 is refused");
+    std::printf("topology self-fusion: one ring closed, the bonded pair left alone\n");
+}
+
 // Recipe steps that move and append topology particles (Recipe::changeParticlePosition, appendNewParticle): three
 // chains of three beads 1 apart with bonds (k 10, length 1); a structural reaction (rate 1e10 while a chain has
 // fewer than four particles) moves bead 0 by 0.3 away from bead 1 and appends a bead 1.2 beyond bead 2, bonded to
@@ -1287,6 +1346,7 @@ int main(int argc, char **argv) {
             test_structural_move_and_append();
             test_topology_topology_enzymatic();
             test_topology_topology_fusion();
+            test_topology_self_fusion();
         }
     } catch (const std::exception &e) {
         std::printf("FAIL exception: %s\n", e.what());
